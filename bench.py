@@ -1,0 +1,158 @@
+"""Benchmark: D3Q19 dense flow past a pinned sphere on one MI355X.
+
+Workload (the D3Q19 TRT sphere case): 256x128x128 cells, sphere radius 12 at
+(64, 64, 64) with halfway bounce-back, freestream u = 0.05 at Re = 200, ALBC
+sponge (width 12) on every face, FP64 drag/lift reduction every step.  One
+"step" = force reduction + fused pull-stream/collide over the whole grid.
+Inputs are resident in HBM before the timed region starts.
+
+Prints one JSON line:
+  value        whole-job MLUPS (fp32 storage), host clock around K steps
+  roofline     the fused stream-collide kernel: algorithmic bytes per launch
+               (2 * Q * 4 B * cells = 152 B/cell) over its mean launch time
+               measured with HIP events on the solve's stream
+  cpu_baseline the FP64 oracle (node oracle/run.js, worker threads) on a
+               bounded number of steps of the same grid
+  fp16_store / bf16_store  the same workload with shifted 16-bit storage
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "getting-air_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import numpy as np  # noqa: E402
+
+import pylbm  # noqa: E402
+from pylbm import scenarios as sc  # noqa: E402
+
+METRIC = "D3Q19 MLUPS & % HBM peak, fp32 vs fp16-store; solves/s + p99 latency served"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_d3q19_sphere_256x128x128.json")
+
+
+def measure(storage, steps, warmup, kernel_reps):
+    desc, mac, meta = sc.sphere(storage=storage)
+    desc["autotune"] = 1
+    q = desc["q"]
+    with pylbm.Solve(desc) as s:
+        s.upload_macro(mac)
+        s.step(warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        t1 = time.perf_counter()
+        forces = s.read_forces(steps)
+        kernel_ms = s.time_step_kernel(kernel_reps)
+        info = s.info()
+        ncells = s.ncells
+    secs = t1 - t0
+    elem = 4 if storage == "f32" else 2
+    bytes_per_launch = 2 * q * elem * ncells
+    achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    cd = forces[-200:, 0].mean() / (0.5 * meta["u0"] ** 2 * meta["area"]) if len(forces) else float("nan")
+    return {
+        "mlups": ncells * steps / secs / 1e6,
+        "ms_per_step": secs * 1e3 / steps,
+        "kernel_ms": kernel_ms,
+        "achieved_gbs": achieved,
+        "bytes_per_launch": bytes_per_launch,
+        "ncells": ncells,
+        "tile": [info["block_x"], info["block_y"], info["swizzle"]],
+        "cd_last200": float(cd),
+        "finite": bool(np.all(np.isfinite(forces))),
+    }
+
+
+def cpu_baseline(steps, threads):
+    import oracle_runner
+
+    desc, mac, _ = sc.sphere()
+    r = oracle_runner.run(desc, mac, steps=steps, threads=threads, want=())
+    s = r["summary"]
+    return {
+        "value": s["mlups"], "unit": "MLUPS", "cores": s["threads"], "kind": "port",
+        "sample": "FP64 JS oracle (node, %d worker threads, %s), %d steps of the 256x128x128 "
+                  "D3Q19 TRT sphere grid (%.1f s)" % (s["threads"], s["cpu_model"], steps, s["seconds"]),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-16bit", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: the dense solve is single-device here "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    dev = pylbm.device_info()
+    print("device: %s, %d CUs, %.0f GB" % (dev["name"], dev["compute_units"], dev["hbm_bytes"] / 1e9),
+          file=sys.stderr, flush=True)
+    main_run = measure("f32", args.steps, args.warmup, 50)
+    print("fp32: %s" % json.dumps(main_run), file=sys.stderr, flush=True)
+    extra = {}
+    if not args.no_16bit:
+        for st in ("f16", "bf16"):
+            r = measure(st, args.steps, args.warmup, 50)
+            r["speedup_vs_f32"] = r["mlups"] / main_run["mlups"]
+            extra[st + "_store"] = r
+            print("%s: %s" % (st, json.dumps(r)), file=sys.stderr, flush=True)
+
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(args.cpu_steps, threads)
+        print("cpu: %s" % json.dumps(cpu), file=sys.stderr, flush=True)
+
+    achieved = main_run["achieved_gbs"]
+    out = {
+        "metric": METRIC,
+        "value": round(main_run["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_run["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded freestream initial field)",
+        "config": {"workload": "D3Q19 TRT flow past a pinned sphere, 256x128x128, R=12, Re=200, "
+                               "sponge 12, per-step FP64 drag/lift, fp32 storage",
+                   "cells": main_run["ncells"], "tile": main_run["tile"]},
+        "pct_hbm_peak": round(100.0 * achieved / HBM_PEAK_GBS, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_step<19, StoreF32, general>", "kernel_ms": round(main_run["kernel_ms"], 5),
+                     "bytes_per_launch": main_run["bytes_per_launch"]},
+        "cpu_baseline": cpu,
+        "drag_cd_last200": round(main_run["cd_last200"], 5),
+    }
+    for k, v in extra.items():
+        out[k] = {"mlups": round(v["mlups"], 1), "speedup_vs_f32": round(v["speedup_vs_f32"], 3),
+                  "kernel_ms": round(v["kernel_ms"], 5), "achieved_gbs": round(v["achieved_gbs"], 1),
+                  "pct_hbm_peak": round(100.0 * v["achieved_gbs"] / HBM_PEAK_GBS, 2),
+                  "cd_last200": round(v["cd_last200"], 5)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,372 @@
+// Thin N-API binding of liblbm.so for Node (N-API version 8, Node >= 12).
+// Every function maps one-to-one onto an include/lbm.h entry point; errors
+// become JavaScript exceptions carrying lbm_last_error().  lbm_step runs on
+// the libuv thread pool and resolves a Promise, so a host loop can await it
+// the way the reference awaits queue.onSubmittedWorkDone().
+#define NAPI_VERSION 8
+#include <node_api.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "lbm.h"
+
+namespace {
+
+#define NAPI_OK(call)                                       \
+  do {                                                      \
+    if ((call) != napi_ok) {                                \
+      napi_throw_error(env, nullptr, "N-API call failed: " #call); \
+      return nullptr;                                       \
+    }                                                       \
+  } while (0)
+
+napi_value throw_lbm(napi_env env, int rc) {
+  std::string msg = "lbm error " + std::to_string(rc) + ": " + lbm_last_error();
+  napi_throw_error(env, nullptr, msg.c_str());
+  return nullptr;
+}
+
+bool get_prop(napi_env env, napi_value obj, const char* name, napi_value* out) {
+  bool has = false;
+  if (napi_has_named_property(env, obj, name, &has) != napi_ok || !has) return false;
+  if (napi_get_named_property(env, obj, name, out) != napi_ok) return false;
+  napi_valuetype t;
+  napi_typeof(env, *out, &t);
+  return t != napi_undefined && t != napi_null;
+}
+
+double num(napi_env env, napi_value obj, const char* name, double dflt) {
+  napi_value v;
+  if (!get_prop(env, obj, name, &v)) return dflt;
+  double d = dflt;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+
+void num_array(napi_env env, napi_value obj, const char* name, double* out, int n) {
+  napi_value v;
+  if (!get_prop(env, obj, name, &v)) return;
+  for (int i = 0; i < n; i++) {
+    napi_value e;
+    if (napi_get_element(env, v, i, &e) == napi_ok) {
+      double d;
+      if (napi_get_value_double(env, e, &d) == napi_ok) out[i] = d;
+    }
+  }
+}
+
+lbm_solve* handle(napi_env env, napi_value v) {
+  void* p = nullptr;
+  napi_get_value_external(env, v, &p);
+  return static_cast<lbm_solve*>(p);
+}
+
+// createSolve({q, nx, ny, nz, collision: 'bgk'|'trt', omega, omegaMinus,
+//              force, walls, wallU, sponge: {width, axes, u},
+//              body: {a, centre} | null, solidEq, storage: 'f32'|'f16'|'bf16',
+//              autotune})
+napi_value CreateSolve(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 1) {
+    napi_throw_type_error(env, nullptr, "createSolve(desc)");
+    return nullptr;
+  }
+  napi_value o = argv[0];
+  lbm_desc d;
+  memset(&d, 0, sizeof d);
+  d.abi_version = LBM_ABI_VERSION;
+  d.q = (int)num(env, o, "q", 19);
+  d.nx = (int)num(env, o, "nx", 1);
+  d.ny = (int)num(env, o, "ny", 1);
+  d.nz = (int)num(env, o, "nz", 1);
+  d.omega = (float)num(env, o, "omega", 1.0);
+  napi_value v;
+  d.collision = LBM_BGK;
+  if (get_prop(env, o, "omegaMinus", &v)) {
+    d.collision = LBM_TRT;
+    d.omega_minus = (float)num(env, o, "omegaMinus", d.omega);
+  }
+  double t3[3] = {0, 0, 0};
+  num_array(env, o, "force", t3, 3);
+  for (int k = 0; k < 3; k++) d.force[k] = (float)t3[k];
+  double w3[3] = {0, 0, 0};
+  num_array(env, o, "walls", w3, 3);
+  for (int k = 0; k < 3; k++) d.walls[k] = w3[k] != 0;
+  double wu[2] = {0, 0};
+  num_array(env, o, "wallU", wu, 2);
+  d.wall_u[0] = (float)wu[0];
+  d.wall_u[1] = (float)wu[1];
+  d.sponge_axes[0] = d.sponge_axes[1] = 1;
+  d.sponge_axes[2] = d.q == 9 ? 0 : 1;
+  if (get_prop(env, o, "sponge", &v)) {
+    d.sponge_width = (float)num(env, v, "width", 0);
+    double ax[3] = {1, 1, d.q == 9 ? 0.0 : 1.0}, su[3] = {0, 0, 0};
+    num_array(env, v, "axes", ax, 3);
+    num_array(env, v, "u", su, 3);
+    for (int k = 0; k < 3; k++) {
+      d.sponge_axes[k] = ax[k] != 0;
+      d.sponge_u[k] = (float)su[k];
+    }
+  }
+  if (get_prop(env, o, "body", &v)) {
+    d.body.shape = d.q == 9 ? LBM_BODY_CIRCLE : LBM_BODY_SPHERE;
+    d.body.radius = (float)num(env, v, "a", 0);
+    double c[3] = {0, 0, 0};
+    num_array(env, v, "centre", c, 3);
+    for (int k = 0; k < 3; k++) d.body.centre[k] = (float)c[k];
+  }
+  d.solid_eq = (int)num(env, o, "solidEq", 1);
+  d.autotune = (int)num(env, o, "autotune", 0);
+  d.storage = LBM_STORE_F32;
+  if (get_prop(env, o, "storage", &v)) {
+    char buf[16] = {0};
+    size_t len = 0;
+    napi_get_value_string_utf8(env, v, buf, sizeof buf, &len);
+    if (!strcmp(buf, "f16")) d.storage = LBM_STORE_F16;
+    else if (!strcmp(buf, "bf16")) d.storage = LBM_STORE_BF16;
+    else if (strcmp(buf, "f32")) {
+      napi_throw_type_error(env, nullptr, "storage must be 'f32', 'f16' or 'bf16'");
+      return nullptr;
+    }
+  }
+  lbm_solve* s = nullptr;
+  int rc = lbm_solve_create(&d, &s);
+  if (rc) return throw_lbm(env, rc);
+  napi_value ext;
+  NAPI_OK(napi_create_external(env, s, nullptr, nullptr, &ext));
+  return ext;
+}
+
+napi_value Destroy(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  lbm_solve_destroy(handle(env, argv[0]));
+  return nullptr;
+}
+
+// Typed-array argument as (data, length in elements).
+bool typed(napi_env env, napi_value v, napi_typedarray_type want, void** data, size_t* len) {
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) return false;
+  napi_typedarray_type t;
+  napi_value ab;
+  size_t off;
+  napi_get_typedarray_info(env, v, &t, len, data, &ab, &off);
+  return t == want;
+}
+
+napi_value UploadMacro(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* data;
+  size_t len;
+  if (argc < 2 || !typed(env, argv[1], napi_float32_array, &data, &len)) {
+    napi_throw_type_error(env, nullptr, "uploadMacro(solve, Float32Array)");
+    return nullptr;
+  }
+  int rc = lbm_upload_macro(handle(env, argv[0]), static_cast<float*>(data));
+  if (rc) return throw_lbm(env, rc);
+  return nullptr;
+}
+
+napi_value UploadF(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* data;
+  size_t len;
+  if (argc < 2 || !typed(env, argv[1], napi_float32_array, &data, &len)) {
+    napi_throw_type_error(env, nullptr, "uploadF(solve, Float32Array)");
+    return nullptr;
+  }
+  int rc = lbm_upload_f(handle(env, argv[0]), static_cast<float*>(data));
+  if (rc) return throw_lbm(env, rc);
+  return nullptr;
+}
+
+// readInto(solve, TypedArray): fills a caller-sized array.
+template <int (*Fn)(lbm_solve*, float*)>
+napi_value ReadFloat(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* data;
+  size_t len;
+  if (argc < 2 || !typed(env, argv[1], napi_float32_array, &data, &len)) {
+    napi_throw_type_error(env, nullptr, "expected (solve, Float32Array)");
+    return nullptr;
+  }
+  int rc = Fn(handle(env, argv[0]), static_cast<float*>(data));
+  if (rc) return throw_lbm(env, rc);
+  return argv[1];
+}
+
+napi_value ReadTotals(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  napi_value ab, arr;
+  void* data;
+  NAPI_OK(napi_create_arraybuffer(env, 32, &data, &ab));
+  int rc = lbm_read_totals(handle(env, argv[0]), static_cast<double*>(data));
+  if (rc) return throw_lbm(env, rc);
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, 4, ab, 0, &arr));
+  return arr;
+}
+
+napi_value ReadForces(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  int32_t maxr = 1;
+  if (argc > 1) napi_get_value_int32(env, argv[1], &maxr);
+  if (maxr < 0) maxr = 0;
+  napi_value ab, arr;
+  void* data;
+  NAPI_OK(napi_create_arraybuffer(env, (size_t)maxr * 48 + 8, &data, &ab));
+  int32_t count = 0;
+  int rc = lbm_read_forces(handle(env, argv[0]), static_cast<double*>(data), maxr, &count);
+  if (rc) return throw_lbm(env, rc);
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, (size_t)count * 6, ab, 0, &arr));
+  return arr;
+}
+
+napi_value ReadClasses(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* data;
+  size_t len;
+  if (argc < 2 || !typed(env, argv[1], napi_uint8_array, &data, &len)) {
+    napi_throw_type_error(env, nullptr, "readClasses(solve, Uint8Array)");
+    return nullptr;
+  }
+  int rc = lbm_read_classes(handle(env, argv[0]), static_cast<uint8_t*>(data));
+  if (rc) return throw_lbm(env, rc);
+  return argv[1];
+}
+
+struct StepWork {
+  napi_async_work work;
+  napi_deferred deferred;
+  lbm_solve* s;
+  int n;
+  int rc;
+  std::string err;
+};
+
+void StepExecute(napi_env, void* p) {
+  StepWork* w = static_cast<StepWork*>(p);
+  w->rc = lbm_step(w->s, w->n);
+  if (!w->rc) w->rc = lbm_sync(w->s);
+  if (w->rc) w->err = lbm_last_error();
+}
+
+void StepComplete(napi_env env, napi_status, void* p) {
+  StepWork* w = static_cast<StepWork*>(p);
+  if (w->rc) {
+    napi_value msg, err;
+    std::string m = "lbm error " + std::to_string(w->rc) + ": " + w->err;
+    napi_create_string_utf8(env, m.c_str(), m.size(), &msg);
+    napi_create_error(env, nullptr, msg, &err);
+    napi_reject_deferred(env, w->deferred, err);
+  } else {
+    napi_value undef;
+    napi_get_undefined(env, &undef);
+    napi_resolve_deferred(env, w->deferred, undef);
+  }
+  napi_delete_async_work(env, w->work);
+  delete w;
+}
+
+// step(solve, n) -> Promise resolved once the n steps have completed.
+napi_value Step(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  StepWork* w = new StepWork();
+  w->s = handle(env, argv[0]);
+  w->n = 1;
+  if (argc > 1) napi_get_value_int32(env, argv[1], &w->n);
+  napi_value promise, name;
+  NAPI_OK(napi_create_promise(env, &w->deferred, &promise));
+  NAPI_OK(napi_create_string_utf8(env, "lbm_step", NAPI_AUTO_LENGTH, &name));
+  NAPI_OK(napi_create_async_work(env, nullptr, name, StepExecute, StepComplete, w, &w->work));
+  NAPI_OK(napi_queue_async_work(env, w->work));
+  return promise;
+}
+
+napi_value Info(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  int64_t step = 0, bytes = 0;
+  int32_t bx = 0, by = 0, swz = 0;
+  int rc = lbm_solve_info(handle(env, argv[0]), &step, &bx, &by, &swz, &bytes);
+  if (rc) return throw_lbm(env, rc);
+  napi_value o, v;
+  NAPI_OK(napi_create_object(env, &o));
+  napi_create_double(env, (double)step, &v);
+  napi_set_named_property(env, o, "step", v);
+  napi_create_int32(env, bx, &v);
+  napi_set_named_property(env, o, "blockX", v);
+  napi_create_int32(env, by, &v);
+  napi_set_named_property(env, o, "blockY", v);
+  napi_create_int32(env, swz, &v);
+  napi_set_named_property(env, o, "swizzle", v);
+  napi_create_double(env, (double)bytes, &v);
+  napi_set_named_property(env, o, "deviceBytes", v);
+  return o;
+}
+
+napi_value DeviceInfo(napi_env env, napi_callback_info) {
+  char name[256] = {0};
+  double hbm = 0;
+  int32_t cus = 0;
+  int rc = lbm_device_info(name, sizeof name, &hbm, &cus);
+  if (rc) return throw_lbm(env, rc);
+  napi_value o, v;
+  NAPI_OK(napi_create_object(env, &o));
+  napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &v);
+  napi_set_named_property(env, o, "name", v);
+  napi_create_double(env, hbm, &v);
+  napi_set_named_property(env, o, "hbmBytes", v);
+  napi_create_int32(env, cus, &v);
+  napi_set_named_property(env, o, "computeUnits", v);
+  return o;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+  struct Fn { const char* name; napi_callback cb; };
+  const Fn fns[] = {
+      {"createSolve", CreateSolve},
+      {"destroy", Destroy},
+      {"uploadMacro", UploadMacro},
+      {"uploadF", UploadF},
+      {"readF", ReadFloat<lbm_read_f>},
+      {"readMacro", ReadFloat<lbm_read_macro>},
+      {"readTotals", ReadTotals},
+      {"readForces", ReadForces},
+      {"readClasses", ReadClasses},
+      {"step", Step},
+      {"info", Info},
+      {"deviceInfo", DeviceInfo},
+  };
+  for (const Fn& f : fns) {
+    napi_value fn;
+    if (napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn) != napi_ok) return nullptr;
+    if (napi_set_named_property(env, exports, f.name, fn) != napi_ok) return nullptr;
+  }
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)

@@ -1,0 +1,408 @@
+// Dense-lattice kernels for gfx950: fused pull-stream + collide (D2Q9, D3Q19,
+// D3Q27; BGK or TRT; Guo forcing; halfway bounce-back on walls and a pinned
+// body; ALBC sponge; SOLID_EQ), equilibrium initialisation, FP64
+// momentum-exchange force and FP64 population totals.
+//
+// Semantics follow the reference kernels
+//   shaders/common_d3_step.wgsl:206-222 (initEq), :224-400 (step)
+//   shaders/lbm_step.wgsl:137-270 (D2Q9 main)
+//   plans/TRT.md:333-349 (TRT delta form)
+//   shaders/common_d3_force.wgsl:171-216, shaders/lbm_force.wgsl:128-147 (MEM)
+// The arithmetic is rewritten in shifted variables g_i = f_i - w_i:
+//   rho - 1 = sum g_i (+ (sum w_i - 1), nonzero only for D2Q9's weights)
+//   rho u   = sum g_i e_i + F/2          (sum w_i e_i = 0 exactly)
+//   g_eq    = w_i ((rho - 1) + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u.u))
+// which is the same collision with the large constant w_i taken out before
+// rounding.
+// Layout (structure of arrays, plane-major): g[i * ncells + (z*ny + y)*nx + x].
+// One thread owns one cell; a wave covers 64 consecutive x, so every plane
+// load and store of a wave is one contiguous 256 B (f32) or 128 B (16-bit)
+// span, shifted by at most one element for the x-moving directions.
+#include <hip/hip_runtime.h>
+#include "dense.h"
+#include "device_common.h"
+#include "lattice_tables.h"
+
+namespace lbm {
+
+template <int Q, class ST, bool GEN>
+__global__ void __launch_bounds__(256)
+k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout, StepArgs a) {
+  using L = Lattice<Q>;
+  uint32_t b = blockIdx.x;
+  if (a.swizzle) {
+    // Hardware hands block b to XCD b % 8.  Re-number so each XCD walks one
+    // contiguous eighth of the tiles: neighbouring rows then share an L2.
+    const uint32_t per = a.nblocks >> 3;
+    if (b < per * 8u) b = (b & 7u) * per + (b >> 3);
+  }
+  const int tx = b % a.ntx;
+  const int rest = b / a.ntx;
+  const int ty = rest % a.nty;
+  const int z = rest / a.nty;
+  const int x = tx * blockDim.x + threadIdx.x;
+  const int y = ty * blockDim.y + threadIdx.y;
+  if (x >= a.nx || y >= a.ny || z >= a.nz) return;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const uint32_t N = a.ncells;
+  const uint32_t cell = (uint32_t)((z * ny + y) * nx + x);
+
+  // periodic neighbour coordinates
+  const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+  const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+  const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
+
+  uint8_t c = 0;
+  if constexpr (GEN) {
+    if (a.cls) c = a.cls[cell];
+  }
+
+  // 1. pull-stream (+ halfway bounce-back on the body, then on walls)
+  float g[Q];
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
+    const int sx = ex == 1 ? xm : (ex == -1 ? xp : x);
+    const int sy = ey == 1 ? ym : (ey == -1 ? yp : y);
+    const int sz = ez == 1 ? zm : (ez == -1 ? zp : z);
+    const uint32_t src = (uint32_t)((sz * ny + sy) * nx + sx);
+    const typename ST::T* plane = gin + (size_t)i * N;
+    if constexpr (GEN) {
+      // w_i = w_opp(i), so a reflected population keeps its shift
+      const typename ST::T* oplane = gin + (size_t)L::opp[i] * N;
+      bool wallOut = false;
+      bool yOut = false;
+      if (a.wall_mask) {
+        const bool xo = (ex == 1 && x == 0) || (ex == -1 && x == nx - 1);
+        yOut = (ey == 1 && y == 0) || (ey == -1 && y == ny - 1);
+        const bool zo = (ez == 1 && z == 0) || (ez == -1 && z == nz - 1);
+        wallOut = ((a.wall_mask & 1) && xo) || ((a.wall_mask & 2) && yOut) || ((a.wall_mask & 4) && zo);
+        yOut = yOut && (a.wall_mask & 2);
+      }
+      if ((c & 2) && (a.cls[src] & 1)) {
+        g[i] = ST::load(oplane, cell);
+      } else if (wallOut) {
+        float corr = 0.f;
+        if (ex != 0 && yOut) {
+          // moving y-wall (common_walls.wgsl:27-31): 2 w e_x u_wall / cs2
+          const float uw = (ey == 1) ? a.wall_u0 : a.wall_u1;
+          corr = 2.f * L::w[i] * (float)ex * uw / (1.0f / 3.0f);
+        }
+        g[i] = ST::load(oplane, cell) + corr;
+      } else {
+        g[i] = ST::load(plane, src);
+      }
+    } else {
+      g[i] = ST::load(plane, src);
+    }
+  }
+
+  // 2. moments
+  float drho = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    drho += g[i];
+    if (L::ex[i] == 1) mx += g[i]; else if (L::ex[i] == -1) mx -= g[i];
+    if (L::ey[i] == 1) my += g[i]; else if (L::ey[i] == -1) my -= g[i];
+    if (L::ez[i] == 1) mz += g[i]; else if (L::ez[i] == -1) mz -= g[i];
+  }
+  drho += a.wsum_m1;
+  const float rho = 1.f + drho;
+  const float rinv = 1.0f / fmaxf(rho, 1e-6f);
+  const float Fx = a.F[0], Fy = a.F[1], Fz = a.F[2];
+  // Guo: u = u* + F / (2 rho)
+  const float ux = (mx + 0.5f * Fx) * rinv;
+  const float uy = (my + 0.5f * Fy) * rinv;
+  const float uz = (mz + 0.5f * Fz) * rinv;
+  const float usq = ux * ux + uy * uy + uz * uz;
+  const float uF = ux * Fx + uy * Fy + uz * Fz;
+
+  bool solid = false;
+  if constexpr (GEN) solid = a.solid_eq && (c & 1);
+
+  if (a.mac) {
+    float4 m = make_float4(rho, ux, uy, uz);
+    if (solid && a.solid_mac_rest) m = make_float4(1.f, 0.f, 0.f, 0.f);
+    reinterpret_cast<float4*>(a.mac)[cell] = m;
+  }
+
+  if constexpr (GEN) {
+    if (solid) {
+      // SOLID_EQ for a pinned body: feq(1, 0) = w_i, i.e. g = 0
+#pragma unroll
+      for (int i = 0; i < Q; i++) ST::store(gout + (size_t)i * N, cell, 0.f);
+      return;
+    }
+  }
+
+  float sp = 0.f;
+  if constexpr (GEN) {
+    if (a.sponge_w > 0.f) {
+      const float big = 1e30f;
+      const float dX = (a.sponge_mask & 1) ? fminf((float)x, (float)(nx - 1 - x)) : big;
+      const float dY = (a.sponge_mask & 2) ? fminf((float)y, (float)(ny - 1 - y)) : big;
+      const float dZ = (a.sponge_mask & 4) ? fminf((float)z, (float)(nz - 1 - z)) : big;
+      float t = 1.f - fminf(dX, fminf(dY, dZ)) / a.sponge_w;
+      t = fminf(fmaxf(t, 0.f), 1.f);
+      sp = t * t * (3.f - 2.f * t);
+    }
+  }
+  const float uFarSq = a.su[0] * a.su[0] + a.su[1] * a.su[1] + a.su[2] * a.su[2];
+
+  // 3. collide: BGK + Guo, TRT delta terms, sponge blend
+  const float om = a.omega, dw = a.dw, guo = a.guo;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const float exf = (float)L::ex[i], eyf = (float)L::ey[i], ezf = (float)L::ez[i];
+    const float w = L::w[i];
+    const float eu = exf * ux + eyf * uy + ezf * uz;
+    const float geq = w * (drho + rho * (3.f * eu + 4.5f * eu * eu - 1.5f * usq));
+    const float eF = exf * Fx + eyf * Fy + ezf * Fz;
+    const float Si = guo * w * (3.f * (eF - uF) + 9.f * eu * eF);
+    float gc = g[i] - om * (g[i] - geq) + Si;
+    if (dw != 0.f) {
+      const float gMinus = 0.5f * (g[i] - g[L::opp[i]]);
+      const float eqMinus = 3.f * w * rho * eu;
+      gc = gc - dw * (gMinus - eqMinus) - 0.5f * dw * (3.f * w * eF);
+    }
+    if constexpr (GEN) {
+      if (sp > 0.f) {
+        const float euF = exf * a.su[0] + eyf * a.su[1] + ezf * a.su[2];
+        const float gt = w * (3.f * euF + 4.5f * euF * euF - 1.5f * uFarSq);
+        gc = gc * (1.f - sp) + gt * sp;
+      }
+    }
+    ST::store(gout + (size_t)i * N, cell, gc);
+  }
+}
+
+template <int Q, class ST>
+__global__ void __launch_bounds__(256)
+k_init_eq(const float* __restrict__ mac, typename ST::T* __restrict__ g, uint32_t N) {
+  using L = Lattice<Q>;
+  const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= N) return;
+  const float4 m = reinterpret_cast<const float4*>(mac)[cell];
+  const float usq = m.y * m.y + m.z * m.z + m.w * m.w;
+  const float drho = m.x - 1.f;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const float eu = (float)L::ex[i] * m.y + (float)L::ey[i] * m.z + (float)L::ez[i] * m.w;
+    const float geq = L::w[i] * (drho + m.x * (3.f * eu + 4.5f * eu * eu - 1.5f * usq));
+    ST::store(g + (size_t)i * N, cell, geq);
+  }
+}
+
+// Wire format (f32 plane-major f) <-> storage (shifted g), f-pack.mjs:93-124.
+template <int Q, class ST>
+__global__ void __launch_bounds__(256)
+k_pack(const float* __restrict__ wire, typename ST::T* __restrict__ g, uint32_t N) {
+  using L = Lattice<Q>;
+  const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= N) return;
+#pragma unroll
+  for (int i = 0; i < Q; i++) ST::store(g + (size_t)i * N, cell, wire[(size_t)i * N + cell] - L::w[i]);
+}
+
+template <int Q, class ST>
+__global__ void __launch_bounds__(256)
+k_unpack(const typename ST::T* __restrict__ g, float* __restrict__ wire, uint32_t N) {
+  using L = Lattice<Q>;
+  const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= N) return;
+#pragma unroll
+  for (int i = 0; i < Q; i++) wire[(size_t)i * N + cell] = ST::load(g + (size_t)i * N, cell) + L::w[i];
+}
+
+// Momentum exchange on the pinned body from pre-stream populations, summed in
+// FP64 (the reference sums per workgroup in f32 and accumulates i32 fixed
+// point, common_d3_force.wgsl:271-282).
+template <int Q, class ST>
+__global__ void __launch_bounds__(256)
+k_force(const typename ST::T* __restrict__ gin, ForceArgs a) {
+  using L = Lattice<Q>;
+  __shared__ double lds[16 * 6];
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < a.count) {
+    const uint32_t cell = a.cells[k];
+    const uint32_t mask = a.masks[k];
+    double fx = 0, fy = 0, fz = 0;
+#pragma unroll
+    for (int i = 1; i < Q; i++) {
+      if (mask & (1u << i)) {
+        const double f_opp = (double)ST::load(gin + (size_t)L::opp[i] * a.ncells, cell) + (double)L::w[i];
+        const double m = 2.0 * f_opp;
+        fx -= L::ex[i] * m;
+        fy -= L::ey[i] * m;
+        fz -= L::ez[i] * m;
+      }
+    }
+    const int x = cell % a.nx;
+    const int y = (cell / a.nx) % a.ny;
+    const int z = cell / (a.nx * a.ny);
+    double rx = x - a.cx, ry = y - a.cy, rz = z - a.cz;
+    if (a.periodic_arm) {
+      rx -= a.nx * rint(rx / a.nx);
+      ry -= a.ny * rint(ry / a.ny);
+      rz = 0.0;
+    }
+    v[0] = fx; v[1] = fy; v[2] = fz;
+    v[3] = ry * fz - rz * fy;
+    v[4] = rz * fx - rx * fz;
+    v[5] = rx * fy - ry * fx;
+  }
+  block_sum<6>(v, lds);
+  partials_then_fixed_order_sum<6>(v, a.partials, a.counter, [&](int j, double s) {
+    // The six writers are lanes of one wave, so every history store issues
+    // before lane 0 advances the count.
+    const unsigned long long n = *a.written;
+    a.history[(size_t)(n % a.capacity) * 6 + j] = s;
+    if (j == 0) *a.written = n + 1;
+  });
+}
+
+template <int Q, class ST>
+__global__ void __launch_bounds__(256)
+k_totals(const typename ST::T* __restrict__ g, TotalsArgs a) {
+  using L = Lattice<Q>;
+  __shared__ double lds[16 * 4];
+  double v[4] = {0, 0, 0, 0};
+  for (uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x; cell < a.ncells; cell += gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+      const double fi = (double)ST::load(g + (size_t)i * a.ncells, cell) + (double)L::w[i];
+      v[0] += fi;
+      v[1] += L::ex[i] * fi;
+      v[2] += L::ey[i] * fi;
+      v[3] += L::ez[i] * fi;
+    }
+  }
+  block_sum<4>(v, lds);
+  partials_then_fixed_order_sum<4>(v, a.partials, a.counter, [&](int j, double s) { a.out[j] = s; });
+}
+
+// Per-cell class byte from the signed distance of the pinned body, evaluated
+// in f32 with one rounding per operation (no contraction) so the host oracle
+// reproduces it exactly:
+//   sphere  phi = sqrt((x-cx)^2 + (y-cy)^2 + (z-cz)^2) - a
+//           (common_d3_geometry.wgsl:119, :174-187)
+//   circle  nearest periodic image of (x-cx, y-cy), then length - a
+//           (common_geometry.wgsl:94-108)
+__global__ void __launch_bounds__(256) k_classify(ClassifyArgs a) {
+  const uint32_t n = (uint32_t)a.nx * a.ny * a.nz;
+  const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= n) return;
+  const int x = cell % a.nx;
+  const int y = (cell / a.nx) % a.ny;
+  const int z = cell / (a.nx * a.ny);
+  float dx = __fsub_rn((float)x, a.cx);
+  float dy = __fsub_rn((float)y, a.cy);
+  float phi;
+  if (a.shape == 2) {
+    const float W = (float)a.nx, H = (float)a.ny;
+    dx = __fsub_rn(dx, __fmul_rn(W, rintf(__fdiv_rn(dx, W))));
+    dy = __fsub_rn(dy, __fmul_rn(H, rintf(__fdiv_rn(dy, H))));
+    phi = __fsub_rn(__fsqrt_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy))), a.radius);
+  } else {
+    const float dz = __fsub_rn((float)z, a.cz);
+    const float s = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+    phi = __fsub_rn(__fsqrt_rn(s), a.radius);
+  }
+  uint8_t c = 0;
+  if (phi < 0.f) c |= 1;
+  if (phi < 2.f) c |= 2;
+  a.cls[cell] = c;
+}
+
+// ---------------------------------------------------------------------------
+// dispatch: one Op<Q, Storage>::run per kernel family
+// ---------------------------------------------------------------------------
+template <template <int, class> class Op, class... A>
+static hipError_t dispatch(int q, int storage, A... args) {
+  switch (q) {
+    case 9:
+      if (storage == kF32) return Op<9, StoreF32>::run(args...);
+      if (storage == kF16) return Op<9, StoreF16>::run(args...);
+      return Op<9, StoreBF16>::run(args...);
+    case 19:
+      if (storage == kF32) return Op<19, StoreF32>::run(args...);
+      if (storage == kF16) return Op<19, StoreF16>::run(args...);
+      return Op<19, StoreBF16>::run(args...);
+    default:
+      if (storage == kF32) return Op<27, StoreF32>::run(args...);
+      if (storage == kF16) return Op<27, StoreF16>::run(args...);
+      return Op<27, StoreBF16>::run(args...);
+  }
+}
+
+static inline dim3 cells_grid(uint32_t n) { return dim3((n + 255) / 256); }
+
+template <int Q, class ST> struct OpStep {
+  static hipError_t run(bool general, const void* fin, void* fout, StepArgs a, int bx, int by, hipStream_t s) {
+    using T = typename ST::T;
+    if (general)
+      hipLaunchKernelGGL((k_step<Q, ST, true>), dim3(a.nblocks), dim3(bx, by), 0, s, (const T*)fin, (T*)fout, a);
+    else
+      hipLaunchKernelGGL((k_step<Q, ST, false>), dim3(a.nblocks), dim3(bx, by), 0, s, (const T*)fin, (T*)fout, a);
+    return hipGetLastError();
+  }
+};
+template <int Q, class ST> struct OpInit {
+  static hipError_t run(const float* mac, void* f, uint32_t N, hipStream_t s) {
+    hipLaunchKernelGGL((k_init_eq<Q, ST>), cells_grid(N), dim3(256), 0, s, mac, (typename ST::T*)f, N);
+    return hipGetLastError();
+  }
+};
+template <int Q, class ST> struct OpPack {
+  static hipError_t run(const float* wire, void* f, uint32_t N, hipStream_t s) {
+    hipLaunchKernelGGL((k_pack<Q, ST>), cells_grid(N), dim3(256), 0, s, wire, (typename ST::T*)f, N);
+    return hipGetLastError();
+  }
+};
+template <int Q, class ST> struct OpUnpack {
+  static hipError_t run(const void* f, float* wire, uint32_t N, hipStream_t s) {
+    hipLaunchKernelGGL((k_unpack<Q, ST>), cells_grid(N), dim3(256), 0, s, (const typename ST::T*)f, wire, N);
+    return hipGetLastError();
+  }
+};
+template <int Q, class ST> struct OpForce {
+  static hipError_t run(const void* fin, ForceArgs a, hipStream_t s) {
+    hipLaunchKernelGGL((k_force<Q, ST>), cells_grid(a.count), dim3(256), 0, s, (const typename ST::T*)fin, a);
+    return hipGetLastError();
+  }
+};
+template <int Q, class ST> struct OpTotals {
+  static hipError_t run(const void* f, TotalsArgs a, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_totals<Q, ST>), dim3(nblocks), dim3(256), 0, s, (const typename ST::T*)f, a);
+    return hipGetLastError();
+  }
+};
+
+hipError_t launch_step(int q, int storage, bool general, const void* fin, void* fout,
+                       const StepArgs& a, int bx, int by, hipStream_t s) {
+  return dispatch<OpStep>(q, storage, general, fin, fout, a, bx, by, s);
+}
+hipError_t launch_init_eq(int q, int storage, const float* mac, void* f, uint32_t N, hipStream_t s) {
+  return dispatch<OpInit>(q, storage, mac, f, N, s);
+}
+hipError_t launch_pack(int q, int storage, const float* wire, void* f, uint32_t N, hipStream_t s) {
+  return dispatch<OpPack>(q, storage, wire, f, N, s);
+}
+hipError_t launch_unpack(int q, int storage, const void* f, float* wire, uint32_t N, hipStream_t s) {
+  return dispatch<OpUnpack>(q, storage, f, wire, N, s);
+}
+hipError_t launch_force(int q, int storage, const void* fin, const ForceArgs& a, hipStream_t s) {
+  return dispatch<OpForce>(q, storage, fin, a, s);
+}
+hipError_t launch_totals(int q, int storage, const void* f, const TotalsArgs& a, int nblocks, hipStream_t s) {
+  return dispatch<OpTotals>(q, storage, f, a, nblocks, s);
+}
+
+hipError_t launch_classify(const ClassifyArgs& a, hipStream_t s) {
+  const uint32_t n = (uint32_t)a.nx * a.ny * a.nz;
+  hipLaunchKernelGGL(k_classify, dim3((n + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace lbm

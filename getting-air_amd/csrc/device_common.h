@@ -1,0 +1,119 @@
+// Device-side helpers shared by the engine's HIP translation units:
+// population storage formats, FP64 wave/block reductions and the fixed-order
+// "last block sums the partials" pattern.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+
+// ---------------------------------------------------------------------------
+// Population storage.  Every format holds the SHIFTED population
+// g_i = f_i - w_i (the reference's f-pack mode 2, common_fpack.wgsl:105-121,
+// applied to all widths), and the kernels collide in g directly.  g is one to
+// two orders of magnitude smaller than f, so the same number of bits carries
+// that much less absolute round-off; the arithmetic is FP32 throughout.
+// load/store move g; the 16-bit formats round to nearest even.
+// ---------------------------------------------------------------------------
+struct StoreF32 {
+  using T = float;
+  static constexpr int kBytes = 4;
+  __device__ __forceinline__ static float load(const T* p, uint32_t i) { return p[i]; }
+  __device__ __forceinline__ static void store(T* p, uint32_t i, float g) { p[i] = g; }
+};
+
+__device__ __forceinline__ float half_bits_to_float(uint16_t h) {
+  return static_cast<float>(__builtin_bit_cast(_Float16, h));
+}
+__device__ __forceinline__ uint16_t float_to_half_bits(float v) {
+  // fptrunc lowers to v_cvt_f16_f32, round-to-nearest-even with f16
+  // subnormals (the default gfx950 mode).
+  return __builtin_bit_cast(uint16_t, static_cast<_Float16>(v));
+}
+__device__ __forceinline__ float bf16_bits_to_float(uint16_t h) {
+  return __uint_as_float(static_cast<uint32_t>(h) << 16);
+}
+__device__ __forceinline__ uint16_t float_to_bf16_bits(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);                                                      // ties to even
+  return static_cast<uint16_t>(u >> 16);
+}
+
+struct StoreF16 {
+  using T = uint16_t;
+  static constexpr int kBytes = 2;
+  __device__ __forceinline__ static float load(const T* p, uint32_t i) { return half_bits_to_float(p[i]); }
+  __device__ __forceinline__ static void store(T* p, uint32_t i, float g) { p[i] = float_to_half_bits(g); }
+};
+
+struct StoreBF16 {
+  using T = uint16_t;
+  static constexpr int kBytes = 2;
+  __device__ __forceinline__ static float load(const T* p, uint32_t i) { return bf16_bits_to_float(p[i]); }
+  __device__ __forceinline__ static void store(T* p, uint32_t i, float g) { p[i] = float_to_bf16_bits(g); }
+};
+
+// ---------------------------------------------------------------------------
+// FP64 reductions.  Wave64 butterfly with __shfl_xor, then one slot per wave
+// in LDS, then wave 0 folds the slots.  The order of every addition is fixed
+// by the launch shape, so results repeat bit for bit.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sums K doubles across a block of up to 1024 threads; lane 0 of wave 0 ends
+// with the totals in v[].  `lds` holds at least 16*K doubles.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double* lds) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthreads = blockDim.x * blockDim.y;
+  const int lane = tid & 63, wave = tid >> 6, nwaves = (nthreads + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) lds[wave * K + k] = v[k];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = lane < nwaves ? lds[lane * K + k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+  }
+}
+
+// Writes this block's K partials, then the last block to arrive sums all
+// blocks' partials in block order and hands the totals to `finish`.  No block
+// waits on another, so the kernel always drains.  `counter` must be zero on
+// entry; the last block resets it.
+template <int K, class Finish>
+__device__ __forceinline__ void partials_then_fixed_order_sum(const double (&v)[K], double* partials,
+                                                              unsigned* counter, Finish finish) {
+  __shared__ bool last;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const unsigned nblocks = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) partials[(size_t)b * K + k] = v[k];
+    __threadfence();
+    last = atomicAdd(counter, 1u) == nblocks - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (tid < K) {
+    double s = 0.0;
+    for (unsigned j = 0; j < nblocks; j++)
+      s += __hip_atomic_load(&partials[(size_t)j * K + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    finish(tid, s);
+  }
+  if (tid == 0) *counter = 0u;
+}
+
+}  // namespace lbm

@@ -1,0 +1,628 @@
+// Host side of the dense engine: the lbm_solve object behind include/lbm.h.
+//
+// A solve owns two population lattices (ping-pong), the macroscopic field,
+// the body classification and its boundary-link list, the FP64 reduction
+// scratch and one HIP stream.  Each lbm_step(n) enqueues, per step, the FP64
+// force reduction on the pre-stream lattice (when a body is present) and the
+// fused stream-collide; only the last step of a call writes the macroscopic
+// field, which is what the reference's readbacks observe
+// (main-3d.js:3506-3539).  Chunks of steps are replayed from captured
+// hipGraphs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/lbm.h"
+#include "dense.h"
+#include "engine.h"
+
+namespace lbm {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(LBM_E_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+#define HIPCHK(call)                                 \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, #call); \
+  } while (0)
+
+namespace {
+
+// D2Q9 keeps the plainly rounded f32 weights (lattice-2d.mjs:119-125), whose
+// sum is 1 + this; the shifted collision adds it back into rho - 1.
+const float kD2Q9WeightSumMinus1 =
+    (float)((double)0.44444444775581360f + 4.0 * (double)0.11111111193895340f + 4.0 * (double)0.027777777984738350f - 1.0);
+
+// Tile shapes the autotuner times; block_x * block_y <= 256 (launch bound).
+const int kShapes[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {256, 1}, {32, 4}, {32, 8}};
+
+struct TuneKey {
+  int q, storage, general, nx, ny, nz;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(q, storage, general, nx, ny, nz) < std::tie(o.q, o.storage, o.general, o.nx, o.ny, o.nz);
+  }
+};
+struct TuneVal { int bx, by, swz; };
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune_cache;
+
+const uint32_t kHistoryCap = 1u << 16;
+const int kGraphChunk = 16;  // steps per captured graph (even: parity returns)
+
+}  // namespace
+
+struct lbm_solve {
+  lbm_desc d{};
+  int q = 0, dim = 3, storage = 0, elem = 4;
+  uint32_t N = 0;
+  bool general = false;
+  void* f[2] = {nullptr, nullptr};
+  int parity = 0;
+  float* mac = nullptr;
+  uint8_t* cls = nullptr;
+  uint32_t* bcells = nullptr;
+  uint32_t* bmasks = nullptr;
+  uint32_t nboundary = 0;
+  double* scratch = nullptr;      // partial sums (force + totals)
+  unsigned* counters = nullptr;   // [force, totals]
+  double* history = nullptr;
+  unsigned long long* written = nullptr;
+  double* totals = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int bx = 64, by = 1, swz = 1;
+  bool tuned = false;
+  bool initialised = false;
+  long long step = 0;
+  size_t bytes = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int graph_bx = 0, graph_by = 0, graph_swz = 0;
+};
+
+namespace {
+
+template <class T>
+int dalloc(lbm_solve* s, T** p, size_t bytes) {
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(LBM_E_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+  }
+  s->bytes += bytes;
+  return LBM_OK;
+}
+
+int validate(const lbm_desc* d) {
+  if (!d) return fail(LBM_E_INVALID, "null descriptor");
+  if (d->abi_version != LBM_ABI_VERSION)
+    return fail(LBM_E_INVALID, "descriptor abi_version %d, library %d", d->abi_version, LBM_ABI_VERSION);
+  if (d->q != 9 && d->q != 19 && d->q != 27) return fail(LBM_E_INVALID, "q must be 9, 19 or 27 (got %d)", d->q);
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(LBM_E_INVALID, "grid %dx%dx%d", d->nx, d->ny, d->nz);
+  if (d->q == 9 && d->nz != 1) return fail(LBM_E_INVALID, "D2Q9 needs nz = 1 (got %d)", d->nz);
+  const double n = (double)d->nx * d->ny * d->nz;
+  if (n * 27 > 4.0e12 || n >= 2147483647.0) return fail(LBM_E_INVALID, "grid too large (%g cells)", n);
+  if (!(d->omega > 0.f && d->omega < 2.f)) return fail(LBM_E_INVALID, "omega %g outside (0, 2)", d->omega);
+  if (d->collision != LBM_BGK && d->collision != LBM_TRT) return fail(LBM_E_INVALID, "collision %d", d->collision);
+  if (d->collision == LBM_TRT && !(d->omega_minus > 0.f && d->omega_minus < 2.f))
+    return fail(LBM_E_INVALID, "omega_minus %g outside (0, 2)", d->omega_minus);
+  if (d->storage < LBM_STORE_F32 || d->storage > LBM_STORE_BF16) return fail(LBM_E_INVALID, "storage %d", d->storage);
+  const int sh = d->body.shape;
+  if (sh != LBM_BODY_NONE) {
+    if (d->q == 9 && sh != LBM_BODY_CIRCLE) return fail(LBM_E_INVALID, "D2Q9 bodies must be circles");
+    if (d->q != 9 && sh != LBM_BODY_SPHERE) return fail(LBM_E_INVALID, "3D bodies must be spheres");
+    if (!(d->body.radius > 0.f)) return fail(LBM_E_INVALID, "body radius %g", d->body.radius);
+  }
+  for (int a = 0; a < 3; a++) {
+    if (!std::isfinite(d->force[a]) || !std::isfinite(d->sponge_u[a]))
+      return fail(LBM_E_INVALID, "non-finite force or sponge velocity");
+  }
+  return LBM_OK;
+}
+
+StepArgs step_args(lbm_solve* s, bool write_mac) {
+  StepArgs a{};
+  const lbm_desc& d = s->d;
+  a.nx = d.nx; a.ny = d.ny; a.nz = d.nz;
+  a.ncells = s->N;
+  a.omega = d.omega;
+  a.dw = d.collision == LBM_TRT ? (float)(d.omega_minus - d.omega) : 0.f;
+  a.guo = 1.f - 0.5f * d.omega;
+  a.wsum_m1 = s->q == 9 ? kD2Q9WeightSumMinus1 : 0.f;
+  for (int k = 0; k < 3; k++) a.F[k] = d.force[k];
+  a.wall_mask = (d.walls[0] ? 1 : 0) | (d.walls[1] ? 2 : 0) | (d.walls[2] && s->dim == 3 ? 4 : 0);
+  a.wall_u0 = d.wall_u[0];
+  a.wall_u1 = d.wall_u[1];
+  a.sponge_w = d.sponge_width;
+  a.sponge_mask = (d.sponge_axes[0] ? 1 : 0) | (d.sponge_axes[1] ? 2 : 0) | (d.sponge_axes[2] && s->dim == 3 ? 4 : 0);
+  for (int k = 0; k < 3; k++) a.su[k] = d.sponge_u[k];
+  a.cls = d.body.shape != LBM_BODY_NONE ? s->cls : nullptr;
+  a.solid_eq = d.solid_eq;
+  a.solid_mac_rest = s->dim == 3;
+  a.mac = write_mac ? s->mac : nullptr;
+  a.ntx = (d.nx + s->bx - 1) / s->bx;
+  a.nty = (d.ny + s->by - 1) / s->by;
+  a.nblocks = (uint32_t)a.ntx * a.nty * d.nz;
+  a.swizzle = s->swz;
+  return a;
+}
+
+ForceArgs force_args(lbm_solve* s) {
+  ForceArgs a{};
+  a.cells = s->bcells;
+  a.masks = s->bmasks;
+  a.count = s->nboundary;
+  a.ncells = s->N;
+  a.nx = s->d.nx; a.ny = s->d.ny; a.nz = s->d.nz;
+  a.cx = s->d.body.centre[0];
+  a.cy = s->d.body.centre[1];
+  a.cz = s->dim == 3 ? s->d.body.centre[2] : 0.0;
+  a.periodic_arm = s->dim == 2;
+  a.partials = s->scratch;
+  a.counter = s->counters;
+  a.history = s->history;
+  a.written = s->written;
+  a.capacity = kHistoryCap;
+  return a;
+}
+
+// One step: force on the pre-stream lattice, then stream-collide.
+int enqueue_step(lbm_solve* s, bool write_mac) {
+  const void* fin = s->f[s->parity];
+  void* fout = s->f[1 - s->parity];
+  if (s->nboundary) {
+    hipError_t e = launch_force(s->q, s->storage, fin, force_args(s), s->stream);
+    if (e != hipSuccess) return hip_fail(e, "force kernel launch");
+  }
+  StepArgs a = step_args(s, write_mac);
+  hipError_t e = launch_step(s->q, s->storage, s->general, fin, fout, a, s->bx, s->by, s->stream);
+  if (e != hipSuccess) return hip_fail(e, "step kernel launch");
+  s->parity ^= 1;
+  s->step++;
+  return LBM_OK;
+}
+
+void drop_graphs(lbm_solve* s) {
+  for (int p = 0; p < 2; p++) {
+    if (s->graph[p]) hipGraphExecDestroy(s->graph[p]);
+    s->graph[p] = nullptr;
+  }
+}
+
+// Captures kGraphChunk steps starting at the current parity.
+int build_graph(lbm_solve* s) {
+  const int p = s->parity;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  const long long step0 = s->step;
+  int rc = LBM_OK;
+  for (int k = 0; k < kGraphChunk && rc == LBM_OK; k++) rc = enqueue_step(s, false);
+  hipError_t e = hipStreamEndCapture(s->stream, &g);
+  s->step = step0;
+  s->parity = p;
+  if (rc != LBM_OK) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  e = hipGraphInstantiate(&s->graph[p], g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  s->graph_bx = s->bx; s->graph_by = s->by; s->graph_swz = s->swz;
+  return LBM_OK;
+}
+
+// Times every candidate tile shape on the live lattices (fin -> fout without
+// swapping, so the state is unchanged) and keeps the fastest.
+int autotune(lbm_solve* s) {
+  TuneKey key{s->q, s->storage, s->general ? 1 : 0, s->d.nx, s->d.ny, s->d.nz};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      s->bx = it->second.bx; s->by = it->second.by; s->swz = it->second.swz;
+      s->tuned = true;
+      return LBM_OK;
+    }
+  }
+  const int reps = s->N >= (1u << 20) ? 3 : 8;
+  float best = 1e30f;
+  TuneVal bestv{s->bx, s->by, s->swz};
+  const void* fin = s->f[s->parity];
+  void* fout = s->f[1 - s->parity];
+  for (const auto& sh : kShapes) {
+    if (sh[0] > 64 && s->d.nx < sh[0] / 2) continue;
+    for (int swz = 0; swz < 2; swz++) {
+      s->bx = sh[0]; s->by = sh[1]; s->swz = swz;
+      StepArgs a = step_args(s, false);
+      HIPCHK(launch_step(s->q, s->storage, s->general, fin, fout, a, s->bx, s->by, s->stream));
+      HIPCHK(hipEventRecord(s->ev0, s->stream));
+      for (int r = 0; r < reps; r++)
+        HIPCHK(launch_step(s->q, s->storage, s->general, fin, fout, a, s->bx, s->by, s->stream));
+      HIPCHK(hipEventRecord(s->ev1, s->stream));
+      HIPCHK(hipEventSynchronize(s->ev1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+      if (ms < best) { best = ms; bestv = TuneVal{sh[0], sh[1], swz}; }
+    }
+  }
+  s->bx = bestv.bx; s->by = bestv.by; s->swz = bestv.swz;
+  s->tuned = true;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = bestv;
+  return LBM_OK;
+}
+
+// Classification -> boundary-link list (ascending cell index) on the host.
+int build_boundary(lbm_solve* s) {
+  const lbm_desc& d = s->d;
+  ClassifyArgs c{};
+  c.nx = d.nx; c.ny = d.ny; c.nz = d.nz;
+  c.shape = d.body.shape == LBM_BODY_CIRCLE ? 2 : 1;
+  c.radius = d.body.radius;
+  c.cx = d.body.centre[0]; c.cy = d.body.centre[1]; c.cz = s->dim == 3 ? d.body.centre[2] : 0.f;
+  c.cls = s->cls;
+  HIPCHK(launch_classify(c, s->stream));
+  std::vector<uint8_t> h(s->N);
+  HIPCHK(hipMemcpyAsync(h.data(), s->cls, s->N, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  static const int ex9[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1}, ey9[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+  static const int ex3[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
+  static const int ey3[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1};
+  static const int ez3[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1};
+  std::vector<uint32_t> cells, masks;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  for (int z = 0; z < nz; z++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const uint32_t cell = (uint32_t)((z * ny + y) * nx + x);
+        const uint8_t cc = h[cell];
+        if (!(cc & 2) || (cc & 1)) continue;
+        uint32_t m = 0;
+        for (int i = 1; i < s->q; i++) {
+          const int ex = s->q == 9 ? ex9[i] : ex3[i];
+          const int ey = s->q == 9 ? ey9[i] : ey3[i];
+          const int ez = s->q == 9 ? 0 : ez3[i];
+          const int sx = (x - ex + nx) % nx, sy = (y - ey + ny) % ny, sz = (z - ez + nz) % nz;
+          if (h[(uint32_t)((sz * ny + sy) * nx + sx)] & 1) m |= 1u << i;
+        }
+        if (m) { cells.push_back(cell); masks.push_back(m); }
+      }
+  s->nboundary = (uint32_t)cells.size();
+  if (s->nboundary) {
+    int rc = dalloc(s, &s->bcells, cells.size() * 4);
+    if (rc) return rc;
+    rc = dalloc(s, &s->bmasks, masks.size() * 4);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s->bcells, cells.data(), cells.size() * 4, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->bmasks, masks.data(), masks.size() * 4, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  return LBM_OK;
+}
+
+void release(lbm_solve* s) {
+  if (!s) return;
+  drop_graphs(s);
+  void* ptrs[] = {s->f[0], s->f[1], s->mac, s->cls, s->bcells, s->bmasks, s->scratch,
+                  s->counters, s->history, s->written, s->totals};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (s->ev0) hipEventDestroy(s->ev0);
+  if (s->ev1) hipEventDestroy(s->ev1);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int reset_history(lbm_solve* s) {
+  HIPCHK(hipMemsetAsync(s->written, 0, sizeof(unsigned long long), s->stream));
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lbm_last_error(void) { return g_last_error.c_str(); }
+
+int lbm_abi_version(void) { return LBM_ABI_VERSION; }
+
+int lbm_device_info(char* name, size_t name_len, double* hbm_bytes, int32_t* cu_count) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, dev));
+  if (name && name_len) {
+    snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (hbm_bytes) *hbm_bytes = (double)p.totalGlobalMem;
+  if (cu_count) *cu_count = p.multiProcessorCount;
+  return LBM_OK;
+}
+
+int lbm_solve_create(const lbm_desc* desc, lbm_solve** out) {
+  if (!out) return fail(LBM_E_INVALID, "null output handle");
+  *out = nullptr;
+  int rc = validate(desc);
+  if (rc) return rc;
+  lbm_solve* s = new lbm_solve();
+  s->d = *desc;
+  s->q = desc->q;
+  s->dim = desc->q == 9 ? 2 : 3;
+  s->storage = desc->storage;
+  s->elem = desc->storage == LBM_STORE_F32 ? 4 : 2;
+  s->N = (uint32_t)((size_t)desc->nx * desc->ny * desc->nz);
+  const bool body = desc->body.shape != LBM_BODY_NONE;
+  const bool walls = desc->walls[0] || desc->walls[1] || (desc->walls[2] && s->dim == 3);
+  s->general = body || walls || desc->sponge_width > 0.f;
+  // default tile: one wave per x-row segment, XCD swizzle on
+  s->bx = 64; s->by = 1; s->swz = 1;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { release(s); return hip_fail(e, "hipStreamCreate"); }
+  if ((e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess) {
+    release(s);
+    return hip_fail(e, "hipEventCreate");
+  }
+  const size_t lattice_bytes = (size_t)s->N * s->q * s->elem;
+  const int nred = 1024;  // max reduction partial blocks
+  if ((rc = dalloc(s, &s->f[0], lattice_bytes)) || (rc = dalloc(s, &s->f[1], lattice_bytes)) ||
+      (rc = dalloc(s, &s->mac, (size_t)s->N * 16)) || (rc = dalloc(s, &s->scratch, (size_t)nred * 8 * 8)) ||
+      (rc = dalloc(s, &s->counters, 64)) || (rc = dalloc(s, &s->history, (size_t)kHistoryCap * 6 * 8)) ||
+      (rc = dalloc(s, &s->written, 8)) || (rc = dalloc(s, &s->totals, 64))) {
+    release(s);
+    return rc;
+  }
+  if ((e = hipMemsetAsync(s->counters, 0, 64, s->stream)) != hipSuccess ||
+      (e = hipMemsetAsync(s->written, 0, 8, s->stream)) != hipSuccess) {
+    release(s);
+    return hip_fail(e, "hipMemsetAsync");
+  }
+  if (body) {
+    if ((rc = dalloc(s, &s->cls, s->N)) || (rc = build_boundary(s))) {
+      release(s);
+      return rc;
+    }
+    // the force partials: one slot of 6 doubles per force block
+    const uint32_t fblocks = (s->nboundary + 255) / 256;
+    if (fblocks > (uint32_t)nred) {
+      hipFree(s->scratch);
+      s->scratch = nullptr;
+      if ((rc = dalloc(s, &s->scratch, (size_t)fblocks * 8 * 8))) {
+        release(s);
+        return rc;
+      }
+    }
+  }
+  if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) {
+    release(s);
+    return hip_fail(e, "hipStreamSynchronize");
+  }
+  *out = s;
+  return LBM_OK;
+}
+
+int lbm_solve_destroy(lbm_solve* s) {
+  if (s) hipStreamSynchronize(s->stream);
+  release(s);
+  return LBM_OK;
+}
+
+int lbm_upload_macro(lbm_solve* s, const float* mac4) {
+  if (!s || !mac4) return fail(LBM_E_INVALID, "null argument");
+  HIPCHK(hipMemcpyAsync(s->mac, mac4, (size_t)s->N * 16, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(launch_init_eq(s->q, s->storage, s->mac, s->f[0], s->N, s->stream));
+  // Both lattices hold the equilibrium (main-3d.js:2994 seeds A and B).
+  HIPCHK(hipMemcpyAsync(s->f[1], s->f[0], (size_t)s->N * s->q * s->elem, hipMemcpyDeviceToDevice, s->stream));
+  s->parity = 0;
+  s->step = 0;
+  s->initialised = true;
+  int rc = reset_history(s);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int lbm_upload_f(lbm_solve* s, const float* f) {
+  if (!s || !f) return fail(LBM_E_INVALID, "null argument");
+  const size_t wire = (size_t)s->N * s->q * 4;
+  float* tmp = nullptr;
+  HIPCHK(hipMalloc((void**)&tmp, wire));
+  hipError_t e = hipMemcpyAsync(tmp, f, wire, hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = launch_pack(s->q, s->storage, tmp, s->f[0], s->N, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->f[1], s->f[0], (size_t)s->N * s->q * s->elem, hipMemcpyDeviceToDevice, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  hipFree(tmp);
+  if (e != hipSuccess) return hip_fail(e, "upload_f");
+  s->parity = 0;
+  s->step = 0;
+  s->initialised = true;
+  return reset_history(s);
+}
+
+int lbm_read_f(lbm_solve* s, float* f) {
+  if (!s || !f) return fail(LBM_E_INVALID, "null argument");
+  const size_t wire = (size_t)s->N * s->q * 4;
+  float* tmp = nullptr;
+  HIPCHK(hipMalloc((void**)&tmp, wire));
+  hipError_t e = launch_unpack(s->q, s->storage, s->f[s->parity], tmp, s->N, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(f, tmp, wire, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  hipFree(tmp);
+  if (e != hipSuccess) return hip_fail(e, "read_f");
+  return LBM_OK;
+}
+
+int lbm_step(lbm_solve* s, int32_t n) {
+  if (!s) return fail(LBM_E_INVALID, "null solve");
+  if (n < 0) return fail(LBM_E_INVALID, "negative step count");
+  if (!s->initialised) return fail(LBM_E_STATE, "step before upload_macro / upload_f");
+  if (n == 0) return LBM_OK;
+  if (s->d.autotune && !s->tuned) {
+    int rc = autotune(s);
+    if (rc) return rc;
+  }
+  if (s->graph_bx != s->bx || s->graph_by != s->by || s->graph_swz != s->swz) drop_graphs(s);
+  int left = n - 1;  // the last step writes the macroscopic field
+  while (left >= kGraphChunk) {
+    if (!s->graph[s->parity]) {
+      int rc = build_graph(s);
+      if (rc) return rc;
+    }
+    HIPCHK(hipGraphLaunch(s->graph[s->parity], s->stream));
+    // kGraphChunk is even, so the parity after the chunk is unchanged
+    s->step += kGraphChunk;
+    left -= kGraphChunk;
+  }
+  while (left-- > 0) {
+    int rc = enqueue_step(s, false);
+    if (rc) return rc;
+  }
+  return enqueue_step(s, true);
+}
+
+int lbm_step_timed(lbm_solve* s, int32_t n, double* ms) {
+  if (!s || !ms) return fail(LBM_E_INVALID, "null argument");
+  if (s->d.autotune && !s->tuned) {
+    int rc = autotune(s);
+    if (rc) return rc;
+  }
+  HIPCHK(hipEventRecord(s->ev0, s->stream));
+  int rc = lbm_step(s, n);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(s->ev1, s->stream));
+  HIPCHK(hipEventSynchronize(s->ev1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, s->ev0, s->ev1));
+  *ms = t;
+  return LBM_OK;
+}
+
+int lbm_sync(lbm_solve* s) {
+  if (!s) return fail(LBM_E_INVALID, "null solve");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int lbm_read_macro(lbm_solve* s, float* mac4) {
+  if (!s || !mac4) return fail(LBM_E_INVALID, "null argument");
+  HIPCHK(hipMemcpyAsync(mac4, s->mac, (size_t)s->N * 16, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int lbm_read_forces(lbm_solve* s, double* out, int32_t max_records, int32_t* count) {
+  if (!s || !count || (max_records > 0 && !out)) return fail(LBM_E_INVALID, "null argument");
+  unsigned long long written = 0;
+  HIPCHK(hipMemcpyAsync(&written, s->written, 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const unsigned long long avail = std::min<unsigned long long>(written, kHistoryCap);
+  const unsigned long long take = std::min<unsigned long long>(avail, (unsigned long long)std::max(0, max_records));
+  // the `take` most recent records, oldest first
+  std::vector<double> ring((size_t)kHistoryCap * 6);
+  if (take) {
+    HIPCHK(hipMemcpyAsync(ring.data(), s->history, ring.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (unsigned long long k = 0; k < take; k++) {
+      const unsigned long long rec = written - take + k;
+      memcpy(out + k * 6, ring.data() + (rec % kHistoryCap) * 6, 48);
+    }
+  }
+  *count = (int32_t)take;
+  return LBM_OK;
+}
+
+int lbm_read_totals(lbm_solve* s, double* out4) {
+  if (!s || !out4) return fail(LBM_E_INVALID, "null argument");
+  TotalsArgs a{};
+  a.ncells = s->N;
+  a.partials = s->scratch;
+  a.counter = s->counters + 1;
+  a.out = s->totals;
+  const int nblocks = (int)std::min<uint32_t>(1024u, (s->N + 255) / 256);
+  HIPCHK(launch_totals(s->q, s->storage, s->f[s->parity], a, nblocks, s->stream));
+  HIPCHK(hipMemcpyAsync(out4, s->totals, 32, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int lbm_read_classes(lbm_solve* s, uint8_t* out) {
+  if (!s || !out) return fail(LBM_E_INVALID, "null argument");
+  if (!s->cls) {
+    memset(out, 0, s->N);
+    return LBM_OK;
+  }
+  HIPCHK(hipMemcpyAsync(out, s->cls, s->N, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int lbm_solve_info(lbm_solve* s, int64_t* step, int32_t* block_x, int32_t* block_y, int32_t* swizzle,
+                   int64_t* device_bytes) {
+  if (!s) return fail(LBM_E_INVALID, "null solve");
+  if (step) *step = s->step;
+  if (block_x) *block_x = s->bx;
+  if (block_y) *block_y = s->by;
+  if (swizzle) *swizzle = s->swz;
+  if (device_bytes) *device_bytes = (int64_t)s->bytes;
+  return LBM_OK;
+}
+
+int lbm_time_step_kernel(lbm_solve* s, int32_t n, double* ms_per_launch) {
+  if (!s || !ms_per_launch || n < 1) return fail(LBM_E_INVALID, "bad argument");
+  if (!s->initialised) return fail(LBM_E_STATE, "time_step_kernel before upload");
+  if (s->d.autotune && !s->tuned) {
+    int rc = autotune(s);
+    if (rc) return rc;
+  }
+  const void* fin = s->f[s->parity];
+  void* fout = s->f[1 - s->parity];
+  StepArgs a = step_args(s, false);
+  HIPCHK(launch_step(s->q, s->storage, s->general, fin, fout, a, s->bx, s->by, s->stream));
+  HIPCHK(hipEventRecord(s->ev0, s->stream));
+  for (int r = 0; r < n; r++) HIPCHK(launch_step(s->q, s->storage, s->general, fin, fout, a, s->bx, s->by, s->stream));
+  HIPCHK(hipEventRecord(s->ev1, s->stream));
+  HIPCHK(hipEventSynchronize(s->ev1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, s->ev0, s->ev1));
+  *ms_per_launch = (double)t / n;
+  return LBM_OK;
+}
+
+int lbm_set_tile(lbm_solve* s, int32_t block_x, int32_t block_y, int32_t swizzle) {
+  if (!s) return fail(LBM_E_INVALID, "null solve");
+  if (block_x < 1 || block_y < 1 || block_x * block_y > 256 || (block_x * block_y) % 64)
+    return fail(LBM_E_INVALID, "tile %dx%d: need a multiple of 64 threads, at most 256", block_x, block_y);
+  s->bx = block_x;
+  s->by = block_y;
+  s->swz = swizzle ? 1 : 0;
+  s->tuned = true;
+  return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,233 @@
+"""ctypes mirror of include/lbm.h, used by the Python tests and bench.py.
+
+Loads getting-air_amd/lib/liblbm.so (built in-tree by ``make``) and raises
+immediately if it is missing: there is no CPU fallback behind this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblbm.so")
+
+ABI_VERSION = 1
+BGK, TRT = 0, 1
+STORE = {"f32": 0, "f16": 1, "bf16": 2}
+BODY_NONE, BODY_SPHERE, BODY_CIRCLE = 0, 1, 2
+
+# Every symbol include/lbm.h declares (checked by tests/test_abi.py).
+EXPORTS = (
+    "lbm_last_error", "lbm_abi_version", "lbm_device_info", "lbm_solve_create",
+    "lbm_solve_destroy", "lbm_upload_macro", "lbm_upload_f", "lbm_read_f", "lbm_step",
+    "lbm_step_timed", "lbm_sync", "lbm_read_macro", "lbm_read_forces", "lbm_read_totals",
+    "lbm_read_classes", "lbm_solve_info", "lbm_set_tile", "lbm_time_step_kernel",
+)
+
+
+class Body(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int32), ("radius", ctypes.c_float), ("centre", ctypes.c_float * 3)]
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32), ("q", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("collision", ctypes.c_int32), ("omega", ctypes.c_float), ("omega_minus", ctypes.c_float),
+        ("force", ctypes.c_float * 3), ("walls", ctypes.c_int32 * 3), ("wall_u", ctypes.c_float * 2),
+        ("sponge_width", ctypes.c_float), ("sponge_axes", ctypes.c_int32 * 3), ("sponge_u", ctypes.c_float * 3),
+        ("body", Body), ("solid_eq", ctypes.c_int32), ("storage", ctypes.c_int32), ("autotune", ctypes.c_int32),
+    ]
+
+
+class LbmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("lbm error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Returns the loaded liblbm.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("liblbm.so not built at %s: run __graft_entry__.build()" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        lib.lbm_last_error.restype = ctypes.c_char_p
+        lib.lbm_solve_create.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(P)]
+        lib.lbm_solve_destroy.argtypes = [P]
+        lib.lbm_upload_macro.argtypes = [P, P]
+        lib.lbm_upload_f.argtypes = [P, P]
+        lib.lbm_read_f.argtypes = [P, P]
+        lib.lbm_step.argtypes = [P, ctypes.c_int32]
+        lib.lbm_step_timed.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+        lib.lbm_sync.argtypes = [P]
+        lib.lbm_read_macro.argtypes = [P, P]
+        lib.lbm_read_forces.argtypes = [P, P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        lib.lbm_read_totals.argtypes = [P, P]
+        lib.lbm_read_classes.argtypes = [P, P]
+        lib.lbm_solve_info.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int64)]
+        lib.lbm_set_tile.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+        lib.lbm_time_step_kernel.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+        lib.lbm_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_int32)]
+        _lib = lib
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise LbmError(rc, load().lbm_last_error().decode())
+
+
+def make_desc(d):
+    """Builds a Desc from the oracle-style dict used throughout the tests:
+    {q, nx, ny, nz, omega, omegaMinus, force, walls, wallU,
+     sponge: {width, axes, u}, body: {a, centre}, solidEq, storage, autotune}."""
+    q = int(d["q"])
+    desc = Desc()
+    desc.abi_version = ABI_VERSION
+    desc.q = q
+    desc.nx, desc.ny = int(d["nx"]), int(d["ny"])
+    desc.nz = 1 if q == 9 else int(d["nz"])
+    desc.omega = float(d["omega"])
+    if d.get("omegaMinus") is not None:
+        desc.collision = TRT
+        desc.omega_minus = float(d["omegaMinus"])
+    for k, v in enumerate((list(d.get("force", [0, 0, 0])) + [0, 0, 0])[:3]):
+        desc.force[k] = float(v)
+    for k, v in enumerate((list(d.get("walls", [0, 0, 0])) + [0, 0, 0])[:3]):
+        desc.walls[k] = int(bool(v))
+    for k, v in enumerate((list(d.get("wallU", [0, 0])) + [0, 0])[:2]):
+        desc.wall_u[k] = float(v)
+    sp = d.get("sponge") or {}
+    desc.sponge_width = float(sp.get("width", 0.0))
+    axes = sp.get("axes", [1, 1, 0] if q == 9 else [1, 1, 1])
+    for k in range(3):
+        desc.sponge_axes[k] = int(bool(axes[k]))
+        desc.sponge_u[k] = float((list(sp.get("u", [0, 0, 0])) + [0, 0, 0])[k])
+    body = d.get("body")
+    if body:
+        desc.body.shape = BODY_CIRCLE if q == 9 else BODY_SPHERE
+        desc.body.radius = float(body["a"])
+        c = list(body["centre"]) + [0.0]
+        for k in range(3):
+            desc.body.centre[k] = float(c[k])
+    desc.solid_eq = int(d.get("solidEq", 1))
+    desc.storage = STORE[d.get("storage", "f32")]
+    desc.autotune = int(d.get("autotune", 0))
+    return desc
+
+
+def device_info():
+    lib = load()
+    name = ctypes.create_string_buffer(256)
+    hbm = ctypes.c_double()
+    cus = ctypes.c_int32()
+    _check(lib.lbm_device_info(name, 256, ctypes.byref(hbm), ctypes.byref(cus)))
+    return {"name": name.value.decode(), "hbm_bytes": hbm.value, "compute_units": cus.value}
+
+
+class Solve:
+    """One dense lattice on the GPU (owns an lbm_solve handle)."""
+
+    def __init__(self, desc_dict):
+        self.lib = load()
+        self.desc_dict = dict(desc_dict)
+        self.desc = make_desc(desc_dict)
+        self.q = self.desc.q
+        self.shape = (self.desc.nz, self.desc.ny, self.desc.nx)
+        self.ncells = self.desc.nx * self.desc.ny * self.desc.nz
+        h = ctypes.c_void_p()
+        _check(self.lib.lbm_solve_create(ctypes.byref(self.desc), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.lbm_solve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload_macro(self, mac4):
+        m = np.ascontiguousarray(mac4, dtype=np.float32).reshape(-1)
+        if m.size != 4 * self.ncells:
+            raise ValueError("mac needs %d floats, got %d" % (4 * self.ncells, m.size))
+        _check(self.lib.lbm_upload_macro(self.h, m.ctypes.data))
+
+    def upload_f(self, f):
+        a = np.ascontiguousarray(f, dtype=np.float32).reshape(-1)
+        if a.size != self.q * self.ncells:
+            raise ValueError("f needs %d floats" % (self.q * self.ncells))
+        _check(self.lib.lbm_upload_f(self.h, a.ctypes.data))
+
+    def read_f(self):
+        out = np.empty(self.q * self.ncells, dtype=np.float32)
+        _check(self.lib.lbm_read_f(self.h, out.ctypes.data))
+        return out
+
+    def step(self, n):
+        _check(self.lib.lbm_step(self.h, int(n)))
+
+    def step_timed(self, n):
+        ms = ctypes.c_double()
+        _check(self.lib.lbm_step_timed(self.h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        _check(self.lib.lbm_sync(self.h))
+
+    def read_macro(self):
+        out = np.empty(4 * self.ncells, dtype=np.float32)
+        _check(self.lib.lbm_read_macro(self.h, out.ctypes.data))
+        return out
+
+    def read_forces(self, max_records=1 << 16):
+        out = np.empty(6 * max_records, dtype=np.float64)
+        cnt = ctypes.c_int32()
+        _check(self.lib.lbm_read_forces(self.h, out.ctypes.data, int(max_records), ctypes.byref(cnt)))
+        return out[: 6 * cnt.value].reshape(-1, 6)
+
+    def read_totals(self):
+        out = np.empty(4, dtype=np.float64)
+        _check(self.lib.lbm_read_totals(self.h, out.ctypes.data))
+        return out
+
+    def read_classes(self):
+        out = np.empty(self.ncells, dtype=np.uint8)
+        _check(self.lib.lbm_read_classes(self.h, out.ctypes.data))
+        return out
+
+    def info(self):
+        step = ctypes.c_int64()
+        bx, by, swz = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        nbytes = ctypes.c_int64()
+        _check(self.lib.lbm_solve_info(self.h, ctypes.byref(step), ctypes.byref(bx), ctypes.byref(by),
+                                       ctypes.byref(swz), ctypes.byref(nbytes)))
+        return {"step": step.value, "block_x": bx.value, "block_y": by.value, "swizzle": swz.value,
+                "device_bytes": nbytes.value}
+
+    def set_tile(self, bx, by, swizzle):
+        _check(self.lib.lbm_set_tile(self.h, int(bx), int(by), int(swizzle)))
+
+    def time_step_kernel(self, n):
+        """Mean device milliseconds of one fused stream-collide launch."""
+        ms = ctypes.c_double()
+        _check(self.lib.lbm_time_step_kernel(self.h, int(n), ctypes.byref(ms)))
+        return ms.value

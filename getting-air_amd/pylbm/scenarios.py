@@ -1,0 +1,161 @@
+"""Workload definitions: solve descriptors plus initial macroscopic fields.
+
+Each function returns (desc, mac) where desc is the descriptor dict accepted
+by pylbm.Solve (and by the oracle) and mac is float32 [rho, ux, uy, uz] per
+cell, cell = (z * ny + y) * nx + x.
+"""
+import math
+
+import numpy as np
+
+CS2 = 1.0 / 3.0
+
+
+def f32(x):
+    return float(np.float32(x))
+
+
+def trt_omega_minus(omega, lam=3.0 / 16.0):
+    """Odd TRT rate for a magic parameter Lambda (plans/TRT.md:129-133):
+    tau_minus = 1/2 + Lambda / (tau_plus - 1/2)."""
+    tau = 1.0 / f32(omega)
+    return 1.0 / (0.5 + lam / (tau - 0.5))
+
+
+def tgv2d(n=256, tau=0.6, u0=0.05):
+    """D2Q9 Taylor-Green vortex, periodic n x n:
+    u = u0 (sin kx cos ky, -cos kx sin ky), p = (u0^2/4)(cos 2kx + cos 2ky),
+    rho = 1 + p / cs^2 (the field of main-tgv.js:69-90 with this sign
+    convention)."""
+    k = 2.0 * math.pi / n
+    x = np.arange(n, dtype=np.float64)[None, :]
+    y = np.arange(n, dtype=np.float64)[:, None]
+    ux = u0 * np.sin(k * x) * np.cos(k * y)
+    uy = -u0 * np.cos(k * x) * np.sin(k * y)
+    p = (u0 * u0 / 4.0) * (np.cos(2 * k * x) + np.cos(2 * k * y))
+    mac = np.zeros((n, n, 4), dtype=np.float32)
+    mac[..., 0] = 1.0 + p / CS2
+    mac[..., 1] = ux
+    mac[..., 2] = uy
+    desc = {"q": 9, "nx": n, "ny": n, "nz": 1, "omega": f32(1.0 / tau)}
+    return desc, mac.reshape(-1)
+
+
+def tgv2d_analytic(n, tau, u0, t):
+    """Analytic TGV velocity at step t (decay exp(-2 nu k^2 t))."""
+    nu = (tau - 0.5) / 3.0
+    k = 2.0 * math.pi / n
+    d = math.exp(-2.0 * nu * k * k * t)
+    x = np.arange(n, dtype=np.float64)[None, :]
+    y = np.arange(n, dtype=np.float64)[:, None]
+    return (u0 * np.sin(k * x) * np.cos(k * y) * d).reshape(-1), (-u0 * np.cos(k * x) * np.sin(k * y) * d).reshape(-1)
+
+
+def sphere(nx=256, ny=128, nz=128, radius=12.0, centre=(64.0, 64.0, 64.0), u0=0.05, re=200.0,
+           trt=True, sponge_width=None, storage="f32"):
+    """D3Q19 flow past a pinned sphere with halfway bounce-back and an ALBC
+    sponge on every face relaxing toward the freestream (SCENARIOS.sphere,
+    d3-scenarios.mjs:350-391): nu = u0 D / Re, tau = 3 nu + 1/2, sponge width
+    max(6, round(D/2)), freestream seeded everywhere with
+    rho = 1 - |u|^2 / (2 cs^2)."""
+    d_ = 2.0 * radius
+    nu = u0 * d_ / re
+    tau = 3.0 * nu + 0.5
+    omega = f32(1.0 / tau)
+    if sponge_width is None:
+        sponge_width = max(6, int(round(d_ / 2.0)))
+    desc = {
+        "q": 19, "nx": nx, "ny": ny, "nz": nz, "omega": omega,
+        "sponge": {"width": float(sponge_width), "axes": [1, 1, 1], "u": [u0, 0.0, 0.0]},
+        "body": {"a": float(radius), "centre": [float(c) for c in centre]},
+        "solidEq": 1, "storage": storage,
+    }
+    if trt:
+        desc["omegaMinus"] = trt_omega_minus(omega)
+    n = nx * ny * nz
+    mac = np.empty((n, 4), dtype=np.float32)
+    mac[:, 0] = 1.0 - (u0 * u0) / (2.0 * CS2)
+    mac[:, 1] = u0
+    mac[:, 2] = 0.0
+    mac[:, 3] = 0.0
+    meta = {"area": math.pi * radius * radius, "u0": u0, "nu": nu, "tau": tau, "re": re}
+    return desc, mac.reshape(-1), meta
+
+
+def beltrami(n=32, q=19, tau=0.7, amp=(0.02, 0.015, 0.01)):
+    """Periodic ABC (Beltrami) flow, curl u = k u, exact decay
+    u(t) = u0 exp(-nu k^2 t) (d3-scenarios.mjs:19-27)."""
+    k = 2.0 * math.pi / n
+    a, b, c = amp
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ux = a * np.sin(k * z) + c * np.cos(k * y)
+    uy = b * np.sin(k * x) + a * np.cos(k * z)
+    uz = c * np.sin(k * y) + b * np.cos(k * x)
+    mac = np.stack([1.0 - 1.5 * (ux * ux + uy * uy + uz * uz), ux, uy, uz], axis=-1).astype(np.float32)
+    desc = {"q": q, "nx": n, "ny": n, "nz": n, "omega": f32(1.0 / tau)}
+    return desc, mac.reshape(-1)
+
+
+def random_field(nx, ny, nz, q=19, seed=1, amp=0.03, tau=0.62, modes=4):
+    """Smooth seeded periodic field: each of rho, ux, uy, uz is a sum of a few
+    random low-wavenumber Fourier modes (so it does not decay to round-off
+    within a short parity run)."""
+    rng = np.random.default_rng(seed)
+    if q == 9:
+        nz = 1
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    comps = []
+    for c in range(4):
+        v = np.zeros(x.shape)
+        for _ in range(modes):
+            kx, ky = rng.integers(0, 3, 2)
+            kz = 0 if q == 9 else rng.integers(0, 3)
+            ph = rng.random() * 2.0 * math.pi
+            v += np.cos(2 * math.pi * (kx * x / nx + ky * y / ny + kz * z / nz) + ph)
+        comps.append(v / modes)
+    mac = np.empty(x.shape + (4,), dtype=np.float32)
+    mac[..., 0] = 1.0 + 0.01 * comps[0]
+    for j in range(3):
+        mac[..., 1 + j] = amp * comps[1 + j]
+    if q == 9:
+        mac[..., 3] = 0.0
+    desc = {"q": q, "nx": nx, "ny": ny, "nz": nz, "omega": f32(1.0 / tau)}
+    return desc, mac.reshape(-1)
+
+
+def turbulence(n=512, seed=2024, urms=0.04, kpeak=8.0, tau=0.55, q=19, storage="f32"):
+    """Decaying turbulence in a periodic box: random-phase solenoidal field
+    with E(k) ~ k^4 exp(-2 (k/kpeak)^2), scaled to u_rms, rho from the
+    incompressible pressure estimate rho = 1 - |u|^2/(2 cs^2).  The reference
+    has no such initialiser; this one is seeded and deterministic."""
+    rng = np.random.default_rng(seed)
+    kx = np.fft.fftfreq(n, d=1.0 / n)
+    kr = np.fft.rfftfreq(n, d=1.0 / n)
+    KZ, KY, KX = np.meshgrid(kx, kx, kr, indexing="ij")
+    k2 = KX * KX + KY * KY + KZ * KZ
+    kk = np.sqrt(k2)
+    k2[0, 0, 0] = 1.0
+    # amplitude per mode so that the shell energy follows E(k): |u_k|^2 ~ E(k) / k^2
+    amp = np.sqrt(np.where(kk > 0, kk ** 4 * np.exp(-2.0 * (kk / kpeak) ** 2) / (4.0 * math.pi * k2), 0.0))
+    comps = []
+    for _ in range(3):
+        ph = rng.random(KX.shape) * 2.0 * math.pi
+        comps.append(amp * np.exp(1j * ph))
+    ux_k, uy_k, uz_k = comps
+    # project onto divergence-free modes
+    dot = (KX * ux_k + KY * uy_k + KZ * uz_k) / k2
+    ux_k = ux_k - KX * dot
+    uy_k = uy_k - KY * dot
+    uz_k = uz_k - KZ * dot
+    u = [np.fft.irfftn(c, s=(n, n, n)) for c in (ux_k, uy_k, uz_k)]
+    rms = math.sqrt(sum(float(np.mean(c * c)) for c in u) / 3.0)
+    scale = urms / rms
+    mac = np.empty((n, n, n, 4), dtype=np.float32)
+    usq = np.zeros((n, n, n))
+    for j, c in enumerate(u):
+        c *= scale
+        usq += c * c
+        mac[..., 1 + j] = c
+    mac[..., 0] = 1.0 - usq / (2.0 * CS2)
+    desc = {"q": q, "nx": n, "ny": n, "nz": n, "omega": f32(1.0 / tau), "storage": storage}
+    return desc, mac.reshape(-1)

@@ -1,0 +1,149 @@
+/*
+ * lbm.h -- C ABI of the MI355X lattice-Boltzmann engine (liblbm.so).
+ *
+ * The reference drives its solver through WebGPU from JavaScript.  Each entry
+ * point below replaces one group of those WebGPU calls; the reference call
+ * site it stands in for is cited beside it.  Plain C types only: opaque
+ * handles, POD descriptors, caller-owned host arrays copied in and out.
+ * Every function returns 0 on success or a negative LBM_E_* code, and
+ * lbm_last_error() then returns a message for the calling thread.
+ *
+ * Threading: one host thread drives a given solve handle at a time (the
+ * reference has one WebGPU queue per page).  Different handles may be driven
+ * from different threads.
+ */
+#ifndef LBM_H
+#define LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+enum {
+  LBM_OK = 0,
+  LBM_E_INVALID = -1,   /* bad argument or descriptor */
+  LBM_E_DEVICE = -2,    /* HIP runtime error */
+  LBM_E_NOMEM = -3,     /* device allocation failed / budget exceeded */
+  LBM_E_REFUSED = -4,   /* configuration the reference refuses */
+  LBM_E_STATE = -5      /* call out of order (e.g. step before init) */
+};
+
+enum { LBM_BGK = 0, LBM_TRT = 1 };
+
+/* Population storage in HBM.  Collision is always computed in FP32.
+ * F16/BF16 store the shifted value f_i - w_i (the reference's f-pack mode 2,
+ * common_fpack.wgsl:91-121, extended from D2Q9 to D3Q19/D3Q27). */
+enum { LBM_STORE_F32 = 0, LBM_STORE_F16 = 1, LBM_STORE_BF16 = 2 };
+
+enum { LBM_BODY_NONE = 0, LBM_BODY_SPHERE = 1, LBM_BODY_CIRCLE = 2 };
+
+/* Pinned rigid body (d3-body.mjs BODY_FIELDS:318 / CardState
+ * common_geometry.wgsl:8-35, restricted to what a pinned body reads). */
+typedef struct lbm_body {
+  int32_t shape;        /* LBM_BODY_* ; SPHERE for Q19/Q27, CIRCLE for Q9   */
+  float radius;         /* a                                                  */
+  float centre[3];      /* cx, cy, cz in lattice cells                        */
+} lbm_body;
+
+/* Solve descriptor: the reference's pipeline override constants
+ * (common_d3_step.wgsl:94-179, lbm_step.wgsl:33-126) plus the grid. */
+typedef struct lbm_desc {
+  int32_t abi_version;  /* LBM_ABI_VERSION                                    */
+  int32_t q;            /* 9 (D2Q9), 19 (D3Q19) or 27 (D3Q27)                  */
+  int32_t nx, ny, nz;   /* nz must be 1 for Q = 9                             */
+  int32_t collision;    /* LBM_BGK or LBM_TRT                                 */
+  float omega;          /* 1/tau (even rate; OMEGA)                           */
+  float omega_minus;    /* TRT odd rate (OMEGA_M, plans/TRT.md:352-367)       */
+  float force[3];       /* uniform Guo body force (FORCE_X/Y/Z)               */
+  int32_t walls[3];     /* halfway no-slip walls per axis (WALL_X/Y/Z)        */
+  float wall_u[2];      /* x-velocity of the y=0 / y=ny-1 walls (WALL_U0/U1)  */
+  float sponge_width;   /* SPONGE_W; <= 0 disables                            */
+  int32_t sponge_axes[3]; /* SPONGE_AX/AY/AZ                                  */
+  float sponge_u[3];    /* SPONGE_UX/UY/UZ                                    */
+  lbm_body body;        /* pinned body with halfway bounce-back               */
+  int32_t solid_eq;     /* SOLID_EQ: hold the body interior at feq(1, 0)      */
+  int32_t storage;      /* LBM_STORE_*                                        */
+  int32_t autotune;     /* 1: time candidate tile shapes on first step        */
+} lbm_desc;
+
+typedef struct lbm_solve lbm_solve;
+
+/* Library / device. */
+const char* lbm_last_error(void);
+int lbm_abi_version(void);
+int lbm_device_info(char* name, size_t name_len, double* hbm_bytes, int32_t* cu_count);
+
+/* Creates a dense solve: device buffers for two population lattices and the
+ * macroscopic field, plus the solid classification of the body.
+ * Replaces main-3d.js:752-807 (createBuffer) and :989-1048
+ * (createComputePipelineAsync with stepConstants), main-tgv.js:218-235. */
+int lbm_solve_create(const lbm_desc* desc, lbm_solve** out);
+int lbm_solve_destroy(lbm_solve* s);
+
+/* Uploads [rho, ux, uy, uz] per cell (float32, 4*ncells) and builds the
+ * equilibrium populations: writeBuffer(mac) + initEq,
+ * main-3d.js:2973-2994 / common_d3_step.wgsl:206-222. */
+int lbm_upload_macro(lbm_solve* s, const float* mac4);
+
+/* Populations in the f32 plane-major wire format f[i*ncells + cell]
+ * (f-pack.mjs:6-14); packed to the storage format on the way in and
+ * unpacked on the way out. Replaces writeF / snapshot readback. */
+int lbm_upload_f(lbm_solve* s, const float* f);
+int lbm_read_f(lbm_solve* s, float* f);
+
+/* Advances n steps (force reduction then fused stream-collide per step) and
+ * leaves the macroscopic field of the last step readable.  Replaces
+ * encodeSteps / debugStepSync, main-3d.js:3122-3171 and :3506,
+ * main-tgv.js:245-265.  Asynchronous with respect to the host. */
+int lbm_step(lbm_solve* s, int32_t n);
+
+/* lbm_step(n) bracketed by HIP events on the solve's own stream; returns the
+ * device time of the n steps in milliseconds. */
+int lbm_step_timed(lbm_solve* s, int32_t n, double* ms);
+
+/* Blocks until all queued work of the solve is complete. */
+int lbm_sync(lbm_solve* s);
+
+/* [rho, ux, uy, uz] per cell, float32: readMacro, main-3d.js:3539. */
+int lbm_read_macro(lbm_solve* s, float* mac4);
+
+/* Momentum-exchange force and torque on the pinned body, FP64, one record
+ * [fx, fy, fz, tx, ty, tz] per step since the last lbm_upload_* (oldest
+ * first).  *count receives the number of records written (<= max_records).
+ * Replaces the force atomics + drain, common_d3_force.wgsl:171-283 and
+ * d3_physics.wgsl:56-75, read by readBody, main-3d.js:4721-4738. */
+int lbm_read_forces(lbm_solve* s, double* out, int32_t max_records, int32_t* count);
+
+/* FP64 totals over the current populations: [mass, px, py, pz]
+ * (the extensive budget of readStats, main-3d.js:3610-3650). */
+int lbm_read_totals(lbm_solve* s, double* out4);
+
+/* Solid classification, one byte per cell: bit 0 = inside the body
+ * (phi < 0), bit 1 = phi < 2 (a cell that can have a solid neighbour). */
+int lbm_read_classes(lbm_solve* s, uint8_t* out);
+
+/* Current step count, tile shape and device bytes held by the solve. */
+int lbm_solve_info(lbm_solve* s, int64_t* step, int32_t* block_x, int32_t* block_y,
+                   int32_t* swizzle, int64_t* device_bytes);
+
+/* Overrides the autotuned tile shape (block_x * block_y threads; swizzle 1
+ * maps consecutive tiles onto one XCD). */
+int lbm_set_tile(lbm_solve* s, int32_t block_x, int32_t block_y, int32_t swizzle);
+
+/* Measurement hook, no counterpart in the reference (its spike page times
+ * passes with timestamp queries, main-3d-spike.js:229-286): launches the
+ * fused stream-collide kernel n times from the current lattice into the other
+ * one without swapping (the state does not advance) and returns the mean
+ * device time of one launch in milliseconds, bracketed by HIP events on the
+ * solve's stream. */
+int lbm_time_step_kernel(lbm_solve* s, int32_t n, double* ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

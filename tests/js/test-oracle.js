@@ -1,0 +1,209 @@
+'use strict';
+// Pins the FP64 oracle (oracle/lbm_oracle.js) against closed forms and exact
+// invariants, chosen so that a dropped term, a wrong sign or a transposed
+// table fails at least one of them:
+//   - 2D Taylor-Green decay (analytic field and energy decay, main-tgv.js:65-90,
+//     tolerances of benchmarks/tgv.json: field L2rel <= 1e-2, rate <= 1%)
+//   - 3D Beltrami (ABC) exact decay u(t) = u0 exp(-nu k^2 t) (d3-scenarios.mjs:19-27)
+//   - exact mass conservation in a periodic box (sum w = 1 exactly)
+//   - exact momentum input N*F per step under Guo forcing, BGK and TRT
+//     (plans/TRT.md sec 8.0 item 6)
+//   - TRT at Lambda = 3/16 reproduces plane Poiseuille exactly with halfway
+//     bounce-back walls; BGK reproduces plane Couette exactly
+//   - steady momentum balance: the MEM force on a pinned sphere in a periodic,
+//     uniformly forced box equals the total force applied to the fluid
+//   - the f32-emulated classification of a sphere at an integer centre equals
+//     the exact integer test |p - c|^2 < a^2
+// Run: node tests/js/test-oracle.js   (exit code 1 on any failure)
+
+const O = require('../../oracle/lbm_oracle.js');
+
+let failures = 0;
+function ok(name, fn) {
+  try {
+    const msg = fn();
+    console.log('ok   ' + name + (msg ? '  (' + msg + ')' : ''));
+  } catch (e) {
+    failures++;
+    console.log('FAIL ' + name + ': ' + (e && e.message ? e.message : e));
+  }
+}
+function assert(c, m) { if (!c) throw new Error(m); }
+
+function uniformMac(N, rho, u) {
+  const m = new Float64Array(4 * N);
+  for (let c = 0; c < N; c++) { m[4 * c] = rho; m[4 * c + 1] = u[0]; m[4 * c + 2] = u[1]; m[4 * c + 3] = u[2]; }
+  return m;
+}
+
+ok('2D TGV: field within 1e-2 and energy decay rate within 1% of analytic', () => {
+  const N = 48, tau = 0.8, u0 = 0.02, T = 600;
+  const nu = (tau - 0.5) / 3, k = 2 * Math.PI / N;
+  const o = new O.DenseOracle({ q: 9, nx: N, ny: N, nz: 1, omega: 1 / tau });
+  const mac = new Float64Array(4 * N * N);
+  for (let y = 0; y < N; y++) for (let x = 0; x < N; x++) {
+    const c = y * N + x;
+    // u = u0 (sin kx cos ky, -cos kx sin ky); p = (u0^2/4)(cos 2kx + cos 2ky)
+    mac[4 * c + 1] = u0 * Math.sin(k * x) * Math.cos(k * y);
+    mac[4 * c + 2] = -u0 * Math.cos(k * x) * Math.sin(k * y);
+    mac[4 * c] = 1 + 3 * (u0 * u0 / 4) * (Math.cos(2 * k * x) + Math.cos(2 * k * y));
+  }
+  o.initEq(mac);
+  const ke = () => { let s = 0; for (let c = 0; c < N * N; c++) s += o.mac[4 * c + 1] ** 2 + o.mac[4 * c + 2] ** 2; return s; };
+  o.step(100);
+  const ke1 = ke();
+  o.step(T - 100);
+  const ke2 = ke();
+  const decay = Math.exp(-nu * 2 * k * k * T);
+  let num = 0, den = 0;
+  for (let y = 0; y < N; y++) for (let x = 0; x < N; x++) {
+    const c = y * N + x;
+    const ax = u0 * Math.sin(k * x) * Math.cos(k * y) * decay, ay = -u0 * Math.cos(k * x) * Math.sin(k * y) * decay;
+    num += (o.mac[4 * c + 1] - ax) ** 2 + (o.mac[4 * c + 2] - ay) ** 2;
+    den += ax * ax + ay * ay;
+  }
+  const l2 = Math.sqrt(num / den);
+  const rate = Math.log(ke1 / ke2) / (T - 100);
+  const rateExact = 4 * nu * k * k;
+  assert(l2 < 1e-2, 'L2rel ' + l2);
+  assert(Math.abs(rate / rateExact - 1) < 0.01, 'rate error ' + (rate / rateExact - 1));
+  return 'L2rel ' + l2.toExponential(2) + ', rate err ' + (rate / rateExact - 1).toExponential(2);
+});
+
+for (const Q of [19, 27]) {
+  ok('D3Q' + Q + ' Beltrami: energy decays at 2 nu k^2 within 2%', () => {
+    const N = 24, tau = 0.7, T = 120;
+    const nu = (tau - 0.5) / 3, k = 2 * Math.PI / N, A = 0.02, B = 0.015, C = 0.01;
+    const o = new O.DenseOracle({ q: Q, nx: N, ny: N, nz: N, omega: 1 / tau });
+    const n3 = N * N * N;
+    const mac = new Float64Array(4 * n3);
+    for (let z = 0; z < N; z++) for (let y = 0; y < N; y++) for (let x = 0; x < N; x++) {
+      const c = (z * N + y) * N + x;
+      const ux = A * Math.sin(k * z) + C * Math.cos(k * y);
+      const uy = B * Math.sin(k * x) + A * Math.cos(k * z);
+      const uz = C * Math.sin(k * y) + B * Math.cos(k * x);
+      mac[4 * c] = 1 - (ux * ux + uy * uy + uz * uz) * 1.5;
+      mac[4 * c + 1] = ux; mac[4 * c + 2] = uy; mac[4 * c + 3] = uz;
+    }
+    o.initEq(mac);
+    const ke = () => { let s = 0; for (let c = 0; c < n3; c++) s += o.mac[4 * c + 1] ** 2 + o.mac[4 * c + 2] ** 2 + o.mac[4 * c + 3] ** 2; return s; };
+    o.step(20);
+    const k1 = ke();
+    o.step(T - 20);
+    const k2 = ke();
+    const rate = Math.log(k1 / k2) / (T - 20);
+    const exact = 2 * nu * k * k;
+    assert(Math.abs(rate / exact - 1) < 0.02, 'rate error ' + (rate / exact - 1));
+    return 'rate err ' + (rate / exact - 1).toExponential(2);
+  });
+}
+
+for (const Q of [9, 19, 27]) {
+  ok('Q' + Q + ' periodic box conserves mass to 1e-13 and gains exactly N*F momentum per step (BGK, TRT)', () => {
+    for (const trt of [false, true]) {
+      const nx = 10, ny = 8, nz = Q === 9 ? 1 : 6, N = nx * ny * nz;
+      const F = [2e-5, -1e-5, Q === 9 ? 0 : 3e-5];
+      const omega = 1 / 0.62;
+      const desc = { q: Q, nx: nx, ny: ny, nz: nz, omega: omega, force: F, exactWeights: true };
+      if (trt) desc.omegaMinus = 1 / (0.5 + 0.1875 / (1 / Math.fround(omega) - 0.5));
+      const o = new O.DenseOracle(desc);
+      const mac = new Float64Array(4 * N);
+      let seed = 7;
+      const rnd = () => { seed = (seed * 1103515245 + 12345) % 2147483648; return seed / 2147483648 - 0.5; };
+      for (let c = 0; c < N; c++) { mac[4 * c] = 1 + 0.01 * rnd(); mac[4 * c + 1] = 0.03 * rnd(); mac[4 * c + 2] = 0.03 * rnd(); mac[4 * c + 3] = Q === 9 ? 0 : 0.03 * rnd(); }
+      o.initEq(mac);
+      const t0 = o.totals();
+      const steps = 40;
+      o.step(steps);
+      const t1 = o.totals();
+      assert(Math.abs(t1[0] - t0[0]) / t0[0] < 1e-13, 'mass drift ' + (t1[0] - t0[0]));
+      for (let a = 0; a < 3; a++) {
+        const expect = N * Math.fround(F[a]) * steps;
+        assert(Math.abs((t1[1 + a] - t0[1 + a]) - expect) < 1e-12, 'momentum axis ' + a + ' got ' + (t1[1 + a] - t0[1 + a]) + ' want ' + expect + (trt ? ' (TRT)' : ''));
+      }
+    }
+  });
+}
+
+function poiseuilleCase(Q, trt) {
+  const H = 8, F = 1e-6, tau = 0.9;
+  const nx = 2, nz = Q === 9 ? 1 : 2;
+  const omega = Math.fround(1 / tau);
+  const desc = { q: Q, nx: nx, ny: H, nz: nz, omega: omega, force: [F, 0, 0], walls: [0, 1, 0], exactWeights: true };
+  if (trt) desc.omegaMinus = 1 / (0.5 + 0.1875 / (1 / omega - 0.5));
+  const o = new O.DenseOracle(desc);
+  o.initEq(uniformMac(nx * H * nz, 1, [0, 0, 0]));
+  o.step(6000);
+  const nu = (1 / omega - 0.5) / 3;
+  let worst = 0;
+  for (let y = 0; y < H; y++) {
+    const exact = Math.fround(F) / (2 * nu) * (y + 0.5) * (H - 0.5 - y);
+    const got = o.mac[4 * (y * nx) + 1];
+    worst = Math.max(worst, Math.abs(got / exact - 1));
+  }
+  return worst;
+}
+
+ok('TRT Lambda=3/16 plane Poiseuille with halfway walls is exact (D2Q9, D3Q19)', () => {
+  const e9 = poiseuilleCase(9, true), e19 = poiseuilleCase(19, true);
+  assert(e9 < 1e-6 && e19 < 1e-6, 'max rel err Q9 ' + e9 + ' Q19 ' + e19);
+  const bgk = poiseuilleCase(9, false);
+  // BGK's halfway wall slips by a tau-dependent amount: the check must see it.
+  assert(bgk > 1e-3, 'BGK unexpectedly exact: ' + bgk);
+  return 'TRT ' + e9.toExponential(1) + ' / ' + e19.toExponential(1) + ', BGK ' + bgk.toExponential(1);
+});
+
+ok('BGK plane Couette with a moving y wall is exact (D2Q9)', () => {
+  const H = 10, U = 0.01, nx = 2;
+  const o = new O.DenseOracle({ q: 9, nx: nx, ny: H, nz: 1, omega: 1 / 0.8, walls: [0, 1, 0], wallU: [0, U], exactWeights: true });
+  o.initEq(uniformMac(nx * H, 1, [0, 0, 0]));
+  o.step(8000);
+  let worst = 0;
+  for (let y = 0; y < H; y++) {
+    const exact = Math.fround(U) * (y + 0.5) / H;
+    worst = Math.max(worst, Math.abs(o.mac[4 * (y * nx) + 1] - exact) / U);
+  }
+  assert(worst < 1e-7, 'max err / U ' + worst);
+  return worst.toExponential(1);
+});
+
+ok('sphere classification equals the exact integer test at an integer centre', () => {
+  const d = O.normalise({ q: 19, nx: 40, ny: 36, nz: 32, omega: 1, body: { a: 12, centre: [20, 18, 16] } });
+  const cls = O.classify(d);
+  for (let z = 0; z < 32; z++) for (let y = 0; y < 36; y++) for (let x = 0; x < 40; x++) {
+    const r2 = (x - 20) ** 2 + (y - 18) ** 2 + (z - 16) ** 2;
+    const c = cls[(z * 36 + y) * 40 + x];
+    assert(((c & 1) !== 0) === (r2 < 144), 'solid bit at ' + [x, y, z]);
+    assert(((c & 2) !== 0) === (r2 < 196), 'near bit at ' + [x, y, z]);
+  }
+});
+
+ok('MEM force on a pinned sphere balances the applied body force at steady state (D3Q19, TRT)', () => {
+  const n = 12, F = 2e-6, omega = Math.fround(1 / 0.8);
+  const o = new O.DenseOracle({
+    q: 19, nx: n, ny: n, nz: n, omega: omega, omegaMinus: 1 / (0.5 + 0.1875 / (1 / omega - 0.5)),
+    force: [F, 0, 0], body: { a: 2.6, centre: [6, 6, 6] },
+  });
+  o.initEq(uniformMac(n * n * n, 1, [0, 0, 0]));
+  o.step(4000);
+  let fluid = 0;
+  for (let c = 0; c < o.N; c++) if (!(o.cls[c] & 1)) fluid++;
+  const want = Math.fround(F) * fluid;
+  const got = o.forceHistory[o.forceHistory.length - 1];
+  const err = Math.abs(got[0] / want - 1);
+  assert(err < 1e-4, 'Fx ' + got[0] + ' want ' + want);
+  assert(Math.abs(got[1]) < 1e-9 * want && Math.abs(got[2]) < 1e-9 * want, 'lateral ' + got[1] + ', ' + got[2]);
+  return 'rel err ' + err.toExponential(2);
+});
+
+ok('uniform flow at the sponge velocity stays uniform under the sponge (D3Q19)', () => {
+  const n = 10, u = [Math.fround(0.05), 0, 0];
+  const o = new O.DenseOracle({ q: 19, nx: n, ny: n, nz: n, omega: 1 / 0.6, sponge: { width: 3, u: u }, exactWeights: true });
+  o.initEq(uniformMac(n * n * n, 1, u));
+  o.step(20);
+  let worst = 0;
+  for (let c = 0; c < o.N; c++) worst = Math.max(worst, Math.abs(o.mac[4 * c + 1] - Math.fround(0.05)), Math.abs(o.mac[4 * c] - 1));
+  assert(worst < 1e-14, 'deviation ' + worst);
+});
+
+process.exitCode = failures ? 1 : 0;

@@ -1,0 +1,218 @@
+"""GPU parity of the dense step (liblbm.so through the C ABI) against the FP64
+oracle (oracle/run.js) on the same seeded inputs.
+
+Tolerances (fp32 storage, FP32 collision):
+  rho, u   norm-wise relative error <= 1e-5 after the run
+  force    |F_gpu - F_oracle| <= 1e-6 * |F_oracle| per step (drag/lift)
+  classes  bit-exact (integer output)
+16-bit storage is reported against the oracle with the wider tolerances
+stated in each test.
+"""
+import numpy as np
+import pytest
+
+import oracle_runner
+import pylbm
+from pylbm import scenarios as sc
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def field_errors(mac_gpu, mac_ref):
+    g = mac_gpu.reshape(-1, 4).astype(np.float64)
+    r = mac_ref.reshape(-1, 4)
+    return {"rho": relerr(g[:, 0], r[:, 0]), "u": relerr(g[:, 1:], r[:, 1:]),
+            "drho": relerr(g[:, 0] - 1.0, r[:, 0] - 1.0)}
+
+
+def force_dev(fg, fr):
+    """Per-component max |F_gpu - F_oracle| over the run, in units of the
+    largest drag the oracle saw (drag and lift share that scale)."""
+    scale = np.abs(fr[:, 0]).max()
+    return (np.abs(fg[:, :3] - fr[:, :3]) / scale).max(axis=0)
+
+
+def run_gpu(desc, mac, steps):
+    with pylbm.Solve(desc) as s:
+        s.upload_macro(mac)
+        s.step(steps)
+        s.sync()
+        return {"mac": s.read_macro(), "force": s.read_forces(), "totals": s.read_totals(),
+                "cls": s.read_classes(), "info": s.info()}
+
+
+@pytest.mark.parametrize("q", [9, 19, 27])
+@pytest.mark.parametrize("trt", [False, True])
+def test_periodic_forced_random_field(q, trt):
+    desc, mac = sc.random_field(24, 20, 12, q=q, seed=3 + q)
+    desc["force"] = [1e-5, -2e-5, 0.0 if q == 9 else 1.5e-5]
+    if trt:
+        desc["omegaMinus"] = sc.trt_omega_minus(desc["omega"])
+    print("case q=%d trt=%s" % (q, trt), flush=True)
+    g = run_gpu(desc, mac, 60)
+    r = oracle_runner.run(desc, mac, steps=60, want=("mac",))
+    e = field_errors(g["mac"], r["mac"])
+    print(e)
+    assert e["rho"] < 1e-5 and e["u"] < 1e-5, e
+    # FP64 totals of the populations agree with the oracle's mass to fp32 level
+    tot = r["summary"]["totals"]
+    assert abs(g["totals"][0] - tot[0]) / tot[0] < 1e-6
+
+
+def test_tgv_d2q9_256_1000_steps():
+    """Config 1 at full size: D2Q9 BGK TGV 256^2, tau 0.6, u0 0.05, 1000 steps."""
+    desc, mac = sc.tgv2d(256, 0.6, 0.05)
+    g = run_gpu(desc, mac, 1000)
+    r = oracle_runner.run(desc, mac, steps=1000, threads=8, want=("mac",))
+    e = field_errors(g["mac"], r["mac"])
+    print(e)
+    assert e["rho"] < 1e-5 and e["u"] < 1e-5, e
+    # kinetic-energy decay against the oracle's
+    ke_g = float(np.sum(g["mac"].reshape(-1, 4)[:, 1:3].astype(np.float64) ** 2))
+    ke_r = float(np.sum(r["mac"][:, 1:3] ** 2))
+    assert abs(ke_g / ke_r - 1.0) < 1e-5
+
+
+@pytest.mark.parametrize("q", [9, 19])
+def test_walls_trt_poiseuille_matches_oracle(q):
+    desc = {"q": q, "nx": 8, "ny": 16, "nz": 1 if q == 9 else 4, "omega": sc.f32(1 / 0.9),
+            "force": [2e-6, 0, 0], "walls": [0, 1, 0]}
+    desc["omegaMinus"] = sc.trt_omega_minus(desc["omega"])
+    n = desc["nx"] * desc["ny"] * desc["nz"]
+    mac = np.tile(np.array([1, 0, 0, 0], np.float32), n)
+    g = run_gpu(desc, mac, 400)
+    r = oracle_runner.run(desc, mac, steps=400, want=("mac",))
+    e = field_errors(g["mac"], r["mac"])
+    assert e["u"] < 1e-5 and e["rho"] < 1e-5, e
+
+
+def test_moving_wall_couette_d2q9():
+    desc = {"q": 9, "nx": 6, "ny": 12, "nz": 1, "omega": sc.f32(1 / 0.8), "walls": [0, 1, 0],
+            "wallU": [0.0, 0.02]}
+    mac = np.tile(np.array([1, 0, 0, 0], np.float32), 72)
+    g = run_gpu(desc, mac, 300)
+    r = oracle_runner.run(desc, mac, steps=300, want=("mac",))
+    e = field_errors(g["mac"], r["mac"])
+    assert e["u"] < 1e-5, e
+
+
+def sphere_small(storage="f32", trt=True):
+    return sc.sphere(nx=64, ny=32, nz=32, radius=5.0, centre=(20.0, 16.0, 16.0), u0=0.05, re=100.0,
+                     trt=trt, storage=storage)
+
+
+def test_sphere_classes_bit_exact():
+    desc, mac, _ = sc.sphere(nx=64, ny=40, nz=36, radius=7.3, centre=(20.25, 19.5, 17.75))
+    with pylbm.Solve(desc) as s:
+        cls = s.read_classes()
+    r = oracle_runner.run(desc, mac, steps=0, want=("cls",))
+    assert np.array_equal(cls, r["cls"])
+    assert int(np.sum(cls & 1)) > 0
+
+
+def test_circle_classes_bit_exact():
+    desc = {"q": 9, "nx": 96, "ny": 48, "nz": 1, "omega": 1.2, "body": {"a": 6.4, "centre": [30.3, 23.7]}}
+    with pylbm.Solve(desc) as s:
+        cls = s.read_classes()
+    r = oracle_runner.run(desc, None, steps=0, want=("cls",))
+    assert np.array_equal(cls, r["cls"])
+
+
+@pytest.mark.parametrize("trt", [False, True])
+def test_sphere_bounceback_sponge_force(trt):
+    desc, mac, meta = sphere_small(trt=trt)
+    steps = 300
+    g = run_gpu(desc, mac, steps)
+    r = oracle_runner.run(desc, mac, steps=steps, threads=4, want=("mac", "force"))
+    e = field_errors(g["mac"], r["mac"])
+    print(e)
+    assert e["rho"] < 1e-5 and e["u"] < 1e-5, e
+    fg, fr = g["force"], r["force"]
+    assert fg.shape == fr.shape == (steps, 6)
+    dev = force_dev(fg, fr)
+    print("max force deviation / max|Fx|:", dev)
+    assert dev.max() < 1e-6, dev
+
+
+def test_circle_cylinder_d2q9_force():
+    desc = {"q": 9, "nx": 128, "ny": 48, "nz": 1, "omega": sc.f32(1 / 0.56),
+            "sponge": {"width": 8, "u": [0.04, 0, 0]}, "body": {"a": 5.0, "centre": [32.0, 24.0]}}
+    n = 128 * 48
+    mac = np.tile(np.array([1 - 0.04 ** 2 * 1.5, 0.04, 0, 0], np.float32), n)
+    g = run_gpu(desc, mac, 400)
+    r = oracle_runner.run(desc, mac, steps=400, want=("mac", "force"))
+    e = field_errors(g["mac"], r["mac"])
+    assert e["u"] < 1e-5 and e["rho"] < 1e-5, e
+    dev = force_dev(g["force"][:, :2], r["force"][:, :2])
+    assert dev.max() < 1e-6, dev
+
+
+@pytest.mark.parametrize("storage,tol_u,tol_f", [("f16", 3e-2, 5e-3), ("bf16", 2e-1, 5e-2)])
+def test_shifted_16bit_storage_error_vs_fp64(storage, tol_u, tol_f):
+    """Shifted 16-bit storage (f - w_i) with FP32 collision, scored against the
+    FP64 oracle: velocity L2 error and drag error reported and bounded."""
+    desc, mac, meta = sphere_small(storage=storage)
+    steps = 300
+    g = run_gpu(desc, mac, steps)
+    d32 = dict(desc, storage="f32")
+    r = oracle_runner.run(d32, mac, steps=steps, threads=4, want=("mac", "force"))
+    e = field_errors(g["mac"], r["mac"])
+    drag = abs(g["force"][-20:, 0].mean() / r["force"][-20:, 0].mean() - 1.0)
+    print(storage, e, "drag err", drag)
+    assert e["u"] < tol_u and drag < tol_f, (e, drag)
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_wire_format_roundtrip_bit_exact(storage):
+    """upload_f packs f32 plane-major populations into shifted 16-bit storage
+    and read_f unpacks them; both must equal the host conversion bit for bit
+    (f-pack.mjs:44-124 semantics: f32 subtract, round to nearest even)."""
+    desc, mac = sc.random_field(16, 8, 8, q=19, seed=5)
+    desc["storage"] = storage
+    rng = np.random.default_rng(9)
+    w = np.array([0.33333337306976318] + [0.055555541068315506] * 6 + [0.027777781710028648] * 12, np.float32)
+    f = (w[:, None] * (1.0 + 0.2 * (rng.random((19, 1024)) - 0.5))).astype(np.float32)
+    f[3, :4] = np.array([w[3] + 2e-8, w[3] - 6e-5, w[3] + 1e-3, w[3]], np.float32)
+    with pylbm.Solve(desc) as s:
+        s.upload_f(f.reshape(-1))
+        back = s.read_f().reshape(19, 1024)
+    dev = (f - w[:, None]).astype(np.float32)
+    if storage == "f16":
+        q = dev.astype(np.float16).astype(np.float32)
+    else:
+        u = dev.view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+        q = (u.astype(np.uint32) << 16).view(np.float32)
+    want = (q + w[:, None]).astype(np.float32)
+    assert np.array_equal(back.view(np.uint32), want.view(np.uint32))
+
+
+def test_sphere_full_config_short_run_and_symmetry():
+    """Config 2 at its full size (256x128x128, sphere R 12 at (64,64,64), TRT,
+    sponge 12): a short parity run against the oracle, then the
+    size-independent checks over a longer GPU run: the force history is
+    complete and finite, and the lateral force stays a small fraction of the
+    drag (the sphere sits half a cell off the box centre line)."""
+    desc, mac, meta = sc.sphere()
+    with pylbm.Solve(desc) as s:
+        s.upload_macro(mac)
+        s.step(6)
+        s.sync()
+        m6 = s.read_macro()
+        f6 = s.read_forces()
+        s.step(494)
+        s.sync()
+        f500 = s.read_forces()
+    r = oracle_runner.run(desc, mac, steps=6, threads=16, want=("mac", "force"))
+    e = field_errors(m6, r["mac"])
+    assert e["u"] < 1e-5 and e["rho"] < 1e-5, e
+    dev = force_dev(f6, r["force"])
+    assert dev.max() < 1e-6, dev
+    assert f500.shape == (500, 6)
+    assert np.all(np.isfinite(f500))
+    tail = f500[-100:]
+    assert np.abs(tail[:, 1:3]).max() < 0.05 * np.abs(tail[:, 0]).mean()
