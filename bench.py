@@ -36,8 +36,12 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_d3q19_sphere_256x128x128.json")
 
 
-def measure(storage, steps, warmup, kernel_reps):
+def measure(storage, steps, warmup, kernel_reps, periodic=False):
     desc, mac, meta = sc.sphere(storage=storage)
+    if periodic:
+        # the same grid with no body, sponge or walls: the step kernel's
+        # periodic fast path, for comparison only
+        desc = {k: v for k, v in desc.items() if k not in ("body", "sponge")}
     desc["autotune"] = 1
     q = desc["q"]
     with pylbm.Solve(desc) as s:
@@ -56,7 +60,7 @@ def measure(storage, steps, warmup, kernel_reps):
     elem = 4 if storage == "f32" else 2
     bytes_per_launch = 2 * q * elem * ncells
     achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
-    cd = forces[-200:, 0].mean() / (0.5 * meta["u0"] ** 2 * meta["area"]) if len(forces) else float("nan")
+    cd = float(forces[-200:, 0].mean() / (0.5 * meta["u0"] ** 2 * meta["area"])) if len(forces) else None
     return {
         "mlups": ncells * steps / secs / 1e6,
         "ms_per_step": secs * 1e3 / steps,
@@ -65,7 +69,7 @@ def measure(storage, steps, warmup, kernel_reps):
         "bytes_per_launch": bytes_per_launch,
         "ncells": ncells,
         "tile": [info["block_x"], info["block_y"], info["swizzle"]],
-        "cd_last200": float(cd),
+        "cd_last200": cd,
         "finite": bool(np.all(np.isfinite(forces))),
     }
 
@@ -100,9 +104,16 @@ def main():
     dev = pylbm.device_info()
     print("device: %s, %d CUs, %.0f GB" % (dev["name"], dev["compute_units"], dev["hbm_bytes"] / 1e9),
           file=sys.stderr, flush=True)
+    copy_gbs, copy_ms = pylbm.copy_bandwidth(1 << 30, 20)
+    print("copy probe: %.1f GB/s" % copy_gbs, file=sys.stderr, flush=True)
     main_run = measure("f32", args.steps, args.warmup, 50)
     print("fp32: %s" % json.dumps(main_run), file=sys.stderr, flush=True)
     extra = {}
+    if not args.no_16bit:
+        per = measure("f32", min(args.steps, 1000), args.warmup, 50, periodic=True)
+        print("fp32 periodic: %s" % json.dumps(per), file=sys.stderr, flush=True)
+        extra["periodic_f32"] = per
+        per["speedup_vs_f32"] = per["mlups"] / main_run["mlups"]
     if not args.no_16bit:
         for st in ("f16", "bf16"):
             r = measure(st, args.steps, args.warmup, 50)
@@ -144,13 +155,15 @@ def main():
                      "kernel": "k_step<19, StoreF32, general>", "kernel_ms": round(main_run["kernel_ms"], 5),
                      "bytes_per_launch": main_run["bytes_per_launch"]},
         "cpu_baseline": cpu,
-        "drag_cd_last200": round(main_run["cd_last200"], 5),
+        "drag_cd_last200": main_run["cd_last200"],
+        "copy_probe_gbs": round(copy_gbs, 1),
+        "frac_of_copy_probe": round(achieved / copy_gbs, 4),
     }
     for k, v in extra.items():
         out[k] = {"mlups": round(v["mlups"], 1), "speedup_vs_f32": round(v["speedup_vs_f32"], 3),
                   "kernel_ms": round(v["kernel_ms"], 5), "achieved_gbs": round(v["achieved_gbs"], 1),
                   "pct_hbm_peak": round(100.0 * v["achieved_gbs"] / HBM_PEAK_GBS, 2),
-                  "cd_last200": round(v["cd_last200"], 5)}
+                  "cd_last200": v["cd_last200"]}
     print(json.dumps(out), flush=True)
 
 

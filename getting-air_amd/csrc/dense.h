@@ -14,10 +14,16 @@ struct StepArgs {
   int32_t nx, ny, nz;
   uint32_t ncells;
   float omega;       // even (BGK) rate
-  float dw;          // TRT: omega_minus - omega; 0 selects plain BGK
   float guo;         // 1 - omega/2
   float wsum_m1;     // (sum of the f32 weights) - 1: 0 for D3Q19/D3Q27, ~1e-8 for D2Q9
   float F[3];        // uniform Guo body force
+  // Collision constants, uniform over the grid (computed on the host).
+  // Pair k couples direction i = lo(k) with opp(i); see k_step.
+  float omega_m;     // odd rate (TRT omega_minus; = omega for BGK)
+  float c1[13];      // 9 (1 - omega/2) w_i (e_i . F)
+  float so[13];      // (1 - omega_m/2) 3 w_i (e_i . F): odd Guo source
+  float ae[14];      // sponge target, even part w_i (4.5 (e_i.U)^2 - 1.5 U.U); [13] = rest
+  float bo[13];      // sponge target, odd part 3 w_i (e_i . U)
   int32_t wall_mask; // bit a: halfway walls on the two faces normal to axis a
   float wall_u0, wall_u1;
   float sponge_w;    // <= 0 disables
@@ -27,25 +33,24 @@ struct StepArgs {
   int32_t solid_eq;
   int32_t solid_mac_rest;  // 3D: a solid cell reports [1, 0, 0, 0]
   float* mac;          // [rho, ux, uy, uz] per cell, or null (not observed)
+  // Momentum-exchange force, fused into the step: tiles holding a fluid cell
+  // with a solid streaming source own a slot of 6 FP64 partials.
+  const int32_t* force_slot;  // per logical tile: slot index or -1; null: no force
+  double* force_partials;     // [slot][6]
+  double body_c[3];           // torque arm origin
+  int32_t periodic_arm;       // 2D: nearest periodic image for the arm
   // tiling of the 1-D grid onto (x-tiles, y-tiles, z)
   int32_t ntx, nty;
   uint32_t nblocks;
   int32_t swizzle;     // 1: consecutive tiles on one XCD
 };
 
-// Momentum-exchange force over the list of fluid cells that have at least
-// one link whose streaming source is solid.
-struct ForceArgs {
-  const uint32_t* cells;   // ascending cell index
-  const uint32_t* masks;   // bit i: the source of direction i is solid
-  uint32_t count;
-  uint32_t ncells;
-  int32_t nx, ny, nz;
-  double cx, cy, cz;
-  int32_t periodic_arm;    // 2D: torque arm is the nearest periodic image
-  double* partials;
-  unsigned* counter;
-  double* history;         // ring of [fx, fy, fz, tx, ty, tz]
+// Sums the force partial slots in slot order and appends one record
+// [fx, fy, fz, tx, ty, tz] to the history ring.
+struct ForceFinalizeArgs {
+  const double* partials;
+  uint32_t nslots;
+  double* history;
   unsigned long long* written;  // records written since reset
   uint32_t capacity;
 };
@@ -65,13 +70,16 @@ struct ClassifyArgs {
   uint8_t* cls;
 };
 
-hipError_t launch_step(int q, int storage, bool general, const void* fin, void* fout,
+hipError_t launch_step(int q, int storage, bool general, bool nontemporal, const void* fin, void* fout,
                        const StepArgs& a, int bx, int by, hipStream_t s);
-hipError_t launch_force(int q, int storage, const void* fin, const ForceArgs& a, hipStream_t s);
+hipError_t launch_force_finalize(const ForceFinalizeArgs& a, hipStream_t s);
 hipError_t launch_totals(int q, int storage, const void* f, const TotalsArgs& a, int nblocks, hipStream_t s);
 hipError_t launch_init_eq(int q, int storage, const float* mac, void* f, uint32_t ncells, hipStream_t s);
 hipError_t launch_pack(int q, int storage, const float* wire, void* f, uint32_t ncells, hipStream_t s);
 hipError_t launch_unpack(int q, int storage, const void* f, float* wire, uint32_t ncells, hipStream_t s);
 hipError_t launch_classify(const ClassifyArgs& a, hipStream_t s);
+// Streaming copy of `bytes` (multiple of 16) used to calibrate HBM bandwidth
+// and the PMC byte counters.
+hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 }  // namespace lbm

@@ -1,7 +1,7 @@
 // Dense-lattice kernels for gfx950: fused pull-stream + collide (D2Q9, D3Q19,
 // D3Q27; BGK or TRT; Guo forcing; halfway bounce-back on walls and a pinned
-// body; ALBC sponge; SOLID_EQ), equilibrium initialisation, FP64
-// momentum-exchange force and FP64 population totals.
+// body; ALBC sponge; SOLID_EQ) with the FP64 momentum-exchange force fused
+// in, equilibrium initialisation and FP64 population totals.
 //
 // Semantics follow the reference kernels
 //   shaders/common_d3_step.wgsl:206-222 (initEq), :224-400 (step)
@@ -14,18 +14,22 @@
 //   g_eq    = w_i ((rho - 1) + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u.u))
 // which is the same collision with the large constant w_i taken out before
 // rounding.
+//
 // Layout (structure of arrays, plane-major): g[i * ncells + (z*ny + y)*nx + x].
 // One thread owns one cell; a wave covers 64 consecutive x, so every plane
 // load and store of a wave is one contiguous 256 B (f32) or 128 B (16-bit)
-// span, shifted by at most one element for the x-moving directions.
+// span, shifted by at most one element for the x-moving directions.  All Q
+// loads are issued back to back (no branch between them); the rare cells next
+// to a wall or the body then patch their reflected directions.
 #include <hip/hip_runtime.h>
 #include "dense.h"
 #include "device_common.h"
+#include "lattice_pairs.h"
 #include "lattice_tables.h"
 
 namespace lbm {
 
-template <int Q, class ST, bool GEN>
+template <int Q, class ST, bool GEN, bool NT>
 __global__ void __launch_bounds__(256)
 k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout, StepArgs a) {
   using L = Lattice<Q>;
@@ -42,137 +46,227 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
   const int z = rest / a.nty;
   const int x = tx * blockDim.x + threadIdx.x;
   const int y = ty * blockDim.y + threadIdx.y;
-  if (x >= a.nx || y >= a.ny || z >= a.nz) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const bool active = x < nx && y < ny;
   const uint32_t N = a.ncells;
-  const uint32_t cell = (uint32_t)((z * ny + y) * nx + x);
+  double fv[6] = {0, 0, 0, 0, 0, 0};  // this cell's force/torque on the body
 
-  // periodic neighbour coordinates
-  const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
-  const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
-  const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
+  if (active) {
+    const uint32_t cell = (uint32_t)((z * ny + y) * nx + x);
+    // periodic neighbour coordinates
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
 
-  uint8_t c = 0;
-  if constexpr (GEN) {
-    if (a.cls) c = a.cls[cell];
-  }
-
-  // 1. pull-stream (+ halfway bounce-back on the body, then on walls)
-  float g[Q];
+    // 1. periodic pull-stream, all Q loads straight-line.  The source of
+    // direction i is cell - e_i, written as per-axis index offsets that wrap
+    // only on the domain faces.
+    const int nxy = nx * ny;
+    const int ox_m = x == 0 ? nx - 1 : -1, ox_p = x == nx - 1 ? 1 - nx : 1;
+    const int oy_m = y == 0 ? (ny - 1) * nx : -nx, oy_p = y == ny - 1 ? (1 - ny) * nx : nx;
+    const int oz_m = z == 0 ? (nz - 1) * nxy : -nxy, oz_p = z == nz - 1 ? (1 - nz) * nxy : nxy;
+    float g[Q];
 #pragma unroll
-  for (int i = 0; i < Q; i++) {
-    const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
-    const int sx = ex == 1 ? xm : (ex == -1 ? xp : x);
-    const int sy = ey == 1 ? ym : (ey == -1 ? yp : y);
-    const int sz = ez == 1 ? zm : (ez == -1 ? zp : z);
-    const uint32_t src = (uint32_t)((sz * ny + sy) * nx + sx);
-    const typename ST::T* plane = gin + (size_t)i * N;
-    if constexpr (GEN) {
-      // w_i = w_opp(i), so a reflected population keeps its shift
-      const typename ST::T* oplane = gin + (size_t)L::opp[i] * N;
-      bool wallOut = false;
-      bool yOut = false;
-      if (a.wall_mask) {
-        const bool xo = (ex == 1 && x == 0) || (ex == -1 && x == nx - 1);
-        yOut = (ey == 1 && y == 0) || (ey == -1 && y == ny - 1);
-        const bool zo = (ez == 1 && z == 0) || (ez == -1 && z == nz - 1);
-        wallOut = ((a.wall_mask & 1) && xo) || ((a.wall_mask & 2) && yOut) || ((a.wall_mask & 4) && zo);
-        yOut = yOut && (a.wall_mask & 2);
-      }
-      if ((c & 2) && (a.cls[src] & 1)) {
-        g[i] = ST::load(oplane, cell);
-      } else if (wallOut) {
-        float corr = 0.f;
-        if (ex != 0 && yOut) {
-          // moving y-wall (common_walls.wgsl:27-31): 2 w e_x u_wall / cs2
-          const float uw = (ey == 1) ? a.wall_u0 : a.wall_u1;
-          corr = 2.f * L::w[i] * (float)ex * uw / (1.0f / 3.0f);
-        }
-        g[i] = ST::load(oplane, cell) + corr;
-      } else {
-        g[i] = ST::load(plane, src);
-      }
-    } else {
-      g[i] = ST::load(plane, src);
+    for (int i = 0; i < Q; i++) {
+      const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
+      const int o = (ex == 1 ? ox_m : (ex == -1 ? ox_p : 0)) + (ey == 1 ? oy_m : (ey == -1 ? oy_p : 0)) +
+                    (ez == 1 ? oz_m : (ez == -1 ? oz_p : 0));
+      g[i] = ld<ST, NT>(gin + (size_t)i * N, (uint32_t)((int)cell + o));
     }
-  }
 
-  // 2. moments
-  float drho = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
+    uint8_t c = 0;
+    if constexpr (GEN) {
+      if (a.cls) c = a.cls[cell];
+      const bool onFace = (a.wall_mask & 1 && (x == 0 || x == nx - 1)) ||
+                          (a.wall_mask & 2 && (y == 0 || y == ny - 1)) ||
+                          (a.wall_mask & 4 && (z == 0 || z == nz - 1));
+      if ((c & 2) || onFace) {
+        // 1b. halfway bounce-back: the body first, then the walls
+        // (common_d3_step.wgsl:250-276).  w_i = w_opp(i), so a reflected
+        // population keeps its shift.
+        double fx = 0, fy = 0, fz = 0;
 #pragma unroll
-  for (int i = 0; i < Q; i++) {
-    drho += g[i];
-    if (L::ex[i] == 1) mx += g[i]; else if (L::ex[i] == -1) mx -= g[i];
-    if (L::ey[i] == 1) my += g[i]; else if (L::ey[i] == -1) my -= g[i];
-    if (L::ez[i] == 1) mz += g[i]; else if (L::ez[i] == -1) mz -= g[i];
-  }
-  drho += a.wsum_m1;
-  const float rho = 1.f + drho;
-  const float rinv = 1.0f / fmaxf(rho, 1e-6f);
-  const float Fx = a.F[0], Fy = a.F[1], Fz = a.F[2];
-  // Guo: u = u* + F / (2 rho)
-  const float ux = (mx + 0.5f * Fx) * rinv;
-  const float uy = (my + 0.5f * Fy) * rinv;
-  const float uz = (mz + 0.5f * Fz) * rinv;
-  const float usq = ux * ux + uy * uy + uz * uz;
-  const float uF = ux * Fx + uy * Fy + uz * Fz;
+        for (int i = 1; i < Q; i++) {
+          const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
+          const int sx = ex == 1 ? xm : (ex == -1 ? xp : x);
+          const int sy = ey == 1 ? ym : (ey == -1 ? yp : y);
+          const int sz = ez == 1 ? zm : (ez == -1 ? zp : z);
+          const typename ST::T* oplane = gin + (size_t)L::opp[i] * N;
+          if ((c & 2) && (a.cls[(uint32_t)((sz * ny + sy) * nx + sx)] & 1)) {
+            g[i] = ST::load(oplane, cell);
+            // momentum exchange on a fluid cell: -e_i (2 f_opp), pinned body
+            if (!(c & 1)) {
+              const double m = 2.0 * ((double)g[i] + (double)L::w[i]);
+              fx -= ex * m;
+              fy -= ey * m;
+              fz -= ez * m;
+            }
+          } else if (onFace) {
+            const bool xo = (ex == 1 && x == 0) || (ex == -1 && x == nx - 1);
+            const bool yo = (ey == 1 && y == 0) || (ey == -1 && y == ny - 1);
+            const bool zo = (ez == 1 && z == 0) || (ez == -1 && z == nz - 1);
+            const bool yWall = (a.wall_mask & 2) && yo;
+            if (((a.wall_mask & 1) && xo) || yWall || ((a.wall_mask & 4) && zo)) {
+              float corr = 0.f;
+              if (ex != 0 && yWall) {
+                // moving y-wall (common_walls.wgsl:27-31): 2 w e_x u_wall / cs2
+                const float uw = (ey == 1) ? a.wall_u0 : a.wall_u1;
+                corr = 2.f * L::w[i] * (float)ex * uw / (1.0f / 3.0f);
+              }
+              g[i] = ST::load(oplane, cell) + corr;
+            }
+          }
+        }
+        if (a.force_slot && (fx != 0.0 || fy != 0.0 || fz != 0.0)) {
+          double rx = x - a.body_c[0], ry = y - a.body_c[1], rz = z - a.body_c[2];
+          if (a.periodic_arm) {
+            rx -= nx * rint(rx / nx);
+            ry -= ny * rint(ry / ny);
+            rz = 0.0;
+          }
+          fv[0] = fx; fv[1] = fy; fv[2] = fz;
+          fv[3] = ry * fz - rz * fy;
+          fv[4] = rz * fx - rx * fz;
+          fv[5] = rx * fy - ry * fx;
+        }
+      }
+    }
 
-  bool solid = false;
-  if constexpr (GEN) solid = a.solid_eq && (c & 1);
+    // 2. pair sums and differences; moments from them
+    constexpr int NP = Pairs<Q>::NP;
+    float S[NP], D[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      const int i = Pairs<Q>::lo(k);
+      S[k] = g[i] + g[L::opp[i]];
+      D[k] = g[i] - g[L::opp[i]];
+    }
+    float drho = g[0] + a.wsum_m1, mx = 0.f, my = 0.f, mz = 0.f;
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      const int i = Pairs<Q>::lo(k);
+      drho += S[k];
+      if (L::ex[i] == 1) mx += D[k]; else if (L::ex[i] == -1) mx -= D[k];
+      if (L::ey[i] == 1) my += D[k]; else if (L::ey[i] == -1) my -= D[k];
+      if (L::ez[i] == 1) mz += D[k]; else if (L::ez[i] == -1) mz -= D[k];
+    }
+    const float rho = 1.f + drho;
+    const float rinv = 1.0f / fmaxf(rho, 1e-6f);
+    const float Fx = a.F[0], Fy = a.F[1], Fz = a.F[2];
+    // Guo: u = u* + F / (2 rho)
+    const float ux = (mx + 0.5f * Fx) * rinv;
+    const float uy = (my + 0.5f * Fy) * rinv;
+    const float uz = (mz + 0.5f * Fz) * rinv;
+    const float usq = ux * ux + uy * uy + uz * uz;
+    const float uF = ux * Fx + uy * Fy + uz * Fz;
 
-  if (a.mac) {
-    float4 m = make_float4(rho, ux, uy, uz);
-    if (solid && a.solid_mac_rest) m = make_float4(1.f, 0.f, 0.f, 0.f);
-    reinterpret_cast<float4*>(a.mac)[cell] = m;
-  }
+    bool solid = false;
+    if constexpr (GEN) solid = a.solid_eq && (c & 1);
 
-  if constexpr (GEN) {
+    if (a.mac) {
+      float4 m = make_float4(rho, ux, uy, uz);
+      if (solid && a.solid_mac_rest) m = make_float4(1.f, 0.f, 0.f, 0.f);
+      reinterpret_cast<float4*>(a.mac)[cell] = m;
+    }
+
     if (solid) {
       // SOLID_EQ for a pinned body: feq(1, 0) = w_i, i.e. g = 0
 #pragma unroll
-      for (int i = 0; i < Q; i++) ST::store(gout + (size_t)i * N, cell, 0.f);
-      return;
-    }
-  }
+      for (int i = 0; i < Q; i++) st<ST, NT>(gout + (size_t)i * N, cell, 0.f);
+    } else {
+      float sp = 0.f;
+      if constexpr (GEN) {
+        if (a.sponge_w > 0.f) {
+          const float big = 1e30f;
+          const float dX = (a.sponge_mask & 1) ? fminf((float)x, (float)(nx - 1 - x)) : big;
+          const float dY = (a.sponge_mask & 2) ? fminf((float)y, (float)(ny - 1 - y)) : big;
+          const float dZ = (a.sponge_mask & 4) ? fminf((float)z, (float)(nz - 1 - z)) : big;
+          float t = 1.f - fminf(dX, fminf(dY, dZ)) / a.sponge_w;
+          t = fminf(fmaxf(t, 0.f), 1.f);
+          sp = t * t * (3.f - 2.f * t);
+        }
+      }
 
-  float sp = 0.f;
-  if constexpr (GEN) {
-    if (a.sponge_w > 0.f) {
-      const float big = 1e30f;
-      const float dX = (a.sponge_mask & 1) ? fminf((float)x, (float)(nx - 1 - x)) : big;
-      const float dY = (a.sponge_mask & 2) ? fminf((float)y, (float)(ny - 1 - y)) : big;
-      const float dZ = (a.sponge_mask & 4) ? fminf((float)z, (float)(nz - 1 - z)) : big;
-      float t = 1.f - fminf(dX, fminf(dY, dZ)) / a.sponge_w;
-      t = fminf(fmaxf(t, 0.f), 1.f);
-      sp = t * t * (3.f - 2.f * t);
-    }
-  }
-  const float uFarSq = a.su[0] * a.su[0] + a.su[1] * a.su[1] + a.su[2] * a.su[2];
-
-  // 3. collide: BGK + Guo, TRT delta terms, sponge blend
-  const float om = a.omega, dw = a.dw, guo = a.guo;
+      // 3. collide pair by pair.  With g+ = (g_i + g_j)/2, g- = (g_i - g_j)/2
+      // (j = opp i) and the equilibrium split into its even part
+      //   A = w (rho - 1 + rho (4.5 (e.u)^2 - 1.5 u.u))
+      // and odd part B = 3 w rho (e.u), BGK/TRT with Guo forcing reads
+      //   P = g+ - omega   (g+ - A) + (1 - omega/2)   w (9 (e.u)(e.F) - 3 u.F)
+      //   M = g- - omega_m (g- - B) + (1 - omega_m/2) 3 w (e.F)
+      //   g_i' = P + M,  g_j' = P - M,
+      // which is the BGK line plus the TRT delta terms of plans/TRT.md:333-349
+      // regrouped (omega_m = omega gives BGK).  The sponge blends P and M
+      // toward the even and odd parts of feq(1, U_far) - w.
+      //
+      // Rounding: every relaxation keeps the form h - omega (h - eq), and the
+      // exact f32 weight w is the LAST factor of A and B.  A pre-rounded
+      // constant such as f32(3 w) or f32((1 - omega_m)/2) would carry the
+      // same relative bias (~1e-8) into every cell and every step, which
+      // breaks mass/momentum conservation systematically: the error then
+      // grows linearly in time instead of as a random walk.
+      const float om = a.omega, omm = a.omega_m;
+      const float c0 = drho - 1.5f * rho * usq;
+      const float g3uF = 3.f * a.guo * uF;
+      {
+        const float w = L::w[0];
+        float p = g[0] - om * (g[0] - w * c0) - w * g3uF;
+        if constexpr (GEN) p += sp * (a.ae[13] - p);
+        st<ST, NT>(gout, cell, p);
+      }
 #pragma unroll
-  for (int i = 0; i < Q; i++) {
-    const float exf = (float)L::ex[i], eyf = (float)L::ey[i], ezf = (float)L::ez[i];
-    const float w = L::w[i];
-    const float eu = exf * ux + eyf * uy + ezf * uz;
-    const float geq = w * (drho + rho * (3.f * eu + 4.5f * eu * eu - 1.5f * usq));
-    const float eF = exf * Fx + eyf * Fy + ezf * Fz;
-    const float Si = guo * w * (3.f * (eF - uF) + 9.f * eu * eF);
-    float gc = g[i] - om * (g[i] - geq) + Si;
-    if (dw != 0.f) {
-      const float gMinus = 0.5f * (g[i] - g[L::opp[i]]);
-      const float eqMinus = 3.f * w * rho * eu;
-      gc = gc - dw * (gMinus - eqMinus) - 0.5f * dw * (3.f * w * eF);
-    }
-    if constexpr (GEN) {
-      if (sp > 0.f) {
-        const float euF = exf * a.su[0] + eyf * a.su[1] + ezf * a.su[2];
-        const float gt = w * (3.f * euF + 4.5f * euF * euF - 1.5f * uFarSq);
-        gc = gc * (1.f - sp) + gt * sp;
+      for (int k = 0; k < NP; k++) {
+        const int i = Pairs<Q>::lo(k);
+        const int j = L::opp[i];
+        const float w = L::w[i];
+        const float eu = (float)L::ex[i] * ux + (float)L::ey[i] * uy + (float)L::ez[i] * uz;
+        const float t = rho * eu;
+        const float A = w * (c0 + 4.5f * (t * eu));
+        const float B = w * (3.f * t);
+        const float Se = a.c1[k] * eu - w * g3uF;
+        const float hS = 0.5f * S[k], hD = 0.5f * D[k];
+        float P = hS - om * (hS - A) + Se;
+        float M = hD - omm * (hD - B) + a.so[k];
+        if constexpr (GEN) {
+          P += sp * (a.ae[k] - P);
+          M += sp * (a.bo[k] - M);
+        }
+        st<ST, NT>(gout + (size_t)i * N, cell, P + M);
+        st<ST, NT>(gout + (size_t)j * N, cell, P - M);
       }
     }
-    ST::store(gout + (size_t)i * N, cell, gc);
+  }
+
+  if constexpr (GEN) {
+    // Tiles touching the body reduce their cells' force in FP64 and park it
+    // in the tile's slot; k_force_finalize sums the slots in order.
+    if (a.force_slot) {
+      const int slot = a.force_slot[b];
+      if (slot >= 0) {
+        __shared__ double lds[16 * 6];
+        block_sum<6>(fv, lds);
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+#pragma unroll
+          for (int k = 0; k < 6; k++) a.force_partials[(size_t)slot * 6 + k] = fv[k];
+        }
+      }
+    }
+  }
+}
+
+// One wave: lane l sums slots l, l+64, ... in order, then a fixed butterfly.
+__global__ void __launch_bounds__(64) k_force_finalize(ForceFinalizeArgs a) {
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (uint32_t s = threadIdx.x; s < a.nslots; s += 64) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] += a.partials[(size_t)s * 6 + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) v[k] = wave_sum(v[k]);
+  if (threadIdx.x == 0) {
+    const unsigned long long n = *a.written;
+#pragma unroll
+    for (int k = 0; k < 6; k++) a.history[(size_t)(n % a.capacity) * 6 + k] = v[k];
+    *a.written = n + 1;
   }
 }
 
@@ -212,54 +306,6 @@ k_unpack(const typename ST::T* __restrict__ g, float* __restrict__ wire, uint32_
   if (cell >= N) return;
 #pragma unroll
   for (int i = 0; i < Q; i++) wire[(size_t)i * N + cell] = ST::load(g + (size_t)i * N, cell) + L::w[i];
-}
-
-// Momentum exchange on the pinned body from pre-stream populations, summed in
-// FP64 (the reference sums per workgroup in f32 and accumulates i32 fixed
-// point, common_d3_force.wgsl:271-282).
-template <int Q, class ST>
-__global__ void __launch_bounds__(256)
-k_force(const typename ST::T* __restrict__ gin, ForceArgs a) {
-  using L = Lattice<Q>;
-  __shared__ double lds[16 * 6];
-  double v[6] = {0, 0, 0, 0, 0, 0};
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < a.count) {
-    const uint32_t cell = a.cells[k];
-    const uint32_t mask = a.masks[k];
-    double fx = 0, fy = 0, fz = 0;
-#pragma unroll
-    for (int i = 1; i < Q; i++) {
-      if (mask & (1u << i)) {
-        const double f_opp = (double)ST::load(gin + (size_t)L::opp[i] * a.ncells, cell) + (double)L::w[i];
-        const double m = 2.0 * f_opp;
-        fx -= L::ex[i] * m;
-        fy -= L::ey[i] * m;
-        fz -= L::ez[i] * m;
-      }
-    }
-    const int x = cell % a.nx;
-    const int y = (cell / a.nx) % a.ny;
-    const int z = cell / (a.nx * a.ny);
-    double rx = x - a.cx, ry = y - a.cy, rz = z - a.cz;
-    if (a.periodic_arm) {
-      rx -= a.nx * rint(rx / a.nx);
-      ry -= a.ny * rint(ry / a.ny);
-      rz = 0.0;
-    }
-    v[0] = fx; v[1] = fy; v[2] = fz;
-    v[3] = ry * fz - rz * fy;
-    v[4] = rz * fx - rx * fz;
-    v[5] = rx * fy - ry * fx;
-  }
-  block_sum<6>(v, lds);
-  partials_then_fixed_order_sum<6>(v, a.partials, a.counter, [&](int j, double s) {
-    // The six writers are lanes of one wave, so every history store issues
-    // before lane 0 advances the count.
-    const unsigned long long n = *a.written;
-    a.history[(size_t)(n % a.capacity) * 6 + j] = s;
-    if (j == 0) *a.written = n + 1;
-  });
 }
 
 template <int Q, class ST>
@@ -315,6 +361,12 @@ __global__ void __launch_bounds__(256) k_classify(ClassifyArgs a) {
   a.cls[cell] = c;
 }
 
+// 16-byte streaming copy, the bandwidth reference for the roofline.
+__global__ void __launch_bounds__(256) k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // dispatch: one Op<Q, Storage>::run per kernel family
 // ---------------------------------------------------------------------------
@@ -339,12 +391,19 @@ static hipError_t dispatch(int q, int storage, A... args) {
 static inline dim3 cells_grid(uint32_t n) { return dim3((n + 255) / 256); }
 
 template <int Q, class ST> struct OpStep {
-  static hipError_t run(bool general, const void* fin, void* fout, StepArgs a, int bx, int by, hipStream_t s) {
+  template <bool GEN, bool NT>
+  static void go(const void* fin, void* fout, const StepArgs& a, int bx, int by, hipStream_t s) {
     using T = typename ST::T;
-    if (general)
-      hipLaunchKernelGGL((k_step<Q, ST, true>), dim3(a.nblocks), dim3(bx, by), 0, s, (const T*)fin, (T*)fout, a);
-    else
-      hipLaunchKernelGGL((k_step<Q, ST, false>), dim3(a.nblocks), dim3(bx, by), 0, s, (const T*)fin, (T*)fout, a);
+    hipLaunchKernelGGL((k_step<Q, ST, GEN, NT>), dim3(a.nblocks), dim3(bx, by), 0, s, (const T*)fin, (T*)fout, a);
+  }
+  static hipError_t run(bool general, bool nt, const void* fin, void* fout, StepArgs a, int bx, int by, hipStream_t s) {
+    if (general) {
+      if (nt) go<true, true>(fin, fout, a, bx, by, s);
+      else go<true, false>(fin, fout, a, bx, by, s);
+    } else {
+      if (nt) go<false, true>(fin, fout, a, bx, by, s);
+      else go<false, false>(fin, fout, a, bx, by, s);
+    }
     return hipGetLastError();
   }
 };
@@ -366,12 +425,6 @@ template <int Q, class ST> struct OpUnpack {
     return hipGetLastError();
   }
 };
-template <int Q, class ST> struct OpForce {
-  static hipError_t run(const void* fin, ForceArgs a, hipStream_t s) {
-    hipLaunchKernelGGL((k_force<Q, ST>), cells_grid(a.count), dim3(256), 0, s, (const typename ST::T*)fin, a);
-    return hipGetLastError();
-  }
-};
 template <int Q, class ST> struct OpTotals {
   static hipError_t run(const void* f, TotalsArgs a, int nblocks, hipStream_t s) {
     hipLaunchKernelGGL((k_totals<Q, ST>), dim3(nblocks), dim3(256), 0, s, (const typename ST::T*)f, a);
@@ -379,9 +432,9 @@ template <int Q, class ST> struct OpTotals {
   }
 };
 
-hipError_t launch_step(int q, int storage, bool general, const void* fin, void* fout,
+hipError_t launch_step(int q, int storage, bool general, bool nontemporal, const void* fin, void* fout,
                        const StepArgs& a, int bx, int by, hipStream_t s) {
-  return dispatch<OpStep>(q, storage, general, fin, fout, a, bx, by, s);
+  return dispatch<OpStep>(q, storage, general, nontemporal, fin, fout, a, bx, by, s);
 }
 hipError_t launch_init_eq(int q, int storage, const float* mac, void* f, uint32_t N, hipStream_t s) {
   return dispatch<OpInit>(q, storage, mac, f, N, s);
@@ -392,16 +445,21 @@ hipError_t launch_pack(int q, int storage, const float* wire, void* f, uint32_t 
 hipError_t launch_unpack(int q, int storage, const void* f, float* wire, uint32_t N, hipStream_t s) {
   return dispatch<OpUnpack>(q, storage, f, wire, N, s);
 }
-hipError_t launch_force(int q, int storage, const void* fin, const ForceArgs& a, hipStream_t s) {
-  return dispatch<OpForce>(q, storage, fin, a, s);
-}
 hipError_t launch_totals(int q, int storage, const void* f, const TotalsArgs& a, int nblocks, hipStream_t s) {
   return dispatch<OpTotals>(q, storage, f, a, nblocks, s);
 }
-
+hipError_t launch_force_finalize(const ForceFinalizeArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_force_finalize, dim3(1), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
 hipError_t launch_classify(const ClassifyArgs& a, hipStream_t s) {
   const uint32_t n = (uint32_t)a.nx * a.ny * a.nz;
   hipLaunchKernelGGL(k_classify, dim3((n + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
   return hipGetLastError();
 }
 

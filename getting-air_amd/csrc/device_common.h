@@ -15,11 +15,26 @@ namespace lbm {
 // that much less absolute round-off; the arithmetic is FP32 throughout.
 // load/store move g; the 16-bit formats round to nearest even.
 // ---------------------------------------------------------------------------
+// Element i of a plane as a 32-bit byte offset from the plane base: the base
+// is wave-uniform, so loads and stores can use the scalar-base + 32-bit
+// vector-offset addressing form instead of 64-bit per-lane address math.
+// Every plane is below 4 GiB (the engine caps cells per plane).
+template <class T>
+__device__ __forceinline__ const T* at(const T* p, uint32_t i) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ T* at(T* p, uint32_t i) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(p) + i * (uint32_t)sizeof(T));
+}
+
 struct StoreF32 {
   using T = float;
   static constexpr int kBytes = 4;
-  __device__ __forceinline__ static float load(const T* p, uint32_t i) { return p[i]; }
-  __device__ __forceinline__ static void store(T* p, uint32_t i, float g) { p[i] = g; }
+  __device__ __forceinline__ static float load(const T* p, uint32_t i) { return *at(p, i); }
+  __device__ __forceinline__ static void store(T* p, uint32_t i, float g) { *at(p, i) = g; }
+  __device__ __forceinline__ static float load_nt(const T* p, uint32_t i) { return __builtin_nontemporal_load(at(p, i)); }
+  __device__ __forceinline__ static void store_nt(T* p, uint32_t i, float g) { __builtin_nontemporal_store(g, at(p, i)); }
 };
 
 __device__ __forceinline__ float half_bits_to_float(uint16_t h) {
@@ -43,16 +58,40 @@ __device__ __forceinline__ uint16_t float_to_bf16_bits(float v) {
 struct StoreF16 {
   using T = uint16_t;
   static constexpr int kBytes = 2;
-  __device__ __forceinline__ static float load(const T* p, uint32_t i) { return half_bits_to_float(p[i]); }
-  __device__ __forceinline__ static void store(T* p, uint32_t i, float g) { p[i] = float_to_half_bits(g); }
+  __device__ __forceinline__ static float load(const T* p, uint32_t i) { return half_bits_to_float(*at(p, i)); }
+  __device__ __forceinline__ static void store(T* p, uint32_t i, float g) { *at(p, i) = float_to_half_bits(g); }
+  __device__ __forceinline__ static float load_nt(const T* p, uint32_t i) {
+    return half_bits_to_float(__builtin_nontemporal_load(at(p, i)));
+  }
+  __device__ __forceinline__ static void store_nt(T* p, uint32_t i, float g) {
+    __builtin_nontemporal_store(float_to_half_bits(g), at(p, i));
+  }
 };
 
 struct StoreBF16 {
   using T = uint16_t;
   static constexpr int kBytes = 2;
-  __device__ __forceinline__ static float load(const T* p, uint32_t i) { return bf16_bits_to_float(p[i]); }
-  __device__ __forceinline__ static void store(T* p, uint32_t i, float g) { p[i] = float_to_bf16_bits(g); }
+  __device__ __forceinline__ static float load(const T* p, uint32_t i) { return bf16_bits_to_float(*at(p, i)); }
+  __device__ __forceinline__ static void store(T* p, uint32_t i, float g) { *at(p, i) = float_to_bf16_bits(g); }
+  __device__ __forceinline__ static float load_nt(const T* p, uint32_t i) {
+    return bf16_bits_to_float(__builtin_nontemporal_load(at(p, i)));
+  }
+  __device__ __forceinline__ static void store_nt(T* p, uint32_t i, float g) {
+    __builtin_nontemporal_store(float_to_bf16_bits(g), at(p, i));
+  }
 };
+
+// Chooses the plain or non-temporal access per kernel instantiation.
+template <class ST, bool NT>
+__device__ __forceinline__ float ld(const typename ST::T* p, uint32_t i) {
+  if constexpr (NT) return ST::load_nt(p, i);
+  else return ST::load(p, i);
+}
+template <class ST, bool NT>
+__device__ __forceinline__ void st(typename ST::T* p, uint32_t i, float g) {
+  if constexpr (NT) ST::store_nt(p, i, g);
+  else ST::store(p, i, g);
+}
 
 // ---------------------------------------------------------------------------
 // FP64 reductions.  Wave64 butterfly with __shfl_xor, then one slot per wave

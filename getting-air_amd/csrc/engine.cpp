@@ -1,13 +1,14 @@
 // Host side of the dense engine: the lbm_solve object behind include/lbm.h.
 //
-// A solve owns two population lattices (ping-pong), the macroscopic field,
-// the body classification and its boundary-link list, the FP64 reduction
-// scratch and one HIP stream.  Each lbm_step(n) enqueues, per step, the FP64
-// force reduction on the pre-stream lattice (when a body is present) and the
-// fused stream-collide; only the last step of a call writes the macroscopic
-// field, which is what the reference's readbacks observe
-// (main-3d.js:3506-3539).  Chunks of steps are replayed from captured
-// hipGraphs.
+// A solve owns two population lattices (ping-pong, shifted storage), the
+// macroscopic field, the body classification, the per-tile force slots and
+// one HIP stream.  Each step is one fused stream-collide launch (which also
+// reduces the momentum-exchange force of the tiles touching the body into
+// FP64 slots) followed, when a body is present, by a one-wave launch that
+// sums the slots in a fixed order into the force history.  Only the last step
+// of an lbm_step(n) call writes the macroscopic field, which is what the
+// reference's readbacks observe (main-3d.js:3506-3539).  Chunks of steps are
+// replayed from captured hipGraphs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,6 +25,7 @@
 #include "../../include/lbm.h"
 #include "dense.h"
 #include "engine.h"
+#include "lattice_pairs.h"
 
 namespace lbm {
 
@@ -69,12 +71,17 @@ struct TuneKey {
     return std::tie(q, storage, general, nx, ny, nz) < std::tie(o.q, o.storage, o.general, o.nx, o.ny, o.nz);
   }
 };
-struct TuneVal { int bx, by, swz; };
+struct TuneVal { int bx, by, swz, nt; };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneVal> g_tune_cache;
 
 const uint32_t kHistoryCap = 1u << 16;
 const int kGraphChunk = 16;  // steps per captured graph (even: parity returns)
+
+const int kEx9[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1}, kEy9[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+const int kEx3[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
+const int kEy3[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1};
+const int kEz3[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1};
 
 }  // namespace
 
@@ -83,27 +90,31 @@ struct lbm_solve {
   int q = 0, dim = 3, storage = 0, elem = 4;
   uint32_t N = 0;
   bool general = false;
+  bool body = false;
   void* f[2] = {nullptr, nullptr};
   int parity = 0;
   float* mac = nullptr;
   uint8_t* cls = nullptr;
-  uint32_t* bcells = nullptr;
-  uint32_t* bmasks = nullptr;
-  uint32_t nboundary = 0;
-  double* scratch = nullptr;      // partial sums (force + totals)
-  unsigned* counters = nullptr;   // [force, totals]
+  std::vector<uint32_t> boundary;   // fluid cells with a solid streaming source
+  int32_t* force_slot = nullptr;    // per tile of the current shape
+  double* force_partials = nullptr;
+  uint32_t nslots = 0;
+  int slot_bx = 0, slot_by = 0;
+  size_t slot_bytes = 0, partial_bytes = 0;
+  double* scratch = nullptr;        // totals partials
+  unsigned* counters = nullptr;
   double* history = nullptr;
   unsigned long long* written = nullptr;
   double* totals = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int bx = 64, by = 1, swz = 1;
+  int bx = 64, by = 1, swz = 1, nt = 0;
   bool tuned = false;
   bool initialised = false;
   long long step = 0;
   size_t bytes = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
-  int graph_bx = 0, graph_by = 0, graph_swz = 0;
+  int graph_bx = 0, graph_by = 0, graph_swz = 0, graph_nt = 0;
 };
 
 namespace {
@@ -119,6 +130,12 @@ int dalloc(lbm_solve* s, T** p, size_t bytes) {
   return LBM_OK;
 }
 
+void dfree(lbm_solve* s, void* p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  s->bytes -= bytes;
+}
+
 int validate(const lbm_desc* d) {
   if (!d) return fail(LBM_E_INVALID, "null descriptor");
   if (d->abi_version != LBM_ABI_VERSION)
@@ -127,7 +144,8 @@ int validate(const lbm_desc* d) {
   if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(LBM_E_INVALID, "grid %dx%dx%d", d->nx, d->ny, d->nz);
   if (d->q == 9 && d->nz != 1) return fail(LBM_E_INVALID, "D2Q9 needs nz = 1 (got %d)", d->nz);
   const double n = (double)d->nx * d->ny * d->nz;
-  if (n * 27 > 4.0e12 || n >= 2147483647.0) return fail(LBM_E_INVALID, "grid too large (%g cells)", n);
+  // one population plane must stay below 4 GiB (32-bit byte offsets in the kernels)
+  if (n * 4.0 >= 4294967296.0) return fail(LBM_E_INVALID, "grid too large (%g cells; the limit is 2^30 per lattice)", n);
   if (!(d->omega > 0.f && d->omega < 2.f)) return fail(LBM_E_INVALID, "omega %g outside (0, 2)", d->omega);
   if (d->collision != LBM_BGK && d->collision != LBM_TRT) return fail(LBM_E_INVALID, "collision %d", d->collision);
   if (d->collision == LBM_TRT && !(d->omega_minus > 0.f && d->omega_minus < 2.f))
@@ -146,13 +164,33 @@ int validate(const lbm_desc* d) {
   return LBM_OK;
 }
 
-StepArgs step_args(lbm_solve* s, bool write_mac) {
+// Per-pair collision constants of k_step (see the kernel's pair form).
+template <int Q>
+void pair_constants(StepArgs& a, const lbm_desc& d) {
+  using L = Lattice<Q>;
+  const float om = a.omega, omm = a.omega_m;
+  const float U[3] = {d.sponge_u[0], d.sponge_u[1], d.sponge_u[2]};
+  const float uu = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
+  for (int k = 0; k < Pairs<Q>::NP; k++) {
+    const int i = Pairs<Q>::lo(k);
+    const float w = L::w[i];
+    const float eF = L::ex[i] * d.force[0] + L::ey[i] * d.force[1] + L::ez[i] * d.force[2];
+    const float eU = L::ex[i] * U[0] + L::ey[i] * U[1] + L::ez[i] * U[2];
+    a.c1[k] = 9.f * (1.f - 0.5f * om) * w * eF;
+    a.so[k] = (1.f - 0.5f * omm) * 3.f * w * eF;
+    a.ae[k] = w * (4.5f * eU * eU - 1.5f * uu);
+    a.bo[k] = 3.f * w * eU;
+  }
+  a.ae[13] = L::w[0] * (-1.5f * uu);
+}
+
+StepArgs step_args(lbm_solve* s, bool write_mac, bool with_force) {
   StepArgs a{};
   const lbm_desc& d = s->d;
+  a.omega_m = d.collision == LBM_TRT ? d.omega_minus : d.omega;
   a.nx = d.nx; a.ny = d.ny; a.nz = d.nz;
   a.ncells = s->N;
   a.omega = d.omega;
-  a.dw = d.collision == LBM_TRT ? (float)(d.omega_minus - d.omega) : 0.f;
   a.guo = 1.f - 0.5f * d.omega;
   a.wsum_m1 = s->q == 9 ? kD2Q9WeightSumMinus1 : 0.f;
   for (int k = 0; k < 3; k++) a.F[k] = d.force[k];
@@ -162,47 +200,75 @@ StepArgs step_args(lbm_solve* s, bool write_mac) {
   a.sponge_w = d.sponge_width;
   a.sponge_mask = (d.sponge_axes[0] ? 1 : 0) | (d.sponge_axes[1] ? 2 : 0) | (d.sponge_axes[2] && s->dim == 3 ? 4 : 0);
   for (int k = 0; k < 3; k++) a.su[k] = d.sponge_u[k];
-  a.cls = d.body.shape != LBM_BODY_NONE ? s->cls : nullptr;
+  a.cls = s->body ? s->cls : nullptr;
   a.solid_eq = d.solid_eq;
   a.solid_mac_rest = s->dim == 3;
   a.mac = write_mac ? s->mac : nullptr;
+  if (with_force && s->body && s->nslots) {
+    a.force_slot = s->force_slot;
+    a.force_partials = s->force_partials;
+  }
+  a.body_c[0] = d.body.centre[0];
+  a.body_c[1] = d.body.centre[1];
+  a.body_c[2] = s->dim == 3 ? d.body.centre[2] : 0.0;
+  a.periodic_arm = s->dim == 2;
   a.ntx = (d.nx + s->bx - 1) / s->bx;
   a.nty = (d.ny + s->by - 1) / s->by;
   a.nblocks = (uint32_t)a.ntx * a.nty * d.nz;
   a.swizzle = s->swz;
+  if (s->q == 9) pair_constants<9>(a, d);
+  else if (s->q == 19) pair_constants<19>(a, d);
+  else pair_constants<27>(a, d);
   return a;
 }
 
-ForceArgs force_args(lbm_solve* s) {
-  ForceArgs a{};
-  a.cells = s->bcells;
-  a.masks = s->bmasks;
-  a.count = s->nboundary;
-  a.ncells = s->N;
-  a.nx = s->d.nx; a.ny = s->d.ny; a.nz = s->d.nz;
-  a.cx = s->d.body.centre[0];
-  a.cy = s->d.body.centre[1];
-  a.cz = s->dim == 3 ? s->d.body.centre[2] : 0.0;
-  a.periodic_arm = s->dim == 2;
-  a.partials = s->scratch;
-  a.counter = s->counters;
-  a.history = s->history;
-  a.written = s->written;
-  a.capacity = kHistoryCap;
-  return a;
+// Maps every tile of the current shape that holds a boundary cell to a slot,
+// in ascending tile order (so the finalize sum order is fixed per shape).
+int build_force_slots(lbm_solve* s) {
+  if (!s->body) return LBM_OK;
+  if (s->slot_bx == s->bx && s->slot_by == s->by && s->force_slot) return LBM_OK;
+  const lbm_desc& d = s->d;
+  const int ntx = (d.nx + s->bx - 1) / s->bx, nty = (d.ny + s->by - 1) / s->by;
+  const size_t ntiles = (size_t)ntx * nty * d.nz;
+  std::vector<int32_t> slot(ntiles, -1);
+  for (uint32_t cell : s->boundary) {
+    const int x = cell % d.nx, y = (cell / d.nx) % d.ny, z = cell / (d.nx * d.ny);
+    slot[((size_t)z * nty + y / s->by) * ntx + x / s->bx] = 0;
+  }
+  uint32_t n = 0;
+  for (auto& v : slot)
+    if (v == 0) v = (int32_t)n++;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  dfree(s, s->force_slot, s->slot_bytes);
+  dfree(s, s->force_partials, s->partial_bytes);
+  s->force_slot = nullptr;
+  s->force_partials = nullptr;
+  s->slot_bytes = ntiles * 4;
+  s->partial_bytes = std::max<size_t>(n, 1) * 6 * 8;
+  int rc = dalloc(s, &s->force_slot, s->slot_bytes);
+  if (rc) return rc;
+  rc = dalloc(s, &s->force_partials, s->partial_bytes);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(s->force_slot, slot.data(), ntiles * 4, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->nslots = n;
+  s->slot_bx = s->bx;
+  s->slot_by = s->by;
+  return LBM_OK;
 }
 
-// One step: force on the pre-stream lattice, then stream-collide.
+// One step: fused stream-collide (+ force partials), then the force sum.
 int enqueue_step(lbm_solve* s, bool write_mac) {
   const void* fin = s->f[s->parity];
   void* fout = s->f[1 - s->parity];
-  if (s->nboundary) {
-    hipError_t e = launch_force(s->q, s->storage, fin, force_args(s), s->stream);
-    if (e != hipSuccess) return hip_fail(e, "force kernel launch");
-  }
-  StepArgs a = step_args(s, write_mac);
-  hipError_t e = launch_step(s->q, s->storage, s->general, fin, fout, a, s->bx, s->by, s->stream);
+  StepArgs a = step_args(s, write_mac, true);
+  hipError_t e = launch_step(s->q, s->storage, s->general, s->nt, fin, fout, a, s->bx, s->by, s->stream);
   if (e != hipSuccess) return hip_fail(e, "step kernel launch");
+  if (a.force_slot) {
+    ForceFinalizeArgs fa{s->force_partials, s->nslots, s->history, s->written, kHistoryCap};
+    e = launch_force_finalize(fa, s->stream);
+    if (e != hipSuccess) return hip_fail(e, "force finalize launch");
+  }
   s->parity ^= 1;
   s->step++;
   return LBM_OK;
@@ -210,7 +276,7 @@ int enqueue_step(lbm_solve* s, bool write_mac) {
 
 void drop_graphs(lbm_solve* s) {
   for (int p = 0; p < 2; p++) {
-    if (s->graph[p]) hipGraphExecDestroy(s->graph[p]);
+    if (s->graph[p]) (void)hipGraphExecDestroy(s->graph[p]);
     s->graph[p] = nullptr;
   }
 }
@@ -227,59 +293,65 @@ int build_graph(lbm_solve* s) {
   s->step = step0;
   s->parity = p;
   if (rc != LBM_OK) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
   e = hipGraphInstantiate(&s->graph[p], g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-  s->graph_bx = s->bx; s->graph_by = s->by; s->graph_swz = s->swz;
+  s->graph_bx = s->bx; s->graph_by = s->by; s->graph_swz = s->swz; s->graph_nt = s->nt;
   return LBM_OK;
 }
 
-// Times every candidate tile shape on the live lattices (fin -> fout without
-// swapping, so the state is unchanged) and keeps the fastest.
+// Times every candidate tile shape, XCD mapping and load/store flavour on the
+// live lattices (fin -> fout without swapping, so the state is unchanged) and
+// keeps the fastest.
 int autotune(lbm_solve* s) {
   TuneKey key{s->q, s->storage, s->general ? 1 : 0, s->d.nx, s->d.ny, s->d.nz};
+  TuneVal bestv{s->bx, s->by, s->swz, s->nt};
+  bool cached = false;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
-      s->bx = it->second.bx; s->by = it->second.by; s->swz = it->second.swz;
-      s->tuned = true;
-      return LBM_OK;
+      bestv = it->second;
+      cached = true;
     }
   }
-  const int reps = s->N >= (1u << 20) ? 3 : 8;
-  float best = 1e30f;
-  TuneVal bestv{s->bx, s->by, s->swz};
-  const void* fin = s->f[s->parity];
-  void* fout = s->f[1 - s->parity];
-  for (const auto& sh : kShapes) {
-    if (sh[0] > 64 && s->d.nx < sh[0] / 2) continue;
-    for (int swz = 0; swz < 2; swz++) {
-      s->bx = sh[0]; s->by = sh[1]; s->swz = swz;
-      StepArgs a = step_args(s, false);
-      HIPCHK(launch_step(s->q, s->storage, s->general, fin, fout, a, s->bx, s->by, s->stream));
-      HIPCHK(hipEventRecord(s->ev0, s->stream));
-      for (int r = 0; r < reps; r++)
-        HIPCHK(launch_step(s->q, s->storage, s->general, fin, fout, a, s->bx, s->by, s->stream));
-      HIPCHK(hipEventRecord(s->ev1, s->stream));
-      HIPCHK(hipEventSynchronize(s->ev1));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-      if (ms < best) { best = ms; bestv = TuneVal{sh[0], sh[1], swz}; }
+  if (!cached) {
+    const int reps = s->N >= (1u << 20) ? 3 : 8;
+    float best = 1e30f;
+    const void* fin = s->f[s->parity];
+    void* fout = s->f[1 - s->parity];
+    for (const auto& sh : kShapes) {
+      if (sh[0] > 64 && s->d.nx < sh[0] / 2) continue;
+      for (int swz = 0; swz < 2; swz++) {
+        for (int nt = 0; nt < 2; nt++) {
+          s->bx = sh[0]; s->by = sh[1]; s->swz = swz; s->nt = nt;
+          StepArgs a = step_args(s, false, false);
+          HIPCHK(launch_step(s->q, s->storage, s->general, nt, fin, fout, a, s->bx, s->by, s->stream));
+          HIPCHK(hipEventRecord(s->ev0, s->stream));
+          for (int r = 0; r < reps; r++)
+            HIPCHK(launch_step(s->q, s->storage, s->general, nt, fin, fout, a, s->bx, s->by, s->stream));
+          HIPCHK(hipEventRecord(s->ev1, s->stream));
+          HIPCHK(hipEventSynchronize(s->ev1));
+          float ms = 0.f;
+          HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+          if (ms < best) { best = ms; bestv = TuneVal{sh[0], sh[1], swz, nt}; }
+        }
+      }
     }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = bestv;
   }
-  s->bx = bestv.bx; s->by = bestv.by; s->swz = bestv.swz;
+  s->bx = bestv.bx; s->by = bestv.by; s->swz = bestv.swz; s->nt = bestv.nt;
   s->tuned = true;
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tune_cache[key] = bestv;
-  return LBM_OK;
+  return build_force_slots(s);
 }
 
-// Classification -> boundary-link list (ascending cell index) on the host.
+// GPU classification -> host list of fluid cells with a solid streaming
+// source, ascending.
 int build_boundary(lbm_solve* s) {
   const lbm_desc& d = s->d;
   ClassifyArgs c{};
@@ -292,51 +364,38 @@ int build_boundary(lbm_solve* s) {
   std::vector<uint8_t> h(s->N);
   HIPCHK(hipMemcpyAsync(h.data(), s->cls, s->N, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  static const int ex9[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1}, ey9[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
-  static const int ex3[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
-  static const int ey3[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1};
-  static const int ez3[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1};
-  std::vector<uint32_t> cells, masks;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
+  s->boundary.clear();
   for (int z = 0; z < nz; z++)
     for (int y = 0; y < ny; y++)
       for (int x = 0; x < nx; x++) {
         const uint32_t cell = (uint32_t)((z * ny + y) * nx + x);
         const uint8_t cc = h[cell];
         if (!(cc & 2) || (cc & 1)) continue;
-        uint32_t m = 0;
         for (int i = 1; i < s->q; i++) {
-          const int ex = s->q == 9 ? ex9[i] : ex3[i];
-          const int ey = s->q == 9 ? ey9[i] : ey3[i];
-          const int ez = s->q == 9 ? 0 : ez3[i];
+          const int ex = s->q == 9 ? kEx9[i] : kEx3[i];
+          const int ey = s->q == 9 ? kEy9[i] : kEy3[i];
+          const int ez = s->q == 9 ? 0 : kEz3[i];
           const int sx = (x - ex + nx) % nx, sy = (y - ey + ny) % ny, sz = (z - ez + nz) % nz;
-          if (h[(uint32_t)((sz * ny + sy) * nx + sx)] & 1) m |= 1u << i;
+          if (h[(uint32_t)((sz * ny + sy) * nx + sx)] & 1) {
+            s->boundary.push_back(cell);
+            break;
+          }
         }
-        if (m) { cells.push_back(cell); masks.push_back(m); }
       }
-  s->nboundary = (uint32_t)cells.size();
-  if (s->nboundary) {
-    int rc = dalloc(s, &s->bcells, cells.size() * 4);
-    if (rc) return rc;
-    rc = dalloc(s, &s->bmasks, masks.size() * 4);
-    if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(s->bcells, cells.data(), cells.size() * 4, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(s->bmasks, masks.data(), masks.size() * 4, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-  }
-  return LBM_OK;
+  return build_force_slots(s);
 }
 
 void release(lbm_solve* s) {
   if (!s) return;
   drop_graphs(s);
-  void* ptrs[] = {s->f[0], s->f[1], s->mac, s->cls, s->bcells, s->bmasks, s->scratch,
+  void* ptrs[] = {s->f[0], s->f[1], s->mac, s->cls, s->force_slot, s->force_partials, s->scratch,
                   s->counters, s->history, s->written, s->totals};
   for (void* p : ptrs)
-    if (p) hipFree(p);
-  if (s->ev0) hipEventDestroy(s->ev0);
-  if (s->ev1) hipEventDestroy(s->ev1);
-  if (s->stream) hipStreamDestroy(s->stream);
+    if (p) (void)hipFree(p);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
 
@@ -358,9 +417,7 @@ int lbm_device_info(char* name, size_t name_len, double* hbm_bytes, int32_t* cu_
   HIPCHK(hipGetDevice(&dev));
   hipDeviceProp_t p;
   HIPCHK(hipGetDeviceProperties(&p, dev));
-  if (name && name_len) {
-    snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
-  }
+  if (name && name_len) snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
   if (hbm_bytes) *hbm_bytes = (double)p.totalGlobalMem;
   if (cu_count) *cu_count = p.multiProcessorCount;
   return LBM_OK;
@@ -378,11 +435,11 @@ int lbm_solve_create(const lbm_desc* desc, lbm_solve** out) {
   s->storage = desc->storage;
   s->elem = desc->storage == LBM_STORE_F32 ? 4 : 2;
   s->N = (uint32_t)((size_t)desc->nx * desc->ny * desc->nz);
-  const bool body = desc->body.shape != LBM_BODY_NONE;
+  s->body = desc->body.shape != LBM_BODY_NONE;
   const bool walls = desc->walls[0] || desc->walls[1] || (desc->walls[2] && s->dim == 3);
-  s->general = body || walls || desc->sponge_width > 0.f;
+  s->general = s->body || walls || desc->sponge_width > 0.f;
   // default tile: one wave per x-row segment, XCD swizzle on
-  s->bx = 64; s->by = 1; s->swz = 1;
+  s->bx = 64; s->by = 1; s->swz = 1; s->nt = 0;
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { release(s); return hip_fail(e, "hipStreamCreate"); }
   if ((e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess) {
@@ -390,9 +447,9 @@ int lbm_solve_create(const lbm_desc* desc, lbm_solve** out) {
     return hip_fail(e, "hipEventCreate");
   }
   const size_t lattice_bytes = (size_t)s->N * s->q * s->elem;
-  const int nred = 1024;  // max reduction partial blocks
+  const int nred = 1024;  // max totals partial blocks
   if ((rc = dalloc(s, &s->f[0], lattice_bytes)) || (rc = dalloc(s, &s->f[1], lattice_bytes)) ||
-      (rc = dalloc(s, &s->mac, (size_t)s->N * 16)) || (rc = dalloc(s, &s->scratch, (size_t)nred * 8 * 8)) ||
+      (rc = dalloc(s, &s->mac, (size_t)s->N * 16)) || (rc = dalloc(s, &s->scratch, (size_t)nred * 4 * 8)) ||
       (rc = dalloc(s, &s->counters, 64)) || (rc = dalloc(s, &s->history, (size_t)kHistoryCap * 6 * 8)) ||
       (rc = dalloc(s, &s->written, 8)) || (rc = dalloc(s, &s->totals, 64))) {
     release(s);
@@ -403,20 +460,10 @@ int lbm_solve_create(const lbm_desc* desc, lbm_solve** out) {
     release(s);
     return hip_fail(e, "hipMemsetAsync");
   }
-  if (body) {
+  if (s->body) {
     if ((rc = dalloc(s, &s->cls, s->N)) || (rc = build_boundary(s))) {
       release(s);
       return rc;
-    }
-    // the force partials: one slot of 6 doubles per force block
-    const uint32_t fblocks = (s->nboundary + 255) / 256;
-    if (fblocks > (uint32_t)nred) {
-      hipFree(s->scratch);
-      s->scratch = nullptr;
-      if ((rc = dalloc(s, &s->scratch, (size_t)fblocks * 8 * 8))) {
-        release(s);
-        return rc;
-      }
     }
   }
   if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) {
@@ -428,7 +475,7 @@ int lbm_solve_create(const lbm_desc* desc, lbm_solve** out) {
 }
 
 int lbm_solve_destroy(lbm_solve* s) {
-  if (s) hipStreamSynchronize(s->stream);
+  if (s) (void)hipStreamSynchronize(s->stream);
   release(s);
   return LBM_OK;
 }
@@ -457,7 +504,7 @@ int lbm_upload_f(lbm_solve* s, const float* f) {
   if (e == hipSuccess) e = launch_pack(s->q, s->storage, tmp, s->f[0], s->N, s->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(s->f[1], s->f[0], (size_t)s->N * s->q * s->elem, hipMemcpyDeviceToDevice, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  hipFree(tmp);
+  (void)hipFree(tmp);
   if (e != hipSuccess) return hip_fail(e, "upload_f");
   s->parity = 0;
   s->step = 0;
@@ -473,7 +520,7 @@ int lbm_read_f(lbm_solve* s, float* f) {
   hipError_t e = launch_unpack(s->q, s->storage, s->f[s->parity], tmp, s->N, s->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(f, tmp, wire, hipMemcpyDeviceToHost, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  hipFree(tmp);
+  (void)hipFree(tmp);
   if (e != hipSuccess) return hip_fail(e, "read_f");
   return LBM_OK;
 }
@@ -487,7 +534,8 @@ int lbm_step(lbm_solve* s, int32_t n) {
     int rc = autotune(s);
     if (rc) return rc;
   }
-  if (s->graph_bx != s->bx || s->graph_by != s->by || s->graph_swz != s->swz) drop_graphs(s);
+  if (s->graph_bx != s->bx || s->graph_by != s->by || s->graph_swz != s->swz || s->graph_nt != s->nt)
+    drop_graphs(s);
   int left = n - 1;  // the last step writes the macroscopic field
   while (left >= kGraphChunk) {
     if (!s->graph[s->parity]) {
@@ -544,8 +592,8 @@ int lbm_read_forces(lbm_solve* s, double* out, int32_t max_records, int32_t* cou
   const unsigned long long avail = std::min<unsigned long long>(written, kHistoryCap);
   const unsigned long long take = std::min<unsigned long long>(avail, (unsigned long long)std::max(0, max_records));
   // the `take` most recent records, oldest first
-  std::vector<double> ring((size_t)kHistoryCap * 6);
   if (take) {
+    std::vector<double> ring((size_t)kHistoryCap * 6);
     HIPCHK(hipMemcpyAsync(ring.data(), s->history, ring.size() * 8, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     for (unsigned long long k = 0; k < take; k++) {
@@ -588,7 +636,7 @@ int lbm_solve_info(lbm_solve* s, int64_t* step, int32_t* block_x, int32_t* block
   if (step) *step = s->step;
   if (block_x) *block_x = s->bx;
   if (block_y) *block_y = s->by;
-  if (swizzle) *swizzle = s->swz;
+  if (swizzle) *swizzle = s->swz | (s->nt << 1);
   if (device_bytes) *device_bytes = (int64_t)s->bytes;
   return LBM_OK;
 }
@@ -602,10 +650,11 @@ int lbm_time_step_kernel(lbm_solve* s, int32_t n, double* ms_per_launch) {
   }
   const void* fin = s->f[s->parity];
   void* fout = s->f[1 - s->parity];
-  StepArgs a = step_args(s, false);
-  HIPCHK(launch_step(s->q, s->storage, s->general, fin, fout, a, s->bx, s->by, s->stream));
+  StepArgs a = step_args(s, false, true);
+  HIPCHK(launch_step(s->q, s->storage, s->general, s->nt, fin, fout, a, s->bx, s->by, s->stream));
   HIPCHK(hipEventRecord(s->ev0, s->stream));
-  for (int r = 0; r < n; r++) HIPCHK(launch_step(s->q, s->storage, s->general, fin, fout, a, s->bx, s->by, s->stream));
+  for (int r = 0; r < n; r++)
+    HIPCHK(launch_step(s->q, s->storage, s->general, s->nt, fin, fout, a, s->bx, s->by, s->stream));
   HIPCHK(hipEventRecord(s->ev1, s->stream));
   HIPCHK(hipEventSynchronize(s->ev1));
   float t = 0.f;
@@ -620,8 +669,40 @@ int lbm_set_tile(lbm_solve* s, int32_t block_x, int32_t block_y, int32_t swizzle
     return fail(LBM_E_INVALID, "tile %dx%d: need a multiple of 64 threads, at most 256", block_x, block_y);
   s->bx = block_x;
   s->by = block_y;
-  s->swz = swizzle ? 1 : 0;
+  s->swz = swizzle & 1;
+  s->nt = (swizzle >> 1) & 1;
   s->tuned = true;
+  return build_force_slots(s);
+}
+
+int lbm_probe_copy_bandwidth(size_t bytes, int32_t reps, double* gbs, double* ms_per_copy) {
+  if (bytes < 16 || reps < 1 || !gbs) return fail(LBM_E_INVALID, "bad argument");
+  bytes &= ~(size_t)15;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, bytes);
+  if (e == hipSuccess) e = hipMalloc(&b, bytes);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipMemsetAsync(a, 0, bytes, st);
+  if (e == hipSuccess) e = launch_copy_probe(a, b, bytes, st);
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_copy_probe(a, b, bytes, st);
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (e1) (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (st) (void)hipStreamDestroy(st);
+  if (b) (void)hipFree(b);
+  if (a) (void)hipFree(a);
+  if (e != hipSuccess) return hip_fail(e, "copy probe");
+  const double ms = (double)t / reps;
+  *gbs = 2.0 * (double)bytes / (ms * 1e-3) / 1e9;
+  if (ms_per_copy) *ms_per_copy = ms;
   return LBM_OK;
 }
 

@@ -22,6 +22,7 @@ EXPORTS = (
     "lbm_solve_destroy", "lbm_upload_macro", "lbm_upload_f", "lbm_read_f", "lbm_step",
     "lbm_step_timed", "lbm_sync", "lbm_read_macro", "lbm_read_forces", "lbm_read_totals",
     "lbm_read_classes", "lbm_solve_info", "lbm_set_tile", "lbm_time_step_kernel",
+    "lbm_probe_copy_bandwidth",
 )
 
 
@@ -75,6 +76,8 @@ def load():
                                        ctypes.POINTER(ctypes.c_int64)]
         lib.lbm_set_tile.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         lib.lbm_time_step_kernel.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+        lib.lbm_probe_copy_bandwidth.argtypes = [ctypes.c_size_t, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_double)]
         lib.lbm_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int32)]
         _lib = lib
@@ -123,6 +126,13 @@ def make_desc(d):
     desc.storage = STORE[d.get("storage", "f32")]
     desc.autotune = int(d.get("autotune", 0))
     return desc
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=20):
+    """(GB/s, ms per copy) of a plain device-to-device streaming copy."""
+    gbs, ms = ctypes.c_double(), ctypes.c_double()
+    _check(load().lbm_probe_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(gbs), ctypes.byref(ms)))
+    return gbs.value, ms.value
 
 
 def device_info():

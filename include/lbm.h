@@ -143,6 +143,13 @@ int lbm_set_tile(lbm_solve* s, int32_t block_x, int32_t block_y, int32_t swizzle
  * solve's stream. */
 int lbm_time_step_kernel(lbm_solve* s, int32_t n, double* ms_per_launch);
 
+/* Measurement hook, no counterpart in the reference: a 16-byte streaming
+ * copy of `bytes` between two fresh device buffers, repeated `reps` times;
+ * returns the achieved read+write bandwidth in GB/s (the practical HBM
+ * ceiling the roofline fraction is read against) and the mean time of one
+ * copy. */
+int lbm_probe_copy_bandwidth(size_t bytes, int32_t reps, double* gbs, double* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif
