@@ -36,15 +36,17 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_d3q19_sphere_256x128x128.json")
 
 
-def measure(storage, steps, warmup, kernel_reps, periodic=False):
+def measure(storage, steps, warmup, kernel_reps, periodic=False, tile=None):
     desc, mac, meta = sc.sphere(storage=storage)
     if periodic:
         # the same grid with no body, sponge or walls: the step kernel's
         # periodic fast path, for comparison only
         desc = {k: v for k, v in desc.items() if k not in ("body", "sponge")}
-    desc["autotune"] = 1
+    desc["autotune"] = 0 if tile else 1
     q = desc["q"]
     with pylbm.Solve(desc) as s:
+        if tile:
+            s.set_tile(*tile)
         s.upload_macro(mac)
         s.step(warmup)
         s.sync()
@@ -95,7 +97,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-16bit", action="store_true")
+    ap.add_argument("--tile", default=None,
+                    help="fixed tile bx,by,flags (flags bit0 XCD swizzle, bit1 non-temporal) instead of autotuning")
     args = ap.parse_args()
+    tile = tuple(int(v) for v in args.tile.split(",")) if args.tile else None
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the dense solve is single-device here "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -106,7 +111,7 @@ def main():
           file=sys.stderr, flush=True)
     copy_gbs, copy_ms = pylbm.copy_bandwidth(1 << 30, 20)
     print("copy probe: %.1f GB/s" % copy_gbs, file=sys.stderr, flush=True)
-    main_run = measure("f32", args.steps, args.warmup, 50)
+    main_run = measure("f32", args.steps, args.warmup, 50, tile=tile)
     print("fp32: %s" % json.dumps(main_run), file=sys.stderr, flush=True)
     extra = {}
     if not args.no_16bit:

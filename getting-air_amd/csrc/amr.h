@@ -1,0 +1,66 @@
+// Launch interface of the block-pool (AMR) kernels, amr_kernels.hip.
+//
+// Two levels: the dense L0 lattice and one pool of fine tiles.  L0 is cut
+// into blocks of RB^3 coarse cells; a refined block owns a slot holding
+// FB^3 fine cells, FB = 2 RB + 2 GHOST (interior [GHOST, GHOST + 2 RB) plus a
+// two-cell ring).  Pool planes are plane-major like the dense lattice:
+// g[i * pool_plane + (slot * FB + fz) * FB * FB + fy * FB + fx], with
+// pool_plane = max_slots * FB^3, holding the shifted populations g = f - w.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "collide.h"
+
+namespace lbm {
+
+constexpr int kGhost = 2;
+
+struct PoolGeom {
+  int32_t nx, ny, nz;        // L0 cells
+  int32_t rb, fb;            // block edge (L0 cells), tile edge (fine cells)
+  int32_t nbx, nby, nbz;     // blocks per axis
+  uint32_t ncells;           // L0 cells
+  uint32_t pool_plane;       // max_slots * fb^3
+  int32_t max_slots;
+  const int32_t* block_slot;     // per block: slot or -1
+  const int32_t* slot_to_block;  // per slot: block or -1
+};
+
+// Manager inputs and outputs (one workgroup; see k_manage).
+struct ManageArgs {
+  int32_t nblocks, max_slots;
+  const float* crit;         // per block criterion value
+  float refine_at;           // an unrefined block refines when crit >= refine_at
+  float keep_at;             // a refined block stays while crit >= keep_at
+  int32_t* block_slot;
+  int32_t* slot_to_block;
+  uint8_t* want;             // out: per block decision
+  int32_t* dying;            // out: [count, (block, slot) pairs ...] ascending block id
+  int32_t* fresh;            // out: [count, (block, slot) pairs ...] ascending block id
+  int32_t* active;           // out: [count, slot ...] ascending
+  int32_t* free_list;        // scratch, max_slots
+  int32_t* status;           // [0] wanted blocks left without a slot (sticky sum)
+};
+
+hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
+                            const CollideParams& fine, const int32_t* active, int nactive, hipStream_t s);
+hipError_t launch_explode(int q, const float* gcoarse, float* gpool, const PoolGeom& g, const int32_t* active,
+                          int nactive, int linear, hipStream_t s);
+hipError_t launch_coalesce(int q, const float* gpool, float* gcoarse, float* mac, const float* mac_pool,
+                           const PoolGeom& g, int orphans, hipStream_t s);
+hipError_t launch_interp(int q, const float* gcoarse, float* gpool, const PoolGeom& g, const int32_t* pairs,
+                         int npairs, float rescale_c2f, hipStream_t s);
+hipError_t launch_average(int q, const float* gpool, float* gcoarse, float* mac, const PoolGeom& g,
+                          const int32_t* pairs, int npairs, float rescale_f2c, hipStream_t s);
+hipError_t launch_criterion(const float* mac, float* crit, const PoolGeom& g, float lead, hipStream_t s);
+hipError_t launch_manage(const ManageArgs& a, hipStream_t s);
+// active = [count, slots with a block ...] ascending
+hipError_t launch_active_list(const int32_t* slot_to_block, int max_slots, int32_t* active, hipStream_t s);
+// pairs = [count, (block, slot) ...] of an active list (2 max_slots + 1 ints)
+hipError_t launch_active_pairs(const int32_t* active, const int32_t* slot_to_block, int max_slots, int32_t* pairs,
+                               hipStream_t s);
+hipError_t launch_pool_totals(int q, const float* gcoarse, const float* gpool, const PoolGeom& g, const int32_t* active,
+                              double* partials, unsigned* counter, double* out, hipStream_t s);
+
+}  // namespace lbm

@@ -1,0 +1,428 @@
+// Host side of the two-level AMR engine: the lbm_amr object behind the
+// lbm_amr_* entry points of include/lbm.h.  It owns a dense L0 solve (its
+// lattices, macroscopic field and stream) and the fine pool: two population
+// buffers (A holds the state between macro-steps), the planar pool
+// macroscopic field, the block maps and the manager's lists.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/lbm.h"
+#include "amr.h"
+#include "engine.h"
+#include "lattice_pairs.h"
+
+using namespace lbm;
+
+#define HIPCHK(call)                                 \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, #call); \
+  } while (0)
+
+struct lbm_amr {
+  lbm_amr_desc d{};
+  lbm_solve* l0 = nullptr;
+  hipStream_t stream = nullptr;
+  PoolGeom G{};
+  int fb = 0;
+  int nblocks = 0;
+  float* pool[2] = {nullptr, nullptr};
+  float* mac_pool = nullptr;
+  int32_t* block_slot = nullptr;
+  int32_t* slot_to_block = nullptr;
+  uint8_t* want = nullptr;
+  float* crit = nullptr;
+  int32_t* dying = nullptr;
+  int32_t* fresh = nullptr;
+  int32_t* active = nullptr;
+  int32_t* free_list = nullptr;
+  int32_t* status = nullptr;
+  int32_t* pairs = nullptr;
+  double* partials = nullptr;
+  unsigned* counter = nullptr;
+  double* totals = nullptr;
+  CollideParams fine{};
+  float c2f = 0.f, f2c = 0.f;
+  int nactive = 0;
+  int exhausted = 0;
+  long long step = 0;
+  bool seeded = false;
+};
+
+namespace {
+
+template <int Q>
+void fine_params(CollideParams& p, float omega, float omega_m, const float* F) {
+  using L = Lattice<Q>;
+  p.omega = omega;
+  p.omega_m = omega_m;
+  p.guo = 1.f - 0.5f * omega;
+  for (int k = 0; k < 3; k++) p.F[k] = F[k];
+  for (int k = 0; k < Pairs<Q>::NP; k++) {
+    const int i = Pairs<Q>::lo(k);
+    const float eF = L::ex[i] * F[0] + L::ey[i] * F[1] + L::ez[i] * F[2];
+    p.c1[k] = 9.f * (1.f - 0.5f * omega) * L::w[i] * eF;
+    p.so[k] = (1.f - 0.5f * omega_m) * 3.f * L::w[i] * eF;
+  }
+}
+
+int validate(const lbm_amr_desc* d) {
+  if (!d) return fail(LBM_E_INVALID, "null descriptor");
+  if (d->abi_version != LBM_ABI_VERSION) return fail(LBM_E_INVALID, "descriptor abi_version %d", d->abi_version);
+  if (d->q != 19 && d->q != 27) return fail(LBM_E_INVALID, "AMR needs q = 19 or 27 (got %d)", d->q);
+  if (d->rb < 2 || d->rb > 16) return fail(LBM_E_INVALID, "rb %d outside [2, 16]", d->rb);
+  if (d->nx < d->rb || d->ny < d->rb || d->nz < d->rb || d->nx % d->rb || d->ny % d->rb || d->nz % d->rb)
+    return fail(LBM_E_INVALID, "grid %dx%dx%d is not a multiple of rb=%d", d->nx, d->ny, d->nz, d->rb);
+  if (d->max_slots < 1) return fail(LBM_E_INVALID, "max_slots %d", d->max_slots);
+  const double tile = std::pow(2.0 * d->rb + 2 * kGhost, 3);
+  if ((double)d->max_slots * tile * 4.0 >= 4294967296.0)
+    return fail(LBM_E_INVALID, "pool plane of %d tiles exceeds 4 GiB", d->max_slots);
+  if (!(d->omega > 0.f && d->omega < 2.f)) return fail(LBM_E_INVALID, "omega %g outside (0, 2)", d->omega);
+  const double tau = 1.0 / d->omega;
+  if (std::fabs(tau - 1.0) < 1e-3)
+    return fail(LBM_E_REFUSED, "tau_c = %g: the fine/coarse non-equilibrium rescale is singular at tau = 1", tau);
+  if (d->collision == LBM_TRT && !(d->omega_minus > 0.f && d->omega_minus < 2.f))
+    return fail(LBM_E_INVALID, "omega_minus %g outside (0, 2)", d->omega_minus);
+  if (d->manage_every < 0 || d->lead < 0) return fail(LBM_E_INVALID, "manage_every / lead must be >= 0");
+  return LBM_OK;
+}
+
+void release(lbm_amr* a) {
+  if (!a) return;
+  void* ptrs[] = {a->pool[0], a->pool[1], a->mac_pool, a->block_slot, a->slot_to_block, a->want, a->crit,
+                  a->dying, a->fresh, a->active, a->free_list, a->status, a->pairs, a->partials, a->counter, a->totals};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (a->l0) lbm_solve_destroy(a->l0);
+  delete a;
+}
+
+template <class T>
+int dalloc(T** p, size_t bytes) {
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(LBM_E_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+  }
+  return LBM_OK;
+}
+
+// Fills the tiles listed in `pairs` ([count, (block, slot)...], device) from
+// the current L0 state.
+int fill_tiles(lbm_amr* a, const int32_t* pairs, int n) {
+  HIPCHK(launch_interp(a->d.q, solve_lattice(a->l0, 0), a->pool[0], a->G, pairs, n, a->c2f, a->stream));
+  return LBM_OK;
+}
+
+int refresh_active(lbm_amr* a) {
+  HIPCHK(launch_active_list(a->slot_to_block, a->d.max_slots, a->active, a->stream));
+  int32_t n = 0;
+  HIPCHK(hipMemcpyAsync(&n, a->active, 4, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  a->nactive = n;
+  return LBM_OK;
+}
+
+// Restricts every tile in use onto the L0 cells under it (the average/drain
+// pass).  Between manager passes the covered L0 step leaves those cells
+// untouched, so this is what makes them a valid interpolation source for a
+// neighbouring fresh tile, and it drains the tiles about to die.
+int restrict_active(lbm_amr* a) {
+  if (a->nactive <= 0) return LBM_OK;
+  HIPCHK(launch_active_pairs(a->active, a->slot_to_block, a->d.max_slots, a->pairs, a->stream));
+  HIPCHK(launch_average(a->d.q, a->pool[0], solve_lattice(a->l0, 0), solve_mac(a->l0), a->G, a->pairs, a->nactive,
+                        a->f2c, a->stream));
+  return LBM_OK;
+}
+
+int manage(lbm_amr* a) {
+  const lbm_amr_desc& d = a->d;
+  HIPCHK(launch_criterion(solve_mac(a->l0), a->crit, a->G, (float)d.lead, a->stream));
+  int rc = restrict_active(a);
+  if (rc) return rc;
+  ManageArgs m{};
+  m.nblocks = a->nblocks;
+  m.max_slots = d.max_slots;
+  m.crit = a->crit;
+  m.refine_at = d.refine_at;
+  m.keep_at = d.keep_at;
+  m.block_slot = a->block_slot;
+  m.slot_to_block = a->slot_to_block;
+  m.want = a->want;
+  m.dying = a->dying;
+  m.fresh = a->fresh;
+  m.active = a->active;
+  m.free_list = a->free_list;
+  m.status = a->status;
+  HIPCHK(launch_manage(m, a->stream));
+  int32_t counts[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(&counts[0], a->dying, 4, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipMemcpyAsync(&counts[1], a->fresh, 4, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipMemcpyAsync(&counts[2], a->active, 4, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipMemcpyAsync(&counts[3], a->status, 4, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  // dying tiles were drained by restrict_active; fill the new ones
+  rc = fill_tiles(a, a->fresh, counts[1]);
+  if (rc) return rc;
+  a->nactive = counts[2];
+  a->exhausted = counts[3];
+  if (counts[3] > 0)
+    return fail(LBM_E_NOMEM, "AMR pool exhausted: %d wanted tiles found no free slot (max_slots %d)", counts[3],
+                d.max_slots);
+  return LBM_OK;
+}
+
+int macro_step(lbm_amr* a) {
+  const lbm_amr_desc& d = a->d;
+  if (d.manage_every > 0 && a->step % d.manage_every == 0) {
+    int rc = manage(a);
+    if (rc) return rc;
+  }
+  float* l0 = solve_lattice(a->l0, 0);
+  HIPCHK(launch_explode(d.q, l0, a->pool[0], a->G, a->active, a->nactive, d.explode_linear, a->stream));
+  HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], a->mac_pool, a->G, a->fine, a->active, a->nactive, a->stream));
+  HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->active, a->nactive, a->stream));
+  HIPCHK(launch_coalesce(d.q, a->pool[0], l0, solve_mac(a->l0), a->mac_pool, a->G, d.orphans, a->stream));
+  int rc = solve_enqueue_covered_step(a->l0, a->block_slot, d.rb, a->G.nbx, a->G.nby);
+  if (rc) return rc;
+  a->step++;
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
+  if (!out) return fail(LBM_E_INVALID, "null output handle");
+  *out = nullptr;
+  int rc = validate(desc);
+  if (rc) return rc;
+  lbm_amr* a = new lbm_amr();
+  a->d = *desc;
+  lbm_desc l0{};
+  l0.abi_version = LBM_ABI_VERSION;
+  l0.q = desc->q;
+  l0.nx = desc->nx; l0.ny = desc->ny; l0.nz = desc->nz;
+  l0.collision = desc->collision;
+  l0.omega = desc->omega;
+  l0.omega_minus = desc->omega_minus;
+  for (int k = 0; k < 3; k++) l0.force[k] = desc->force[k];
+  l0.storage = LBM_STORE_F32;
+  if ((rc = lbm_solve_create(&l0, &a->l0))) {
+    release(a);
+    return rc;
+  }
+  a->stream = solve_stream(a->l0);
+  const int rb = desc->rb;
+  a->fb = 2 * rb + 2 * kGhost;
+  PoolGeom& G = a->G;
+  G.nx = desc->nx; G.ny = desc->ny; G.nz = desc->nz;
+  G.rb = rb; G.fb = a->fb;
+  G.nbx = desc->nx / rb; G.nby = desc->ny / rb; G.nbz = desc->nz / rb;
+  G.ncells = (uint32_t)((size_t)desc->nx * desc->ny * desc->nz);
+  G.max_slots = desc->max_slots;
+  G.pool_plane = (uint32_t)((size_t)desc->max_slots * a->fb * a->fb * a->fb);
+  a->nblocks = G.nbx * G.nby * G.nbz;
+  const size_t pool_bytes = (size_t)G.pool_plane * desc->q * 4;
+  if ((rc = dalloc(&a->pool[0], pool_bytes)) || (rc = dalloc(&a->pool[1], pool_bytes)) ||
+      (rc = dalloc(&a->mac_pool, (size_t)G.pool_plane * 16)) || (rc = dalloc(&a->block_slot, (size_t)a->nblocks * 4)) ||
+      (rc = dalloc(&a->slot_to_block, (size_t)desc->max_slots * 4)) || (rc = dalloc(&a->want, (size_t)a->nblocks)) ||
+      (rc = dalloc(&a->crit, (size_t)a->nblocks * 4)) ||
+      (rc = dalloc(&a->dying, (size_t)(2 * a->nblocks + 1) * 4)) ||
+      (rc = dalloc(&a->fresh, (size_t)(2 * a->nblocks + 1) * 4)) ||
+      (rc = dalloc(&a->active, (size_t)(desc->max_slots + 1) * 4)) ||
+      (rc = dalloc(&a->free_list, (size_t)desc->max_slots * 4)) || (rc = dalloc(&a->status, 16)) ||
+      (rc = dalloc(&a->pairs, (size_t)(2 * desc->max_slots + 1) * 4)) ||
+      (rc = dalloc(&a->partials, 256 * 4 * 8)) || (rc = dalloc(&a->counter, 16)) || (rc = dalloc(&a->totals, 32))) {
+    release(a);
+    return rc;
+  }
+  G.block_slot = a->block_slot;
+  G.slot_to_block = a->slot_to_block;
+  hipError_t e = hipMemsetAsync(a->block_slot, 0xff, (size_t)a->nblocks * 4, a->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(a->slot_to_block, 0xff, (size_t)desc->max_slots * 4, a->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(a->want, 0, (size_t)a->nblocks, a->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(a->crit, 0, (size_t)a->nblocks * 4, a->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(a->active, 0, 4, a->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(a->status, 0, 16, a->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(a->counter, 0, 16, a->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(a->mac_pool, 0, (size_t)G.pool_plane * 16, a->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(a->stream);
+  if (e != hipSuccess) {
+    release(a);
+    return hip_fail(e, "AMR init");
+  }
+  // per-level relaxation: tau_f = 2 tau_c - 1/2; TRT keeps Lambda
+  const double tc = 1.0 / desc->omega, tf = 2.0 * tc - 0.5;
+  double tfm = tf;
+  if (desc->collision == LBM_TRT) {
+    const double lambda = (tc - 0.5) * (1.0 / desc->omega_minus - 0.5);
+    tfm = 0.5 + lambda / (tf - 0.5);
+  }
+  const float Ff[3] = {0.5f * desc->force[0], 0.5f * desc->force[1], 0.5f * desc->force[2]};
+  if (desc->q == 19) fine_params<19>(a->fine, (float)(1.0 / tf), (float)(1.0 / tfm), Ff);
+  else fine_params<27>(a->fine, (float)(1.0 / tf), (float)(1.0 / tfm), Ff);
+  // post-collision non-equilibrium rescale (common_d3_pool.wgsl:172-187)
+  a->c2f = (float)(0.5 * (tf - 1.0) / (tc - 1.0));
+  a->f2c = (float)(2.0 * (tc - 1.0) / (tf - 1.0));
+  *out = a;
+  return LBM_OK;
+}
+
+int lbm_amr_destroy(lbm_amr* a) {
+  if (a) (void)hipStreamSynchronize(a->stream);
+  release(a);
+  return LBM_OK;
+}
+
+int lbm_amr_upload_topology(lbm_amr* a, const int32_t* block_slot, const int32_t* slot_to_block) {
+  if (!a || !block_slot || !slot_to_block) return fail(LBM_E_INVALID, "null argument");
+  // the two maps must be inverse to each other
+  for (int b = 0; b < a->nblocks; b++) {
+    const int s = block_slot[b];
+    if (s >= a->d.max_slots || (s >= 0 && slot_to_block[s] != b))
+      return fail(LBM_E_INVALID, "blockSlot[%d] = %d is not inverted by slotToBlock", b, s);
+  }
+  for (int s = 0; s < a->d.max_slots; s++) {
+    const int b = slot_to_block[s];
+    if (b >= a->nblocks || (b >= 0 && block_slot[b] != s))
+      return fail(LBM_E_INVALID, "slotToBlock[%d] = %d is not inverted by blockSlot", s, b);
+  }
+  if (a->seeded) {
+    int rc0 = restrict_active(a);
+    if (rc0) return rc0;
+  }
+  HIPCHK(hipMemcpyAsync(a->block_slot, block_slot, (size_t)a->nblocks * 4, hipMemcpyHostToDevice, a->stream));
+  HIPCHK(hipMemcpyAsync(a->slot_to_block, slot_to_block, (size_t)a->d.max_slots * 4, hipMemcpyHostToDevice, a->stream));
+  int rc = refresh_active(a);
+  if (rc) return rc;
+  if (a->seeded) {
+    // fill every tile of the new topology from the current L0 state
+    std::vector<int32_t> pairs(1 + 2 * a->nactive);
+    pairs[0] = a->nactive;
+    int k = 0;
+    for (int s = 0; s < a->d.max_slots; s++)
+      if (slot_to_block[s] >= 0) { pairs[1 + 2 * k] = slot_to_block[s]; pairs[2 + 2 * k] = s; k++; }
+    HIPCHK(hipMemcpyAsync(a->fresh, pairs.data(), pairs.size() * 4, hipMemcpyHostToDevice, a->stream));
+    rc = fill_tiles(a, a->fresh, a->nactive);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(a->stream));
+  }
+  return LBM_OK;
+}
+
+int lbm_amr_read_topology(lbm_amr* a, int32_t* block_slot, int32_t* slot_to_block) {
+  if (!a) return fail(LBM_E_INVALID, "null argument");
+  if (block_slot) HIPCHK(hipMemcpyAsync(block_slot, a->block_slot, (size_t)a->nblocks * 4, hipMemcpyDeviceToHost, a->stream));
+  if (slot_to_block)
+    HIPCHK(hipMemcpyAsync(slot_to_block, a->slot_to_block, (size_t)a->d.max_slots * 4, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  return LBM_OK;
+}
+
+int lbm_amr_upload_macro(lbm_amr* a, const float* mac4) {
+  if (!a || !mac4) return fail(LBM_E_INVALID, "null argument");
+  int rc = lbm_upload_macro(a->l0, mac4);
+  if (rc) return rc;
+  a->seeded = true;
+  a->step = 0;
+  // fill every tile of the current topology from the seeded L0 field
+  std::vector<int32_t> s2b(a->d.max_slots);
+  HIPCHK(hipMemcpy(s2b.data(), a->slot_to_block, s2b.size() * 4, hipMemcpyDeviceToHost));
+  std::vector<int32_t> pairs(1);
+  for (int s = 0; s < a->d.max_slots; s++)
+    if (s2b[s] >= 0) { pairs.push_back(s2b[s]); pairs.push_back(s); }
+  const int n = (int)(pairs.size() - 1) / 2;
+  pairs[0] = n;
+  HIPCHK(hipMemcpyAsync(a->fresh, pairs.data(), pairs.size() * 4, hipMemcpyHostToDevice, a->stream));
+  rc = fill_tiles(a, a->fresh, n);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(a->stream));
+  return refresh_active(a);
+}
+
+int lbm_amr_step(lbm_amr* a, int32_t n) {
+  if (!a) return fail(LBM_E_INVALID, "null argument");
+  if (!a->seeded) return fail(LBM_E_STATE, "step before upload_macro");
+  for (int s = 0; s < n; s++) {
+    int rc = macro_step(a);
+    if (rc) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(a->stream));
+  return LBM_OK;
+}
+
+int lbm_amr_manage_now(lbm_amr* a) {
+  if (!a) return fail(LBM_E_INVALID, "null argument");
+  if (!a->seeded) return fail(LBM_E_STATE, "manage before upload_macro");
+  int rc = manage(a);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(a->stream));
+  return LBM_OK;
+}
+
+int lbm_amr_read_macro(lbm_amr* a, float* mac4) {
+  if (!a) return fail(LBM_E_INVALID, "null argument");
+  return lbm_read_macro(a->l0, mac4);
+}
+
+int lbm_amr_read_pool_macro(lbm_amr* a, float* mac_pool) {
+  if (!a || !mac_pool) return fail(LBM_E_INVALID, "null argument");
+  HIPCHK(hipMemcpy(mac_pool, a->mac_pool, (size_t)a->G.pool_plane * 16, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
+int lbm_amr_read_f(lbm_amr* a, float* f_l0, float* f_pool) {
+  if (!a) return fail(LBM_E_INVALID, "null argument");
+  if (f_l0) {
+    int rc = lbm_read_f(a->l0, f_l0);
+    if (rc) return rc;
+  }
+  if (f_pool) {
+    const size_t n = (size_t)a->G.pool_plane;
+    HIPCHK(hipMemcpy(f_pool, a->pool[0], n * a->d.q * 4, hipMemcpyDeviceToHost));
+    for (int i = 0; i < a->d.q; i++) {
+      const float w = a->d.q == 19 ? Lattice<19>::w[i] : Lattice<27>::w[i];
+      float* p = f_pool + (size_t)i * n;
+      for (size_t c = 0; c < n; c++) p[c] += w;
+    }
+  }
+  return LBM_OK;
+}
+
+int lbm_amr_read_criterion(lbm_amr* a, float* crit, uint8_t* want) {
+  if (!a) return fail(LBM_E_INVALID, "null argument");
+  if (crit) HIPCHK(hipMemcpy(crit, a->crit, (size_t)a->nblocks * 4, hipMemcpyDeviceToHost));
+  if (want) HIPCHK(hipMemcpy(want, a->want, (size_t)a->nblocks, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
+int lbm_amr_read_active(lbm_amr* a, int32_t* list) {
+  if (!a || !list) return fail(LBM_E_INVALID, "null argument");
+  HIPCHK(hipMemcpy(list, a->active, (size_t)(a->nactive + 1) * 4, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
+int lbm_amr_read_totals(lbm_amr* a, double* out4) {
+  if (!a || !out4) return fail(LBM_E_INVALID, "null argument");
+  HIPCHK(launch_pool_totals(a->d.q, solve_lattice(a->l0, 0), a->pool[0], a->G, a->active, a->partials, a->counter,
+                            a->totals, a->stream));
+  HIPCHK(hipMemcpyAsync(out4, a->totals, 32, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  return LBM_OK;
+}
+
+int lbm_amr_info(lbm_amr* a, int64_t* step, int32_t* active_tiles, int32_t* exhausted) {
+  if (!a) return fail(LBM_E_INVALID, "null argument");
+  if (step) *step = a->step;
+  if (active_tiles) *active_tiles = a->nactive;
+  if (exhausted) *exhausted = a->exhausted;
+  return LBM_OK;
+}
+
+}  // extern "C"

@@ -33,10 +33,24 @@ struct StepArgs {
   int32_t solid_eq;
   int32_t solid_mac_rest;  // 3D: a solid cell reports [1, 0, 0, 0]
   float* mac;          // [rho, ux, uy, uz] per cell, or null (not observed)
+  // Block refinement map of an AMR hierarchy: a cell whose block holds a
+  // slot is solved by the finer level and skipped here
+  // (coveredByFiner, common_d3_step.wgsl:104-114).  Null without AMR.
+  const int32_t* cover;
+  int32_t cover_rb, cover_nbx, cover_nby;
   // Momentum-exchange force, fused into the step: tiles holding a fluid cell
   // with a solid streaming source own a slot of 6 FP64 partials.
   const int32_t* force_slot;  // per logical tile: slot index or -1; null: no force
-  double* force_partials;     // [slot][6]
+  double* force_partials;     // [slot][6] of this step (lattice parity buffer)
+  // The previous step's partials are summed by the first wave of block 0 of
+  // this launch (in the same fixed order as k_force_finalize), so a run of
+  // steps needs no separate reduction launch per step.
+  int32_t finalize_prev;
+  const double* prev_partials;
+  uint32_t nslots;
+  double* history;
+  unsigned long long* written;
+  uint32_t hist_cap;
   double body_c[3];           // torque arm origin
   int32_t periodic_arm;       // 2D: nearest periodic image for the arm
   // tiling of the 1-D grid onto (x-tiles, y-tiles, z)

@@ -29,10 +29,35 @@
 
 namespace lbm {
 
+// One wave (64 lanes, all active): lane l sums slots l, l+64, ... of `p` in
+// order, a fixed butterfly finishes, lane 0 appends the record.  Used by
+// k_force_finalize and, for the previous step, inside k_step.
+__device__ __forceinline__ void sum_force_slots(const double* __restrict__ p, uint32_t nslots, double* history,
+                                                unsigned long long* written, uint32_t cap, int lane) {
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (uint32_t s = lane; s < nslots; s += 64) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] += p[(size_t)s * 6 + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+    const unsigned long long n = *written;
+#pragma unroll
+    for (int k = 0; k < 6; k++) history[(size_t)(n % cap) * 6 + k] = v[k];
+    *written = n + 1;
+  }
+}
+
 template <int Q, class ST, bool GEN, bool NT>
 __global__ void __launch_bounds__(256)
 k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout, StepArgs a) {
   using L = Lattice<Q>;
+  if constexpr (GEN) {
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if (a.finalize_prev && blockIdx.x == 0 && tid < 64)
+      sum_force_slots(a.prev_partials, a.nslots, a.history, a.written, a.hist_cap, tid);
+  }
   uint32_t b = blockIdx.x;
   if (a.swizzle) {
     // Hardware hands block b to XCD b % 8.  Re-number so each XCD walks one
@@ -47,7 +72,11 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
   const int x = tx * blockDim.x + threadIdx.x;
   const int y = ty * blockDim.y + threadIdx.y;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const bool active = x < nx && y < ny;
+  bool active = x < nx && y < ny;
+  if constexpr (GEN) {
+    if (active && a.cover)
+      active = a.cover[((z / a.cover_rb) * a.cover_nby + y / a.cover_rb) * a.cover_nbx + x / a.cover_rb] < 0;
+  }
   const uint32_t N = a.ncells;
   double fv[6] = {0, 0, 0, 0, 0, 0};  // this cell's force/torque on the body
 
@@ -255,19 +284,7 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
 
 // One wave: lane l sums slots l, l+64, ... in order, then a fixed butterfly.
 __global__ void __launch_bounds__(64) k_force_finalize(ForceFinalizeArgs a) {
-  double v[6] = {0, 0, 0, 0, 0, 0};
-  for (uint32_t s = threadIdx.x; s < a.nslots; s += 64) {
-#pragma unroll
-    for (int k = 0; k < 6; k++) v[k] += a.partials[(size_t)s * 6 + k];
-  }
-#pragma unroll
-  for (int k = 0; k < 6; k++) v[k] = wave_sum(v[k]);
-  if (threadIdx.x == 0) {
-    const unsigned long long n = *a.written;
-#pragma unroll
-    for (int k = 0; k < 6; k++) a.history[(size_t)(n % a.capacity) * 6 + k] = v[k];
-    *a.written = n + 1;
-  }
+  sum_force_slots(a.partials, a.nslots, a.history, a.written, a.capacity, threadIdx.x);
 }
 
 template <int Q, class ST>
@@ -361,10 +378,12 @@ __global__ void __launch_bounds__(256) k_classify(ClassifyArgs a) {
   a.cls[cell] = c;
 }
 
-// 16-byte streaming copy, the bandwidth reference for the roofline.
-__global__ void __launch_bounds__(256) k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+// 16-byte streaming copy, one element per thread with non-temporal loads and
+// stores: the practical HBM ceiling the roofline fraction is read against.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_copy_probe(const u32x4* __restrict__ src, u32x4* __restrict__ dst, size_t n) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 // ---------------------------------------------------------------------------
@@ -459,7 +478,7 @@ hipError_t launch_classify(const ClassifyArgs& a, hipStream_t s) {
 }
 hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s) {
   const size_t n = bytes / 16;
-  hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
+  hipLaunchKernelGGL(k_copy_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const u32x4*)src, (u32x4*)dst, n);
   return hipGetLastError();
 }
 

@@ -109,6 +109,7 @@ struct lbm_solve {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int bx = 64, by = 1, swz = 1, nt = 0;
+  bool force_pending = false;       // last step's force partials not yet summed
   bool tuned = false;
   bool initialised = false;
   long long step = 0;
@@ -205,8 +206,15 @@ StepArgs step_args(lbm_solve* s, bool write_mac, bool with_force) {
   a.solid_mac_rest = s->dim == 3;
   a.mac = write_mac ? s->mac : nullptr;
   if (with_force && s->body && s->nslots) {
+    // partial buffers alternate with the lattice parity
+    const size_t stride = (size_t)s->nslots * 6;
     a.force_slot = s->force_slot;
-    a.force_partials = s->force_partials;
+    a.force_partials = s->force_partials + stride * s->parity;
+    a.prev_partials = s->force_partials + stride * (1 - s->parity);
+    a.nslots = s->nslots;
+    a.history = s->history;
+    a.written = s->written;
+    a.hist_cap = kHistoryCap;
   }
   a.body_c[0] = d.body.centre[0];
   a.body_c[1] = d.body.centre[1];
@@ -244,7 +252,7 @@ int build_force_slots(lbm_solve* s) {
   s->force_slot = nullptr;
   s->force_partials = nullptr;
   s->slot_bytes = ntiles * 4;
-  s->partial_bytes = std::max<size_t>(n, 1) * 6 * 8;
+  s->partial_bytes = 2 * std::max<size_t>(n, 1) * 6 * 8;  // one set per lattice parity
   int rc = dalloc(s, &s->force_slot, s->slot_bytes);
   if (rc) return rc;
   rc = dalloc(s, &s->force_partials, s->partial_bytes);
@@ -257,22 +265,61 @@ int build_force_slots(lbm_solve* s) {
   return LBM_OK;
 }
 
-// One step: fused stream-collide (+ force partials), then the force sum.
-int enqueue_step(lbm_solve* s, bool write_mac) {
+// One step: fused stream-collide with this step's force partials.  With
+// `finalize_prev` the same launch also appends the previous step's force
+// record; flush_force appends the last one.
+int enqueue_step(lbm_solve* s, bool write_mac, bool finalize_prev) {
   const void* fin = s->f[s->parity];
   void* fout = s->f[1 - s->parity];
   StepArgs a = step_args(s, write_mac, true);
+  a.finalize_prev = finalize_prev && a.force_slot ? 1 : 0;
   hipError_t e = launch_step(s->q, s->storage, s->general, s->nt, fin, fout, a, s->bx, s->by, s->stream);
   if (e != hipSuccess) return hip_fail(e, "step kernel launch");
-  if (a.force_slot) {
-    ForceFinalizeArgs fa{s->force_partials, s->nslots, s->history, s->written, kHistoryCap};
-    e = launch_force_finalize(fa, s->stream);
-    if (e != hipSuccess) return hip_fail(e, "force finalize launch");
-  }
+  s->force_pending = a.force_slot != nullptr;
   s->parity ^= 1;
   s->step++;
   return LBM_OK;
 }
+
+// Appends the force record of the last enqueued step.
+int flush_force(lbm_solve* s) {
+  if (!s->force_pending) return LBM_OK;
+  ForceFinalizeArgs fa{s->force_partials + (size_t)s->nslots * 6 * (1 - s->parity), s->nslots, s->history,
+                       s->written, kHistoryCap};
+  hipError_t e = launch_force_finalize(fa, s->stream);
+  if (e != hipSuccess) return hip_fail(e, "force finalize launch");
+  s->force_pending = false;
+  return LBM_OK;
+}
+
+}  // namespace
+
+namespace lbm {
+
+int solve_enqueue_covered_step(lbm_solve* s, const int32_t* cover, int rb, int nbx, int nby) {
+  const void* fin = s->f[s->parity];
+  void* fout = s->f[1 - s->parity];
+  StepArgs a = step_args(s, true, false);
+  a.cover = cover;
+  a.cover_rb = rb;
+  a.cover_nbx = nbx;
+  a.cover_nby = nby;
+  hipError_t e = launch_step(s->q, s->storage, true, s->nt, fin, fout, a, s->bx, s->by, s->stream);
+  if (e != hipSuccess) return hip_fail(e, "covered step launch");
+  s->parity ^= 1;
+  s->step++;
+  return LBM_OK;
+}
+
+float* solve_lattice(lbm_solve* s, int which) {
+  return static_cast<float*>(s->f[which ? 1 - s->parity : s->parity]);
+}
+float* solve_mac(lbm_solve* s) { return s->mac; }
+hipStream_t solve_stream(lbm_solve* s) { return s->stream; }
+
+}  // namespace lbm
+
+namespace {
 
 void drop_graphs(lbm_solve* s) {
   for (int p = 0; p < 2; p++) {
@@ -287,11 +334,13 @@ int build_graph(lbm_solve* s) {
   hipGraph_t g = nullptr;
   HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   const long long step0 = s->step;
+  const bool pending0 = s->force_pending;
   int rc = LBM_OK;
-  for (int k = 0; k < kGraphChunk && rc == LBM_OK; k++) rc = enqueue_step(s, false);
+  for (int k = 0; k < kGraphChunk && rc == LBM_OK; k++) rc = enqueue_step(s, false, true);
   hipError_t e = hipStreamEndCapture(s->stream, &g);
   s->step = step0;
   s->parity = p;
+  s->force_pending = pending0;
   if (rc != LBM_OK) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
@@ -400,6 +449,7 @@ void release(lbm_solve* s) {
 }
 
 int reset_history(lbm_solve* s) {
+  s->force_pending = false;
   HIPCHK(hipMemsetAsync(s->written, 0, sizeof(unsigned long long), s->stream));
   return LBM_OK;
 }
@@ -536,22 +586,32 @@ int lbm_step(lbm_solve* s, int32_t n) {
   }
   if (s->graph_bx != s->bx || s->graph_by != s->by || s->graph_swz != s->swz || s->graph_nt != s->nt)
     drop_graphs(s);
-  int left = n - 1;  // the last step writes the macroscopic field
+  // First step eagerly (nothing earlier in this call to finalize), then
+  // captured chunks and single steps that each finalize their predecessor's
+  // force, the last step writing the macroscopic field, then the last force.
+  int rc = enqueue_step(s, n == 1, false);
+  if (rc) return rc;
+  int left = n - 2;  // steps between the first and the last
   while (left >= kGraphChunk) {
     if (!s->graph[s->parity]) {
-      int rc = build_graph(s);
+      rc = build_graph(s);
       if (rc) return rc;
     }
     HIPCHK(hipGraphLaunch(s->graph[s->parity], s->stream));
     // kGraphChunk is even, so the parity after the chunk is unchanged
     s->step += kGraphChunk;
+    s->force_pending = s->body && s->nslots;
     left -= kGraphChunk;
   }
   while (left-- > 0) {
-    int rc = enqueue_step(s, false);
+    rc = enqueue_step(s, false, true);
     if (rc) return rc;
   }
-  return enqueue_step(s, true);
+  if (n >= 2) {
+    rc = enqueue_step(s, true, true);
+    if (rc) return rc;
+  }
+  return flush_force(s);
 }
 
 int lbm_step_timed(lbm_solve* s, int32_t n, double* ms) {

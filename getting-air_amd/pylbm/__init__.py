@@ -23,7 +23,12 @@ EXPORTS = (
     "lbm_step_timed", "lbm_sync", "lbm_read_macro", "lbm_read_forces", "lbm_read_totals",
     "lbm_read_classes", "lbm_solve_info", "lbm_set_tile", "lbm_time_step_kernel",
     "lbm_probe_copy_bandwidth",
+    "lbm_amr_create", "lbm_amr_destroy", "lbm_amr_upload_topology", "lbm_amr_read_topology",
+    "lbm_amr_upload_macro", "lbm_amr_step", "lbm_amr_manage_now", "lbm_amr_read_macro",
+    "lbm_amr_read_pool_macro", "lbm_amr_read_f", "lbm_amr_read_criterion", "lbm_amr_read_active",
+    "lbm_amr_read_totals", "lbm_amr_info",
 )
+GHOST = 2
 
 
 class Body(ctypes.Structure):
@@ -38,6 +43,17 @@ class Desc(ctypes.Structure):
         ("force", ctypes.c_float * 3), ("walls", ctypes.c_int32 * 3), ("wall_u", ctypes.c_float * 2),
         ("sponge_width", ctypes.c_float), ("sponge_axes", ctypes.c_int32 * 3), ("sponge_u", ctypes.c_float * 3),
         ("body", Body), ("solid_eq", ctypes.c_int32), ("storage", ctypes.c_int32), ("autotune", ctypes.c_int32),
+    ]
+
+
+class AmrDesc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32), ("q", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("rb", ctypes.c_int32), ("max_slots", ctypes.c_int32), ("collision", ctypes.c_int32),
+        ("omega", ctypes.c_float), ("omega_minus", ctypes.c_float), ("force", ctypes.c_float * 3),
+        ("manage_every", ctypes.c_int32), ("refine_at", ctypes.c_float), ("keep_at", ctypes.c_float),
+        ("lead", ctypes.c_int32), ("explode_linear", ctypes.c_int32), ("orphans", ctypes.c_int32),
     ]
 
 
@@ -80,6 +96,22 @@ def load():
                                                  ctypes.POINTER(ctypes.c_double)]
         lib.lbm_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int32)]
+        A = ctypes.POINTER(AmrDesc)
+        i32p, i64p = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)
+        lib.lbm_amr_create.argtypes = [A, ctypes.POINTER(P)]
+        lib.lbm_amr_destroy.argtypes = [P]
+        lib.lbm_amr_upload_topology.argtypes = [P, P, P]
+        lib.lbm_amr_read_topology.argtypes = [P, P, P]
+        lib.lbm_amr_upload_macro.argtypes = [P, P]
+        lib.lbm_amr_step.argtypes = [P, ctypes.c_int32]
+        lib.lbm_amr_manage_now.argtypes = [P]
+        lib.lbm_amr_read_macro.argtypes = [P, P]
+        lib.lbm_amr_read_pool_macro.argtypes = [P, P]
+        lib.lbm_amr_read_f.argtypes = [P, P, P]
+        lib.lbm_amr_read_criterion.argtypes = [P, P, P]
+        lib.lbm_amr_read_active.argtypes = [P, P]
+        lib.lbm_amr_read_totals.argtypes = [P, P]
+        lib.lbm_amr_info.argtypes = [P, i64p, i32p, i32p]
         _lib = lib
     return _lib
 
@@ -241,3 +273,124 @@ class Solve:
         ms = ctypes.c_double()
         _check(self.lib.lbm_time_step_kernel(self.h, int(n), ctypes.byref(ms)))
         return ms.value
+
+
+def make_amr_desc(d):
+    """AmrDesc from a dict {q, nx, ny, nz, rb, maxSlots, omega, omegaMinus,
+    force, manageEvery, refineAt, keepAt, lead, explodeLinear, orphans}."""
+    a = AmrDesc()
+    a.abi_version = ABI_VERSION
+    a.q = int(d["q"])
+    a.nx, a.ny, a.nz = int(d["nx"]), int(d["ny"]), int(d["nz"])
+    a.rb = int(d["rb"])
+    a.max_slots = int(d["maxSlots"])
+    a.omega = float(d["omega"])
+    if d.get("omegaMinus") is not None:
+        a.collision = TRT
+        a.omega_minus = float(d["omegaMinus"])
+    for k, v in enumerate((list(d.get("force", [0, 0, 0])) + [0, 0, 0])[:3]):
+        a.force[k] = float(v)
+    a.manage_every = int(d.get("manageEvery", 0))
+    a.refine_at = float(d.get("refineAt", 1e30))
+    a.keep_at = float(d.get("keepAt", 1e30))
+    a.lead = int(d.get("lead", 0))
+    a.explode_linear = int(bool(d.get("explodeLinear", 0)))
+    a.orphans = int(bool(d.get("orphans", 1)))
+    return a
+
+
+class Amr:
+    """Two-level block-pool solve on the GPU (owns an lbm_amr handle)."""
+
+    def __init__(self, desc_dict):
+        self.lib = load()
+        self.desc_dict = dict(desc_dict)
+        self.desc = make_amr_desc(desc_dict)
+        d = self.desc
+        self.q, self.rb, self.max_slots = d.q, d.rb, d.max_slots
+        self.ncells = d.nx * d.ny * d.nz
+        self.nblocks = (d.nx // d.rb) * (d.ny // d.rb) * (d.nz // d.rb)
+        self.fb = 2 * d.rb + 2 * GHOST
+        self.pool_plane = d.max_slots * self.fb ** 3
+        h = ctypes.c_void_p()
+        _check(self.lib.lbm_amr_create(ctypes.byref(self.desc), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.lbm_amr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload_topology(self, block_slot, slot_to_block):
+        bs = np.ascontiguousarray(block_slot, dtype=np.int32)
+        sb = np.ascontiguousarray(slot_to_block, dtype=np.int32)
+        if bs.size != self.nblocks or sb.size != self.max_slots:
+            raise ValueError("topology sizes %d/%d, want %d/%d" % (bs.size, sb.size, self.nblocks, self.max_slots))
+        _check(self.lib.lbm_amr_upload_topology(self.h, bs.ctypes.data, sb.ctypes.data))
+
+    def read_topology(self):
+        bs = np.empty(self.nblocks, dtype=np.int32)
+        sb = np.empty(self.max_slots, dtype=np.int32)
+        _check(self.lib.lbm_amr_read_topology(self.h, bs.ctypes.data, sb.ctypes.data))
+        return bs, sb
+
+    def upload_macro(self, mac4):
+        m = np.ascontiguousarray(mac4, dtype=np.float32).reshape(-1)
+        if m.size != 4 * self.ncells:
+            raise ValueError("mac needs %d floats, got %d" % (4 * self.ncells, m.size))
+        _check(self.lib.lbm_amr_upload_macro(self.h, m.ctypes.data))
+
+    def step(self, n):
+        _check(self.lib.lbm_amr_step(self.h, int(n)))
+
+    def manage_now(self):
+        _check(self.lib.lbm_amr_manage_now(self.h))
+
+    def read_macro(self):
+        out = np.empty(4 * self.ncells, dtype=np.float32)
+        _check(self.lib.lbm_amr_read_macro(self.h, out.ctypes.data))
+        return out
+
+    def read_pool_macro(self):
+        out = np.empty(4 * self.pool_plane, dtype=np.float32)
+        _check(self.lib.lbm_amr_read_pool_macro(self.h, out.ctypes.data))
+        return out
+
+    def read_f(self):
+        f0 = np.empty(self.q * self.ncells, dtype=np.float32)
+        fp = np.empty(self.q * self.pool_plane, dtype=np.float32)
+        _check(self.lib.lbm_amr_read_f(self.h, f0.ctypes.data, fp.ctypes.data))
+        return f0, fp
+
+    def read_criterion(self):
+        crit = np.empty(self.nblocks, dtype=np.float32)
+        want = np.empty(self.nblocks, dtype=np.uint8)
+        _check(self.lib.lbm_amr_read_criterion(self.h, crit.ctypes.data, want.ctypes.data))
+        return crit, want
+
+    def read_active(self):
+        out = np.empty(self.max_slots + 1, dtype=np.int32)
+        _check(self.lib.lbm_amr_read_active(self.h, out.ctypes.data))
+        return out[1: 1 + out[0]].copy()
+
+    def read_totals(self):
+        out = np.empty(4, dtype=np.float64)
+        _check(self.lib.lbm_amr_read_totals(self.h, out.ctypes.data))
+        return out
+
+    def info(self):
+        step, act, exh = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib.lbm_amr_info(self.h, ctypes.byref(step), ctypes.byref(act), ctypes.byref(exh)))
+        return {"step": step.value, "active_tiles": act.value, "exhausted": exh.value}

@@ -150,6 +150,76 @@ int lbm_time_step_kernel(lbm_solve* s, int32_t n, double* ms_per_launch);
  * copy. */
 int lbm_probe_copy_bandwidth(size_t bytes, int32_t reps, double* gbs, double* ms_per_copy);
 
+/* ------------------------------------------------------------------------
+ * Two-level block-structured AMR: the dense L0 lattice plus one pool of
+ * fine tiles (refinement ratio 2).  L0 is cut into blocks of rb^3 cells; a
+ * refined block owns a pool slot of fb^3 fine cells, fb = 2 rb + 4 (the
+ * interior plus a two-cell ring), d3-amr.mjs:38-75.  Each macro-step runs
+ * the manager (every manage_every steps), explode, two fine substeps,
+ * coalesce and the covered L0 step, in the order of main-3d.js:3219-3319.
+ * Storage is f32 (shifted) on both levels.
+ * ------------------------------------------------------------------------ */
+typedef struct lbm_amr_desc {
+  int32_t abi_version;  /* LBM_ABI_VERSION                                    */
+  int32_t q;            /* 19 or 27                                           */
+  int32_t nx, ny, nz;   /* L0 grid, periodic, multiples of rb                 */
+  int32_t rb;           /* L0 cells per block edge (tile interior 2 rb)       */
+  int32_t max_slots;    /* pool capacity in tiles                             */
+  int32_t collision;    /* LBM_BGK or LBM_TRT                                 */
+  float omega;          /* L0 rate 1/tau_c; fine tau_f = 2 tau_c - 1/2        */
+  float omega_minus;    /* L0 TRT odd rate; Lambda is kept on the fine level  */
+  float force[3];       /* Guo body force in L0 units                         */
+  int32_t manage_every; /* macro-steps between manager passes; 0: static      */
+  float refine_at;      /* |curl u| block maximum that refines a block        */
+  float keep_at;        /* |curl u| below which a refined block coarsens      */
+  int32_t lead;         /* criterion box dilation in L0 cells                 */
+  int32_t explode_linear; /* EXPLODE_LINEAR (common_d3_amr_explode.wgsl)      */
+  int32_t orphans;      /* ORPHANS (common_d3_amr_coalesce.wgsl)              */
+} lbm_amr_desc;
+
+typedef struct lbm_amr lbm_amr;
+
+/* Replaces the per-level pool buffers and pipelines, main-3d.js:775-807,
+ * :1223-1700.  Refuses tau_c within 1e-3 of 1 (main-3d.js:430-442). */
+int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out);
+int lbm_amr_destroy(lbm_amr* a);
+
+/* Block maps (blockSlot[nblocks], slotToBlock[max_slots], -1 = none), as
+ * refineWhere / refineHierarchy build them (d3-amr.mjs:80-99, :603-651).
+ * Replaces uploadLevelTopology, main-3d.js:822. */
+int lbm_amr_upload_topology(lbm_amr* a, const int32_t* block_slot, const int32_t* slot_to_block);
+int lbm_amr_read_topology(lbm_amr* a, int32_t* block_slot, int32_t* slot_to_block);
+
+/* L0 [rho, ux, uy, uz] seed -> equilibrium, then every refined tile is
+ * filled from L0 by interpolation (reset, main-3d.js:2973-3035). */
+int lbm_amr_upload_macro(lbm_amr* a, const float* mac4);
+
+/* Advances n macro-steps (encodeSteps, main-3d.js:3122-3335).  Returns
+ * LBM_E_NOMEM when the manager could not place a wanted tile (pool
+ * exhaustion is a hard failure, common_d3_manage.wgsl:500-525). */
+int lbm_amr_step(lbm_amr* a, int32_t n);
+
+/* Runs the manager once now (criterion, decide, drain, coarsen, refine,
+ * fill) without stepping. */
+int lbm_amr_manage_now(lbm_amr* a);
+
+/* L0 macroscopic field (covered cells hold the coalesced projection of the
+ * fine tiles). */
+int lbm_amr_read_macro(lbm_amr* a, float* mac4);
+/* Pool macroscopic field, planar: [4][max_slots * fb^3]. */
+int lbm_amr_read_pool_macro(lbm_amr* a, float* mac_pool);
+/* Populations, f32 wire format: L0 [q][ncells]; pool [q][max_slots * fb^3]. */
+int lbm_amr_read_f(lbm_amr* a, float* f_l0, float* f_pool);
+/* Per block |curl u| maximum from the last manager pass, and its decision. */
+int lbm_amr_read_criterion(lbm_amr* a, float* crit, uint8_t* want);
+/* [count, slot ...] of the slots in use, ascending. */
+int lbm_amr_read_active(lbm_amr* a, int32_t* list);
+/* FP64 [mass, px, py, pz] of the composite state (unrefined L0 cells plus
+ * fine interiors weighted 1/8). */
+int lbm_amr_read_totals(lbm_amr* a, double* out4);
+/* step count, tiles in use, wanted tiles left unplaced (sticky). */
+int lbm_amr_info(lbm_amr* a, int64_t* step, int32_t* active_tiles, int32_t* exhausted);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,3 +62,41 @@ int main(void) {
     want = [ctypes.sizeof(D), D.body.offset, D.solid_eq.offset, D.autotune.offset,
             ctypes.sizeof(pylbm.Body), D.sponge_u.offset]
     assert got == want
+
+
+def test_amr_descriptor_layout_matches_c():
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "lbm.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(lbm_amr_desc), offsetof(lbm_amr_desc, omega),
+         offsetof(lbm_amr_desc, force), offsetof(lbm_amr_desc, manage_every),
+         offsetof(lbm_amr_desc, lead), offsetof(lbm_amr_desc, orphans));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "t")
+        subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe])
+        got = [int(v) for v in subprocess.check_output([exe]).split()]
+    A = pylbm.AmrDesc
+    want = [ctypes.sizeof(A), A.omega.offset, A.force.offset, A.manage_every.offset, A.lead.offset,
+            A.orphans.offset]
+    assert got == want
+
+
+def test_amr_refusals_without_gpu():
+    lib = pylbm.load()
+    h = ctypes.c_void_p()
+    base = {"q": 27, "nx": 32, "ny": 32, "nz": 32, "rb": 8, "maxSlots": 8, "omega": 1.0 / 0.6}
+    # tau_c = 1 makes the non-equilibrium rescale singular (main-3d.js:430-442)
+    d = pylbm.make_amr_desc(dict(base, omega=1.0))
+    assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -4
+    assert b"tau_c" in lib.lbm_last_error()
+    d = pylbm.make_amr_desc(dict(base, nx=30))
+    assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    d = pylbm.make_amr_desc(dict(base, q=9))
+    assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
