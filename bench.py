@@ -89,6 +89,42 @@ def cpu_baseline(steps, threads):
     }
 
 
+def measure_amr(steps, manage_every=50):
+    """The two-level AMR workload: D3Q27 BGK, 128^3 root, 8^3-cell blocks
+    (16^3-cell tiles), seeded vortex rings (radius 20, core 4, circulation
+    0.1), |curl u| criterion with hysteresis, manager every 50 macro-steps.
+    Cell updates per macro-step count the L0 grid plus two fine substeps of
+    every tile interior (the reference's cellUpdatesPerMacroStep)."""
+    desc, mac, meta = sc.vortex_rings(n=128, radius=20.0, core=4.0, circulation=0.1, seed=7, jitter=8.0,
+                                      tau=0.52, q=27)
+    w0 = meta["vorticity_max"]
+    adesc = dict(desc, rb=8, maxSlots=2048, manageEvery=manage_every, lead=1,
+                 refineAt=float(np.float32(0.25 * w0)), keepAt=float(np.float32(0.15 * w0)))
+    n0 = 128 ** 3
+    interior = 16 ** 3
+    with pylbm.Amr(adesc) as a:
+        a.upload_macro(mac)
+        a.manage_now()
+        t_start = a.read_totals()
+        updates, seconds, tiles = 0, 0.0, []
+        done = 0
+        while done < steps:
+            k = min(manage_every, steps - done)
+            t0 = time.perf_counter()
+            a.step(k)
+            seconds += time.perf_counter() - t0
+            nt = a.info()["active_tiles"]
+            tiles.append(nt)
+            updates += k * (n0 + 2 * nt * interior)
+            done += k
+        t_end = a.read_totals()
+        info = a.info()
+    return {"mlups": updates / seconds / 1e6, "seconds": seconds, "macro_steps": steps,
+            "cell_updates": updates, "tiles_min": min(tiles), "tiles_max": max(tiles), "tiles_last": tiles[-1],
+            "exhausted": info["exhausted"], "mass_drift_rel": float((t_end[0] - t_start[0]) / t_start[0]),
+            "refine_at": adesc["refineAt"], "keep_at": adesc["keepAt"], "tau": 0.52}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +133,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-16bit", action="store_true")
+    ap.add_argument("--amr-steps", type=int, default=2000)
+    ap.add_argument("--no-amr", action="store_true")
     ap.add_argument("--tile", default=None,
                     help="fixed tile bx,by,flags (flags bit0 XCD swizzle, bit1 non-temporal) instead of autotuning")
     args = ap.parse_args()
@@ -125,6 +163,13 @@ def main():
             r["speedup_vs_f32"] = r["mlups"] / main_run["mlups"]
             extra[st + "_store"] = r
             print("%s: %s" % (st, json.dumps(r)), file=sys.stderr, flush=True)
+
+    if not args.no_amr:
+        r = measure_amr(args.amr_steps)
+        extra_amr = r
+        print("amr: %s" % json.dumps(r), file=sys.stderr, flush=True)
+    else:
+        extra_amr = None
 
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -169,6 +214,9 @@ def main():
                   "kernel_ms": round(v["kernel_ms"], 5), "achieved_gbs": round(v["achieved_gbs"], 1),
                   "pct_hbm_peak": round(100.0 * v["achieved_gbs"] / HBM_PEAK_GBS, 2),
                   "cd_last200": v["cd_last200"]}
+    if extra_amr:
+        out["amr_d3q27_vortex_rings_128"] = {k: (float("%.5g" % v) if isinstance(v, float) else v)
+                                             for k, v in extra_amr.items()}
     print(json.dumps(out), flush=True)
 
 

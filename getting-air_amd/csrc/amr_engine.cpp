@@ -77,6 +77,10 @@ int validate(const lbm_amr_desc* d) {
   if (d->rb < 2 || d->rb > 16) return fail(LBM_E_INVALID, "rb %d outside [2, 16]", d->rb);
   if (d->nx < d->rb || d->ny < d->rb || d->nz < d->rb || d->nx % d->rb || d->ny % d->rb || d->nz % d->rb)
     return fail(LBM_E_INVALID, "grid %dx%dx%d is not a multiple of rb=%d", d->nx, d->ny, d->nz, d->rb);
+  // the nearest-periodic-image tile lookup needs three blocks per axis
+  // (localInTile, common_d3_pool.wgsl:81-93)
+  if (d->nx / d->rb < 3 || d->ny / d->rb < 3 || d->nz / d->rb < 3)
+    return fail(LBM_E_INVALID, "grid %dx%dx%d has fewer than 3 blocks of %d on an axis", d->nx, d->ny, d->nz, d->rb);
   if (d->max_slots < 1) return fail(LBM_E_INVALID, "max_slots %d", d->max_slots);
   const double tile = std::pow(2.0 * d->rb + 2 * kGhost, 3);
   if ((double)d->max_slots * tile * 4.0 >= 4294967296.0)
@@ -257,16 +261,13 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
     release(a);
     return hip_fail(e, "AMR init");
   }
-  // per-level relaxation: tau_f = 2 tau_c - 1/2; TRT keeps Lambda
+  // per-level relaxation tau_f = 2 tau_c - 1/2 (main-3d.js:1188-1193).  The
+  // fine step is BGK at OMEGA_FINE with the L0 body force unchanged, as the
+  // pool step's overrides are set (main-3d.js:1387-1399,
+  // common_d3_amr_step1.wgsl:347-386), whatever the L0 collision is.
   const double tc = 1.0 / desc->omega, tf = 2.0 * tc - 0.5;
-  double tfm = tf;
-  if (desc->collision == LBM_TRT) {
-    const double lambda = (tc - 0.5) * (1.0 / desc->omega_minus - 0.5);
-    tfm = 0.5 + lambda / (tf - 0.5);
-  }
-  const float Ff[3] = {0.5f * desc->force[0], 0.5f * desc->force[1], 0.5f * desc->force[2]};
-  if (desc->q == 19) fine_params<19>(a->fine, (float)(1.0 / tf), (float)(1.0 / tfm), Ff);
-  else fine_params<27>(a->fine, (float)(1.0 / tf), (float)(1.0 / tfm), Ff);
+  if (desc->q == 19) fine_params<19>(a->fine, (float)(1.0 / tf), (float)(1.0 / tf), desc->force);
+  else fine_params<27>(a->fine, (float)(1.0 / tf), (float)(1.0 / tf), desc->force);
   // post-collision non-equilibrium rescale (common_d3_pool.wgsl:172-187)
   a->c2f = (float)(0.5 * (tf - 1.0) / (tc - 1.0));
   a->f2c = (float)(2.0 * (tc - 1.0) / (tf - 1.0));

@@ -393,10 +393,10 @@ __device__ __forceinline__ float vort_mag(const float4* mac, const PoolGeom& G, 
   const float3 yp = mac_u(mac, G, x, y + 1, z), ym = mac_u(mac, G, x, y - 1, z);
   const float3 zp = mac_u(mac, G, x, y, z + 1), zm = mac_u(mac, G, x, y, z - 1);
   // omega = (dUz/dy - dUy/dz, dUx/dz - dUz/dx, dUy/dx - dUx/dy)
-  const float wx = __fsub_rn(__fmul_rn(__fsub_rn(yp.z, ym.z), 0.5f), __fmul_rn(__fsub_rn(zp.y, zm.y), 0.5f));
-  const float wy = __fsub_rn(__fmul_rn(__fsub_rn(zp.x, zm.x), 0.5f), __fmul_rn(__fsub_rn(xp.z, xm.z), 0.5f));
-  const float wz = __fsub_rn(__fmul_rn(__fsub_rn(xp.y, xm.y), 0.5f), __fmul_rn(__fsub_rn(yp.x, ym.x), 0.5f));
-  return __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(wx, wx), __fmul_rn(wy, wy)), __fmul_rn(wz, wz)));
+  const float wx = sub_rn(mul_rn(sub_rn(yp.z, ym.z), 0.5f), mul_rn(sub_rn(zp.y, zm.y), 0.5f));
+  const float wy = sub_rn(mul_rn(sub_rn(zp.x, zm.x), 0.5f), mul_rn(sub_rn(xp.z, xm.z), 0.5f));
+  const float wz = sub_rn(mul_rn(sub_rn(xp.y, xm.y), 0.5f), mul_rn(sub_rn(yp.x, ym.x), 0.5f));
+  return sqrt_rn(add_rn(add_rn(mul_rn(wx, wx), mul_rn(wy, wy)), mul_rn(wz, wz)));
 }
 
 __global__ void __launch_bounds__(64) k_criterion(const float4* __restrict__ mac, float* __restrict__ crit, PoolGeom G,

@@ -359,18 +359,18 @@ __global__ void __launch_bounds__(256) k_classify(ClassifyArgs a) {
   const int x = cell % a.nx;
   const int y = (cell / a.nx) % a.ny;
   const int z = cell / (a.nx * a.ny);
-  float dx = __fsub_rn((float)x, a.cx);
-  float dy = __fsub_rn((float)y, a.cy);
+  float dx = sub_rn((float)x, a.cx);
+  float dy = sub_rn((float)y, a.cy);
   float phi;
   if (a.shape == 2) {
     const float W = (float)a.nx, H = (float)a.ny;
-    dx = __fsub_rn(dx, __fmul_rn(W, rintf(__fdiv_rn(dx, W))));
-    dy = __fsub_rn(dy, __fmul_rn(H, rintf(__fdiv_rn(dy, H))));
-    phi = __fsub_rn(__fsqrt_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy))), a.radius);
+    dx = sub_rn(dx, mul_rn(W, rintf(div_rn(dx, W))));
+    dy = sub_rn(dy, mul_rn(H, rintf(div_rn(dy, H))));
+    phi = sub_rn(sqrt_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy))), a.radius);
   } else {
-    const float dz = __fsub_rn((float)z, a.cz);
-    const float s = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
-    phi = __fsub_rn(__fsqrt_rn(s), a.radius);
+    const float dz = sub_rn((float)z, a.cz);
+    const float s = add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz));
+    phi = sub_rn(sqrt_rn(s), a.radius);
   }
   uint8_t c = 0;
   if (phi < 0.f) c |= 1;

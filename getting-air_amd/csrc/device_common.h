@@ -94,6 +94,38 @@ __device__ __forceinline__ void st(typename ST::T* p, uint32_t i, float g) {
 }
 
 // ---------------------------------------------------------------------------
+// Correctly rounded binary32 arithmetic for integer decisions (solid class
+// bytes, refinement criterion) that a host restatement must reproduce bit for
+// bit.  Each operation is evaluated in binary64 and rounded once to binary32.
+// A product of two binary32 values is exact in binary64; for +, -, / and
+// sqrt, rounding to binary64 first and then to binary32 gives the correctly
+// rounded binary32 result because 53 >= 2 * 24 + 2 (Figueroa, "When is
+// double rounding innocuous?", SIGNUM 1995).
+// ---------------------------------------------------------------------------
+// contract(off) keeps the operations free of the contraction flag, so that
+// no later pass may fuse one of them with another into an FMA.
+__device__ __forceinline__ float add_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return (float)((double)a + (double)b);
+}
+__device__ __forceinline__ float sub_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return (float)((double)a - (double)b);
+}
+__device__ __forceinline__ float mul_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return (float)((double)a * (double)b);
+}
+__device__ __forceinline__ float div_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return (float)((double)a / (double)b);
+}
+__device__ __forceinline__ float sqrt_rn(float a) {
+#pragma clang fp contract(off)
+  return (float)__builtin_sqrt((double)a);
+}
+
+// ---------------------------------------------------------------------------
 // FP64 reductions.  Wave64 butterfly with __shfl_xor, then one slot per wave
 // in LDS, then wave 0 folds the slots.  The order of every addition is fixed
 // by the launch shape, so results repeat bit for bit.

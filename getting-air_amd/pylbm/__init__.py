@@ -294,7 +294,7 @@ def make_amr_desc(d):
     a.refine_at = float(d.get("refineAt", 1e30))
     a.keep_at = float(d.get("keepAt", 1e30))
     a.lead = int(d.get("lead", 0))
-    a.explode_linear = int(bool(d.get("explodeLinear", 0)))
+    a.explode_linear = int(bool(d.get("explodeLinear", 1)))
     a.orphans = int(bool(d.get("orphans", 1)))
     return a
 

@@ -159,3 +159,78 @@ def turbulence(n=512, seed=2024, urms=0.04, kpeak=8.0, tau=0.55, q=19, storage="
     mac[..., 0] = 1.0 - usq / (2.0 * CS2)
     desc = {"q": q, "nx": n, "ny": n, "nz": n, "omega": f32(1.0 / tau), "storage": storage}
     return desc, mac.reshape(-1)
+
+
+def mulberry32(seed):
+    """Seeded 32-bit PRNG, the generator the reference seeds its
+    perturbations with (d3-scenarios.mjs:207-216); same sequence as the JS
+    host's mulberry32."""
+    state = [seed & 0xFFFFFFFF]
+
+    def imul(a, b):
+        return (a * b) & 0xFFFFFFFF
+
+    def nxt():
+        state[0] = (state[0] + 0x6D2B79F5) & 0xFFFFFFFF
+        t = state[0]
+        t = imul(t ^ (t >> 15), t | 1)
+        t ^= (t + imul(t ^ (t >> 7), t | 61)) & 0xFFFFFFFF
+        return ((t ^ (t >> 14)) & 0xFFFFFFFF) / 4294967296.0
+
+    return nxt
+
+
+def vortex_rings(n=128, radius=20.0, core=4.0, circulation=0.1, seed=7, jitter=8.0, tau=0.6, q=27, rings=8):
+    """Seeded random vortex-ring field in a periodic n^3 box.
+
+    Up to eight rings (the first `rings` in x-fastest order) sit on the 2x2x2
+    lattice of centres n/4 + (n/2) {0, 1}^3, each centre moved by up to +-`jitter` cells per axis from mulberry32(seed), the
+    ring axis cycling x, y, z.  Each ring is a Gaussian (Lamb-Oseen) core of
+    radius `core` carrying `circulation` around a circle of radius `radius`:
+    the vorticity field w = Gamma / (pi a^2) exp(-d^2 / a^2) t_hat, with d
+    the distance to the ring's centre line and t_hat its tangent.  The
+    velocity is the periodic Biot-Savart solution u = curl(-lap^-1 w),
+    computed spectrally, so it is divergence-free; rho = 1 - |u|^2 / (2 cs^2).
+    The reference has no such initialiser; this one is new and deterministic.
+    """
+    rnd = mulberry32(seed)
+    ax = np.arange(n, dtype=np.float64)
+    Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")
+    w = [np.zeros((n, n, n)) for _ in range(3)]
+    k = 0
+    for cz in (0, 1):
+        for cy in (0, 1):
+            for cx in (0, 1):
+                c = [n / 4 + n / 2 * v + jitter * (2.0 * rnd() - 1.0) for v in (cx, cy, cz)]
+                if k >= rings:
+                    continue
+                axis = k % 3
+                k += 1
+                # periodic offsets from the centre, nearest image
+                d = [(G - c[j] + n / 2) % n - n / 2 for j, G in enumerate((X, Y, Z))]
+                a1, a2 = (axis + 1) % 3, (axis + 2) % 3
+                rho_p = np.sqrt(d[a1] ** 2 + d[a2] ** 2)
+                dist2 = (rho_p - radius) ** 2 + d[axis] ** 2
+                mag = circulation / (math.pi * core * core) * np.exp(-dist2 / (core * core))
+                inv = 1.0 / np.maximum(rho_p, 1e-12)
+                # tangent of the circle around `axis`: (-d2, d1) / rho in the (a1, a2) plane
+                w[a1] += -mag * d[a2] * inv
+                w[a2] += mag * d[a1] * inv
+    kf = np.fft.fftfreq(n, d=1.0 / n) * (2.0 * math.pi / n)
+    kr = np.fft.rfftfreq(n, d=1.0 / n) * (2.0 * math.pi / n)
+    KZ, KY, KX = np.meshgrid(kf, kf, kr, indexing="ij")
+    k2 = KX * KX + KY * KY + KZ * KZ
+    k2[0, 0, 0] = 1.0
+    wk = [np.fft.rfftn(c) for c in w]
+    # psi = w / k^2 (lap psi = -w), u = curl psi = i k x psi
+    px, py, pz = (c / k2 for c in wk)
+    uk = [1j * (KY * pz - KZ * py), 1j * (KZ * px - KX * pz), 1j * (KX * py - KY * px)]
+    u = [np.fft.irfftn(c, s=(n, n, n)) for c in uk]
+    mac = np.empty((n, n, n, 4), dtype=np.float32)
+    usq = u[0] ** 2 + u[1] ** 2 + u[2] ** 2
+    mac[..., 0] = 1.0 - usq / (2.0 * CS2)
+    for j in range(3):
+        mac[..., 1 + j] = u[j]
+    wmax = float(np.sqrt(w[0] ** 2 + w[1] ** 2 + w[2] ** 2).max())
+    desc = {"q": q, "nx": n, "ny": n, "nz": n, "omega": f32(1.0 / tau)}
+    return desc, mac.reshape(-1), {"vorticity_max": wmax}

@@ -167,8 +167,8 @@ typedef struct lbm_amr_desc {
   int32_t max_slots;    /* pool capacity in tiles                             */
   int32_t collision;    /* LBM_BGK or LBM_TRT                                 */
   float omega;          /* L0 rate 1/tau_c; fine tau_f = 2 tau_c - 1/2        */
-  float omega_minus;    /* L0 TRT odd rate; Lambda is kept on the fine level  */
-  float force[3];       /* Guo body force in L0 units                         */
+  float omega_minus;    /* L0 TRT odd rate; the fine level is BGK             */
+  float force[3];       /* Guo body force, applied unchanged on both levels   */
   int32_t manage_every; /* macro-steps between manager passes; 0: static      */
   float refine_at;      /* |curl u| block maximum that refines a block        */
   float keep_at;        /* |curl u| below which a refined block coarsens      */

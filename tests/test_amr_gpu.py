@@ -1,0 +1,177 @@
+"""GPU parity of the two-level block pool (lbm_amr_* through the C ABI)
+against the FP64 oracle (oracle/amr_oracle.py).
+
+Float state: L0 rho/u everywhere and the tile interiors within 1e-5 of the
+oracle after several macro-steps.  Integer state (criterion flags, block
+maps, active list) bit-exact: the oracle's binary32 criterion is evaluated on
+the engine's own macroscopic field, so both sides decide from the same input.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "getting-air_amd"))
+
+import amr_oracle as ao  # noqa: E402
+import pylbm  # noqa: E402
+from pylbm import scenarios  # noqa: E402
+from test_amr_oracle import SEAMS, ids, smooth_field, topology  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def pool_interiors(amr, f_pool):
+    """[q, slots, 2rb, 2rb, 2rb] interior view of a read_f pool array."""
+    fb, lo, hi = amr.fb, ao.GHOST, ao.GHOST + 2 * amr.rb
+    p = f_pool.reshape(amr.q, amr.max_slots, fb, fb, fb)
+    return p[:, :, lo:hi, lo:hi, lo:hi]
+
+
+def compare(amr, orc, tol=1e-5, label=""):
+    mac = amr.read_macro().reshape(-1, 4).astype(np.float64)
+    ref = orc.mac()
+    umax = np.abs(ref[:, 1:]).max()
+    e_rho = np.abs(mac[:, 0] - ref[:, 0]).max()
+    e_u = np.abs(mac[:, 1:] - ref[:, 1:]).max() / umax
+    f0, fp = amr.read_f()
+    cov = orc.covered_mask().reshape(-1)
+    e_f0 = np.abs(f0.reshape(amr.q, -1)[:, ~cov] - orc.l0_f()[:, ~cov]).max() if (~cov).any() else 0.0
+    act = ao.active_list(orc.slot_to_block)
+    lo, hi = ao.GHOST, ao.GHOST + orc.rb2
+    gp = pool_interiors(amr, fp)[:, act]
+    op = orc.pool[:, act, lo:hi, lo:hi, lo:hi]
+    e_fp = np.abs(gp - op).max() if act.size else 0.0
+    say(label, "rho %.2e  u(rel) %.2e  f_l0 %.2e  f_pool %.2e" % (e_rho, e_u, e_f0, e_fp))
+    assert e_rho < tol and e_u < tol
+    assert e_f0 < tol * 0.1 and e_fp < tol * 0.1
+
+
+def make(desc, mac, blocks=()):
+    amr = pylbm.Amr(desc)
+    orc = ao.AmrOracle(desc)
+    if blocks:
+        bs, sb = topology(orc, blocks)
+        amr.upload_topology(bs, sb)
+        orc.upload_topology(bs, sb)
+    amr.upload_macro(mac)
+    orc.upload_macro(mac)
+    return amr, orc
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_no_tiles_matches_the_oracle(q):
+    n = 16
+    desc = {"q": q, "nx": n, "ny": n, "nz": n, "rb": 4, "maxSlots": 4, "omega": 1 / 0.6,
+            "force": [1e-5, 0.0, -2e-6]}
+    say("no tiles q", q)
+    amr, orc = make(desc, smooth_field(n, n, n, 3))
+    with amr:
+        amr.step(10)
+        for _ in range(10):
+            orc.macro_step()
+        compare(amr, orc, label="no tiles")
+        assert amr.read_active().size == 0
+
+
+@pytest.mark.parametrize("shape", sorted(SEAMS))
+@pytest.mark.parametrize("linear", [0, 1])
+def test_static_seams_match_the_oracle(shape, linear):
+    n = 16
+    desc = {"q": 27, "nx": n, "ny": n, "nz": n, "rb": 4, "maxSlots": 8, "omega": 1 / 0.6,
+            "explodeLinear": linear}
+    say("static seams", shape, "linear", linear)
+    orc0 = ao.AmrOracle(desc)
+    amr, orc = make(desc, smooth_field(n, n, n, 11, amp=0.04), ids(orc0, SEAMS[shape]))
+    with amr:
+        t0 = amr.read_totals()
+        compare(amr, orc, label="after seed")
+        for k in range(3):
+            amr.step(2)
+            orc.macro_step()
+            orc.macro_step()
+            compare(amr, orc, label="macro-step %d" % (2 * k + 2))
+        t = amr.read_totals()
+        # binary32 populations: mass to ~1e-7 over a few steps
+        assert abs(t[0] - t0[0]) < 1e-6 * t0[0]
+        assert np.abs(t[1:] - t0[1:]).max() < 1e-6 * t0[0]
+        bs, sb = amr.read_topology()
+        assert (bs == orc.block_slot).all() and (sb == orc.slot_to_block).all()
+        assert (amr.read_active() == ao.active_list(orc.slot_to_block)).all()
+
+
+def test_all_refined_matches_the_oracle():
+    n = 12
+    desc = {"q": 19, "nx": n, "ny": n, "nz": n, "rb": 4, "maxSlots": 27, "omega": 1 / 0.7,
+            "force": [2e-6, 1e-6, 0.0]}
+    say("all refined")
+    amr, orc = make(desc, smooth_field(n, n, n, 5), range(27))
+    with amr:
+        amr.step(4)
+        for _ in range(4):
+            orc.macro_step()
+        compare(amr, orc, label="all refined")
+
+
+def test_manager_is_bit_exact_and_fields_follow():
+    # two decaying rings; thresholds chosen so that blocks refine after the
+    # first pass and others coarsen later, reusing freed slots
+    n, rb = 32, 4
+    desc = {"q": 27, "nx": n, "ny": n, "nz": n, "rb": rb, "maxSlots": 64, "omega": 1 / 0.56,
+            "manageEvery": 2, "lead": 0, "refineAt": 0.0014, "keepAt": 0.00138}
+    _, mac, _ = scenarios.vortex_rings(n=n, radius=5.0, core=1.5, circulation=0.05, seed=3, jitter=2.0,
+                                       tau=0.56, q=27, rings=2)
+    say("manager")
+    amr, orc = make(desc, mac)
+    with amr:
+        prev_bs, prev_sb = amr.read_topology()
+        changes = 0
+        for step in range(8):
+            manage = step % 2 == 0
+            if manage:
+                m = amr.read_macro().reshape(n, n, n, 4)
+                want_crit = ao.block_criterion(
+                    ao.vorticity_magnitude(np.moveaxis(m, -1, 0), np.float32), rb, desc["lead"])
+            amr.step(1)
+            if manage:
+                crit, want = amr.read_criterion()
+                # bit-exact criterion and decisions from the same field
+                assert (crit.view(np.uint32) == want_crit.view(np.uint32)).all(), step
+                dec = ao.decide(prev_bs, crit, np.float32(desc["refineAt"]), np.float32(desc["keepAt"]))
+                assert (want == dec).all(), step
+                bs, sb = amr.read_topology()
+                tbs, tsb, dying, fresh, exh = ao.manage_slots(prev_bs, prev_sb, dec)
+                assert (bs == tbs).all() and (sb == tsb).all(), step
+                assert (amr.read_active() == ao.active_list(tsb)).all(), step
+                assert exh == 0 and amr.info()["exhausted"] == 0
+                changes += len(dying) + len(fresh)
+                prev_bs, prev_sb = bs, sb
+                orc.macro_step(crit=crit)
+            else:
+                orc.macro_step()
+            compare(amr, orc, label="step %d" % step)
+        say("topology changes", changes, "active", amr.info()["active_tiles"])
+        assert amr.info()["active_tiles"] > 0 and changes > 38
+
+
+def test_pool_exhaustion_is_a_hard_failure():
+    n = 16
+    desc = {"q": 19, "nx": n, "ny": n, "nz": n, "rb": 4, "maxSlots": 2, "omega": 1 / 0.6,
+            "manageEvery": 1, "refineAt": -1.0, "keepAt": -1.0}
+    amr = pylbm.Amr(desc)
+    with amr:
+        amr.upload_macro(smooth_field(n, n, n, 2))
+        with pytest.raises(pylbm.LbmError) as e:
+            amr.step(1)
+        assert e.value.code == -3 and "exhausted" in str(e.value)
+        info = amr.info()
+        assert info["exhausted"] == 64 - 2 and info["active_tiles"] == 2
+        bs, sb = amr.read_topology()
+        assert sb.tolist() == [0, 1] and (bs[2:] == -1).all()
