@@ -25,7 +25,16 @@ struct PoolGeom {
   int32_t max_slots;
   const int32_t* block_slot;     // per block: slot or -1
   const int32_t* slot_to_block;  // per slot: block or -1
+  const uint8_t* ccls;           // per L0 cell: kCovered | kNear | kSeam
 };
+
+// L0 cell classes, rebuilt by launch_cell_class after every topology change.
+constexpr uint8_t kCovered = 1;  // the cell's block holds a tile
+constexpr uint8_t kNear = 2;     // unrefined, with a covered lattice neighbour
+constexpr uint8_t kSeam = 4;     // covered, with an unrefined lattice neighbour
+
+// Block edges the pool kernels are compiled for.
+constexpr bool supported_rb(int rb) { return rb == 4 || rb == 8 || rb == 16; }
 
 // Manager inputs and outputs (one workgroup; see k_manage).
 struct ManageArgs {
@@ -43,12 +52,29 @@ struct ManageArgs {
   int32_t* status;           // [0] wanted blocks left without a slot (sticky sum)
 };
 
+// Work lists of the surface passes, rebuilt with the cell classes after every
+// topology change (entries in no particular order: each entry's pass writes
+// only its own cell, so the order cannot change a result).
+struct SurfacePlan {
+  uint32_t* seam;   // covered L0 cells with an unrefined lattice neighbour
+  uint32_t* near;   // unrefined L0 cells with a covered lattice neighbour
+  uint32_t* ring;   // pool cells (slot * FB^3 + local) of ring cells over unrefined L0 cells
+  uint32_t* count;  // [seam, near, ring]
+  uint32_t cap_seam, cap_near, cap_ring;
+};
+// Capacity of the lists for a pool of max_slots tiles.
+void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint32_t* near, uint32_t* ring);
+
+// Rebuilds ccls and the plan lists (count must be zeroed first).
+hipError_t launch_cell_class(int q, uint8_t* ccls, const PoolGeom& g, const SurfacePlan& plan, const int32_t* active,
+                             int nactive, hipStream_t s);
 hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
                             const CollideParams& fine, const int32_t* active, int nactive, hipStream_t s);
-hipError_t launch_explode(int q, const float* gcoarse, float* gpool, const PoolGeom& g, const int32_t* active,
-                          int nactive, int linear, hipStream_t s);
+hipError_t launch_explode(int q, const float* gcoarse, float* gpool, const PoolGeom& g, const SurfacePlan& plan,
+                          uint32_t nring, int linear, hipStream_t s);
 hipError_t launch_coalesce(int q, const float* gpool, float* gcoarse, float* mac, const float* mac_pool,
-                           const PoolGeom& g, int orphans, hipStream_t s);
+                           const PoolGeom& g, const int32_t* active, int nactive, const SurfacePlan& plan,
+                           uint32_t nseam, uint32_t nnear, int orphans, hipStream_t s);
 hipError_t launch_interp(int q, const float* gcoarse, float* gpool, const PoolGeom& g, const int32_t* pairs,
                          int npairs, float rescale_c2f, hipStream_t s);
 hipError_t launch_average(int q, const float* gpool, float* gcoarse, float* mac, const PoolGeom& g,

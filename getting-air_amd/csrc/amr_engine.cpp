@@ -42,6 +42,9 @@ struct lbm_amr {
   int32_t* free_list = nullptr;
   int32_t* status = nullptr;
   int32_t* pairs = nullptr;
+  uint8_t* ccls = nullptr;
+  SurfacePlan plan{};
+  uint32_t nseam = 0, nnear = 0, nring = 0;
   double* partials = nullptr;
   unsigned* counter = nullptr;
   double* totals = nullptr;
@@ -74,7 +77,7 @@ int validate(const lbm_amr_desc* d) {
   if (!d) return fail(LBM_E_INVALID, "null descriptor");
   if (d->abi_version != LBM_ABI_VERSION) return fail(LBM_E_INVALID, "descriptor abi_version %d", d->abi_version);
   if (d->q != 19 && d->q != 27) return fail(LBM_E_INVALID, "AMR needs q = 19 or 27 (got %d)", d->q);
-  if (d->rb < 2 || d->rb > 16) return fail(LBM_E_INVALID, "rb %d outside [2, 16]", d->rb);
+  if (!supported_rb(d->rb)) return fail(LBM_E_INVALID, "rb %d: the pool kernels are built for rb = 4, 8, 16", d->rb);
   if (d->nx < d->rb || d->ny < d->rb || d->nz < d->rb || d->nx % d->rb || d->ny % d->rb || d->nz % d->rb)
     return fail(LBM_E_INVALID, "grid %dx%dx%d is not a multiple of rb=%d", d->nx, d->ny, d->nz, d->rb);
   // the nearest-periodic-image tile lookup needs three blocks per axis
@@ -98,7 +101,8 @@ int validate(const lbm_amr_desc* d) {
 void release(lbm_amr* a) {
   if (!a) return;
   void* ptrs[] = {a->pool[0], a->pool[1], a->mac_pool, a->block_slot, a->slot_to_block, a->want, a->crit,
-                  a->dying, a->fresh, a->active, a->free_list, a->status, a->pairs, a->partials, a->counter, a->totals};
+                  a->dying, a->fresh, a->active, a->free_list, a->status, a->pairs, a->ccls, a->partials, a->counter, a->totals,
+                  a->plan.seam, a->plan.near, a->plan.ring, a->plan.count};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (a->l0) lbm_solve_destroy(a->l0);
@@ -122,12 +126,31 @@ int fill_tiles(lbm_amr* a, const int32_t* pairs, int n) {
   return LBM_OK;
 }
 
+// Rebuilds the L0 cell classes and the surface passes' work lists for the
+// current block maps and active list (a->nactive must be current).
+int rebuild_plan(lbm_amr* a) {
+  HIPCHK(hipMemsetAsync(a->plan.count, 0, 16, a->stream));
+  HIPCHK(launch_cell_class(a->d.q, a->ccls, a->G, a->plan, a->active, a->nactive, a->stream));
+  uint32_t c[3] = {0, 0, 0};
+  HIPCHK(hipMemcpyAsync(c, a->plan.count, 12, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  if (c[0] > a->plan.cap_seam || c[1] > a->plan.cap_near || c[2] > a->plan.cap_ring)
+    return fail(LBM_E_DEVICE, "surface work lists overflow (%u/%u seam, %u/%u near, %u/%u ring)", c[0],
+                a->plan.cap_seam, c[1], a->plan.cap_near, c[2], a->plan.cap_ring);
+  a->nseam = c[0];
+  a->nnear = c[1];
+  a->nring = c[2];
+  return LBM_OK;
+}
+
 int refresh_active(lbm_amr* a) {
   HIPCHK(launch_active_list(a->slot_to_block, a->d.max_slots, a->active, a->stream));
   int32_t n = 0;
   HIPCHK(hipMemcpyAsync(&n, a->active, 4, hipMemcpyDeviceToHost, a->stream));
   HIPCHK(hipStreamSynchronize(a->stream));
   a->nactive = n;
+  int rc = rebuild_plan(a);
+  if (rc) return rc;
   return LBM_OK;
 }
 
@@ -169,10 +192,12 @@ int manage(lbm_amr* a) {
   HIPCHK(hipMemcpyAsync(&counts[2], a->active, 4, hipMemcpyDeviceToHost, a->stream));
   HIPCHK(hipMemcpyAsync(&counts[3], a->status, 4, hipMemcpyDeviceToHost, a->stream));
   HIPCHK(hipStreamSynchronize(a->stream));
+  a->nactive = counts[2];
+  rc = rebuild_plan(a);
+  if (rc) return rc;
   // dying tiles were drained by restrict_active; fill the new ones
   rc = fill_tiles(a, a->fresh, counts[1]);
   if (rc) return rc;
-  a->nactive = counts[2];
   a->exhausted = counts[3];
   if (counts[3] > 0)
     return fail(LBM_E_NOMEM, "AMR pool exhausted: %d wanted tiles found no free slot (max_slots %d)", counts[3],
@@ -187,10 +212,11 @@ int macro_step(lbm_amr* a) {
     if (rc) return rc;
   }
   float* l0 = solve_lattice(a->l0, 0);
-  HIPCHK(launch_explode(d.q, l0, a->pool[0], a->G, a->active, a->nactive, d.explode_linear, a->stream));
+  HIPCHK(launch_explode(d.q, l0, a->pool[0], a->G, a->plan, a->nring, d.explode_linear, a->stream));
   HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], a->mac_pool, a->G, a->fine, a->active, a->nactive, a->stream));
   HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->active, a->nactive, a->stream));
-  HIPCHK(launch_coalesce(d.q, a->pool[0], l0, solve_mac(a->l0), a->mac_pool, a->G, d.orphans, a->stream));
+  HIPCHK(launch_coalesce(d.q, a->pool[0], l0, solve_mac(a->l0), a->mac_pool, a->G, a->active, a->nactive, a->plan,
+                         a->nseam, a->nnear, d.orphans, a->stream));
   int rc = solve_enqueue_covered_step(a->l0, a->block_slot, d.rb, a->G.nbx, a->G.nby);
   if (rc) return rc;
   a->step++;
@@ -232,6 +258,7 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
   G.max_slots = desc->max_slots;
   G.pool_plane = (uint32_t)((size_t)desc->max_slots * a->fb * a->fb * a->fb);
   a->nblocks = G.nbx * G.nby * G.nbz;
+  plan_capacity(rb, desc->max_slots, G.ncells, &a->plan.cap_seam, &a->plan.cap_near, &a->plan.cap_ring);
   const size_t pool_bytes = (size_t)G.pool_plane * desc->q * 4;
   if ((rc = dalloc(&a->pool[0], pool_bytes)) || (rc = dalloc(&a->pool[1], pool_bytes)) ||
       (rc = dalloc(&a->mac_pool, (size_t)G.pool_plane * 16)) || (rc = dalloc(&a->block_slot, (size_t)a->nblocks * 4)) ||
@@ -242,12 +269,17 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
       (rc = dalloc(&a->active, (size_t)(desc->max_slots + 1) * 4)) ||
       (rc = dalloc(&a->free_list, (size_t)desc->max_slots * 4)) || (rc = dalloc(&a->status, 16)) ||
       (rc = dalloc(&a->pairs, (size_t)(2 * desc->max_slots + 1) * 4)) ||
+      (rc = dalloc(&a->ccls, (size_t)G.ncells)) || (rc = dalloc(&a->plan.count, 16)) ||
+      (rc = dalloc(&a->plan.seam, (size_t)a->plan.cap_seam * 4)) ||
+      (rc = dalloc(&a->plan.near, (size_t)a->plan.cap_near * 4)) ||
+      (rc = dalloc(&a->plan.ring, (size_t)a->plan.cap_ring * 4)) ||
       (rc = dalloc(&a->partials, 256 * 4 * 8)) || (rc = dalloc(&a->counter, 16)) || (rc = dalloc(&a->totals, 32))) {
     release(a);
     return rc;
   }
   G.block_slot = a->block_slot;
   G.slot_to_block = a->slot_to_block;
+  G.ccls = a->ccls;
   hipError_t e = hipMemsetAsync(a->block_slot, 0xff, (size_t)a->nblocks * 4, a->stream);
   if (e == hipSuccess) e = hipMemsetAsync(a->slot_to_block, 0xff, (size_t)desc->max_slots * 4, a->stream);
   if (e == hipSuccess) e = hipMemsetAsync(a->want, 0, (size_t)a->nblocks, a->stream);
@@ -256,6 +288,7 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
   if (e == hipSuccess) e = hipMemsetAsync(a->status, 0, 16, a->stream);
   if (e == hipSuccess) e = hipMemsetAsync(a->counter, 0, 16, a->stream);
   if (e == hipSuccess) e = hipMemsetAsync(a->mac_pool, 0, (size_t)G.pool_plane * 16, a->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(a->ccls, 0, (size_t)G.ncells, a->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(a->stream);
   if (e != hipSuccess) {
     release(a);

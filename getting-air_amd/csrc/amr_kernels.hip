@@ -5,7 +5,7 @@
 //                  omega_fine)
 //   explode        shaders/common_d3_amr_explode.wgsl:98-214
 //   coalesce       shaders/common_d3_amr_coalesce.wgsl:126-285 (+ orphans)
-//   interp fill    shaders/common_d3_amr_interp.wgsl:142-185 with
+//   interp fill    shaders/common_d3_amr_interp.wgsl:109-185 with
 //                  interpCoarseToFine3, common_d3_pool.wgsl:192-220
 //   average/drain  shaders/common_d3_amr_average.wgsl:75-154
 //   criterion      shaders/common_d3_criterion.wgsl:122-158, with the
@@ -16,7 +16,16 @@
 //                  instead of an atomic free list.
 // Populations are stored shifted (g = f - w); the surface passes convert to
 // f where their arithmetic is not linear in f with unit total weight.
+//
+// The block edge RB is a template parameter (4, 8 or 16), so every tile
+// index is shift-and-mask arithmetic.  Which L0 cells are covered, next to a
+// covered cell, or on the covered side of a seam is read from a class byte
+// per L0 cell (k_cell_class, rebuilt after each topology change) instead of
+// being re-derived from the block map in every pass.  Neighbour coordinates
+// move by at most one period, so they wrap with a compare, never a modulo.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "amr.h"
 #include "collide.h"
@@ -26,54 +35,137 @@
 
 namespace lbm {
 
-__device__ __forceinline__ int wrapi(int v, int n) { return ((v % n) + n) % n; }
+template <int RB>
+struct Tile {
+  static constexpr int RB2 = 2 * RB;
+  static constexpr int FB = RB2 + 2 * kGhost;
+  static constexpr int CELLS = FB * FB * FB;
+  static constexpr int LO = kGhost, HI = kGhost + RB2;
+  static constexpr int LOG_RB = RB == 4 ? 2 : (RB == 8 ? 3 : 4);
+  // FB = 4 (RB/2 + 1), so a tile is a whole number of 64-lane waves and
+  // every wave of a tile-major launch lies in one tile
+  static_assert(CELLS % 64 == 0, "tile cells must fill whole waves");
+};
 
-__device__ __forceinline__ int block_id(const PoolGeom& G, int bx, int by, int bz) {
-  return (bz * G.nby + by) * G.nbx + bx;
+// Tile index of a tile-major launch, made wave-uniform (scalar registers).
+template <int RB>
+__device__ __forceinline__ uint32_t wave_tile(uint32_t t) {
+  return __builtin_amdgcn_readfirstlane(t / (uint32_t)Tile<RB>::CELLS);
 }
-__device__ __forceinline__ bool coarse_refined(const PoolGeom& G, int cx, int cy, int cz) {
-  return G.block_slot[block_id(G, cx / G.rb, cy / G.rb, cz / G.rb)] >= 0;
-}
+
+// v in [-n, 2n) -> [0, n)
+__device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
 __device__ __forceinline__ uint32_t coarse_cell(const PoolGeom& G, int x, int y, int z) {
   return (uint32_t)((z * G.ny + y) * G.nx + x);
 }
-__device__ __forceinline__ uint32_t pool_cell(const PoolGeom& G, int slot, int fx, int fy, int fz) {
-  return (uint32_t)(((slot * G.fb + fz) * G.fb + fy) * G.fb + fx);
+template <int RB>
+__device__ __forceinline__ uint32_t pool_cell(int slot, int fx, int fy, int fz) {
+  constexpr int FB = Tile<RB>::FB;
+  return (uint32_t)(slot * Tile<RB>::CELLS + (fz * FB + fy) * FB + fx);
 }
-// Tile-local index of global fine coordinate gcoord relative to block b on
-// one axis, taking the periodic image nearest the block (localInTile,
-// common_d3_pool.wgsl:75-85).
-__device__ __forceinline__ int local_in_tile(int gcoord, int b, int nfine, int rb2) {
-  int d = gcoord - b * rb2;
+__device__ __forceinline__ int block_id(const PoolGeom& G, int bx, int by, int bz) {
+  return (bz * G.nby + by) * G.nbx + bx;
+}
+// Is L0 cell (x, y, z), each coordinate in [-n, 2n), under a tile.
+__device__ __forceinline__ bool covered_at(const PoolGeom& G, int x, int y, int z) {
+  return G.ccls[coarse_cell(G, wrap1(x, G.nx), wrap1(y, G.ny), wrap1(z, G.nz))] & kCovered;
+}
+// Tile-local index of global fine coordinate g relative to block coordinate
+// b on one axis, nearest periodic image (localInTile, common_d3_pool.wgsl:
+// 81-93; valid with three or more blocks per axis, which the engine checks).
+template <int RB>
+__device__ __forceinline__ int local_in_tile(int g, int b, int nfine) {
+  int d = g - b * Tile<RB>::RB2;
   if (d < -nfine / 2) d += nfine;
   if (d > nfine / 2) d -= nfine;
   return d + kGhost;
 }
 
 // ---------------------------------------------------------------------------
+// L0 cell classes
+// ---------------------------------------------------------------------------
+template <int Q, int RB>
+__global__ void __launch_bounds__(256) k_cell_class(uint8_t* __restrict__ ccls, PoolGeom G, SurfacePlan plan) {
+  using L = Lattice<Q>;
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G.ncells) return;
+  const int x = c % G.nx, y = (c / G.nx) % G.ny, z = c / (G.nx * G.ny);
+  auto cov = [&](int cx, int cy, int cz) {
+    cx = wrap1(cx, G.nx);
+    cy = wrap1(cy, G.ny);
+    cz = wrap1(cz, G.nz);
+    return G.block_slot[block_id(G, cx >> Tile<RB>::LOG_RB, cy >> Tile<RB>::LOG_RB, cz >> Tile<RB>::LOG_RB)] >= 0;
+  };
+  const bool self = cov(x, y, z);
+  bool other = false;  // a lattice neighbour with the opposite state
+#pragma unroll
+  for (int i = 1; i < Q; i++) other = other || (cov(x + L::ex[i], y + L::ey[i], z + L::ez[i]) != self);
+  ccls[c] = self ? (uint8_t)(kCovered | (other ? kSeam : 0)) : (uint8_t)(other ? kNear : 0);
+  if (other) {
+    if (self) {
+      const uint32_t k = atomicAdd(&plan.count[0], 1u);
+      if (k < plan.cap_seam) plan.seam[k] = c;
+    } else {
+      const uint32_t k = atomicAdd(&plan.count[1], 1u);
+      if (k < plan.cap_near) plan.near[k] = c;
+    }
+  }
+}
+
+// Ring cells that explode writes: those whose L0 cell is unrefined.
+template <int RB>
+__global__ void __launch_bounds__(256) k_ring_plan(PoolGeom G, SurfacePlan plan, const int32_t* __restrict__ active,
+                                                   int nactive) {
+  using T = Tile<RB>;
+  constexpr int FB = T::FB, RB2 = T::RB2, LO = T::LO, HI = T::HI;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = t / (uint32_t)T::CELLS;
+  if (k >= (uint32_t)nactive) return;
+  const uint32_t r = t - k * (uint32_t)T::CELLS;
+  const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
+  if (fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI) return;
+  const int slot = active[1 + k];
+  const int id = G.slot_to_block[slot];
+  const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
+  const int gx = wrap1(bx * RB2 + fx - kGhost, 2 * G.nx);
+  const int gy = wrap1(by * RB2 + fy - kGhost, 2 * G.ny);
+  const int gz = wrap1(bz * RB2 + fz - kGhost, 2 * G.nz);
+  if (G.ccls[coarse_cell(G, gx >> 1, gy >> 1, gz >> 1)] & kCovered) return;
+  const uint32_t e = atomicAdd(&plan.count[2], 1u);
+  if (e < plan.cap_ring) plan.ring[e] = (uint32_t)slot * T::CELLS + r;
+}
+
+void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint32_t* near, uint32_t* ring) {
+  const uint64_t fb = 2 * rb + 2 * kGhost, inner = (uint64_t)(2 * rb) * (2 * rb) * (2 * rb);
+  const uint64_t shell = (uint64_t)rb * rb * rb - (uint64_t)(rb - 2) * (rb - 2) * (rb - 2);
+  *seam = (uint32_t)std::min<uint64_t>((uint64_t)max_slots * shell, ncells);
+  *near = ncells;
+  *ring = (uint32_t)((uint64_t)max_slots * (fb * fb * fb - inner));
+}
+
+// ---------------------------------------------------------------------------
 // pool step
 // ---------------------------------------------------------------------------
-template <int Q>
+template <int Q, int RB>
 __global__ void __launch_bounds__(256)
 k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __restrict__ mac_pool, PoolGeom G,
             CollideParams P, const int32_t* __restrict__ active, int nactive) {
   using L = Lattice<Q>;
-  const int FB = G.fb;
-  const uint32_t tile = (uint32_t)FB * FB * FB;
+  using T = Tile<RB>;
+  constexpr int FB = T::FB, RB2 = T::RB2, LO = T::LO, HI = T::HI;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t k = t / tile;
+  const uint32_t k = wave_tile<RB>(t);
   if (k >= (uint32_t)nactive) return;
-  const uint32_t r = t - k * tile;
+  const uint32_t r = t - k * (uint32_t)T::CELLS;
   const int slot = active[1 + k];
   const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
   const int id = G.slot_to_block[slot];
   const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
-  const int RB2 = 2 * G.rb;
-  const int lo = kGhost, hi = kGhost + RB2;
-  const int offx = fx <= lo ? -1 : (fx + 1 >= hi ? 1 : 0);
-  const int offy = fy <= lo ? -1 : (fy + 1 >= hi ? 1 : 0);
-  const int offz = fz <= lo ? -1 : (fz + 1 >= hi ? 1 : 0);
-  const int nbxw = wrapi(bx + offx, G.nbx), nbyw = wrapi(by + offy, G.nby), nbzw = wrapi(bz + offz, G.nbz);
+  const int offx = fx <= LO ? -1 : (fx + 1 >= HI ? 1 : 0);
+  const int offy = fy <= LO ? -1 : (fy + 1 >= HI ? 1 : 0);
+  const int offz = fz <= LO ? -1 : (fz + 1 >= HI ? 1 : 0);
+  const int nbxw = wrap1(bx + offx, G.nbx), nbyw = wrap1(by + offy, G.nby), nbzw = wrap1(bz + offz, G.nbz);
   int nX = -1, nY = -1, nZ = -1, nXY = -1, nXZ = -1, nYZ = -1, nXYZ = -1;
   if (offx) nX = G.block_slot[block_id(G, nbxw, by, bz)];
   if (offy) nY = G.block_slot[block_id(G, bx, nbyw, bz)];
@@ -87,9 +179,9 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     int sx = fx - L::ex[i], sy = fy - L::ey[i], sz = fz - L::ez[i];
-    const int ox = (sx < lo || sx >= hi) ? offx : 0;
-    const int oy = (sy < lo || sy >= hi) ? offy : 0;
-    const int oz = (sz < lo || sz >= hi) ? offz : 0;
+    const int ox = (sx < LO || sx >= HI) ? offx : 0;
+    const int oy = (sy < LO || sy >= HI) ? offy : 0;
+    const int oz = (sz < LO || sz >= HI) ? offz : 0;
     int ns = -1;
     if (ox && oy && oz) ns = nXYZ;
     else if (ox && oy) ns = nXY;
@@ -100,6 +192,7 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
     else if (oz) ns = nZ;
     int src_slot = slot;
     if (ns >= 0) {
+      // the source lies in a refined neighbour: its interior, same time level
       src_slot = ns;
       sx -= ox * RB2;
       sy -= oy * RB2;
@@ -108,98 +201,92 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
     sx = min(max(sx, 0), FB - 1);
     sy = min(max(sy, 0), FB - 1);
     sz = min(max(sz, 0), FB - 1);
-    g[i] = gin[(size_t)i * G.pool_plane + pool_cell(G, src_slot, sx, sy, sz)];
+    g[i] = *at(gin + (size_t)i * G.pool_plane, pool_cell<RB>(src_slot, sx, sy, sz));
   }
 
   float drho, rho, ux, uy, uz;
   moments<Q>(g, P.F, drho, rho, ux, uy, uz);
-  const uint32_t cell = pool_cell(G, slot, fx, fy, fz);
-  mac_pool[cell] = rho;
-  mac_pool[(size_t)G.pool_plane + cell] = ux;
-  mac_pool[2 * (size_t)G.pool_plane + cell] = uy;
-  mac_pool[3 * (size_t)G.pool_plane + cell] = uz;
-  const bool interior = fx >= lo && fx < hi && fy >= lo && fy < hi && fz >= lo && fz < hi;
+  const uint32_t cell = pool_cell<RB>(slot, fx, fy, fz);
+  *at(mac_pool, cell) = rho;
+  *at(mac_pool + G.pool_plane, cell) = ux;
+  *at(mac_pool + 2 * (size_t)G.pool_plane, cell) = uy;
+  *at(mac_pool + 3 * (size_t)G.pool_plane, cell) = uz;
+  const bool interior = fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI;
   // Ring cells advect and store without colliding (common_d3_amr_step1.wgsl
   // COLLIDE_RING = 0): coalesce averages uncollided interface states.
   if (interior) collide<Q>(g, P, drho, rho, ux, uy, uz);
 #pragma unroll
-  for (int i = 0; i < Q; i++) gout[(size_t)i * G.pool_plane + cell] = g[i];
+  for (int i = 0; i < Q; i++) *at(gout + (size_t)i * G.pool_plane, cell) = g[i];
 }
 
 // ---------------------------------------------------------------------------
 // explode: coarse state at t -> ring of the pool buffer the first substep reads
 // ---------------------------------------------------------------------------
-template <int Q>
+template <int Q, int RB>
 __global__ void __launch_bounds__(256)
-k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const int32_t* __restrict__ active,
-          int nactive, int linear) {
+k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const uint32_t* __restrict__ ring,
+          uint32_t nring, int linear) {
   using L = Lattice<Q>;
-  const int FB = G.fb;
-  const uint32_t tile = (uint32_t)FB * FB * FB;
+  using T = Tile<RB>;
+  constexpr int FB = T::FB, RB2 = T::RB2;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t k = t / tile;
-  if (k >= (uint32_t)nactive) return;
-  const uint32_t r = t - k * tile;
-  const int slot = active[1 + k];
+  if (t >= nring) return;
+  // a ring cell over an unrefined L0 cell (k_ring_plan)
+  const uint32_t cell = ring[t];
+  const int slot = (int)(cell / (uint32_t)T::CELLS);
+  const uint32_t r = cell - (uint32_t)slot * T::CELLS;
   const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
-  const int RB2 = 2 * G.rb;
-  const int lo = kGhost, hi = kGhost + RB2;
-  if (fx >= lo && fx < hi && fy >= lo && fy < hi && fz >= lo && fz < hi) return;  // ring only
   const int id = G.slot_to_block[slot];
   const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
   const int NFx = 2 * G.nx, NFy = 2 * G.ny, NFz = 2 * G.nz;
-  const int gx = wrapi(bx * RB2 + fx - kGhost, NFx);
-  const int gy = wrapi(by * RB2 + fy - kGhost, NFy);
-  const int gz = wrapi(bz * RB2 + fz - kGhost, NFz);
-  const int vx = gx / 2, vy = gy / 2, vz = gz / 2;
-  // a ring cell over a refined coarse cell is served by the neighbour tile
-  if (coarse_refined(G, vx, vy, vz)) return;
-  const uint32_t N = G.ncells;
+  const int gx = wrap1(bx * RB2 + fx - kGhost, NFx);
+  const int gy = wrap1(by * RB2 + fy - kGhost, NFy);
+  const int gz = wrap1(bz * RB2 + fz - kGhost, NFz);
+  const int vx = gx >> 1, vy = gy >> 1, vz = gz >> 1;
   const uint32_t vc = coarse_cell(G, vx, vy, vz);
-  const int xm = wrapi(vx - 1, G.nx), xp = wrapi(vx + 1, G.nx);
-  const int ym = wrapi(vy - 1, G.ny), yp = wrapi(vy + 1, G.ny);
-  const int zm = wrapi(vz - 1, G.nz), zp = wrapi(vz + 1, G.nz);
-  const bool okx = !coarse_refined(G, xm, vy, vz) && !coarse_refined(G, xp, vy, vz);
-  const bool oky = !coarse_refined(G, vx, ym, vz) && !coarse_refined(G, vx, yp, vz);
-  const bool okz = !coarse_refined(G, vx, vy, zm) && !coarse_refined(G, vx, vy, zp);
+  const uint32_t N = G.ncells;
+  const int xm = wrap1(vx - 1, G.nx), xp = wrap1(vx + 1, G.nx);
+  const int ym = wrap1(vy - 1, G.ny), yp = wrap1(vy + 1, G.ny);
+  const int zm = wrap1(vz - 1, G.nz), zp = wrap1(vz + 1, G.nz);
   const uint32_t cmx = coarse_cell(G, xm, vy, vz), cpx = coarse_cell(G, xp, vy, vz);
   const uint32_t cmy = coarse_cell(G, vx, ym, vz), cpy = coarse_cell(G, vx, yp, vz);
   const uint32_t cmz = coarse_cell(G, vx, vy, zm), cpz = coarse_cell(G, vx, vy, zp);
+  const bool okx = !((G.ccls[cmx] | G.ccls[cpx]) & kCovered);
+  const bool oky = !((G.ccls[cmy] | G.ccls[cpy]) & kCovered);
+  const bool okz = !((G.ccls[cmz] | G.ccls[cpz]) & kCovered);
   // position of the fine cell inside its coarse cell, in coarse units
   const float dx = ((float)(gx - 2 * vx) - 0.5f) * 0.5f;
   const float dy = ((float)(gy - 2 * vy) - 0.5f) * 0.5f;
   const float dz = ((float)(gz - 2 * vz) - 0.5f) * 0.5f;
-  const uint32_t cell = pool_cell(G, slot, fx, fy, fz);
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
-    const int tx = wrapi(vx + ex, G.nx), ty = wrapi(vy + ey, G.ny), tz = wrapi(vz + ez, G.nz);
     const float* plane = gc + (size_t)i * N;
     float out;
-    if (!coarse_refined(G, tx, ty, tz)) {
+    if (!covered_at(G, vx + ex, vy + ey, vz + ez)) {
       // The coarse target is unrefined: only a population that streams from
       // this ring cell into this tile's interior is kept (in-orphan, uniform
       // injection); everything else is zeroed (f = 0, i.e. g = -w).
-      const int dxg = wrapi(gx + ex, NFx), dyg = wrapi(gy + ey, NFy), dzg = wrapi(gz + ez, NFz);
-      const bool into = coarse_refined(G, dxg / 2, dyg / 2, dzg / 2) && dxg / RB2 == bx && dyg / RB2 == by &&
-                        dzg / RB2 == bz;
-      out = into ? plane[vc] : -L::w[i];
+      const int dxg = wrap1(gx + ex, NFx), dyg = wrap1(gy + ey, NFy), dzg = wrap1(gz + ez, NFz);
+      const bool into = covered_at(G, dxg >> 1, dyg >> 1, dzg >> 1) && (dxg >> (T::LOG_RB + 1)) == bx &&
+                        (dyg >> (T::LOG_RB + 1)) == by && (dzg >> (T::LOG_RB + 1)) == bz;
+      out = into ? *at(plane, vc) : -L::w[i];
     } else if (!linear) {
-      out = plane[vc];
+      out = *at(plane, vc);
     } else {
       // linear reconstruction d . G with the gradient projected
       // perpendicular to e_i (common_d3_amr_explode.wgsl:197-212)
-      float Gx = okx ? 0.5f * (plane[cpx] - plane[cmx]) : 0.f;
-      float Gy = oky ? 0.5f * (plane[cpy] - plane[cmy]) : 0.f;
-      float Gz = okz ? 0.5f * (plane[cpz] - plane[cmz]) : 0.f;
+      float Gx = okx ? 0.5f * (*at(plane, cpx) - *at(plane, cmx)) : 0.f;
+      float Gy = oky ? 0.5f * (*at(plane, cpy) - *at(plane, cmy)) : 0.f;
+      float Gz = okz ? 0.5f * (*at(plane, cpz) - *at(plane, cmz)) : 0.f;
       const float ee = (float)(ex * ex + ey * ey + ez * ez);
       const float proj = (Gx * ex + Gy * ey + Gz * ez) / ee;
       Gx -= ex * proj;
       Gy -= ey * proj;
       Gz -= ez * proj;
-      out = plane[vc] + (dx * Gx + dy * Gy + dz * Gz);
+      out = *at(plane, vc) + (dx * Gx + dy * Gy + dz * Gz);
     }
-    gp[(size_t)i * G.pool_plane + cell] = out;
+    *at(gp + (size_t)i * G.pool_plane, cell) = out;
   }
 }
 
@@ -207,78 +294,111 @@ k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, cons
 // coalesce: fine ring outflux -> time-t coarse cells under refined blocks,
 // plus the orphan correction on unrefined cells next to them
 // ---------------------------------------------------------------------------
-template <int Q>
+// mac projection of every covered L0 cell from its 8 children (rho mean,
+// mass-weighted u; common_d3_amr_coalesce.wgsl:245-261)
+template <int RB>
 __global__ void __launch_bounds__(256)
-k_coalesce(const float* __restrict__ gp, float* __restrict__ gc, float* __restrict__ mac,
-           const float* __restrict__ mac_pool, PoolGeom G, int orphans) {
+k_coalesce_mac(float* __restrict__ mac, const float* __restrict__ mac_pool, PoolGeom G,
+               const int32_t* __restrict__ active, int nactive) {
+  constexpr uint32_t per = RB * RB * RB;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = t / per;
+  if (k >= (uint32_t)nactive) return;
+  const uint32_t r = t - k * per;
+  const int slot = active[1 + k];
+  const int id = G.slot_to_block[slot];
+  const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
+  const int lx = r % RB, ly = (r / RB) % RB, lz = r / (RB * RB);
+  const int f0x = kGhost + 2 * lx, f0y = kGhost + 2 * ly, f0z = kGhost + 2 * lz;
+  float rs = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const uint32_t pc = pool_cell<RB>(slot, f0x + (c & 1), f0y + ((c >> 1) & 1), f0z + ((c >> 2) & 1));
+    const float rho = *at(mac_pool, pc);
+    rs += rho;
+    mx += rho * *at(mac_pool + G.pool_plane, pc);
+    my += rho * *at(mac_pool + 2 * (size_t)G.pool_plane, pc);
+    mz += rho * *at(mac_pool + 3 * (size_t)G.pool_plane, pc);
+  }
+  const float inv = 1.f / fmaxf(rs, 1e-6f);
+  const uint32_t cc = coarse_cell(G, bx * RB + lx, by * RB + ly, bz * RB + lz);
+  reinterpret_cast<float4*>(mac)[cc] = make_float4(rs * 0.125f, mx * inv, my * inv, mz * inv);
+}
+
+// Covered seam cells: for every direction whose target is unrefined, the
+// mean of the 8 ring cells covering the target (common_d3_amr_coalesce.wgsl:
+// 264-284).
+template <int Q, int RB>
+__global__ void __launch_bounds__(256)
+k_coalesce_seam(const float* __restrict__ gp, float* __restrict__ gc, PoolGeom G, const uint32_t* __restrict__ seam,
+                uint32_t nseam) {
   using L = Lattice<Q>;
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= G.ncells) return;
+  using T = Tile<RB>;
+  constexpr int FB = T::FB;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nseam) return;
+  const uint32_t c = seam[t];
   const int cx = c % G.nx, cy = (c / G.nx) % G.ny, cz = c / (G.nx * G.ny);
-  const int RB2 = 2 * G.rb;
+  const int bx = cx >> T::LOG_RB, by = cy >> T::LOG_RB, bz = cz >> T::LOG_RB;
+  const int slot = G.block_slot[block_id(G, bx, by, bz)];
+  const int NFx = 2 * G.nx, NFy = 2 * G.ny, NFz = 2 * G.nz;
+#pragma unroll
+  for (int i = 1; i < Q; i++) {
+    const int tx = cx + L::ex[i], ty = cy + L::ey[i], tz = cz + L::ez[i];
+    if (covered_at(G, tx, ty, tz)) continue;
+    // the target is one coarse cell outside the block: its 8 children are a
+    // 2x2x2 cube of this tile's ring
+    const int lx = local_in_tile<RB>(2 * tx, bx, NFx);
+    const int ly = local_in_tile<RB>(2 * ty, by, NFy);
+    const int lz = local_in_tile<RB>(2 * tz, bz, NFz);
+    const float* p = gp + (size_t)i * G.pool_plane;
+    const uint32_t b0 = pool_cell<RB>(slot, lx, ly, lz);
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) s += *at(p, b0 + (k & 1) + ((k >> 1) & 1) * FB + ((k >> 2) & 1) * FB * FB);
+    *at(gc + (size_t)i * G.ncells, c) = s * 0.125f;  // an average of g is the average of f minus w
+  }
+}
+
+// Orphans on unrefined L0 cells next to the refined region
+// (coalesceOrphans, common_d3_amr_coalesce.wgsl:126-211).
+template <int Q, int RB>
+__global__ void __launch_bounds__(256)
+k_orphans(const float* __restrict__ gp, float* __restrict__ gc, PoolGeom G, const uint32_t* __restrict__ near,
+          uint32_t nnear) {
+  using L = Lattice<Q>;
+  using T = Tile<RB>;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnear) return;
+  const uint32_t c = near[t];
+  const int cx = c % G.nx, cy = (c / G.nx) % G.ny, cz = c / (G.nx * G.ny);
   const int NFx = 2 * G.nx, NFy = 2 * G.ny, NFz = 2 * G.nz;
   const uint32_t N = G.ncells;
-  const int bx = cx / G.rb, by = cy / G.rb, bz = cz / G.rb;
-  const int slot = G.block_slot[block_id(G, bx, by, bz)];
-  if (slot >= 0) {
-    // moments projected from the 8 children (mass-weighted velocity)
-    const int f0x = kGhost + 2 * (cx - bx * G.rb), f0y = kGhost + 2 * (cy - by * G.rb),
-              f0z = kGhost + 2 * (cz - bz * G.rb);
-    float rs = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
-    for (int k = 0; k < 8; k++) {
-      const uint32_t pc = pool_cell(G, slot, f0x + (k & 1), f0y + ((k >> 1) & 1), f0z + ((k >> 2) & 1));
-      const float rho = mac_pool[pc];
-      rs += rho;
-      mx += rho * mac_pool[(size_t)G.pool_plane + pc];
-      my += rho * mac_pool[2 * (size_t)G.pool_plane + pc];
-      mz += rho * mac_pool[3 * (size_t)G.pool_plane + pc];
-    }
-    const float inv = 1.f / fmaxf(rs, 1e-6f);
-    reinterpret_cast<float4*>(mac)[c] = make_float4(rs * 0.125f, mx * inv, my * inv, mz * inv);
-#pragma unroll
-    for (int i = 0; i < Q; i++) {
-      const int tx = wrapi(cx + L::ex[i], G.nx), ty = wrapi(cy + L::ey[i], G.ny), tz = wrapi(cz + L::ez[i], G.nz);
-      if (coarse_refined(G, tx, ty, tz)) continue;
-      float s = 0.f;
-      for (int k = 0; k < 8; k++) {
-        const int px = 2 * tx + (k & 1), py = 2 * ty + ((k >> 1) & 1), pz = 2 * tz + ((k >> 2) & 1);
-        s += gp[(size_t)i * G.pool_plane +
-                pool_cell(G, slot, local_in_tile(px, bx, NFx, RB2), local_in_tile(py, by, NFy, RB2),
-                          local_in_tile(pz, bz, NFz, RB2))];
-      }
-      gc[(size_t)i * N + c] = s * 0.125f;  // an average of g is the average of f minus w
-    }
-    return;
-  }
-  if (!orphans) return;
-  bool near = false;
-  for (int i = 1; i < Q; i++)
-    near = near || coarse_refined(G, wrapi(cx + L::ex[i], G.nx), wrapi(cy + L::ey[i], G.ny), wrapi(cz + L::ez[i], G.nz));
-  if (!near) return;
 #pragma unroll
   for (int i = 1; i < Q; i++) {
     const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
-    const int tx = wrapi(cx + ex, G.nx), ty = wrapi(cy + ey, G.ny), tz = wrapi(cz + ez, G.nz);
-    if (coarse_refined(G, tx, ty, tz)) continue;
+    const int tx = wrap1(cx + ex, G.nx), ty = wrap1(cy + ey, G.ny), tz = wrap1(cz + ez, G.nz);
+    if (G.ccls[coarse_cell(G, tx, ty, tz)] & kCovered) continue;
     float s = 0.f;   // sum of f over the children found in refined tiles
     float nIn = 0.f;
+#pragma unroll 1
     for (int k = 0; k < 8; k++) {
       const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
-      const int px = wrapi(2 * tx + ox, NFx), py = wrapi(2 * ty + oy, NFy), pz = wrapi(2 * tz + oz, NFz);
-      const int qx = wrapi(px - ex, NFx), qy = wrapi(py - ey, NFy), qz = wrapi(pz - ez, NFz);
-      const int qbx = qx / RB2, qby = qy / RB2, qbz = qz / RB2;
+      const int px = 2 * tx + ox, py = 2 * ty + oy, pz = 2 * tz + oz;
+      const int qx = wrap1(px - ex, NFx), qy = wrap1(py - ey, NFy), qz = wrap1(pz - ez, NFz);
+      const int qbx = qx >> (T::LOG_RB + 1), qby = qy >> (T::LOG_RB + 1), qbz = qz >> (T::LOG_RB + 1);
       const int qs = G.block_slot[block_id(G, qbx, qby, qbz)];
       if (qs >= 0) {
-        s += gp[(size_t)i * G.pool_plane +
-                pool_cell(G, qs, local_in_tile(px, qbx, NFx, RB2), local_in_tile(py, qby, NFy, RB2),
-                          local_in_tile(pz, qbz, NFz, RB2))] + L::w[i];
+        s += *at(gp + (size_t)i * G.pool_plane,
+                 pool_cell<RB>(qs, local_in_tile<RB>(px, qbx, NFx), local_in_tile<RB>(py, qby, NFy),
+                               local_in_tile<RB>(pz, qbz, NFz))) + L::w[i];
       }
-      const int hx = wrapi(2 * cx + ox + ex, NFx), hy = wrapi(2 * cy + oy + ey, NFy), hz = wrapi(2 * cz + oz + ez, NFz);
-      if (coarse_refined(G, hx / 2, hy / 2, hz / 2)) nIn += 1.f;
+      const int hx = wrap1(2 * cx + ox + ex, NFx), hy = wrap1(2 * cy + oy + ey, NFy), hz = wrap1(2 * cz + oz + ez, NFz);
+      if (G.ccls[coarse_cell(G, hx >> 1, hy >> 1, hz >> 1)] & kCovered) nIn += 1.f;
     }
-    const float fc = gc[(size_t)i * N + c] + L::w[i];
+    const float fc = *at(gc + (size_t)i * N, c) + L::w[i];
     const float net = s - nIn * fc;
-    if (net != 0.f) gc[(size_t)i * N + c] = (fc + net * 0.125f) - L::w[i];
+    if (net != 0.f) *at(gc + (size_t)i * N, c) = (fc + net * 0.125f) - L::w[i];
   }
 }
 
@@ -298,23 +418,23 @@ __device__ __forceinline__ void coarse_sample(const float* __restrict__ gc, uint
   for (int i = 0; i < Q; i++) fneq[i] = g[i] - geq<Q>(i, rho, ux, uy, uz);
 }
 
-template <int Q>
+template <int Q, int RB>
 __global__ void __launch_bounds__(256)
 k_interp(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const int32_t* __restrict__ pairs,
          int npairs, float rescale) {
-  const int FB = G.fb;
-  const uint32_t tile = (uint32_t)FB * FB * FB;
+  using T = Tile<RB>;
+  constexpr int FB = T::FB;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t k = t / tile;
+  const uint32_t k = t / (uint32_t)T::CELLS;
   if (k >= (uint32_t)npairs) return;
-  const uint32_t r = t - k * tile;
+  const uint32_t r = t - k * (uint32_t)T::CELLS;
   const int id = pairs[1 + 2 * k], slot = pairs[2 + 2 * k];
   const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
   const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
   // coarse-unit position origin - 1/4 + (j - GHOST)/2 (fineToCoarseUnit3)
-  const float px = (float)(bx * G.rb) - 0.25f + 0.5f * (float)(fx - kGhost);
-  const float py = (float)(by * G.rb) - 0.25f + 0.5f * (float)(fy - kGhost);
-  const float pz = (float)(bz * G.rb) - 0.25f + 0.5f * (float)(fz - kGhost);
+  const float px = (float)(bx * RB) - 0.25f + 0.5f * (float)(fx - kGhost);
+  const float py = (float)(by * RB) - 0.25f + 0.5f * (float)(fy - kGhost);
+  const float pz = (float)(bz * RB) - 0.25f + 0.5f * (float)(fz - kGhost);
   const int c0x = (int)floorf(px), c0y = (int)floorf(py), c0z = (int)floorf(pz);
   const float tx = px - c0x, ty = py - c0y, tz = pz - c0z;
   float rho = 0.f, ux = 0.f, uy = 0.f, uz = 0.f, fneq[Q];
@@ -323,7 +443,7 @@ k_interp(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const
   for (int s = 0; s < 8; s++) {
     const int i = s & 1, j = (s >> 1) & 1, kk = (s >> 2) & 1;
     const float w = (i ? tx : 1.f - tx) * (j ? ty : 1.f - ty) * (kk ? tz : 1.f - tz);
-    const uint32_t cell = coarse_cell(G, wrapi(c0x + i, G.nx), wrapi(c0y + j, G.ny), wrapi(c0z + kk, G.nz));
+    const uint32_t cell = coarse_cell(G, wrap1(c0x + i, G.nx), wrap1(c0y + j, G.ny), wrap1(c0z + kk, G.nz));
     float sr, sx, sy, sz, sf[Q];
     coarse_sample<Q>(gc, G.ncells, cell, sr, sx, sy, sz, sf);
     rho += w * sr;
@@ -333,29 +453,29 @@ k_interp(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const
 #pragma unroll
     for (int q = 0; q < Q; q++) fneq[q] += w * sf[q];
   }
-  const uint32_t cell = pool_cell(G, slot, fx, fy, fz);
+  const uint32_t cell = pool_cell<RB>(slot, fx, fy, fz);
 #pragma unroll
   for (int q = 0; q < Q; q++) gp[(size_t)q * G.pool_plane + cell] = geq<Q>(q, rho, ux, uy, uz) + rescale * fneq[q];
 }
 
-template <int Q>
+template <int Q, int RB>
 __global__ void __launch_bounds__(256)
 k_average(const float* __restrict__ gp, float* __restrict__ gc, float* __restrict__ mac, PoolGeom G,
           const int32_t* __restrict__ pairs, int npairs, float rescale) {
-  const uint32_t per = (uint32_t)G.rb * G.rb * G.rb;
+  constexpr uint32_t per = RB * RB * RB;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k = t / per;
   if (k >= (uint32_t)npairs) return;
   const uint32_t r = t - k * per;
   const int id = pairs[1 + 2 * k], slot = pairs[2 + 2 * k];
-  const int lx = r % G.rb, ly = (r / G.rb) % G.rb, lz = r / (G.rb * G.rb);
+  const int lx = r % RB, ly = (r / RB) % RB, lz = r / (RB * RB);
   const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
   float rs = 0.f, mx = 0.f, my = 0.f, mz = 0.f, sneq[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) sneq[q] = 0.f;
   for (int c = 0; c < 8; c++) {
-    const uint32_t pc = pool_cell(G, slot, kGhost + 2 * lx + (c & 1), kGhost + 2 * ly + ((c >> 1) & 1),
-                                  kGhost + 2 * lz + ((c >> 2) & 1));
+    const uint32_t pc = pool_cell<RB>(slot, kGhost + 2 * lx + (c & 1), kGhost + 2 * ly + ((c >> 1) & 1),
+                                      kGhost + 2 * lz + ((c >> 2) & 1));
     float g[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) g[q] = gp[(size_t)q * G.pool_plane + pc];
@@ -372,7 +492,7 @@ k_average(const float* __restrict__ gp, float* __restrict__ gc, float* __restric
   const float rho = rs * 0.125f;
   const float inv = 1.f / fmaxf(rs, 1e-6f);
   const float ux = mx * inv, uy = my * inv, uz = mz * inv;
-  const uint32_t cc = coarse_cell(G, bx * G.rb + lx, by * G.rb + ly, bz * G.rb + lz);
+  const uint32_t cc = coarse_cell(G, bx * RB + lx, by * RB + ly, bz * RB + lz);
 #pragma unroll
   for (int q = 0; q < Q; q++) gc[(size_t)q * G.ncells + cc] = geq<Q>(q, rho, ux, uy, uz) + rescale * (sneq[q] * 0.125f);
   reinterpret_cast<float4*>(mac)[cc] = make_float4(rho, ux, uy, uz);
@@ -380,11 +500,12 @@ k_average(const float* __restrict__ gp, float* __restrict__ gc, float* __restric
 
 // ---------------------------------------------------------------------------
 // criterion: max |curl u| over each block's L0 box dilated by `lead`, from
-// the macroscopic field, central differences, one rounding per operation so
-// the host twin reproduces it bit for bit
+// the macroscopic field, central differences, each operation correctly
+// rounded so the host restatement reproduces it bit for bit
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float3 mac_u(const float4* mac, const PoolGeom& G, int x, int y, int z) {
-  const float4 m = mac[coarse_cell(G, wrapi(x, G.nx), wrapi(y, G.ny), wrapi(z, G.nz))];
+  const float4 m = mac[coarse_cell(G, ((x % G.nx) + G.nx) % G.nx, ((y % G.ny) + G.ny) % G.ny,
+                                   ((z % G.nz) + G.nz) % G.nz)];
   return make_float3(m.y, m.z, m.w);
 }
 
@@ -531,29 +652,27 @@ __global__ void __launch_bounds__(256) k_active_pairs(const int32_t* active, con
 // FP64 totals of the composite state: unrefined L0 cells + interior fine
 // cells weighted 1/8 (a fine cell is 1/8 of a coarse cell's volume)
 // ---------------------------------------------------------------------------
-template <int Q>
+template <int Q, int RB>
 __global__ void __launch_bounds__(256)
 k_pool_totals(const float* __restrict__ gc, const float* __restrict__ gp, PoolGeom G, const int32_t* __restrict__ active,
               double* partials, unsigned* counter, double* out) {
   using L = Lattice<Q>;
+  constexpr uint32_t rb2 = 2 * RB, per = rb2 * rb2 * rb2;
   __shared__ double lds[16 * 4];
   double v[4] = {0, 0, 0, 0};
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < G.ncells; c += stride) {
-    const int cx = c % G.nx, cy = (c / G.nx) % G.ny, cz = c / (G.nx * G.ny);
-    if (coarse_refined(G, cx, cy, cz)) continue;
+    if (G.ccls[c] & kCovered) continue;
     for (int i = 0; i < Q; i++) {
       const double f = (double)gc[(size_t)i * G.ncells + c] + (double)L::w[i];
       v[0] += f; v[1] += L::ex[i] * f; v[2] += L::ey[i] * f; v[3] += L::ez[i] * f;
     }
   }
   const int nact = active[0];
-  const uint32_t rb2 = 2 * G.rb;
-  const uint32_t per = rb2 * rb2 * rb2;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < (uint32_t)nact * per; t += stride) {
     const uint32_t k = t / per, r = t - k * per;
     const int slot = active[1 + k];
-    const uint32_t pc = pool_cell(G, slot, kGhost + r % rb2, kGhost + (r / rb2) % rb2, kGhost + r / (rb2 * rb2));
+    const uint32_t pc = pool_cell<RB>(slot, kGhost + r % rb2, kGhost + (r / rb2) % rb2, kGhost + r / (rb2 * rb2));
     for (int i = 0; i < Q; i++) {
       const double f = 0.125 * ((double)gp[(size_t)i * G.pool_plane + pc] + (double)L::w[i]);
       v[0] += f; v[1] += L::ex[i] * f; v[2] += L::ey[i] * f; v[3] += L::ez[i] * f;
@@ -564,52 +683,127 @@ k_pool_totals(const float* __restrict__ gc, const float* __restrict__ gp, PoolGe
 }
 
 // ---------------------------------------------------------------------------
-// launches
+// launches: dispatch on (q, rb)
 // ---------------------------------------------------------------------------
 static inline dim3 grid_for(uint64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+template <template <int, int> class Op, class... Args>
+static hipError_t dispatch_q_rb(int q, int rb, Args&&... args) {
+  switch (q * 100 + rb) {
+    case 1904: return Op<19, 4>::run(args...);
+    case 1908: return Op<19, 8>::run(args...);
+    case 1916: return Op<19, 16>::run(args...);
+    case 2704: return Op<27, 4>::run(args...);
+    case 2708: return Op<27, 8>::run(args...);
+    case 2716: return Op<27, 16>::run(args...);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int Q, int RB>
+struct OpCellClass {
+  static hipError_t run(uint8_t* ccls, const PoolGeom& g, const SurfacePlan& plan, const int32_t* active, int nactive,
+                        hipStream_t s) {
+    hipLaunchKernelGGL((k_cell_class<Q, RB>), grid_for(g.ncells), dim3(256), 0, s, ccls, g, plan);
+    if (nactive > 0)
+      hipLaunchKernelGGL((k_ring_plan<RB>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s, g, plan,
+                         active, nactive);
+    return hipGetLastError();
+  }
+};
+template <int Q, int RB>
+struct OpPoolStep {
+  static hipError_t run(const float* gin, float* gout, float* mac_pool, const PoolGeom& g, const CollideParams& p,
+                        const int32_t* active, int nactive, hipStream_t s) {
+    hipLaunchKernelGGL((k_pool_step<Q, RB>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s, gin,
+                       gout, mac_pool, g, p, active, nactive);
+    return hipGetLastError();
+  }
+};
+template <int Q, int RB>
+struct OpExplode {
+  static hipError_t run(const float* gc, float* gp, const PoolGeom& g, const SurfacePlan& plan, uint32_t nring,
+                        int linear, hipStream_t s) {
+    hipLaunchKernelGGL((k_explode<Q, RB>), grid_for(nring), dim3(256), 0, s, gc, gp, g, plan.ring, nring, linear);
+    return hipGetLastError();
+  }
+};
+template <int Q, int RB>
+struct OpCoalesce {
+  static hipError_t run(const float* gp, float* gc, float* mac, const float* mac_pool, const PoolGeom& g,
+                        const int32_t* active, int nactive, const SurfacePlan& plan, uint32_t nseam, uint32_t nnear,
+                        int orphans, hipStream_t s) {
+    if (nactive > 0)
+      hipLaunchKernelGGL((k_coalesce_mac<RB>), grid_for((uint64_t)nactive * RB * RB * RB), dim3(256), 0, s, mac,
+                         mac_pool, g, active, nactive);
+    if (nseam > 0)
+      hipLaunchKernelGGL((k_coalesce_seam<Q, RB>), grid_for(nseam), dim3(256), 0, s, gp, gc, g, plan.seam, nseam);
+    if (orphans && nnear > 0)
+      hipLaunchKernelGGL((k_orphans<Q, RB>), grid_for(nnear), dim3(256), 0, s, gp, gc, g, plan.near, nnear);
+    return hipGetLastError();
+  }
+};
+template <int Q, int RB>
+struct OpInterp {
+  static hipError_t run(const float* gc, float* gp, const PoolGeom& g, const int32_t* pairs, int npairs, float rescale,
+                        hipStream_t s) {
+    hipLaunchKernelGGL((k_interp<Q, RB>), grid_for((uint64_t)npairs * Tile<RB>::CELLS), dim3(256), 0, s, gc, gp, g,
+                       pairs, npairs, rescale);
+    return hipGetLastError();
+  }
+};
+template <int Q, int RB>
+struct OpAverage {
+  static hipError_t run(const float* gp, float* gc, float* mac, const PoolGeom& g, const int32_t* pairs, int npairs,
+                        float rescale, hipStream_t s) {
+    hipLaunchKernelGGL((k_average<Q, RB>), grid_for((uint64_t)npairs * RB * RB * RB), dim3(256), 0, s, gp, gc, mac, g,
+                       pairs, npairs, rescale);
+    return hipGetLastError();
+  }
+};
+template <int Q, int RB>
+struct OpTotals {
+  static hipError_t run(const float* gc, const float* gp, const PoolGeom& g, const int32_t* active, double* partials,
+                        unsigned* counter, double* out, hipStream_t s) {
+    hipLaunchKernelGGL((k_pool_totals<Q, RB>), dim3(256), dim3(256), 0, s, gc, gp, g, active, partials, counter, out);
+    return hipGetLastError();
+  }
+};
+
+hipError_t launch_cell_class(int q, uint8_t* ccls, const PoolGeom& g, const SurfacePlan& plan, const int32_t* active,
+                             int nactive, hipStream_t s) {
+  return dispatch_q_rb<OpCellClass>(q, g.rb, ccls, g, plan, active, nactive, s);
+}
 
 hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
                             const CollideParams& fine, const int32_t* active, int nactive, hipStream_t s) {
   if (nactive <= 0) return hipSuccess;
-  const dim3 grid = grid_for((uint64_t)nactive * g.fb * g.fb * g.fb);
-  if (q == 19) hipLaunchKernelGGL(k_pool_step<19>, grid, dim3(256), 0, s, gin, gout, mac_pool, g, fine, active, nactive);
-  else hipLaunchKernelGGL(k_pool_step<27>, grid, dim3(256), 0, s, gin, gout, mac_pool, g, fine, active, nactive);
-  return hipGetLastError();
+  return dispatch_q_rb<OpPoolStep>(q, g.rb, gin, gout, mac_pool, g, fine, active, nactive, s);
 }
 
-hipError_t launch_explode(int q, const float* gc, float* gp, const PoolGeom& g, const int32_t* active, int nactive,
-                          int linear, hipStream_t s) {
-  if (nactive <= 0) return hipSuccess;
-  const dim3 grid = grid_for((uint64_t)nactive * g.fb * g.fb * g.fb);
-  if (q == 19) hipLaunchKernelGGL(k_explode<19>, grid, dim3(256), 0, s, gc, gp, g, active, nactive, linear);
-  else hipLaunchKernelGGL(k_explode<27>, grid, dim3(256), 0, s, gc, gp, g, active, nactive, linear);
-  return hipGetLastError();
+hipError_t launch_explode(int q, const float* gc, float* gp, const PoolGeom& g, const SurfacePlan& plan,
+                          uint32_t nring, int linear, hipStream_t s) {
+  if (nring == 0) return hipSuccess;
+  return dispatch_q_rb<OpExplode>(q, g.rb, gc, gp, g, plan, nring, linear, s);
 }
 
 hipError_t launch_coalesce(int q, const float* gp, float* gc, float* mac, const float* mac_pool, const PoolGeom& g,
-                           int orphans, hipStream_t s) {
-  const dim3 grid = grid_for(g.ncells);
-  if (q == 19) hipLaunchKernelGGL(k_coalesce<19>, grid, dim3(256), 0, s, gp, gc, mac, mac_pool, g, orphans);
-  else hipLaunchKernelGGL(k_coalesce<27>, grid, dim3(256), 0, s, gp, gc, mac, mac_pool, g, orphans);
-  return hipGetLastError();
+                           const int32_t* active, int nactive, const SurfacePlan& plan, uint32_t nseam,
+                           uint32_t nnear, int orphans, hipStream_t s) {
+  return dispatch_q_rb<OpCoalesce>(q, g.rb, gp, gc, mac, mac_pool, g, active, nactive, plan, nseam, nnear, orphans,
+                                   s);
 }
 
 hipError_t launch_interp(int q, const float* gc, float* gp, const PoolGeom& g, const int32_t* pairs, int npairs,
                          float rescale, hipStream_t s) {
   if (npairs <= 0) return hipSuccess;
-  const dim3 grid = grid_for((uint64_t)npairs * g.fb * g.fb * g.fb);
-  if (q == 19) hipLaunchKernelGGL(k_interp<19>, grid, dim3(256), 0, s, gc, gp, g, pairs, npairs, rescale);
-  else hipLaunchKernelGGL(k_interp<27>, grid, dim3(256), 0, s, gc, gp, g, pairs, npairs, rescale);
-  return hipGetLastError();
+  return dispatch_q_rb<OpInterp>(q, g.rb, gc, gp, g, pairs, npairs, rescale, s);
 }
 
 hipError_t launch_average(int q, const float* gp, float* gc, float* mac, const PoolGeom& g, const int32_t* pairs,
                           int npairs, float rescale, hipStream_t s) {
   if (npairs <= 0) return hipSuccess;
-  const dim3 grid = grid_for((uint64_t)npairs * g.rb * g.rb * g.rb);
-  if (q == 19) hipLaunchKernelGGL(k_average<19>, grid, dim3(256), 0, s, gp, gc, mac, g, pairs, npairs, rescale);
-  else hipLaunchKernelGGL(k_average<27>, grid, dim3(256), 0, s, gp, gc, mac, g, pairs, npairs, rescale);
-  return hipGetLastError();
+  return dispatch_q_rb<OpAverage>(q, g.rb, gp, gc, mac, g, pairs, npairs, rescale, s);
 }
 
 hipError_t launch_criterion(const float* mac, float* crit, const PoolGeom& g, float lead, hipStream_t s) {
@@ -636,10 +830,7 @@ hipError_t launch_active_pairs(const int32_t* active, const int32_t* slot_to_blo
 
 hipError_t launch_pool_totals(int q, const float* gc, const float* gp, const PoolGeom& g, const int32_t* active,
                               double* partials, unsigned* counter, double* out, hipStream_t s) {
-  const dim3 grid(256);
-  if (q == 19) hipLaunchKernelGGL(k_pool_totals<19>, grid, dim3(256), 0, s, gc, gp, g, active, partials, counter, out);
-  else hipLaunchKernelGGL(k_pool_totals<27>, grid, dim3(256), 0, s, gc, gp, g, active, partials, counter, out);
-  return hipGetLastError();
+  return dispatch_q_rb<OpTotals>(q, g.rb, gc, gp, g, active, partials, counter, out, s);
 }
 
 }  // namespace lbm

@@ -101,5 +101,8 @@ def test_amr_refusals_without_gpu():
     d = pylbm.make_amr_desc(dict(base, nx=16))  # two blocks on x
     assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
     assert b"fewer than 3 blocks" in lib.lbm_last_error()
+    d = pylbm.make_amr_desc(dict(base, nx=36, ny=36, nz=36, rb=6))
+    assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"rb = 4, 8, 16" in lib.lbm_last_error()
     d = pylbm.make_amr_desc(dict(base, q=9))
     assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
