@@ -343,6 +343,225 @@ napi_value DeviceInfo(napi_env env, napi_callback_info) {
   return o;
 }
 
+// ---------------------------------------------------------------------------
+// Two-level AMR solve (lbm_amr_*)
+// ---------------------------------------------------------------------------
+lbm_amr* amr_handle(napi_env env, napi_value v) {
+  void* p = nullptr;
+  napi_get_value_external(env, v, &p);
+  return static_cast<lbm_amr*>(p);
+}
+
+// createAmr({q, nx, ny, nz, rb, maxSlots, omega, omegaMinus, force,
+//            manageEvery, refineAt, keepAt, lead, explodeLinear, orphans})
+napi_value CreateAmr(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 1) {
+    napi_throw_type_error(env, nullptr, "createAmr(desc)");
+    return nullptr;
+  }
+  napi_value o = argv[0], v;
+  lbm_amr_desc d;
+  memset(&d, 0, sizeof d);
+  d.abi_version = LBM_ABI_VERSION;
+  d.q = (int)num(env, o, "q", 27);
+  d.nx = (int)num(env, o, "nx", 1);
+  d.ny = (int)num(env, o, "ny", 1);
+  d.nz = (int)num(env, o, "nz", 1);
+  d.rb = (int)num(env, o, "rb", 8);
+  d.max_slots = (int)num(env, o, "maxSlots", 1);
+  d.omega = (float)num(env, o, "omega", 1.0);
+  d.collision = LBM_BGK;
+  if (get_prop(env, o, "omegaMinus", &v)) {
+    d.collision = LBM_TRT;
+    d.omega_minus = (float)num(env, o, "omegaMinus", d.omega);
+  }
+  double t3[3] = {0, 0, 0};
+  num_array(env, o, "force", t3, 3);
+  for (int k = 0; k < 3; k++) d.force[k] = (float)t3[k];
+  d.manage_every = (int)num(env, o, "manageEvery", 0);
+  d.refine_at = (float)num(env, o, "refineAt", 1e30);
+  d.keep_at = (float)num(env, o, "keepAt", 1e30);
+  d.lead = (int)num(env, o, "lead", 0);
+  d.explode_linear = (int)num(env, o, "explodeLinear", 1);
+  d.orphans = (int)num(env, o, "orphans", 1);
+  lbm_amr* a = nullptr;
+  int rc = lbm_amr_create(&d, &a);
+  if (rc) return throw_lbm(env, rc);
+  napi_value ext;
+  NAPI_OK(napi_create_external(env, a, nullptr, nullptr, &ext));
+  return ext;
+}
+
+napi_value AmrDestroy(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  lbm_amr_destroy(amr_handle(env, argv[0]));
+  return nullptr;
+}
+
+// amrUploadTopology(amr, Int32Array blockSlot, Int32Array slotToBlock) and
+// amrReadTopology(amr, Int32Array, Int32Array)
+template <bool Upload>
+napi_value AmrTopology(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void *bs, *sb;
+  size_t nb, ns;
+  if (argc < 3 || !typed(env, argv[1], napi_int32_array, &bs, &nb) || !typed(env, argv[2], napi_int32_array, &sb, &ns)) {
+    napi_throw_type_error(env, nullptr, "expected (amr, Int32Array blockSlot, Int32Array slotToBlock)");
+    return nullptr;
+  }
+  lbm_amr* a = amr_handle(env, argv[0]);
+  const int rc = Upload ? lbm_amr_upload_topology(a, static_cast<int32_t*>(bs), static_cast<int32_t*>(sb))
+                        : lbm_amr_read_topology(a, static_cast<int32_t*>(bs), static_cast<int32_t*>(sb));
+  if (rc) return throw_lbm(env, rc);
+  return nullptr;
+}
+
+// amrUploadMacro / amrReadMacro (amr, Float32Array)
+template <int (*Fn)(lbm_amr*, float*)>
+napi_value AmrFloat(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* data;
+  size_t len;
+  if (argc < 2 || !typed(env, argv[1], napi_float32_array, &data, &len)) {
+    napi_throw_type_error(env, nullptr, "expected (amr, Float32Array)");
+    return nullptr;
+  }
+  int rc = Fn(amr_handle(env, argv[0]), static_cast<float*>(data));
+  if (rc) return throw_lbm(env, rc);
+  return argv[1];
+}
+
+int amr_upload_macro(lbm_amr* a, float* m) { return lbm_amr_upload_macro(a, m); }
+
+napi_value AmrReadCriterion(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void *crit, *want;
+  size_t n1, n2;
+  if (argc < 3 || !typed(env, argv[1], napi_float32_array, &crit, &n1) || !typed(env, argv[2], napi_uint8_array, &want, &n2)) {
+    napi_throw_type_error(env, nullptr, "amrReadCriterion(amr, Float32Array, Uint8Array)");
+    return nullptr;
+  }
+  int rc = lbm_amr_read_criterion(amr_handle(env, argv[0]), static_cast<float*>(crit), static_cast<uint8_t*>(want));
+  if (rc) return throw_lbm(env, rc);
+  return nullptr;
+}
+
+napi_value AmrReadActive(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* data;
+  size_t len;
+  if (argc < 2 || !typed(env, argv[1], napi_int32_array, &data, &len)) {
+    napi_throw_type_error(env, nullptr, "amrReadActive(amr, Int32Array(maxSlots + 1))");
+    return nullptr;
+  }
+  int rc = lbm_amr_read_active(amr_handle(env, argv[0]), static_cast<int32_t*>(data));
+  if (rc) return throw_lbm(env, rc);
+  return argv[1];
+}
+
+napi_value AmrReadTotals(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  napi_value ab, arr;
+  void* data;
+  NAPI_OK(napi_create_arraybuffer(env, 32, &data, &ab));
+  int rc = lbm_amr_read_totals(amr_handle(env, argv[0]), static_cast<double*>(data));
+  if (rc) return throw_lbm(env, rc);
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, 4, ab, 0, &arr));
+  return arr;
+}
+
+napi_value AmrManageNow(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  int rc = lbm_amr_manage_now(amr_handle(env, argv[0]));
+  if (rc) return throw_lbm(env, rc);
+  return nullptr;
+}
+
+napi_value AmrInfo(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  int64_t step = 0;
+  int32_t act = 0, exh = 0;
+  int rc = lbm_amr_info(amr_handle(env, argv[0]), &step, &act, &exh);
+  if (rc) return throw_lbm(env, rc);
+  napi_value o, v;
+  NAPI_OK(napi_create_object(env, &o));
+  napi_create_double(env, (double)step, &v);
+  napi_set_named_property(env, o, "step", v);
+  napi_create_int32(env, act, &v);
+  napi_set_named_property(env, o, "activeTiles", v);
+  napi_create_int32(env, exh, &v);
+  napi_set_named_property(env, o, "exhausted", v);
+  return o;
+}
+
+struct AmrStepWork {
+  napi_async_work work;
+  napi_deferred deferred;
+  lbm_amr* a;
+  int n;
+  int rc;
+  std::string err;
+};
+
+void AmrStepExecute(napi_env, void* p) {
+  AmrStepWork* w = static_cast<AmrStepWork*>(p);
+  w->rc = lbm_amr_step(w->a, w->n);
+  if (w->rc) w->err = lbm_last_error();
+}
+
+void AmrStepComplete(napi_env env, napi_status, void* p) {
+  AmrStepWork* w = static_cast<AmrStepWork*>(p);
+  if (w->rc) {
+    napi_value msg, err;
+    std::string m = "lbm error " + std::to_string(w->rc) + ": " + w->err;
+    napi_create_string_utf8(env, m.c_str(), m.size(), &msg);
+    napi_create_error(env, nullptr, msg, &err);
+    napi_reject_deferred(env, w->deferred, err);
+  } else {
+    napi_value undef;
+    napi_get_undefined(env, &undef);
+    napi_resolve_deferred(env, w->deferred, undef);
+  }
+  napi_delete_async_work(env, w->work);
+  delete w;
+}
+
+// amrStep(amr, n) -> Promise resolved once the n macro-steps have completed.
+napi_value AmrStep(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  AmrStepWork* w = new AmrStepWork();
+  w->a = amr_handle(env, argv[0]);
+  w->n = 1;
+  if (argc > 1) napi_get_value_int32(env, argv[1], &w->n);
+  napi_value promise, name;
+  NAPI_OK(napi_create_promise(env, &w->deferred, &promise));
+  NAPI_OK(napi_create_string_utf8(env, "lbm_amr_step", NAPI_AUTO_LENGTH, &name));
+  NAPI_OK(napi_create_async_work(env, nullptr, name, AmrStepExecute, AmrStepComplete, w, &w->work));
+  NAPI_OK(napi_queue_async_work(env, w->work));
+  return promise;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct Fn { const char* name; napi_callback cb; };
   const Fn fns[] = {
@@ -358,6 +577,18 @@ napi_value Init(napi_env env, napi_value exports) {
       {"step", Step},
       {"info", Info},
       {"deviceInfo", DeviceInfo},
+      {"createAmr", CreateAmr},
+      {"amrDestroy", AmrDestroy},
+      {"amrUploadTopology", AmrTopology<true>},
+      {"amrReadTopology", AmrTopology<false>},
+      {"amrUploadMacro", AmrFloat<amr_upload_macro>},
+      {"amrReadMacro", AmrFloat<lbm_amr_read_macro>},
+      {"amrReadCriterion", AmrReadCriterion},
+      {"amrReadActive", AmrReadActive},
+      {"amrReadTotals", AmrReadTotals},
+      {"amrManageNow", AmrManageNow},
+      {"amrInfo", AmrInfo},
+      {"amrStep", AmrStep},
   };
   for (const Fn& f : fns) {
     napi_value fn;

@@ -254,6 +254,79 @@ export class DenseSolver {
   }
 }
 
+// Two-level AMR solve (the reference's ?levels=2 explode/coalesce path,
+// main-3d.js:3122-3335) over the addon.  `scenario` is
+// { desc: {q, nx, ny, nz, rb, maxSlots, omega, omegaMinus, force,
+//          manageEvery, refineAt, keepAt, lead, explodeLinear, orphans},
+//   macro: () => Float32Array(4 * cells), topology: optional {blockSlot,
+//   slotToBlock} as d3-amr.mjs refineWhere builds them }.
+export class AmrSolver {
+  constructor(scenario) {
+    const a = loadAddon();
+    this.addon = a;
+    this.desc = scenario.desc;
+    this.macroFn = scenario.macro;
+    this.topology = scenario.topology || null;
+    this.NX = this.desc.nx; this.NY = this.desc.ny; this.NZ = this.desc.nz;
+    this.NCELLS = this.NX * this.NY * this.NZ;
+    this.Q = this.desc.q;
+    this.RB = this.desc.rb;
+    this.maxSlots = this.desc.maxSlots;
+    this.nBlocks = (this.NX / this.RB) * (this.NY / this.RB) * (this.NZ / this.RB);
+    this.handle = a.createAmr(this.desc);
+    this.step = 0;
+  }
+
+  destroy() {
+    if (this.handle) { this.addon.amrDestroy(this.handle); this.handle = null; }
+  }
+
+  // uploadLevelTopology + writeBuffer(mac) + initEq + interp seed
+  // (main-3d.js:822, :2973-3035).
+  reset() {
+    if (this.topology) this.addon.amrUploadTopology(this.handle, this.topology.blockSlot, this.topology.slotToBlock);
+    this.addon.amrUploadMacro(this.handle, this.macroFn());
+    this.step = 0;
+    return Promise.resolve();
+  }
+
+  getStep() { return this.step; }
+
+  async debugStepSync(n) {
+    await this.addon.amrStep(this.handle, n | 0);
+    this.step += n | 0;
+    const info = this.addon.amrInfo(this.handle);
+    return { step: this.step, exhausted: info.exhausted > 0 };
+  }
+
+  async manageNow() { this.addon.amrManageNow(this.handle); }
+
+  async readMacro() {
+    const out = new Float32Array(4 * this.NCELLS);
+    this.addon.amrReadMacro(this.handle, out);
+    return out;
+  }
+
+  // Block maps, criterion and active list (the reference's debugPoolState).
+  async debugPoolState() {
+    const blockSlot = new Int32Array(this.nBlocks), slotToBlock = new Int32Array(this.maxSlots);
+    this.addon.amrReadTopology(this.handle, blockSlot, slotToBlock);
+    const crit = new Float32Array(this.nBlocks), want = new Uint8Array(this.nBlocks);
+    this.addon.amrReadCriterion(this.handle, crit, want);
+    const act = new Int32Array(this.maxSlots + 1);
+    this.addon.amrReadActive(this.handle, act);
+    const info = this.addon.amrInfo(this.handle);
+    return { blockSlot: blockSlot, slotToBlock: slotToBlock, crit: crit, want: want,
+      active: act.slice(0, 1 + act[0]), activeTiles: info.activeTiles, exhausted: info.exhausted };
+  }
+
+  // FP64 composite mass and momentum (unrefined L0 cells + fine interiors / 8).
+  async readStats() {
+    const t = this.addon.amrReadTotals(this.handle);
+    return { step: this.step, mass: t[0], px: t[1], py: t[2], pz: t[3] };
+  }
+}
+
 export function deviceInfo() { return loadAddon().deviceInfo(); }
 
 export { velocities, weights, SUPPORTED_Q };

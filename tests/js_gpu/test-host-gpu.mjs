@@ -2,7 +2,8 @@
 // liblbm.so, checked against the FP64 oracle run in the same process.
 // Run: node tests/js_gpu/test-host-gpu.mjs   (needs an MI355X)
 import { createRequire } from 'module';
-import { DenseSolver, sphereScenario, tgvScenario, deviceInfo } from '../../getting-air_amd/js/lbm-host.mjs';
+import { DenseSolver, AmrSolver, sphereScenario, tgvScenario, deviceInfo } from '../../getting-air_amd/js/lbm-host.mjs';
+import { makePool, refineWhere, manageSlots, activeSlotList } from '../../getting-air_amd/js/d3-amr.mjs';
 
 const require = createRequire(import.meta.url);
 const O = require('../../oracle/lbm_oracle.js');
@@ -79,6 +80,54 @@ function relErr(gpuMac, oracleMac, comp) {
     const e = Math.sqrt(num / den);
     assert(e < 1e-5, 'u err ' + e);
     return 'u ' + e.toExponential(2);
+  });
+
+  await ok('AMR (D3Q27, 2 levels): block maps and active list follow d3-amr.mjs manageSlots bit for bit', async () => {
+    const n = 32, rb = 4, k = 2 * Math.PI / n, A = 0.02;
+    const macro = new Float32Array(4 * n * n * n);
+    for (let z = 0; z < n; z++) for (let y = 0; y < n; y++) for (let x = 0; x < n; x++) {
+      const c = 4 * ((z * n + y) * n + x);
+      const ux = A * Math.sin(k * x) * Math.cos(k * y) * Math.cos(k * z);
+      const uy = -A * Math.cos(k * x) * Math.sin(k * y) * Math.cos(k * z);
+      macro[c] = 1 - 1.5 * (ux * ux + uy * uy); macro[c + 1] = ux; macro[c + 2] = uy; macro[c + 3] = 0;
+    }
+    // thresholds on the scale A k sqrt(3); 64 of the 512 blocks reach 0.9 of it
+    const wmax = A * k * Math.sqrt(3);
+    const pool = makePool({ dims: [n, n, n], rb: rb, maxSlots: 128 });
+    const topo = refineWhere(pool, (b) => b.bx === 3 && b.by === 3);
+    const s = new AmrSolver({
+      desc: { q: 27, nx: n, ny: n, nz: n, rb: rb, maxSlots: 128, omega: 1 / 0.6, manageEvery: 2, lead: 0,
+        refineAt: 0.9 * wmax, keepAt: 0.85 * wmax },
+      macro: () => macro, topology: topo,
+    });
+    await s.reset();
+    let prev = await s.debugPoolState();
+    assert(prev.activeTiles === topo.activeSlots, 'seeded tiles ' + prev.activeTiles);
+    const st0 = await s.readStats();
+    let changes = 0;
+    for (let step = 0; step < 6; step++) {
+      await s.debugStepSync(1);
+      if (step % 2 !== 0) continue;
+      const cur = await s.debugPoolState();
+      for (let b = 0; b < pool.nBlocks; b++) {
+        const held = prev.blockSlot[b] >= 0;
+        const w = held ? cur.crit[b] >= Math.fround(0.85 * wmax) : cur.crit[b] >= Math.fround(0.9 * wmax);
+        assert((w ? 1 : 0) === cur.want[b], 'want of block ' + b + ' at step ' + step);
+      }
+      const twin = manageSlots(prev.blockSlot, prev.slotToBlock, cur.want);
+      for (let b = 0; b < pool.nBlocks; b++) assert(twin.blockSlot[b] === cur.blockSlot[b], 'blockSlot ' + b);
+      for (let q = 0; q < 128; q++) assert(twin.slotToBlock[q] === cur.slotToBlock[q], 'slotToBlock ' + q);
+      const act = activeSlotList(twin.slotToBlock);
+      assert(act.length === cur.active.length && act.every((v, i) => v === cur.active[i]), 'active list');
+      changes += twin.coarsened.length + twin.refined.length;
+      prev = cur;
+    }
+    const st = await s.readStats();
+    s.destroy();
+    assert(changes > 0, 'no topology change exercised');
+    const dm = Math.abs(st.mass - st0.mass) / st0.mass;
+    assert(dm < 1e-6, 'mass drift ' + dm);
+    return changes + ' topology changes, ' + prev.activeTiles + ' tiles, mass drift ' + dm.toExponential(2);
   });
 
   await ok('invalid descriptor raises with the library message', () => {
