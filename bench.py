@@ -125,6 +125,53 @@ def measure_amr(steps, manage_every=50):
             "refine_at": adesc["refineAt"], "keep_at": adesc["keepAt"], "tau": 0.52}
 
 
+def measure_serve(count, rate=50.0, seed=5, steps=2000, budget=200e9, scale=1.0):
+    """Config-5 serving: `count` requests with seeded Poisson arrivals at
+    `rate`/s (2D cylinder channels nx 256-1024 and 3D boxed spheres 64^3-192^3,
+    Re uniform in [50, 500], shifted fp16 storage), submitted at their arrival
+    times to the batched serve loop; then the same requests one at a time
+    through the plain single-solve path (create, upload, step, read back,
+    destroy).  Both paths compute bit-identical results (tests/test_serve_gpu.py)."""
+    import threading
+    mix = sc.serve_mix(count=count, rate=rate, seed=seed, steps=steps, storage="f16", scale=scale)
+    cells = [d["nx"] * d["ny"] * (1 if d["q"] == 9 else d["nz"]) for _, d, _, _, _ in mix]
+    updates = float(sum(c * st for c, (_, _, _, _, st) in zip(cells, mix)))
+    with pylbm.Server(budget_bytes=budget, chunk_steps=32, autotune=1) as sv:
+        tickets = []
+
+        def submitter():
+            t0 = sv.clock()
+            for arrival, d, m, _, st in mix:
+                dt = arrival - (sv.clock() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                tickets.append(sv.submit(d, m, st))
+
+        th = threading.Thread(target=submitter)
+        t_start = time.perf_counter()
+        th.start()
+        th.join()
+        sv.drain(-1)
+        wall = time.perf_counter() - t_start
+        st = sv.stats()
+        finite = all(np.isfinite(sv.read_macro(t)).all() for t in tickets[:: max(1, len(tickets) // 16)])
+    t_seq = time.perf_counter()
+    for _, d, m, _, nsteps in mix:
+        with pylbm.Solve(dict(d, autotune=0)) as s:
+            s.upload_macro(m)
+            s.step(nsteps)
+            s.read_macro()
+            s.read_forces(nsteps)
+    seq = time.perf_counter() - t_seq
+    return {"requests": count, "arrival_rate_per_s": rate, "last_arrival_s": mix[-1][0],
+            "served_wall_s": wall, "served_solves_per_s": count / wall, "served_mlups": updates / wall / 1e6,
+            "p50_latency_s": st["p50_latency"], "p99_latency_s": st["p99_latency"],
+            "peak_resident": st["peak_resident"], "peak_pool_bytes": st["peak_resident_bytes"],
+            "chunks": st["chunks"], "one_at_a_time_wall_s": seq, "one_at_a_time_solves_per_s": count / seq,
+            "one_at_a_time_mlups": updates / seq / 1e6, "speedup_solves_per_s": seq / wall,
+            "cell_updates": updates, "finite": bool(finite), "budget_bytes": budget}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +182,8 @@ def main():
     ap.add_argument("--no-16bit", action="store_true")
     ap.add_argument("--amr-steps", type=int, default=2000)
     ap.add_argument("--no-amr", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--tile", default=None,
                     help="fixed tile bx,by,flags (flags bit0 XCD swizzle, bit1 non-temporal) instead of autotuning")
     args = ap.parse_args()
@@ -170,6 +219,11 @@ def main():
         print("amr: %s" % json.dumps(r), file=sys.stderr, flush=True)
     else:
         extra_amr = None
+
+    extra_serve = None
+    if not args.no_serve:
+        extra_serve = measure_serve(args.serve_requests)
+        print("serve: %s" % json.dumps(extra_serve), file=sys.stderr, flush=True)
 
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -214,6 +268,8 @@ def main():
                   "kernel_ms": round(v["kernel_ms"], 5), "achieved_gbs": round(v["achieved_gbs"], 1),
                   "pct_hbm_peak": round(100.0 * v["achieved_gbs"] / HBM_PEAK_GBS, 2),
                   "cd_last200": v["cd_last200"]}
+    if extra_serve:
+        out["serve_config5"] = {k: (float("%.5g" % v) if isinstance(v, float) else v) for k, v in extra_serve.items()}
     if extra_amr:
         out["amr_d3q27_vortex_rings_128"] = {k: (float("%.5g" % v) if isinstance(v, float) else v)
                                              for k, v in extra_amr.items()}

@@ -57,6 +57,31 @@ struct StepArgs {
   int32_t ntx, nty;
   uint32_t nblocks;
   int32_t swizzle;     // 1: consecutive tiles on one XCD
+  // Step counter of the dither of 16-bit stores: this launch reads *salt_in
+  // and block 0 writes *salt_out = *salt_in + 1 for the next (the two words
+  // alternate with the lattice parity, so graph replays keep counting).
+  const uint32_t* salt_in;
+  uint32_t* salt_out;
+};
+
+// The serve loop's batched step: per-solve state in device memory.
+struct BatchSolve {
+  StepArgs args[2];      // by lattice parity p: f[p] -> f[p ^ 1], force buffers of that parity
+  void* f[2];            // the two lattices
+  float* mac;            // written on the solve's last step of the chunk
+  int32_t parity;        // lattice parity at launch 0 of the chunk
+  int32_t remaining;     // steps left at launch 0 (the solve idles once launch >= remaining)
+  int32_t pending;       // force partials of the step before launch 0 not yet summed
+  uint32_t step0;        // steps done at launch 0 (seeds the 16-bit dither)
+};
+struct BatchTile {
+  uint32_t solve;        // index into the solve table
+  uint32_t tile;         // tile of that solve, numbered as k_step numbers its blocks
+};
+struct BatchArgs {
+  const BatchSolve* solves;
+  const BatchTile* tiles;
+  int32_t launch;        // index of this launch within the chunk
 };
 
 // Sums the force partial slots in slot order and appends one record
@@ -86,6 +111,8 @@ struct ClassifyArgs {
 
 hipError_t launch_step(int q, int storage, bool general, bool nontemporal, const void* fin, void* fout,
                        const StepArgs& a, int bx, int by, hipStream_t s);
+hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, const BatchArgs& b, uint32_t ntiles,
+                             int bx, int by, hipStream_t s);
 hipError_t launch_force_finalize(const ForceFinalizeArgs& a, hipStream_t s);
 hipError_t launch_totals(int q, int storage, const void* f, const TotalsArgs& a, int nblocks, hipStream_t s);
 hipError_t launch_init_eq(int q, int storage, const float* mac, void* f, uint32_t ncells, hipStream_t s);

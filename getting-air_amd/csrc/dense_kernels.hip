@@ -49,22 +49,14 @@ __device__ __forceinline__ void sum_force_slots(const double* __restrict__ p, ui
   }
 }
 
+// One step of tile b of a solve: every thread of the block pulls, collides
+// and stores its cell; a tile with a force slot leaves its FP64 force there.
+// `salt` seeds the dither of 16-bit stores; `mac` (or null) receives the
+// macroscopic field.  Shared by the single-solve and the batched kernels.
 template <int Q, class ST, bool GEN, bool NT>
-__global__ void __launch_bounds__(256)
-k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout, StepArgs a) {
+__device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout,
+                                          const StepArgs& a, uint32_t b, uint32_t salt, float* mac) {
   using L = Lattice<Q>;
-  if constexpr (GEN) {
-    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    if (a.finalize_prev && blockIdx.x == 0 && tid < 64)
-      sum_force_slots(a.prev_partials, a.nslots, a.history, a.written, a.hist_cap, tid);
-  }
-  uint32_t b = blockIdx.x;
-  if (a.swizzle) {
-    // Hardware hands block b to XCD b % 8.  Re-number so each XCD walks one
-    // contiguous eighth of the tiles: neighbouring rows then share an L2.
-    const uint32_t per = a.nblocks >> 3;
-    if (b < per * 8u) b = (b & 7u) * per + (b >> 3);
-  }
   const int tx = b % a.ntx;
   const int rest = b / a.ntx;
   const int ty = rest % a.nty;
@@ -192,16 +184,16 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
     bool solid = false;
     if constexpr (GEN) solid = a.solid_eq && (c & 1);
 
-    if (a.mac) {
+    if (mac) {
       float4 m = make_float4(rho, ux, uy, uz);
       if (solid && a.solid_mac_rest) m = make_float4(1.f, 0.f, 0.f, 0.f);
-      reinterpret_cast<float4*>(a.mac)[cell] = m;
+      reinterpret_cast<float4*>(mac)[cell] = m;
     }
 
     if (solid) {
       // SOLID_EQ for a pinned body: feq(1, 0) = w_i, i.e. g = 0
 #pragma unroll
-      for (int i = 0; i < Q; i++) st<ST, NT>(gout + (size_t)i * N, cell, 0.f);
+      for (int i = 0; i < Q; i++) st<ST, NT>(gout + (size_t)i * N, cell, 0.f);  // exact in every format
     } else {
       float sp = 0.f;
       if constexpr (GEN) {
@@ -235,12 +227,14 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
       // grows linearly in time instead of as a random walk.
       const float om = a.omega, omm = a.omega_m;
       const float c0 = drho - 1.5f * rho * usq;
+      // random bits of this cell and step for the 16-bit stores
+      const uint32_t hsh = ST::kBytes == 2 ? cell_hash(cell, salt) : 0u;
       const float g3uF = 3.f * a.guo * uF;
       {
         const float w = L::w[0];
         float p = g[0] - om * (g[0] - w * c0) - w * g3uF;
         if constexpr (GEN) p += sp * (a.ae[13] - p);
-        st<ST, NT>(gout, cell, p);
+        st_sr<ST, NT>(gout, cell, p, ST::dither(hsh, 0));
       }
 #pragma unroll
       for (int k = 0; k < NP; k++) {
@@ -259,8 +253,8 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
           P += sp * (a.ae[k] - P);
           M += sp * (a.bo[k] - M);
         }
-        st<ST, NT>(gout + (size_t)i * N, cell, P + M);
-        st<ST, NT>(gout + (size_t)j * N, cell, P - M);
+        st_sr<ST, NT>(gout + (size_t)i * N, cell, P + M, ST::dither(hsh, (11 * (2 * k + 1)) % 17));
+        st_sr<ST, NT>(gout + (size_t)j * N, cell, P - M, ST::dither(hsh, (11 * (2 * k + 2)) % 17));
       }
     }
   }
@@ -280,6 +274,53 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
       }
     }
   }
+}
+
+template <int Q, class ST, bool GEN, bool NT>
+__global__ void __launch_bounds__(256)
+k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout, StepArgs a) {
+  if constexpr (GEN) {
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if (a.finalize_prev && blockIdx.x == 0 && tid < 64)
+      sum_force_slots(a.prev_partials, a.nslots, a.history, a.written, a.hist_cap, tid);
+  }
+  uint32_t b = blockIdx.x;
+  if (a.swizzle) {
+    // Hardware hands block b to XCD b % 8.  Re-number so each XCD walks one
+    // contiguous eighth of the tiles: neighbouring rows then share an L2.
+    const uint32_t per = a.nblocks >> 3;
+    if (b < per * 8u) b = (b & 7u) * per + (b >> 3);
+  }
+  uint32_t salt = 0;
+  if constexpr (ST::kBytes == 2) {
+    salt = *a.salt_in;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *a.salt_out = salt + 1;
+  }
+  step_tile<Q, ST, GEN, NT>(gin, gout, a, b, salt, a.mac);
+}
+
+// Many solves in one launch (the serve loop's packed pool): block k steps
+// tile tiles[k].tile of solve tiles[k].solve.  A solve steps while
+// launch < remaining; its lattice parity is its base parity flipped by the
+// launch index, so every launch of a captured chunk is one step of every
+// active solve.
+template <int Q, class ST, bool GEN, bool NT>
+__global__ void __launch_bounds__(256) k_step_batch(BatchArgs B) {
+  const BatchTile e = B.tiles[blockIdx.x];
+  const BatchSolve& S = B.solves[e.solve];
+  if (B.launch >= S.remaining) return;
+  const int p = S.parity ^ (B.launch & 1);
+  const StepArgs& a = S.args[p];
+  if constexpr (GEN) {
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if (e.tile == 0 && (B.launch > 0 || S.pending) && a.force_slot && tid < 64)
+      sum_force_slots(a.prev_partials, a.nslots, a.history, a.written, a.hist_cap, tid);
+  }
+  // the dither counter of the solve is its step count
+  const uint32_t salt = S.step0 + B.launch;
+  float* mac = B.launch + 1 == S.remaining ? S.mac : nullptr;
+  step_tile<Q, ST, GEN, NT>(static_cast<const typename ST::T*>(S.f[p]), static_cast<typename ST::T*>(S.f[p ^ 1]), a,
+                            e.tile, salt, mac);
 }
 
 // One wave: lane l sums slots l, l+64, ... in order, then a fixed butterfly.
@@ -426,6 +467,22 @@ template <int Q, class ST> struct OpStep {
     return hipGetLastError();
   }
 };
+template <int Q, class ST> struct OpStepBatch {
+  template <bool GEN, bool NT>
+  static void go(const BatchArgs& b, uint32_t ntiles, int bx, int by, hipStream_t s) {
+    hipLaunchKernelGGL((k_step_batch<Q, ST, GEN, NT>), dim3(ntiles), dim3(bx, by), 0, s, b);
+  }
+  static hipError_t run(bool general, bool nt, BatchArgs b, uint32_t ntiles, int bx, int by, hipStream_t s) {
+    if (general) {
+      if (nt) go<true, true>(b, ntiles, bx, by, s);
+      else go<true, false>(b, ntiles, bx, by, s);
+    } else {
+      if (nt) go<false, true>(b, ntiles, bx, by, s);
+      else go<false, false>(b, ntiles, bx, by, s);
+    }
+    return hipGetLastError();
+  }
+};
 template <int Q, class ST> struct OpInit {
   static hipError_t run(const float* mac, void* f, uint32_t N, hipStream_t s) {
     hipLaunchKernelGGL((k_init_eq<Q, ST>), cells_grid(N), dim3(256), 0, s, mac, (typename ST::T*)f, N);
@@ -454,6 +511,11 @@ template <int Q, class ST> struct OpTotals {
 hipError_t launch_step(int q, int storage, bool general, bool nontemporal, const void* fin, void* fout,
                        const StepArgs& a, int bx, int by, hipStream_t s) {
   return dispatch<OpStep>(q, storage, general, nontemporal, fin, fout, a, bx, by, s);
+}
+hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, const BatchArgs& b, uint32_t ntiles,
+                             int bx, int by, hipStream_t s) {
+  if (ntiles == 0) return hipSuccess;
+  return dispatch<OpStepBatch>(q, storage, general, nontemporal, b, ntiles, bx, by, s);
 }
 hipError_t launch_init_eq(int q, int storage, const float* mac, void* f, uint32_t N, hipStream_t s) {
   return dispatch<OpInit>(q, storage, mac, f, N, s);

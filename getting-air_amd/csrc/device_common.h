@@ -35,6 +35,8 @@ struct StoreF32 {
   __device__ __forceinline__ static void store(T* p, uint32_t i, float g) { *at(p, i) = g; }
   __device__ __forceinline__ static float load_nt(const T* p, uint32_t i) { return __builtin_nontemporal_load(at(p, i)); }
   __device__ __forceinline__ static void store_nt(T* p, uint32_t i, float g) { __builtin_nontemporal_store(g, at(p, i)); }
+  __device__ __forceinline__ static uint32_t dither(uint32_t, int) { return 0; }
+  __device__ __forceinline__ static T round_sr(float g, uint32_t) { return g; }
 };
 
 __device__ __forceinline__ float half_bits_to_float(uint16_t h) {
@@ -55,9 +57,20 @@ __device__ __forceinline__ uint16_t float_to_bf16_bits(float v) {
   return static_cast<uint16_t>(u >> 16);
 }
 
+// Stochastic rounding of the 16-bit formats: r, uniform over the bits that
+// the format drops, is added to the binary32 magnitude bits, and the sum is
+// truncated toward zero.  The expected stored value is then g itself, so the
+// rounding error has no systematic part even where the same g is stored step
+// after step (a steady flow), which round-to-nearest-even cannot avoid.
 struct StoreF16 {
   using T = uint16_t;
   static constexpr int kBytes = 2;
+  // 13 dropped bits for binary16 normals (f16 subnormals truncate a few more)
+  __device__ __forceinline__ static uint32_t dither(uint32_t h, int off) { return __builtin_amdgcn_ubfe(h, off, 13); }
+  __device__ __forceinline__ static T round_sr(float g, uint32_t r) {
+    const float d = __uint_as_float(__float_as_uint(g) + r);
+    return __builtin_bit_cast(uint16_t, __builtin_amdgcn_cvt_pkrtz(d, 0.f)[0]);
+  }
   __device__ __forceinline__ static float load(const T* p, uint32_t i) { return half_bits_to_float(*at(p, i)); }
   __device__ __forceinline__ static void store(T* p, uint32_t i, float g) { *at(p, i) = float_to_half_bits(g); }
   __device__ __forceinline__ static float load_nt(const T* p, uint32_t i) {
@@ -71,6 +84,10 @@ struct StoreF16 {
 struct StoreBF16 {
   using T = uint16_t;
   static constexpr int kBytes = 2;
+  __device__ __forceinline__ static uint32_t dither(uint32_t h, int off) { return __builtin_amdgcn_ubfe(h, off, 16); }
+  __device__ __forceinline__ static T round_sr(float g, uint32_t r) {
+    return static_cast<uint16_t>((__float_as_uint(g) + r) >> 16);
+  }
   __device__ __forceinline__ static float load(const T* p, uint32_t i) { return bf16_bits_to_float(*at(p, i)); }
   __device__ __forceinline__ static void store(T* p, uint32_t i, float g) { *at(p, i) = float_to_bf16_bits(g); }
   __device__ __forceinline__ static float load_nt(const T* p, uint32_t i) {
@@ -91,6 +108,24 @@ template <class ST, bool NT>
 __device__ __forceinline__ void st(typename ST::T* p, uint32_t i, float g) {
   if constexpr (NT) ST::store_nt(p, i, g);
   else ST::store(p, i, g);
+}
+// Store with stochastic rounding (16-bit formats); r from ST::dither.
+template <class ST, bool NT>
+__device__ __forceinline__ void st_sr(typename ST::T* p, uint32_t i, float g, uint32_t r) {
+  const typename ST::T v = ST::round_sr(g, r);
+  if constexpr (NT) __builtin_nontemporal_store(v, at(p, i));
+  else *at(p, i) = v;
+}
+
+// Per-cell, per-step 32-bit hash for the dither (a murmur3-style finalizer).
+__device__ __forceinline__ uint32_t cell_hash(uint32_t cell, uint32_t salt) {
+  uint32_t h = cell * 0x9E3779B1u + salt * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  return h;
 }
 
 // ---------------------------------------------------------------------------

@@ -106,6 +106,7 @@ struct lbm_solve {
   double* history = nullptr;
   unsigned long long* written = nullptr;
   double* totals = nullptr;
+  uint32_t* salt = nullptr;         // dither step counters, one per lattice parity
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int bx = 64, by = 1, swz = 1, nt = 0;
@@ -116,16 +117,23 @@ struct lbm_solve {
   size_t bytes = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   int graph_bx = 0, graph_by = 0, graph_swz = 0, graph_nt = 0;
+  Arena* arena = nullptr;           // pool the buffers come from (serve loop), or null
+  bool own_stream = true;
 };
 
 namespace {
 
 template <class T>
 int dalloc(lbm_solve* s, T** p, size_t bytes) {
-  hipError_t e = hipMalloc((void**)p, bytes);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return fail(LBM_E_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+  if (s->arena) {
+    *p = static_cast<T*>(s->arena->alloc(bytes));
+    if (!*p) return fail(LBM_E_NOMEM, "pool has no room for %zu bytes", bytes);
+  } else {
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess) {
+      *p = nullptr;
+      return fail(LBM_E_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
   }
   s->bytes += bytes;
   return LBM_OK;
@@ -133,7 +141,8 @@ int dalloc(lbm_solve* s, T** p, size_t bytes) {
 
 void dfree(lbm_solve* s, void* p, size_t bytes) {
   if (!p) return;
-  (void)hipFree(p);
+  if (s->arena) s->arena->release(p);
+  else (void)hipFree(p);
   s->bytes -= bytes;
 }
 
@@ -205,6 +214,8 @@ StepArgs step_args(lbm_solve* s, bool write_mac, bool with_force) {
   a.solid_eq = d.solid_eq;
   a.solid_mac_rest = s->dim == 3;
   a.mac = write_mac ? s->mac : nullptr;
+  a.salt_in = s->salt + s->parity;
+  a.salt_out = s->salt + (1 - s->parity);
   if (with_force && s->body && s->nslots) {
     // partial buffers alternate with the lattice parity
     const size_t stride = (size_t)s->nslots * 6;
@@ -439,12 +450,15 @@ void release(lbm_solve* s) {
   if (!s) return;
   drop_graphs(s);
   void* ptrs[] = {s->f[0], s->f[1], s->mac, s->cls, s->force_slot, s->force_partials, s->scratch,
-                  s->counters, s->history, s->written, s->totals};
-  for (void* p : ptrs)
-    if (p) (void)hipFree(p);
+                  s->counters, s->history, s->written, s->totals, s->salt};
+  for (void* p : ptrs) {
+    if (!p) continue;
+    if (s->arena) s->arena->release(p);
+    else (void)hipFree(p);
+  }
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+  if (s->stream && s->own_stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
 
@@ -455,6 +469,130 @@ int reset_history(lbm_solve* s) {
 }
 
 }  // namespace
+
+namespace {
+
+const int kTotalsBlocks = 1024;  // max totals partial blocks
+
+int create_impl(const lbm_desc* desc, Arena* arena, hipStream_t stream, int bx, int by, lbm_solve** out) {
+  if (!out) return fail(LBM_E_INVALID, "null output handle");
+  *out = nullptr;
+  int rc = validate(desc);
+  if (rc) return rc;
+  lbm_solve* s = new lbm_solve();
+  s->arena = arena;
+  s->d = *desc;
+  s->q = desc->q;
+  s->dim = desc->q == 9 ? 2 : 3;
+  s->storage = desc->storage;
+  s->elem = desc->storage == LBM_STORE_F32 ? 4 : 2;
+  s->N = (uint32_t)((size_t)desc->nx * desc->ny * desc->nz);
+  s->body = desc->body.shape != LBM_BODY_NONE;
+  const bool walls = desc->walls[0] || desc->walls[1] || (desc->walls[2] && s->dim == 3);
+  s->general = s->body || walls || desc->sponge_width > 0.f;
+  // default tile: one wave per x-row segment, XCD swizzle on
+  s->bx = 64; s->by = 1; s->swz = 1; s->nt = 0;
+  hipError_t e = hipSuccess;
+  if (stream) {
+    // a pool member: the pool's stream and the batched kernel's tile
+    s->stream = stream;
+    s->own_stream = false;
+    s->bx = bx; s->by = by; s->swz = 0;
+    s->d.autotune = 0;
+  } else {
+    e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { release(s); return hip_fail(e, "hipStreamCreate"); }
+  }
+  if ((e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess) {
+    release(s);
+    return hip_fail(e, "hipEventCreate");
+  }
+  const size_t lattice_bytes = (size_t)s->N * s->q * s->elem;
+  const int nred = kTotalsBlocks;
+  if ((rc = dalloc(s, &s->f[0], lattice_bytes)) || (rc = dalloc(s, &s->f[1], lattice_bytes)) ||
+      (rc = dalloc(s, &s->mac, (size_t)s->N * 16)) || (rc = dalloc(s, &s->scratch, (size_t)nred * 4 * 8)) ||
+      (rc = dalloc(s, &s->counters, 64)) || (rc = dalloc(s, &s->history, (size_t)kHistoryCap * 6 * 8)) ||
+      (rc = dalloc(s, &s->written, 8)) || (rc = dalloc(s, &s->totals, 64)) || (rc = dalloc(s, &s->salt, 8))) {
+    release(s);
+    return rc;
+  }
+  if ((e = hipMemsetAsync(s->counters, 0, 64, s->stream)) != hipSuccess ||
+      (e = hipMemsetAsync(s->written, 0, 8, s->stream)) != hipSuccess ||
+      (e = hipMemsetAsync(s->salt, 0, 8, s->stream)) != hipSuccess) {
+    release(s);
+    return hip_fail(e, "hipMemsetAsync");
+  }
+  if (s->body) {
+    if ((rc = dalloc(s, &s->cls, s->N)) || (rc = build_boundary(s))) {
+      release(s);
+      return rc;
+    }
+  }
+  if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) {
+    release(s);
+    return hip_fail(e, "hipStreamSynchronize");
+  }
+  *out = s;
+  return LBM_OK;
+}
+
+}  // namespace
+
+namespace lbm {
+
+size_t solve_footprint(const lbm_desc* d) {
+  const size_t n = (size_t)d->nx * d->ny * (d->q == 9 ? 1 : d->nz);
+  const size_t elem = d->storage == LBM_STORE_F32 ? 4 : 2;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t total = 2 * al(n * d->q * elem) + al(n * 16) + al((size_t)kTotalsBlocks * 4 * 8) + al(64) +
+                 al((size_t)kHistoryCap * 6 * 8) + al(8) + al(64) + al(8);
+  if (d->body.shape != LBM_BODY_NONE) {
+    // class bytes, one slot index per tile (tiles of at least 64 cells) and
+    // two parity sets of 6 partials per slot (at most one slot per tile)
+    const size_t tiles = (n + 63) / 64 + d->ny * (d->q == 9 ? 1 : d->nz);
+    total += al(n) + al(tiles * 4) + al(2 * tiles * 6 * 8);
+  }
+  return total;
+}
+
+int solve_create_in(const lbm_desc* d, Arena* arena, hipStream_t stream, int bx, int by, lbm_solve** out) {
+  if (!arena || !stream) return fail(LBM_E_INVALID, "pool solve needs an arena and a stream");
+  return create_impl(d, arena, stream, bx, by, out);
+}
+
+int solve_batch_entry(lbm_solve* s, BatchSolve* e, uint32_t* ntiles) {
+  const int keep = s->parity;
+  for (int p = 0; p < 2; p++) {
+    s->parity = p;
+    e->args[p] = step_args(s, false, s->body);
+    e->f[p] = s->f[p];
+  }
+  s->parity = keep;
+  e->mac = s->mac;
+  *ntiles = e->args[0].nblocks;
+  return LBM_OK;
+}
+
+int solve_advance(lbm_solve* s, int steps, bool finalize_force) {
+  s->step += steps;
+  s->parity ^= steps & 1;
+  if (finalize_force && s->body && s->nslots) {
+    // the partials of the last step live in the buffer of the parity it read
+    ForceFinalizeArgs fa{};
+    fa.partials = s->force_partials + (size_t)s->nslots * 6 * (1 - s->parity);
+    fa.nslots = s->nslots;
+    fa.history = s->history;
+    fa.written = s->written;
+    fa.capacity = kHistoryCap;
+    HIPCHK(launch_force_finalize(fa, s->stream));
+  }
+  s->initialised = true;
+  return LBM_OK;
+}
+
+uint32_t solve_cells(const lbm_solve* s) { return s->N; }
+
+}  // namespace lbm
 
 extern "C" {
 
@@ -473,56 +611,8 @@ int lbm_device_info(char* name, size_t name_len, double* hbm_bytes, int32_t* cu_
   return LBM_OK;
 }
 
-int lbm_solve_create(const lbm_desc* desc, lbm_solve** out) {
-  if (!out) return fail(LBM_E_INVALID, "null output handle");
-  *out = nullptr;
-  int rc = validate(desc);
-  if (rc) return rc;
-  lbm_solve* s = new lbm_solve();
-  s->d = *desc;
-  s->q = desc->q;
-  s->dim = desc->q == 9 ? 2 : 3;
-  s->storage = desc->storage;
-  s->elem = desc->storage == LBM_STORE_F32 ? 4 : 2;
-  s->N = (uint32_t)((size_t)desc->nx * desc->ny * desc->nz);
-  s->body = desc->body.shape != LBM_BODY_NONE;
-  const bool walls = desc->walls[0] || desc->walls[1] || (desc->walls[2] && s->dim == 3);
-  s->general = s->body || walls || desc->sponge_width > 0.f;
-  // default tile: one wave per x-row segment, XCD swizzle on
-  s->bx = 64; s->by = 1; s->swz = 1; s->nt = 0;
-  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) { release(s); return hip_fail(e, "hipStreamCreate"); }
-  if ((e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess) {
-    release(s);
-    return hip_fail(e, "hipEventCreate");
-  }
-  const size_t lattice_bytes = (size_t)s->N * s->q * s->elem;
-  const int nred = 1024;  // max totals partial blocks
-  if ((rc = dalloc(s, &s->f[0], lattice_bytes)) || (rc = dalloc(s, &s->f[1], lattice_bytes)) ||
-      (rc = dalloc(s, &s->mac, (size_t)s->N * 16)) || (rc = dalloc(s, &s->scratch, (size_t)nred * 4 * 8)) ||
-      (rc = dalloc(s, &s->counters, 64)) || (rc = dalloc(s, &s->history, (size_t)kHistoryCap * 6 * 8)) ||
-      (rc = dalloc(s, &s->written, 8)) || (rc = dalloc(s, &s->totals, 64))) {
-    release(s);
-    return rc;
-  }
-  if ((e = hipMemsetAsync(s->counters, 0, 64, s->stream)) != hipSuccess ||
-      (e = hipMemsetAsync(s->written, 0, 8, s->stream)) != hipSuccess) {
-    release(s);
-    return hip_fail(e, "hipMemsetAsync");
-  }
-  if (s->body) {
-    if ((rc = dalloc(s, &s->cls, s->N)) || (rc = build_boundary(s))) {
-      release(s);
-      return rc;
-    }
-  }
-  if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) {
-    release(s);
-    return hip_fail(e, "hipStreamSynchronize");
-  }
-  *out = s;
-  return LBM_OK;
-}
+
+int lbm_solve_create(const lbm_desc* desc, lbm_solve** out) { return create_impl(desc, nullptr, nullptr, 0, 0, out); }
 
 int lbm_solve_destroy(lbm_solve* s) {
   if (s) (void)hipStreamSynchronize(s->stream);

@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../include/lbm.h"
+#include "dense.h"
 
 namespace lbm {
 extern thread_local std::string g_last_error;
@@ -19,4 +20,24 @@ int solve_enqueue_covered_step(lbm_solve* s, const int32_t* cover, int rb, int n
 float* solve_lattice(lbm_solve* s, int which);
 float* solve_mac(lbm_solve* s);
 hipStream_t solve_stream(lbm_solve* s);
+
+// Hooks the serve loop uses.  A solve can live in a larger device pool: its
+// buffers then come from `Arena` instead of hipMalloc, and it runs on the
+// pool's stream with a fixed tile shape (the batched kernel's).
+struct Arena {
+  virtual ~Arena() = default;
+  virtual void* alloc(size_t bytes) = 0;  // null when the pool has no room
+  virtual void release(void* p) = 0;
+};
+// Device bytes a solve of this descriptor takes (what solve_create_in asks
+// the arena for, alignment included).
+size_t solve_footprint(const lbm_desc* d);
+int solve_create_in(const lbm_desc* d, Arena* arena, hipStream_t stream, int bx, int by, lbm_solve** out);
+// The solve's batch table entry (step arguments by lattice parity, lattice
+// and field pointers; the progress fields are the caller's) and its tiles.
+int solve_batch_entry(lbm_solve* s, BatchSolve* e, uint32_t* ntiles);
+// Host bookkeeping after `steps` batched steps: step count, lattice parity,
+// and the force history of the last step summed (enqueued on the stream).
+int solve_advance(lbm_solve* s, int steps, bool finalize_force);
+uint32_t solve_cells(const lbm_solve* s);
 }  // namespace lbm

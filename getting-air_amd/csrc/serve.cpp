@@ -1,0 +1,639 @@
+// Batched serving: the lbm_serve_* entry points of include/lbm.h.
+//
+// Many independent dense solves live in one packed device pool (one
+// allocation of the byte budget, first-fit with coalescing) and advance
+// together.  A host worker thread owns all device work:
+//
+//   loop:  admit queued requests that fit the pool (FIFO)
+//          for each lattice class (Q, storage, general): refresh the device
+//            solve table; rebuild the tile table and the captured chunk
+//            graph when the class's membership changed
+//          replay the chunk: K launches per class, launch j steps every
+//            member with j < remaining (k_step_batch)
+//          retire the members that finished: sum their last force record,
+//            copy the macroscopic field and the force history back, free
+//            their pool memory, wake waiters
+//
+// Requests that arrive while others run are admitted at the next chunk
+// boundary, so the device always holds the whole resident set's work.
+// A member's arithmetic depends only on its own data, its class's tile shape
+// and its own step count (the 16-bit dither is seeded by it), so its result
+// is the same alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/lbm.h"
+#include "dense.h"
+#include "engine.h"
+
+using namespace lbm;
+
+#define HIPCHK_RET(call)                             \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, #call); \
+  } while (0)
+
+namespace {
+
+// First-fit pool over one device allocation.
+class Pool : public Arena {
+ public:
+  int init(size_t bytes) {
+    hipError_t e = hipMalloc((void**)&base_, bytes);
+    if (e != hipSuccess) return fail(LBM_E_NOMEM, "pool hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    size_ = bytes;
+    free_[0] = bytes;
+    return LBM_OK;
+  }
+  ~Pool() override {
+    if (base_) (void)hipFree(base_);
+  }
+  void* alloc(size_t bytes) override {
+    std::lock_guard<std::mutex> g(mu_);
+    bytes = (bytes + 255) & ~(size_t)255;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, len = it->second;
+      free_.erase(it);
+      if (len > bytes) free_[off + bytes] = len - bytes;
+      used_[off] = bytes;
+      in_use_ += bytes;
+      peak_ = std::max(peak_, in_use_);
+      return base_ + off;
+    }
+    return nullptr;
+  }
+  void release(void* p) override {
+    std::lock_guard<std::mutex> g(mu_);
+    const size_t off = static_cast<char*>(p) - base_;
+    auto u = used_.find(off);
+    if (u == used_.end()) return;
+    size_t start = off, len = u->second;
+    in_use_ -= len;
+    used_.erase(u);
+    auto next = free_.lower_bound(start);
+    if (next != free_.end() && next->first == start + len) {
+      len += next->second;
+      next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == start) {
+        start = prev->first;
+        len += prev->second;
+        free_.erase(prev);
+      }
+    }
+    free_[start] = len;
+  }
+  size_t size() const { return size_; }
+  size_t in_use() const { return in_use_; }
+  size_t peak() const { return peak_; }
+
+ private:
+  char* base_ = nullptr;
+  size_t size_ = 0, in_use_ = 0, peak_ = 0;
+  std::map<size_t, size_t> free_;
+  std::unordered_map<size_t, size_t> used_;
+  std::mutex mu_;
+};
+
+struct Request {
+  int64_t ticket = 0;
+  lbm_desc d{};
+  std::vector<float> mac_in;
+  int steps = 0;
+  size_t footprint = 0;
+  uint32_t cells = 0;
+  double t_submit = 0, t_admit = 0, t_finish = 0;
+  int status = LBM_OK;
+  std::string error;
+  bool done = false;
+  std::vector<float> mac_out;
+  std::vector<double> forces;  // 6 per recorded step
+};
+
+struct Member {
+  std::shared_ptr<Request> r;
+  lbm_solve* s = nullptr;
+  BatchSolve entry{};
+  uint32_t ntiles = 0;
+  int done_steps = 0;
+  bool pending = false;  // force partials of the last step not yet summed
+};
+
+struct ClassKey {
+  int q, storage, general;
+  bool operator<(const ClassKey& o) const {
+    return std::tie(q, storage, general) < std::tie(o.q, o.storage, o.general);
+  }
+};
+
+struct Class {
+  ClassKey key{};
+  int bx = 64, by = 1, nt = 0;
+  std::vector<Member*> members;
+  bool dirty = true;  // membership changed since the tables were built
+  BatchSolve* d_solves = nullptr;
+  BatchTile* d_tiles = nullptr;
+  size_t cap_solves = 0, cap_tiles = 0;
+  uint32_t ntiles = 0;
+  std::vector<BatchSolve> h_solves;
+};
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct lbm_serve {
+  lbm_serve_desc d{};
+  int device = 0;
+  std::chrono::steady_clock::time_point t0;
+  Pool pool;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<Request>> queue;
+  std::map<int64_t, std::shared_ptr<Request>> requests;
+  std::vector<std::unique_ptr<Member>> resident;
+  std::map<ClassKey, std::unique_ptr<Class>> classes;
+  int64_t next_ticket = 1;
+  bool stop = false;
+  std::thread worker;
+  // graph of the current chunk (all classes); rebuilt when any class changes
+  hipGraphExec_t graph = nullptr;
+  // stats
+  int64_t submitted = 0, completed = 0, failed = 0;
+  double cell_updates = 0, first_submit = -1, last_finish = 0;
+  std::vector<double> latencies;
+  int peak_resident = 0, chunks = 0;
+  size_t resident_bytes = 0, peak_resident_bytes = 0;
+};
+
+namespace {
+
+void finish(lbm_serve* sv, const std::shared_ptr<Request>& r, int status, const std::string& err) {
+  std::lock_guard<std::mutex> g(sv->mu);
+  r->status = status;
+  r->error = err;
+  r->done = true;
+  r->t_finish = now_s(sv->t0);
+  if (status == LBM_OK) {
+    sv->completed++;
+    sv->cell_updates += (double)r->cells * r->steps;
+    sv->latencies.push_back(r->t_finish - r->t_submit);
+    sv->last_finish = std::max(sv->last_finish, r->t_finish);
+  } else {
+    sv->failed++;
+  }
+  sv->cv_done.notify_all();
+}
+
+// Tile shape of a lattice class: the dense autotuner run once on the class's
+// first request (its own grid), or one wave per row segment.
+void choose_tile(lbm_serve* sv, Class* c, const Request& r) {
+  if (!sv->d.autotune) return;
+  lbm_desc d = r.d;
+  d.autotune = 1;
+  lbm_solve* t = nullptr;
+  if (lbm_solve_create(&d, &t)) return;
+  int64_t step = 0, bytes = 0;
+  int32_t bx = 64, by = 1, flags = 0;
+  if (!lbm_upload_macro(t, r.mac_in.data()) && !lbm_step(t, 1) &&
+      !lbm_solve_info(t, &step, &bx, &by, &flags, &bytes)) {
+    c->bx = bx;
+    c->by = by;
+    c->nt = (flags >> 1) & 1;
+  }
+  lbm_solve_destroy(t);
+}
+
+// Admits queued requests that fit, in arrival order.
+void admit(lbm_serve* sv) {
+  for (;;) {
+    std::shared_ptr<Request> r;
+    {
+      std::lock_guard<std::mutex> g(sv->mu);
+      if (sv->queue.empty()) return;
+      if (sv->d.max_resident > 0 && (int)sv->resident.size() >= sv->d.max_resident) return;
+      if (sv->resident_bytes + sv->queue.front()->footprint > sv->pool.size()) return;
+      r = sv->queue.front();
+      sv->queue.pop_front();
+    }
+    const ClassKey key{r->d.q, r->d.storage,
+                       (r->d.body.shape != LBM_BODY_NONE || r->d.walls[0] || r->d.walls[1] ||
+                        (r->d.walls[2] && r->d.q != 9) || r->d.sponge_width > 0.f)
+                           ? 1
+                           : 0};
+    auto& cp = sv->classes[key];
+    if (!cp) {
+      cp = std::make_unique<Class>();
+      cp->key = key;
+      choose_tile(sv, cp.get(), *r);
+    }
+    auto m = std::make_unique<Member>();
+    m->r = r;
+    int rc = solve_create_in(&r->d, &sv->pool, sv->stream, cp->bx, cp->by, &m->s);
+    if (rc == LBM_E_NOMEM) {
+      // fragmentation: wait for a retirement and try again
+      std::lock_guard<std::mutex> g(sv->mu);
+      sv->queue.push_front(r);
+      return;
+    }
+    if (!rc) rc = lbm_upload_macro(m->s, r->mac_in.data());
+    if (!rc) rc = solve_batch_entry(m->s, &m->entry, &m->ntiles);
+    if (rc) {
+      const std::string err = lbm_last_error();
+      if (m->s) lbm_solve_destroy(m->s);
+      finish(sv, r, rc, err);
+      continue;
+    }
+    r->mac_in.clear();
+    r->mac_in.shrink_to_fit();
+    std::lock_guard<std::mutex> g(sv->mu);
+    r->t_admit = now_s(sv->t0);
+    sv->resident_bytes += r->footprint;
+    sv->peak_resident_bytes = std::max(sv->peak_resident_bytes, sv->resident_bytes);
+    cp->members.push_back(m.get());
+    cp->dirty = true;
+    sv->resident.push_back(std::move(m));
+    sv->peak_resident = std::max(sv->peak_resident, (int)sv->resident.size());
+  }
+}
+
+template <class T>
+int grow(T** p, size_t* cap, size_t n) {
+  if (n <= *cap) return LBM_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  const size_t c = std::max(n, *cap * 2);
+  hipError_t e = hipMalloc((void**)p, c * sizeof(T));
+  if (e != hipSuccess) return hip_fail(e, "serve table");
+  *cap = c;
+  return LBM_OK;
+}
+
+// Uploads every class's solve table for this chunk and, when membership
+// changed, its tile table.  Returns whether the chunk graph must be rebuilt.
+int prepare(lbm_serve* sv, bool* rebuild) {
+  *rebuild = sv->graph == nullptr;
+  for (auto& kv : sv->classes) {
+    Class* c = kv.second.get();
+    if (c->dirty) {
+      *rebuild = true;
+      std::vector<BatchTile> tiles;
+      for (uint32_t k = 0; k < c->members.size(); k++)
+        for (uint32_t t = 0; t < c->members[k]->ntiles; t++) tiles.push_back(BatchTile{k, t});
+      c->ntiles = (uint32_t)tiles.size();
+      int rc = grow(&c->d_tiles, &c->cap_tiles, std::max<size_t>(tiles.size(), 1));
+      if (rc) return rc;
+      if (!tiles.empty())
+        HIPCHK_RET(hipMemcpyAsync(c->d_tiles, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice,
+                                  sv->stream));
+      rc = grow(&c->d_solves, &c->cap_solves, std::max<size_t>(c->members.size(), 1));
+      if (rc) return rc;
+      c->dirty = false;
+    }
+    c->h_solves.resize(c->members.size());
+    for (size_t k = 0; k < c->members.size(); k++) {
+      Member* m = c->members[k];
+      BatchSolve e = m->entry;
+      e.parity = m->done_steps & 1;
+      e.remaining = m->r->steps - m->done_steps;
+      e.pending = m->pending ? 1 : 0;
+      e.step0 = (uint32_t)m->done_steps;
+      c->h_solves[k] = e;
+    }
+    if (!c->h_solves.empty())
+      HIPCHK_RET(hipMemcpyAsync(c->d_solves, c->h_solves.data(), c->h_solves.size() * sizeof(BatchSolve),
+                                hipMemcpyHostToDevice, sv->stream));
+  }
+  return LBM_OK;
+}
+
+int enqueue_chunk(lbm_serve* sv, int K) {
+  for (int j = 0; j < K; j++) {
+    for (auto& kv : sv->classes) {
+      Class* c = kv.second.get();
+      if (c->members.empty()) continue;
+      BatchArgs b{c->d_solves, c->d_tiles, j};
+      HIPCHK_RET(launch_step_batch(c->key.q, c->key.storage, c->key.general != 0, c->nt != 0, b, c->ntiles, c->bx,
+                                   c->by, sv->stream));
+    }
+  }
+  return LBM_OK;
+}
+
+int run_chunk(lbm_serve* sv) {
+  const int K = std::max(1, sv->d.chunk_steps);
+  bool rebuild = false;
+  int rc = prepare(sv, &rebuild);
+  if (rc) return rc;
+  if (rebuild) {
+    if (sv->graph) (void)hipGraphExecDestroy(sv->graph);
+    sv->graph = nullptr;
+    hipGraph_t g = nullptr;
+    HIPCHK_RET(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_chunk(sv, K);
+    hipError_t e = hipStreamEndCapture(sv->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "chunk capture");
+    e = hipGraphInstantiate(&sv->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "chunk instantiate");
+  }
+  HIPCHK_RET(hipGraphLaunch(sv->graph, sv->stream));
+  HIPCHK_RET(hipStreamSynchronize(sv->stream));
+  sv->chunks++;
+  // bookkeeping and retirement
+  std::vector<std::unique_ptr<Member>> keep, done;
+  std::vector<int> done_adv;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    for (auto& m : sv->resident) {
+      const int adv = std::min(m->r->steps - m->done_steps, K);
+      m->done_steps += adv;
+      if (m->done_steps >= m->r->steps) {
+        done_adv.push_back(adv);
+        done.push_back(std::move(m));
+      } else {
+        solve_advance(m->s, adv, false);
+        m->pending = m->entry.args[0].force_slot != nullptr;
+        keep.push_back(std::move(m));
+      }
+    }
+    sv->resident.swap(keep);
+    for (auto& kv : sv->classes) {
+      Class* c = kv.second.get();
+      std::vector<Member*> mem;
+      for (Member* m : c->members)
+        if (m->done_steps < m->r->steps) mem.push_back(m);
+      if (mem.size() != c->members.size()) {
+        c->members.swap(mem);
+        c->dirty = true;
+      }
+    }
+  }
+  for (size_t k = 0; k < done.size(); k++) {
+    Member& m = *done[k];
+    Request& r = *m.r;
+    int rc2 = solve_advance(m.s, done_adv[k], true);
+    if (!rc2) {
+      r.mac_out.resize((size_t)r.cells * 4);
+      rc2 = lbm_read_macro(m.s, r.mac_out.data());
+    }
+    if (!rc2 && m.entry.args[0].force_slot) {
+      const int maxr = std::min(r.steps, 1 << 16);
+      r.forces.resize((size_t)maxr * 6);
+      int32_t cnt = 0;
+      rc2 = lbm_read_forces(m.s, r.forces.data(), maxr, &cnt);
+      r.forces.resize((size_t)cnt * 6);
+    }
+    const std::string err = rc2 ? lbm_last_error() : "";
+    lbm_solve_destroy(m.s);
+    {
+      std::lock_guard<std::mutex> g(sv->mu);
+      sv->resident_bytes -= r.footprint;
+    }
+    finish(sv, m.r, rc2, err);
+  }
+  return LBM_OK;
+}
+
+void worker(lbm_serve* sv) {
+  (void)hipSetDevice(sv->device);
+  for (;;) {
+    admit(sv);
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      if (sv->resident.empty()) {
+        if (sv->stop) break;
+        sv->cv_work.wait(lk, [&] { return sv->stop || !sv->queue.empty(); });
+        continue;
+      }
+    }
+    const int rc = run_chunk(sv);
+    if (rc) {
+      // a device error ends every resident request with it
+      const std::string err = lbm_last_error();
+      std::vector<std::unique_ptr<Member>> all;
+      {
+        std::lock_guard<std::mutex> g(sv->mu);
+        all.swap(sv->resident);
+        for (auto& kv : sv->classes) {
+          kv.second->members.clear();
+          kv.second->dirty = true;
+        }
+        sv->resident_bytes = 0;
+      }
+      for (auto& m : all) {
+        lbm_solve_destroy(m->s);
+        finish(sv, m->r, rc, err);
+      }
+    }
+  }
+  // requests still queued when stopping
+  std::deque<std::shared_ptr<Request>> left;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    left.swap(sv->queue);
+  }
+  for (auto& r : left) finish(sv, r, LBM_E_STATE, "server stopped before the request ran");
+}
+
+std::shared_ptr<Request> find(lbm_serve* sv, int64_t ticket) {
+  auto it = sv->requests.find(ticket);
+  return it == sv->requests.end() ? nullptr : it->second;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_serve_create(const lbm_serve_desc* desc, lbm_serve** out) {
+  if (!out || !desc) return fail(LBM_E_INVALID, "null argument");
+  *out = nullptr;
+  if (desc->abi_version != LBM_ABI_VERSION) return fail(LBM_E_INVALID, "descriptor abi_version %d", desc->abi_version);
+  if (!(desc->budget_bytes >= 1 << 20)) return fail(LBM_E_INVALID, "budget_bytes %g below 1 MiB", desc->budget_bytes);
+  if (desc->max_resident < 0 || desc->chunk_steps < 0) return fail(LBM_E_INVALID, "negative max_resident / chunk_steps");
+  auto sv = std::make_unique<lbm_serve>();
+  sv->d = *desc;
+  if (sv->d.chunk_steps == 0) sv->d.chunk_steps = 32;
+  sv->t0 = std::chrono::steady_clock::now();
+  hipError_t e = hipGetDevice(&sv->device);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  int rc = sv->pool.init((size_t)desc->budget_bytes);
+  if (rc) return rc;
+  e = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  lbm_serve* p = sv.release();
+  p->worker = std::thread(worker, p);
+  *out = p;
+  return LBM_OK;
+}
+
+int lbm_serve_destroy(lbm_serve* sv) {
+  if (!sv) return LBM_OK;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->stop = true;
+    sv->cv_work.notify_all();
+  }
+  if (sv->worker.joinable()) sv->worker.join();
+  if (sv->graph) (void)hipGraphExecDestroy(sv->graph);
+  for (auto& kv : sv->classes) {
+    if (kv.second->d_solves) (void)hipFree(kv.second->d_solves);
+    if (kv.second->d_tiles) (void)hipFree(kv.second->d_tiles);
+  }
+  if (sv->stream) (void)hipStreamDestroy(sv->stream);
+  delete sv;
+  return LBM_OK;
+}
+
+int lbm_serve_submit(lbm_serve* sv, const lbm_desc* desc, const float* mac4, int32_t steps, int64_t* ticket) {
+  if (!sv || !desc || !mac4 || !ticket) return fail(LBM_E_INVALID, "null argument");
+  if (steps < 1) return fail(LBM_E_INVALID, "steps %d", steps);
+  // validate the descriptor now (lbm_solve_create would on the worker)
+  lbm_desc d = *desc;
+  d.autotune = 0;
+  if (d.abi_version != LBM_ABI_VERSION) return fail(LBM_E_INVALID, "descriptor abi_version %d", d.abi_version);
+  if (d.q != 9 && d.q != 19 && d.q != 27) return fail(LBM_E_INVALID, "q must be 9, 19 or 27 (got %d)", d.q);
+  if (d.nx < 1 || d.ny < 1 || d.nz < 1) return fail(LBM_E_INVALID, "grid %dx%dx%d", d.nx, d.ny, d.nz);
+  auto r = std::make_shared<Request>();
+  r->d = d;
+  r->steps = steps;
+  r->cells = (uint32_t)((size_t)d.nx * d.ny * (d.q == 9 ? 1 : d.nz));
+  r->footprint = solve_footprint(&d);
+  if (r->footprint > (size_t)sv->d.budget_bytes)
+    return fail(LBM_E_NOMEM, "request needs %zu bytes, more than the whole budget (%.0f)", r->footprint,
+                sv->d.budget_bytes);
+  r->mac_in.assign(mac4, mac4 + (size_t)r->cells * 4);
+  std::lock_guard<std::mutex> g(sv->mu);
+  r->ticket = sv->next_ticket++;
+  r->t_submit = now_s(sv->t0);
+  if (sv->first_submit < 0) sv->first_submit = r->t_submit;
+  sv->submitted++;
+  sv->requests[r->ticket] = r;
+  sv->queue.push_back(r);
+  sv->cv_work.notify_all();
+  *ticket = r->ticket;
+  return LBM_OK;
+}
+
+int lbm_serve_wait(lbm_serve* sv, int64_t ticket, double timeout_s, lbm_serve_result* out) {
+  if (!sv) return fail(LBM_E_INVALID, "null argument");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto r = find(sv, ticket);
+  if (!r) return fail(LBM_E_INVALID, "unknown ticket %lld", (long long)ticket);
+  auto ready = [&] { return r->done; };
+  if (timeout_s < 0) sv->cv_done.wait(lk, ready);
+  else if (!sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready))
+    return fail(LBM_E_STATE, "request %lld not finished within %g s", (long long)ticket, timeout_s);
+  if (out) {
+    memset(out, 0, sizeof *out);
+    out->ticket = r->ticket;
+    out->status = r->status;
+    out->steps = r->steps;
+    out->cells = r->cells;
+    out->t_submit = r->t_submit;
+    out->t_admit = r->t_admit;
+    out->t_finish = r->t_finish;
+    if (r->forces.size() >= 6)
+      for (int k = 0; k < 6; k++) out->force[k] = r->forces[r->forces.size() - 6 + k];
+  }
+  if (r->status) return fail(r->status, "request %lld: %s", (long long)ticket, r->error.c_str());
+  return LBM_OK;
+}
+
+int lbm_serve_read_macro(lbm_serve* sv, int64_t ticket, float* mac4) {
+  if (!sv || !mac4) return fail(LBM_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(sv->mu);
+  auto r = find(sv, ticket);
+  if (!r) return fail(LBM_E_INVALID, "unknown ticket %lld", (long long)ticket);
+  if (!r->done || r->status) return fail(LBM_E_STATE, "request %lld has no result", (long long)ticket);
+  std::copy(r->mac_out.begin(), r->mac_out.end(), mac4);
+  return LBM_OK;
+}
+
+int lbm_serve_read_forces(lbm_serve* sv, int64_t ticket, double* out6, int32_t max_records, int32_t* count) {
+  if (!sv || !count || (max_records > 0 && !out6)) return fail(LBM_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(sv->mu);
+  auto r = find(sv, ticket);
+  if (!r) return fail(LBM_E_INVALID, "unknown ticket %lld", (long long)ticket);
+  if (!r->done || r->status) return fail(LBM_E_STATE, "request %lld has no result", (long long)ticket);
+  const int32_t n = std::min<int32_t>((int32_t)(r->forces.size() / 6), std::max(max_records, 0));
+  std::copy(r->forces.end() - (size_t)n * 6, r->forces.end(), out6);
+  *count = n;
+  return LBM_OK;
+}
+
+int lbm_serve_release(lbm_serve* sv, int64_t ticket) {
+  if (!sv) return fail(LBM_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(sv->mu);
+  auto r = find(sv, ticket);
+  if (!r) return fail(LBM_E_INVALID, "unknown ticket %lld", (long long)ticket);
+  if (!r->done) return fail(LBM_E_STATE, "request %lld still running", (long long)ticket);
+  sv->requests.erase(ticket);
+  return LBM_OK;
+}
+
+int lbm_serve_drain(lbm_serve* sv, double timeout_s) {
+  if (!sv) return fail(LBM_E_INVALID, "null argument");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto idle = [&] { return sv->completed + sv->failed == sv->submitted; };
+  if (timeout_s < 0) sv->cv_done.wait(lk, idle);
+  else if (!sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), idle))
+    return fail(LBM_E_STATE, "%lld requests still running after %g s",
+                (long long)(sv->submitted - sv->completed - sv->failed), timeout_s);
+  return LBM_OK;
+}
+
+int lbm_serve_read_stats(lbm_serve* sv, lbm_serve_stats* out) {
+  if (!sv || !out) return fail(LBM_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(sv->mu);
+  memset(out, 0, sizeof *out);
+  out->submitted = sv->submitted;
+  out->completed = sv->completed;
+  out->failed = sv->failed;
+  out->resident = (int64_t)sv->resident.size();
+  out->queued = (int64_t)sv->queue.size();
+  out->cell_updates = sv->cell_updates;
+  out->first_submit = sv->first_submit;
+  out->last_finish = sv->last_finish;
+  if (!sv->latencies.empty()) {
+    std::vector<double> l = sv->latencies;
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double p) { return l[std::min(l.size() - 1, (size_t)(p * (l.size() - 1) + 0.5))]; };
+    out->p50_latency = pct(0.50);
+    out->p99_latency = pct(0.99);
+  }
+  const double span = sv->last_finish - sv->first_submit;
+  out->mlups = span > 0 ? sv->cell_updates / span / 1e6 : 0;
+  out->pool_bytes = (double)sv->pool.size();
+  out->peak_resident_bytes = (double)sv->pool.peak();
+  out->peak_resident = sv->peak_resident;
+  out->chunks = sv->chunks;
+  return LBM_OK;
+}
+
+double lbm_serve_clock(lbm_serve* sv) { return sv ? now_s(sv->t0) : 0.0; }
+
+}  // extern "C"

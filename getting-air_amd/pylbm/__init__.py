@@ -27,6 +27,8 @@ EXPORTS = (
     "lbm_amr_upload_macro", "lbm_amr_step", "lbm_amr_manage_now", "lbm_amr_read_macro",
     "lbm_amr_read_pool_macro", "lbm_amr_read_f", "lbm_amr_read_criterion", "lbm_amr_read_active",
     "lbm_amr_read_totals", "lbm_amr_info",
+    "lbm_serve_create", "lbm_serve_destroy", "lbm_serve_submit", "lbm_serve_wait", "lbm_serve_read_macro",
+    "lbm_serve_read_forces", "lbm_serve_release", "lbm_serve_drain", "lbm_serve_read_stats", "lbm_serve_clock",
 )
 GHOST = 2
 
@@ -55,6 +57,26 @@ class AmrDesc(ctypes.Structure):
         ("manage_every", ctypes.c_int32), ("refine_at", ctypes.c_float), ("keep_at", ctypes.c_float),
         ("lead", ctypes.c_int32), ("explode_linear", ctypes.c_int32), ("orphans", ctypes.c_int32),
     ]
+
+
+class ServeDesc(ctypes.Structure):
+    _fields_ = [("abi_version", ctypes.c_int32), ("max_resident", ctypes.c_int32), ("budget_bytes", ctypes.c_double),
+                ("chunk_steps", ctypes.c_int32), ("autotune", ctypes.c_int32)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_int64), ("status", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("cells", ctypes.c_uint32), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_finish", ctypes.c_double), ("force", ctypes.c_double * 6)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
+                ("resident", ctypes.c_int64), ("queued", ctypes.c_int64), ("cell_updates", ctypes.c_double),
+                ("first_submit", ctypes.c_double), ("last_finish", ctypes.c_double),
+                ("p50_latency", ctypes.c_double), ("p99_latency", ctypes.c_double), ("mlups", ctypes.c_double),
+                ("pool_bytes", ctypes.c_double), ("peak_resident_bytes", ctypes.c_double),
+                ("peak_resident", ctypes.c_int32), ("chunks", ctypes.c_int32)]
 
 
 class LbmError(RuntimeError):
@@ -112,6 +134,17 @@ def load():
         lib.lbm_amr_read_active.argtypes = [P, P]
         lib.lbm_amr_read_totals.argtypes = [P, P]
         lib.lbm_amr_info.argtypes = [P, i64p, i32p, i32p]
+        lib.lbm_serve_create.argtypes = [ctypes.POINTER(ServeDesc), ctypes.POINTER(P)]
+        lib.lbm_serve_destroy.argtypes = [P]
+        lib.lbm_serve_submit.argtypes = [P, ctypes.POINTER(Desc), P, ctypes.c_int32, i64p]
+        lib.lbm_serve_wait.argtypes = [P, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ServeResult)]
+        lib.lbm_serve_read_macro.argtypes = [P, ctypes.c_int64, P]
+        lib.lbm_serve_read_forces.argtypes = [P, ctypes.c_int64, P, ctypes.c_int32, i32p]
+        lib.lbm_serve_release.argtypes = [P, ctypes.c_int64]
+        lib.lbm_serve_drain.argtypes = [P, ctypes.c_double]
+        lib.lbm_serve_read_stats.argtypes = [P, ctypes.POINTER(ServeStats)]
+        lib.lbm_serve_clock.argtypes = [P]
+        lib.lbm_serve_clock.restype = ctypes.c_double
         _lib = lib
     return _lib
 
@@ -394,3 +427,81 @@ class Amr:
         step, act, exh = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
         _check(self.lib.lbm_amr_info(self.h, ctypes.byref(step), ctypes.byref(act), ctypes.byref(exh)))
         return {"step": step.value, "active_tiles": act.value, "exhausted": exh.value}
+
+
+class Server:
+    """Batched serve loop on the GPU (owns an lbm_serve handle)."""
+
+    def __init__(self, budget_bytes, max_resident=0, chunk_steps=32, autotune=0):
+        self.lib = load()
+        d = ServeDesc()
+        d.abi_version = ABI_VERSION
+        d.max_resident = int(max_resident)
+        d.budget_bytes = float(budget_bytes)
+        d.chunk_steps = int(chunk_steps)
+        d.autotune = int(autotune)
+        h = ctypes.c_void_p()
+        _check(self.lib.lbm_serve_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self.cells = {}
+
+    def close(self):
+        if self.h:
+            self.lib.lbm_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, desc_dict, mac4, steps):
+        desc = make_desc(desc_dict)
+        n = desc.nx * desc.ny * desc.nz
+        m = np.ascontiguousarray(mac4, dtype=np.float32).reshape(-1)
+        if m.size != 4 * n:
+            raise ValueError("mac needs %d floats, got %d" % (4 * n, m.size))
+        t = ctypes.c_int64()
+        _check(self.lib.lbm_serve_submit(self.h, ctypes.byref(desc), m.ctypes.data, int(steps), ctypes.byref(t)))
+        self.cells[t.value] = n
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        r = ServeResult()
+        _check(self.lib.lbm_serve_wait(self.h, int(ticket), float(timeout), ctypes.byref(r)))
+        return {"ticket": r.ticket, "status": r.status, "steps": r.steps, "cells": r.cells,
+                "t_submit": r.t_submit, "t_admit": r.t_admit, "t_finish": r.t_finish, "force": list(r.force)}
+
+    def read_macro(self, ticket):
+        out = np.empty(4 * self.cells[ticket], dtype=np.float32)
+        _check(self.lib.lbm_serve_read_macro(self.h, int(ticket), out.ctypes.data))
+        return out
+
+    def read_forces(self, ticket, max_records=1 << 16):
+        out = np.empty(6 * max_records, dtype=np.float64)
+        cnt = ctypes.c_int32()
+        _check(self.lib.lbm_serve_read_forces(self.h, int(ticket), out.ctypes.data, int(max_records),
+                                              ctypes.byref(cnt)))
+        return out[: 6 * cnt.value].reshape(-1, 6)
+
+    def release(self, ticket):
+        _check(self.lib.lbm_serve_release(self.h, int(ticket)))
+        self.cells.pop(ticket, None)
+
+    def drain(self, timeout=-1.0):
+        _check(self.lib.lbm_serve_drain(self.h, float(timeout)))
+
+    def stats(self):
+        st = ServeStats()
+        _check(self.lib.lbm_serve_read_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServeStats._fields_}
+
+    def clock(self):
+        return self.lib.lbm_serve_clock(self.h)

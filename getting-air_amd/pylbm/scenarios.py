@@ -234,3 +234,64 @@ def vortex_rings(n=128, radius=20.0, core=4.0, circulation=0.1, seed=7, jitter=8
     wmax = float(np.sqrt(w[0] ** 2 + w[1] ** 2 + w[2] ** 2).max())
     desc = {"q": q, "nx": n, "ny": n, "nz": n, "omega": f32(1.0 / tau)}
     return desc, mac.reshape(-1), {"vorticity_max": wmax}
+
+
+def cylinder2d(nx=256, u0=0.1, re=100.0, trt=True, storage="f32"):
+    """D2Q9 channel of nx x nx/4 cells with a pinned cylinder of diameter
+    D = ny/8 at (ny/2 + D, ny/2 + 1/2) (slightly off the mid-line so the wake
+    can shed), halfway bounce-back, freestream u0 seeded everywhere and a sponge
+    of width max(6, D) on the x faces relaxing toward it (the channel pages'
+    setup, main-channel.js): nu = u0 D / Re, tau = 3 nu + 1/2."""
+    ny = nx // 4
+    d_ = ny / 8.0
+    nu = u0 * d_ / re
+    tau = 3.0 * nu + 0.5
+    omega = f32(1.0 / tau)
+    desc = {
+        "q": 9, "nx": nx, "ny": ny, "nz": 1, "omega": omega,
+        "sponge": {"width": float(max(6, int(round(d_)))), "axes": [1, 0, 0], "u": [u0, 0.0, 0.0]},
+        "body": {"a": d_ / 2.0, "centre": [ny / 2.0 + d_, ny / 2.0 + 0.5]},
+        "solidEq": 1, "storage": storage,
+    }
+    if trt:
+        desc["omegaMinus"] = trt_omega_minus(omega)
+    n = nx * ny
+    mac = np.empty((n, 4), dtype=np.float32)
+    mac[:, 0] = 1.0 - (u0 * u0) / (2.0 * CS2)
+    mac[:, 1] = u0
+    mac[:, 2] = 0.0
+    mac[:, 3] = 0.0
+    meta = {"area": d_, "u0": u0, "nu": nu, "tau": tau, "re": re}
+    return desc, mac.reshape(-1), meta
+
+
+def boxed_sphere(n=64, u0=0.08, re=100.0, trt=True, storage="f32"):
+    """D3Q19 sphere of diameter n/4 in an n^3 box at (n/3, n/2, n/2), sponge
+    on every face toward the freestream (the boxed-sphere serving domain)."""
+    radius = n / 8.0
+    return sphere(nx=n, ny=n, nz=n, radius=radius, centre=(n / 3.0, n / 2.0, n / 2.0), u0=u0, re=re, trt=trt,
+                  storage=storage)
+
+
+def serve_mix(count=256, rate=50.0, seed=5, steps=2000, storage="f16", scale=1.0):
+    """Seeded serving workload: Poisson arrivals at `rate` per second; each
+    request is, with equal odds, a 2D cylinder channel with nx uniform in
+    [256, 1024] (multiples of 32) or a 3D boxed sphere with n uniform in
+    [64, 192] (multiples of 16); Re uniform in [50, 500].  `scale` shrinks the
+    grids for tests.  Returns [(arrival_s, desc, mac, meta, steps)]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(count):
+        t += rng.exponential(1.0 / rate)
+        re = float(rng.uniform(50.0, 500.0))
+        if rng.random() < 0.5:
+            nx = int(round(rng.uniform(256, 1024) * scale / 32.0)) * 32
+            nx = max(nx, 64)
+            desc, mac, meta = cylinder2d(nx=nx, re=re, storage=storage)
+        else:
+            n = int(round(rng.uniform(64, 192) * scale / 16.0)) * 16
+            n = max(n, 32)
+            desc, mac, meta = boxed_sphere(n=n, re=re, storage=storage)
+        out.append((t, desc, mac, meta, steps))
+    return out

@@ -220,6 +220,70 @@ int lbm_amr_read_totals(lbm_amr* a, double* out4);
 /* step count, tiles in use, wanted tiles left unplaced (sticky). */
 int lbm_amr_info(lbm_amr* a, int64_t* step, int32_t* active_tiles, int32_t* exhausted);
 
+
+/* ------------------------------------------------------------------------
+ * Batched serving.  No counterpart in the reference (it runs one solve per
+ * page); the north-star serving loop.  Many independent dense solves of
+ * uneven sizes stay resident in one packed device pool and advance together:
+ * every launch of the batched step kernel steps every resident solve of one
+ * lattice class (Q, storage) once.  A host worker thread admits queued
+ * requests as pool memory frees up (against the byte budget), runs
+ * graph-captured chunks of steps, retires finished solves between chunks and
+ * copies their results back.  A solve's result does not depend on what else
+ * is resident: tiles, rounding and force summation order are its own.
+ * ------------------------------------------------------------------------ */
+typedef struct lbm_serve_desc {
+  int32_t abi_version;   /* LBM_ABI_VERSION                                    */
+  int32_t max_resident;  /* resident solves at most (0: limited by the budget)  */
+  double budget_bytes;   /* device pool size: admission budget                 */
+  int32_t chunk_steps;   /* steps per captured chunk (admission granularity)   */
+  int32_t autotune;      /* 1: tile shape per lattice class by timing          */
+} lbm_serve_desc;
+
+typedef struct lbm_serve_result {
+  int64_t ticket;
+  int32_t status;        /* LBM_OK, or the request's error code                */
+  int32_t steps;
+  uint32_t cells;
+  double t_submit;       /* seconds on the server clock                        */
+  double t_admit;
+  double t_finish;       /* results copied back                                */
+  double force[6];       /* last step's [fx, fy, fz, tx, ty, tz] (FP64)        */
+} lbm_serve_result;
+
+typedef struct lbm_serve_stats {
+  int64_t submitted, completed, failed, resident, queued;
+  double cell_updates;   /* sum of cells x steps over completed solves         */
+  double first_submit, last_finish;  /* server clock, seconds                  */
+  double p50_latency, p99_latency;   /* submit -> finish over completed, s     */
+  double mlups;          /* cell_updates / (last_finish - first_submit) / 1e6  */
+  double pool_bytes, peak_resident_bytes;
+  int32_t peak_resident;
+  int32_t chunks;        /* chunks executed                                    */
+} lbm_serve_stats;
+
+typedef struct lbm_serve lbm_serve;
+
+int lbm_serve_create(const lbm_serve_desc* desc, lbm_serve** out);
+/* Stops the worker after the requests already running; queued ones fail. */
+int lbm_serve_destroy(lbm_serve* sv);
+/* Queues a solve: descriptor (autotune ignored), initial [rho, ux, uy, uz]
+ * per cell (copied), steps to run.  Refuses (LBM_E_NOMEM) a request larger
+ * than the whole budget. */
+int lbm_serve_submit(lbm_serve* sv, const lbm_desc* desc, const float* mac4, int32_t steps, int64_t* ticket);
+/* Waits up to timeout_s (< 0: forever) for a request; LBM_E_STATE on timeout. */
+int lbm_serve_wait(lbm_serve* sv, int64_t ticket, double timeout_s, lbm_serve_result* out);
+/* Results of a finished request: final macroscopic field, force history. */
+int lbm_serve_read_macro(lbm_serve* sv, int64_t ticket, float* mac4);
+int lbm_serve_read_forces(lbm_serve* sv, int64_t ticket, double* out6, int32_t max_records, int32_t* count);
+/* Drops a finished request's host-side results. */
+int lbm_serve_release(lbm_serve* sv, int64_t ticket);
+/* Waits until every submitted request has finished. */
+int lbm_serve_drain(lbm_serve* sv, double timeout_s);
+int lbm_serve_read_stats(lbm_serve* sv, lbm_serve_stats* out);
+/* Server clock (seconds since create), for callers scheduling arrivals. */
+double lbm_serve_clock(lbm_serve* sv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,3 +106,38 @@ def test_amr_refusals_without_gpu():
     assert b"rb = 4, 8, 16" in lib.lbm_last_error()
     d = pylbm.make_amr_desc(dict(base, q=9))
     assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
+
+
+def test_serve_struct_layouts_match_c():
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "lbm.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_serve_desc), offsetof(lbm_serve_desc, budget_bytes),
+         offsetof(lbm_serve_desc, autotune), sizeof(lbm_serve_result), offsetof(lbm_serve_result, t_submit),
+         offsetof(lbm_serve_result, force), sizeof(lbm_serve_stats), offsetof(lbm_serve_stats, peak_resident),
+         offsetof(lbm_serve_stats, p99_latency));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "t")
+        subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe])
+        got = [int(v) for v in subprocess.check_output([exe]).split()]
+    D, R, S = pylbm.ServeDesc, pylbm.ServeResult, pylbm.ServeStats
+    want = [ctypes.sizeof(D), D.budget_bytes.offset, D.autotune.offset, ctypes.sizeof(R), R.t_submit.offset,
+            R.force.offset, ctypes.sizeof(S), S.peak_resident.offset, S.p99_latency.offset]
+    assert got == want
+
+
+def test_serve_refusals_without_gpu():
+    lib = pylbm.load()
+    h = ctypes.c_void_p()
+    d = pylbm.ServeDesc()
+    d.abi_version = pylbm.ABI_VERSION
+    d.budget_bytes = 1000.0
+    assert lib.lbm_serve_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"budget" in lib.lbm_last_error()

@@ -1,0 +1,128 @@
+"""GPU tests of the batched serve loop (lbm_serve_* through the C ABI).
+
+- a request's result is bit-for-bit the same alone and in a mix of uneven
+  2D/3D, fp32/fp16 requests that arrive while others run;
+- a served solve equals the plain single-solve path bit for bit (same tile
+  shape, same rounding, same force summation order), so the serve loop
+  inherits the single-solve parity with the FP64 oracle;
+- admission respects the byte budget and a request larger than the budget
+  is refused.
+"""
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "getting-air_amd"))
+
+import pylbm  # noqa: E402
+from pylbm import scenarios as sc  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def mix():
+    return [
+        sc.cylinder2d(nx=256, re=120.0, storage="f16"),
+        sc.boxed_sphere(n=48, re=80.0, storage="f16"),
+        sc.cylinder2d(nx=320, re=60.0, storage="f32"),
+        sc.boxed_sphere(n=32, re=150.0, storage="f32"),
+        sc.cylinder2d(nx=192, re=300.0, storage="bf16"),
+        sc.boxed_sphere(n=64, re=200.0, storage="f16"),
+    ]
+
+
+def test_alone_equals_in_the_mix_bit_for_bit():
+    reqs = mix()
+    target = 1  # the 48^3 f16 sphere
+    desc, mac, _ = reqs[target]
+    say("alone")
+    with pylbm.Server(budget_bytes=2 << 30, chunk_steps=24) as sv:
+        t = sv.submit(desc, mac, 150)
+        sv.wait(t, 120)
+        alone_mac, alone_f = sv.read_macro(t), sv.read_forces(t)
+    say("mix")
+    with pylbm.Server(budget_bytes=2 << 30, chunk_steps=24) as sv:
+        tickets = {}
+        # three up front, the rest while those run (they join mid-flight)
+        for k in (0, 2, 3):
+            tickets[k] = sv.submit(reqs[k][0], reqs[k][1], 100 + 10 * k)
+        time.sleep(0.05)
+        for k in (1, 4, 5):
+            tickets[k] = sv.submit(reqs[k][0], reqs[k][1], 150 if k == target else 90)
+        sv.drain(300)
+        mix_mac, mix_f = sv.read_macro(tickets[target]), sv.read_forces(tickets[target])
+        st = sv.stats()
+        for k, t in tickets.items():
+            r = sv.wait(t, 1)
+            assert r["status"] == 0 and np.isfinite(sv.read_macro(t)).all(), k
+    say("stats", {k: st[k] for k in ("completed", "peak_resident", "chunks", "p50_latency", "p99_latency")})
+    assert st["completed"] == 6 and st["peak_resident"] >= 3
+    assert np.array_equal(alone_mac.view(np.uint32), mix_mac.view(np.uint32))
+    assert alone_f.shape == (150, 6) and np.array_equal(alone_f, mix_f)
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 5])
+def test_served_equals_the_plain_solve(k):
+    desc, mac, _ = mix()[k]
+    steps = 70
+    with pylbm.Server(budget_bytes=2 << 30, chunk_steps=32) as sv:
+        t = sv.submit(desc, mac, steps)
+        sv.wait(t, 120)
+        smac, sf = sv.read_macro(t), sv.read_forces(t)
+    d = dict(desc, autotune=0)
+    with pylbm.Solve(d) as s:
+        s.upload_macro(mac)
+        s.step(steps)
+        pmac, pf = s.read_macro(), s.read_forces(steps)
+    say(k, desc["q"], desc.get("storage"), "max |d mac|", float(np.abs(smac - pmac).max()))
+    assert np.array_equal(smac.view(np.uint32), pmac.view(np.uint32))
+    assert np.array_equal(sf, pf)
+
+
+def test_budget_admission_and_refusal():
+    reqs = [sc.boxed_sphere(n=48, re=100.0, storage="f32") for _ in range(5)]
+    # the footprint as the library computes it, from the refusal of a budget
+    # that is too small; then a budget that holds two of these solves at once
+    with pylbm.Server(budget_bytes=1 << 20) as sv:
+        with pytest.raises(pylbm.LbmError) as e:
+            sv.submit(reqs[0][0], reqs[0][1], 10)
+        assert e.value.code == -3 and "budget" in str(e.value)
+        foot = int(str(e.value).split("needs ")[1].split(" ")[0])
+    budget = int(2.5 * foot)
+    with pylbm.Server(budget_bytes=budget, chunk_steps=16) as sv:
+        ts = [sv.submit(d, m, 40) for d, m, _ in reqs]
+        sv.drain(300)
+        st = sv.stats()
+        for t in ts:
+            assert sv.wait(t, 1)["status"] == 0
+    say("footprint", foot, "budget", budget, "peak resident", st["peak_resident"], st["peak_resident_bytes"])
+    assert st["peak_resident"] == 2
+    assert st["peak_resident_bytes"] <= budget
+    assert st["completed"] == 5 and st["p99_latency"] >= st["p50_latency"] > 0
+
+
+def test_concurrent_submitters():
+    reqs = mix()
+    results = {}
+    with pylbm.Server(budget_bytes=2 << 30, chunk_steps=16) as sv:
+        def client(k):
+            d, m, _ = reqs[k]
+            t = sv.submit(d, m, 50)
+            results[k] = sv.wait(t, 300)
+        th = [threading.Thread(target=client, args=(k,)) for k in range(len(reqs))]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        st = sv.stats()
+    assert len(results) == len(reqs) and all(r["status"] == 0 for r in results.values())
+    assert st["completed"] == len(reqs)
