@@ -67,16 +67,9 @@ lbm_solve* handle(napi_env env, napi_value v) {
 //              force, walls, wallU, sponge: {width, axes, u},
 //              body: {a, centre} | null, solidEq, storage: 'f32'|'f16'|'bf16',
 //              autotune})
-napi_value CreateSolve(napi_env env, napi_callback_info info) {
-  size_t argc = 1;
-  napi_value argv[1];
-  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  if (argc < 1) {
-    napi_throw_type_error(env, nullptr, "createSolve(desc)");
-    return nullptr;
-  }
-  napi_value o = argv[0];
-  lbm_desc d;
+// Fills an lbm_desc from a JS descriptor object; false (with a pending JS
+// exception) on a bad field.
+bool parse_desc(napi_env env, napi_value o, lbm_desc& d) {
   memset(&d, 0, sizeof d);
   d.abi_version = LBM_ABI_VERSION;
   d.q = (int)num(env, o, "q", 19);
@@ -130,9 +123,22 @@ napi_value CreateSolve(napi_env env, napi_callback_info info) {
     else if (!strcmp(buf, "bf16")) d.storage = LBM_STORE_BF16;
     else if (strcmp(buf, "f32")) {
       napi_throw_type_error(env, nullptr, "storage must be 'f32', 'f16' or 'bf16'");
-      return nullptr;
+      return false;
     }
   }
+  return true;
+}
+
+napi_value CreateSolve(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 1) {
+    napi_throw_type_error(env, nullptr, "createSolve(desc)");
+    return nullptr;
+  }
+  lbm_desc d;
+  if (!parse_desc(env, argv[0], d)) return nullptr;
   lbm_solve* s = nullptr;
   int rc = lbm_solve_create(&d, &s);
   if (rc) return throw_lbm(env, rc);
@@ -562,6 +568,218 @@ napi_value AmrStep(napi_env env, napi_callback_info info) {
   return promise;
 }
 
+// ---------------------------------------------------------------------------
+// Batched serving (lbm_serve_*)
+// ---------------------------------------------------------------------------
+lbm_serve* serve_handle(napi_env env, napi_value v) {
+  void* p = nullptr;
+  napi_get_value_external(env, v, &p);
+  return static_cast<lbm_serve*>(p);
+}
+
+int64_t ticket_of(napi_env env, napi_value v) {
+  int64_t t = 0;
+  napi_get_value_int64(env, v, &t);
+  return t;
+}
+
+// serveCreate({budgetBytes, maxResident, chunkSteps, autotune})
+napi_value ServeCreate(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 1) {
+    napi_throw_type_error(env, nullptr, "serveCreate(desc)");
+    return nullptr;
+  }
+  lbm_serve_desc d;
+  memset(&d, 0, sizeof d);
+  d.abi_version = LBM_ABI_VERSION;
+  d.budget_bytes = num(env, argv[0], "budgetBytes", 0);
+  d.max_resident = (int)num(env, argv[0], "maxResident", 0);
+  d.chunk_steps = (int)num(env, argv[0], "chunkSteps", 32);
+  d.autotune = (int)num(env, argv[0], "autotune", 0);
+  d.target_cells = num(env, argv[0], "targetCells", 0);
+  lbm_serve* sv = nullptr;
+  int rc = lbm_serve_create(&d, &sv);
+  if (rc) return throw_lbm(env, rc);
+  napi_value ext;
+  NAPI_OK(napi_create_external(env, sv, nullptr, nullptr, &ext));
+  return ext;
+}
+
+napi_value ServeDestroy(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  lbm_serve_destroy(serve_handle(env, argv[0]));
+  return nullptr;
+}
+
+// serveSubmit(server, desc, Float32Array mac, steps) -> ticket
+napi_value ServeSubmit(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* data;
+  size_t len;
+  if (argc < 4 || !typed(env, argv[2], napi_float32_array, &data, &len)) {
+    napi_throw_type_error(env, nullptr, "serveSubmit(server, desc, Float32Array, steps)");
+    return nullptr;
+  }
+  lbm_desc d;
+  if (!parse_desc(env, argv[1], d)) return nullptr;
+  const size_t cells = (size_t)d.nx * d.ny * (d.q == 9 ? 1 : d.nz);
+  if (len < 4 * cells) {
+    napi_throw_range_error(env, nullptr, "mac must hold 4 floats per cell");
+    return nullptr;
+  }
+  int32_t steps = 0;
+  napi_get_value_int32(env, argv[3], &steps);
+  int64_t t = 0;
+  int rc = lbm_serve_submit(serve_handle(env, argv[0]), &d, static_cast<float*>(data), steps, &t);
+  if (rc) return throw_lbm(env, rc);
+  napi_value v;
+  NAPI_OK(napi_create_int64(env, t, &v));
+  return v;
+}
+
+struct WaitWork {
+  napi_async_work work;
+  napi_deferred deferred;
+  lbm_serve* sv;
+  int64_t ticket;
+  double timeout;
+  lbm_serve_result r;
+  int rc;
+  std::string err;
+};
+
+void WaitExecute(napi_env, void* p) {
+  WaitWork* w = static_cast<WaitWork*>(p);
+  w->rc = lbm_serve_wait(w->sv, w->ticket, w->timeout, &w->r);
+  if (w->rc) w->err = lbm_last_error();
+}
+
+void WaitComplete(napi_env env, napi_status, void* p) {
+  WaitWork* w = static_cast<WaitWork*>(p);
+  if (w->rc) {
+    napi_value msg, err;
+    std::string m = "lbm error " + std::to_string(w->rc) + ": " + w->err;
+    napi_create_string_utf8(env, m.c_str(), m.size(), &msg);
+    napi_create_error(env, nullptr, msg, &err);
+    napi_reject_deferred(env, w->deferred, err);
+  } else {
+    napi_value o, v, arr;
+    napi_create_object(env, &o);
+    napi_create_int64(env, w->r.ticket, &v);
+    napi_set_named_property(env, o, "ticket", v);
+    napi_create_int32(env, w->r.steps, &v);
+    napi_set_named_property(env, o, "steps", v);
+    napi_create_double(env, w->r.cells, &v);
+    napi_set_named_property(env, o, "cells", v);
+    napi_create_double(env, w->r.t_submit, &v);
+    napi_set_named_property(env, o, "tSubmit", v);
+    napi_create_double(env, w->r.t_admit, &v);
+    napi_set_named_property(env, o, "tAdmit", v);
+    napi_create_double(env, w->r.t_finish, &v);
+    napi_set_named_property(env, o, "tFinish", v);
+    napi_create_array_with_length(env, 6, &arr);
+    for (uint32_t k = 0; k < 6; k++) {
+      napi_create_double(env, w->r.force[k], &v);
+      napi_set_element(env, arr, k, v);
+    }
+    napi_set_named_property(env, o, "force", arr);
+    napi_resolve_deferred(env, w->deferred, o);
+  }
+  napi_delete_async_work(env, w->work);
+  delete w;
+}
+
+// serveWait(server, ticket, timeoutSeconds) -> Promise<result>
+napi_value ServeWait(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  WaitWork* w = new WaitWork();
+  w->sv = serve_handle(env, argv[0]);
+  w->ticket = ticket_of(env, argv[1]);
+  w->timeout = -1;
+  if (argc > 2) napi_get_value_double(env, argv[2], &w->timeout);
+  napi_value promise, name;
+  NAPI_OK(napi_create_promise(env, &w->deferred, &promise));
+  NAPI_OK(napi_create_string_utf8(env, "lbm_serve_wait", NAPI_AUTO_LENGTH, &name));
+  NAPI_OK(napi_create_async_work(env, nullptr, name, WaitExecute, WaitComplete, w, &w->work));
+  NAPI_OK(napi_queue_async_work(env, w->work));
+  return promise;
+}
+
+// serveReadMacro(server, ticket, Float32Array)
+napi_value ServeReadMacro(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* data;
+  size_t len;
+  if (argc < 3 || !typed(env, argv[2], napi_float32_array, &data, &len)) {
+    napi_throw_type_error(env, nullptr, "serveReadMacro(server, ticket, Float32Array)");
+    return nullptr;
+  }
+  int rc = lbm_serve_read_macro(serve_handle(env, argv[0]), ticket_of(env, argv[1]), static_cast<float*>(data));
+  if (rc) return throw_lbm(env, rc);
+  return argv[2];
+}
+
+// serveReadForces(server, ticket, maxRecords) -> Float64Array(6 * count)
+napi_value ServeReadForces(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  int32_t maxr = 1;
+  if (argc > 2) napi_get_value_int32(env, argv[2], &maxr);
+  if (maxr < 0) maxr = 0;
+  napi_value ab, arr;
+  void* data;
+  NAPI_OK(napi_create_arraybuffer(env, (size_t)maxr * 48 + 8, &data, &ab));
+  int32_t count = 0;
+  int rc = lbm_serve_read_forces(serve_handle(env, argv[0]), ticket_of(env, argv[1]), static_cast<double*>(data), maxr,
+                                 &count);
+  if (rc) return throw_lbm(env, rc);
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, (size_t)count * 6, ab, 0, &arr));
+  return arr;
+}
+
+napi_value ServeRelease(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  int rc = lbm_serve_release(serve_handle(env, argv[0]), ticket_of(env, argv[1]));
+  if (rc) return throw_lbm(env, rc);
+  return nullptr;
+}
+
+napi_value ServeStats(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  lbm_serve_stats st;
+  int rc = lbm_serve_read_stats(serve_handle(env, argv[0]), &st);
+  if (rc) return throw_lbm(env, rc);
+  napi_value o, v;
+  NAPI_OK(napi_create_object(env, &o));
+  const struct { const char* k; double v; } f[] = {
+      {"submitted", (double)st.submitted}, {"completed", (double)st.completed}, {"failed", (double)st.failed},
+      {"resident", (double)st.resident}, {"queued", (double)st.queued}, {"cellUpdates", st.cell_updates},
+      {"p50Latency", st.p50_latency}, {"p99Latency", st.p99_latency}, {"mlups", st.mlups},
+      {"poolBytes", st.pool_bytes}, {"peakResidentBytes", st.peak_resident_bytes},
+      {"peakResident", (double)st.peak_resident}, {"chunks", (double)st.chunks}};
+  for (const auto& e : f) {
+    napi_create_double(env, e.v, &v);
+    napi_set_named_property(env, o, e.k, v);
+  }
+  return o;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct Fn { const char* name; napi_callback cb; };
   const Fn fns[] = {
@@ -589,6 +807,14 @@ napi_value Init(napi_env env, napi_value exports) {
       {"amrManageNow", AmrManageNow},
       {"amrInfo", AmrInfo},
       {"amrStep", AmrStep},
+      {"serveCreate", ServeCreate},
+      {"serveDestroy", ServeDestroy},
+      {"serveSubmit", ServeSubmit},
+      {"serveWait", ServeWait},
+      {"serveReadMacro", ServeReadMacro},
+      {"serveReadForces", ServeReadForces},
+      {"serveRelease", ServeRelease},
+      {"serveStats", ServeStats},
   };
   for (const Fn& f : fns) {
     napi_value fn;

@@ -4,18 +4,20 @@
 // allocation of the byte budget, first-fit with coalescing) and advance
 // together.  A host worker thread owns all device work:
 //
-//   loop:  admit queued requests that fit the pool (FIFO)
-//          for each lattice class (Q, storage, general): refresh the device
-//            solve table; rebuild the tile table and the captured chunk
-//            graph when the class's membership changed
-//          replay the chunk: K launches per class, launch j steps every
-//            member with j < remaining (k_step_batch)
-//          retire the members that finished: sum their last force record,
-//            copy the macroscopic field and the force history back, free
-//            their pool memory, wake waiters
+//   loop:  launch the chunk of the resident set on the compute stream:
+//            per lattice class (Q, storage, general) a device solve table,
+//            a tile table and a captured graph of K launches, launch j
+//            stepping every member with j < remaining (k_step_batch)
+//          while it runs, on the setup stream: copy back the members that
+//            finished in the previous chunk (last force record, macroscopic
+//            field, force history), free their pool memory, and admit
+//            queued requests (FIFO) that fit the byte budget, until the
+//            resident cells reach the target that fills the device
+//          wait for the chunk; members that reached their step count are
+//            set aside for copy-back during the next chunk
 //
-// Requests that arrive while others run are admitted at the next chunk
-// boundary, so the device always holds the whole resident set's work.
+// Requests that arrive while others run join at the next chunk boundary;
+// host set-up and copy-back overlap device work.
 // A member's arithmetic depends only on its own data, its class's tile shape
 // and its own step count (the 16-bit dither is seeded by it), so its result
 // is the same alone and in any mix.
@@ -166,6 +168,11 @@ struct lbm_serve {
   std::chrono::steady_clock::time_point t0;
   Pool pool;
   hipStream_t stream = nullptr;
+  hipStream_t setup = nullptr;       // admissions and copy-back
+  hipEvent_t setup_done = nullptr;   // recorded after each admission round
+  std::vector<std::unique_ptr<Member>> finishing;  // done, results not yet copied back
+  std::vector<int> finishing_adv;
+  double resident_cells = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<std::shared_ptr<Request>> queue;
@@ -231,6 +238,7 @@ void admit(lbm_serve* sv) {
       std::lock_guard<std::mutex> g(sv->mu);
       if (sv->queue.empty()) return;
       if (sv->d.max_resident > 0 && (int)sv->resident.size() >= sv->d.max_resident) return;
+      if (sv->d.target_cells > 0 && sv->resident_cells >= sv->d.target_cells) return;
       if (sv->resident_bytes + sv->queue.front()->footprint > sv->pool.size()) return;
       r = sv->queue.front();
       sv->queue.pop_front();
@@ -248,7 +256,7 @@ void admit(lbm_serve* sv) {
     }
     auto m = std::make_unique<Member>();
     m->r = r;
-    int rc = solve_create_in(&r->d, &sv->pool, sv->stream, cp->bx, cp->by, &m->s);
+    int rc = solve_create_in(&r->d, &sv->pool, sv->setup, cp->bx, cp->by, &m->s);
     if (rc == LBM_E_NOMEM) {
       // fragmentation: wait for a retirement and try again
       std::lock_guard<std::mutex> g(sv->mu);
@@ -268,6 +276,7 @@ void admit(lbm_serve* sv) {
     std::lock_guard<std::mutex> g(sv->mu);
     r->t_admit = now_s(sv->t0);
     sv->resident_bytes += r->footprint;
+    sv->resident_cells += r->cells;
     sv->peak_resident_bytes = std::max(sv->peak_resident_bytes, sv->resident_bytes);
     cp->members.push_back(m.get());
     cp->dirty = true;
@@ -339,8 +348,11 @@ int enqueue_chunk(lbm_serve* sv, int K) {
   return LBM_OK;
 }
 
-int run_chunk(lbm_serve* sv) {
+// Launches the chunk of the current resident set on the compute stream
+// (after the set-up stream's last admissions) and returns without waiting.
+int launch_chunk(lbm_serve* sv) {
   const int K = std::max(1, sv->d.chunk_steps);
+  HIPCHK_RET(hipStreamWaitEvent(sv->stream, sv->setup_done, 0));
   bool rebuild = false;
   int rc = prepare(sv, &rebuild);
   if (rc) return rc;
@@ -361,93 +373,125 @@ int run_chunk(lbm_serve* sv) {
     if (e != hipSuccess) return hip_fail(e, "chunk instantiate");
   }
   HIPCHK_RET(hipGraphLaunch(sv->graph, sv->stream));
+  return LBM_OK;
+}
+
+// After the chunk: advance every member by its steps; members that reached
+// their step count leave the resident set for copy-back.
+int complete_chunk(lbm_serve* sv) {
+  const int K = std::max(1, sv->d.chunk_steps);
   HIPCHK_RET(hipStreamSynchronize(sv->stream));
+  std::lock_guard<std::mutex> g(sv->mu);
   sv->chunks++;
-  // bookkeeping and retirement
-  std::vector<std::unique_ptr<Member>> keep, done;
-  std::vector<int> done_adv;
+  std::vector<std::unique_ptr<Member>> keep;
+  for (auto& m : sv->resident) {
+    const int adv = std::min(m->r->steps - m->done_steps, K);
+    m->done_steps += adv;
+    if (m->done_steps >= m->r->steps) {
+      sv->resident_cells -= m->r->cells;
+      sv->finishing_adv.push_back(adv);
+      sv->finishing.push_back(std::move(m));
+    } else {
+      solve_advance(m->s, adv, false);
+      m->pending = m->entry.args[0].force_slot != nullptr;
+      keep.push_back(std::move(m));
+    }
+  }
+  sv->resident.swap(keep);
+  for (auto& kv : sv->classes) {
+    Class* c = kv.second.get();
+    std::vector<Member*> mem;
+    for (Member* m : c->members)
+      if (m->done_steps < m->r->steps) mem.push_back(m);
+    if (mem.size() != c->members.size()) {
+      c->members.swap(mem);
+      c->dirty = true;
+    }
+  }
+  return LBM_OK;
+}
+
+// Copies back the results of the members set aside by complete_chunk (on the
+// set-up stream) and frees their pool memory.
+void retire(lbm_serve* sv) {
+  std::vector<std::unique_ptr<Member>> done;
+  std::vector<int> adv;
   {
     std::lock_guard<std::mutex> g(sv->mu);
-    for (auto& m : sv->resident) {
-      const int adv = std::min(m->r->steps - m->done_steps, K);
-      m->done_steps += adv;
-      if (m->done_steps >= m->r->steps) {
-        done_adv.push_back(adv);
-        done.push_back(std::move(m));
-      } else {
-        solve_advance(m->s, adv, false);
-        m->pending = m->entry.args[0].force_slot != nullptr;
-        keep.push_back(std::move(m));
-      }
-    }
-    sv->resident.swap(keep);
-    for (auto& kv : sv->classes) {
-      Class* c = kv.second.get();
-      std::vector<Member*> mem;
-      for (Member* m : c->members)
-        if (m->done_steps < m->r->steps) mem.push_back(m);
-      if (mem.size() != c->members.size()) {
-        c->members.swap(mem);
-        c->dirty = true;
-      }
-    }
+    done.swap(sv->finishing);
+    adv.swap(sv->finishing_adv);
   }
   for (size_t k = 0; k < done.size(); k++) {
     Member& m = *done[k];
     Request& r = *m.r;
-    int rc2 = solve_advance(m.s, done_adv[k], true);
-    if (!rc2) {
+    int rc = solve_advance(m.s, adv[k], true);
+    if (!rc) {
       r.mac_out.resize((size_t)r.cells * 4);
-      rc2 = lbm_read_macro(m.s, r.mac_out.data());
+      rc = lbm_read_macro(m.s, r.mac_out.data());
     }
-    if (!rc2 && m.entry.args[0].force_slot) {
+    if (!rc && m.entry.args[0].force_slot) {
       const int maxr = std::min(r.steps, 1 << 16);
       r.forces.resize((size_t)maxr * 6);
       int32_t cnt = 0;
-      rc2 = lbm_read_forces(m.s, r.forces.data(), maxr, &cnt);
+      rc = lbm_read_forces(m.s, r.forces.data(), maxr, &cnt);
       r.forces.resize((size_t)cnt * 6);
     }
-    const std::string err = rc2 ? lbm_last_error() : "";
+    const std::string err = rc ? lbm_last_error() : "";
     lbm_solve_destroy(m.s);
     {
       std::lock_guard<std::mutex> g(sv->mu);
       sv->resident_bytes -= r.footprint;
     }
-    finish(sv, m.r, rc2, err);
+    finish(sv, m.r, rc, err);
   }
-  return LBM_OK;
+}
+
+void fail_resident(lbm_serve* sv, int rc) {
+  const std::string err = lbm_last_error();
+  std::vector<std::unique_ptr<Member>> all;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    all.swap(sv->resident);
+    for (auto& m : sv->finishing) all.push_back(std::move(m));
+    sv->finishing.clear();
+    sv->finishing_adv.clear();
+    for (auto& kv : sv->classes) {
+      kv.second->members.clear();
+      kv.second->dirty = true;
+    }
+    sv->resident_bytes = 0;
+    sv->resident_cells = 0;
+  }
+  for (auto& m : all) {
+    lbm_solve_destroy(m->s);
+    finish(sv, m->r, rc, err);
+  }
 }
 
 void worker(lbm_serve* sv) {
   (void)hipSetDevice(sv->device);
   for (;;) {
-    admit(sv);
+    bool running = false;
     {
-      std::unique_lock<std::mutex> lk(sv->mu);
-      if (sv->resident.empty()) {
-        if (sv->stop) break;
-        sv->cv_work.wait(lk, [&] { return sv->stop || !sv->queue.empty(); });
-        continue;
-      }
+      std::lock_guard<std::mutex> g(sv->mu);
+      running = !sv->resident.empty();
     }
-    const int rc = run_chunk(sv);
+    int rc = LBM_OK;
+    if (running) rc = launch_chunk(sv);
+    // host work and copies overlap the chunk
+    retire(sv);
+    admit(sv);
+    (void)hipEventRecord(sv->setup_done, sv->setup);
+    if (!rc && running) rc = complete_chunk(sv);
     if (rc) {
-      // a device error ends every resident request with it
-      const std::string err = lbm_last_error();
-      std::vector<std::unique_ptr<Member>> all;
-      {
-        std::lock_guard<std::mutex> g(sv->mu);
-        all.swap(sv->resident);
-        for (auto& kv : sv->classes) {
-          kv.second->members.clear();
-          kv.second->dirty = true;
-        }
-        sv->resident_bytes = 0;
-      }
-      for (auto& m : all) {
-        lbm_solve_destroy(m->s);
-        finish(sv, m->r, rc, err);
-      }
+      fail_resident(sv, rc);
+      continue;
+    }
+    if (!running) {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      if (!sv->resident.empty() || !sv->finishing.empty()) continue;
+      if (sv->stop) break;
+      if (sv->queue.empty()) sv->cv_work.wait(lk, [&] { return sv->stop || !sv->queue.empty(); });
     }
   }
   // requests still queued when stopping
@@ -473,7 +517,8 @@ int lbm_serve_create(const lbm_serve_desc* desc, lbm_serve** out) {
   *out = nullptr;
   if (desc->abi_version != LBM_ABI_VERSION) return fail(LBM_E_INVALID, "descriptor abi_version %d", desc->abi_version);
   if (!(desc->budget_bytes >= 1 << 20)) return fail(LBM_E_INVALID, "budget_bytes %g below 1 MiB", desc->budget_bytes);
-  if (desc->max_resident < 0 || desc->chunk_steps < 0) return fail(LBM_E_INVALID, "negative max_resident / chunk_steps");
+  if (desc->max_resident < 0 || desc->chunk_steps < 0 || desc->target_cells < 0)
+    return fail(LBM_E_INVALID, "negative max_resident / chunk_steps / target_cells");
   auto sv = std::make_unique<lbm_serve>();
   sv->d = *desc;
   if (sv->d.chunk_steps == 0) sv->d.chunk_steps = 32;
@@ -483,7 +528,10 @@ int lbm_serve_create(const lbm_serve_desc* desc, lbm_serve** out) {
   int rc = sv->pool.init((size_t)desc->budget_bytes);
   if (rc) return rc;
   e = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->setup, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->setup_done, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(sv->setup_done, sv->setup);
+  if (e != hipSuccess) return hip_fail(e, "serve streams");
   lbm_serve* p = sv.release();
   p->worker = std::thread(worker, p);
   *out = p;
@@ -504,6 +552,8 @@ int lbm_serve_destroy(lbm_serve* sv) {
     if (kv.second->d_tiles) (void)hipFree(kv.second->d_tiles);
   }
   if (sv->stream) (void)hipStreamDestroy(sv->stream);
+  if (sv->setup) (void)hipStreamDestroy(sv->setup);
+  if (sv->setup_done) (void)hipEventDestroy(sv->setup_done);
   delete sv;
   return LBM_OK;
 }

@@ -327,6 +327,42 @@ export class AmrSolver {
   }
 }
 
+// Batched serve loop: many independent solves resident in one device pool,
+// admitted against a byte budget and stepped together (lbm_serve_*).
+// submit() returns a ticket at once; result() resolves when the solve is done.
+export class ServeClient {
+  constructor(opts) {
+    const a = loadAddon();
+    this.addon = a;
+    this.handle = a.serveCreate(opts || {});
+    this.cells = new Map();
+  }
+
+  destroy() {
+    if (this.handle) { this.addon.serveDestroy(this.handle); this.handle = null; }
+  }
+
+  // scenario: {desc, macro()} as sphereScenario / tgvScenario build them
+  submit(scenario, steps) {
+    const d = scenario.desc;
+    const t = this.addon.serveSubmit(this.handle, d, scenario.macro(), steps | 0);
+    this.cells.set(t, d.nx * d.ny * (d.q === 9 ? 1 : d.nz));
+    return t;
+  }
+
+  // -> {ticket, steps, cells, tSubmit, tAdmit, tFinish, force, macro, forces}
+  async result(ticket, timeoutSeconds) {
+    const r = await this.addon.serveWait(this.handle, ticket, timeoutSeconds === undefined ? -1 : timeoutSeconds);
+    r.macro = this.addon.serveReadMacro(this.handle, ticket, new Float32Array(4 * this.cells.get(ticket)));
+    r.forces = this.addon.serveReadForces(this.handle, ticket, r.steps);
+    return r;
+  }
+
+  release(ticket) { this.addon.serveRelease(this.handle, ticket); this.cells.delete(ticket); }
+
+  stats() { return this.addon.serveStats(this.handle); }
+}
+
 export function deviceInfo() { return loadAddon().deviceInfo(); }
 
 export { velocities, weights, SUPPORTED_Q };

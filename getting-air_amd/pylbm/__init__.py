@@ -61,7 +61,7 @@ class AmrDesc(ctypes.Structure):
 
 class ServeDesc(ctypes.Structure):
     _fields_ = [("abi_version", ctypes.c_int32), ("max_resident", ctypes.c_int32), ("budget_bytes", ctypes.c_double),
-                ("chunk_steps", ctypes.c_int32), ("autotune", ctypes.c_int32)]
+                ("chunk_steps", ctypes.c_int32), ("autotune", ctypes.c_int32), ("target_cells", ctypes.c_double)]
 
 
 class ServeResult(ctypes.Structure):
@@ -432,7 +432,7 @@ class Amr:
 class Server:
     """Batched serve loop on the GPU (owns an lbm_serve handle)."""
 
-    def __init__(self, budget_bytes, max_resident=0, chunk_steps=32, autotune=0):
+    def __init__(self, budget_bytes, max_resident=0, chunk_steps=32, autotune=0, target_cells=0.0):
         self.lib = load()
         d = ServeDesc()
         d.abi_version = ABI_VERSION
@@ -440,6 +440,7 @@ class Server:
         d.budget_bytes = float(budget_bytes)
         d.chunk_steps = int(chunk_steps)
         d.autotune = int(autotune)
+        d.target_cells = float(target_cells)
         h = ctypes.c_void_p()
         _check(self.lib.lbm_serve_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h

@@ -238,6 +238,8 @@ typedef struct lbm_serve_desc {
   double budget_bytes;   /* device pool size: admission budget                 */
   int32_t chunk_steps;   /* steps per captured chunk (admission granularity)   */
   int32_t autotune;      /* 1: tile shape per lattice class by timing          */
+  double target_cells;   /* stop admitting once this many cells are resident
+                            (enough to fill the device; 0: no limit)          */
 } lbm_serve_desc;
 
 typedef struct lbm_serve_result {

@@ -2,7 +2,7 @@
 // liblbm.so, checked against the FP64 oracle run in the same process.
 // Run: node tests/js_gpu/test-host-gpu.mjs   (needs an MI355X)
 import { createRequire } from 'module';
-import { DenseSolver, AmrSolver, sphereScenario, tgvScenario, deviceInfo } from '../../getting-air_amd/js/lbm-host.mjs';
+import { DenseSolver, AmrSolver, ServeClient, sphereScenario, tgvScenario, deviceInfo } from '../../getting-air_amd/js/lbm-host.mjs';
 import { makePool, refineWhere, manageSlots, activeSlotList } from '../../getting-air_amd/js/d3-amr.mjs';
 
 const require = createRequire(import.meta.url);
@@ -128,6 +128,34 @@ function relErr(gpuMac, oracleMac, comp) {
     const dm = Math.abs(st.mass - st0.mass) / st0.mass;
     assert(dm < 1e-6, 'mass drift ' + dm);
     return changes + ' topology changes, ' + prev.activeTiles + ' tiles, mass drift ' + dm.toExponential(2);
+  });
+
+  await ok('serve loop: a sphere gives the same bits alone and in a mix, and matches DenseSolver', async () => {
+    const sphere = sphereScenario({ R: 4, dims: [48, 24, 24], centre: [14, 12, 12], re: 80, trt: true });
+    const others = [tgvScenario({ n: 64, tau: 0.6, u0: 0.05 }),
+      sphereScenario({ R: 3, dims: [32, 32, 32], centre: [10, 16, 16], re: 60, trt: true })];
+    const alone = new ServeClient({ budgetBytes: 1 << 30, chunkSteps: 16 });
+    const ra = await alone.result(alone.submit(sphere, 60), 120);
+    alone.destroy();
+    const mixed = new ServeClient({ budgetBytes: 1 << 30, chunkSteps: 16 });
+    const t0 = mixed.submit(others[0], 50);
+    const t1 = mixed.submit(sphere, 60);
+    const t2 = mixed.submit(others[1], 40);
+    const [r0, r1, r2] = await Promise.all([mixed.result(t0, 120), mixed.result(t1, 120), mixed.result(t2, 120)]);
+    const st = mixed.stats();
+    mixed.destroy();
+    assert(r0.steps === 50 && r2.steps === 40 && st.completed === 3, JSON.stringify(st));
+    for (let c = 0; c < ra.macro.length; c++) assert(Object.is(ra.macro[c], r1.macro[c]), 'macro differs at ' + c);
+    for (let c = 0; c < ra.forces.length; c++) assert(Object.is(ra.forces[c], r1.forces[c]), 'force differs at ' + c);
+    const s = new DenseSolver(sphere);
+    await s.reset();
+    await s.debugStepSync(60);
+    const dm = await s.readMacro();
+    s.destroy();
+    let dmax = 0;
+    for (let c = 0; c < dm.length; c++) dmax = Math.max(dmax, Math.abs(dm[c] - r1.macro[c]));
+    assert(dmax < 1e-6, 'served vs DenseSolver ' + dmax);
+    return 'p50 ' + st.p50Latency.toFixed(3) + ' s, served vs dense ' + dmax.toExponential(2);
   });
 
   await ok('invalid descriptor raises with the library message', () => {
