@@ -127,6 +127,69 @@ def measure_amr(steps, manage_every=50):
             "refine_at": adesc["refineAt"], "keep_at": adesc["keepAt"], "tau": 0.52}
 
 
+def enstrophy(mac, n):
+    """0.5 <|curl u|^2> with central differences on the periodic box
+    (main-3d.js:3627-3630), u from [rho, ux, uy, uz] planes."""
+    m = mac.reshape(n, n, n, 4).astype(np.float64)
+    u = [m[..., k] for k in (1, 2, 3)]
+
+    def d(c, axis):  # axis 0 = z, 1 = y, 2 = x
+        return 0.5 * (np.roll(u[c], -1, axis) - np.roll(u[c], 1, axis))
+
+    wx = d(2, 1) - d(1, 0)
+    wy = d(0, 0) - d(2, 2)
+    wz = d(1, 2) - d(0, 1)
+    return 0.5 * float(np.mean(wx * wx + wy * wy + wz * wz))
+
+
+def measure_turbulence(n=512, steps=2000, n_small=128, threads=16):
+    """Config 4: decaying turbulence, E(k) ~ k^4 exp(-2 (k/8)^2), u_rms 0.04,
+    tau 0.55, in fp32 and shifted fp16 / bf16 storage.  At n^3 the 16-bit
+    runs are compared with the fp32 run; at n_small^3 (the same spectrum, peak
+    wavenumber scaled by n_small/n) every format is compared with the FP64
+    oracle, which is what shows fp32 standing in for FP64 at n^3."""
+    import oracle_runner
+
+    def run(desc, mac, storage, nn):
+        with pylbm.Solve(dict(desc, storage=storage, autotune=1)) as s:
+            s.upload_macro(mac)
+            s.step(20)
+            s.sync()
+            s.upload_macro(mac)
+            t0 = time.perf_counter()
+            s.step(steps)
+            s.sync()
+            secs = time.perf_counter() - t0
+            out = s.read_macro()
+        return out, nn ** 3 * steps / secs / 1e6
+
+    def errs(m, ref):
+        u, r = m.reshape(-1, 4)[:, 1:].astype(np.float64), ref.reshape(-1, 4)[:, 1:].astype(np.float64)
+        return float(np.sqrt(((u - r) ** 2).sum() / (r ** 2).sum()))
+
+    res = {"n": n, "steps": steps, "n_small": n_small}
+    desc, mac = sc.turbulence(n=n)
+    ref, ml = run(desc, mac, "f32", n)
+    ens_ref = enstrophy(ref, n)
+    res["f32"] = {"mlups": ml, "enstrophy": ens_ref, "enstrophy0": enstrophy(mac, n)}
+    for st in ("f16", "bf16"):
+        m, ml = run(desc, mac, st, n)
+        res[st] = {"mlups": ml, "speedup_vs_f32": ml / res["f32"]["mlups"], "u_rel_l2_vs_f32": errs(m, ref),
+                   "enstrophy_rel_vs_f32": enstrophy(m, n) / ens_ref - 1.0}
+        del m
+    del ref, mac
+    desc, mac = sc.turbulence(n=n_small, kpeak=8.0 * n_small / n)
+    o = oracle_runner.run(desc, mac, steps=steps, threads=threads, want=("mac",))
+    oref = o["mac"].reshape(-1)
+    ens_o = enstrophy(oref, n_small)
+    res["small_fp64_oracle"] = {"enstrophy": ens_o, "oracle_mlups": o["summary"]["mlups"]}
+    for st in ("f32", "f16", "bf16"):
+        m, _ = run(desc, mac, st, n_small)
+        res["small_fp64_oracle"][st] = {"u_rel_l2_vs_fp64": errs(m, oref),
+                                        "enstrophy_rel_vs_fp64": enstrophy(m, n_small) / ens_o - 1.0}
+    return res
+
+
 def measure_serve(count, rate=50.0, seed=5, steps=2000, budget=200e9, scale=1.0):
     """Config-5 serving: `count` requests with seeded Poisson arrivals at
     `rate`/s (2D cylinder channels nx 256-1024 and 3D boxed spheres 64^3-192^3,
@@ -192,6 +255,8 @@ def main():
     ap.add_argument("--no-amr", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--turbulence", action="store_true",
+                    help="also run the 512^3 decaying-turbulence precision study (several minutes)")
     ap.add_argument("--tile", default=None,
                     help="fixed tile bx,by,flags (flags bit0 XCD swizzle, bit1 non-temporal) instead of autotuning")
     args = ap.parse_args()
@@ -232,6 +297,11 @@ def main():
     if not args.no_serve:
         extra_serve = measure_serve(args.serve_requests)
         print("serve: %s" % json.dumps(extra_serve), file=sys.stderr, flush=True)
+
+    extra_turb = None
+    if args.turbulence:
+        extra_turb = measure_turbulence()
+        print("turbulence: %s" % json.dumps(extra_turb), file=sys.stderr, flush=True)
 
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
@@ -278,6 +348,8 @@ def main():
                   "cd_last1000": v["cd_last1000"],
                   "drag_dev_vs_f32_pct": (100.0 * (v["cd_last1000"] / main_run["cd_last1000"] - 1.0)
                                           if v["cd_last1000"] is not None and main_run["cd_last1000"] else None)}
+    if extra_turb:
+        out["turbulence_d3q19_512"] = extra_turb
     if extra_serve:
         out["serve_config5"] = {k: (float("%.5g" % v) if isinstance(v, float) else v) for k, v in extra_serve.items()}
     if extra_amr:

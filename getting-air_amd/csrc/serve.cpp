@@ -134,7 +134,9 @@ struct Member {
   BatchSolve entry{};
   uint32_t ntiles = 0;
   int done_steps = 0;
-  bool pending = false;  // force partials of the last step not yet summed
+  bool pending = false;   // force partials of the last step not yet summed
+  bool in_chunk = false;  // part of the chunk in flight (members admitted
+                          // while a chunk runs join the next one)
 };
 
 struct ClassKey {
@@ -153,7 +155,10 @@ struct Class {
   BatchTile* d_tiles = nullptr;
   size_t cap_solves = 0, cap_tiles = 0;
   uint32_t ntiles = 0;
+  // host sources of the tables' async uploads: they must outlive the copies,
+  // so they live here until the next chunk (each chunk ends with a sync)
   std::vector<BatchSolve> h_solves;
+  std::vector<BatchTile> h_tiles;
 };
 
 double now_s(std::chrono::steady_clock::time_point t0) {
@@ -305,7 +310,8 @@ int prepare(lbm_serve* sv, bool* rebuild) {
     Class* c = kv.second.get();
     if (c->dirty) {
       *rebuild = true;
-      std::vector<BatchTile> tiles;
+      std::vector<BatchTile>& tiles = c->h_tiles;
+      tiles.clear();
       for (uint32_t k = 0; k < c->members.size(); k++)
         for (uint32_t t = 0; t < c->members[k]->ntiles; t++) tiles.push_back(BatchTile{k, t});
       c->ntiles = (uint32_t)tiles.size();
@@ -352,6 +358,10 @@ int enqueue_chunk(lbm_serve* sv, int K) {
 // (after the set-up stream's last admissions) and returns without waiting.
 int launch_chunk(lbm_serve* sv) {
   const int K = std::max(1, sv->d.chunk_steps);
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    for (auto& m : sv->resident) m->in_chunk = true;
+  }
   HIPCHK_RET(hipStreamWaitEvent(sv->stream, sv->setup_done, 0));
   bool rebuild = false;
   int rc = prepare(sv, &rebuild);
@@ -376,8 +386,9 @@ int launch_chunk(lbm_serve* sv) {
   return LBM_OK;
 }
 
-// After the chunk: advance every member by its steps; members that reached
-// their step count leave the resident set for copy-back.
+// After the chunk: advance every member that ran in it by its steps (those
+// admitted during the chunk have not moved); members that reached their step
+// count leave the resident set for copy-back.
 int complete_chunk(lbm_serve* sv) {
   const int K = std::max(1, sv->d.chunk_steps);
   HIPCHK_RET(hipStreamSynchronize(sv->stream));
@@ -385,6 +396,10 @@ int complete_chunk(lbm_serve* sv) {
   sv->chunks++;
   std::vector<std::unique_ptr<Member>> keep;
   for (auto& m : sv->resident) {
+    if (!m->in_chunk) {
+      keep.push_back(std::move(m));
+      continue;
+    }
     const int adv = std::min(m->r->steps - m->done_steps, K);
     m->done_steps += adv;
     if (m->done_steps >= m->r->steps) {

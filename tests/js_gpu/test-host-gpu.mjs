@@ -134,9 +134,14 @@ function relErr(gpuMac, oracleMac, comp) {
     const sphere = sphereScenario({ R: 4, dims: [48, 24, 24], centre: [14, 12, 12], re: 80, trt: true });
     const others = [tgvScenario({ n: 64, tau: 0.6, u0: 0.05 }),
       sphereScenario({ R: 3, dims: [32, 32, 32], centre: [10, 16, 16], re: 60, trt: true })];
-    const alone = new ServeClient({ budgetBytes: 1 << 30, chunkSteps: 16 });
-    const ra = await alone.result(alone.submit(sphere, 60), 120);
-    alone.destroy();
+    const aloneRun = async () => {
+      const c = new ServeClient({ budgetBytes: 1 << 30, chunkSteps: 16 });
+      const r = await c.result(c.submit(sphere, 60), 120);
+      c.destroy();
+      return r;
+    };
+    const ra = await aloneRun();
+    const rb = await aloneRun();
     const mixed = new ServeClient({ budgetBytes: 1 << 30, chunkSteps: 16 });
     const t0 = mixed.submit(others[0], 50);
     const t1 = mixed.submit(sphere, 60);
@@ -144,18 +149,25 @@ function relErr(gpuMac, oracleMac, comp) {
     const [r0, r1, r2] = await Promise.all([mixed.result(t0, 120), mixed.result(t1, 120), mixed.result(t2, 120)]);
     const st = mixed.stats();
     mixed.destroy();
-    assert(r0.steps === 50 && r2.steps === 40 && st.completed === 3, JSON.stringify(st));
-    for (let c = 0; c < ra.macro.length; c++) assert(Object.is(ra.macro[c], r1.macro[c]), 'macro differs at ' + c);
-    for (let c = 0; c < ra.forces.length; c++) assert(Object.is(ra.forces[c], r1.forces[c]), 'force differs at ' + c);
     const s = new DenseSolver(sphere);
     await s.reset();
     await s.debugStepSync(60);
     const dm = await s.readMacro();
     s.destroy();
-    let dmax = 0;
-    for (let c = 0; c < dm.length; c++) dmax = Math.max(dmax, Math.abs(dm[c] - r1.macro[c]));
-    assert(dmax < 1e-6, 'served vs DenseSolver ' + dmax);
-    return 'p50 ' + st.p50Latency.toFixed(3) + ' s, served vs dense ' + dmax.toExponential(2);
+    const cmp = (x, y) => {
+      let n = 0, mx = 0;
+      for (let c = 0; c < x.length; c++) if (!Object.is(x[c], y[c])) { n++; mx = Math.max(mx, Math.abs(x[c] - y[c])); }
+      return n + ' (max ' + mx.toExponential(2) + ')';
+    };
+    const report = 'alone/alone ' + cmp(ra.macro, rb.macro) + ', alone/mix ' + cmp(ra.macro, r1.macro) +
+      ', alone/dense ' + cmp(ra.macro, dm) + ', mix/dense ' + cmp(r1.macro, dm) +
+      ', steps ' + [ra.steps, rb.steps, r1.steps, r0.steps, r2.steps].join('/');
+    console.log('     ' + report);
+    assert(r0.steps === 50 && r2.steps === 40 && st.completed === 3, JSON.stringify(st));
+    assert(cmp(ra.macro, r1.macro).startsWith('0 '), report);
+    assert(cmp(ra.forces, r1.forces).startsWith('0 '), 'forces differ: ' + report);
+    assert(cmp(r1.macro, dm).startsWith('0 '), report);
+    return 'p50 ' + st.p50Latency.toFixed(3) + ' s; ' + report;
   });
 
   await ok('invalid descriptor raises with the library message', () => {

@@ -110,6 +110,54 @@ def test_budget_admission_and_refusal():
     assert st["completed"] == 5 and st["p99_latency"] >= st["p50_latency"] > 0
 
 
+def test_alone_equals_mix_all_submitted_at_once():
+    # the JS host test's mix: a small f32 TRT sphere, a D2Q9 TGV and a second
+    # f32 sphere, all submitted before the first chunk
+    sphere = sc.sphere(nx=48, ny=24, nz=24, radius=4.0, centre=(14.0, 12.0, 12.0), re=80.0)
+    tgv = sc.tgv2d(n=64)
+    sphere2 = sc.sphere(nx=32, ny=32, nz=32, radius=3.0, centre=(10.0, 16.0, 16.0), re=60.0)
+    with pylbm.Server(budget_bytes=1 << 30, chunk_steps=16) as sv:
+        t = sv.submit(sphere[0], sphere[1], 60)
+        sv.wait(t, 120)
+        a = sv.read_macro(t)
+    with pylbm.Server(budget_bytes=1 << 30, chunk_steps=16) as sv:
+        sv.submit(tgv[0], tgv[1], 50)
+        t = sv.submit(sphere[0], sphere[1], 60)
+        sv.submit(sphere2[0], sphere2[1], 40)
+        sv.drain(120)
+        b = sv.read_macro(t)
+    diff = np.nonzero(a.view(np.uint32) != b.view(np.uint32))[0]
+    say("differing", diff.size, "first", diff[:8], "max", float(np.abs(a - b).max()) if diff.size else 0.0)
+    assert diff.size == 0
+
+
+def test_joining_while_a_chunk_runs():
+    # a request admitted while another request's chunk is in flight takes no
+    # steps in that chunk: it must still run exactly its own step count
+    sphere = sc.sphere(nx=48, ny=24, nz=24, radius=4.0, centre=(14.0, 12.0, 12.0), re=80.0)
+    long_run = sc.tgv2d(n=256)
+    steps = 60
+    with pylbm.Server(budget_bytes=1 << 30, chunk_steps=8) as sv:
+        tl = sv.submit(long_run[0], long_run[1], 4000)
+        t_end = time.time() + 60
+        while sv.stats()["chunks"] < 2 and time.time() < t_end:
+            time.sleep(0.001)
+        t = sv.submit(sphere[0], sphere[1], steps)
+        r = sv.wait(t, 120)
+        served = sv.read_macro(t)
+        rl = sv.wait(tl, 120)
+    say("target admitted", r["t_admit"], "long request finished", rl["t_finish"])
+    assert r["t_admit"] < rl["t_finish"], "the target did not overlap a running chunk"
+    d = dict(sphere[0], autotune=0)
+    with pylbm.Solve(d) as s:
+        s.upload_macro(sphere[1])
+        s.step(steps)
+        plain = s.read_macro()
+    diff = np.nonzero(served.view(np.uint32) != plain.view(np.uint32))[0]
+    say("differing", diff.size, "max", float(np.abs(served - plain).max()) if diff.size else 0.0)
+    assert diff.size == 0
+
+
 def test_concurrent_submitters():
     reqs = mix()
     results = {}
