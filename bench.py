@@ -72,7 +72,7 @@ def measure(storage, steps, warmup, kernel_reps, periodic=False, tile=None):
         "achieved_gbs": achieved,
         "bytes_per_launch": bytes_per_launch,
         "ncells": ncells,
-        "tile": [info["block_x"], info["block_y"], info["swizzle"]],
+        "tile": [info["block_x"], info["block_y"], info["flags"]],
         "cd_last1000": cd,
         "finite": bool(np.all(np.isfinite(forces))),
     }
@@ -258,7 +258,8 @@ def main():
     ap.add_argument("--turbulence", action="store_true",
                     help="also run the 512^3 decaying-turbulence precision study (several minutes)")
     ap.add_argument("--tile", default=None,
-                    help="fixed tile bx,by,flags (flags bit0 XCD swizzle, bit1 non-temporal) instead of autotuning")
+                    help="fixed tile bx,by,flags (flags bit0 XCD swizzle, bit1 non-temporal, bits 4-7 waves-per-SIMD cap) "
+                         "instead of autotuning")
     args = ap.parse_args()
     tile = tuple(int(v) for v in args.tile.split(",")) if args.tile else None
     if args.gpus != 1:

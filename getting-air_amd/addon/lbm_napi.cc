@@ -314,8 +314,8 @@ napi_value Info(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   int64_t step = 0, bytes = 0;
-  int32_t bx = 0, by = 0, swz = 0;
-  int rc = lbm_solve_info(handle(env, argv[0]), &step, &bx, &by, &swz, &bytes);
+  int32_t bx = 0, by = 0, flags = 0;
+  int rc = lbm_solve_info(handle(env, argv[0]), &step, &bx, &by, &flags, &bytes);
   if (rc) return throw_lbm(env, rc);
   napi_value o, v;
   NAPI_OK(napi_create_object(env, &o));
@@ -325,8 +325,8 @@ napi_value Info(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, o, "blockX", v);
   napi_create_int32(env, by, &v);
   napi_set_named_property(env, o, "blockY", v);
-  napi_create_int32(env, swz, &v);
-  napi_set_named_property(env, o, "swizzle", v);
+  napi_create_int32(env, flags, &v);
+  napi_set_named_property(env, o, "flags", v);
   napi_create_double(env, (double)bytes, &v);
   napi_set_named_property(env, o, "deviceBytes", v);
   return o;

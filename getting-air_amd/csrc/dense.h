@@ -30,6 +30,9 @@ struct StepArgs {
   int32_t sponge_mask;
   float su[3];
   const uint8_t* cls;  // per-cell class (bit0 solid, bit1 near); null: no body
+  // Per-cell link mask, read only where cls bit1 is set: bit i is set when
+  // the streaming source of direction i (cell - e_i, periodic) is solid.
+  const uint32_t* links;
   int32_t solid_eq;
   int32_t solid_mac_rest;  // 3D: a solid cell reports [1, 0, 0, 0]
   float* mac;          // [rho, ux, uy, uz] per cell, or null (not observed)
@@ -107,18 +110,27 @@ struct ClassifyArgs {
   float radius;
   float cx, cy, cz;
   uint8_t* cls;
+  uint32_t* links;         // filled by launch_links from cls
 };
 
-hipError_t launch_step(int q, int storage, bool general, bool nontemporal, const void* fin, void* fout,
+// `occ` caps residency at that many waves per SIMD (0: no cap) by giving
+// each block its share of the CU's LDS; the step kernels stream HBM, and the
+// best number of loads in flight depends on the element width and the grid.
+hipError_t launch_step(int q, int storage, bool general, bool nontemporal, int occ, const void* fin, void* fout,
                        const StepArgs& a, int bx, int by, hipStream_t s);
-hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, const BatchArgs& b, uint32_t ntiles,
-                             int bx, int by, hipStream_t s);
+hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, const BatchArgs& b,
+                             uint32_t ntiles, int bx, int by, hipStream_t s);
+// Whether `occ` waves per SIMD can be set exactly for blocks of `threads`.
+bool occupancy_cap_ok(int occ, int threads);
 hipError_t launch_force_finalize(const ForceFinalizeArgs& a, hipStream_t s);
 hipError_t launch_totals(int q, int storage, const void* f, const TotalsArgs& a, int nblocks, hipStream_t s);
 hipError_t launch_init_eq(int q, int storage, const float* mac, void* f, uint32_t ncells, hipStream_t s);
 hipError_t launch_pack(int q, int storage, const float* wire, void* f, uint32_t ncells, hipStream_t s);
 hipError_t launch_unpack(int q, int storage, const void* f, float* wire, uint32_t ncells, hipStream_t s);
 hipError_t launch_classify(const ClassifyArgs& a, hipStream_t s);
+// Link masks of the cells next to the body (StepArgs::links) from the class
+// bytes of launch_classify; zero for every other cell.
+hipError_t launch_links(int q, const ClassifyArgs& a, hipStream_t s);
 // Streaming copy of `bytes` (multiple of 16) used to calibrate HBM bandwidth
 // and the PMC byte counters.
 hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);

@@ -74,10 +74,6 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
 
   if (active) {
     const uint32_t cell = (uint32_t)((z * ny + y) * nx + x);
-    // periodic neighbour coordinates
-    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
-    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
-    const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
 
     // 1. periodic pull-stream, all Q loads straight-line.  The source of
     // direction i is cell - e_i, written as per-axis index offsets that wrap
@@ -104,16 +100,17 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
       if ((c & 2) || onFace) {
         // 1b. halfway bounce-back: the body first, then the walls
         // (common_d3_step.wgsl:250-276).  w_i = w_opp(i), so a reflected
-        // population keeps its shift.
+        // population keeps its shift.  Which sources are solid comes from the
+        // cell's precomputed link mask (one load instead of a class byte per
+        // neighbour, which would keep Q addresses live in this rare branch and
+        // cost the whole kernel occupancy).
         double fx = 0, fy = 0, fz = 0;
+        const uint32_t lm = (c & 2) ? a.links[cell] : 0u;
 #pragma unroll
         for (int i = 1; i < Q; i++) {
           const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
-          const int sx = ex == 1 ? xm : (ex == -1 ? xp : x);
-          const int sy = ey == 1 ? ym : (ey == -1 ? yp : y);
-          const int sz = ez == 1 ? zm : (ez == -1 ? zp : z);
           const typename ST::T* oplane = gin + (size_t)L::opp[i] * N;
-          if ((c & 2) && (a.cls[(uint32_t)((sz * ny + sy) * nx + sx)] & 1)) {
+          if ((lm >> i) & 1u) {
             g[i] = ST::load(oplane, cell);
             // momentum exchange on a fluid cell: -e_i (2 f_opp), pinned body
             if (!(c & 1)) {
@@ -202,9 +199,14 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
           const float dX = (a.sponge_mask & 1) ? fminf((float)x, (float)(nx - 1 - x)) : big;
           const float dY = (a.sponge_mask & 2) ? fminf((float)y, (float)(ny - 1 - y)) : big;
           const float dZ = (a.sponge_mask & 4) ? fminf((float)z, (float)(nz - 1 - z)) : big;
-          float t = 1.f - fminf(dX, fminf(dY, dZ)) / a.sponge_w;
-          t = fminf(fmaxf(t, 0.f), 1.f);
-          sp = t * t * (3.f - 2.f * t);
+          const float d = fminf(dX, fminf(dY, dZ));
+          // beyond the band d / w >= 1 rounds to >= 1, so t clamps to 0 there:
+          // only cells inside the band pay for the division
+          if (d < a.sponge_w) {
+            float t = 1.f - d / a.sponge_w;
+            t = fminf(fmaxf(t, 0.f), 1.f);
+            sp = t * t * (3.f - 2.f * t);
+          }
         }
       }
 
@@ -277,7 +279,7 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
 }
 
 template <int Q, class ST, bool GEN, bool NT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q == 27 ? 6 : 8)))
 k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout, StepArgs a) {
   if constexpr (GEN) {
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
@@ -305,22 +307,29 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
 // launch index, so every launch of a captured chunk is one step of every
 // active solve.
 template <int Q, class ST, bool GEN, bool NT>
-__global__ void __launch_bounds__(256) k_step_batch(BatchArgs B) {
-  const BatchTile e = B.tiles[blockIdx.x];
-  const BatchSolve& S = B.solves[e.solve];
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q == 27 ? 6 : 8))) k_step_batch(BatchArgs B) {
+  // The tables are written before the launch and only read during it: read
+  // them through the constant address space, so their fields are scalar loads
+  // that may be repeated at will instead of values held in registers.
+  using CSolve = const __attribute__((address_space(4))) BatchSolve;
+  using CTile = const __attribute__((address_space(4))) BatchTile;
+  CTile* tp = ((CTile*)B.tiles) + blockIdx.x;
+  const uint32_t tile = tp->tile;
+  CSolve* sp = ((CSolve*)B.solves) + tp->solve;
+  CSolve& S = *sp;
   if (B.launch >= S.remaining) return;
   const int p = S.parity ^ (B.launch & 1);
-  const StepArgs& a = S.args[p];
+  const StepArgs& a = *(const StepArgs*)(&sp->args[p]);
   if constexpr (GEN) {
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    if (e.tile == 0 && (B.launch > 0 || S.pending) && a.force_slot && tid < 64)
+    if (tile == 0 && (B.launch > 0 || S.pending) && a.force_slot && tid < 64)
       sum_force_slots(a.prev_partials, a.nslots, a.history, a.written, a.hist_cap, tid);
   }
   // the dither counter of the solve is its step count
   const uint32_t salt = S.step0 + B.launch;
   float* mac = B.launch + 1 == S.remaining ? S.mac : nullptr;
   step_tile<Q, ST, GEN, NT>(static_cast<const typename ST::T*>(S.f[p]), static_cast<typename ST::T*>(S.f[p ^ 1]), a,
-                            e.tile, salt, mac);
+                            tile, salt, mac);
 }
 
 // One wave: lane l sums slots l, l+64, ... in order, then a fixed butterfly.
@@ -419,6 +428,30 @@ __global__ void __launch_bounds__(256) k_classify(ClassifyArgs a) {
   a.cls[cell] = c;
 }
 
+// Link masks for the bounce-back branch of the step: bit i of a cell next to
+// the body is set when its pull source cell - e_i (periodic) is solid.
+template <int Q>
+__global__ void __launch_bounds__(256) k_links(ClassifyArgs a) {
+  using L = Lattice<Q>;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const uint32_t n = (uint32_t)nx * ny * nz;
+  const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= n) return;
+  uint32_t m = 0;
+  if (a.cls[cell] & 2) {
+    const int x = cell % nx, y = (cell / nx) % ny, z = cell / (nx * ny);
+#pragma unroll
+    for (int i = 1; i < Q; i++) {
+      const int sx = x - L::ex[i], sy = y - L::ey[i], sz = z - L::ez[i];
+      const int wx = sx < 0 ? sx + nx : (sx >= nx ? sx - nx : sx);
+      const int wy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
+      const int wz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
+      if (a.cls[(uint32_t)((wz * ny + wy) * nx + wx)] & 1) m |= 1u << i;
+    }
+  }
+  a.links[cell] = m;
+}
+
 // 16-byte streaming copy, one element per thread with non-temporal loads and
 // stores: the practical HBM ceiling the roofline fraction is read against.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -450,35 +483,79 @@ static hipError_t dispatch(int q, int storage, A... args) {
 
 static inline dim3 cells_grid(uint32_t n) { return dim3((n + 255) / 256); }
 
+// LDS bytes per CU of the current device (160 KiB on gfx950).
+static size_t lds_per_cu() {
+  static const size_t bytes = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || v <= 0)
+      v = 160 * 1024;
+    return (size_t)v;
+  }();
+  return bytes;
+}
+
+// Blocks per CU that give `occ` waves on each of the CU's 4 SIMDs, and the
+// LDS share A = L / blocks that admits exactly that many: blocks * A <= L <
+// (blocks + 1) * A.  A is a multiple of 1 KiB for the shapes used here, so
+// the hardware's allocation granule does not round it up.
+static size_t occ_share(int occ, int threads) {
+  const int wpb = threads / 64;
+  if (occ <= 0 || wpb <= 0 || (4 * occ) % wpb) return 0;
+  const size_t blocks = (size_t)(4 * occ / wpb), L = lds_per_cu();
+  const size_t A = L / blocks / 1024 * 1024;
+  if (A == 0 || (blocks + 1) * A <= L) return 0;
+  return A;
+}
+bool occupancy_cap_ok(int occ, int threads) { return occ == 0 || occ_share(occ, threads) != 0; }
+
+// Dynamic LDS for a launch of kernel K under the cap: its share minus the
+// kernel's own static LDS (queried once per kernel).
+template <auto K>
+static size_t occ_dynamic_lds(int occ, int threads) {
+  const size_t A = occ_share(occ, threads);
+  if (!A) return 0;
+  static const size_t static_bytes = [] {
+    hipFuncAttributes fa{};
+    return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(K)) == hipSuccess ? fa.sharedSizeBytes
+                                                                                     : (size_t)0;
+  }();
+  return A > static_bytes ? A - static_bytes : 0;
+}
+
 template <int Q, class ST> struct OpStep {
   template <bool GEN, bool NT>
-  static void go(const void* fin, void* fout, const StepArgs& a, int bx, int by, hipStream_t s) {
+  static void go(int occ, const void* fin, void* fout, const StepArgs& a, int bx, int by, hipStream_t s) {
     using T = typename ST::T;
-    hipLaunchKernelGGL((k_step<Q, ST, GEN, NT>), dim3(a.nblocks), dim3(bx, by), 0, s, (const T*)fin, (T*)fout, a);
+    constexpr auto k = k_step<Q, ST, GEN, NT>;
+    hipLaunchKernelGGL(k, dim3(a.nblocks), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, (const T*)fin,
+                       (T*)fout, a);
   }
-  static hipError_t run(bool general, bool nt, const void* fin, void* fout, StepArgs a, int bx, int by, hipStream_t s) {
+  static hipError_t run(bool general, bool nt, int occ, const void* fin, void* fout, StepArgs a, int bx, int by,
+                        hipStream_t s) {
     if (general) {
-      if (nt) go<true, true>(fin, fout, a, bx, by, s);
-      else go<true, false>(fin, fout, a, bx, by, s);
+      if (nt) go<true, true>(occ, fin, fout, a, bx, by, s);
+      else go<true, false>(occ, fin, fout, a, bx, by, s);
     } else {
-      if (nt) go<false, true>(fin, fout, a, bx, by, s);
-      else go<false, false>(fin, fout, a, bx, by, s);
+      if (nt) go<false, true>(occ, fin, fout, a, bx, by, s);
+      else go<false, false>(occ, fin, fout, a, bx, by, s);
     }
     return hipGetLastError();
   }
 };
 template <int Q, class ST> struct OpStepBatch {
   template <bool GEN, bool NT>
-  static void go(const BatchArgs& b, uint32_t ntiles, int bx, int by, hipStream_t s) {
-    hipLaunchKernelGGL((k_step_batch<Q, ST, GEN, NT>), dim3(ntiles), dim3(bx, by), 0, s, b);
+  static void go(int occ, const BatchArgs& b, uint32_t ntiles, int bx, int by, hipStream_t s) {
+    constexpr auto k = k_step_batch<Q, ST, GEN, NT>;
+    hipLaunchKernelGGL(k, dim3(ntiles), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, b);
   }
-  static hipError_t run(bool general, bool nt, BatchArgs b, uint32_t ntiles, int bx, int by, hipStream_t s) {
+  static hipError_t run(bool general, bool nt, int occ, BatchArgs b, uint32_t ntiles, int bx, int by, hipStream_t s) {
     if (general) {
-      if (nt) go<true, true>(b, ntiles, bx, by, s);
-      else go<true, false>(b, ntiles, bx, by, s);
+      if (nt) go<true, true>(occ, b, ntiles, bx, by, s);
+      else go<true, false>(occ, b, ntiles, bx, by, s);
     } else {
-      if (nt) go<false, true>(b, ntiles, bx, by, s);
-      else go<false, false>(b, ntiles, bx, by, s);
+      if (nt) go<false, true>(occ, b, ntiles, bx, by, s);
+      else go<false, false>(occ, b, ntiles, bx, by, s);
     }
     return hipGetLastError();
   }
@@ -508,14 +585,14 @@ template <int Q, class ST> struct OpTotals {
   }
 };
 
-hipError_t launch_step(int q, int storage, bool general, bool nontemporal, const void* fin, void* fout,
+hipError_t launch_step(int q, int storage, bool general, bool nontemporal, int occ, const void* fin, void* fout,
                        const StepArgs& a, int bx, int by, hipStream_t s) {
-  return dispatch<OpStep>(q, storage, general, nontemporal, fin, fout, a, bx, by, s);
+  return dispatch<OpStep>(q, storage, general, nontemporal, occ, fin, fout, a, bx, by, s);
 }
-hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, const BatchArgs& b, uint32_t ntiles,
-                             int bx, int by, hipStream_t s) {
+hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, const BatchArgs& b,
+                             uint32_t ntiles, int bx, int by, hipStream_t s) {
   if (ntiles == 0) return hipSuccess;
-  return dispatch<OpStepBatch>(q, storage, general, nontemporal, b, ntiles, bx, by, s);
+  return dispatch<OpStepBatch>(q, storage, general, nontemporal, occ, b, ntiles, bx, by, s);
 }
 hipError_t launch_init_eq(int q, int storage, const float* mac, void* f, uint32_t N, hipStream_t s) {
   return dispatch<OpInit>(q, storage, mac, f, N, s);
@@ -536,6 +613,14 @@ hipError_t launch_force_finalize(const ForceFinalizeArgs& a, hipStream_t s) {
 hipError_t launch_classify(const ClassifyArgs& a, hipStream_t s) {
   const uint32_t n = (uint32_t)a.nx * a.ny * a.nz;
   hipLaunchKernelGGL(k_classify, dim3((n + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_links(int q, const ClassifyArgs& a, hipStream_t s) {
+  const uint32_t n = (uint32_t)a.nx * a.ny * a.nz;
+  const dim3 grid((n + 255) / 256);
+  if (q == 9) hipLaunchKernelGGL(k_links<9>, grid, dim3(256), 0, s, a);
+  else if (q == 19) hipLaunchKernelGGL(k_links<19>, grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_links<27>, grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s) {

@@ -71,7 +71,7 @@ struct TuneKey {
     return std::tie(q, storage, general, nx, ny, nz) < std::tie(o.q, o.storage, o.general, o.nx, o.ny, o.nz);
   }
 };
-struct TuneVal { int bx, by, swz, nt; };
+struct TuneVal { int bx, by, swz, nt, occ; };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneVal> g_tune_cache;
 
@@ -95,6 +95,7 @@ struct lbm_solve {
   int parity = 0;
   float* mac = nullptr;
   uint8_t* cls = nullptr;
+  uint32_t* links = nullptr;        // bounce-back link masks of the cells next to the body
   std::vector<uint32_t> boundary;   // fluid cells with a solid streaming source
   int32_t* force_slot = nullptr;    // per tile of the current shape
   double* force_partials = nullptr;
@@ -110,13 +111,14 @@ struct lbm_solve {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int bx = 64, by = 1, swz = 1, nt = 0;
+  int occ = 0;                      // waves-per-SIMD cap of the step launches (0: none)
   bool force_pending = false;       // last step's force partials not yet summed
   bool tuned = false;
   bool initialised = false;
   long long step = 0;
   size_t bytes = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
-  int graph_bx = 0, graph_by = 0, graph_swz = 0, graph_nt = 0;
+  int graph_bx = 0, graph_by = 0, graph_swz = 0, graph_nt = 0, graph_occ = 0;
   Arena* arena = nullptr;           // pool the buffers come from (serve loop), or null
   bool own_stream = true;
 };
@@ -211,6 +213,7 @@ StepArgs step_args(lbm_solve* s, bool write_mac, bool with_force) {
   a.sponge_mask = (d.sponge_axes[0] ? 1 : 0) | (d.sponge_axes[1] ? 2 : 0) | (d.sponge_axes[2] && s->dim == 3 ? 4 : 0);
   for (int k = 0; k < 3; k++) a.su[k] = d.sponge_u[k];
   a.cls = s->body ? s->cls : nullptr;
+  a.links = s->body ? s->links : nullptr;
   a.solid_eq = d.solid_eq;
   a.solid_mac_rest = s->dim == 3;
   a.mac = write_mac ? s->mac : nullptr;
@@ -284,7 +287,7 @@ int enqueue_step(lbm_solve* s, bool write_mac, bool finalize_prev) {
   void* fout = s->f[1 - s->parity];
   StepArgs a = step_args(s, write_mac, true);
   a.finalize_prev = finalize_prev && a.force_slot ? 1 : 0;
-  hipError_t e = launch_step(s->q, s->storage, s->general, s->nt, fin, fout, a, s->bx, s->by, s->stream);
+  hipError_t e = launch_step(s->q, s->storage, s->general, s->nt, s->occ, fin, fout, a, s->bx, s->by, s->stream);
   if (e != hipSuccess) return hip_fail(e, "step kernel launch");
   s->force_pending = a.force_slot != nullptr;
   s->parity ^= 1;
@@ -315,7 +318,7 @@ int solve_enqueue_covered_step(lbm_solve* s, const int32_t* cover, int rb, int n
   a.cover_rb = rb;
   a.cover_nbx = nbx;
   a.cover_nby = nby;
-  hipError_t e = launch_step(s->q, s->storage, true, s->nt, fin, fout, a, s->bx, s->by, s->stream);
+  hipError_t e = launch_step(s->q, s->storage, true, s->nt, s->occ, fin, fout, a, s->bx, s->by, s->stream);
   if (e != hipSuccess) return hip_fail(e, "covered step launch");
   s->parity ^= 1;
   s->step++;
@@ -360,16 +363,18 @@ int build_graph(lbm_solve* s) {
   e = hipGraphInstantiate(&s->graph[p], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-  s->graph_bx = s->bx; s->graph_by = s->by; s->graph_swz = s->swz; s->graph_nt = s->nt;
+  s->graph_bx = s->bx; s->graph_by = s->by; s->graph_swz = s->swz; s->graph_nt = s->nt; s->graph_occ = s->occ;
   return LBM_OK;
 }
 
-// Times every candidate tile shape, XCD mapping and load/store flavour on the
-// live lattices (fin -> fout without swapping, so the state is unchanged) and
-// keeps the fastest.
+// Times every candidate tile shape, XCD mapping, load/store flavour and
+// occupancy cap on the live lattices (fin -> fout without swapping, so the
+// state is unchanged), then times the four fastest again, interleaved, and
+// keeps the fastest of those: one short timing per candidate is too noisy to
+// separate shapes within a few percent.
 int autotune(lbm_solve* s) {
   TuneKey key{s->q, s->storage, s->general ? 1 : 0, s->d.nx, s->d.ny, s->d.nz};
-  TuneVal bestv{s->bx, s->by, s->swz, s->nt};
+  TuneVal bestv{s->bx, s->by, s->swz, s->nt, s->occ};
   bool cached = false;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -381,31 +386,51 @@ int autotune(lbm_solve* s) {
   }
   if (!cached) {
     const int reps = s->N >= (1u << 20) ? 3 : 8;
-    float best = 1e30f;
     const void* fin = s->f[s->parity];
     void* fout = s->f[1 - s->parity];
+    auto time_of = [&](const TuneVal& v, int n, float* ms) -> int {
+      s->bx = v.bx; s->by = v.by; s->swz = v.swz; s->nt = v.nt; s->occ = v.occ;
+      StepArgs a = step_args(s, false, false);
+      HIPCHK(launch_step(s->q, s->storage, s->general, v.nt, v.occ, fin, fout, a, v.bx, v.by, s->stream));
+      HIPCHK(hipEventRecord(s->ev0, s->stream));
+      for (int r = 0; r < n; r++)
+        HIPCHK(launch_step(s->q, s->storage, s->general, v.nt, v.occ, fin, fout, a, v.bx, v.by, s->stream));
+      HIPCHK(hipEventRecord(s->ev1, s->stream));
+      HIPCHK(hipEventSynchronize(s->ev1));
+      HIPCHK(hipEventElapsedTime(ms, s->ev0, s->ev1));
+      return LBM_OK;
+    };
+    std::vector<std::pair<float, TuneVal>> cand;
     for (const auto& sh : kShapes) {
       if (sh[0] > 64 && s->d.nx < sh[0] / 2) continue;
-      for (int swz = 0; swz < 2; swz++) {
-        for (int nt = 0; nt < 2; nt++) {
-          s->bx = sh[0]; s->by = sh[1]; s->swz = swz; s->nt = nt;
-          StepArgs a = step_args(s, false, false);
-          HIPCHK(launch_step(s->q, s->storage, s->general, nt, fin, fout, a, s->bx, s->by, s->stream));
-          HIPCHK(hipEventRecord(s->ev0, s->stream));
-          for (int r = 0; r < reps; r++)
-            HIPCHK(launch_step(s->q, s->storage, s->general, nt, fin, fout, a, s->bx, s->by, s->stream));
-          HIPCHK(hipEventRecord(s->ev1, s->stream));
-          HIPCHK(hipEventSynchronize(s->ev1));
-          float ms = 0.f;
-          HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-          if (ms < best) { best = ms; bestv = TuneVal{sh[0], sh[1], swz, nt}; }
-        }
+      for (int occ : {0, 4}) {
+        if (!occupancy_cap_ok(occ, sh[0] * sh[1])) continue;
+        for (int swz = 0; swz < 2; swz++)
+          for (int nt = 0; nt < 2; nt++) {
+            const TuneVal v{sh[0], sh[1], swz, nt, occ};
+            float ms = 0.f;
+            int rc = time_of(v, reps, &ms);
+            if (rc) return rc;
+            cand.push_back({ms, v});
+          }
       }
     }
+    std::sort(cand.begin(), cand.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    cand.resize(std::min<size_t>(cand.size(), 4));
+    for (auto& c : cand) c.first = 1e30f;
+    for (int round = 0; round < 3; round++)
+      for (auto& c : cand) {
+        float ms = 0.f;
+        int rc = time_of(c.second, 2 * reps, &ms);
+        if (rc) return rc;
+        c.first = std::min(c.first, ms);
+      }
+    std::sort(cand.begin(), cand.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    if (!cand.empty()) bestv = cand.front().second;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = bestv;
   }
-  s->bx = bestv.bx; s->by = bestv.by; s->swz = bestv.swz; s->nt = bestv.nt;
+  s->bx = bestv.bx; s->by = bestv.by; s->swz = bestv.swz; s->nt = bestv.nt; s->occ = bestv.occ;
   s->tuned = true;
   return build_force_slots(s);
 }
@@ -420,7 +445,9 @@ int build_boundary(lbm_solve* s) {
   c.radius = d.body.radius;
   c.cx = d.body.centre[0]; c.cy = d.body.centre[1]; c.cz = s->dim == 3 ? d.body.centre[2] : 0.f;
   c.cls = s->cls;
+  c.links = s->links;
   HIPCHK(launch_classify(c, s->stream));
+  HIPCHK(launch_links(s->q, c, s->stream));
   std::vector<uint8_t> h(s->N);
   HIPCHK(hipMemcpyAsync(h.data(), s->cls, s->N, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -449,7 +476,7 @@ int build_boundary(lbm_solve* s) {
 void release(lbm_solve* s) {
   if (!s) return;
   drop_graphs(s);
-  void* ptrs[] = {s->f[0], s->f[1], s->mac, s->cls, s->force_slot, s->force_partials, s->scratch,
+  void* ptrs[] = {s->f[0], s->f[1], s->mac, s->cls, s->links, s->force_slot, s->force_partials, s->scratch,
                   s->counters, s->history, s->written, s->totals, s->salt};
   for (void* p : ptrs) {
     if (!p) continue;
@@ -523,7 +550,8 @@ int create_impl(const lbm_desc* desc, Arena* arena, hipStream_t stream, int bx, 
     return hip_fail(e, "hipMemsetAsync");
   }
   if (s->body) {
-    if ((rc = dalloc(s, &s->cls, s->N)) || (rc = build_boundary(s))) {
+    if ((rc = dalloc(s, &s->cls, s->N)) || (rc = dalloc(s, &s->links, (size_t)s->N * 4)) ||
+        (rc = build_boundary(s))) {
       release(s);
       return rc;
     }
@@ -547,10 +575,10 @@ size_t solve_footprint(const lbm_desc* d) {
   size_t total = 2 * al(n * d->q * elem) + al(n * 16) + al((size_t)kTotalsBlocks * 4 * 8) + al(64) +
                  al((size_t)kHistoryCap * 6 * 8) + al(8) + al(64) + al(8);
   if (d->body.shape != LBM_BODY_NONE) {
-    // class bytes, one slot index per tile (tiles of at least 64 cells) and
-    // two parity sets of 6 partials per slot (at most one slot per tile)
+    // class bytes, link masks, one slot index per tile (tiles of at least 64
+    // cells) and two parity sets of 6 partials per slot (at most one per tile)
     const size_t tiles = (n + 63) / 64 + d->ny * (d->q == 9 ? 1 : d->nz);
-    total += al(n) + al(tiles * 4) + al(2 * tiles * 6 * 8);
+    total += al(n) + al(n * 4) + al(tiles * 4) + al(2 * tiles * 6 * 8);
   }
   return total;
 }
@@ -674,7 +702,8 @@ int lbm_step(lbm_solve* s, int32_t n) {
     int rc = autotune(s);
     if (rc) return rc;
   }
-  if (s->graph_bx != s->bx || s->graph_by != s->by || s->graph_swz != s->swz || s->graph_nt != s->nt)
+  if (s->graph_bx != s->bx || s->graph_by != s->by || s->graph_swz != s->swz || s->graph_nt != s->nt ||
+      s->graph_occ != s->occ)
     drop_graphs(s);
   // First step eagerly (nothing earlier in this call to finalize), then
   // captured chunks and single steps that each finalize their predecessor's
@@ -780,13 +809,13 @@ int lbm_read_classes(lbm_solve* s, uint8_t* out) {
   return LBM_OK;
 }
 
-int lbm_solve_info(lbm_solve* s, int64_t* step, int32_t* block_x, int32_t* block_y, int32_t* swizzle,
+int lbm_solve_info(lbm_solve* s, int64_t* step, int32_t* block_x, int32_t* block_y, int32_t* flags,
                    int64_t* device_bytes) {
   if (!s) return fail(LBM_E_INVALID, "null solve");
   if (step) *step = s->step;
   if (block_x) *block_x = s->bx;
   if (block_y) *block_y = s->by;
-  if (swizzle) *swizzle = s->swz | (s->nt << 1);
+  if (flags) *flags = s->swz | (s->nt << 1) | (s->occ << 4);
   if (device_bytes) *device_bytes = (int64_t)s->bytes;
   return LBM_OK;
 }
@@ -801,10 +830,10 @@ int lbm_time_step_kernel(lbm_solve* s, int32_t n, double* ms_per_launch) {
   const void* fin = s->f[s->parity];
   void* fout = s->f[1 - s->parity];
   StepArgs a = step_args(s, false, true);
-  HIPCHK(launch_step(s->q, s->storage, s->general, s->nt, fin, fout, a, s->bx, s->by, s->stream));
+  HIPCHK(launch_step(s->q, s->storage, s->general, s->nt, s->occ, fin, fout, a, s->bx, s->by, s->stream));
   HIPCHK(hipEventRecord(s->ev0, s->stream));
   for (int r = 0; r < n; r++)
-    HIPCHK(launch_step(s->q, s->storage, s->general, s->nt, fin, fout, a, s->bx, s->by, s->stream));
+    HIPCHK(launch_step(s->q, s->storage, s->general, s->nt, s->occ, fin, fout, a, s->bx, s->by, s->stream));
   HIPCHK(hipEventRecord(s->ev1, s->stream));
   HIPCHK(hipEventSynchronize(s->ev1));
   float t = 0.f;
@@ -813,14 +842,19 @@ int lbm_time_step_kernel(lbm_solve* s, int32_t n, double* ms_per_launch) {
   return LBM_OK;
 }
 
-int lbm_set_tile(lbm_solve* s, int32_t block_x, int32_t block_y, int32_t swizzle) {
+int lbm_set_tile(lbm_solve* s, int32_t block_x, int32_t block_y, int32_t flags) {
   if (!s) return fail(LBM_E_INVALID, "null solve");
   if (block_x < 1 || block_y < 1 || block_x * block_y > 256 || (block_x * block_y) % 64)
     return fail(LBM_E_INVALID, "tile %dx%d: need a multiple of 64 threads, at most 256", block_x, block_y);
+  const int occ = (flags >> 4) & 15;
+  if (!occupancy_cap_ok(occ, block_x * block_y))
+    return fail(LBM_E_INVALID, "occupancy cap %d waves per SIMD cannot be set for %d-thread blocks", occ,
+                block_x * block_y);
   s->bx = block_x;
   s->by = block_y;
-  s->swz = swizzle & 1;
-  s->nt = (swizzle >> 1) & 1;
+  s->swz = flags & 1;
+  s->nt = (flags >> 1) & 1;
+  s->occ = occ;
   s->tuned = true;
   return build_force_slots(s);
 }

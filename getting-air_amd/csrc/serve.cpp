@@ -148,7 +148,7 @@ struct ClassKey {
 
 struct Class {
   ClassKey key{};
-  int bx = 64, by = 1, nt = 0;
+  int bx = 64, by = 1, nt = 0, occ = 0;
   std::vector<Member*> members;
   bool dirty = true;  // membership changed since the tables were built
   BatchSolve* d_solves = nullptr;
@@ -231,6 +231,7 @@ void choose_tile(lbm_serve* sv, Class* c, const Request& r) {
     c->bx = bx;
     c->by = by;
     c->nt = (flags >> 1) & 1;
+    c->occ = (flags >> 4) & 15;
   }
   lbm_solve_destroy(t);
 }
@@ -347,8 +348,8 @@ int enqueue_chunk(lbm_serve* sv, int K) {
       Class* c = kv.second.get();
       if (c->members.empty()) continue;
       BatchArgs b{c->d_solves, c->d_tiles, j};
-      HIPCHK_RET(launch_step_batch(c->key.q, c->key.storage, c->key.general != 0, c->nt != 0, b, c->ntiles, c->bx,
-                                   c->by, sv->stream));
+      HIPCHK_RET(launch_step_batch(c->key.q, c->key.storage, c->key.general != 0, c->nt != 0, c->occ, b, c->ntiles,
+                                   c->bx, c->by, sv->stream));
     }
   }
   return LBM_OK;

@@ -291,15 +291,16 @@ class Solve:
 
     def info(self):
         step = ctypes.c_int64()
-        bx, by, swz = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        bx, by, flags = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         nbytes = ctypes.c_int64()
         _check(self.lib.lbm_solve_info(self.h, ctypes.byref(step), ctypes.byref(bx), ctypes.byref(by),
-                                       ctypes.byref(swz), ctypes.byref(nbytes)))
-        return {"step": step.value, "block_x": bx.value, "block_y": by.value, "swizzle": swz.value,
+                                       ctypes.byref(flags), ctypes.byref(nbytes)))
+        return {"step": step.value, "block_x": bx.value, "block_y": by.value, "flags": flags.value,
                 "device_bytes": nbytes.value}
 
-    def set_tile(self, bx, by, swizzle):
-        _check(self.lib.lbm_set_tile(self.h, int(bx), int(by), int(swizzle)))
+    def set_tile(self, bx, by, flags):
+        """flags: bit 0 XCD swizzle, bit 1 non-temporal, bits 4-7 waves-per-SIMD cap."""
+        _check(self.lib.lbm_set_tile(self.h, int(bx), int(by), int(flags)))
 
     def time_step_kernel(self, n):
         """Mean device milliseconds of one fused stream-collide launch."""

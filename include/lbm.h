@@ -127,13 +127,16 @@ int lbm_read_totals(lbm_solve* s, double* out4);
  * (phi < 0), bit 1 = phi < 2 (a cell that can have a solid neighbour). */
 int lbm_read_classes(lbm_solve* s, uint8_t* out);
 
-/* Current step count, tile shape and device bytes held by the solve. */
+/* Current step count, tile shape, tile flags (as lbm_set_tile takes them)
+ * and device bytes held by the solve. */
 int lbm_solve_info(lbm_solve* s, int64_t* step, int32_t* block_x, int32_t* block_y,
-                   int32_t* swizzle, int64_t* device_bytes);
+                   int32_t* flags, int64_t* device_bytes);
 
-/* Overrides the autotuned tile shape (block_x * block_y threads; swizzle 1
- * maps consecutive tiles onto one XCD). */
-int lbm_set_tile(lbm_solve* s, int32_t block_x, int32_t block_y, int32_t swizzle);
+/* Overrides the autotuned tile shape: block_x * block_y threads (a multiple
+ * of 64, at most 256); flags bit 0 maps consecutive tiles onto one XCD, bit 1
+ * selects non-temporal loads and stores, bits 4-7 cap residency at that many
+ * waves per SIMD (0: no cap; 4 is available for every shape). */
+int lbm_set_tile(lbm_solve* s, int32_t block_x, int32_t block_y, int32_t flags);
 
 /* Measurement hook, no counterpart in the reference (its spike page times
  * passes with timestamp queries, main-3d-spike.js:229-286): launches the

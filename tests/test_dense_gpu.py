@@ -216,3 +216,31 @@ def test_sphere_full_config_short_run_and_symmetry():
     assert np.all(np.isfinite(f500))
     tail = f500[-100:]
     assert np.abs(tail[:, 1:3]).max() < 0.05 * np.abs(tail[:, 0]).mean()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_tile_shapes_and_flags_give_the_same_field(storage):
+    # every cell's update depends on its own pulls only, and the 16-bit dither
+    # on (cell, step): the field is the same bits for every tile shape, XCD
+    # mapping, load flavour and occupancy cap; the force sums change order
+    desc, mac, _ = sphere_small(trt=True)
+    desc = dict(desc, storage=storage, autotune=0)
+    steps = 40
+    runs = []
+    for bx, by, flags in [(64, 1, 1), (256, 1, 2), (128, 2, 0x43), (64, 4, 0x40), (64, 1, 0x42), (32, 8, 0)]:
+        with pylbm.Solve(desc) as s:
+            s.set_tile(bx, by, flags)
+            s.upload_macro(mac)
+            s.step(steps)
+            assert s.info()["flags"] == flags
+            runs.append(((bx, by, flags), s.read_macro(), s.read_forces()))
+    ref_key, ref_mac, ref_f = runs[0]
+    for key, m, f in runs[1:]:
+        assert np.array_equal(m.view(np.uint32), ref_mac.view(np.uint32)), (key, ref_key)
+        scale = np.abs(ref_f[:, 0]).max()
+        assert f.shape == ref_f.shape and np.abs(f - ref_f).max() <= 1e-12 * scale, key
+    with pylbm.Solve(desc) as s:
+        with pytest.raises(pylbm.LbmError):
+            # 7 waves per SIMD of 1-wave blocks: 28 blocks per CU, and an LDS
+            # share of L / 28 (rounded to 1 KiB) would still admit a 29th
+            s.set_tile(64, 1, 0x70)
