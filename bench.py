@@ -258,7 +258,8 @@ def main():
     ap.add_argument("--turbulence", action="store_true",
                     help="also run the 512^3 decaying-turbulence precision study (several minutes)")
     ap.add_argument("--tile", default=None,
-                    help="fixed tile bx,by,flags (flags bit0 XCD swizzle, bit1 non-temporal, bits 4-7 waves-per-SIMD cap) "
+                    help="fixed tile bx,by,flags (flags bit0 XCD swizzle, bit1 non-temporal, bit2 linear, bits 4-7 "
+                         "waves-per-SIMD cap) "
                          "instead of autotuning")
     args = ap.parse_args()
     tile = tuple(int(v) for v in args.tile.split(",")) if args.tile else None

@@ -9,6 +9,23 @@ namespace lbm {
 
 enum Storage { kF32 = 0, kF16 = 1, kBF16 = 2 };
 
+// floor(n / d) for 0 <= n < 2^31 as a multiply and a shift: with
+// l = ceil(log2 d) and m = ceil(2^(31+l) / d), 2^(31+l) <= m d <= 2^(31+l) + 2^l,
+// which is the condition of Granlund & Montgomery ("Division by invariant
+// integers using multiplication", PLDI 1994, Theorem 4.2) for N = 31.
+// m < 2^32 for every d <= 2^31, so n m fits 64 bits.
+struct InvariantDiv {
+  uint32_t m, shift;
+};
+inline InvariantDiv make_invariant_div(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) l++;
+  return InvariantDiv{(uint32_t)(((1ull << (31 + l)) + d - 1) / d), 31 + l};
+}
+__host__ __device__ inline uint32_t div_invariant(uint32_t n, InvariantDiv v) {
+  return (uint32_t)(((uint64_t)n * v.m) >> v.shift);
+}
+
 // Everything one fused stream-collide launch needs besides the two lattices.
 struct StepArgs {
   int32_t nx, ny, nz;
@@ -56,9 +73,14 @@ struct StepArgs {
   uint32_t hist_cap;
   double body_c[3];           // torque arm origin
   int32_t periodic_arm;       // 2D: nearest periodic image for the arm
-  // tiling of the 1-D grid onto (x-tiles, y-tiles, z)
+  // tiling of the 1-D grid onto (x-tiles, y-tiles, z), or, with `linear`,
+  // block b covering cells [b T, (b + 1) T) of the flattened index (T threads
+  // per block), so that no lane idles when nx is not a multiple of the block
+  // width; x, y, z then come from two divisions by invariant integers
   int32_t ntx, nty;
   uint32_t nblocks;
+  int32_t linear;
+  InvariantDiv div_nx, div_ny;
   int32_t swizzle;     // 1: consecutive tiles on one XCD
   // Step counter of the dither of 16-bit stores: this launch reads *salt_in
   // and block 0 writes *salt_out = *salt_in + 1 for the next (the two words

@@ -57,14 +57,25 @@ template <int Q, class ST, bool GEN, bool NT>
 __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout,
                                           const StepArgs& a, uint32_t b, uint32_t salt, float* mac) {
   using L = Lattice<Q>;
-  const int tx = b % a.ntx;
-  const int rest = b / a.ntx;
-  const int ty = rest % a.nty;
-  const int z = rest / a.nty;
-  const int x = tx * blockDim.x + threadIdx.x;
-  const int y = ty * blockDim.y + threadIdx.y;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  bool active = x < nx && y < ny;
+  int x, y, z;
+  bool active;
+  if (a.linear) {
+    const uint32_t c = b * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
+    const uint32_t row = div_invariant(c, a.div_nx);
+    z = (int)div_invariant(row, a.div_ny);
+    y = (int)(row - (uint32_t)z * ny);
+    x = (int)(c - row * nx);
+    active = c < a.ncells;
+  } else {
+    const int tx = b % a.ntx;
+    const int rest = b / a.ntx;
+    const int ty = rest % a.nty;
+    z = rest / a.nty;
+    x = tx * blockDim.x + threadIdx.x;
+    y = ty * blockDim.y + threadIdx.y;
+    active = x < nx && y < ny;
+  }
   if constexpr (GEN) {
     if (active && a.cover)
       active = a.cover[((z / a.cover_rb) * a.cover_nby + y / a.cover_rb) * a.cover_nbx + x / a.cover_rb] < 0;

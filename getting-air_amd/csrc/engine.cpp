@@ -71,7 +71,7 @@ struct TuneKey {
     return std::tie(q, storage, general, nx, ny, nz) < std::tie(o.q, o.storage, o.general, o.nx, o.ny, o.nz);
   }
 };
-struct TuneVal { int bx, by, swz, nt, occ; };
+struct TuneVal { int bx, by, swz, nt, occ, lin; };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneVal> g_tune_cache;
 
@@ -100,7 +100,7 @@ struct lbm_solve {
   int32_t* force_slot = nullptr;    // per tile of the current shape
   double* force_partials = nullptr;
   uint32_t nslots = 0;
-  int slot_bx = 0, slot_by = 0;
+  int slot_bx = 0, slot_by = 0, slot_lin = -1;
   size_t slot_bytes = 0, partial_bytes = 0;
   double* scratch = nullptr;        // totals partials
   unsigned* counters = nullptr;
@@ -112,13 +112,14 @@ struct lbm_solve {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int bx = 64, by = 1, swz = 1, nt = 0;
   int occ = 0;                      // waves-per-SIMD cap of the step launches (0: none)
+  int lin = 0;                      // linear tiles (blocks of consecutive cells)
   bool force_pending = false;       // last step's force partials not yet summed
   bool tuned = false;
   bool initialised = false;
   long long step = 0;
   size_t bytes = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
-  int graph_bx = 0, graph_by = 0, graph_swz = 0, graph_nt = 0, graph_occ = 0;
+  int graph_bx = 0, graph_by = 0, graph_swz = 0, graph_nt = 0, graph_occ = 0, graph_lin = 0;
   Arena* arena = nullptr;           // pool the buffers come from (serve loop), or null
   bool own_stream = true;
 };
@@ -234,9 +235,17 @@ StepArgs step_args(lbm_solve* s, bool write_mac, bool with_force) {
   a.body_c[1] = d.body.centre[1];
   a.body_c[2] = s->dim == 3 ? d.body.centre[2] : 0.0;
   a.periodic_arm = s->dim == 2;
-  a.ntx = (d.nx + s->bx - 1) / s->bx;
-  a.nty = (d.ny + s->by - 1) / s->by;
-  a.nblocks = (uint32_t)a.ntx * a.nty * d.nz;
+  if (s->lin) {
+    const uint32_t t = (uint32_t)(s->bx * s->by);
+    a.linear = 1;
+    a.nblocks = (s->N + t - 1) / t;
+    a.div_nx = make_invariant_div((uint32_t)d.nx);
+    a.div_ny = make_invariant_div((uint32_t)d.ny);
+  } else {
+    a.ntx = (d.nx + s->bx - 1) / s->bx;
+    a.nty = (d.ny + s->by - 1) / s->by;
+    a.nblocks = (uint32_t)a.ntx * a.nty * d.nz;
+  }
   a.swizzle = s->swz;
   if (s->q == 9) pair_constants<9>(a, d);
   else if (s->q == 19) pair_constants<19>(a, d);
@@ -248,14 +257,15 @@ StepArgs step_args(lbm_solve* s, bool write_mac, bool with_force) {
 // in ascending tile order (so the finalize sum order is fixed per shape).
 int build_force_slots(lbm_solve* s) {
   if (!s->body) return LBM_OK;
-  if (s->slot_bx == s->bx && s->slot_by == s->by && s->force_slot) return LBM_OK;
+  if (s->slot_bx == s->bx && s->slot_by == s->by && s->slot_lin == s->lin && s->force_slot) return LBM_OK;
   const lbm_desc& d = s->d;
   const int ntx = (d.nx + s->bx - 1) / s->bx, nty = (d.ny + s->by - 1) / s->by;
-  const size_t ntiles = (size_t)ntx * nty * d.nz;
+  const uint32_t t = (uint32_t)(s->bx * s->by);
+  const size_t ntiles = s->lin ? (s->N + t - 1) / t : (size_t)ntx * nty * d.nz;
   std::vector<int32_t> slot(ntiles, -1);
   for (uint32_t cell : s->boundary) {
     const int x = cell % d.nx, y = (cell / d.nx) % d.ny, z = cell / (d.nx * d.ny);
-    slot[((size_t)z * nty + y / s->by) * ntx + x / s->bx] = 0;
+    slot[s->lin ? cell / t : ((size_t)z * nty + y / s->by) * ntx + x / s->bx] = 0;
   }
   uint32_t n = 0;
   for (auto& v : slot)
@@ -276,6 +286,7 @@ int build_force_slots(lbm_solve* s) {
   s->nslots = n;
   s->slot_bx = s->bx;
   s->slot_by = s->by;
+  s->slot_lin = s->lin;
   return LBM_OK;
 }
 
@@ -364,6 +375,7 @@ int build_graph(lbm_solve* s) {
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
   s->graph_bx = s->bx; s->graph_by = s->by; s->graph_swz = s->swz; s->graph_nt = s->nt; s->graph_occ = s->occ;
+  s->graph_lin = s->lin;
   return LBM_OK;
 }
 
@@ -374,7 +386,7 @@ int build_graph(lbm_solve* s) {
 // separate shapes within a few percent.
 int autotune(lbm_solve* s) {
   TuneKey key{s->q, s->storage, s->general ? 1 : 0, s->d.nx, s->d.ny, s->d.nz};
-  TuneVal bestv{s->bx, s->by, s->swz, s->nt, s->occ};
+  TuneVal bestv{s->bx, s->by, s->swz, s->nt, s->occ, s->lin};
   bool cached = false;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -389,7 +401,7 @@ int autotune(lbm_solve* s) {
     const void* fin = s->f[s->parity];
     void* fout = s->f[1 - s->parity];
     auto time_of = [&](const TuneVal& v, int n, float* ms) -> int {
-      s->bx = v.bx; s->by = v.by; s->swz = v.swz; s->nt = v.nt; s->occ = v.occ;
+      s->bx = v.bx; s->by = v.by; s->swz = v.swz; s->nt = v.nt; s->occ = v.occ; s->lin = v.lin;
       StepArgs a = step_args(s, false, false);
       HIPCHK(launch_step(s->q, s->storage, s->general, v.nt, v.occ, fin, fout, a, v.bx, v.by, s->stream));
       HIPCHK(hipEventRecord(s->ev0, s->stream));
@@ -401,20 +413,23 @@ int autotune(lbm_solve* s) {
       return LBM_OK;
     };
     std::vector<std::pair<float, TuneVal>> cand;
-    for (const auto& sh : kShapes) {
-      if (sh[0] > 64 && s->d.nx < sh[0] / 2) continue;
-      for (int occ : {0, 4}) {
-        if (!occupancy_cap_ok(occ, sh[0] * sh[1])) continue;
-        for (int swz = 0; swz < 2; swz++)
-          for (int nt = 0; nt < 2; nt++) {
-            const TuneVal v{sh[0], sh[1], swz, nt, occ};
-            float ms = 0.f;
-            int rc = time_of(v, reps, &ms);
-            if (rc) return rc;
-            cand.push_back({ms, v});
-          }
+    for (int lin = 0; lin < 2; lin++)
+      for (const auto& sh : kShapes) {
+        // linear tiles only need a thread count: one shape per count
+        if (lin && sh[1] != 1) continue;
+        if (!lin && sh[0] > 64 && s->d.nx < sh[0] / 2) continue;
+        for (int occ : {0, 4}) {
+          if (!occupancy_cap_ok(occ, sh[0] * sh[1])) continue;
+          for (int swz = 0; swz < 2; swz++)
+            for (int nt = 0; nt < 2; nt++) {
+              const TuneVal v{sh[0], sh[1], swz, nt, occ, lin};
+              float ms = 0.f;
+              int rc = time_of(v, reps, &ms);
+              if (rc) return rc;
+              cand.push_back({ms, v});
+            }
+        }
       }
-    }
     std::sort(cand.begin(), cand.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
     cand.resize(std::min<size_t>(cand.size(), 4));
     for (auto& c : cand) c.first = 1e30f;
@@ -431,6 +446,7 @@ int autotune(lbm_solve* s) {
     g_tune_cache[key] = bestv;
   }
   s->bx = bestv.bx; s->by = bestv.by; s->swz = bestv.swz; s->nt = bestv.nt; s->occ = bestv.occ;
+  s->lin = bestv.lin;
   s->tuned = true;
   return build_force_slots(s);
 }
@@ -501,7 +517,8 @@ namespace {
 
 const int kTotalsBlocks = 1024;  // max totals partial blocks
 
-int create_impl(const lbm_desc* desc, Arena* arena, hipStream_t stream, int bx, int by, lbm_solve** out) {
+int create_impl(const lbm_desc* desc, Arena* arena, hipStream_t stream, int bx, int by, int flags,
+                lbm_solve** out) {
   if (!out) return fail(LBM_E_INVALID, "null output handle");
   *out = nullptr;
   int rc = validate(desc);
@@ -517,14 +534,21 @@ int create_impl(const lbm_desc* desc, Arena* arena, hipStream_t stream, int bx, 
   s->body = desc->body.shape != LBM_BODY_NONE;
   const bool walls = desc->walls[0] || desc->walls[1] || (desc->walls[2] && s->dim == 3);
   s->general = s->body || walls || desc->sponge_width > 0.f;
-  // default tile: one wave per x-row segment, XCD swizzle on
-  s->bx = 64; s->by = 1; s->swz = 1; s->nt = 0;
+  {
+    int flags = 0;
+    default_tile(desc->storage, &s->bx, &s->by, &flags);
+    s->swz = flags & 1;
+    s->nt = (flags >> 1) & 1;
+    s->lin = (flags >> 2) & 1;
+    s->occ = (flags >> 4) & 15;
+  }
   hipError_t e = hipSuccess;
   if (stream) {
     // a pool member: the pool's stream and the batched kernel's tile
     s->stream = stream;
     s->own_stream = false;
     s->bx = bx; s->by = by; s->swz = 0;
+    s->lin = (flags >> 2) & 1;
     s->d.autotune = 0;
   } else {
     e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -583,9 +607,21 @@ size_t solve_footprint(const lbm_desc* d) {
   return total;
 }
 
-int solve_create_in(const lbm_desc* d, Arena* arena, hipStream_t stream, int bx, int by, lbm_solve** out) {
+void default_tile(int storage, int* bx, int* by, int* flags) {
+  // Linear 256-cell tiles leave no lane idle at any width, non-temporal
+  // access keeps the streamed lattices out of the caches, and fp32 (256 B
+  // per wave load) runs best with residency capped at 4 waves per SIMD while
+  // the 16-bit formats want every wave they can get (D3Q19 sphere tile sweep
+  // on MI355X).
+  *bx = 256;
+  *by = 1;
+  *flags = 0x6 | (storage == LBM_STORE_F32 ? 0x40 : 0);
+}
+
+int solve_create_in(const lbm_desc* d, Arena* arena, hipStream_t stream, int bx, int by, int flags,
+                    lbm_solve** out) {
   if (!arena || !stream) return fail(LBM_E_INVALID, "pool solve needs an arena and a stream");
-  return create_impl(d, arena, stream, bx, by, out);
+  return create_impl(d, arena, stream, bx, by, flags, out);
 }
 
 int solve_batch_entry(lbm_solve* s, BatchSolve* e, uint32_t* ntiles) {
@@ -640,7 +676,9 @@ int lbm_device_info(char* name, size_t name_len, double* hbm_bytes, int32_t* cu_
 }
 
 
-int lbm_solve_create(const lbm_desc* desc, lbm_solve** out) { return create_impl(desc, nullptr, nullptr, 0, 0, out); }
+int lbm_solve_create(const lbm_desc* desc, lbm_solve** out) {
+  return create_impl(desc, nullptr, nullptr, 0, 0, 0, out);
+}
 
 int lbm_solve_destroy(lbm_solve* s) {
   if (s) (void)hipStreamSynchronize(s->stream);
@@ -703,7 +741,7 @@ int lbm_step(lbm_solve* s, int32_t n) {
     if (rc) return rc;
   }
   if (s->graph_bx != s->bx || s->graph_by != s->by || s->graph_swz != s->swz || s->graph_nt != s->nt ||
-      s->graph_occ != s->occ)
+      s->graph_occ != s->occ || s->graph_lin != s->lin)
     drop_graphs(s);
   // First step eagerly (nothing earlier in this call to finalize), then
   // captured chunks and single steps that each finalize their predecessor's
@@ -815,7 +853,7 @@ int lbm_solve_info(lbm_solve* s, int64_t* step, int32_t* block_x, int32_t* block
   if (step) *step = s->step;
   if (block_x) *block_x = s->bx;
   if (block_y) *block_y = s->by;
-  if (flags) *flags = s->swz | (s->nt << 1) | (s->occ << 4);
+  if (flags) *flags = s->swz | (s->nt << 1) | (s->lin << 2) | (s->occ << 4);
   if (device_bytes) *device_bytes = (int64_t)s->bytes;
   return LBM_OK;
 }
@@ -854,6 +892,7 @@ int lbm_set_tile(lbm_solve* s, int32_t block_x, int32_t block_y, int32_t flags) 
   s->by = block_y;
   s->swz = flags & 1;
   s->nt = (flags >> 1) & 1;
+  s->lin = (flags >> 2) & 1;
   s->occ = occ;
   s->tuned = true;
   return build_force_slots(s);

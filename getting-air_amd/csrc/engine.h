@@ -32,7 +32,13 @@ struct Arena {
 // Device bytes a solve of this descriptor takes (what solve_create_in asks
 // the arena for, alignment included).
 size_t solve_footprint(const lbm_desc* d);
-int solve_create_in(const lbm_desc* d, Arena* arena, hipStream_t stream, int bx, int by, lbm_solve** out);
+// Tile of a solve that is not autotuned (and of a serve class without the
+// autotuner), as lbm_set_tile takes it.
+void default_tile(int storage, int* bx, int* by, int* flags);
+// `flags` as lbm_set_tile takes them; a member uses its tile layout (bit 2,
+// linear tiles), the batched launch the rest.
+int solve_create_in(const lbm_desc* d, Arena* arena, hipStream_t stream, int bx, int by, int flags,
+                    lbm_solve** out);
 // The solve's batch table entry (step arguments by lattice parity, lattice
 // and field pointers; the progress fields are the caller's) and its tiles.
 int solve_batch_entry(lbm_solve* s, BatchSolve* e, uint32_t* ntiles);

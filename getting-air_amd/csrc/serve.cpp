@@ -148,7 +148,7 @@ struct ClassKey {
 
 struct Class {
   ClassKey key{};
-  int bx = 64, by = 1, nt = 0, occ = 0;
+  int bx = 64, by = 1, nt = 0, occ = 0, lin = 0;
   std::vector<Member*> members;
   bool dirty = true;  // membership changed since the tables were built
   BatchSolve* d_solves = nullptr;
@@ -217,7 +217,10 @@ void finish(lbm_serve* sv, const std::shared_ptr<Request>& r, int status, const 
 }
 
 // Tile shape of a lattice class: the dense autotuner run once on the class's
-// first request (its own grid), or one wave per row segment.
+// first request (its own grid), or the plain solve's default tile.  The class's
+// members have other widths, so its tiles are linear (blocks of consecutive
+// cells, no idle lanes at any width) with the tuned thread count, load
+// flavour and occupancy cap.
 void choose_tile(lbm_serve* sv, Class* c, const Request& r) {
   if (!sv->d.autotune) return;
   lbm_desc d = r.d;
@@ -228,10 +231,11 @@ void choose_tile(lbm_serve* sv, Class* c, const Request& r) {
   int32_t bx = 64, by = 1, flags = 0;
   if (!lbm_upload_macro(t, r.mac_in.data()) && !lbm_step(t, 1) &&
       !lbm_solve_info(t, &step, &bx, &by, &flags, &bytes)) {
-    c->bx = bx;
-    c->by = by;
+    c->bx = bx * by;
+    c->by = 1;
     c->nt = (flags >> 1) & 1;
     c->occ = (flags >> 4) & 15;
+    c->lin = 1;
   }
   lbm_solve_destroy(t);
 }
@@ -258,11 +262,16 @@ void admit(lbm_serve* sv) {
     if (!cp) {
       cp = std::make_unique<Class>();
       cp->key = key;
+      int flags = 0;
+      default_tile(key.storage, &cp->bx, &cp->by, &flags);
+      cp->nt = (flags >> 1) & 1;
+      cp->lin = (flags >> 2) & 1;
+      cp->occ = (flags >> 4) & 15;
       choose_tile(sv, cp.get(), *r);
     }
     auto m = std::make_unique<Member>();
     m->r = r;
-    int rc = solve_create_in(&r->d, &sv->pool, sv->setup, cp->bx, cp->by, &m->s);
+    int rc = solve_create_in(&r->d, &sv->pool, sv->setup, cp->bx, cp->by, cp->lin << 2, &m->s);
     if (rc == LBM_E_NOMEM) {
       // fragmentation: wait for a retirement and try again
       std::lock_guard<std::mutex> g(sv->mu);

@@ -299,7 +299,7 @@ class Solve:
                 "device_bytes": nbytes.value}
 
     def set_tile(self, bx, by, flags):
-        """flags: bit 0 XCD swizzle, bit 1 non-temporal, bits 4-7 waves-per-SIMD cap."""
+        """flags: bit 0 XCD swizzle, bit 1 non-temporal, bit 2 linear tiles, bits 4-7 waves-per-SIMD cap."""
         _check(self.lib.lbm_set_tile(self.h, int(bx), int(by), int(flags)))
 
     def time_step_kernel(self, n):

@@ -134,8 +134,10 @@ int lbm_solve_info(lbm_solve* s, int64_t* step, int32_t* block_x, int32_t* block
 
 /* Overrides the autotuned tile shape: block_x * block_y threads (a multiple
  * of 64, at most 256); flags bit 0 maps consecutive tiles onto one XCD, bit 1
- * selects non-temporal loads and stores, bits 4-7 cap residency at that many
- * waves per SIMD (0: no cap; 4 is available for every shape). */
+ * selects non-temporal loads and stores, bit 2 makes tiles linear (a block
+ * steps block_x * block_y consecutive cells of the x-fastest index instead of
+ * a block_x by block_y patch of one z plane), bits 4-7 cap residency at that
+ * many waves per SIMD (0: no cap; 4 is available for every shape). */
 int lbm_set_tile(lbm_solve* s, int32_t block_x, int32_t block_y, int32_t flags);
 
 /* Measurement hook, no counterpart in the reference (its spike page times

@@ -218,16 +218,27 @@ def test_sphere_full_config_short_run_and_symmetry():
     assert np.abs(tail[:, 1:3]).max() < 0.05 * np.abs(tail[:, 0]).mean()
 
 
+TILES = [(64, 1, 1), (256, 1, 2), (128, 2, 0x43), (64, 4, 0x40), (64, 1, 0x42), (32, 8, 0),
+         (256, 1, 0x4), (64, 1, 0x6), (128, 1, 0x47), (256, 1, 0x46)]
+
+
 @pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_tile_shapes_and_flags_give_the_same_field(storage):
+@pytest.mark.parametrize("case", ["sphere64", "sphere80x36x30", "cylinder224"])
+def test_tile_shapes_and_flags_give_the_same_field(storage, case):
     # every cell's update depends on its own pulls only, and the 16-bit dither
-    # on (cell, step): the field is the same bits for every tile shape, XCD
+    # on (cell, step): the field is the same bits for every tile shape (2D
+    # patches and linear runs of cells that cross rows and planes), XCD
     # mapping, load flavour and occupancy cap; the force sums change order
-    desc, mac, _ = sphere_small(trt=True)
+    if case == "sphere64":
+        desc, mac, _ = sphere_small(trt=True)
+    elif case == "sphere80x36x30":
+        desc, mac, _ = sc.sphere(nx=80, ny=36, nz=30, radius=5.0, centre=(21.0, 17.5, 14.25), u0=0.05, re=100.0)
+    else:
+        desc, mac, _ = sc.cylinder2d(nx=224, re=100.0)
     desc = dict(desc, storage=storage, autotune=0)
     steps = 40
     runs = []
-    for bx, by, flags in [(64, 1, 1), (256, 1, 2), (128, 2, 0x43), (64, 4, 0x40), (64, 1, 0x42), (32, 8, 0)]:
+    for bx, by, flags in TILES:
         with pylbm.Solve(desc) as s:
             s.set_tile(bx, by, flags)
             s.upload_macro(mac)
