@@ -167,18 +167,25 @@ def measure_turbulence(n=512, steps=2000, n_small=128, threads=16):
         u, r = m.reshape(-1, 4)[:, 1:].astype(np.float64), ref.reshape(-1, 4)[:, 1:].astype(np.float64)
         return float(np.sqrt(((u - r) ** 2).sum() / (r ** 2).sum()))
 
+    def say(*a):
+        print("turbulence:", *a, file=sys.stderr, flush=True)
+
     res = {"n": n, "steps": steps, "n_small": n_small}
     desc, mac = sc.turbulence(n=n)
+    say("initial field %d^3 ready" % n)
     ref, ml = run(desc, mac, "f32", n)
     ens_ref = enstrophy(ref, n)
     res["f32"] = {"mlups": ml, "enstrophy": ens_ref, "enstrophy0": enstrophy(mac, n)}
+    say("f32", res["f32"])
     for st in ("f16", "bf16"):
         m, ml = run(desc, mac, st, n)
         res[st] = {"mlups": ml, "speedup_vs_f32": ml / res["f32"]["mlups"], "u_rel_l2_vs_f32": errs(m, ref),
                    "enstrophy_rel_vs_f32": enstrophy(m, n) / ens_ref - 1.0}
+        say(st, res[st])
         del m
     del ref, mac
     desc, mac = sc.turbulence(n=n_small, kpeak=8.0 * n_small / n)
+    say("FP64 oracle, %d^3, %d steps, %d threads" % (n_small, steps, threads))
     o = oracle_runner.run(desc, mac, steps=steps, threads=threads, want=("mac",))
     oref = o["mac"].reshape(-1)
     ens_o = enstrophy(oref, n_small)
