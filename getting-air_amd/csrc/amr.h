@@ -68,6 +68,7 @@ void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint3
 // Rebuilds ccls and the plan lists (count must be zeroed first).
 hipError_t launch_cell_class(int q, uint8_t* ccls, const PoolGeom& g, const SurfacePlan& plan, const int32_t* active,
                              int nactive, hipStream_t s);
+// mac_pool null: the substep's macroscopic field is not written.
 hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
                             const CollideParams& fine, const int32_t* active, int nactive, hipStream_t s);
 hipError_t launch_explode(int q, const float* gcoarse, float* gpool, const PoolGeom& g, const SurfacePlan& plan,

@@ -213,7 +213,8 @@ int macro_step(lbm_amr* a) {
   }
   float* l0 = solve_lattice(a->l0, 0);
   HIPCHK(launch_explode(d.q, l0, a->pool[0], a->G, a->plan, a->nring, d.explode_linear, a->stream));
-  HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], a->mac_pool, a->G, a->fine, a->active, a->nactive, a->stream));
+  // only the second substep's field is read (coalesce, lbm_amr_read_pool_macro)
+  HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->active, a->nactive, a->stream));
   HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->active, a->nactive, a->stream));
   HIPCHK(launch_coalesce(d.q, a->pool[0], l0, solve_mac(a->l0), a->mac_pool, a->G, a->active, a->nactive, a->plan,
                          a->nseam, a->nnear, d.orphans, a->stream));

@@ -82,6 +82,18 @@ __device__ __forceinline__ int local_in_tile(int g, int b, int nfine) {
   return d + kGhost;
 }
 
+// Velocity and weight of direction i when i is uniform over the block (a
+// surface pass launched with one grid row per direction): compares and
+// selects on the scalar unit instead of a table indexed at run time.
+template <int Q>
+__device__ __forceinline__ void direction(int i, int& ex, int& ey, int& ez, float& w) {
+  using L = Lattice<Q>;
+  ex = 0, ey = 0, ez = 0, w = L::w[0];
+#pragma unroll
+  for (int k = 1; k < Q; k++)
+    if (i == k) ex = L::ex[k], ey = L::ey[k], ez = L::ez[k], w = L::w[k];
+}
+
 // ---------------------------------------------------------------------------
 // L0 cell classes
 // ---------------------------------------------------------------------------
@@ -207,10 +219,12 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
   float drho, rho, ux, uy, uz;
   moments<Q>(g, P.F, drho, rho, ux, uy, uz);
   const uint32_t cell = pool_cell<RB>(slot, fx, fy, fz);
-  *at(mac_pool, cell) = rho;
-  *at(mac_pool + G.pool_plane, cell) = ux;
-  *at(mac_pool + 2 * (size_t)G.pool_plane, cell) = uy;
-  *at(mac_pool + 3 * (size_t)G.pool_plane, cell) = uz;
+  if (mac_pool) {
+    *at(mac_pool, cell) = rho;
+    *at(mac_pool + G.pool_plane, cell) = ux;
+    *at(mac_pool + 2 * (size_t)G.pool_plane, cell) = uy;
+    *at(mac_pool + 3 * (size_t)G.pool_plane, cell) = uz;
+  }
   const bool interior = fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI;
   // Ring cells advect and store without colliding (common_d3_amr_step1.wgsl
   // COLLIDE_RING = 0): coalesce averages uncollided interface states.
@@ -222,15 +236,21 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
 // ---------------------------------------------------------------------------
 // explode: coarse state at t -> ring of the pool buffer the first substep reads
 // ---------------------------------------------------------------------------
+// One thread per (ring cell, direction), the direction uniform per grid row:
+// each population is an independent short chain of loads, so the pass is
+// not bound by the latency of Q dependent lookups per thread.
 template <int Q, int RB>
 __global__ void __launch_bounds__(256)
 k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const uint32_t* __restrict__ ring,
           uint32_t nring, int linear) {
-  using L = Lattice<Q>;
   using T = Tile<RB>;
   constexpr int FB = T::FB, RB2 = T::RB2;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nring) return;
+  const int i = blockIdx.y;
+  int ex, ey, ez;
+  float wi;
+  direction<Q>(i, ex, ey, ez, wi);
   // a ring cell over an unrefined L0 cell (k_ring_plan)
   const uint32_t cell = ring[t];
   const int slot = (int)(cell / (uint32_t)T::CELLS);
@@ -244,50 +264,45 @@ k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, cons
   const int gz = wrap1(bz * RB2 + fz - kGhost, NFz);
   const int vx = gx >> 1, vy = gy >> 1, vz = gz >> 1;
   const uint32_t vc = coarse_cell(G, vx, vy, vz);
-  const uint32_t N = G.ncells;
-  const int xm = wrap1(vx - 1, G.nx), xp = wrap1(vx + 1, G.nx);
-  const int ym = wrap1(vy - 1, G.ny), yp = wrap1(vy + 1, G.ny);
-  const int zm = wrap1(vz - 1, G.nz), zp = wrap1(vz + 1, G.nz);
-  const uint32_t cmx = coarse_cell(G, xm, vy, vz), cpx = coarse_cell(G, xp, vy, vz);
-  const uint32_t cmy = coarse_cell(G, vx, ym, vz), cpy = coarse_cell(G, vx, yp, vz);
-  const uint32_t cmz = coarse_cell(G, vx, vy, zm), cpz = coarse_cell(G, vx, vy, zp);
-  const bool okx = !((G.ccls[cmx] | G.ccls[cpx]) & kCovered);
-  const bool oky = !((G.ccls[cmy] | G.ccls[cpy]) & kCovered);
-  const bool okz = !((G.ccls[cmz] | G.ccls[cpz]) & kCovered);
-  // position of the fine cell inside its coarse cell, in coarse units
-  const float dx = ((float)(gx - 2 * vx) - 0.5f) * 0.5f;
-  const float dy = ((float)(gy - 2 * vy) - 0.5f) * 0.5f;
-  const float dz = ((float)(gz - 2 * vz) - 0.5f) * 0.5f;
-#pragma unroll
-  for (int i = 0; i < Q; i++) {
-    const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
-    const float* plane = gc + (size_t)i * N;
-    float out;
-    if (!covered_at(G, vx + ex, vy + ey, vz + ez)) {
-      // The coarse target is unrefined: only a population that streams from
-      // this ring cell into this tile's interior is kept (in-orphan, uniform
-      // injection); everything else is zeroed (f = 0, i.e. g = -w).
-      const int dxg = wrap1(gx + ex, NFx), dyg = wrap1(gy + ey, NFy), dzg = wrap1(gz + ez, NFz);
-      const bool into = covered_at(G, dxg >> 1, dyg >> 1, dzg >> 1) && (dxg >> (T::LOG_RB + 1)) == bx &&
-                        (dyg >> (T::LOG_RB + 1)) == by && (dzg >> (T::LOG_RB + 1)) == bz;
-      out = into ? *at(plane, vc) : -L::w[i];
-    } else if (!linear) {
-      out = *at(plane, vc);
-    } else {
-      // linear reconstruction d . G with the gradient projected
-      // perpendicular to e_i (common_d3_amr_explode.wgsl:197-212)
-      float Gx = okx ? 0.5f * (*at(plane, cpx) - *at(plane, cmx)) : 0.f;
-      float Gy = oky ? 0.5f * (*at(plane, cpy) - *at(plane, cmy)) : 0.f;
-      float Gz = okz ? 0.5f * (*at(plane, cpz) - *at(plane, cmz)) : 0.f;
-      const float ee = (float)(ex * ex + ey * ey + ez * ez);
-      const float proj = (Gx * ex + Gy * ey + Gz * ez) / ee;
-      Gx -= ex * proj;
-      Gy -= ey * proj;
-      Gz -= ez * proj;
-      out = *at(plane, vc) + (dx * Gx + dy * Gy + dz * Gz);
-    }
-    *at(gp + (size_t)i * G.pool_plane, cell) = out;
+  const float* plane = gc + (size_t)i * G.ncells;
+  float out;
+  if (!covered_at(G, vx + ex, vy + ey, vz + ez)) {
+    // The coarse target is unrefined: only a population that streams from
+    // this ring cell into this tile's interior is kept (in-orphan, uniform
+    // injection); everything else is zeroed (f = 0, i.e. g = -w).
+    const int dxg = wrap1(gx + ex, NFx), dyg = wrap1(gy + ey, NFy), dzg = wrap1(gz + ez, NFz);
+    const bool into = covered_at(G, dxg >> 1, dyg >> 1, dzg >> 1) && (dxg >> (T::LOG_RB + 1)) == bx &&
+                      (dyg >> (T::LOG_RB + 1)) == by && (dzg >> (T::LOG_RB + 1)) == bz;
+    out = into ? *at(plane, vc) : -wi;
+  } else if (!linear) {
+    out = *at(plane, vc);
+  } else {
+    // linear reconstruction d . G with the gradient projected
+    // perpendicular to e_i (common_d3_amr_explode.wgsl:197-212)
+    const int xm = wrap1(vx - 1, G.nx), xp = wrap1(vx + 1, G.nx);
+    const int ym = wrap1(vy - 1, G.ny), yp = wrap1(vy + 1, G.ny);
+    const int zm = wrap1(vz - 1, G.nz), zp = wrap1(vz + 1, G.nz);
+    const uint32_t cmx = coarse_cell(G, xm, vy, vz), cpx = coarse_cell(G, xp, vy, vz);
+    const uint32_t cmy = coarse_cell(G, vx, ym, vz), cpy = coarse_cell(G, vx, yp, vz);
+    const uint32_t cmz = coarse_cell(G, vx, vy, zm), cpz = coarse_cell(G, vx, vy, zp);
+    const bool okx = !((G.ccls[cmx] | G.ccls[cpx]) & kCovered);
+    const bool oky = !((G.ccls[cmy] | G.ccls[cpy]) & kCovered);
+    const bool okz = !((G.ccls[cmz] | G.ccls[cpz]) & kCovered);
+    // position of the fine cell inside its coarse cell, in coarse units
+    const float dx = ((float)(gx - 2 * vx) - 0.5f) * 0.5f;
+    const float dy = ((float)(gy - 2 * vy) - 0.5f) * 0.5f;
+    const float dz = ((float)(gz - 2 * vz) - 0.5f) * 0.5f;
+    float Gx = okx ? 0.5f * (*at(plane, cpx) - *at(plane, cmx)) : 0.f;
+    float Gy = oky ? 0.5f * (*at(plane, cpy) - *at(plane, cmy)) : 0.f;
+    float Gz = okz ? 0.5f * (*at(plane, cpz) - *at(plane, cmz)) : 0.f;
+    const float ee = (float)(ex * ex + ey * ey + ez * ez);
+    const float proj = (Gx * ex + Gy * ey + Gz * ez) / ee;
+    Gx -= ex * proj;
+    Gy -= ey * proj;
+    Gz -= ez * proj;
+    out = *at(plane, vc) + (dx * Gx + dy * Gy + dz * Gz);
   }
+  *at(gp + (size_t)i * G.pool_plane, cell) = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -327,79 +342,79 @@ k_coalesce_mac(float* __restrict__ mac, const float* __restrict__ mac_pool, Pool
 
 // Covered seam cells: for every direction whose target is unrefined, the
 // mean of the 8 ring cells covering the target (common_d3_amr_coalesce.wgsl:
-// 264-284).
+// 264-284).  One thread per (seam cell, direction i = 1 + grid row).
 template <int Q, int RB>
 __global__ void __launch_bounds__(256)
 k_coalesce_seam(const float* __restrict__ gp, float* __restrict__ gc, PoolGeom G, const uint32_t* __restrict__ seam,
                 uint32_t nseam) {
-  using L = Lattice<Q>;
   using T = Tile<RB>;
   constexpr int FB = T::FB;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nseam) return;
+  const int i = 1 + blockIdx.y;
+  int ex, ey, ez;
+  float wi;
+  direction<Q>(i, ex, ey, ez, wi);
   const uint32_t c = seam[t];
   const int cx = c % G.nx, cy = (c / G.nx) % G.ny, cz = c / (G.nx * G.ny);
+  const int tx = cx + ex, ty = cy + ey, tz = cz + ez;
+  if (covered_at(G, tx, ty, tz)) return;
   const int bx = cx >> T::LOG_RB, by = cy >> T::LOG_RB, bz = cz >> T::LOG_RB;
   const int slot = G.block_slot[block_id(G, bx, by, bz)];
-  const int NFx = 2 * G.nx, NFy = 2 * G.ny, NFz = 2 * G.nz;
+  // the target is one coarse cell outside the block: its 8 children are a
+  // 2x2x2 cube of this tile's ring
+  const int lx = local_in_tile<RB>(2 * tx, bx, 2 * G.nx);
+  const int ly = local_in_tile<RB>(2 * ty, by, 2 * G.ny);
+  const int lz = local_in_tile<RB>(2 * tz, bz, 2 * G.nz);
+  const float* p = gp + (size_t)i * G.pool_plane;
+  const uint32_t b0 = pool_cell<RB>(slot, lx, ly, lz);
+  float s = 0.f;
 #pragma unroll
-  for (int i = 1; i < Q; i++) {
-    const int tx = cx + L::ex[i], ty = cy + L::ey[i], tz = cz + L::ez[i];
-    if (covered_at(G, tx, ty, tz)) continue;
-    // the target is one coarse cell outside the block: its 8 children are a
-    // 2x2x2 cube of this tile's ring
-    const int lx = local_in_tile<RB>(2 * tx, bx, NFx);
-    const int ly = local_in_tile<RB>(2 * ty, by, NFy);
-    const int lz = local_in_tile<RB>(2 * tz, bz, NFz);
-    const float* p = gp + (size_t)i * G.pool_plane;
-    const uint32_t b0 = pool_cell<RB>(slot, lx, ly, lz);
-    float s = 0.f;
-#pragma unroll
-    for (int k = 0; k < 8; k++) s += *at(p, b0 + (k & 1) + ((k >> 1) & 1) * FB + ((k >> 2) & 1) * FB * FB);
-    *at(gc + (size_t)i * G.ncells, c) = s * 0.125f;  // an average of g is the average of f minus w
-  }
+  for (int k = 0; k < 8; k++) s += *at(p, b0 + (k & 1) + ((k >> 1) & 1) * FB + ((k >> 2) & 1) * FB * FB);
+  *at(gc + (size_t)i * G.ncells, c) = s * 0.125f;  // an average of g is the average of f minus w
 }
 
 // Orphans on unrefined L0 cells next to the refined region
-// (coalesceOrphans, common_d3_amr_coalesce.wgsl:126-211).
+// (coalesceOrphans, common_d3_amr_coalesce.wgsl:126-211).  One thread per
+// (near cell, direction i = 1 + grid row); the 8 child lookups of a
+// direction are independent loads.
 template <int Q, int RB>
 __global__ void __launch_bounds__(256)
 k_orphans(const float* __restrict__ gp, float* __restrict__ gc, PoolGeom G, const uint32_t* __restrict__ near,
           uint32_t nnear) {
-  using L = Lattice<Q>;
   using T = Tile<RB>;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nnear) return;
+  const int i = 1 + blockIdx.y;
+  int ex, ey, ez;
+  float wi;
+  direction<Q>(i, ex, ey, ez, wi);
   const uint32_t c = near[t];
   const int cx = c % G.nx, cy = (c / G.nx) % G.ny, cz = c / (G.nx * G.ny);
   const int NFx = 2 * G.nx, NFy = 2 * G.ny, NFz = 2 * G.nz;
   const uint32_t N = G.ncells;
+  const int tx = wrap1(cx + ex, G.nx), ty = wrap1(cy + ey, G.ny), tz = wrap1(cz + ez, G.nz);
+  if (G.ccls[coarse_cell(G, tx, ty, tz)] & kCovered) return;
+  float s = 0.f;   // sum of f over the children found in refined tiles
+  float nIn = 0.f;
 #pragma unroll
-  for (int i = 1; i < Q; i++) {
-    const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
-    const int tx = wrap1(cx + ex, G.nx), ty = wrap1(cy + ey, G.ny), tz = wrap1(cz + ez, G.nz);
-    if (G.ccls[coarse_cell(G, tx, ty, tz)] & kCovered) continue;
-    float s = 0.f;   // sum of f over the children found in refined tiles
-    float nIn = 0.f;
-#pragma unroll 1
-    for (int k = 0; k < 8; k++) {
-      const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
-      const int px = 2 * tx + ox, py = 2 * ty + oy, pz = 2 * tz + oz;
-      const int qx = wrap1(px - ex, NFx), qy = wrap1(py - ey, NFy), qz = wrap1(pz - ez, NFz);
-      const int qbx = qx >> (T::LOG_RB + 1), qby = qy >> (T::LOG_RB + 1), qbz = qz >> (T::LOG_RB + 1);
-      const int qs = G.block_slot[block_id(G, qbx, qby, qbz)];
-      if (qs >= 0) {
-        s += *at(gp + (size_t)i * G.pool_plane,
-                 pool_cell<RB>(qs, local_in_tile<RB>(px, qbx, NFx), local_in_tile<RB>(py, qby, NFy),
-                               local_in_tile<RB>(pz, qbz, NFz))) + L::w[i];
-      }
-      const int hx = wrap1(2 * cx + ox + ex, NFx), hy = wrap1(2 * cy + oy + ey, NFy), hz = wrap1(2 * cz + oz + ez, NFz);
-      if (G.ccls[coarse_cell(G, hx >> 1, hy >> 1, hz >> 1)] & kCovered) nIn += 1.f;
+  for (int k = 0; k < 8; k++) {
+    const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+    const int px = 2 * tx + ox, py = 2 * ty + oy, pz = 2 * tz + oz;
+    const int qx = wrap1(px - ex, NFx), qy = wrap1(py - ey, NFy), qz = wrap1(pz - ez, NFz);
+    const int qbx = qx >> (T::LOG_RB + 1), qby = qy >> (T::LOG_RB + 1), qbz = qz >> (T::LOG_RB + 1);
+    const int qs = G.block_slot[block_id(G, qbx, qby, qbz)];
+    if (qs >= 0) {
+      s += *at(gp + (size_t)i * G.pool_plane,
+               pool_cell<RB>(qs, local_in_tile<RB>(px, qbx, NFx), local_in_tile<RB>(py, qby, NFy),
+                             local_in_tile<RB>(pz, qbz, NFz))) + wi;
     }
-    const float fc = *at(gc + (size_t)i * N, c) + L::w[i];
-    const float net = s - nIn * fc;
-    if (net != 0.f) *at(gc + (size_t)i * N, c) = (fc + net * 0.125f) - L::w[i];
+    const int hx = wrap1(2 * cx + ox + ex, NFx), hy = wrap1(2 * cy + oy + ey, NFy), hz = wrap1(2 * cz + oz + ez, NFz);
+    if (G.ccls[coarse_cell(G, hx >> 1, hy >> 1, hz >> 1)] & kCovered) nIn += 1.f;
   }
+  const float fc = *at(gc + (size_t)i * N, c) + wi;
+  const float net = s - nIn * fc;
+  if (net != 0.f) *at(gc + (size_t)i * N, c) = (fc + net * 0.125f) - wi;
 }
 
 // ---------------------------------------------------------------------------
@@ -685,7 +700,7 @@ k_pool_totals(const float* __restrict__ gc, const float* __restrict__ gp, PoolGe
 // ---------------------------------------------------------------------------
 // launches: dispatch on (q, rb)
 // ---------------------------------------------------------------------------
-static inline dim3 grid_for(uint64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+static inline dim3 grid_for(uint64_t n, unsigned rows = 1) { return dim3((unsigned)((n + 255) / 256), rows); }
 
 template <template <int, int> class Op, class... Args>
 static hipError_t dispatch_q_rb(int q, int rb, Args&&... args) {
@@ -724,7 +739,7 @@ template <int Q, int RB>
 struct OpExplode {
   static hipError_t run(const float* gc, float* gp, const PoolGeom& g, const SurfacePlan& plan, uint32_t nring,
                         int linear, hipStream_t s) {
-    hipLaunchKernelGGL((k_explode<Q, RB>), grid_for(nring), dim3(256), 0, s, gc, gp, g, plan.ring, nring, linear);
+    hipLaunchKernelGGL((k_explode<Q, RB>), grid_for(nring, Q), dim3(256), 0, s, gc, gp, g, plan.ring, nring, linear);
     return hipGetLastError();
   }
 };
@@ -737,9 +752,10 @@ struct OpCoalesce {
       hipLaunchKernelGGL((k_coalesce_mac<RB>), grid_for((uint64_t)nactive * RB * RB * RB), dim3(256), 0, s, mac,
                          mac_pool, g, active, nactive);
     if (nseam > 0)
-      hipLaunchKernelGGL((k_coalesce_seam<Q, RB>), grid_for(nseam), dim3(256), 0, s, gp, gc, g, plan.seam, nseam);
+      hipLaunchKernelGGL((k_coalesce_seam<Q, RB>), grid_for(nseam, Q - 1), dim3(256), 0, s, gp, gc, g, plan.seam,
+                         nseam);
     if (orphans && nnear > 0)
-      hipLaunchKernelGGL((k_orphans<Q, RB>), grid_for(nnear), dim3(256), 0, s, gp, gc, g, plan.near, nnear);
+      hipLaunchKernelGGL((k_orphans<Q, RB>), grid_for(nnear, Q - 1), dim3(256), 0, s, gp, gc, g, plan.near, nnear);
     return hipGetLastError();
   }
 };
