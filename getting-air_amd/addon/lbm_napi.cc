@@ -359,7 +359,8 @@ lbm_amr* amr_handle(napi_env env, napi_value v) {
 }
 
 // createAmr({q, nx, ny, nz, rb, maxSlots, omega, omegaMinus, force,
-//            manageEvery, refineAt, keepAt, lead, explodeLinear, orphans})
+//            manageEvery, refineAt, keepAt, lead, explodeLinear, orphans,
+//            interface: 'explode' | 'interp', reflux})
 napi_value CreateAmr(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
@@ -393,6 +394,18 @@ napi_value CreateAmr(napi_env env, napi_callback_info info) {
   d.lead = (int)num(env, o, "lead", 0);
   d.explode_linear = (int)num(env, o, "explodeLinear", 1);
   d.orphans = (int)num(env, o, "orphans", 1);
+  napi_value iv;
+  if (get_prop(env, o, "interface", &iv)) {
+    char buf[16] = {0};
+    size_t len = 0;
+    napi_get_value_string_utf8(env, iv, buf, sizeof buf, &len);
+    if (!strcmp(buf, "interp")) d.interface_interp = 1;
+    else if (strcmp(buf, "explode")) {
+      napi_throw_type_error(env, nullptr, "interface must be 'explode' or 'interp'");
+      return nullptr;
+    }
+  }
+  d.reflux = (int)num(env, o, "reflux", 0);
   lbm_amr* a = nullptr;
   int rc = lbm_amr_create(&d, &a);
   if (rc) return throw_lbm(env, rc);

@@ -59,8 +59,10 @@ struct SurfacePlan {
   uint32_t* seam;   // covered L0 cells with an unrefined lattice neighbour
   uint32_t* near;   // unrefined L0 cells with a covered lattice neighbour
   uint32_t* ring;   // pool cells (slot * FB^3 + local) of ring cells over unrefined L0 cells
-  uint32_t* count;  // [seam, near, ring]
-  uint32_t cap_seam, cap_near, cap_ring;
+  uint32_t* flux;   // unrefined L0 cells with a covered cell among their 26 neighbours
+                    // (the seam flux correction's cells); null without it
+  uint32_t* count;  // [seam, near, ring, flux]
+  uint32_t cap_seam, cap_near, cap_ring, cap_flux;
 };
 // Capacity of the lists for a pool of max_slots tiles.
 void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint32_t* near, uint32_t* ring);
@@ -69,15 +71,27 @@ void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint3
 hipError_t launch_cell_class(int q, uint8_t* ccls, const PoolGeom& g, const SurfacePlan& plan, const int32_t* active,
                              int nactive, hipStream_t s);
 // mac_pool null: the substep's macroscopic field is not written.
+// collide_ring: ring cells collide as well (the interp interface).
 hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
-                            const CollideParams& fine, const int32_t* active, int nactive, hipStream_t s);
+                            const CollideParams& fine, const int32_t* active, int nactive, int collide_ring,
+                            hipStream_t s);
+// Seam flux of the fine solver over one substep, from the pool buffer that
+// substep streams: [dm, dpx, dpy, dpz] / 8 per flux cell, written
+// (accumulate 0) or added (1).
+hipError_t launch_flux(int q, const float* gpool, float* acc, const PoolGeom& g, const SurfacePlan& plan,
+                       uint32_t nflux, int accumulate, hipStream_t s);
+// Flux correction of the unrefined flux cells of the coarse state at t + 1
+// (gc_t1, mac) from the coarse state at t and the accumulated fine flux.
+hipError_t launch_reflux(int q, const float* gc_t, float* gc_t1, float* mac, const PoolGeom& g, const SurfacePlan& plan,
+                         uint32_t nflux, float* acc, const float* force, hipStream_t s);
 hipError_t launch_explode(int q, const float* gcoarse, float* gpool, const PoolGeom& g, const SurfacePlan& plan,
                           uint32_t nring, int linear, hipStream_t s);
 hipError_t launch_coalesce(int q, const float* gpool, float* gcoarse, float* mac, const float* mac_pool,
                            const PoolGeom& g, const int32_t* active, int nactive, const SurfacePlan& plan,
                            uint32_t nseam, uint32_t nnear, int orphans, hipStream_t s);
+// ring_only: the ring cells alone (the interp interface's per-step refresh).
 hipError_t launch_interp(int q, const float* gcoarse, float* gpool, const PoolGeom& g, const int32_t* pairs,
-                         int npairs, float rescale_c2f, hipStream_t s);
+                         int npairs, float rescale_c2f, int ring_only, hipStream_t s);
 hipError_t launch_average(int q, const float* gpool, float* gcoarse, float* mac, const PoolGeom& g,
                           const int32_t* pairs, int npairs, float rescale_f2c, hipStream_t s);
 hipError_t launch_criterion(const float* mac, float* crit, const PoolGeom& g, float lead, hipStream_t s);

@@ -44,7 +44,8 @@ struct lbm_amr {
   int32_t* pairs = nullptr;
   uint8_t* ccls = nullptr;
   SurfacePlan plan{};
-  uint32_t nseam = 0, nnear = 0, nring = 0;
+  uint32_t nseam = 0, nnear = 0, nring = 0, nflux = 0;
+  float* flux_acc = nullptr;        // [dm, dpx, dpy, dpz] per flux cell (reflux only)
   double* partials = nullptr;
   unsigned* counter = nullptr;
   double* totals = nullptr;
@@ -95,6 +96,10 @@ int validate(const lbm_amr_desc* d) {
   if (d->collision == LBM_TRT && !(d->omega_minus > 0.f && d->omega_minus < 2.f))
     return fail(LBM_E_INVALID, "omega_minus %g outside (0, 2)", d->omega_minus);
   if (d->manage_every < 0 || d->lead < 0) return fail(LBM_E_INVALID, "manage_every / lead must be >= 0");
+  // the flux correction measures a mismatch that explode/coalesce does not
+  // create (main-3d.js:1333-1336)
+  if (d->reflux && !d->interface_interp)
+    return fail(LBM_E_INVALID, "reflux needs the interp interface: explode/coalesce is conservative by construction");
   return LBM_OK;
 }
 
@@ -102,7 +107,7 @@ void release(lbm_amr* a) {
   if (!a) return;
   void* ptrs[] = {a->pool[0], a->pool[1], a->mac_pool, a->block_slot, a->slot_to_block, a->want, a->crit,
                   a->dying, a->fresh, a->active, a->free_list, a->status, a->pairs, a->ccls, a->partials, a->counter, a->totals,
-                  a->plan.seam, a->plan.near, a->plan.ring, a->plan.count};
+                  a->plan.seam, a->plan.near, a->plan.ring, a->plan.flux, a->plan.count, a->flux_acc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (a->l0) lbm_solve_destroy(a->l0);
@@ -122,7 +127,7 @@ int dalloc(T** p, size_t bytes) {
 // Fills the tiles listed in `pairs` ([count, (block, slot)...], device) from
 // the current L0 state.
 int fill_tiles(lbm_amr* a, const int32_t* pairs, int n) {
-  HIPCHK(launch_interp(a->d.q, solve_lattice(a->l0, 0), a->pool[0], a->G, pairs, n, a->c2f, a->stream));
+  HIPCHK(launch_interp(a->d.q, solve_lattice(a->l0, 0), a->pool[0], a->G, pairs, n, a->c2f, 0, a->stream));
   return LBM_OK;
 }
 
@@ -131,15 +136,16 @@ int fill_tiles(lbm_amr* a, const int32_t* pairs, int n) {
 int rebuild_plan(lbm_amr* a) {
   HIPCHK(hipMemsetAsync(a->plan.count, 0, 16, a->stream));
   HIPCHK(launch_cell_class(a->d.q, a->ccls, a->G, a->plan, a->active, a->nactive, a->stream));
-  uint32_t c[3] = {0, 0, 0};
-  HIPCHK(hipMemcpyAsync(c, a->plan.count, 12, hipMemcpyDeviceToHost, a->stream));
+  uint32_t c[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(c, a->plan.count, 16, hipMemcpyDeviceToHost, a->stream));
   HIPCHK(hipStreamSynchronize(a->stream));
-  if (c[0] > a->plan.cap_seam || c[1] > a->plan.cap_near || c[2] > a->plan.cap_ring)
-    return fail(LBM_E_DEVICE, "surface work lists overflow (%u/%u seam, %u/%u near, %u/%u ring)", c[0],
-                a->plan.cap_seam, c[1], a->plan.cap_near, c[2], a->plan.cap_ring);
+  if (c[0] > a->plan.cap_seam || c[1] > a->plan.cap_near || c[2] > a->plan.cap_ring || c[3] > a->plan.cap_flux)
+    return fail(LBM_E_DEVICE, "surface work lists overflow (%u/%u seam, %u/%u near, %u/%u ring, %u/%u flux)", c[0],
+                a->plan.cap_seam, c[1], a->plan.cap_near, c[2], a->plan.cap_ring, c[3], a->plan.cap_flux);
   a->nseam = c[0];
   a->nnear = c[1];
   a->nring = c[2];
+  a->nflux = c[3];
   return LBM_OK;
 }
 
@@ -205,17 +211,53 @@ int manage(lbm_amr* a) {
   return LBM_OK;
 }
 
+// The interp interface (main-3d.js:3344-3357): the coarse step first (cells
+// under a tile skipped), each tile's ring re-interpolated from the coarse
+// state at t, two substeps in which the ring collides as well, every tile
+// restricted onto the coarse state at t + 1, and with reflux the fine seam
+// flux measured before each substep and the difference to the coarse one
+// corrected onto the unrefined side.
+int macro_step_interp(lbm_amr* a) {
+  const lbm_amr_desc& d = a->d;
+  int rc = solve_enqueue_covered_step(a->l0, a->block_slot, d.rb, a->G.nbx, a->G.nby);
+  if (rc) return rc;
+  const float* ct = solve_lattice(a->l0, 1);  // the state the coarse step read (t)
+  float* ct1 = solve_lattice(a->l0, 0);       // the one it wrote (t + 1)
+  if (a->nactive > 0) {
+    HIPCHK(launch_active_pairs(a->active, a->slot_to_block, d.max_slots, a->pairs, a->stream));
+    HIPCHK(launch_interp(d.q, ct, a->pool[0], a->G, a->pairs, a->nactive, a->c2f, 1, a->stream));
+    if (d.reflux) HIPCHK(launch_flux(d.q, a->pool[0], a->flux_acc, a->G, a->plan, a->nflux, 0, a->stream));
+    HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->active, a->nactive, 1,
+                            a->stream));
+    if (d.reflux) HIPCHK(launch_flux(d.q, a->pool[1], a->flux_acc, a->G, a->plan, a->nflux, 1, a->stream));
+    HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->active, a->nactive, 1,
+                            a->stream));
+    HIPCHK(launch_average(d.q, a->pool[0], ct1, solve_mac(a->l0), a->G, a->pairs, a->nactive, a->f2c, a->stream));
+    if (d.reflux)
+      HIPCHK(launch_reflux(d.q, ct, ct1, solve_mac(a->l0), a->G, a->plan, a->nflux, a->flux_acc, d.force,
+                           a->stream));
+  }
+  return LBM_OK;
+}
+
 int macro_step(lbm_amr* a) {
   const lbm_amr_desc& d = a->d;
   if (d.manage_every > 0 && a->step % d.manage_every == 0) {
     int rc = manage(a);
     if (rc) return rc;
   }
+  if (d.interface_interp) {
+    int rc = macro_step_interp(a);
+    if (rc) return rc;
+    a->step++;
+    return LBM_OK;
+  }
   float* l0 = solve_lattice(a->l0, 0);
   HIPCHK(launch_explode(d.q, l0, a->pool[0], a->G, a->plan, a->nring, d.explode_linear, a->stream));
   // only the second substep's field is read (coalesce, lbm_amr_read_pool_macro)
-  HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->active, a->nactive, a->stream));
-  HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->active, a->nactive, a->stream));
+  HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->active, a->nactive, 0, a->stream));
+  HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->active, a->nactive, 0,
+                          a->stream));
   HIPCHK(launch_coalesce(d.q, a->pool[0], l0, solve_mac(a->l0), a->mac_pool, a->G, a->active, a->nactive, a->plan,
                          a->nseam, a->nnear, d.orphans, a->stream));
   int rc = solve_enqueue_covered_step(a->l0, a->block_slot, d.rb, a->G.nbx, a->G.nby);
@@ -277,6 +319,13 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
       (rc = dalloc(&a->partials, 256 * 4 * 8)) || (rc = dalloc(&a->counter, 16)) || (rc = dalloc(&a->totals, 32))) {
     release(a);
     return rc;
+  }
+  if (desc->reflux) {
+    a->plan.cap_flux = G.ncells;
+    if ((rc = dalloc(&a->plan.flux, (size_t)G.ncells * 4)) || (rc = dalloc(&a->flux_acc, (size_t)G.ncells * 16))) {
+      release(a);
+      return rc;
+    }
   }
   G.block_slot = a->block_slot;
   G.slot_to_block = a->slot_to_block;

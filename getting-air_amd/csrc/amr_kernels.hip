@@ -8,6 +8,9 @@
 //   interp fill    shaders/common_d3_amr_interp.wgsl:109-185 with
 //                  interpCoarseToFine3, common_d3_pool.wgsl:192-220
 //   average/drain  shaders/common_d3_amr_average.wgsl:75-154
+//   flux / reflux  shaders/common_d3_amr_flux.wgsl:80-194 and
+//                  common_d3_amr_reflux.wgsl:60-165 (the interp interface's
+//                  optional seam correction)
 //   criterion      shaders/common_d3_criterion.wgsl:122-158, with the
 //                  vorticity magnitude |curl u| in place of Q
 //   manager        shaders/common_d3_manage.wgsl:430-546 and
@@ -114,6 +117,17 @@ __global__ void __launch_bounds__(256) k_cell_class(uint8_t* __restrict__ ccls, 
 #pragma unroll
   for (int i = 1; i < Q; i++) other = other || (cov(x + L::ex[i], y + L::ey[i], z + L::ez[i]) != self);
   ccls[c] = self ? (uint8_t)(kCovered | (other ? kSeam : 0)) : (uint8_t)(other ? kNear : 0);
+  if (plan.flux && !self) {
+    // a refined block among the blocks the 26 neighbours span
+    // (common_d3_amr_flux.wgsl:101-122): any covered cell in the 3x3x3 box
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 27; k++) any = any || cov(x + k % 3 - 1, y + (k / 3) % 3 - 1, z + k / 9 - 1);
+    if (any) {
+      const uint32_t j = atomicAdd(&plan.count[3], 1u);
+      if (j < plan.cap_flux) plan.flux[j] = c;
+    }
+  }
   if (other) {
     if (self) {
       const uint32_t k = atomicAdd(&plan.count[0], 1u);
@@ -162,7 +176,7 @@ void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint3
 template <int Q, int RB>
 __global__ void __launch_bounds__(256)
 k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __restrict__ mac_pool, PoolGeom G,
-            CollideParams P, const int32_t* __restrict__ active, int nactive) {
+            CollideParams P, const int32_t* __restrict__ active, int nactive, int collide_ring) {
   using L = Lattice<Q>;
   using T = Tile<RB>;
   constexpr int FB = T::FB, RB2 = T::RB2, LO = T::LO, HI = T::HI;
@@ -227,8 +241,9 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
   }
   const bool interior = fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI;
   // Ring cells advect and store without colliding (common_d3_amr_step1.wgsl
-  // COLLIDE_RING = 0): coalesce averages uncollided interface states.
-  if (interior) collide<Q>(g, P, drho, rho, ux, uy, uz);
+  // COLLIDE_RING = 0): coalesce averages uncollided interface states.  The
+  // interp interface collides them too (COLLIDE_RING = 1, main-3d.js:1395).
+  if (interior || collide_ring) collide<Q>(g, P, drho, rho, ux, uy, uz);
 #pragma unroll
   for (int i = 0; i < Q; i++) *at(gout + (size_t)i * G.pool_plane, cell) = g[i];
 }
@@ -236,21 +251,15 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
 // ---------------------------------------------------------------------------
 // explode: coarse state at t -> ring of the pool buffer the first substep reads
 // ---------------------------------------------------------------------------
-// One thread per (ring cell, direction), the direction uniform per grid row:
-// each population is an independent short chain of loads, so the pass is
-// not bound by the latency of Q dependent lookups per thread.
 template <int Q, int RB>
 __global__ void __launch_bounds__(256)
 k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const uint32_t* __restrict__ ring,
           uint32_t nring, int linear) {
+  using L = Lattice<Q>;
   using T = Tile<RB>;
   constexpr int FB = T::FB, RB2 = T::RB2;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nring) return;
-  const int i = blockIdx.y;
-  int ex, ey, ez;
-  float wi;
-  direction<Q>(i, ex, ey, ez, wi);
   // a ring cell over an unrefined L0 cell (k_ring_plan)
   const uint32_t cell = ring[t];
   const int slot = (int)(cell / (uint32_t)T::CELLS);
@@ -264,45 +273,50 @@ k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, cons
   const int gz = wrap1(bz * RB2 + fz - kGhost, NFz);
   const int vx = gx >> 1, vy = gy >> 1, vz = gz >> 1;
   const uint32_t vc = coarse_cell(G, vx, vy, vz);
-  const float* plane = gc + (size_t)i * G.ncells;
-  float out;
-  if (!covered_at(G, vx + ex, vy + ey, vz + ez)) {
-    // The coarse target is unrefined: only a population that streams from
-    // this ring cell into this tile's interior is kept (in-orphan, uniform
-    // injection); everything else is zeroed (f = 0, i.e. g = -w).
-    const int dxg = wrap1(gx + ex, NFx), dyg = wrap1(gy + ey, NFy), dzg = wrap1(gz + ez, NFz);
-    const bool into = covered_at(G, dxg >> 1, dyg >> 1, dzg >> 1) && (dxg >> (T::LOG_RB + 1)) == bx &&
-                      (dyg >> (T::LOG_RB + 1)) == by && (dzg >> (T::LOG_RB + 1)) == bz;
-    out = into ? *at(plane, vc) : -wi;
-  } else if (!linear) {
-    out = *at(plane, vc);
-  } else {
-    // linear reconstruction d . G with the gradient projected
-    // perpendicular to e_i (common_d3_amr_explode.wgsl:197-212)
-    const int xm = wrap1(vx - 1, G.nx), xp = wrap1(vx + 1, G.nx);
-    const int ym = wrap1(vy - 1, G.ny), yp = wrap1(vy + 1, G.ny);
-    const int zm = wrap1(vz - 1, G.nz), zp = wrap1(vz + 1, G.nz);
-    const uint32_t cmx = coarse_cell(G, xm, vy, vz), cpx = coarse_cell(G, xp, vy, vz);
-    const uint32_t cmy = coarse_cell(G, vx, ym, vz), cpy = coarse_cell(G, vx, yp, vz);
-    const uint32_t cmz = coarse_cell(G, vx, vy, zm), cpz = coarse_cell(G, vx, vy, zp);
-    const bool okx = !((G.ccls[cmx] | G.ccls[cpx]) & kCovered);
-    const bool oky = !((G.ccls[cmy] | G.ccls[cpy]) & kCovered);
-    const bool okz = !((G.ccls[cmz] | G.ccls[cpz]) & kCovered);
-    // position of the fine cell inside its coarse cell, in coarse units
-    const float dx = ((float)(gx - 2 * vx) - 0.5f) * 0.5f;
-    const float dy = ((float)(gy - 2 * vy) - 0.5f) * 0.5f;
-    const float dz = ((float)(gz - 2 * vz) - 0.5f) * 0.5f;
-    float Gx = okx ? 0.5f * (*at(plane, cpx) - *at(plane, cmx)) : 0.f;
-    float Gy = oky ? 0.5f * (*at(plane, cpy) - *at(plane, cmy)) : 0.f;
-    float Gz = okz ? 0.5f * (*at(plane, cpz) - *at(plane, cmz)) : 0.f;
-    const float ee = (float)(ex * ex + ey * ey + ez * ez);
-    const float proj = (Gx * ex + Gy * ey + Gz * ez) / ee;
-    Gx -= ex * proj;
-    Gy -= ey * proj;
-    Gz -= ez * proj;
-    out = *at(plane, vc) + (dx * Gx + dy * Gy + dz * Gz);
+  const uint32_t N = G.ncells;
+  const int xm = wrap1(vx - 1, G.nx), xp = wrap1(vx + 1, G.nx);
+  const int ym = wrap1(vy - 1, G.ny), yp = wrap1(vy + 1, G.ny);
+  const int zm = wrap1(vz - 1, G.nz), zp = wrap1(vz + 1, G.nz);
+  const uint32_t cmx = coarse_cell(G, xm, vy, vz), cpx = coarse_cell(G, xp, vy, vz);
+  const uint32_t cmy = coarse_cell(G, vx, ym, vz), cpy = coarse_cell(G, vx, yp, vz);
+  const uint32_t cmz = coarse_cell(G, vx, vy, zm), cpz = coarse_cell(G, vx, vy, zp);
+  const bool okx = !((G.ccls[cmx] | G.ccls[cpx]) & kCovered);
+  const bool oky = !((G.ccls[cmy] | G.ccls[cpy]) & kCovered);
+  const bool okz = !((G.ccls[cmz] | G.ccls[cpz]) & kCovered);
+  // position of the fine cell inside its coarse cell, in coarse units
+  const float dx = ((float)(gx - 2 * vx) - 0.5f) * 0.5f;
+  const float dy = ((float)(gy - 2 * vy) - 0.5f) * 0.5f;
+  const float dz = ((float)(gz - 2 * vz) - 0.5f) * 0.5f;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
+    const float* plane = gc + (size_t)i * N;
+    float out;
+    if (!covered_at(G, vx + ex, vy + ey, vz + ez)) {
+      // The coarse target is unrefined: only a population that streams from
+      // this ring cell into this tile's interior is kept (in-orphan, uniform
+      // injection); everything else is zeroed (f = 0, i.e. g = -w).
+      const int dxg = wrap1(gx + ex, NFx), dyg = wrap1(gy + ey, NFy), dzg = wrap1(gz + ez, NFz);
+      const bool into = covered_at(G, dxg >> 1, dyg >> 1, dzg >> 1) && (dxg >> (T::LOG_RB + 1)) == bx &&
+                        (dyg >> (T::LOG_RB + 1)) == by && (dzg >> (T::LOG_RB + 1)) == bz;
+      out = into ? *at(plane, vc) : -L::w[i];
+    } else if (!linear) {
+      out = *at(plane, vc);
+    } else {
+      // linear reconstruction d . G with the gradient projected
+      // perpendicular to e_i (common_d3_amr_explode.wgsl:197-212)
+      float Gx = okx ? 0.5f * (*at(plane, cpx) - *at(plane, cmx)) : 0.f;
+      float Gy = oky ? 0.5f * (*at(plane, cpy) - *at(plane, cmy)) : 0.f;
+      float Gz = okz ? 0.5f * (*at(plane, cpz) - *at(plane, cmz)) : 0.f;
+      const float ee = (float)(ex * ex + ey * ey + ez * ez);
+      const float proj = (Gx * ex + Gy * ey + Gz * ez) / ee;
+      Gx -= ex * proj;
+      Gy -= ey * proj;
+      Gz -= ez * proj;
+      out = *at(plane, vc) + (dx * Gx + dy * Gy + dz * Gz);
+    }
+    *at(gp + (size_t)i * G.pool_plane, cell) = out;
   }
-  *at(gp + (size_t)i * G.pool_plane, cell) = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -436,15 +450,18 @@ __device__ __forceinline__ void coarse_sample(const float* __restrict__ gc, uint
 template <int Q, int RB>
 __global__ void __launch_bounds__(256)
 k_interp(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const int32_t* __restrict__ pairs,
-         int npairs, float rescale) {
+         int npairs, float rescale, int ring_only) {
   using T = Tile<RB>;
-  constexpr int FB = T::FB;
+  constexpr int FB = T::FB, LO = T::LO, HI = T::HI;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k = t / (uint32_t)T::CELLS;
   if (k >= (uint32_t)npairs) return;
   const uint32_t r = t - k * (uint32_t)T::CELLS;
   const int id = pairs[1 + 2 * k], slot = pairs[2 + 2 * k];
   const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
+  // GHOST_ONLY (common_d3_amr_interp.wgsl:4-9): interior cells keep their
+  // evolved state
+  if (ring_only && fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI) return;
   const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
   // coarse-unit position origin - 1/4 + (j - GHOST)/2 (fineToCoarseUnit3)
   const float px = (float)(bx * RB) - 0.25f + 0.5f * (float)(fx - kGhost);
@@ -511,6 +528,122 @@ k_average(const float* __restrict__ gp, float* __restrict__ gc, float* __restric
 #pragma unroll
   for (int q = 0; q < Q; q++) gc[(size_t)q * G.ncells + cc] = geq<Q>(q, rho, ux, uy, uz) + rescale * (sneq[q] * 0.125f);
   reinterpret_cast<float4*>(mac)[cc] = make_float4(rho, ux, uy, uz);
+}
+
+// ---------------------------------------------------------------------------
+// seam flux and its correction (the interp interface, opt-in)
+// ---------------------------------------------------------------------------
+// What the fine solver moved across the seam in one substep, read from the
+// pool buffer that substep streams (common_d3_amr_flux.wgsl:131-193).  For
+// each of the flux cell's 8 fine cells p and each direction: OUT, the
+// population that streams to p from the interior of a refined tile; IN, the
+// population that streams from p (in the receiving tile's own ring copy)
+// into a refined tile.  [dm, dp] / 8 (V_fine / V_coarse) per list entry.
+template <int Q, int RB>
+__global__ void __launch_bounds__(256)
+k_flux(const float* __restrict__ gp, float4* __restrict__ acc, PoolGeom G, const uint32_t* __restrict__ cells,
+       uint32_t n, int accumulate) {
+  using L = Lattice<Q>;
+  using T = Tile<RB>;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t c = cells[t];
+  const int cx = c % G.nx, cy = (c / G.nx) % G.ny, cz = c / (G.nx * G.ny);
+  const int NFx = 2 * G.nx, NFy = 2 * G.ny, NFz = 2 * G.nz;
+  constexpr int SH = T::LOG_RB + 1;
+  float dm = 0.f, px = 0.f, py = 0.f, pz = 0.f;
+  for (int oc = 0; oc < 8; oc++) {
+    const int p0 = 2 * cx + (oc & 1), p1 = 2 * cy + ((oc >> 1) & 1), p2 = 2 * cz + ((oc >> 2) & 1);
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+      const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
+      const float* plane = gp + (size_t)i * G.pool_plane;
+      const int qx = wrap1(p0 - ex, NFx), qy = wrap1(p1 - ey, NFy), qz = wrap1(p2 - ez, NFz);
+      const int so = G.block_slot[block_id(G, qx >> SH, qy >> SH, qz >> SH)];
+      if (so >= 0) {
+        const float f = *at(plane, pool_cell<RB>(so, local_in_tile<RB>(qx, qx >> SH, NFx),
+                                                 local_in_tile<RB>(qy, qy >> SH, NFy),
+                                                 local_in_tile<RB>(qz, qz >> SH, NFz))) + L::w[i];
+        dm += f;
+        px += f * ex;
+        py += f * ey;
+        pz += f * ez;
+      }
+      const int tx = wrap1(p0 + ex, NFx), ty = wrap1(p1 + ey, NFy), tz = wrap1(p2 + ez, NFz);
+      const int si = G.block_slot[block_id(G, tx >> SH, ty >> SH, tz >> SH)];
+      if (si >= 0) {
+        const float f = *at(plane, pool_cell<RB>(si, local_in_tile<RB>(p0, tx >> SH, NFx),
+                                                 local_in_tile<RB>(p1, ty >> SH, NFy),
+                                                 local_in_tile<RB>(p2, tz >> SH, NFz))) + L::w[i];
+        dm -= f;
+        px -= f * ex;
+        py -= f * ey;
+        pz -= f * ez;
+      }
+    }
+  }
+  float4 v = make_float4(dm * 0.125f, px * 0.125f, py * 0.125f, pz * 0.125f);
+  if (accumulate) {
+    const float4 o = acc[t];
+    v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
+  }
+  acc[t] = v;
+}
+
+// The correction onto the unrefined flux cells of the coarse state at t + 1
+// (common_d3_amr_reflux.wgsl:105-164): delta = fine flux + (coarse_in -
+// coarse_out), both coarse terms read from the state at t, applied as
+// f_i += w_i (d_rho + 3 e_i . d_mom); then the cell's [rho, u] republished
+// with the Guo half-force.  The applied delta is left in acc.
+template <int Q, int RB>
+__global__ void __launch_bounds__(256)
+k_reflux(const float* __restrict__ gct, float* __restrict__ gct1, float4* __restrict__ mac, PoolGeom G,
+         const uint32_t* __restrict__ cells, uint32_t n, float4* __restrict__ acc, float Fx, float Fy, float Fz) {
+  using L = Lattice<Q>;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t c = cells[t];
+  const int cx = c % G.nx, cy = (c / G.nx) % G.ny, cz = c / (G.nx * G.ny);
+  const uint32_t N = G.ncells;
+  const float4 a = acc[t];
+  float dm = a.x, px = a.y, py = a.z, pz = a.w;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
+    // coarse_out: what this cell pulled out of the refined region
+    const uint32_t s = coarse_cell(G, wrap1(cx - ex, G.nx), wrap1(cy - ey, G.ny), wrap1(cz - ez, G.nz));
+    if (G.ccls[s] & kCovered) {
+      const float v = *at(gct + (size_t)i * N, s) + L::w[i];
+      dm -= v;
+      px -= v * ex;
+      py -= v * ey;
+      pz -= v * ez;
+    }
+    // coarse_in: what the refined region pulled out of this cell
+    if (covered_at(G, cx + ex, cy + ey, cz + ez)) {
+      const float v = *at(gct + (size_t)i * N, c) + L::w[i];
+      dm += v;
+      px += v * ex;
+      py += v * ey;
+      pz += v * ez;
+    }
+  }
+  acc[t] = make_float4(dm, px, py, pz);
+  float rho = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
+    float* gi = at(gct1 + (size_t)i * N, c);
+    const float g = *gi + L::w[i] * (dm + 3.f * (ex * px + ey * py + ez * pz));
+    *gi = g;
+    const float f = g + L::w[i];
+    rho += f;
+    mx += f * ex;
+    my += f * ey;
+    mz += f * ez;
+  }
+  const float den = fmaxf(rho, 1e-6f);
+  mac[c] = make_float4(rho, mx / den + Fx / (2.f * den), my / den + Fy / (2.f * den), mz / den + Fz / (2.f * den));
 }
 
 // ---------------------------------------------------------------------------
@@ -729,9 +862,27 @@ struct OpCellClass {
 template <int Q, int RB>
 struct OpPoolStep {
   static hipError_t run(const float* gin, float* gout, float* mac_pool, const PoolGeom& g, const CollideParams& p,
-                        const int32_t* active, int nactive, hipStream_t s) {
+                        const int32_t* active, int nactive, int collide_ring, hipStream_t s) {
     hipLaunchKernelGGL((k_pool_step<Q, RB>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s, gin,
-                       gout, mac_pool, g, p, active, nactive);
+                       gout, mac_pool, g, p, active, nactive, collide_ring);
+    return hipGetLastError();
+  }
+};
+template <int Q, int RB>
+struct OpFlux {
+  static hipError_t run(const float* gp, float* acc, const PoolGeom& g, const SurfacePlan& plan, uint32_t n,
+                        int accumulate, hipStream_t s) {
+    hipLaunchKernelGGL((k_flux<Q, RB>), grid_for(n), dim3(256), 0, s, gp, reinterpret_cast<float4*>(acc), g, plan.flux,
+                       n, accumulate);
+    return hipGetLastError();
+  }
+};
+template <int Q, int RB>
+struct OpReflux {
+  static hipError_t run(const float* gct, float* gct1, float* mac, const PoolGeom& g, const SurfacePlan& plan,
+                        uint32_t n, float* acc, const float* F, hipStream_t s) {
+    hipLaunchKernelGGL((k_reflux<Q, RB>), grid_for(n), dim3(256), 0, s, gct, gct1, reinterpret_cast<float4*>(mac), g,
+                       plan.flux, n, reinterpret_cast<float4*>(acc), F[0], F[1], F[2]);
     return hipGetLastError();
   }
 };
@@ -739,7 +890,7 @@ template <int Q, int RB>
 struct OpExplode {
   static hipError_t run(const float* gc, float* gp, const PoolGeom& g, const SurfacePlan& plan, uint32_t nring,
                         int linear, hipStream_t s) {
-    hipLaunchKernelGGL((k_explode<Q, RB>), grid_for(nring, Q), dim3(256), 0, s, gc, gp, g, plan.ring, nring, linear);
+    hipLaunchKernelGGL((k_explode<Q, RB>), grid_for(nring), dim3(256), 0, s, gc, gp, g, plan.ring, nring, linear);
     return hipGetLastError();
   }
 };
@@ -762,9 +913,9 @@ struct OpCoalesce {
 template <int Q, int RB>
 struct OpInterp {
   static hipError_t run(const float* gc, float* gp, const PoolGeom& g, const int32_t* pairs, int npairs, float rescale,
-                        hipStream_t s) {
+                        int ring_only, hipStream_t s) {
     hipLaunchKernelGGL((k_interp<Q, RB>), grid_for((uint64_t)npairs * Tile<RB>::CELLS), dim3(256), 0, s, gc, gp, g,
-                       pairs, npairs, rescale);
+                       pairs, npairs, rescale, ring_only);
     return hipGetLastError();
   }
 };
@@ -792,9 +943,20 @@ hipError_t launch_cell_class(int q, uint8_t* ccls, const PoolGeom& g, const Surf
 }
 
 hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
-                            const CollideParams& fine, const int32_t* active, int nactive, hipStream_t s) {
+                            const CollideParams& fine, const int32_t* active, int nactive, int collide_ring,
+                            hipStream_t s) {
   if (nactive <= 0) return hipSuccess;
-  return dispatch_q_rb<OpPoolStep>(q, g.rb, gin, gout, mac_pool, g, fine, active, nactive, s);
+  return dispatch_q_rb<OpPoolStep>(q, g.rb, gin, gout, mac_pool, g, fine, active, nactive, collide_ring, s);
+}
+hipError_t launch_flux(int q, const float* gp, float* acc, const PoolGeom& g, const SurfacePlan& plan, uint32_t nflux,
+                       int accumulate, hipStream_t s) {
+  if (nflux == 0) return hipSuccess;
+  return dispatch_q_rb<OpFlux>(q, g.rb, gp, acc, g, plan, nflux, accumulate, s);
+}
+hipError_t launch_reflux(int q, const float* gct, float* gct1, float* mac, const PoolGeom& g, const SurfacePlan& plan,
+                         uint32_t nflux, float* acc, const float* force, hipStream_t s) {
+  if (nflux == 0) return hipSuccess;
+  return dispatch_q_rb<OpReflux>(q, g.rb, gct, gct1, mac, g, plan, nflux, acc, force, s);
 }
 
 hipError_t launch_explode(int q, const float* gc, float* gp, const PoolGeom& g, const SurfacePlan& plan,
@@ -811,9 +973,9 @@ hipError_t launch_coalesce(int q, const float* gp, float* gc, float* mac, const 
 }
 
 hipError_t launch_interp(int q, const float* gc, float* gp, const PoolGeom& g, const int32_t* pairs, int npairs,
-                         float rescale, hipStream_t s) {
+                         float rescale, int ring_only, hipStream_t s) {
   if (npairs <= 0) return hipSuccess;
-  return dispatch_q_rb<OpInterp>(q, g.rb, gc, gp, g, pairs, npairs, rescale, s);
+  return dispatch_q_rb<OpInterp>(q, g.rb, gc, gp, g, pairs, npairs, rescale, ring_only, s);
 }
 
 hipError_t launch_average(int q, const float* gp, float* gc, float* mac, const PoolGeom& g, const int32_t* pairs,

@@ -536,7 +536,7 @@ int create_impl(const lbm_desc* desc, Arena* arena, hipStream_t stream, int bx, 
   s->general = s->body || walls || desc->sponge_width > 0.f;
   {
     int flags = 0;
-    default_tile(desc->storage, &s->bx, &s->by, &flags);
+    default_tile(desc->storage, desc->nx, &s->bx, &s->by, &flags);
     s->swz = flags & 1;
     s->nt = (flags >> 1) & 1;
     s->lin = (flags >> 2) & 1;
@@ -607,15 +607,17 @@ size_t solve_footprint(const lbm_desc* d) {
   return total;
 }
 
-void default_tile(int storage, int* bx, int* by, int* flags) {
-  // Linear 256-cell tiles leave no lane idle at any width, non-temporal
-  // access keeps the streamed lattices out of the caches, and fp32 (256 B
-  // per wave load) runs best with residency capped at 4 waves per SIMD while
-  // the 16-bit formats want every wave they can get (D3Q19 sphere tile sweep
-  // on MI355X).
+void default_tile(int storage, int nx, int* bx, int* by, int* flags) {
+  // 256-cell tiles with non-temporal access (the streamed lattices stay out
+  // of the caches).  Rows that are whole tiles take 2D 256x1 tiles (z and y
+  // wave-uniform, fewer per-lane instructions); other widths take linear
+  // tiles, which leave no lane idle.  Those two layouts number their tiles
+  // alike when nx is a multiple of 256.  fp32 (256 B per wave load) runs
+  // best with residency capped at 4 waves per SIMD, the 16-bit formats want
+  // every wave they can get (D3Q19 sphere tile sweep on MI355X).
   *bx = 256;
   *by = 1;
-  *flags = 0x6 | (storage == LBM_STORE_F32 ? 0x40 : 0);
+  *flags = 0x2 | (nx > 0 && nx % 256 == 0 ? 0 : 0x4) | (storage == LBM_STORE_F32 ? 0x40 : 0);
 }
 
 int solve_create_in(const lbm_desc* d, Arena* arena, hipStream_t stream, int bx, int by, int flags,

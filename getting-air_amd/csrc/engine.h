@@ -32,9 +32,9 @@ struct Arena {
 // Device bytes a solve of this descriptor takes (what solve_create_in asks
 // the arena for, alignment included).
 size_t solve_footprint(const lbm_desc* d);
-// Tile of a solve that is not autotuned (and of a serve class without the
-// autotuner), as lbm_set_tile takes it.
-void default_tile(int storage, int* bx, int* by, int* flags);
+// Tile of a solve that is not autotuned, as lbm_set_tile takes it; nx <= 0
+// (a serve class, whose members differ in width) always gives linear tiles.
+void default_tile(int storage, int nx, int* bx, int* by, int* flags);
 // `flags` as lbm_set_tile takes them; a member uses its tile layout (bit 2,
 // linear tiles), the batched launch the rest.
 int solve_create_in(const lbm_desc* d, Arena* arena, hipStream_t stream, int bx, int by, int flags,

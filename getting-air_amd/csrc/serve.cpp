@@ -263,7 +263,7 @@ void admit(lbm_serve* sv) {
       cp = std::make_unique<Class>();
       cp->key = key;
       int flags = 0;
-      default_tile(key.storage, &cp->bx, &cp->by, &flags);
+      default_tile(key.storage, 0, &cp->bx, &cp->by, &flags);
       cp->nt = (flags >> 1) & 1;
       cp->lin = (flags >> 2) & 1;
       cp->occ = (flags >> 4) & 15;

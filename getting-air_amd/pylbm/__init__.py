@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblbm.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 BGK, TRT = 0, 1
 STORE = {"f32": 0, "f16": 1, "bf16": 2}
 BODY_NONE, BODY_SPHERE, BODY_CIRCLE = 0, 1, 2
@@ -56,6 +56,7 @@ class AmrDesc(ctypes.Structure):
         ("omega", ctypes.c_float), ("omega_minus", ctypes.c_float), ("force", ctypes.c_float * 3),
         ("manage_every", ctypes.c_int32), ("refine_at", ctypes.c_float), ("keep_at", ctypes.c_float),
         ("lead", ctypes.c_int32), ("explode_linear", ctypes.c_int32), ("orphans", ctypes.c_int32),
+        ("interface_interp", ctypes.c_int32), ("reflux", ctypes.c_int32),
     ]
 
 
@@ -311,7 +312,8 @@ class Solve:
 
 def make_amr_desc(d):
     """AmrDesc from a dict {q, nx, ny, nz, rb, maxSlots, omega, omegaMinus,
-    force, manageEvery, refineAt, keepAt, lead, explodeLinear, orphans}."""
+    force, manageEvery, refineAt, keepAt, lead, explodeLinear, orphans,
+    interface ('explode' or 'interp'), reflux}."""
     a = AmrDesc()
     a.abi_version = ABI_VERSION
     a.q = int(d["q"])
@@ -330,6 +332,11 @@ def make_amr_desc(d):
     a.lead = int(d.get("lead", 0))
     a.explode_linear = int(bool(d.get("explodeLinear", 1)))
     a.orphans = int(bool(d.get("orphans", 1)))
+    iface = d.get("interface", "explode")
+    if iface not in ("explode", "interp"):
+        raise ValueError("interface must be 'explode' or 'interp'")
+    a.interface_interp = int(iface == "interp")
+    a.reflux = int(bool(d.get("reflux", 0)))
     return a
 
 

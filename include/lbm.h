@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 1
+#define LBM_ABI_VERSION 2
 
 enum {
   LBM_OK = 0,
@@ -180,6 +180,15 @@ typedef struct lbm_amr_desc {
   int32_t lead;         /* criterion box dilation in L0 cells                 */
   int32_t explode_linear; /* EXPLODE_LINEAR (common_d3_amr_explode.wgsl)      */
   int32_t orphans;      /* ORPHANS (common_d3_amr_coalesce.wgsl)              */
+  int32_t interface_interp; /* 1: the interp coarse/fine interface
+                         * (?interface=interp, main-3d.js:1266-1277, pass order
+                         * :3344-3357): refinement-blind coarse step, ring
+                         * re-interpolated from the coarse state at t, ring
+                         * cells collide, every tile restricted each step.
+                         * 0: explode/coalesce (the default; conservative).  */
+  int32_t reflux;       /* ?reflux=1 (main-3d.js:1330-1343): seam mass and
+                         * momentum flux correction, common_d3_amr_flux.wgsl +
+                         * common_d3_amr_reflux.wgsl; needs interface_interp  */
 } lbm_amr_desc;
 
 typedef struct lbm_amr lbm_amr;

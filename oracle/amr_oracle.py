@@ -20,6 +20,13 @@ explode/coalesce AMR macro-step:
                       out in ascending block id (d3-amr.mjs:75-99)
   active list         shaders/amr_active_list.wgsl:41-70
   macro-step order    main-3d.js:3122-3335; per-level tau main-3d.js:1188-1193
+  interp interface    ?interface=interp, main-3d.js:1266-1277 and :3344-3357:
+                      coarse step first, ring re-interpolated from the coarse
+                      state at t (GHOST_ONLY, TIME_BLEND 0), ring cells collide
+                      (COLLIDE_RING = 1), every tile restricted after the two
+                      substeps
+  flux / reflux       shaders/common_d3_amr_flux.wgsl:80-194 and
+                      shaders/common_d3_amr_reflux.wgsl:60-165 (?reflux=1)
 
 One deliberate difference from the reference's manager pass, shared with
 the engine: before the manager runs, EVERY tile in use is restricted onto
@@ -41,6 +48,9 @@ uniform and linear explosion, on flat faces, edges and corners; a uniform
 flow stays uniform; restriction after interpolation is the identity on
 fields linear in space; the criterion meets the closed forms of
 test-d3-criterion.js (pure shear |w| = gamma, solid rotation |w| = 2 Omega).
+The interp interface is not conservative; with the flux correction the
+composite mass and momentum are conserved to float64 rounding (the claim of
+common_d3_amr_flux.wgsl:8-11), and the same check fails without it.
 """
 import numpy as np
 
@@ -216,6 +226,11 @@ class AmrOracle:
         self.lead = int(desc.get("lead", 0))
         self.linear = bool(desc.get("explodeLinear", 1))
         self.orphans = bool(desc.get("orphans", 1))
+        self.interp_iface = desc.get("interface", "explode") == "interp"
+        self.reflux_on = bool(desc.get("reflux", 0))
+        if self.reflux_on and not self.interp_iface:
+            raise ValueError("reflux needs the interp interface")
+        self.flux_acc = None
         self.block_slot = np.full(self.nblocks, -1, dtype=np.int32)
         self.slot_to_block = np.full(self.max_slots, -1, dtype=np.int32)
         n = self.fb
@@ -312,8 +327,10 @@ class AmrOracle:
 
     def pool_step(self):
         """One fine substep of every tile: gather, moments into mac_pool,
-        interior BGK at omega_f with the unchanged body force, ring cells
-        advect only (COLLIDE_RING = 0, common_d3_amr_step1.wgsl:369-372)."""
+        interior BGK at omega_f with the unchanged body force; ring cells
+        advect only with explode/coalesce (COLLIDE_RING = 0) and collide too
+        with the interp interface (COLLIDE_RING = 1,
+        common_d3_amr_step1.wgsl:369-372, main-3d.js:1395)."""
         T, n = self.T, self.fb
         flat = self.pool.reshape(self.q, -1)
         new = self.pool.copy()
@@ -328,7 +345,7 @@ class AmrOracle:
             for k, v in enumerate((rho, ux, uy, uz)):
                 self.mac_pool[k, slot] = v
             col = collide(T, g, rho, ux, uy, uz, self.omega_f, self.omega_f, self.F)
-            new[:, slot] = np.where(interior, col, g)
+            new[:, slot] = col if self.interp_iface else np.where(interior, col, g)
         self.pool = new
 
     def explode(self):
@@ -454,18 +471,25 @@ class AmrOracle:
             net = s - nin * fc
             f0[i, cz[sel], cy[sel], cx[sel]] = (fc + net * 0.125)[sel]
 
-    def _coarse_sample(self, x, y, z):
-        f = self.f0[:, z % self.nz, y % self.ny, x % self.nx]
+    def _coarse_sample(self, x, y, z, f0=None):
+        f = (self.f0 if f0 is None else f0)[:, z % self.nz, y % self.ny, x % self.nx]
         rho, ux, uy, uz = moments(self.T, f)
         return rho, ux, uy, uz, f - feq(self.T, rho, ux, uy, uz)
 
-    def interp(self, pairs):
+    def interp(self, pairs, f0=None, ring_only=False):
         """Fill whole tiles (ring included) from L0: trilinear rho, u and
-        fneq x 1/2 (tau_f - 1)/(tau_c - 1) (common_d3_pool.wgsl:172-220)."""
+        fneq x 1/2 (tau_f - 1)/(tau_c - 1) (common_d3_pool.wgsl:172-220).
+        ring_only: the ring cells alone (GHOST_ONLY = 1,
+        common_d3_amr_interp.wgsl:4-9), from the coarse populations f0."""
         n = self.fb
+        src = self.f0 if f0 is None else f0
+        lo, hi = GHOST, GHOST + self.rb2
         for b, slot in pairs:
             bx, by, bz = self.block_xyz(b)
             fz, fy, fx = (a.ravel() for a in np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"))
+            if ring_only:
+                ring = ~((fx >= lo) & (fx < hi) & (fy >= lo) & (fy < hi) & (fz >= lo) & (fz < hi))
+                fz, fy, fx = fz[ring], fy[ring], fx[ring]
             P = [o * self.rb - 0.25 + 0.5 * (j - GHOST) for o, j in ((bx, fx), (by, fy), (bz, fz))]
             c0 = [np.floor(p).astype(np.int64) for p in P]
             t = [p - c for p, c in zip(P, c0)]
@@ -475,7 +499,7 @@ class AmrOracle:
             for k in range(8):
                 i, j, kk = k & 1, (k >> 1) & 1, (k >> 2) & 1
                 w = (t[0] if i else 1 - t[0]) * (t[1] if j else 1 - t[1]) * (t[2] if kk else 1 - t[2])
-                sr, sx, sy, sz, sf = self._coarse_sample(c0[0] + i, c0[1] + j, c0[2] + kk)
+                sr, sx, sy, sz, sf = self._coarse_sample(c0[0] + i, c0[1] + j, c0[2] + kk, src)
                 rho += w * sr
                 u[0] += w * sx
                 u[1] += w * sy
@@ -537,12 +561,119 @@ class AmrOracle:
     def macro_step(self, crit=None):
         if self.manage_every > 0 and self.step_count % self.manage_every == 0:
             self.manage(crit)
-        self.explode()
-        self.pool_step()
-        self.pool_step()
-        self.coalesce()
-        self.l0_step()
+        if self.interp_iface:
+            self._macro_step_interp()
+        else:
+            self.explode()
+            self.pool_step()
+            self.pool_step()
+            self.coalesce()
+            self.l0_step()
         self.step_count += 1
+
+    def _macro_step_interp(self):
+        """main-3d.js:3344-3357: coarse step, ring from the coarse state at t,
+        [flux], substep A, [flux], substep B, average, [reflux]."""
+        ft = self.f0.copy()
+        self.l0_step()
+        pairs = [(int(self.slot_to_block[s]), int(s)) for s in active_list(self.slot_to_block)]
+        if not pairs:
+            return
+        self.interp(pairs, f0=ft, ring_only=True)
+        if self.reflux_on:
+            acc = self.flux()
+        self.pool_step()
+        if self.reflux_on:
+            acc = acc + self.flux()
+        self.pool_step()
+        self.average(pairs)
+        if self.reflux_on:
+            self.reflux(ft, acc)
+
+    def flux_cells(self):
+        """Unrefined L0 cells with a refined block among the blocks their 26
+        neighbours span (common_d3_amr_flux.wgsl:101-122): (x, y, z) arrays."""
+        cov = self.covered_mask()
+        near = np.zeros_like(cov)
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    near |= np.roll(cov, (-dz, -dy, -dx), (0, 1, 2))
+        z, y, x = np.nonzero(near & ~cov)
+        return x, y, z
+
+    def _slot_of_fine(self, p):
+        b = [p[k] // self.rb2 for k in range(3)]
+        return self.block_slot[self.block_id(b[0], b[1], b[2])], b
+
+    def flux(self):
+        """(Fout - Fin)/8 over the 8 fine cells of every flux cell, from the
+        pool state the next substep streams (common_d3_amr_flux.wgsl:131-193):
+        array [4, ncells_listed] of [dm, dpx, dpy, dpz]."""
+        T = self.T
+        x, y, z = self.flux_cells()
+        NF = (2 * self.nx, 2 * self.ny, 2 * self.nz)
+        flat = self.pool.reshape(self.q, self.max_slots, self.fb, self.fb, self.fb)
+        dm = np.zeros(x.size)
+        dp = np.zeros((3, x.size))
+        for oc in range(8):
+            p = [2 * x + (oc & 1), 2 * y + ((oc >> 1) & 1), 2 * z + ((oc >> 2) & 1)]
+            for i in range(T.q):
+                e = (T.ex[i], T.ey[i], T.ez[i])
+                q = [(p[k] - e[k]) % NF[k] for k in range(3)]
+                so, bq = self._slot_of_fine(q)
+                sel = so >= 0
+                if sel.any():
+                    l = [self.local_in_tile(q[k][sel], bq[k][sel], NF[k]) for k in range(3)]
+                    f = flat[i, so[sel], l[2], l[1], l[0]]
+                    dm[sel] += f
+                    for k in range(3):
+                        dp[k][sel] += f * e[k]
+                t = [(p[k] + e[k]) % NF[k] for k in range(3)]
+                si, bt = self._slot_of_fine(t)
+                sel = si >= 0
+                if sel.any():
+                    l = [self.local_in_tile(p[k][sel], bt[k][sel], NF[k]) for k in range(3)]
+                    f = flat[i, si[sel], l[2], l[1], l[0]]
+                    dm[sel] -= f
+                    for k in range(3):
+                        dp[k][sel] -= f * e[k]
+        return np.vstack([dm, dp]) * 0.125
+
+    def reflux(self, ft, acc):
+        """delta = acc + (coarse_in - coarse_out) applied to the unrefined
+        flux cells as f_i += w_i (d_rho + 3 e_i . d_mom), then their mac
+        republished (common_d3_amr_reflux.wgsl:105-164)."""
+        T = self.T
+        x, y, z = self.flux_cells()
+        dm, dp = acc[0].copy(), acc[1:].copy()
+        for i in range(T.q):
+            e = (T.ex[i], T.ey[i], T.ez[i])
+            sx, sy, sz = (x - e[0]) % self.nx, (y - e[1]) % self.ny, (z - e[2]) % self.nz
+            sel = self.covered(sx, sy, sz)
+            v = ft[i, sz, sy, sx]
+            dm -= np.where(sel, v, 0.0)
+            for k in range(3):
+                dp[k] -= np.where(sel, v * e[k], 0.0)
+            sel = self.covered(x + e[0], y + e[1], z + e[2])
+            v = ft[i, z, y, x]
+            dm += np.where(sel, v, 0.0)
+            for k in range(3):
+                dp[k] += np.where(sel, v * e[k], 0.0)
+        self.flux_acc = np.vstack([dm, dp])
+        rho = np.zeros(x.size)
+        m = np.zeros((3, x.size))
+        for i in range(T.q):
+            e = (T.ex[i], T.ey[i], T.ez[i])
+            fo = self.f0[i, z, y, x] + T.w[i] * (dm + 3.0 * (e[0] * dp[0] + e[1] * dp[1] + e[2] * dp[2]))
+            self.f0[i, z, y, x] = fo
+            rho += fo
+            for k in range(3):
+                m[k] += fo * e[k]
+        den = np.maximum(rho, 1e-6)
+        self.mac0[0, z, y, x] = rho
+        for k in range(3):
+            self.mac0[1 + k, z, y, x] = m[k] / den + self.F[k] / (2.0 * den)
 
     # --- read-outs ----------------------------------------------------------
     def totals(self):

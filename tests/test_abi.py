@@ -70,9 +70,9 @@ def test_amr_descriptor_layout_matches_c():
 #include <stddef.h>
 #include "lbm.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(lbm_amr_desc), offsetof(lbm_amr_desc, omega),
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_amr_desc), offsetof(lbm_amr_desc, omega),
          offsetof(lbm_amr_desc, force), offsetof(lbm_amr_desc, manage_every),
-         offsetof(lbm_amr_desc, lead), offsetof(lbm_amr_desc, orphans));
+         offsetof(lbm_amr_desc, lead), offsetof(lbm_amr_desc, orphans), offsetof(lbm_amr_desc, reflux));
   return 0;
 }
 """
@@ -84,7 +84,7 @@ int main(void) {
         got = [int(v) for v in subprocess.check_output([exe]).split()]
     A = pylbm.AmrDesc
     want = [ctypes.sizeof(A), A.omega.offset, A.force.offset, A.manage_every.offset, A.lead.offset,
-            A.orphans.offset]
+            A.orphans.offset, A.reflux.offset]
     assert got == want
 
 
@@ -106,6 +106,10 @@ def test_amr_refusals_without_gpu():
     assert b"rb = 4, 8, 16" in lib.lbm_last_error()
     d = pylbm.make_amr_desc(dict(base, q=9))
     assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    # the seam flux correction needs the interp interface (main-3d.js:1333-1336)
+    d = pylbm.make_amr_desc(dict(base, reflux=1))
+    assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"interp" in lib.lbm_last_error()
 
 
 def test_serve_struct_layouts_match_c():

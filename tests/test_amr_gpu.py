@@ -107,6 +107,34 @@ def test_static_seams_match_the_oracle(shape, linear):
         assert (amr.read_active() == ao.active_list(orc.slot_to_block)).all()
 
 
+@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("reflux", [0, 1])
+def test_interp_interface_and_reflux_match_the_oracle(q, reflux):
+    n = 16
+    desc = {"q": q, "nx": n, "ny": n, "nz": n, "rb": 4, "maxSlots": 8, "omega": 1 / 0.6,
+            "interface": "interp", "reflux": reflux}
+    say("interp interface q", q, "reflux", reflux)
+    orc0 = ao.AmrOracle(desc)
+    amr, orc = make(desc, smooth_field(n, n, n, 11, amp=0.04), ids(orc0, SEAMS["mixed"]))
+    with amr:
+        t0 = amr.read_totals()
+        for k in range(3):
+            amr.step(2)
+            orc.macro_step()
+            orc.macro_step()
+            compare(amr, orc, label="macro-step %d" % (2 * k + 2))
+        t = amr.read_totals()
+        drift = max(abs(t[0] - t0[0]), np.abs(t[1:] - t0[1:]).max()) / t0[0]
+        ot = orc.totals()
+        say("composite drift gpu %.2e" % drift)
+        if reflux:
+            # exact in the oracle; binary32 rounding on the GPU
+            assert drift < 1e-6
+        bs, sb = amr.read_topology()
+        assert (bs == orc.block_slot).all() and (sb == orc.slot_to_block).all()
+        assert ot.shape == (4,)
+
+
 def test_all_refined_matches_the_oracle():
     n = 12
     desc = {"q": 19, "nx": n, "ny": n, "nz": n, "rb": 4, "maxSlots": 27, "omega": 1 / 0.7,

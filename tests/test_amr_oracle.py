@@ -152,6 +152,48 @@ def test_conservation_check_has_teeth_without_orphans():
     assert abs(o.totals()[0] - t0[0]) > 1e-8 * t0[0]
 
 
+@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("shape", sorted(SEAMS))
+def test_interp_interface_with_reflux_is_conservative(q, shape):
+    # common_d3_amr_flux.wgsl:8-11: with the flux correction the composite
+    # mass and momentum are conserved exactly; without it they drift
+    n, rb = 16, 4
+    drift = {}
+    for reflux in (0, 1):
+        desc = {"q": q, "nx": n, "ny": n, "nz": n, "rb": rb, "maxSlots": 8, "omega": 1 / 0.6,
+                "interface": "interp", "reflux": reflux, "exactWeights": True}
+        o = ao.AmrOracle(desc)
+        o.upload_topology(*topology(o, ids(o, SEAMS[shape])))
+        o.upload_macro(smooth_field(n, n, n, 11, amp=0.04))
+        t0 = o.totals()
+        worst = 0.0
+        for _ in range(3):
+            o.macro_step()
+            t = o.totals()
+            worst = max(worst, abs(t[0] - t0[0]) / t0[0], np.abs(t[1:] - t0[1:]).max() / t0[0])
+        drift[reflux] = worst
+    assert drift[1] < 1e-14, drift
+    assert drift[0] > 1e-7, drift
+
+
+def test_interp_interface_all_refined_is_a_dense_fine_lattice():
+    # with every block refined no ring cell is read (DIRECT_GHOST), so the
+    # interp interface steps the same dense fine lattice as explode/coalesce
+    n, rb = 12, 4
+    desc = {"q": 27, "nx": n, "ny": n, "nz": n, "rb": rb, "maxSlots": 27, "omega": f32(1 / 0.7),
+            "force": [f32(2e-6), f32(1e-6), 0.0], "interface": "interp", "reflux": 1}
+    o = ao.AmrOracle(desc)
+    o.upload_topology(*topology(o, range(o.nblocks)))
+    o.upload_macro(smooth_field(n, n, n, 5))
+    o.pool = o.pool.astype(np.float32).astype(np.float64)
+    f_start = fine_global(o).reshape(o.q, -1)
+    for _ in range(2):
+        o.macro_step()
+    js = run_js({"q": 27, "nx": 2 * n, "ny": 2 * n, "nz": 2 * n, "omega": o.omega_f, "force": desc["force"]},
+                f=f_start, steps=4, want=("f",))
+    assert np.abs(fine_global(o).reshape(o.q, -1) - js["f"]).max() < 2e-9
+
+
 def test_uniform_flow_stays_uniform():
     n, rb = 16, 4
     u = (0.03, -0.02, 0.01)
