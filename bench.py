@@ -91,17 +91,20 @@ def cpu_baseline(steps, threads):
     }
 
 
-def measure_amr(steps, manage_every=50):
+def measure_amr(steps, manage_every=50, interface="explode", reflux=0):
     """The two-level AMR workload: D3Q27 BGK, 128^3 root, 8^3-cell blocks
     (16^3-cell tiles), seeded vortex rings (radius 20, core 4, circulation
     0.1), |curl u| criterion with hysteresis, manager every 50 macro-steps.
     Cell updates per macro-step count the L0 grid plus two fine substeps of
-    every tile interior (the reference's cellUpdatesPerMacroStep)."""
+    every tile interior (the reference's cellUpdatesPerMacroStep).
+    `interface`/`reflux` select the coarse/fine coupling (explode/coalesce,
+    or interp with or without the seam flux correction)."""
     desc, mac, meta = sc.vortex_rings(n=128, radius=20.0, core=4.0, circulation=0.1, seed=7, jitter=8.0,
                                       tau=0.52, q=27)
     w0 = meta["vorticity_max"]
     adesc = dict(desc, rb=8, maxSlots=2048, manageEvery=manage_every, lead=1,
-                 refineAt=float(np.float32(0.25 * w0)), keepAt=float(np.float32(0.15 * w0)))
+                 refineAt=float(np.float32(0.25 * w0)), keepAt=float(np.float32(0.15 * w0)),
+                 interface=interface, reflux=reflux)
     n0 = 128 ** 3
     interior = 16 ** 3
     with pylbm.Amr(adesc) as a:
@@ -124,7 +127,39 @@ def measure_amr(steps, manage_every=50):
     return {"mlups": updates / seconds / 1e6, "seconds": seconds, "macro_steps": steps,
             "cell_updates": updates, "tiles_min": min(tiles), "tiles_max": max(tiles), "tiles_last": tiles[-1],
             "exhausted": info["exhausted"], "mass_drift_rel": float((t_end[0] - t_start[0]) / t_start[0]),
-            "refine_at": adesc["refineAt"], "keep_at": adesc["keepAt"], "tau": 0.52}
+            "momentum_drift_rel": float(np.abs(t_end[1:] - t_start[1:]).max() / t_start[0]),
+            "refine_at": adesc["refineAt"], "keep_at": adesc["keepAt"], "tau": 0.52,
+            "interface": interface, "reflux": reflux}
+
+
+def measure_amr_slab(steps, reflux):
+    """The seam flux correction where it is valid: a static slab of refined
+    blocks (z in the middle half, every x and y), so every seam is a flat face
+    -- at a convex corner the fine lattice's channels do not tile the coarse
+    one and the correction perturbs rather than conserves
+    (common_d3_amr_flux.wgsl:13-32).  Same D3Q27 vortex-ring field and 128^3
+    root as the AMR workload, interp interface."""
+    desc, mac, _ = sc.vortex_rings(n=128, radius=20.0, core=4.0, circulation=0.1, seed=7, jitter=8.0, tau=0.52, q=27)
+    nb = 16
+    ids = [(bz * nb + by) * nb + bx for bz in range(4, 12) for by in range(nb) for bx in range(nb)]
+    bs = np.full(nb ** 3, -1, dtype=np.int32)
+    sb = np.full(len(ids), -1, dtype=np.int32)
+    for k, b in enumerate(ids):
+        bs[b] = k
+        sb[k] = b
+    adesc = dict(desc, rb=8, maxSlots=len(ids), manageEvery=0, interface="interp", reflux=reflux)
+    with pylbm.Amr(adesc) as a:
+        a.upload_topology(bs, sb)
+        a.upload_macro(mac)
+        t_start = a.read_totals()
+        t0 = time.perf_counter()
+        a.step(steps)  # returns after the stream drains
+        seconds = time.perf_counter() - t0
+        t_end = a.read_totals()
+    updates = steps * (128 ** 3 + 2 * len(ids) * 16 ** 3)
+    return {"mlups": updates / seconds / 1e6, "macro_steps": steps, "tiles": len(ids), "reflux": reflux,
+            "mass_drift_rel": float((t_end[0] - t_start[0]) / t_start[0]),
+            "momentum_drift_rel": float(np.abs(t_end[1:] - t_start[1:]).max() / t_start[0])}
 
 
 def enstrophy(mac, n):
@@ -295,10 +330,17 @@ def main():
             extra[st + "_store"] = r
             print("%s: %s" % (st, json.dumps(r)), file=sys.stderr, flush=True)
 
+    extra_amr_interp = None
     if not args.no_amr:
         r = measure_amr(args.amr_steps)
         extra_amr = r
         print("amr: %s" % json.dumps(r), file=sys.stderr, flush=True)
+        extra_amr_interp = {"dynamic": measure_amr(args.amr_steps, interface="interp", reflux=0)}
+        print("amr interp: %s" % json.dumps(extra_amr_interp["dynamic"]), file=sys.stderr, flush=True)
+        for rf in (0, 1):
+            r = measure_amr_slab(min(args.amr_steps, 500), rf)
+            extra_amr_interp["slab_reflux" if rf else "slab_no_reflux"] = r
+            print("amr interp slab reflux=%d: %s" % (rf, json.dumps(r)), file=sys.stderr, flush=True)
     else:
         extra_amr = None
 
@@ -361,9 +403,13 @@ def main():
         out["turbulence_d3q19_512"] = extra_turb
     if extra_serve:
         out["serve_config5"] = {k: (float("%.5g" % v) if isinstance(v, float) else v) for k, v in extra_serve.items()}
+    def short(d):
+        return {k: (float("%.5g" % v) if isinstance(v, float) else v) for k, v in d.items()}
+
     if extra_amr:
-        out["amr_d3q27_vortex_rings_128"] = {k: (float("%.5g" % v) if isinstance(v, float) else v)
-                                             for k, v in extra_amr.items()}
+        out["amr_d3q27_vortex_rings_128"] = short(extra_amr)
+    if extra_amr_interp:
+        out["amr_d3q27_vortex_rings_128_interp_interface"] = {k: short(v) for k, v in extra_amr_interp.items()}
     print(json.dumps(out), flush=True)
 
 

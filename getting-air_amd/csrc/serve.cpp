@@ -217,12 +217,14 @@ void finish(lbm_serve* sv, const std::shared_ptr<Request>& r, int status, const 
 }
 
 // Tile shape of a lattice class: the dense autotuner run once on the class's
-// first request (its own grid), or the plain solve's default tile.  The class's
-// members have other widths, so its tiles are linear (blocks of consecutive
-// cells, no idle lanes at any width) with the tuned thread count, load
-// flavour and occupancy cap.
+// first request (its own grid) when that grid is large enough for its
+// timings to speak for the class's big members (2^20 cells; a small grid is
+// launch-bound and would pick by noise), else the plain solve's default
+// tile.  The class's members have other widths, so its tiles are linear
+// (blocks of consecutive cells, no idle lanes at any width) with the tuned
+// thread count, load flavour and occupancy cap.
 void choose_tile(lbm_serve* sv, Class* c, const Request& r) {
-  if (!sv->d.autotune) return;
+  if (!sv->d.autotune || r.cells < (1u << 20)) return;
   lbm_desc d = r.d;
   d.autotune = 1;
   lbm_solve* t = nullptr;

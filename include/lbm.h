@@ -251,7 +251,8 @@ typedef struct lbm_serve_desc {
   int32_t max_resident;  /* resident solves at most (0: limited by the budget)  */
   double budget_bytes;   /* device pool size: admission budget                 */
   int32_t chunk_steps;   /* steps per captured chunk (admission granularity)   */
-  int32_t autotune;      /* 1: tile shape per lattice class by timing          */
+  int32_t autotune;      /* 1: tile shape per lattice class by timing its first
+                            request, when that holds 2^20 cells or more       */
   double target_cells;   /* stop admitting once this many cells are resident
                             (enough to fill the device; 0: no limit)          */
 } lbm_serve_desc;
