@@ -232,7 +232,7 @@ def measure_turbulence(n=512, steps=2000, n_small=128, threads=16):
     return res
 
 
-def measure_serve(count, rate=50.0, seed=5, steps=2000, budget=200e9, scale=1.0):
+def measure_serve(count, rate=50.0, seed=5, steps=2000, budget=200e9, scale=1.0, baseline=True):
     """Config-5 serving: `count` requests with seeded Poisson arrivals at
     `rate`/s (2D cylinder channels nx 256-1024 and 3D boxed spheres 64^3-192^3,
     Re uniform in [50, 500], shifted fp16 storage), submitted at their arrival
@@ -273,6 +273,13 @@ def measure_serve(count, rate=50.0, seed=5, steps=2000, budget=200e9, scale=1.0)
                 s.read_forces(nsteps)
         return time.perf_counter() - t
 
+    served = {"requests": count, "arrival_rate_per_s": rate, "last_arrival_s": mix[-1][0],
+              "served_wall_s": wall, "served_solves_per_s": count / wall, "served_mlups": updates / wall / 1e6,
+              "p50_latency_s": st["p50_latency"], "p99_latency_s": st["p99_latency"],
+              "peak_resident": st["peak_resident"], "peak_pool_bytes": st["peak_resident_bytes"],
+              "chunks": st["chunks"], "cell_updates": updates, "finite": bool(finite), "budget_bytes": budget}
+    if not baseline:
+        return served
     seq = one_at_a_time("f16")
     seq32 = one_at_a_time("f32")
     return {"requests": count, "arrival_rate_per_s": rate, "last_arrival_s": mix[-1][0],
@@ -297,6 +304,12 @@ def main():
     ap.add_argument("--no-amr", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-serve-baseline", action="store_true",
+                    help="serve leg without the one-at-a-time comparison runs (for profiling the served path)")
+    ap.add_argument("--only-serve", action="store_true",
+                    help="run the serve leg alone and print its object (profiling; not the benchmark line)")
+    ap.add_argument("--no-amr-interp", action="store_true",
+                    help="AMR leg with the explode/coalesce interface only")
     ap.add_argument("--turbulence", action="store_true",
                     help="also run the 512^3 decaying-turbulence precision study (several minutes)")
     ap.add_argument("--tile", default=None,
@@ -309,6 +322,11 @@ def main():
         print("bench.py runs on exactly one GPU: the dense solve is single-device here "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.only_serve:
+        r = measure_serve(args.serve_requests, baseline=not args.no_serve_baseline)
+        print(json.dumps({"serve_config5": r}), flush=True)
+        return
 
     dev = pylbm.device_info()
     print("device: %s, %d CUs, %.0f GB" % (dev["name"], dev["compute_units"], dev["hbm_bytes"] / 1e9),
@@ -335,18 +353,19 @@ def main():
         r = measure_amr(args.amr_steps)
         extra_amr = r
         print("amr: %s" % json.dumps(r), file=sys.stderr, flush=True)
-        extra_amr_interp = {"dynamic": measure_amr(args.amr_steps, interface="interp", reflux=0)}
-        print("amr interp: %s" % json.dumps(extra_amr_interp["dynamic"]), file=sys.stderr, flush=True)
-        for rf in (0, 1):
-            r = measure_amr_slab(min(args.amr_steps, 500), rf)
-            extra_amr_interp["slab_reflux" if rf else "slab_no_reflux"] = r
-            print("amr interp slab reflux=%d: %s" % (rf, json.dumps(r)), file=sys.stderr, flush=True)
+        if not args.no_amr_interp:
+            extra_amr_interp = {"dynamic": measure_amr(args.amr_steps, interface="interp", reflux=0)}
+            print("amr interp: %s" % json.dumps(extra_amr_interp["dynamic"]), file=sys.stderr, flush=True)
+            for rf in (0, 1):
+                r = measure_amr_slab(min(args.amr_steps, 500), rf)
+                extra_amr_interp["slab_reflux" if rf else "slab_no_reflux"] = r
+                print("amr interp slab reflux=%d: %s" % (rf, json.dumps(r)), file=sys.stderr, flush=True)
     else:
         extra_amr = None
 
     extra_serve = None
     if not args.no_serve:
-        extra_serve = measure_serve(args.serve_requests)
+        extra_serve = measure_serve(args.serve_requests, baseline=not args.no_serve_baseline)
         print("serve: %s" % json.dumps(extra_serve), file=sys.stderr, flush=True)
 
     extra_turb = None
