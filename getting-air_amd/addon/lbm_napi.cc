@@ -197,6 +197,47 @@ napi_value UploadF(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// Bytes of a Uint8Array or an ArrayBuffer.
+bool bytes_of(napi_env env, napi_value v, void** data, size_t* len) {
+  bool is = false;
+  napi_is_arraybuffer(env, v, &is);
+  if (is) return napi_get_arraybuffer_info(env, v, data, len) == napi_ok;
+  return typed(env, v, napi_uint8_array, data, len);
+}
+
+// saveState(solve) -> ArrayBuffer (lbm_state_header + populations)
+napi_value SaveState(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  lbm_solve* s = handle(env, argv[0]);
+  size_t n = 0;
+  int rc = lbm_state_size(s, &n);
+  if (rc) return throw_lbm(env, rc);
+  void* data = nullptr;
+  napi_value ab;
+  NAPI_OK(napi_create_arraybuffer(env, n, &data, &ab));
+  rc = lbm_save_state(s, data, n);
+  if (rc) return throw_lbm(env, rc);
+  return ab;
+}
+
+// loadState(solve, ArrayBuffer | Uint8Array)
+napi_value LoadState(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* data;
+  size_t len;
+  if (argc < 2 || !bytes_of(env, argv[1], &data, &len)) {
+    napi_throw_type_error(env, nullptr, "loadState(solve, ArrayBuffer | Uint8Array)");
+    return nullptr;
+  }
+  int rc = lbm_load_state(handle(env, argv[0]), data, len);
+  if (rc) return throw_lbm(env, rc);
+  return nullptr;
+}
+
 // readInto(solve, TypedArray): fills a caller-sized array.
 template <int (*Fn)(lbm_solve*, float*)>
 napi_value ReadFloat(napi_env env, napi_callback_info info) {
@@ -657,6 +698,67 @@ napi_value ServeSubmit(napi_env env, napi_callback_info info) {
   return v;
 }
 
+// serveSubmitState(server, desc, ArrayBuffer | Uint8Array state, steps, keepState) -> ticket
+// serveSubmitState(server, desc, Float32Array mac, steps, keepState) -> ticket
+napi_value ServeSubmitState(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 5) {
+    napi_throw_type_error(env, nullptr, "serveSubmitState(server, desc, state | mac, steps, keepState)");
+    return nullptr;
+  }
+  lbm_desc d;
+  if (!parse_desc(env, argv[1], d)) return nullptr;
+  void* data;
+  size_t len;
+  const float* mac = nullptr;
+  const void* state = nullptr;
+  size_t state_bytes = 0;
+  if (typed(env, argv[2], napi_float32_array, &data, &len)) {
+    const size_t cells = (size_t)d.nx * d.ny * (d.q == 9 ? 1 : d.nz);
+    if (len < 4 * cells) {
+      napi_throw_range_error(env, nullptr, "mac must hold 4 floats per cell");
+      return nullptr;
+    }
+    mac = static_cast<const float*>(data);
+  } else if (bytes_of(env, argv[2], &data, &len)) {
+    state = data;
+    state_bytes = len;
+  } else {
+    napi_throw_type_error(env, nullptr, "state must be an ArrayBuffer or Uint8Array, mac a Float32Array");
+    return nullptr;
+  }
+  int32_t steps = 0;
+  napi_get_value_int32(env, argv[3], &steps);
+  bool keep = false;
+  napi_get_value_bool(env, argv[4], &keep);
+  int64_t t = 0;
+  int rc = lbm_serve_submit_state(serve_handle(env, argv[0]), &d, mac, state, state_bytes, steps, keep ? 1 : 0, &t);
+  if (rc) return throw_lbm(env, rc);
+  napi_value v;
+  NAPI_OK(napi_create_int64(env, t, &v));
+  return v;
+}
+
+// serveReadState(server, ticket) -> ArrayBuffer
+napi_value ServeReadState(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  lbm_serve* sv = serve_handle(env, argv[0]);
+  const int64_t t = ticket_of(env, argv[1]);
+  size_t n = 0;
+  int rc = lbm_serve_read_state(sv, t, nullptr, &n);
+  if (rc) return throw_lbm(env, rc);
+  void* data = nullptr;
+  napi_value ab;
+  NAPI_OK(napi_create_arraybuffer(env, n, &data, &ab));
+  rc = lbm_serve_read_state(sv, t, data, &n);
+  if (rc) return throw_lbm(env, rc);
+  return ab;
+}
+
 struct WaitWork {
   napi_async_work work;
   napi_deferred deferred;
@@ -823,6 +925,10 @@ napi_value Init(napi_env env, napi_value exports) {
       {"serveCreate", ServeCreate},
       {"serveDestroy", ServeDestroy},
       {"serveSubmit", ServeSubmit},
+      {"serveSubmitState", ServeSubmitState},
+      {"serveReadState", ServeReadState},
+      {"saveState", SaveState},
+      {"loadState", LoadState},
       {"serveWait", ServeWait},
       {"serveReadMacro", ServeReadMacro},
       {"serveReadForces", ServeReadForces},

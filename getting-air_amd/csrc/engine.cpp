@@ -108,6 +108,11 @@ struct lbm_solve {
   unsigned long long* written = nullptr;
   double* totals = nullptr;
   uint32_t* salt = nullptr;         // dither step counters, one per lattice parity
+  // the counter the next step reads is salt0 + (step - step_at_salt0): the
+  // kernels advance the device words by one per step, and an upload
+  // restarts them at 0 while a loaded state sets them
+  uint32_t salt0 = 0;
+  long long step_at_salt0 = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int bx = 64, by = 1, swz = 1, nt = 0;
@@ -592,6 +597,29 @@ int create_impl(const lbm_desc* desc, Arena* arena, hipStream_t stream, int bx, 
 
 namespace lbm {
 
+static_assert(sizeof(lbm_state_header) == 64, "lbm_state_header is 64 bytes");
+
+uint32_t solve_salt(const lbm_solve* s) { return s->salt0 + (uint32_t)(s->step - s->step_at_salt0); }
+
+int state_header_check(const lbm_desc* d, const void* buf, size_t bytes, lbm_state_header* out) {
+  if (!buf || bytes < sizeof(lbm_state_header)) return fail(LBM_E_INVALID, "state of %zu bytes has no header", bytes);
+  lbm_state_header h;
+  memcpy(&h, buf, sizeof h);
+  if (memcmp(h.magic, "LBMSTATE", 8) || h.version != 1) return fail(LBM_E_INVALID, "not a version-1 solve state");
+  const int nz = d->q == 9 ? 1 : d->nz;
+  if (h.q != d->q || h.nx != d->nx || h.ny != d->ny || h.nz != nz || h.storage != d->storage)
+    return fail(LBM_E_INVALID, "state of q%d %dx%dx%d storage %d does not match the solve (q%d %dx%dx%d storage %d)",
+                h.q, h.nx, h.ny, h.nz, h.storage, d->q, d->nx, d->ny, nz, d->storage);
+  const size_t elem = d->storage == LBM_STORE_F32 ? 4 : 2;
+  const size_t payload = (size_t)d->nx * d->ny * nz * d->q * elem;
+  if (h.payload_bytes != payload || bytes < sizeof h + payload)
+    return fail(LBM_E_INVALID, "state payload of %llu bytes (buffer %zu), the solve needs %zu",
+                (unsigned long long)h.payload_bytes, bytes, payload);
+  if (h.step < 0) return fail(LBM_E_INVALID, "state step %lld", (long long)h.step);
+  if (out) *out = h;
+  return LBM_OK;
+}
+
 size_t solve_footprint(const lbm_desc* d) {
   const size_t n = (size_t)d->nx * d->ny * (d->q == 9 ? 1 : d->nz);
   const size_t elem = d->storage == LBM_STORE_F32 ? 4 : 2;
@@ -694,8 +722,11 @@ int lbm_upload_macro(lbm_solve* s, const float* mac4) {
   HIPCHK(launch_init_eq(s->q, s->storage, s->mac, s->f[0], s->N, s->stream));
   // Both lattices hold the equilibrium (main-3d.js:2994 seeds A and B).
   HIPCHK(hipMemcpyAsync(s->f[1], s->f[0], (size_t)s->N * s->q * s->elem, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(hipMemsetAsync(s->salt, 0, 8, s->stream));
   s->parity = 0;
   s->step = 0;
+  s->salt0 = 0;
+  s->step_at_salt0 = 0;
   s->initialised = true;
   int rc = reset_history(s);
   if (rc) return rc;
@@ -711,12 +742,63 @@ int lbm_upload_f(lbm_solve* s, const float* f) {
   hipError_t e = hipMemcpyAsync(tmp, f, wire, hipMemcpyHostToDevice, s->stream);
   if (e == hipSuccess) e = launch_pack(s->q, s->storage, tmp, s->f[0], s->N, s->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(s->f[1], s->f[0], (size_t)s->N * s->q * s->elem, hipMemcpyDeviceToDevice, s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->salt, 0, 8, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return hip_fail(e, "upload_f");
   s->parity = 0;
   s->step = 0;
+  s->salt0 = 0;
+  s->step_at_salt0 = 0;
   s->initialised = true;
+  return reset_history(s);
+}
+
+int lbm_state_size(lbm_solve* s, size_t* bytes) {
+  if (!s || !bytes) return fail(LBM_E_INVALID, "null argument");
+  *bytes = sizeof(lbm_state_header) + (size_t)s->N * s->q * s->elem;
+  return LBM_OK;
+}
+
+int lbm_save_state(lbm_solve* s, void* buf, size_t bytes) {
+  if (!s || !buf) return fail(LBM_E_INVALID, "null argument");
+  if (!s->initialised) return fail(LBM_E_STATE, "save_state before upload_macro / upload_f / load_state");
+  const size_t payload = (size_t)s->N * s->q * s->elem;
+  if (bytes < sizeof(lbm_state_header) + payload)
+    return fail(LBM_E_INVALID, "state buffer of %zu bytes, need %zu", bytes, sizeof(lbm_state_header) + payload);
+  HIPCHK(hipMemcpyAsync(static_cast<char*>(buf) + sizeof(lbm_state_header), s->f[s->parity], payload,
+                        hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  lbm_state_header h{};
+  memcpy(h.magic, "LBMSTATE", 8);
+  h.version = 1;
+  h.q = s->q;
+  h.nx = s->d.nx; h.ny = s->d.ny; h.nz = s->d.nz;
+  h.storage = s->storage;
+  h.salt = solve_salt(s);
+  h.step = s->step;
+  h.payload_bytes = payload;
+  memcpy(buf, &h, sizeof h);
+  return LBM_OK;
+}
+
+int lbm_load_state(lbm_solve* s, const void* buf, size_t bytes) {
+  if (!s || !buf) return fail(LBM_E_INVALID, "null argument");
+  lbm_state_header h{};
+  int rc = state_header_check(&s->d, buf, bytes, &h);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(s->f[0], static_cast<const char*>(buf) + sizeof(lbm_state_header), h.payload_bytes,
+                        hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->f[1], s->f[0], h.payload_bytes, hipMemcpyDeviceToDevice, s->stream));
+  const uint32_t words[2] = {h.salt, h.salt};
+  HIPCHK(hipMemcpyAsync(s->salt, words, 8, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->parity = 0;
+  s->step = h.step;
+  s->salt0 = h.salt;
+  s->step_at_salt0 = h.step;
+  s->initialised = true;
+  s->force_pending = false;
   return reset_history(s);
 }
 

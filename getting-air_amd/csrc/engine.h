@@ -46,4 +46,9 @@ int solve_batch_entry(lbm_solve* s, BatchSolve* e, uint32_t* ntiles);
 // and the force history of the last step summed (enqueued on the stream).
 int solve_advance(lbm_solve* s, int steps, bool finalize_force);
 uint32_t solve_cells(const lbm_solve* s);
+// Dither counter of the solve's next step.
+uint32_t solve_salt(const lbm_solve* s);
+// Validates a saved state against a descriptor (magic, version, q, grid,
+// storage, payload size) and returns its header.
+int state_header_check(const lbm_desc* d, const void* buf, size_t bytes, lbm_state_header* out);
 }  // namespace lbm

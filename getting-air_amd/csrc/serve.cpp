@@ -126,6 +126,9 @@ struct Request {
   bool done = false;
   std::vector<float> mac_out;
   std::vector<double> forces;  // 6 per recorded step
+  std::vector<uint8_t> state_in;   // start from this saved state instead of mac_in
+  bool keep_state = false;
+  std::vector<uint8_t> state_out;  // the final state (keep_state)
 };
 
 struct Member {
@@ -280,7 +283,9 @@ void admit(lbm_serve* sv) {
       sv->queue.push_front(r);
       return;
     }
-    if (!rc) rc = lbm_upload_macro(m->s, r->mac_in.data());
+    if (!rc)
+      rc = r->state_in.empty() ? lbm_upload_macro(m->s, r->mac_in.data())
+                               : lbm_load_state(m->s, r->state_in.data(), r->state_in.size());
     if (!rc) rc = solve_batch_entry(m->s, &m->entry, &m->ntiles);
     if (rc) {
       const std::string err = lbm_last_error();
@@ -290,6 +295,8 @@ void admit(lbm_serve* sv) {
     }
     r->mac_in.clear();
     r->mac_in.shrink_to_fit();
+    r->state_in.clear();
+    r->state_in.shrink_to_fit();
     std::lock_guard<std::mutex> g(sv->mu);
     r->t_admit = now_s(sv->t0);
     sv->resident_bytes += r->footprint;
@@ -343,7 +350,7 @@ int prepare(lbm_serve* sv, bool* rebuild) {
       e.parity = m->done_steps & 1;
       e.remaining = m->r->steps - m->done_steps;
       e.pending = m->pending ? 1 : 0;
-      e.step0 = (uint32_t)m->done_steps;
+      e.step0 = solve_salt(m->s);  // the dither counter of the solve's next step
       c->h_solves[k] = e;
     }
     if (!c->h_solves.empty())
@@ -455,6 +462,14 @@ void retire(lbm_serve* sv) {
     if (!rc) {
       r.mac_out.resize((size_t)r.cells * 4);
       rc = lbm_read_macro(m.s, r.mac_out.data());
+    }
+    if (!rc && r.keep_state) {
+      size_t n = 0;
+      rc = lbm_state_size(m.s, &n);
+      if (!rc) {
+        r.state_out.resize(n);
+        rc = lbm_save_state(m.s, r.state_out.data(), n);
+      }
     }
     if (!rc && m.entry.args[0].force_slot) {
       const int maxr = std::min(r.steps, 1 << 16);
@@ -586,7 +601,14 @@ int lbm_serve_destroy(lbm_serve* sv) {
 }
 
 int lbm_serve_submit(lbm_serve* sv, const lbm_desc* desc, const float* mac4, int32_t steps, int64_t* ticket) {
-  if (!sv || !desc || !mac4 || !ticket) return fail(LBM_E_INVALID, "null argument");
+  if (!mac4) return fail(LBM_E_INVALID, "null argument");
+  return lbm_serve_submit_state(sv, desc, mac4, nullptr, 0, steps, 0, ticket);
+}
+
+int lbm_serve_submit_state(lbm_serve* sv, const lbm_desc* desc, const float* mac4, const void* state,
+                           size_t state_bytes, int32_t steps, int32_t keep_state, int64_t* ticket) {
+  if (!sv || !desc || !ticket) return fail(LBM_E_INVALID, "null argument");
+  if ((mac4 == nullptr) == (state == nullptr)) return fail(LBM_E_INVALID, "give exactly one of mac4 and state");
   if (steps < 1) return fail(LBM_E_INVALID, "steps %d", steps);
   // validate the descriptor now (lbm_solve_create would on the worker)
   lbm_desc d = *desc;
@@ -602,7 +624,15 @@ int lbm_serve_submit(lbm_serve* sv, const lbm_desc* desc, const float* mac4, int
   if (r->footprint > (size_t)sv->d.budget_bytes)
     return fail(LBM_E_NOMEM, "request needs %zu bytes, more than the whole budget (%.0f)", r->footprint,
                 sv->d.budget_bytes);
-  r->mac_in.assign(mac4, mac4 + (size_t)r->cells * 4);
+  if (state) {
+    int rc = state_header_check(&d, state, state_bytes, nullptr);
+    if (rc) return rc;
+    const uint8_t* b = static_cast<const uint8_t*>(state);
+    r->state_in.assign(b, b + state_bytes);
+  } else {
+    r->mac_in.assign(mac4, mac4 + (size_t)r->cells * 4);
+  }
+  r->keep_state = keep_state != 0;
   std::lock_guard<std::mutex> g(sv->mu);
   r->ticket = sv->next_ticket++;
   r->t_submit = now_s(sv->t0);
@@ -659,6 +689,22 @@ int lbm_serve_read_forces(lbm_serve* sv, int64_t ticket, double* out6, int32_t m
   const int32_t n = std::min<int32_t>((int32_t)(r->forces.size() / 6), std::max(max_records, 0));
   std::copy(r->forces.end() - (size_t)n * 6, r->forces.end(), out6);
   *count = n;
+  return LBM_OK;
+}
+
+int lbm_serve_read_state(lbm_serve* sv, int64_t ticket, void* buf, size_t* size) {
+  if (!sv || !size) return fail(LBM_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(sv->mu);
+  auto r = find(sv, ticket);
+  if (!r) return fail(LBM_E_INVALID, "unknown ticket %lld", (long long)ticket);
+  if (!r->done || r->status) return fail(LBM_E_STATE, "request %lld has no result", (long long)ticket);
+  if (!r->keep_state) return fail(LBM_E_STATE, "request %lld was not submitted with keep_state", (long long)ticket);
+  if (buf) {
+    if (*size < r->state_out.size())
+      return fail(LBM_E_INVALID, "state buffer of %zu bytes, need %zu", *size, r->state_out.size());
+    std::copy(r->state_out.begin(), r->state_out.end(), static_cast<uint8_t*>(buf));
+  }
+  *size = r->state_out.size();
   return LBM_OK;
 }
 

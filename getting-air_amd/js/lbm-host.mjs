@@ -191,6 +191,16 @@ export class DenseSolver {
     return out;
   }
 
+  // Exact checkpoint (lbm_state_header + populations in the storage format):
+  // restore() and stepping on gives the uninterrupted run's bits, which the
+  // f32 wire format of readF / uploadF does not for fp32 storage.
+  async saveState() { return this.addon.saveState(this.handle); }
+
+  async loadState(state) {
+    this.addon.loadState(this.handle, state);
+    this.step = Number(new DataView(state.buffer || state, state.byteOffset || 0).getBigInt64(40, true));
+  }
+
   // Kinetic energy, enstrophy (central-difference vorticity), extrema and the
   // extensive budget (main-3d.js:3610-3650).  mass/px/py/pz come from the
   // FP64 device reduction over the populations.
@@ -342,11 +352,18 @@ export class ServeClient {
     if (this.handle) { this.addon.serveDestroy(this.handle); this.handle = null; }
   }
 
-  // scenario: {desc, macro()} as sphereScenario / tgvScenario build them
-  submit(scenario, steps) {
+  // scenario: {desc, macro()} as sphereScenario / tgvScenario build them.
+  // opts.state: start from a saved state (DenseSolver.saveState or
+  // result().state) instead of the scenario's field; opts.keepState: the
+  // result carries the final state, so a later request can resume the solve.
+  submit(scenario, steps, opts) {
     const d = scenario.desc;
-    const t = this.addon.serveSubmit(this.handle, d, scenario.macro(), steps | 0);
+    const o = opts || {};
+    const t = (o.state || o.keepState)
+      ? this.addon.serveSubmitState(this.handle, d, o.state || scenario.macro(), steps | 0, !!o.keepState)
+      : this.addon.serveSubmit(this.handle, d, scenario.macro(), steps | 0);
     this.cells.set(t, d.nx * d.ny * (d.q === 9 ? 1 : d.nz));
+    if (o.keepState) this.keep = (this.keep || new Set()).add(t);
     return t;
   }
 
@@ -355,6 +372,7 @@ export class ServeClient {
     const r = await this.addon.serveWait(this.handle, ticket, timeoutSeconds === undefined ? -1 : timeoutSeconds);
     r.macro = this.addon.serveReadMacro(this.handle, ticket, new Float32Array(4 * this.cells.get(ticket)));
     r.forces = this.addon.serveReadForces(this.handle, ticket, r.steps);
+    if (this.keep && this.keep.has(ticket)) r.state = this.addon.serveReadState(this.handle, ticket);
     return r;
   }
 

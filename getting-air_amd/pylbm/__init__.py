@@ -22,13 +22,14 @@ EXPORTS = (
     "lbm_solve_destroy", "lbm_upload_macro", "lbm_upload_f", "lbm_read_f", "lbm_step",
     "lbm_step_timed", "lbm_sync", "lbm_read_macro", "lbm_read_forces", "lbm_read_totals",
     "lbm_read_classes", "lbm_solve_info", "lbm_set_tile", "lbm_time_step_kernel",
-    "lbm_probe_copy_bandwidth",
+    "lbm_probe_copy_bandwidth", "lbm_state_size", "lbm_save_state", "lbm_load_state",
     "lbm_amr_create", "lbm_amr_destroy", "lbm_amr_upload_topology", "lbm_amr_read_topology",
     "lbm_amr_upload_macro", "lbm_amr_step", "lbm_amr_manage_now", "lbm_amr_read_macro",
     "lbm_amr_read_pool_macro", "lbm_amr_read_f", "lbm_amr_read_criterion", "lbm_amr_read_active",
     "lbm_amr_read_totals", "lbm_amr_info",
     "lbm_serve_create", "lbm_serve_destroy", "lbm_serve_submit", "lbm_serve_wait", "lbm_serve_read_macro",
     "lbm_serve_read_forces", "lbm_serve_release", "lbm_serve_drain", "lbm_serve_read_stats", "lbm_serve_clock",
+    "lbm_serve_submit_state", "lbm_serve_read_state",
 )
 GHOST = 2
 
@@ -115,6 +116,9 @@ def load():
                                        ctypes.POINTER(ctypes.c_int64)]
         lib.lbm_set_tile.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         lib.lbm_time_step_kernel.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+        lib.lbm_state_size.argtypes = [P, ctypes.POINTER(ctypes.c_size_t)]
+        lib.lbm_save_state.argtypes = [P, P, ctypes.c_size_t]
+        lib.lbm_load_state.argtypes = [P, P, ctypes.c_size_t]
         lib.lbm_probe_copy_bandwidth.argtypes = [ctypes.c_size_t, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                                  ctypes.POINTER(ctypes.c_double)]
         lib.lbm_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double),
@@ -144,6 +148,9 @@ def load():
         lib.lbm_serve_release.argtypes = [P, ctypes.c_int64]
         lib.lbm_serve_drain.argtypes = [P, ctypes.c_double]
         lib.lbm_serve_read_stats.argtypes = [P, ctypes.POINTER(ServeStats)]
+        lib.lbm_serve_submit_state.argtypes = [P, ctypes.POINTER(Desc), P, P, ctypes.c_size_t, ctypes.c_int32,
+                                                ctypes.c_int32, i64p]
+        lib.lbm_serve_read_state.argtypes = [P, ctypes.c_int64, P, ctypes.POINTER(ctypes.c_size_t)]
         lib.lbm_serve_clock.argtypes = [P]
         lib.lbm_serve_clock.restype = ctypes.c_double
         _lib = lib
@@ -298,6 +305,18 @@ class Solve:
                                        ctypes.byref(flags), ctypes.byref(nbytes)))
         return {"step": step.value, "block_x": bx.value, "block_y": by.value, "flags": flags.value,
                 "device_bytes": nbytes.value}
+
+    def save_state(self):
+        """Exact checkpoint (lbm_state_header + populations in the storage format), bytes."""
+        size = ctypes.c_size_t()
+        _check(self.lib.lbm_state_size(self.h, ctypes.byref(size)))
+        out = np.empty(size.value, dtype=np.uint8)
+        _check(self.lib.lbm_save_state(self.h, out.ctypes.data, out.size))
+        return out.tobytes()
+
+    def load_state(self, state):
+        b = np.frombuffer(bytes(state), dtype=np.uint8)
+        _check(self.lib.lbm_load_state(self.h, b.ctypes.data, b.size))
 
     def set_tile(self, bx, by, flags):
         """flags: bit 0 XCD swizzle, bit 1 non-temporal, bit 2 linear tiles, bits 4-7 waves-per-SIMD cap."""
@@ -471,16 +490,32 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def submit(self, desc_dict, mac4, steps):
+    def submit(self, desc_dict, mac4, steps, state=None, keep_state=False):
+        """Queue a solve from [rho, u] per cell (mac4) or from a saved state
+        (bytes from Solve.save_state / read_state; mac4 None)."""
         desc = make_desc(desc_dict)
         n = desc.nx * desc.ny * desc.nz
-        m = np.ascontiguousarray(mac4, dtype=np.float32).reshape(-1)
-        if m.size != 4 * n:
-            raise ValueError("mac needs %d floats, got %d" % (4 * n, m.size))
         t = ctypes.c_int64()
-        _check(self.lib.lbm_serve_submit(self.h, ctypes.byref(desc), m.ctypes.data, int(steps), ctypes.byref(t)))
+        if state is None:
+            m = np.ascontiguousarray(mac4, dtype=np.float32).reshape(-1)
+            if m.size != 4 * n:
+                raise ValueError("mac needs %d floats, got %d" % (4 * n, m.size))
+            _check(self.lib.lbm_serve_submit_state(self.h, ctypes.byref(desc), m.ctypes.data, None, 0, int(steps),
+                                                   int(bool(keep_state)), ctypes.byref(t)))
+        else:
+            b = np.frombuffer(bytes(state), dtype=np.uint8)
+            _check(self.lib.lbm_serve_submit_state(self.h, ctypes.byref(desc), None, b.ctypes.data, b.size,
+                                                   int(steps), int(bool(keep_state)), ctypes.byref(t)))
         self.cells[t.value] = n
         return t.value
+
+    def read_state(self, ticket):
+        """Final state of a request submitted with keep_state (bytes)."""
+        size = ctypes.c_size_t()
+        _check(self.lib.lbm_serve_read_state(self.h, int(ticket), None, ctypes.byref(size)))
+        out = np.empty(size.value, dtype=np.uint8)
+        _check(self.lib.lbm_serve_read_state(self.h, int(ticket), out.ctypes.data, ctypes.byref(size)))
+        return out.tobytes()
 
     def wait(self, ticket, timeout=-1.0):
         r = ServeResult()

@@ -96,6 +96,37 @@ int lbm_upload_macro(lbm_solve* s, const float* mac4);
 int lbm_upload_f(lbm_solve* s, const float* f);
 int lbm_read_f(lbm_solve* s, float* f);
 
+/* Exact checkpoint of a solve: this header, then the current lattice's
+ * populations in the solve's own storage format (shifted f - w, f32 / f16 /
+ * bf16 bits), plane-major.  The wire format above is an interchange format:
+ * for fp32 storage its f = g + w rounds away low bits of g, so a run resumed
+ * from it drifts from the uninterrupted one.  A run resumed from this state
+ * with the same descriptor gives the fields and force records of the
+ * uninterrupted run bit for bit (the 16-bit dither counter travels with it).
+ * The reference's snapshots are the wire format (f-pack.mjs:6-14,
+ * debugSnapshotSave main-amr.js:2017); this exactness has no counterpart
+ * there.  All integers little-endian. */
+typedef struct lbm_state_header {
+  char magic[8];         /* "LBMSTATE"                                          */
+  uint32_t version;      /* 1                                                   */
+  int32_t q, nx, ny, nz; /* lattice and grid                                    */
+  int32_t storage;       /* LBM_STORE_*                                         */
+  uint32_t salt;         /* dither counter of the next step                     */
+  int64_t step;          /* steps done                                          */
+  uint64_t payload_bytes;/* q * ncells * element bytes that follow              */
+  uint8_t reserved[8];
+} lbm_state_header;
+
+/* Bytes of this solve's state (header + populations). */
+int lbm_state_size(lbm_solve* s, size_t* bytes);
+/* Writes the state after the work enqueued so far; buf holds >= state_size. */
+int lbm_save_state(lbm_solve* s, void* buf, size_t bytes);
+/* Replaces the populations, step count and dither counter by a state saved
+ * from a solve of the same q, grid and storage (LBM_E_INVALID otherwise).
+ * The force history restarts; the macroscopic field is written by the next
+ * step. */
+int lbm_load_state(lbm_solve* s, const void* buf, size_t bytes);
+
 /* Advances n steps (force reduction then fused stream-collide per step) and
  * leaves the macroscopic field of the last step readable.  Replaces
  * encodeSteps / debugStepSync, main-3d.js:3122-3171 and :3506,
@@ -288,6 +319,16 @@ int lbm_serve_destroy(lbm_serve* sv);
  * per cell (copied), steps to run.  Refuses (LBM_E_NOMEM) a request larger
  * than the whole budget. */
 int lbm_serve_submit(lbm_serve* sv, const lbm_desc* desc, const float* mac4, int32_t steps, int64_t* ticket);
+/* Resumable requests: exactly one of mac4 (a fresh start, as above) and
+ * state (a state from lbm_save_state or lbm_serve_read_state for the same
+ * descriptor; copied) gives the starting point; keep_state 1 keeps the final
+ * state for lbm_serve_read_state.  A solve split into requests that hand
+ * their states on gives the uninterrupted solve's bits. */
+int lbm_serve_submit_state(lbm_serve* sv, const lbm_desc* desc, const float* mac4, const void* state,
+                           size_t state_bytes, int32_t steps, int32_t keep_state, int64_t* ticket);
+/* The final state of a finished request submitted with keep_state: with buf
+ * null only *size is set; else buf must hold *size bytes. */
+int lbm_serve_read_state(lbm_serve* sv, int64_t ticket, void* buf, size_t* size);
 /* Waits up to timeout_s (< 0: forever) for a request; LBM_E_STATE on timeout. */
 int lbm_serve_wait(lbm_serve* sv, int64_t ticket, double timeout_s, lbm_serve_result* out);
 /* Results of a finished request: final macroscopic field, force history. */

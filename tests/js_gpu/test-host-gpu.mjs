@@ -170,6 +170,34 @@ function relErr(gpuMac, oracleMac, comp) {
     return 'p50 ' + st.p50Latency.toFixed(3) + ' s; ' + report;
   });
 
+  await ok('checkpoint: saveState/loadState and a ServeClient request resumed from it give the straight run bits', async () => {
+    const sc = sphereScenario({ R: 4, dims: [48, 24, 24], centre: [14, 12, 12], re: 80, trt: true, storage: 'f16' });
+    const a = new DenseSolver(sc);
+    await a.reset();
+    await a.debugStepSync(50);
+    const ref = await a.readMacro();
+    a.destroy();
+    const b = new DenseSolver(sc);
+    await b.reset();
+    await b.debugStepSync(20);
+    const st = await b.saveState();
+    b.destroy();
+    const c = new DenseSolver(sc);
+    await c.loadState(st);
+    assert(c.getStep() === 20, 'step after loadState ' + c.getStep());
+    await c.debugStepSync(30);
+    const m1 = await c.readMacro();
+    c.destroy();
+    const sv = new ServeClient({ budgetBytes: 1 << 30, chunkSteps: 8 });
+    const r = await sv.result(sv.submit(sc, 30, { state: st, keepState: true }), 120);
+    sv.destroy();
+    const diff = (x, y) => { let n = 0; for (let k = 0; k < x.length; k++) if (!Object.is(x[k], y[k])) n++; return n; };
+    const d1 = diff(m1, ref), d2 = diff(r.macro, ref);
+    assert(d1 === 0 && d2 === 0, 'differing values: resumed solver ' + d1 + ', resumed request ' + d2);
+    assert(r.state && r.state.byteLength === st.byteLength, 'served request kept its state');
+    return 'state ' + st.byteLength + ' bytes, 0 differing values';
+  });
+
   await ok('invalid descriptor raises with the library message', () => {
     let msg = '';
     try { new DenseSolver({ desc: { q: 19, nx: 4, ny: 4, nz: 4, omega: 3 }, params: {}, macro: null }); } catch (e) { msg = e.message; }

@@ -145,3 +145,25 @@ def test_serve_refusals_without_gpu():
     d.budget_bytes = 1000.0
     assert lib.lbm_serve_create(ctypes.byref(d), ctypes.byref(h)) == -1
     assert b"budget" in lib.lbm_last_error()
+
+
+def test_state_header_layout():
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "lbm.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(lbm_state_header), offsetof(lbm_state_header, version),
+         offsetof(lbm_state_header, storage), offsetof(lbm_state_header, salt), offsetof(lbm_state_header, step),
+         offsetof(lbm_state_header, payload_bytes));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "t")
+        subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe])
+        got = [int(v) for v in subprocess.check_output([exe]).split()]
+    # the layout tests/test_state_gpu.py reads with struct "<8sIiiiiiI4xqQ"
+    assert got == [64, 8, 28, 32, 40, 48]
