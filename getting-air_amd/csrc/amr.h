@@ -3,14 +3,16 @@
 // Two levels: the dense L0 lattice and one pool of fine tiles.  L0 is cut
 // into blocks of RB^3 coarse cells; a refined block owns a slot holding
 // FB^3 fine cells, FB = 2 RB + 2 GHOST (interior [GHOST, GHOST + 2 RB) plus a
-// two-cell ring).  Pool planes are plane-major like the dense lattice:
+// two-cell ring).  The pool is plane-major:
 // g[i * pool_plane + (slot * FB + fz) * FB * FB + fy * FB + fx], with
 // pool_plane = max_slots * FB^3, holding the shifted populations g = f - w.
+// L0 is a dense solve's lattice in its block layout (dense.h lat_elem).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "collide.h"
+#include "dense.h"
 
 namespace lbm {
 

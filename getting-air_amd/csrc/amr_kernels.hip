@@ -273,7 +273,6 @@ k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, cons
   const int gz = wrap1(bz * RB2 + fz - kGhost, NFz);
   const int vx = gx >> 1, vy = gy >> 1, vz = gz >> 1;
   const uint32_t vc = coarse_cell(G, vx, vy, vz);
-  const uint32_t N = G.ncells;
   const int xm = wrap1(vx - 1, G.nx), xp = wrap1(vx + 1, G.nx);
   const int ym = wrap1(vy - 1, G.ny), yp = wrap1(vy + 1, G.ny);
   const int zm = wrap1(vz - 1, G.nz), zp = wrap1(vz + 1, G.nz);
@@ -290,7 +289,7 @@ k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, cons
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
-    const float* plane = gc + (size_t)i * N;
+    auto l0 = [&](uint32_t c) { return gc[lat_elem(Q, i, c)]; };  // population i of L0 cell c
     float out;
     if (!covered_at(G, vx + ex, vy + ey, vz + ez)) {
       // The coarse target is unrefined: only a population that streams from
@@ -299,21 +298,21 @@ k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, cons
       const int dxg = wrap1(gx + ex, NFx), dyg = wrap1(gy + ey, NFy), dzg = wrap1(gz + ez, NFz);
       const bool into = covered_at(G, dxg >> 1, dyg >> 1, dzg >> 1) && (dxg >> (T::LOG_RB + 1)) == bx &&
                         (dyg >> (T::LOG_RB + 1)) == by && (dzg >> (T::LOG_RB + 1)) == bz;
-      out = into ? *at(plane, vc) : -L::w[i];
+      out = into ? l0(vc) : -L::w[i];
     } else if (!linear) {
-      out = *at(plane, vc);
+      out = l0(vc);
     } else {
       // linear reconstruction d . G with the gradient projected
       // perpendicular to e_i (common_d3_amr_explode.wgsl:197-212)
-      float Gx = okx ? 0.5f * (*at(plane, cpx) - *at(plane, cmx)) : 0.f;
-      float Gy = oky ? 0.5f * (*at(plane, cpy) - *at(plane, cmy)) : 0.f;
-      float Gz = okz ? 0.5f * (*at(plane, cpz) - *at(plane, cmz)) : 0.f;
+      float Gx = okx ? 0.5f * (l0(cpx) - l0(cmx)) : 0.f;
+      float Gy = oky ? 0.5f * (l0(cpy) - l0(cmy)) : 0.f;
+      float Gz = okz ? 0.5f * (l0(cpz) - l0(cmz)) : 0.f;
       const float ee = (float)(ex * ex + ey * ey + ez * ez);
       const float proj = (Gx * ex + Gy * ey + Gz * ez) / ee;
       Gx -= ex * proj;
       Gy -= ey * proj;
       Gz -= ez * proj;
-      out = *at(plane, vc) + (dx * Gx + dy * Gy + dz * Gz);
+      out = l0(vc) + (dx * Gx + dy * Gy + dz * Gz);
     }
     *at(gp + (size_t)i * G.pool_plane, cell) = out;
   }
@@ -385,7 +384,7 @@ k_coalesce_seam(const float* __restrict__ gp, float* __restrict__ gc, PoolGeom G
   float s = 0.f;
 #pragma unroll
   for (int k = 0; k < 8; k++) s += *at(p, b0 + (k & 1) + ((k >> 1) & 1) * FB + ((k >> 2) & 1) * FB * FB);
-  *at(gc + (size_t)i * G.ncells, c) = s * 0.125f;  // an average of g is the average of f minus w
+  gc[lat_elem(Q, i, c)] = s * 0.125f;  // an average of g is the average of f minus w
 }
 
 // Orphans on unrefined L0 cells next to the refined region
@@ -406,7 +405,6 @@ k_orphans(const float* __restrict__ gp, float* __restrict__ gc, PoolGeom G, cons
   const uint32_t c = near[t];
   const int cx = c % G.nx, cy = (c / G.nx) % G.ny, cz = c / (G.nx * G.ny);
   const int NFx = 2 * G.nx, NFy = 2 * G.ny, NFz = 2 * G.nz;
-  const uint32_t N = G.ncells;
   const int tx = wrap1(cx + ex, G.nx), ty = wrap1(cy + ey, G.ny), tz = wrap1(cz + ez, G.nz);
   if (G.ccls[coarse_cell(G, tx, ty, tz)] & kCovered) return;
   float s = 0.f;   // sum of f over the children found in refined tiles
@@ -426,20 +424,21 @@ k_orphans(const float* __restrict__ gp, float* __restrict__ gc, PoolGeom G, cons
     const int hx = wrap1(2 * cx + ox + ex, NFx), hy = wrap1(2 * cy + oy + ey, NFy), hz = wrap1(2 * cz + oz + ez, NFz);
     if (G.ccls[coarse_cell(G, hx >> 1, hy >> 1, hz >> 1)] & kCovered) nIn += 1.f;
   }
-  const float fc = *at(gc + (size_t)i * N, c) + wi;
+  float& gci = gc[lat_elem(Q, i, c)];
+  const float fc = gci + wi;
   const float net = s - nIn * fc;
-  if (net != 0.f) *at(gc + (size_t)i * N, c) = (fc + net * 0.125f) - wi;
+  if (net != 0.f) gci = (fc + net * 0.125f) - wi;
 }
 
 // ---------------------------------------------------------------------------
 // interp fill of newly refined tiles, and average/drain of dying ones
 // ---------------------------------------------------------------------------
 template <int Q>
-__device__ __forceinline__ void coarse_sample(const float* __restrict__ gc, uint32_t N, uint32_t cell, float& rho,
+__device__ __forceinline__ void coarse_sample(const float* __restrict__ gc, uint32_t cell, float& rho,
                                               float& ux, float& uy, float& uz, float (&fneq)[Q]) {
   float g[Q];
 #pragma unroll
-  for (int i = 0; i < Q; i++) g[i] = gc[(size_t)i * N + cell];
+  for (int i = 0; i < Q; i++) g[i] = gc[lat_elem(Q, i, cell)];
   const float F0[3] = {0.f, 0.f, 0.f};
   float drho;
   moments<Q>(g, F0, drho, rho, ux, uy, uz);
@@ -477,7 +476,7 @@ k_interp(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const
     const float w = (i ? tx : 1.f - tx) * (j ? ty : 1.f - ty) * (kk ? tz : 1.f - tz);
     const uint32_t cell = coarse_cell(G, wrap1(c0x + i, G.nx), wrap1(c0y + j, G.ny), wrap1(c0z + kk, G.nz));
     float sr, sx, sy, sz, sf[Q];
-    coarse_sample<Q>(gc, G.ncells, cell, sr, sx, sy, sz, sf);
+    coarse_sample<Q>(gc, cell, sr, sx, sy, sz, sf);
     rho += w * sr;
     ux += w * sx;
     uy += w * sy;
@@ -526,7 +525,7 @@ k_average(const float* __restrict__ gp, float* __restrict__ gc, float* __restric
   const float ux = mx * inv, uy = my * inv, uz = mz * inv;
   const uint32_t cc = coarse_cell(G, bx * RB + lx, by * RB + ly, bz * RB + lz);
 #pragma unroll
-  for (int q = 0; q < Q; q++) gc[(size_t)q * G.ncells + cc] = geq<Q>(q, rho, ux, uy, uz) + rescale * (sneq[q] * 0.125f);
+  for (int q = 0; q < Q; q++) gc[lat_elem(Q, q, cc)] = geq<Q>(q, rho, ux, uy, uz) + rescale * (sneq[q] * 0.125f);
   reinterpret_cast<float4*>(mac)[cc] = make_float4(rho, ux, uy, uz);
 }
 
@@ -604,7 +603,6 @@ k_reflux(const float* __restrict__ gct, float* __restrict__ gct1, float4* __rest
   if (t >= n) return;
   const uint32_t c = cells[t];
   const int cx = c % G.nx, cy = (c / G.nx) % G.ny, cz = c / (G.nx * G.ny);
-  const uint32_t N = G.ncells;
   const float4 a = acc[t];
   float dm = a.x, px = a.y, py = a.z, pz = a.w;
 #pragma unroll
@@ -613,7 +611,7 @@ k_reflux(const float* __restrict__ gct, float* __restrict__ gct1, float4* __rest
     // coarse_out: what this cell pulled out of the refined region
     const uint32_t s = coarse_cell(G, wrap1(cx - ex, G.nx), wrap1(cy - ey, G.ny), wrap1(cz - ez, G.nz));
     if (G.ccls[s] & kCovered) {
-      const float v = *at(gct + (size_t)i * N, s) + L::w[i];
+      const float v = gct[lat_elem(Q, i, s)] + L::w[i];
       dm -= v;
       px -= v * ex;
       py -= v * ey;
@@ -621,7 +619,7 @@ k_reflux(const float* __restrict__ gct, float* __restrict__ gct1, float4* __rest
     }
     // coarse_in: what the refined region pulled out of this cell
     if (covered_at(G, cx + ex, cy + ey, cz + ez)) {
-      const float v = *at(gct + (size_t)i * N, c) + L::w[i];
+      const float v = gct[lat_elem(Q, i, c)] + L::w[i];
       dm += v;
       px += v * ex;
       py += v * ey;
@@ -633,7 +631,7 @@ k_reflux(const float* __restrict__ gct, float* __restrict__ gct1, float4* __rest
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
-    float* gi = at(gct1 + (size_t)i * N, c);
+    float* gi = gct1 + lat_elem(Q, i, c);
     const float g = *gi + L::w[i] * (dm + 3.f * (ex * px + ey * py + ez * pz));
     *gi = g;
     const float f = g + L::w[i];
@@ -812,7 +810,7 @@ k_pool_totals(const float* __restrict__ gc, const float* __restrict__ gp, PoolGe
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < G.ncells; c += stride) {
     if (G.ccls[c] & kCovered) continue;
     for (int i = 0; i < Q; i++) {
-      const double f = (double)gc[(size_t)i * G.ncells + c] + (double)L::w[i];
+      const double f = (double)gc[lat_elem(Q, i, c)] + (double)L::w[i];
       v[0] += f; v[1] += L::ex[i] * f; v[2] += L::ey[i] * f; v[3] += L::ez[i] * f;
     }
   }

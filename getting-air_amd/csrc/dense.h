@@ -26,6 +26,24 @@ __host__ __device__ inline uint32_t div_invariant(uint32_t n, InvariantDiv v) {
   return (uint32_t)(((uint64_t)n * v.m) >> v.shift);
 }
 
+// Lattice layout.  Cells (flat index c = (z ny + y) nx + x) are grouped into
+// blocks of kLatBlock consecutive cells, and a block stores its Q populations
+// as Q runs of kLatBlock elements: population i of cell c lies at element
+//   ((c / kLatBlock) Q + i) kLatBlock + c % kLatBlock.
+// A step's 2Q streams then walk a few contiguous regions of HBM (the blocks
+// of the source rows and of the own row) instead of 2Q planes a lattice-size
+// apart, whose concurrent accesses crowd the same DRAM channels and banks.
+// Every access of a wave stays one contiguous 256 B (f32) or 128 B (16-bit)
+// span, as in a plane-major layout.
+constexpr uint32_t kLatBlock = 256;
+__host__ __device__ constexpr size_t lattice_elems(int q, uint32_t ncells) {
+  return (size_t)((ncells + kLatBlock - 1) / kLatBlock) * (size_t)q * kLatBlock;
+}
+// Element of population i of cell c.
+__host__ __device__ inline size_t lat_elem(int q, int i, uint32_t c) {
+  return ((size_t)(c / kLatBlock) * q + i) * kLatBlock + c % kLatBlock;
+}
+
 // Everything one fused stream-collide launch needs besides the two lattices.
 struct StepArgs {
   int32_t nx, ny, nz;
@@ -140,15 +158,25 @@ struct ClassifyArgs {
 // best number of loads in flight depends on the element width and the grid.
 hipError_t launch_step(int q, int storage, bool general, bool nontemporal, int occ, const void* fin, void* fout,
                        const StepArgs& a, int bx, int by, hipStream_t s);
-hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, const BatchArgs& b,
-                             uint32_t ntiles, int bx, int by, hipStream_t s);
+// `rows`: step_row_mode holds for every solve of the batch.
+hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, bool rows,
+                             const BatchArgs& b, uint32_t ntiles, int bx, int by, hipStream_t s);
 // Whether `occ` waves per SIMD can be set exactly for blocks of `threads`.
 bool occupancy_cap_ok(int occ, int threads);
 hipError_t launch_force_finalize(const ForceFinalizeArgs& a, hipStream_t s);
 hipError_t launch_totals(int q, int storage, const void* f, const TotalsArgs& a, int nblocks, hipStream_t s);
+// The wire format is plane-major f ([Q][ncells] float); the lattice uses the
+// block layout below.
 hipError_t launch_init_eq(int q, int storage, const float* mac, void* f, uint32_t ncells, hipStream_t s);
 hipError_t launch_pack(int q, int storage, const float* wire, void* f, uint32_t ncells, hipStream_t s);
 hipError_t launch_unpack(int q, int storage, const void* f, float* wire, uint32_t ncells, hipStream_t s);
+
+// Row mode of the step kernels: every wave of a tile lies in one grid row and
+// rows are whole layout blocks, so a source row's address is wave-uniform.
+inline bool step_row_mode(int nx, bool linear, int bx, int by) {
+  if (nx % (int)kLatBlock != 0) return false;
+  return linear ? (int)kLatBlock % (bx * by) == 0 : bx % 64 == 0;
+}
 hipError_t launch_classify(const ClassifyArgs& a, hipStream_t s);
 // Link masks of the cells next to the body (StepArgs::links) from the class
 // bytes of launch_classify; zero for every other cell.

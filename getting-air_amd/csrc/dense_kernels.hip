@@ -15,13 +15,15 @@
 // which is the same collision with the large constant w_i taken out before
 // rounding.
 //
-// Layout (structure of arrays, plane-major): g[i * ncells + (z*ny + y)*nx + x].
+// Layout: blocks of 256 consecutive cells, each holding its Q populations as
+// Q runs of 256 elements (dense.h kLatBlock, lat_elem).
 // One thread owns one cell; a wave covers 64 consecutive x, so every plane
 // load and store of a wave is one contiguous 256 B (f32) or 128 B (16-bit)
 // span, shifted by at most one element for the x-moving directions.  All Q
 // loads are issued back to back (no branch between them); the rare cells next
 // to a wall or the body then patch their reflected directions.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "dense.h"
 #include "device_common.h"
 #include "lattice_pairs.h"
@@ -49,11 +51,28 @@ __device__ __forceinline__ void sum_force_slots(const double* __restrict__ p, ui
   }
 }
 
+// First element of the block holding cell c (its population i is then at
+// element i * kLatBlock + c % kLatBlock of the block).
+template <int Q, class T>
+__device__ __forceinline__ T* block_of(T* g, uint32_t c) {
+  return g + (size_t)(c / kLatBlock) * (Q * kLatBlock);
+}
+
+// Lattice offset of cell x of a row, relative to the row's first element
+// (rows are whole layout blocks).
+template <int Q>
+__device__ __forceinline__ uint32_t row_off(int x) {
+  return (uint32_t)(x >> 8) * (Q * kLatBlock) + (uint32_t)(x & 255);
+}
+
 // One step of tile b of a solve: every thread of the block pulls, collides
 // and stores its cell; a tile with a force slot leaves its FP64 force there.
 // `salt` seeds the dither of 16-bit stores; `mac` (or null) receives the
 // macroscopic field.  Shared by the single-solve and the batched kernels.
-template <int Q, class ST, bool GEN, bool NT>
+// ROW (step_row_mode): each wave lies in one row, so y, z and every source
+// row's base address are wave-uniform and only the x part of an address is
+// per lane; otherwise each lane locates its sources' blocks itself.
+template <int Q, class ST, bool GEN, bool NT, bool ROW>
 __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout,
                                           const StepArgs& a, uint32_t b, uint32_t salt, float* mac) {
   using L = Lattice<Q>;
@@ -63,9 +82,10 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
   if (a.linear) {
     const uint32_t c = b * (blockDim.x * blockDim.y) + threadIdx.y * blockDim.x + threadIdx.x;
     const uint32_t row = div_invariant(c, a.div_nx);
-    z = (int)div_invariant(row, a.div_ny);
-    y = (int)(row - (uint32_t)z * ny);
     x = (int)(c - row * nx);
+    const uint32_t r = ROW ? __builtin_amdgcn_readfirstlane(row) : row;
+    z = (int)div_invariant(r, a.div_ny);
+    y = (int)(r - (uint32_t)z * ny);
     active = c < a.ncells;
   } else {
     const int tx = b % a.ntx;
@@ -74,33 +94,64 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
     z = rest / a.nty;
     x = tx * blockDim.x + threadIdx.x;
     y = ty * blockDim.y + threadIdx.y;
+    if constexpr (ROW) y = __builtin_amdgcn_readfirstlane(y);
     active = x < nx && y < ny;
   }
   if constexpr (GEN) {
     if (active && a.cover)
       active = a.cover[((z / a.cover_rb) * a.cover_nby + y / a.cover_rb) * a.cover_nbx + x / a.cover_rb] < 0;
   }
-  const uint32_t N = a.ncells;
   double fv[6] = {0, 0, 0, 0, 0, 0};  // this cell's force/torque on the body
 
   if (active) {
     const uint32_t cell = (uint32_t)((z * ny + y) * nx + x);
 
-    // 1. periodic pull-stream, all Q loads straight-line.  The source of
-    // direction i is cell - e_i, written as per-axis index offsets that wrap
-    // only on the domain faces.
-    const int nxy = nx * ny;
-    const int ox_m = x == 0 ? nx - 1 : -1, ox_p = x == nx - 1 ? 1 - nx : 1;
-    const int oy_m = y == 0 ? (ny - 1) * nx : -nx, oy_p = y == ny - 1 ? (1 - ny) * nx : nx;
-    const int oz_m = z == 0 ? (nz - 1) * nxy : -nxy, oz_p = z == nz - 1 ? (1 - nz) * nxy : nxy;
-    float g[Q];
-#pragma unroll
-    for (int i = 0; i < Q; i++) {
-      const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
-      const int o = (ex == 1 ? ox_m : (ex == -1 ? ox_p : 0)) + (ey == 1 ? oy_m : (ey == -1 ? oy_p : 0)) +
-                    (ez == 1 ? oz_m : (ez == -1 ? oz_p : 0));
-      g[i] = ld<ST, NT>(gin + (size_t)i * N, (uint32_t)((int)cell + o));
+    // This cell's populations: population i at self + i * kLatBlock, lane
+    // offset self_lo.
+    using T = typename ST::T;
+    size_t self;
+    uint32_t self_lo;
+    if constexpr (ROW) {
+      self = (size_t)(z * ny + y) * nx * Q;
+      self_lo = row_off<Q>(x);
+    } else {
+      self = (size_t)(cell / kLatBlock) * (Q * kLatBlock);
+      self_lo = cell % kLatBlock;
     }
+
+    // 1. periodic pull-stream, all Q loads straight-line.  The source of
+    // direction i is cell - e_i, wrapping only on the domain faces.
+    float g[Q];
+    if constexpr (ROW) {
+      const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+      const uint32_t lo_m = row_off<Q>(xm), lo_p = row_off<Q>(xp);
+      const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+      const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
+#pragma unroll
+      for (int i = 0; i < Q; i++) {
+        const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
+        const int sy = ey == 1 ? ym : (ey == -1 ? yp : y);
+        const int sz = ez == 1 ? zm : (ez == -1 ? zp : z);
+        const T* src = gin + (size_t)(sz * ny + sy) * nx * Q + i * kLatBlock;
+        g[i] = ld<ST, NT>(src, ex == 1 ? lo_m : (ex == -1 ? lo_p : self_lo));
+      }
+    } else {
+      const int nxy = nx * ny;
+      const int ox_m = x == 0 ? nx - 1 : -1, ox_p = x == nx - 1 ? 1 - nx : 1;
+      const int oy_m = y == 0 ? (ny - 1) * nx : -nx, oy_p = y == ny - 1 ? (1 - ny) * nx : nx;
+      const int oz_m = z == 0 ? (nz - 1) * nxy : -nxy, oz_p = z == nz - 1 ? (1 - nz) * nxy : nxy;
+#pragma unroll
+      for (int i = 0; i < Q; i++) {
+        const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
+        const int o = (ex == 1 ? ox_m : (ex == -1 ? ox_p : 0)) + (ey == 1 ? oy_m : (ey == -1 ? oy_p : 0)) +
+                      (ez == 1 ? oz_m : (ez == -1 ? oz_p : 0));
+        const uint32_t sc = (uint32_t)((int)cell + o);
+        const T* src = gin + (size_t)(sc / kLatBlock) * (Q * kLatBlock) + i * kLatBlock;
+        g[i] = ld<ST, NT>(src, sc % kLatBlock);
+      }
+    }
+    const T* in_self = gin + self;
+    T* out_self = gout + self;
 
     uint8_t c = 0;
     if constexpr (GEN) {
@@ -120,9 +171,9 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
 #pragma unroll
         for (int i = 1; i < Q; i++) {
           const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
-          const typename ST::T* oplane = gin + (size_t)L::opp[i] * N;
+          const T* oplane = in_self + L::opp[i] * kLatBlock;
           if ((lm >> i) & 1u) {
-            g[i] = ST::load(oplane, cell);
+            g[i] = ST::load(oplane, self_lo);
             // momentum exchange on a fluid cell: -e_i (2 f_opp), pinned body
             if (!(c & 1)) {
               const double m = 2.0 * ((double)g[i] + (double)L::w[i]);
@@ -142,7 +193,7 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
                 const float uw = (ey == 1) ? a.wall_u0 : a.wall_u1;
                 corr = 2.f * L::w[i] * (float)ex * uw / (1.0f / 3.0f);
               }
-              g[i] = ST::load(oplane, cell) + corr;
+              g[i] = ST::load(oplane, self_lo) + corr;
             }
           }
         }
@@ -201,7 +252,7 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
     if (solid) {
       // SOLID_EQ for a pinned body: feq(1, 0) = w_i, i.e. g = 0
 #pragma unroll
-      for (int i = 0; i < Q; i++) st<ST, NT>(gout + (size_t)i * N, cell, 0.f);  // exact in every format
+      for (int i = 0; i < Q; i++) st<ST, NT>(out_self + i * kLatBlock, self_lo, 0.f);  // exact in every format
     } else {
       float sp = 0.f;
       if constexpr (GEN) {
@@ -247,7 +298,7 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
         const float w = L::w[0];
         float p = g[0] - om * (g[0] - w * c0) - w * g3uF;
         if constexpr (GEN) p += sp * (a.ae[13] - p);
-        st_sr<ST, NT>(gout, cell, p, ST::dither(hsh, 0));
+        st_sr<ST, NT>(out_self, self_lo, p, ST::dither(hsh, 0));
       }
 #pragma unroll
       for (int k = 0; k < NP; k++) {
@@ -266,8 +317,8 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
           P += sp * (a.ae[k] - P);
           M += sp * (a.bo[k] - M);
         }
-        st_sr<ST, NT>(gout + (size_t)i * N, cell, P + M, ST::dither(hsh, (11 * (2 * k + 1)) % 17));
-        st_sr<ST, NT>(gout + (size_t)j * N, cell, P - M, ST::dither(hsh, (11 * (2 * k + 2)) % 17));
+        st_sr<ST, NT>(out_self + i * kLatBlock, self_lo, P + M, ST::dither(hsh, (11 * (2 * k + 1)) % 17));
+        st_sr<ST, NT>(out_self + j * kLatBlock, self_lo, P - M, ST::dither(hsh, (11 * (2 * k + 2)) % 17));
       }
     }
   }
@@ -289,7 +340,7 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
   }
 }
 
-template <int Q, class ST, bool GEN, bool NT>
+template <int Q, class ST, bool GEN, bool NT, bool ROW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q == 27 ? 6 : 8)))
 k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout, StepArgs a) {
   if constexpr (GEN) {
@@ -309,7 +360,7 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
     salt = *a.salt_in;
     if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *a.salt_out = salt + 1;
   }
-  step_tile<Q, ST, GEN, NT>(gin, gout, a, b, salt, a.mac);
+  step_tile<Q, ST, GEN, NT, ROW>(gin, gout, a, b, salt, a.mac);
 }
 
 // Many solves in one launch (the serve loop's packed pool): block k steps
@@ -317,7 +368,7 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
 // launch < remaining; its lattice parity is its base parity flipped by the
 // launch index, so every launch of a captured chunk is one step of every
 // active solve.
-template <int Q, class ST, bool GEN, bool NT>
+template <int Q, class ST, bool GEN, bool NT, bool ROW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q == 27 ? 6 : 8))) k_step_batch(BatchArgs B) {
   // The tables are written before the launch and only read during it: read
   // them through the constant address space, so their fields are scalar loads
@@ -339,8 +390,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q == 2
   // the dither counter of the solve is its step count
   const uint32_t salt = S.step0 + B.launch;
   float* mac = B.launch + 1 == S.remaining ? S.mac : nullptr;
-  step_tile<Q, ST, GEN, NT>(static_cast<const typename ST::T*>(S.f[p]), static_cast<typename ST::T*>(S.f[p ^ 1]), a,
-                            tile, salt, mac);
+  step_tile<Q, ST, GEN, NT, ROW>(static_cast<const typename ST::T*>(S.f[p]),
+                                 static_cast<typename ST::T*>(S.f[p ^ 1]), a, tile, salt, mac);
 }
 
 // One wave: lane l sums slots l, l+64, ... in order, then a fixed butterfly.
@@ -361,7 +412,7 @@ k_init_eq(const float* __restrict__ mac, typename ST::T* __restrict__ g, uint32_
   for (int i = 0; i < Q; i++) {
     const float eu = (float)L::ex[i] * m.y + (float)L::ey[i] * m.z + (float)L::ez[i] * m.w;
     const float geq = L::w[i] * (drho + m.x * (3.f * eu + 4.5f * eu * eu - 1.5f * usq));
-    ST::store(g + (size_t)i * N, cell, geq);
+    ST::store(block_of<Q>(g, cell) + i * kLatBlock, cell % kLatBlock, geq);
   }
 }
 
@@ -373,7 +424,8 @@ k_pack(const float* __restrict__ wire, typename ST::T* __restrict__ g, uint32_t 
   const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= N) return;
 #pragma unroll
-  for (int i = 0; i < Q; i++) ST::store(g + (size_t)i * N, cell, wire[(size_t)i * N + cell] - L::w[i]);
+  for (int i = 0; i < Q; i++)
+    ST::store(block_of<Q>(g, cell) + i * kLatBlock, cell % kLatBlock, wire[(size_t)i * N + cell] - L::w[i]);
 }
 
 template <int Q, class ST>
@@ -383,7 +435,8 @@ k_unpack(const typename ST::T* __restrict__ g, float* __restrict__ wire, uint32_
   const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= N) return;
 #pragma unroll
-  for (int i = 0; i < Q; i++) wire[(size_t)i * N + cell] = ST::load(g + (size_t)i * N, cell) + L::w[i];
+  for (int i = 0; i < Q; i++)
+    wire[(size_t)i * N + cell] = ST::load(block_of<Q>(g, cell) + i * kLatBlock, cell % kLatBlock) + L::w[i];
 }
 
 template <int Q, class ST>
@@ -395,7 +448,7 @@ k_totals(const typename ST::T* __restrict__ g, TotalsArgs a) {
   for (uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x; cell < a.ncells; cell += gridDim.x * blockDim.x) {
 #pragma unroll
     for (int i = 0; i < Q; i++) {
-      const double fi = (double)ST::load(g + (size_t)i * a.ncells, cell) + (double)L::w[i];
+      const double fi = (double)ST::load(block_of<Q>(g, cell) + i * kLatBlock, cell % kLatBlock) + (double)L::w[i];
       v[0] += fi;
       v[1] += L::ex[i] * fi;
       v[2] += L::ey[i] * fi;
@@ -534,40 +587,41 @@ static size_t occ_dynamic_lds(int occ, int threads) {
   return A > static_bytes ? A - static_bytes : 0;
 }
 
+// Runtime flags -> template arguments of the step kernels.
+template <class F>
+static void with_flags(bool general, bool nt, bool rows, F f) {
+  auto r = [&](auto G, auto N) {
+    if (rows) f(G, N, std::true_type{});
+    else f(G, N, std::false_type{});
+  };
+  auto n = [&](auto G) {
+    if (nt) r(G, std::true_type{});
+    else r(G, std::false_type{});
+  };
+  if (general) n(std::true_type{});
+  else n(std::false_type{});
+}
+
 template <int Q, class ST> struct OpStep {
-  template <bool GEN, bool NT>
-  static void go(int occ, const void* fin, void* fout, const StepArgs& a, int bx, int by, hipStream_t s) {
-    using T = typename ST::T;
-    constexpr auto k = k_step<Q, ST, GEN, NT>;
-    hipLaunchKernelGGL(k, dim3(a.nblocks), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, (const T*)fin,
-                       (T*)fout, a);
-  }
   static hipError_t run(bool general, bool nt, int occ, const void* fin, void* fout, StepArgs a, int bx, int by,
                         hipStream_t s) {
-    if (general) {
-      if (nt) go<true, true>(occ, fin, fout, a, bx, by, s);
-      else go<true, false>(occ, fin, fout, a, bx, by, s);
-    } else {
-      if (nt) go<false, true>(occ, fin, fout, a, bx, by, s);
-      else go<false, false>(occ, fin, fout, a, bx, by, s);
-    }
+    using T = typename ST::T;
+    const bool rows = step_row_mode(a.nx, a.linear != 0, bx, by);
+    with_flags(general, nt, rows, [&](auto G, auto N, auto R) {
+      constexpr auto k = k_step<Q, ST, decltype(G)::value, decltype(N)::value, decltype(R)::value>;
+      hipLaunchKernelGGL(k, dim3(a.nblocks), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, (const T*)fin,
+                         (T*)fout, a);
+    });
     return hipGetLastError();
   }
 };
 template <int Q, class ST> struct OpStepBatch {
-  template <bool GEN, bool NT>
-  static void go(int occ, const BatchArgs& b, uint32_t ntiles, int bx, int by, hipStream_t s) {
-    constexpr auto k = k_step_batch<Q, ST, GEN, NT>;
-    hipLaunchKernelGGL(k, dim3(ntiles), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, b);
-  }
-  static hipError_t run(bool general, bool nt, int occ, BatchArgs b, uint32_t ntiles, int bx, int by, hipStream_t s) {
-    if (general) {
-      if (nt) go<true, true>(occ, b, ntiles, bx, by, s);
-      else go<true, false>(occ, b, ntiles, bx, by, s);
-    } else {
-      if (nt) go<false, true>(occ, b, ntiles, bx, by, s);
-      else go<false, false>(occ, b, ntiles, bx, by, s);
-    }
+  static hipError_t run(bool general, bool nt, int occ, bool rows, BatchArgs b, uint32_t ntiles, int bx, int by,
+                        hipStream_t s) {
+    with_flags(general, nt, rows, [&](auto G, auto N, auto R) {
+      constexpr auto k = k_step_batch<Q, ST, decltype(G)::value, decltype(N)::value, decltype(R)::value>;
+      hipLaunchKernelGGL(k, dim3(ntiles), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, b);
+    });
     return hipGetLastError();
   }
 };
@@ -600,10 +654,10 @@ hipError_t launch_step(int q, int storage, bool general, bool nontemporal, int o
                        const StepArgs& a, int bx, int by, hipStream_t s) {
   return dispatch<OpStep>(q, storage, general, nontemporal, occ, fin, fout, a, bx, by, s);
 }
-hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, const BatchArgs& b,
-                             uint32_t ntiles, int bx, int by, hipStream_t s) {
+hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, bool rows,
+                             const BatchArgs& b, uint32_t ntiles, int bx, int by, hipStream_t s) {
   if (ntiles == 0) return hipSuccess;
-  return dispatch<OpStepBatch>(q, storage, general, nontemporal, occ, b, ntiles, bx, by, s);
+  return dispatch<OpStepBatch>(q, storage, general, nontemporal, occ, rows, b, ntiles, bx, by, s);
 }
 hipError_t launch_init_eq(int q, int storage, const float* mac, void* f, uint32_t N, hipStream_t s) {
   return dispatch<OpInit>(q, storage, mac, f, N, s);

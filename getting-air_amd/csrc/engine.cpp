@@ -563,7 +563,7 @@ int create_impl(const lbm_desc* desc, Arena* arena, hipStream_t stream, int bx, 
     release(s);
     return hip_fail(e, "hipEventCreate");
   }
-  const size_t lattice_bytes = (size_t)s->N * s->q * s->elem;
+  const size_t lattice_bytes = lattice_elems(s->q, s->N) * s->elem;
   const int nred = kTotalsBlocks;
   if ((rc = dalloc(s, &s->f[0], lattice_bytes)) || (rc = dalloc(s, &s->f[1], lattice_bytes)) ||
       (rc = dalloc(s, &s->mac, (size_t)s->N * 16)) || (rc = dalloc(s, &s->scratch, (size_t)nred * 4 * 8)) ||
@@ -624,7 +624,7 @@ size_t solve_footprint(const lbm_desc* d) {
   const size_t n = (size_t)d->nx * d->ny * (d->q == 9 ? 1 : d->nz);
   const size_t elem = d->storage == LBM_STORE_F32 ? 4 : 2;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  size_t total = 2 * al(n * d->q * elem) + al(n * 16) + al((size_t)kTotalsBlocks * 4 * 8) + al(64) +
+  size_t total = 2 * al(lattice_elems(d->q, (uint32_t)n) * elem) + al(n * 16) + al((size_t)kTotalsBlocks * 4 * 8) + al(64) +
                  al((size_t)kHistoryCap * 6 * 8) + al(8) + al(64) + al(8);
   if (d->body.shape != LBM_BODY_NONE) {
     // class bytes, link masks, one slot index per tile (tiles of at least 64
@@ -721,7 +721,7 @@ int lbm_upload_macro(lbm_solve* s, const float* mac4) {
   HIPCHK(hipMemcpyAsync(s->mac, mac4, (size_t)s->N * 16, hipMemcpyHostToDevice, s->stream));
   HIPCHK(launch_init_eq(s->q, s->storage, s->mac, s->f[0], s->N, s->stream));
   // Both lattices hold the equilibrium (main-3d.js:2994 seeds A and B).
-  HIPCHK(hipMemcpyAsync(s->f[1], s->f[0], (size_t)s->N * s->q * s->elem, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->f[1], s->f[0], lattice_elems(s->q, s->N) * s->elem, hipMemcpyDeviceToDevice, s->stream));
   HIPCHK(hipMemsetAsync(s->salt, 0, 8, s->stream));
   s->parity = 0;
   s->step = 0;
@@ -741,7 +741,8 @@ int lbm_upload_f(lbm_solve* s, const float* f) {
   HIPCHK(hipMalloc((void**)&tmp, wire));
   hipError_t e = hipMemcpyAsync(tmp, f, wire, hipMemcpyHostToDevice, s->stream);
   if (e == hipSuccess) e = launch_pack(s->q, s->storage, tmp, s->f[0], s->N, s->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(s->f[1], s->f[0], (size_t)s->N * s->q * s->elem, hipMemcpyDeviceToDevice, s->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s->f[1], s->f[0], lattice_elems(s->q, s->N) * s->elem, hipMemcpyDeviceToDevice, s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->salt, 0, 8, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   (void)hipFree(tmp);
@@ -752,6 +753,20 @@ int lbm_upload_f(lbm_solve* s, const float* f) {
   s->step_at_salt0 = 0;
   s->initialised = true;
   return reset_history(s);
+}
+
+// Copies between the lattice's block layout and plane-major order:
+// to_planes true reads `lat` and writes `planes`, false the reverse.
+static void blocks_to_planes(char* lat, char* planes, int q, uint32_t n, int elem, bool to_planes) {
+  for (uint32_t b0 = 0; b0 < n; b0 += kLatBlock) {
+    const size_t run = (size_t)std::min<uint32_t>(kLatBlock, n - b0) * elem;
+    for (int i = 0; i < q; i++) {
+      char* l = lat + lat_elem(q, i, b0) * elem;
+      char* p = planes + ((size_t)i * n + b0) * elem;
+      if (to_planes) memcpy(p, l, run);
+      else memcpy(l, p, run);
+    }
+  }
 }
 
 int lbm_state_size(lbm_solve* s, size_t* bytes) {
@@ -766,9 +781,12 @@ int lbm_save_state(lbm_solve* s, void* buf, size_t bytes) {
   const size_t payload = (size_t)s->N * s->q * s->elem;
   if (bytes < sizeof(lbm_state_header) + payload)
     return fail(LBM_E_INVALID, "state buffer of %zu bytes, need %zu", bytes, sizeof(lbm_state_header) + payload);
-  HIPCHK(hipMemcpyAsync(static_cast<char*>(buf) + sizeof(lbm_state_header), s->f[s->parity], payload,
-                        hipMemcpyDeviceToHost, s->stream));
+  // The payload holds plane-major populations ([Q][cells]); the lattice's
+  // blocks are reordered on the host.
+  std::vector<char> lat(lattice_elems(s->q, s->N) * s->elem);
+  HIPCHK(hipMemcpyAsync(lat.data(), s->f[s->parity], lat.size(), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  blocks_to_planes(lat.data(), static_cast<char*>(buf) + sizeof(lbm_state_header), s->q, s->N, s->elem, true);
   lbm_state_header h{};
   memcpy(h.magic, "LBMSTATE", 8);
   h.version = 1;
@@ -787,9 +805,11 @@ int lbm_load_state(lbm_solve* s, const void* buf, size_t bytes) {
   lbm_state_header h{};
   int rc = state_header_check(&s->d, buf, bytes, &h);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(s->f[0], static_cast<const char*>(buf) + sizeof(lbm_state_header), h.payload_bytes,
-                        hipMemcpyHostToDevice, s->stream));
-  HIPCHK(hipMemcpyAsync(s->f[1], s->f[0], h.payload_bytes, hipMemcpyDeviceToDevice, s->stream));
+  std::vector<char> lat(lattice_elems(s->q, s->N) * s->elem, 0);
+  blocks_to_planes(lat.data(), const_cast<char*>(static_cast<const char*>(buf)) + sizeof(lbm_state_header), s->q, s->N,
+                   s->elem, false);
+  HIPCHK(hipMemcpyAsync(s->f[0], lat.data(), lat.size(), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->f[1], s->f[0], lat.size(), hipMemcpyDeviceToDevice, s->stream));
   const uint32_t words[2] = {h.salt, h.salt};
   HIPCHK(hipMemcpyAsync(s->salt, words, 8, hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));

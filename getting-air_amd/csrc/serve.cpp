@@ -144,8 +144,9 @@ struct Member {
 
 struct ClassKey {
   int q, storage, general;
+  int rows;  // nx is a multiple of the layout block (row-mode kernels)
   bool operator<(const ClassKey& o) const {
-    return std::tie(q, storage, general) < std::tie(o.q, o.storage, o.general);
+    return std::tie(q, storage, general, rows) < std::tie(o.q, o.storage, o.general, o.rows);
   }
 };
 
@@ -262,7 +263,8 @@ void admit(lbm_serve* sv) {
                        (r->d.body.shape != LBM_BODY_NONE || r->d.walls[0] || r->d.walls[1] ||
                         (r->d.walls[2] && r->d.q != 9) || r->d.sponge_width > 0.f)
                            ? 1
-                           : 0};
+                           : 0,
+                       r->d.nx % (int)kLatBlock == 0 ? 1 : 0};
     auto& cp = sv->classes[key];
     if (!cp) {
       cp = std::make_unique<Class>();
@@ -366,8 +368,9 @@ int enqueue_chunk(lbm_serve* sv, int K) {
       Class* c = kv.second.get();
       if (c->members.empty()) continue;
       BatchArgs b{c->d_solves, c->d_tiles, j};
-      HIPCHK_RET(launch_step_batch(c->key.q, c->key.storage, c->key.general != 0, c->nt != 0, c->occ, b, c->ntiles,
-                                   c->bx, c->by, sv->stream));
+      const bool rows = c->key.rows && step_row_mode((int)kLatBlock, c->lin != 0, c->bx, c->by);
+      HIPCHK_RET(launch_step_batch(c->key.q, c->key.storage, c->key.general != 0, c->nt != 0, c->occ, rows, b,
+                                   c->ntiles, c->bx, c->by, sv->stream));
     }
   }
   return LBM_OK;
