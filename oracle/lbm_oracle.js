@@ -14,9 +14,10 @@
 // arithmetic, operation by operation, exactly as the HIP classification
 // kernel does, so the two agree bit for bit.
 //
-// Pinned bodies only: the body velocity is zero, so Ladd's moving-wall term
-// vanishes on body links.  Moving walls (the 2D Couette form,
-// shaders/common_walls.wgsl:27-31) keep their correction term.
+// The pinned body's velocity is zero, so Ladd's moving-wall term vanishes on
+// its links.  A moving rigid body (rigid_oracle.js) passes its own link masks,
+// body velocity and window through stepSlab's `R`.  Moving walls (the 2D
+// Couette form, shaders/common_walls.wgsl:27-31) keep their correction term.
 //
 // Parity pins for this file live in tests/js/test-oracle.js and
 // tests/test_oracle.py (closed forms and invariants, see those files).
@@ -151,8 +152,12 @@ function normalise(d) {
 
 // One fused pull-stream + collide over the x-rows r = z * NY + y in [r0, r1).
 // `fin`/`fout` are plane-major: f[i * N + (z * NY + y) * NX + x].  `mac`
-// receives [rho, ux, uy, uz] per cell when non-null.
-function stepSlab(S, T, cls, fin, fout, mac, r0, r1) {
+// receives [rho, ux, uy, uz] per cell when non-null.  `R` (optional) is a
+// moving rigid body: { links[cell] (bit i: the pull source of direction i is
+// solid), us(x, y, z) (body velocity at the cell), rhoWLocal, winCoord(x, y,
+// z) (sponge coordinates) } (common_d3_step.wgsl:224-400 with HAS_BODY and
+// USE_BOUNCEBACK).
+function stepSlab(S, T, cls, fin, fout, mac, r0, r1, R) {
   const Q = T.Q, NX = S.nx, NY = S.ny, NZ = S.nz, N = NX * NY * NZ;
   const ex = T.ex, ey = T.ey, ez = T.ez, w = T.w, opp = T.opp;
   const om = S.omega, dw = S.dw;
@@ -161,7 +166,7 @@ function stepSlab(S, T, cls, fin, fout, mac, r0, r1) {
   const anyWall = wallX || wallY || wallZ;
   const sw = S.sponge.width, sax = S.sponge.axes, su = S.sponge.u;
   const uFarSq = su[0] * su[0] + su[1] * su[1] + su[2] * su[2];
-  const hasBody = !!S.body;
+  const hasBody = !!S.body || !!R;
   const solidEq = hasBody && S.solidEq;
   const f = new Float64Array(Q);
   const fo = new Float64Array(Q);
@@ -186,12 +191,19 @@ function stepSlab(S, T, cls, fin, fout, mac, r0, r1) {
       for (let x = 0; x < NX; x++) {
         const cell = (z * NY + y) * NX + x;
         const c = cls[cell];
+        // body velocity and wall density of a moving body, where needed
+        let us = null, rhoW = 1;
+        const lm = R ? R.links[cell] : 0;
+        if (R && (lm || (c & CLS_SOLID))) us = R.us(x, y, z);
+        if (lm && R.rhoWLocal) { rhoW = 0; for (let i = 0; i < Q; i++) rhoW += fin[i * N + cell]; }
         // 1. pull-stream with body bounce-back, then wall bounce-back.
         for (let i = 0; i < Q; i++) {
           const sx = x - ex[i];
           const src = rowBase[i] + (sx < 0 ? sx + NX : (sx >= NX ? sx - NX : sx));
-          if (c !== 0 && (c & CLS_NEAR) && (cls[src] & CLS_SOLID)) {
-            f[i] = fin[opp[i] * N + cell];
+          if (R ? ((lm >>> i) & 1) : (c !== 0 && (c & CLS_NEAR) && (cls[src] & CLS_SOLID))) {
+            // Ladd's moving-wall term 2 w rho_w (e . u_body) / cs2
+            const corr = R ? 2 * w[i] * rhoW * (ex[i] * us[0] + ey[i] * us[1] + ez[i] * us[2]) * 3 : 0;
+            f[i] = fin[opp[i] * N + cell] + corr;
           } else if (anyWall && (rowOut[i] || (wallX && (sx < 0 || sx >= NX)))) {
             // Halfway wall; y-walls may move along x (common_walls.wgsl:27-31).
             let corr = 0;
@@ -218,23 +230,35 @@ function stepSlab(S, T, cls, fin, fout, mac, r0, r1) {
         const usq = ux * ux + uy * uy + uz * uz;
         const solid = solidEq && (c & CLS_SOLID);
         if (mac) {
-          if (solid && S.dim === 3) {
+          if (solid && R) {
+            mac[4 * cell] = 1; mac[4 * cell + 1] = us[0]; mac[4 * cell + 2] = us[1]; mac[4 * cell + 3] = us[2];
+          } else if (solid && S.dim === 3) {
             mac[4 * cell] = 1; mac[4 * cell + 1] = 0; mac[4 * cell + 2] = 0; mac[4 * cell + 3] = 0;
           } else {
             mac[4 * cell] = rho; mac[4 * cell + 1] = ux; mac[4 * cell + 2] = uy; mac[4 * cell + 3] = uz;
           }
         }
         if (solid) {
-          // SOLID_EQ: feq(1, u_body) with u_body = 0 for a pinned body.
-          for (let i = 0; i < Q; i++) fout[i * N + cell] = w[i];
+          // SOLID_EQ: feq(1, u_body); u_body = 0 for a pinned body.
+          if (R) {
+            const uu = us[0] * us[0] + us[1] * us[1] + us[2] * us[2];
+            for (let i = 0; i < Q; i++) {
+              const eu = ex[i] * us[0] + ey[i] * us[1] + ez[i] * us[2];
+              fout[i * N + cell] = w[i] * (1 + 3 * eu + 4.5 * eu * eu - 1.5 * uu);
+            }
+          } else {
+            for (let i = 0; i < Q; i++) fout[i * N + cell] = w[i];
+          }
           continue;
         }
         // sponge weight (common_d3_sponge.wgsl:10-14, common_sponge.wgsl:7-11)
         let sp = 0;
         if (sw > 0) {
-          const dX = sax[0] ? Math.min(x, NX - 1 - x) : 1e30;
-          const dY = sax[1] ? Math.min(y, NY - 1 - y) : 1e30;
-          const dZ = sax[2] ? Math.min(z, NZ - 1 - z) : 1e30;
+          // window coordinates under a moving window (common_d3_step.wgsl:332-337)
+          const wp = R ? R.winCoord(x, y, z) : [x, y, z];
+          const dX = sax[0] ? Math.min(wp[0], NX - 1 - wp[0]) : 1e30;
+          const dY = sax[1] ? Math.min(wp[1], NY - 1 - wp[1]) : 1e30;
+          const dZ = sax[2] ? Math.min(wp[2], NZ - 1 - wp[2]) : 1e30;
           let t = 1 - Math.min(dX, Math.min(dY, dZ)) / sw;
           t = t < 0 ? 0 : (t > 1 ? 1 : t);
           sp = t * t * (3 - 2 * t);
