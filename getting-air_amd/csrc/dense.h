@@ -9,6 +9,8 @@ namespace lbm {
 
 enum Storage { kF32 = 0, kF16 = 1, kBF16 = 2 };
 
+struct RigidBody;  // rigid.h
+
 // floor(n / d) for 0 <= n < 2^31 as a multiply and a shift: with
 // l = ceil(log2 d) and m = ceil(2^(31+l) / d), 2^(31+l) <= m d <= 2^(31+l) + 2^l,
 // which is the condition of Granlund & Montgomery ("Division by invariant
@@ -70,6 +72,11 @@ struct StepArgs {
   const uint32_t* links;
   int32_t solid_eq;
   int32_t solid_mac_rest;  // 3D: a solid cell reports [1, 0, 0, 0]
+  // Moving rigid body (3D; null otherwise): cls/links hold its pose, the
+  // body links take Ladd's moving-wall term at the local wall density, the
+  // solid interior is held at feq(1, u_body) and reports [1, u_body], and
+  // the sponge is measured in window coordinates.
+  const RigidBody* rigid;
   float* mac;          // [rho, ux, uy, uz] per cell, or null (not observed)
   // Block refinement map of an AMR hierarchy: a cell whose block holds a
   // slot is solved by the finer level and skipped here

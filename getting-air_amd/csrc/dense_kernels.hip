@@ -28,6 +28,7 @@
 #include "device_common.h"
 #include "lattice_pairs.h"
 #include "lattice_tables.h"
+#include "rigid_device.h"
 
 namespace lbm {
 
@@ -72,7 +73,9 @@ __device__ __forceinline__ uint32_t row_off(int x) {
 // ROW (step_row_mode): each wave lies in one row, so y, z and every source
 // row's base address are wave-uniform and only the x part of an address is
 // per lane; otherwise each lane locates its sources' blocks itself.
-template <int Q, class ST, bool GEN, bool NT, bool ROW>
+// RIGID (with GEN): a moving rigid body, a.rigid (its own instantiations, so
+// that the kernels without one keep their register budget).
+template <int Q, class ST, bool GEN, bool NT, bool ROW, bool RIGID = false>
 __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout,
                                           const StepArgs& a, uint32_t b, uint32_t salt, float* mac) {
   using L = Lattice<Q>;
@@ -168,12 +171,24 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
         // cost the whole kernel occupancy).
         double fx = 0, fy = 0, fz = 0;
         const uint32_t lm = (c & 2) ? a.links[cell] : 0u;
+        // a moving body's velocity here and the wall density: this cell's
+        // own pre-stream density (common_d3_step.wgsl:236-242)
+        float wux = 0.f, wuy = 0.f, wuz = 0.f, rho_w = 1.f;
+        if (RIGID && lm) {
+          body_velocity_f(a.rigid, x, y, z, wux, wuy, wuz);
+          if (a.rigid->rho_w_local) {
+            rho_w = 0.f;
+            for (int i = 0; i < Q; i++) rho_w += ST::load(in_self + i * kLatBlock, self_lo) + L::w[i];
+          }
+        }
 #pragma unroll
         for (int i = 1; i < Q; i++) {
           const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
           const T* oplane = in_self + L::opp[i] * kLatBlock;
           if ((lm >> i) & 1u) {
             g[i] = ST::load(oplane, self_lo);
+            // Ladd's moving-wall term 2 w rho_w (e . u_body) / cs2
+            if constexpr (RIGID) g[i] += 2.f * L::w[i] * rho_w * ((float)ex * wux + (float)ey * wuy + (float)ez * wuz) * 3.f;
             // momentum exchange on a fluid cell: -e_i (2 f_opp), pinned body
             if (!(c & 1)) {
               const double m = 2.0 * ((double)g[i] + (double)L::w[i]);
@@ -242,25 +257,44 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
 
     bool solid = false;
     if constexpr (GEN) solid = a.solid_eq && (c & 1);
+    // a moving body's interior velocity (zero for a pinned body)
+    float sux = 0.f, suy = 0.f, suz = 0.f;
+    if constexpr (RIGID) {
+      if (solid) body_velocity_f(a.rigid, x, y, z, sux, suy, suz);
+    }
 
     if (mac) {
       float4 m = make_float4(rho, ux, uy, uz);
-      if (solid && a.solid_mac_rest) m = make_float4(1.f, 0.f, 0.f, 0.f);
+      if (solid && a.solid_mac_rest) m = make_float4(1.f, sux, suy, suz);
       reinterpret_cast<float4*>(mac)[cell] = m;
     }
 
     if (solid) {
-      // SOLID_EQ for a pinned body: feq(1, 0) = w_i, i.e. g = 0
+      // SOLID_EQ: feq(1, u_body), g = feq - w; for a pinned body g = 0,
+      // exact in every format
+      const float uu = sux * sux + suy * suy + suz * suz;
 #pragma unroll
-      for (int i = 0; i < Q; i++) st<ST, NT>(out_self + i * kLatBlock, self_lo, 0.f);  // exact in every format
+      for (int i = 0; i < Q; i++) {
+        const float eu = (float)L::ex[i] * sux + (float)L::ey[i] * suy + (float)L::ez[i] * suz;
+        st<ST, NT>(out_self + i * kLatBlock, self_lo, L::w[i] * (3.f * eu + 4.5f * eu * eu - 1.5f * uu));
+      }
     } else {
       float sp = 0.f;
       if constexpr (GEN) {
         if (a.sponge_w > 0.f) {
           const float big = 1e30f;
-          const float dX = (a.sponge_mask & 1) ? fminf((float)x, (float)(nx - 1 - x)) : big;
-          const float dY = (a.sponge_mask & 2) ? fminf((float)y, (float)(ny - 1 - y)) : big;
-          const float dZ = (a.sponge_mask & 4) ? fminf((float)z, (float)(nz - 1 - z)) : big;
+          // under a moving window, distances in window coordinates
+          // (common_d3_step.wgsl:332-337)
+          float px = (float)x, py = (float)y, pz = (float)z;
+          if constexpr (RIGID) {
+            const RigidBody* B = a.rigid;
+            px = (float)win_coord(x, B->c[0], B->win_a[0], B->win_n[0]);
+            py = (float)win_coord(y, B->c[1], B->win_a[1], B->win_n[1]);
+            pz = (float)win_coord(z, B->c[2], B->win_a[2], B->win_n[2]);
+          }
+          const float dX = (a.sponge_mask & 1) ? fminf(px, (float)(nx - 1) - px) : big;
+          const float dY = (a.sponge_mask & 2) ? fminf(py, (float)(ny - 1) - py) : big;
+          const float dZ = (a.sponge_mask & 4) ? fminf(pz, (float)(nz - 1) - pz) : big;
           const float d = fminf(dX, fminf(dY, dZ));
           // beyond the band d / w >= 1 rounds to >= 1, so t clamps to 0 there:
           // only cells inside the band pay for the division
@@ -340,7 +374,7 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
   }
 }
 
-template <int Q, class ST, bool GEN, bool NT, bool ROW>
+template <int Q, class ST, bool GEN, bool NT, bool ROW, bool RIGID = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q == 27 ? 6 : 8)))
 k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout, StepArgs a) {
   if constexpr (GEN) {
@@ -360,7 +394,7 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
     salt = *a.salt_in;
     if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *a.salt_out = salt + 1;
   }
-  step_tile<Q, ST, GEN, NT, ROW>(gin, gout, a, b, salt, a.mac);
+  step_tile<Q, ST, GEN, NT, ROW, RIGID>(gin, gout, a, b, salt, a.mac);
 }
 
 // Many solves in one launch (the serve loop's packed pool): block k steps
@@ -607,6 +641,17 @@ template <int Q, class ST> struct OpStep {
                         hipStream_t s) {
     using T = typename ST::T;
     const bool rows = step_row_mode(a.nx, a.linear != 0, bx, by);
+    if constexpr (Q != 9) {
+      if (a.rigid) {
+        // a moving body runs the general kernel's rigid instantiation
+        with_flags(true, nt, rows, [&](auto, auto N, auto R) {
+          constexpr auto k = k_step<Q, ST, true, decltype(N)::value, decltype(R)::value, true>;
+          hipLaunchKernelGGL(k, dim3(a.nblocks), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, (const T*)fin,
+                             (T*)fout, a);
+        });
+        return hipGetLastError();
+      }
+    }
     with_flags(general, nt, rows, [&](auto G, auto N, auto R) {
       constexpr auto k = k_step<Q, ST, decltype(G)::value, decltype(N)::value, decltype(R)::value>;
       hipLaunchKernelGGL(k, dim3(a.nblocks), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, (const T*)fin,

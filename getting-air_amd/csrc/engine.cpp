@@ -24,6 +24,7 @@
 
 #include "../../include/lbm.h"
 #include "dense.h"
+#include "rigid.h"
 #include "engine.h"
 #include "lattice_pairs.h"
 
@@ -127,6 +128,12 @@ struct lbm_solve {
   int graph_bx = 0, graph_by = 0, graph_swz = 0, graph_nt = 0, graph_occ = 0, graph_lin = 0;
   Arena* arena = nullptr;           // pool the buffers come from (serve loop), or null
   bool own_stream = true;
+  // moving rigid body (lbm_rigid_attach): device record, force partials,
+  // launch arguments of its kernels
+  lbm::RigidBody* rigid = nullptr;
+  double* rigid_partials = nullptr;
+  unsigned* rigid_counter = nullptr;
+  lbm::RigidArgs rargs{};
 };
 
 namespace {
@@ -218,8 +225,9 @@ StepArgs step_args(lbm_solve* s, bool write_mac, bool with_force) {
   a.sponge_w = d.sponge_width;
   a.sponge_mask = (d.sponge_axes[0] ? 1 : 0) | (d.sponge_axes[1] ? 2 : 0) | (d.sponge_axes[2] && s->dim == 3 ? 4 : 0);
   for (int k = 0; k < 3; k++) a.su[k] = d.sponge_u[k];
-  a.cls = s->body ? s->cls : nullptr;
-  a.links = s->body ? s->links : nullptr;
+  a.cls = s->body || s->rigid ? s->cls : nullptr;
+  a.links = s->body || s->rigid ? s->links : nullptr;
+  a.rigid = s->rigid;
   a.solid_eq = d.solid_eq;
   a.solid_mac_rest = s->dim == 3;
   a.mac = write_mac ? s->mac : nullptr;
@@ -301,6 +309,12 @@ int build_force_slots(lbm_solve* s) {
 int enqueue_step(lbm_solve* s, bool write_mac, bool finalize_prev) {
   const void* fin = s->f[s->parity];
   void* fout = s->f[1 - s->parity];
+  if (s->rigid) {
+    // force at the current pose and the integrator, then the new pose's cells
+    hipError_t e = launch_rigid_force(s->q, s->storage, fin, s->rargs, s->stream);
+    if (e == hipSuccess) e = launch_rigid_classify(s->q, s->rargs, s->stream);
+    if (e != hipSuccess) return hip_fail(e, "rigid body launch");
+  }
   StepArgs a = step_args(s, write_mac, true);
   a.finalize_prev = finalize_prev && a.force_slot ? 1 : 0;
   hipError_t e = launch_step(s->q, s->storage, s->general, s->nt, s->occ, fin, fout, a, s->bx, s->by, s->stream);
@@ -504,6 +518,8 @@ void release(lbm_solve* s) {
     if (s->arena) s->arena->release(p);
     else (void)hipFree(p);
   }
+  for (void* p : {(void*)s->rigid, (void*)s->rigid_partials, (void*)s->rigid_counter})
+    if (p) (void)hipFree(p);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->stream && s->own_stream) (void)hipStreamDestroy(s->stream);
@@ -777,6 +793,7 @@ int lbm_state_size(lbm_solve* s, size_t* bytes) {
 
 int lbm_save_state(lbm_solve* s, void* buf, size_t bytes) {
   if (!s || !buf) return fail(LBM_E_INVALID, "null argument");
+  if (s->rigid) return fail(LBM_E_STATE, "the state of a solve with a rigid body is not checkpointed");
   if (!s->initialised) return fail(LBM_E_STATE, "save_state before upload_macro / upload_f / load_state");
   const size_t payload = (size_t)s->N * s->q * s->elem;
   if (bytes < sizeof(lbm_state_header) + payload)
@@ -1030,6 +1047,148 @@ int lbm_probe_copy_bandwidth(size_t bytes, int32_t reps, double* gbs, double* ms
   const double ms = (double)t / reps;
   *gbs = 2.0 * (double)bytes / (ms * 1e-3) / 1e9;
   if (ms_per_copy) *ms_per_copy = ms;
+  return LBM_OK;
+}
+
+// --- moving rigid body --------------------------------------------------
+
+namespace {
+
+// Quaternion helpers in the FP64 operation order of the host restatement
+// (d3-body.mjs:161-218), for the attach-time angular momentum.
+void q_mul_h(const double p[4], const double q[4], double o[4]) {
+  o[0] = p[0] * q[0] - p[1] * q[1] - p[2] * q[2] - p[3] * q[3];
+  o[1] = p[0] * q[1] + p[1] * q[0] + p[2] * q[3] - p[3] * q[2];
+  o[2] = p[0] * q[2] - p[1] * q[3] + p[2] * q[0] + p[3] * q[1];
+  o[3] = p[0] * q[3] + p[1] * q[2] - p[2] * q[1] + p[3] * q[0];
+}
+void q_rotate_h(const double q[4], const double v[3], bool inverse, double o[3]) {
+  const double qc[4] = {q[0], -q[1], -q[2], -q[3]};
+  const double vq[4] = {0.0, v[0], v[1], v[2]};
+  double t[4], u[4];
+  if (inverse) { q_mul_h(qc, vq, t); q_mul_h(t, q, u); }
+  else { q_mul_h(q, vq, t); q_mul_h(t, qc, u); }
+  o[0] = u[1]; o[1] = u[2]; o[2] = u[3];
+}
+
+}  // namespace
+
+int lbm_rigid_attach(lbm_solve* s, const lbm_rigid_body* b) {
+  if (!s || !b) return fail(LBM_E_INVALID, "null argument");
+  if (s->dim != 3) return fail(LBM_E_INVALID, "a rigid body needs a 3D solve (q 19 or 27)");
+  if (s->body) return fail(LBM_E_INVALID, "the solve already has a pinned body in its descriptor");
+  if (s->arena) return fail(LBM_E_INVALID, "rigid bodies attach to standalone solves only");
+  const double a = b->a, bb = b->shape == LBM_SHAPE_ROUNDBOX ? b->b : b->a,
+               c = b->shape == LBM_SHAPE_SPHERE ? b->a : b->c, r = b->shape == LBM_SHAPE_ROUNDBOX ? b->r : 0.0;
+  if (b->shape < LBM_SHAPE_SPHERE || b->shape > LBM_SHAPE_ROUNDBOX) return fail(LBM_E_INVALID, "shape %d", b->shape);
+  if (!(a > 0.0) || !(bb > 0.0) || !(c > 0.0) || !(r >= 0.0) || r > std::min(a, std::min(bb, c)))
+    return fail(LBM_E_INVALID, "body dimensions a %g b %g c %g r %g", a, bb, c, r);
+  if (!(b->density > 0.0)) return fail(LBM_E_INVALID, "density %g", b->density);
+  if (!(b->v_max >= 0.0) || !(b->o_max >= 0.0)) return fail(LBM_E_INVALID, "clamps v_max %g o_max %g", b->v_max, b->o_max);
+  const double qn = std::sqrt(b->q[0] * b->q[0] + b->q[1] * b->q[1] + b->q[2] * b->q[2] + b->q[3] * b->q[3]);
+  if (!(std::fabs(qn - 1.0) < 1e-9)) return fail(LBM_E_INVALID, "orientation quaternion of norm %.17g", qn);
+  const double n[3] = {(double)s->d.nx, (double)s->d.ny, (double)s->d.nz};
+  for (int k = 0; k < 3; k++)
+    if (!std::isfinite(b->x[k]) || !std::isfinite(b->v[k]) || !std::isfinite(b->omega[k]))
+      return fail(LBM_E_INVALID, "non-finite body state");
+
+  RigidBody h{};
+  h.shape = b->shape;
+  h.a = a; h.b = bb; h.cz = c; h.r = r;
+  const double pi = 3.14159265358979323846;
+  double vol;
+  if (b->shape == LBM_SHAPE_SPHERE) vol = (4.0 / 3.0) * pi * (a * a * a);
+  else if (b->shape == LBM_SHAPE_SPHEROID) vol = (4.0 / 3.0) * pi * a * a * c;
+  else vol = 8.0 * a * bb * c;
+  h.mass = b->density * vol;
+  const double m = h.mass;
+  if (b->shape == LBM_SHAPE_SPHERE) { h.I[0] = h.I[1] = h.I[2] = 0.4 * m * a * a; }
+  else if (b->shape == LBM_SHAPE_SPHEROID) { h.I[0] = h.I[1] = m * (a * a + c * c) / 5.0; h.I[2] = 2.0 * m * a * a / 5.0; }
+  else {
+    h.I[0] = m * (bb * bb + c * c) / 3.0;
+    h.I[1] = m * (a * a + c * c) / 3.0;
+    h.I[2] = m * (a * a + bb * bb) / 3.0;
+  }
+  h.rc = b->shape == LBM_SHAPE_SPHERE ? a : (b->shape == LBM_SHAPE_SPHEROID ? std::max(a, c) : std::sqrt(a * a + bb * bb + c * c));
+  for (int k = 0; k < 4; k++) h.q[k] = b->q[k];
+  double Lb[3], Lbody[3];
+  q_rotate_h(h.q, b->omega, true, Lb);
+  for (int k = 0; k < 3; k++) Lbody[k] = Lb[k] * h.I[k];
+  q_rotate_h(h.q, Lbody, false, h.L);
+  for (int k = 0; k < 3; k++) {
+    h.c[k] = b->x[k];
+    h.v[k] = b->v[k];
+    h.w[k] = b->omega[k];
+    h.g[k] = b->gravity[k];
+    h.win_n[k] = b->window[k] ? n[k] : 0.0;
+    h.win_a[k] = b->window_anchor[k];
+  }
+  h.v_max = b->v_max;
+  h.o_max = b->o_max;
+  h.pinned = b->pinned ? 1 : 0;
+  h.planar = b->planar ? 1 : 0;
+  h.no_fluid = b->no_fluid_force ? 1 : 0;
+  h.swept = b->swept_force ? 1 : 0;
+  h.rho_w_local = b->rho_w_local ? 1 : 0;
+
+  RigidArgs& ra = s->rargs;
+  ra = RigidArgs{};
+  ra.nx = s->d.nx; ra.ny = s->d.ny; ra.nz = s->d.nz;
+  ra.half = (int)std::ceil(h.rc) + 3;
+  const int dims[3] = {s->d.nx, s->d.ny, s->d.nz};
+  const int move = (int)std::ceil(std::min(h.v_max, (double)std::max(dims[0], std::max(dims[1], dims[2])))) + 1;
+  for (int k = 0; k < 3; k++) {
+    ra.fbox[k] = std::min(2 * ra.half + 2, dims[k]);
+    ra.cbox[k] = std::min(2 * ra.half + 2 + move, dims[k]);
+  }
+  int rc = LBM_OK;
+  if (!s->cls && (rc = dalloc(s, &s->cls, s->N))) return rc;
+  if (!s->links && (rc = dalloc(s, &s->links, (size_t)s->N * 4))) return rc;
+  if (!s->rigid) {
+    HIPCHK(hipMalloc((void**)&s->rigid, sizeof(RigidBody)));
+    HIPCHK(hipMalloc((void**)&s->rigid_counter, 16));
+  }
+  if (s->rigid_partials) {
+    (void)hipFree(s->rigid_partials);
+    s->rigid_partials = nullptr;
+  }
+  ra.body = s->rigid;
+  ra.cls = s->cls;
+  ra.links = s->links;
+  ra.counter = s->rigid_counter;
+  ra.history = s->history;
+  ra.written = s->written;
+  ra.hist_cap = kHistoryCap;
+  HIPCHK(hipMalloc((void**)&s->rigid_partials, (size_t)rigid_force_blocks(ra) * 6 * 8));
+  ra.partials = s->rigid_partials;
+  HIPCHK(hipMemsetAsync(s->cls, 0, s->N, s->stream));
+  HIPCHK(hipMemsetAsync(s->links, 0, (size_t)s->N * 4, s->stream));
+  HIPCHK(hipMemsetAsync(s->rigid_counter, 0, 16, s->stream));
+  HIPCHK(hipMemcpyAsync(s->rigid, &h, sizeof h, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(launch_rigid_classify(s->q, ra, s->stream));
+  rc = reset_history(s);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (!s->general) {
+    s->general = true;
+    s->tuned = false;
+  }
+  drop_graphs(s);
+  return LBM_OK;
+}
+
+int lbm_rigid_read(lbm_solve* s, lbm_rigid_state* out) {
+  if (!s || !out) return fail(LBM_E_INVALID, "null argument");
+  if (!s->rigid) return fail(LBM_E_STATE, "no rigid body attached");
+  RigidBody h{};
+  HIPCHK(hipMemcpyAsync(&h, s->rigid, sizeof h, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (int k = 0; k < 3; k++) {
+    out->x[k] = h.c[k]; out->v[k] = h.v[k]; out->L[k] = h.L[k]; out->omega[k] = h.w[k]; out->d[k] = h.d[k];
+    out->force[k] = h.force[k]; out->torque[k] = h.torque[k]; out->inertia[k] = h.I[k];
+  }
+  for (int k = 0; k < 4; k++) out->q[k] = h.q[k];
+  out->mass = h.mass;
   return LBM_OK;
 }
 

@@ -29,8 +29,9 @@ EXPORTS = (
     "lbm_amr_read_totals", "lbm_amr_info",
     "lbm_serve_create", "lbm_serve_destroy", "lbm_serve_submit", "lbm_serve_wait", "lbm_serve_read_macro",
     "lbm_serve_read_forces", "lbm_serve_release", "lbm_serve_drain", "lbm_serve_read_stats", "lbm_serve_clock",
-    "lbm_serve_submit_state", "lbm_serve_read_state",
+    "lbm_serve_submit_state", "lbm_serve_read_state", "lbm_rigid_attach", "lbm_rigid_read",
 )
+SHAPE = {"sphere": 0, "spheroid": 1, "roundbox": 2}
 GHOST = 2
 
 
@@ -59,6 +60,54 @@ class AmrDesc(ctypes.Structure):
         ("lead", ctypes.c_int32), ("explode_linear", ctypes.c_int32), ("orphans", ctypes.c_int32),
         ("interface_interp", ctypes.c_int32), ("reflux", ctypes.c_int32),
     ]
+
+
+class RigidBody(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int32), ("pinned", ctypes.c_int32),
+                ("a", ctypes.c_double), ("b", ctypes.c_double), ("c", ctypes.c_double), ("r", ctypes.c_double),
+                ("density", ctypes.c_double), ("x", ctypes.c_double * 3), ("q", ctypes.c_double * 4),
+                ("v", ctypes.c_double * 3), ("omega", ctypes.c_double * 3), ("gravity", ctypes.c_double * 3),
+                ("v_max", ctypes.c_double), ("o_max", ctypes.c_double),
+                ("planar", ctypes.c_int32), ("no_fluid_force", ctypes.c_int32), ("swept_force", ctypes.c_int32),
+                ("rho_w_local", ctypes.c_int32), ("window", ctypes.c_int32 * 3), ("reserved", ctypes.c_int32),
+                ("window_anchor", ctypes.c_double * 3)]
+
+
+class RigidState(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_double * 3), ("q", ctypes.c_double * 4), ("v", ctypes.c_double * 3),
+                ("L", ctypes.c_double * 3), ("omega", ctypes.c_double * 3), ("d", ctypes.c_double * 3),
+                ("force", ctypes.c_double * 3), ("torque", ctypes.c_double * 3),
+                ("mass", ctypes.c_double), ("inertia", ctypes.c_double * 3)]
+
+
+def make_rigid(p):
+    """Rigid-body parameters (the oracle's naming: shape {kind, a, b, c, r}, x,
+    q, v, omega, density, pinned, gravity, vMax, oMax, planar, noFluidForce,
+    swept, rhoWLocal, window, anchor) -> RigidBody."""
+    sh = p["shape"]
+    kind = sh["kind"] if isinstance(sh["kind"], int) else SHAPE[sh["kind"]]
+    b = RigidBody()
+    b.shape = kind
+    b.pinned = 1 if p.get("pinned") else 0
+    b.a = float(sh["a"])
+    b.b = float(sh.get("b", sh["a"]))
+    b.c = float(sh.get("c", sh["a"]))
+    b.r = float(sh.get("r", 0.0))
+    b.density = float(p.get("density", 1.0))
+    b.x[:] = [float(v) for v in p["x"]]
+    b.q[:] = [float(v) for v in p.get("q", (1.0, 0.0, 0.0, 0.0))]
+    b.v[:] = [float(v) for v in p.get("v", (0.0, 0.0, 0.0))]
+    b.omega[:] = [float(v) for v in p.get("omega", (0.0, 0.0, 0.0))]
+    b.gravity[:] = [float(v) for v in p.get("gravity", (0.0, 0.0, 0.0))]
+    b.v_max = float(p.get("vMax", 0.2))
+    b.o_max = float(p.get("oMax", 0.5))
+    b.planar = 1 if p.get("planar") else 0
+    b.no_fluid_force = 1 if p.get("noFluidForce") else 0
+    b.swept_force = 0 if p.get("swept") is False else 1
+    b.rho_w_local = 0 if p.get("rhoWLocal") is False else 1
+    b.window[:] = [1 if v else 0 for v in p.get("window", (0, 0, 0))]
+    b.window_anchor[:] = [float(v) for v in p.get("anchor", (0.0, 0.0, 0.0))]
+    return b
 
 
 class ServeDesc(ctypes.Structure):
@@ -118,6 +167,8 @@ def load():
         lib.lbm_time_step_kernel.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
         lib.lbm_state_size.argtypes = [P, ctypes.POINTER(ctypes.c_size_t)]
         lib.lbm_save_state.argtypes = [P, P, ctypes.c_size_t]
+        lib.lbm_rigid_attach.argtypes = [P, ctypes.POINTER(RigidBody)]
+        lib.lbm_rigid_read.argtypes = [P, ctypes.POINTER(RigidState)]
         lib.lbm_load_state.argtypes = [P, P, ctypes.c_size_t]
         lib.lbm_probe_copy_bandwidth.argtypes = [ctypes.c_size_t, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                                  ctypes.POINTER(ctypes.c_double)]
@@ -327,6 +378,16 @@ class Solve:
         ms = ctypes.c_double()
         _check(self.lib.lbm_time_step_kernel(self.h, int(n), ctypes.byref(ms)))
         return ms.value
+
+    def attach_rigid(self, params):
+        """Attaches a moving rigid body (see make_rigid for the parameters)."""
+        b = make_rigid(params)
+        _check(self.lib.lbm_rigid_attach(self.h, ctypes.byref(b)))
+
+    def read_rigid(self):
+        st = RigidState()
+        _check(self.lib.lbm_rigid_read(self.h, ctypes.byref(st)))
+        return {k: (list(getattr(st, k)) if k != "mass" else st.mass) for k, _ in RigidState._fields_}
 
 
 def make_amr_desc(d):

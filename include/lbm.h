@@ -187,6 +187,61 @@ int lbm_time_step_kernel(lbm_solve* s, int32_t n, double* ms_per_launch);
 int lbm_probe_copy_bandwidth(size_t bytes, int32_t reps, double* gbs, double* ms_per_copy);
 
 /* ------------------------------------------------------------------------
+ * Moving rigid body of a dense 3D solve (sharp bounce-back coupling).  The
+ * body state is FP64 on the device and advances every step, in the
+ * reference's order (main-3d.js:3109-3148): the fluid force and torque on
+ * the pre-stream populations at the current pose (momentum exchange with
+ * Ladd's moving-wall term at the local wall density, plus the swept-cell
+ * term, common_d3_force.wgsl:171-222), the 6-DOF integrator
+ * (d3_physics.wgsl:56-158, d3-body.mjs:234-297), the solid classification
+ * at the new pose, then the fluid step at the new pose (moving-wall
+ * bounce-back, the interior held at feq(1, u_body), the sponge in window
+ * coordinates, common_d3_step.wgsl:224-400).  The force records go to the
+ * solve's force history (lbm_read_forces).
+ * ------------------------------------------------------------------------ */
+enum { LBM_SHAPE_SPHERE = 0, LBM_SHAPE_SPHEROID = 1, LBM_SHAPE_ROUNDBOX = 2 };  /* d3-body.mjs:36 SHAPE */
+
+/* Replaces packBodyState / makeBodyState (d3-body.mjs:299-354) plus the
+ * physics pipeline's override constants (main-3d.js:1060-1070). */
+typedef struct lbm_rigid_body {
+  int32_t shape;          /* LBM_SHAPE_*                                          */
+  int32_t pinned;         /* 1: force measured, pose held                         */
+  double a, b, c, r;      /* sphere a; spheroid semi-axes (a, a, c); round box
+                             half-extents (a, b, c) and corner radius r           */
+  double density;         /* mass = density * volume                              */
+  double x[3];            /* centre of mass, cells                                */
+  double q[4];            /* orientation [w, x, y, z], body to world              */
+  double v[3];            /* velocity, cells per step                             */
+  double omega[3];        /* angular velocity (world), radians per step           */
+  double gravity[3];      /* acceleration (GX/GY/GZ), buoyancy-corrected          */
+  double v_max, o_max;    /* magnitude clamps on |v| and |omega|                  */
+  int32_t planar;         /* PLANAR: no z translation, rotation about z only      */
+  int32_t no_fluid_force; /* NO_FLUID_FORCE: force measured, never applied         */
+  int32_t swept_force;    /* SWEPT_FORCE: the swept-cell term (1 in the reference)  */
+  int32_t rho_w_local;    /* RHO_W_LOCAL: local wall density (1 in the reference) */
+  int32_t window[3];      /* moving-window axes: the body wraps, the sponge follows */
+  int32_t reserved;
+  double window_anchor[3]; /* WIN_AX/AY/AZ                                         */
+} lbm_rigid_body;
+
+typedef struct lbm_rigid_state {
+  double x[3], q[4], v[3];
+  double L[3];            /* angular momentum (world)                             */
+  double omega[3];        /* angular velocity at the last half step               */
+  double d[3];            /* displacement since attach (not wrapped)              */
+  double force[3], torque[3]; /* last measured fluid force and torque             */
+  double mass, inertia[3];
+} lbm_rigid_state;
+
+/* Attaches a moving body to a dense 3D solve created with body NONE
+ * (standalone solves only), classifies the cells at its pose and clears the
+ * force history.  Replaces writeBuffer(bodyBuf, packBodyState(...)),
+ * main-3d.js:2983-2990. */
+int lbm_rigid_attach(lbm_solve* s, const lbm_rigid_body* body);
+/* The body's current state (readback of BodyState3D). */
+int lbm_rigid_read(lbm_solve* s, lbm_rigid_state* out);
+
+/* ------------------------------------------------------------------------
  * Two-level block-structured AMR: the dense L0 lattice plus one pool of
  * fine tiles (refinement ratio 2).  L0 is cut into blocks of rb^3 cells; a
  * refined block owns a pool slot of fb^3 fine cells, fb = 2 rb + 4 (the

@@ -10,7 +10,10 @@
 //     "fIn":     "f.f32"       optional float32 plane-major populations
 //     "steps":   n,
 //     "threads": T             worker threads over x-rows (default 1)
-//     "out":     { "mac": path, "f": path, "force": path, "cls": path } }
+//     "out":     { "mac": path, "f": path, "force": path, "cls": path, "body": path },
+//     "rigid":   optional rigid-body parameters (rigid_oracle.js makeBody); the
+//                run is then single-threaded and "body" receives, per step,
+//                [x(3), q(4), v(3), L(3), w(3), d(3)] }
 //
 // Writes float64 little-endian files (mac: 4 per cell, f: Q planes, force:
 // 6 per step, cls: uint8 per cell) and prints one JSON line with the wall
@@ -21,6 +24,7 @@ const os = require('os');
 const path = require('path');
 const { Worker, isMainThread, parentPort, workerData } = require('worker_threads');
 const O = require('./lbm_oracle.js');
+const R = require('./rigid_oracle.js');
 
 function readF32(p) {
   const b = fs.readFileSync(p);
@@ -53,6 +57,7 @@ async function main() {
   const job = JSON.parse(fs.readFileSync(process.argv[2], 'utf8'));
   const base = path.dirname(path.resolve(process.argv[2]));
   const rel = (p) => path.resolve(base, p);
+  if (job.rigid) return rigidMain(job, rel);
   const S = O.normalise(job.desc);
   const T = O.tables(S.q, S.exactWeights);
   const N = S.nx * S.ny * S.nz;
@@ -130,4 +135,37 @@ async function main() {
     cpu_model: cpus.length ? cpus[0].model : 'unknown', cpu_count: cpus.length,
     totals: totals,
   }));
+}
+
+// A dense 3D solve with one moving rigid body (rigid_oracle.js).
+function rigidMain(job, rel) {
+  const o = new R.RigidOracle(job.desc, job.rigid);
+  const N = o.S.nx * o.S.ny * o.S.nz, Q = o.T.Q;
+  if (job.macIn) o.initEq(readF32(rel(job.macIn)));
+  const steps = job.steps | 0;
+  const t0 = process.hrtime.bigint();
+  o.step(steps);
+  const seconds = Number(process.hrtime.bigint() - t0) / 1e9;
+  const out = job.out || {};
+  if (out.mac) writeArr(rel(out.mac), o.mac);
+  if (out.f) writeArr(rel(out.f), o.f);
+  if (out.cls) writeArr(rel(out.cls), o.cls);
+  if (out.force) {
+    const F = new Float64Array(6 * steps);
+    o.forceHistory.forEach((r, s) => F.set(r, 6 * s));
+    writeArr(rel(out.force), F);
+  }
+  if (out.body) {
+    const B = new Float64Array(19 * steps);
+    o.bodyHistory.forEach((b, s) => B.set([].concat(b.x, b.q, b.v, b.L, b.w, b.d), 19 * s));
+    writeArr(rel(out.body), B);
+  }
+  const totals = [0, 0, 0, 0];
+  for (let i = 0; i < Q; i++) {
+    let sum = 0;
+    for (let c = 0; c < N; c++) sum += o.f[i * N + c];
+    totals[0] += sum; totals[1] += sum * o.T.ex[i]; totals[2] += sum * o.T.ey[i]; totals[3] += sum * o.T.ez[i];
+  }
+  console.log(JSON.stringify({ steps: steps, cells: N, q: Q, threads: 1, seconds: seconds, totals: totals,
+                               mass: o.B.mass, inertia: o.B.I }));
 }

@@ -14,15 +14,18 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 RUN_JS = os.path.join(ROOT, "oracle", "run.js")
 
 
-def run(desc, mac=None, f=None, steps=0, threads=1, want=("mac", "force"), timeout=900):
-    """desc: oracle descriptor dict; mac: float32 [rho,ux,uy,uz]*N; returns
+def run(desc, mac=None, f=None, steps=0, threads=1, want=("mac", "force"), timeout=900, rigid=None):
+    """desc: oracle descriptor dict; mac: float32 [rho,ux,uy,uz]*N; rigid:
+    rigid-body parameters (oracle/rigid_oracle.js makeBody) or None; returns
     {'mac': (N,4) f64, 'force': (steps,6) f64, 'f': (Q,N) f64, 'cls': (N,) u8,
-     'summary': {...}}."""
+     'body': (steps,19) f64 [x, q, v, L, w, d], 'summary': {...}}."""
     q = int(desc["q"])
     nz = 1 if q == 9 else int(desc["nz"])
     n = int(desc["nx"]) * int(desc["ny"]) * nz
     with tempfile.TemporaryDirectory(prefix="lbm_oracle_") as d:
         job = {"desc": desc, "steps": int(steps), "threads": int(threads), "out": {}}
+        if rigid is not None:
+            job["rigid"] = rigid
         if mac is not None:
             np.ascontiguousarray(mac, dtype=np.float32).tofile(os.path.join(d, "mac_in.f32"))
             job["macIn"] = "mac_in.f32"
@@ -46,4 +49,6 @@ def run(desc, mac=None, f=None, steps=0, threads=1, want=("mac", "force"), timeo
             out["f"] = np.fromfile(os.path.join(d, "f.bin"), dtype=np.float64).reshape(q, n)
         if "cls" in want:
             out["cls"] = np.fromfile(os.path.join(d, "cls.bin"), dtype=np.uint8)
+        if "body" in want:
+            out["body"] = np.fromfile(os.path.join(d, "body.bin"), dtype=np.float64).reshape(-1, 19)
         return out

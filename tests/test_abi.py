@@ -167,3 +167,35 @@ int main(void) {
         got = [int(v) for v in subprocess.check_output([exe]).split()]
     # the layout tests/test_state_gpu.py reads with struct "<8sIiiiiiI4xqQ"
     assert got == [64, 8, 28, 32, 40, 48]
+
+
+def test_rigid_struct_layouts_match_c():
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "lbm.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_rigid_body), offsetof(lbm_rigid_body, a),
+         offsetof(lbm_rigid_body, q), offsetof(lbm_rigid_body, v_max), offsetof(lbm_rigid_body, planar),
+         offsetof(lbm_rigid_body, window_anchor), sizeof(lbm_rigid_state), offsetof(lbm_rigid_state, mass));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "t")
+        subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe])
+        got = [int(v) for v in subprocess.check_output([exe]).split()]
+    B, S = pylbm.RigidBody, pylbm.RigidState
+    want = [ctypes.sizeof(B), B.a.offset, B.q.offset, B.v_max.offset, B.planar.offset, B.window_anchor.offset,
+            ctypes.sizeof(S), S.mass.offset]
+    assert got == want
+
+
+def test_rigid_attach_refusals_without_gpu():
+    lib = pylbm.load()
+    assert lib.lbm_rigid_attach(None, None) == -1
+    assert b"null" in lib.lbm_last_error()
+    st = pylbm.RigidState()
+    assert lib.lbm_rigid_read(None, ctypes.byref(st)) == -1
