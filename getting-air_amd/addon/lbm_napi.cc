@@ -373,6 +373,89 @@ napi_value Info(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// rigidAttach(solve, {shape: {kind, a, b, c, r}, x, q, v, omega, density | mass,
+//                     pinned, gravity, vMax, oMax, planar, noFluidForce, swept,
+//                     rhoWLocal, window, anchor})
+// The body object of d3-body.mjs makeBodyState (mass, not density) is taken as
+// it is; the flags are the physics pipeline's overrides (main-3d.js:1060-1070).
+napi_value RigidAttach(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  lbm_rigid_body b;
+  memset(&b, 0, sizeof b);
+  napi_value sh;
+  if (!get_prop(env, argv[1], "shape", &sh)) {
+    napi_throw_type_error(env, nullptr, "rigid body needs a shape");
+    return nullptr;
+  }
+  b.shape = (int)num(env, sh, "kind", 0);
+  b.a = num(env, sh, "a", 0);
+  b.b = num(env, sh, "b", b.a);
+  b.c = num(env, sh, "c", b.a);
+  b.r = num(env, sh, "r", 0);
+  const double pi = 3.14159265358979323846;
+  const double vol = b.shape == LBM_SHAPE_SPHERE ? (4.0 / 3.0) * pi * b.a * b.a * b.a
+                   : b.shape == LBM_SHAPE_SPHEROID ? (4.0 / 3.0) * pi * b.a * b.a * b.c : 8.0 * b.a * b.b * b.c;
+  napi_value v;
+  b.density = get_prop(env, argv[1], "mass", &v) && !get_prop(env, argv[1], "density", &v)
+                  ? num(env, argv[1], "mass", 0) / vol
+                  : num(env, argv[1], "density", 1.0);
+  num_array(env, argv[1], "x", b.x, 3);
+  b.q[0] = 1.0;
+  num_array(env, argv[1], "q", b.q, 4);
+  num_array(env, argv[1], "v", b.v, 3);
+  num_array(env, argv[1], "omega", b.omega, 3);
+  num_array(env, argv[1], "gravity", b.gravity, 3);
+  b.pinned = num(env, argv[1], "pinned", 0) != 0;
+  b.v_max = num(env, argv[1], "vMax", 0.2);
+  b.o_max = num(env, argv[1], "oMax", 0.5);
+  b.planar = num(env, argv[1], "planar", 0) != 0;
+  b.no_fluid_force = num(env, argv[1], "noFluidForce", 0) != 0;
+  b.swept_force = num(env, argv[1], "swept", 1) != 0;
+  b.rho_w_local = num(env, argv[1], "rhoWLocal", 1) != 0;
+  double w3[3] = {0, 0, 0};
+  num_array(env, argv[1], "window", w3, 3);
+  for (int k = 0; k < 3; k++) b.window[k] = w3[k] != 0;
+  num_array(env, argv[1], "anchor", b.window_anchor, 3);
+  int rc = lbm_rigid_attach(handle(env, argv[0]), &b);
+  if (rc) return throw_lbm(env, rc);
+  return nullptr;
+}
+
+napi_value RigidRead(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  lbm_rigid_state st;
+  int rc = lbm_rigid_read(handle(env, argv[0]), &st);
+  if (rc) return throw_lbm(env, rc);
+  napi_value o;
+  NAPI_OK(napi_create_object(env, &o));
+  auto arr = [&](const char* name, const double* p, int n) {
+    napi_value a, e;
+    napi_create_array_with_length(env, n, &a);
+    for (int k = 0; k < n; k++) {
+      napi_create_double(env, p[k], &e);
+      napi_set_element(env, a, k, e);
+    }
+    napi_set_named_property(env, o, name, a);
+  };
+  arr("x", st.x, 3);
+  arr("q", st.q, 4);
+  arr("v", st.v, 3);
+  arr("L", st.L, 3);
+  arr("omega", st.omega, 3);
+  arr("d", st.d, 3);
+  arr("force", st.force, 3);
+  arr("torque", st.torque, 3);
+  arr("inertia", st.inertia, 3);
+  napi_value m;
+  napi_create_double(env, st.mass, &m);
+  napi_set_named_property(env, o, "mass", m);
+  return o;
+}
+
 napi_value DeviceInfo(napi_env env, napi_callback_info) {
   char name[256] = {0};
   double hbm = 0;
@@ -928,6 +1011,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"serveSubmitState", ServeSubmitState},
       {"serveReadState", ServeReadState},
       {"saveState", SaveState},
+      {"rigidAttach", RigidAttach},
+      {"rigidRead", RigidRead},
       {"loadState", LoadState},
       {"serveWait", ServeWait},
       {"serveReadMacro", ServeReadMacro},

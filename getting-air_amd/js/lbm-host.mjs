@@ -237,9 +237,35 @@ export class DenseSolver {
     };
   }
 
+  // A moving rigid body (writeBuffer(bodyBuf, packBodyState(...)),
+  // main-3d.js:2983-2990): `body` is d3-body.mjs makeBodyState's object
+  // ({shape, x, q, v, omega, mass | density}) with the physics overrides
+  // {pinned, gravity, vMax, oMax, planar, noFluidForce, swept, rhoWLocal,
+  // window, anchor}.  Attach after reset(): the body restarts from `body`.
+  attachBody(body) {
+    this.addon.rigidAttach(this.handle, body);
+    this.rigid = true;
+  }
+
   // Last step's force and torque, with Cd/Cl/Cs = F / (1/2 U^2 A)
-  // (main-3d.js:4721-4738).
+  // (main-3d.js:4721-4738); with a rigid body also its BodyState3D fields.
   async readBody() {
+    if (this.rigid) {
+      const r = this.addon.rigidRead(this.handle);
+      const b = { step: this.step, mass: r.mass,
+                  fx: r.force[0], fy: r.force[1], fz: r.force[2], tx: r.torque[0], ty: r.torque[1], tz: r.torque[2] };
+      ['c', 'v', 'L', 'w', 'd'].forEach((k) => {
+        const src = { c: r.x, v: r.v, L: r.L, w: r.omega, d: r.d }[k];
+        ['x', 'y', 'z'].forEach((a, i) => { b[k + a] = src[i]; });
+      });
+      b.qw = r.q[0]; b.qx = r.q[1]; b.qy = r.q[2]; b.qz = r.q[3];
+      const u = this.params.u0, A = this.params.area;
+      if (u && A) {
+        const q = 0.5 * u * u * A;
+        b.cd = b.fx / q; b.cl = b.fy / q; b.cs = b.fz / q;
+      }
+      return b;
+    }
     if (!this.desc.body) return null;
     const f = this.addon.readForces(this.handle, 1);
     if (f.length < 6) return null;

@@ -7,6 +7,7 @@ import { makePool, refineWhere, manageSlots, activeSlotList } from '../../gettin
 
 const require = createRequire(import.meta.url);
 const O = require('../../oracle/lbm_oracle.js');
+const RO = require('../../oracle/rigid_oracle.js');
 
 let failures = 0;
 async function ok(name, fn) {
@@ -196,6 +197,29 @@ function relErr(gpuMac, oracleMac, comp) {
     assert(d1 === 0 && d2 === 0, 'differing values: resumed solver ' + d1 + ', resumed request ' + d2);
     assert(r.state && r.state.byteLength === st.byteLength, 'served request kept its state');
     return 'state ' + st.byteLength + ' bytes, 0 differing values';
+  });
+
+  await ok('moving body: a falling spheroid through attachBody/readBody matches the rigid-body oracle', async () => {
+    const desc = { q: 19, nx: 40, ny: 24, nz: 24, omega: 1 / 0.7 };
+    const N = 40 * 24 * 24;
+    const macro = () => { const m = new Float32Array(4 * N); for (let c = 0; c < N; c++) m[4 * c] = 1; return m; };
+    const body = { shape: { kind: 1, a: 4, c: 2.5 }, x: [12.4, 12.1, 11.8], q: RO.qFromAxisAngle([0, 1, 0], 0.6),
+                   density: 1.5, gravity: [2e-4, 0, 0] };
+    const s = new DenseSolver({ desc: desc, params: {}, macro: macro });
+    await s.reset();
+    s.attachBody(body);
+    await s.debugStepSync(80);
+    const b = await s.readBody();
+    const mac = await s.readMacro();
+    s.destroy();
+    const o = new RO.RigidOracle(desc, body);
+    o.initEq(macro());
+    o.step(80);
+    const eu = relErr(mac, o.mac, [1, 2, 3]);
+    const ex = Math.max(Math.abs(b.cx - o.B.x[0]), Math.abs(b.cy - o.B.x[1]), Math.abs(b.cz - o.B.x[2]));
+    const eq = Math.max(...[b.qw, b.qx, b.qy, b.qz].map((v, k) => Math.abs(v - o.B.q[k])));
+    assert(eu < 1e-5 && ex < 1e-6 && eq < 1e-6, 'u ' + eu + ' x ' + ex + ' q ' + eq);
+    return 'u ' + eu.toExponential(2) + ', x ' + ex.toExponential(2) + ', vx ' + b.vx.toExponential(3);
   });
 
   await ok('invalid descriptor raises with the library message', () => {
