@@ -104,9 +104,20 @@ __global__ void __launch_bounds__(256) k_rigid_force(const typename ST::T* __res
   using T = typename ST::T;
   __shared__ double lds[16 * 6];
   __shared__ bool last;
-  __shared__ double tot[6];
+  __shared__ double ahead[7];  // the pose one step ahead: centre, orientation
   const RigidBody* B = a.body;
   const int tid = threadIdx.x;
+  if (tid == 0 && B->swept) {
+    // by v and the cached omega (common_d3_geometry.wgsl get_phi3Ahead)
+    const double w[3] = {B->w[0], B->w[1], B->w[2]};
+    const double q[4] = {B->q[0], B->q[1], B->q[2], B->q[3]};
+    double e[4], qa[4];
+    exp_map(w, 1.0, e);
+    q_mul(e, q, qa);
+    for (int k = 0; k < 3; k++) ahead[k] = B->c[k] + B->v[k];
+    for (int k = 0; k < 4; k++) ahead[3 + k] = qa[k];
+  }
+  __syncthreads();
   double fv[6] = {0, 0, 0, 0, 0, 0};
   const uint32_t t = blockIdx.x * blockDim.x + tid;
   const uint32_t nbox = (uint32_t)a.fbox[0] * a.fbox[1] * a.fbox[2];
@@ -145,13 +156,8 @@ __global__ void __launch_bounds__(256) k_rigid_force(const typename ST::T* __res
         }
       }
       if (B->swept) {
-        // the pose one step ahead by v and the cached omega
-        double ca[3], e[4], qa[4];
-        for (int k = 0; k < 3; k++) ca[k] = B->c[k] + B->v[k];
-        const double w[3] = {B->w[0], B->w[1], B->w[2]};
-        const double q[4] = {B->q[0], B->q[1], B->q[2], B->q[3]};
-        exp_map(w, 1.0, e);
-        q_mul(e, q, qa);
+        const double ca[3] = {ahead[0], ahead[1], ahead[2]};
+        const double qa[4] = {ahead[3], ahead[4], ahead[5], ahead[6]};
         if (solid_now != (body_phi(B, ca, qa, x, y, z) < 0.0)) {
           double mx = 0, my = 0, mz = 0;
           for (int i = 0; i < Q; i++) {
@@ -182,13 +188,16 @@ __global__ void __launch_bounds__(256) k_rigid_force(const typename ST::T* __res
   __syncthreads();
   if (!last) return;
   __threadfence();
-  if (tid < 6) {
-    double s = 0.0;
-    for (unsigned j = 0; j < nblocks; j++)
-      s += __hip_atomic_load(&a.partials[(size_t)j * 6 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    tot[tid] = s;
+  // wave 0: lane l sums blocks l, l + 64, ... in order, then a fixed
+  // butterfly, so the totals repeat bit for bit
+  if (tid >= 64) return;
+  double tot[6];
+  for (int k = 0; k < 6; k++) {
+    double sk = 0.0;
+    for (unsigned j = tid; j < nblocks; j += 64)
+      sk += __hip_atomic_load(&a.partials[(size_t)j * 6 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    tot[k] = wave_sum(sk);
   }
-  __syncthreads();
   if (tid == 0) {
     *a.counter = 0u;
     const unsigned long long n = *a.written;
@@ -219,9 +228,12 @@ __global__ void __launch_bounds__(256) k_rigid_classify(RigidArgs a) {
   if (len3(r[0], r[1], r[2]) <= B->rc + 2.0) {
     const double c[3] = {B->c[0], B->c[1], B->c[2]};
     const double q[4] = {B->q[0], B->q[1], B->q[2], B->q[3]};
-    if (body_phi(B, c, q, x, y, z) < 0.0) {
+    const double phi = body_phi(B, c, q, x, y, z);
+    if (phi < 0.0) {
       cl = 1;
-    } else {
+    } else if (phi < 1.75) {
+      // a signed distance is 1-Lipschitz and a link at most sqrt(3) long, so
+      // only cells this close can pull from a solid cell
       for (int i = 1; i < Q; i++)
         if (body_phi(B, c, q, x - L::ex[i], y - L::ey[i], z - L::ez[i]) < 0.0) lm |= 1u << i;
       if (lm) cl = 2;
