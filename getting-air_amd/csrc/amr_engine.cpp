@@ -30,6 +30,7 @@ struct lbm_amr {
   PoolGeom G{};
   int fb = 0;
   int nblocks = 0;
+  size_t pool_cells = 0;  // max_slots * fb^3: the compact plane of the host arrays
   float* pool[2] = {nullptr, nullptr};
   float* mac_pool = nullptr;
   int32_t* block_slot = nullptr;
@@ -299,7 +300,12 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
   G.nbx = desc->nx / rb; G.nby = desc->ny / rb; G.nbz = desc->nz / rb;
   G.ncells = (uint32_t)((size_t)desc->nx * desc->ny * desc->nz);
   G.max_slots = desc->max_slots;
-  G.pool_plane = (uint32_t)((size_t)desc->max_slots * a->fb * a->fb * a->fb);
+  // Planes are padded by 4 KiB: the plane size max_slots * fb^3 is a
+  // multiple of 2^14 elements for every power-of-two slot count, and the
+  // pool step's Q streams at one plane stride apart would share their low
+  // address bits (the same DRAM channels and banks).
+  a->pool_cells = (size_t)desc->max_slots * a->fb * a->fb * a->fb;
+  G.pool_plane = (uint32_t)(a->pool_cells + 1024);
   a->nblocks = G.nbx * G.nby * G.nbz;
   plan_capacity(rb, desc->max_slots, G.ncells, &a->plan.cap_seam, &a->plan.cap_near, &a->plan.cap_ring);
   const size_t pool_bytes = (size_t)G.pool_plane * desc->q * 4;
@@ -457,7 +463,9 @@ int lbm_amr_read_macro(lbm_amr* a, float* mac4) {
 
 int lbm_amr_read_pool_macro(lbm_amr* a, float* mac_pool) {
   if (!a || !mac_pool) return fail(LBM_E_INVALID, "null argument");
-  HIPCHK(hipMemcpy(mac_pool, a->mac_pool, (size_t)a->G.pool_plane * 16, hipMemcpyDeviceToHost));
+  for (int k = 0; k < 4; k++)
+    HIPCHK(hipMemcpy(mac_pool + (size_t)k * a->pool_cells, a->mac_pool + (size_t)k * a->G.pool_plane,
+                     a->pool_cells * 4, hipMemcpyDeviceToHost));
   return LBM_OK;
 }
 
@@ -468,8 +476,10 @@ int lbm_amr_read_f(lbm_amr* a, float* f_l0, float* f_pool) {
     if (rc) return rc;
   }
   if (f_pool) {
-    const size_t n = (size_t)a->G.pool_plane;
-    HIPCHK(hipMemcpy(f_pool, a->pool[0], n * a->d.q * 4, hipMemcpyDeviceToHost));
+    const size_t n = a->pool_cells;
+    for (int i = 0; i < a->d.q; i++)
+      HIPCHK(hipMemcpy(f_pool + (size_t)i * n, a->pool[0] + (size_t)i * a->G.pool_plane, n * 4,
+                       hipMemcpyDeviceToHost));
     for (int i = 0; i < a->d.q; i++) {
       const float w = a->d.q == 19 ? Lattice<19>::w[i] : Lattice<27>::w[i];
       float* p = f_pool + (size_t)i * n;
