@@ -437,7 +437,7 @@ int autotune(lbm_solve* s) {
         // linear tiles only need a thread count: one shape per count
         if (lin && sh[1] != 1) continue;
         if (!lin && sh[0] > 64 && s->d.nx < sh[0] / 2) continue;
-        for (int occ : {0, 4}) {
+        for (int occ : {0, 4, 6}) {
           if (!occupancy_cap_ok(occ, sh[0] * sh[1])) continue;
           for (int swz = 0; swz < 2; swz++)
             for (int nt = 0; nt < 2; nt++) {
@@ -628,9 +628,11 @@ int state_header_check(const lbm_desc* d, const void* buf, size_t bytes, lbm_sta
                 h.q, h.nx, h.ny, h.nz, h.storage, d->q, d->nx, d->ny, nz, d->storage);
   const size_t elem = d->storage == LBM_STORE_F32 ? 4 : 2;
   const size_t payload = (size_t)d->nx * d->ny * nz * d->q * elem;
-  if (h.payload_bytes != payload || bytes < sizeof h + payload)
+  if (h.payload_bytes != payload || bytes < sizeof h + payload + h.body_bytes)
     return fail(LBM_E_INVALID, "state payload of %llu bytes (buffer %zu), the solve needs %zu",
                 (unsigned long long)h.payload_bytes, bytes, payload);
+  if (h.body_bytes != 0 && h.body_bytes != sizeof(RigidBody))
+    return fail(LBM_E_INVALID, "rigid body record of %u bytes, expected %zu", h.body_bytes, sizeof(RigidBody));
   if (h.step < 0) return fail(LBM_E_INVALID, "state step %lld", (long long)h.step);
   if (out) *out = h;
   return LBM_OK;
@@ -657,11 +659,12 @@ void default_tile(int storage, int nx, int* bx, int* by, int* flags) {
   // wave-uniform, fewer per-lane instructions); other widths take linear
   // tiles, which leave no lane idle.  Those two layouts number their tiles
   // alike when nx is a multiple of 256.  fp32 (256 B per wave load) runs
-  // best with residency capped at 4 waves per SIMD, the 16-bit formats want
-  // every wave they can get (D3Q19 sphere tile sweep on MI355X).
+  // best with residency capped at 6 waves per SIMD (104.9 us against 106.8
+  // at 4 and 108.4 uncapped), the 16-bit formats want every wave they can
+  // get (D3Q19 sphere tile sweep on MI355X, block layout).
   *bx = 256;
   *by = 1;
-  *flags = 0x2 | (nx > 0 && nx % 256 == 0 ? 0 : 0x4) | (storage == LBM_STORE_F32 ? 0x40 : 0);
+  *flags = 0x2 | (nx > 0 && nx % 256 == 0 ? 0 : 0x4) | (storage == LBM_STORE_F32 ? 0x60 : 0);
 }
 
 int solve_create_in(const lbm_desc* d, Arena* arena, hipStream_t stream, int bx, int by, int flags,
@@ -787,23 +790,30 @@ static void blocks_to_planes(char* lat, char* planes, int q, uint32_t n, int ele
 
 int lbm_state_size(lbm_solve* s, size_t* bytes) {
   if (!s || !bytes) return fail(LBM_E_INVALID, "null argument");
-  *bytes = sizeof(lbm_state_header) + (size_t)s->N * s->q * s->elem;
+  *bytes = sizeof(lbm_state_header) + (size_t)s->N * s->q * s->elem + (s->rigid ? sizeof(RigidBody) : 0);
   return LBM_OK;
 }
 
 int lbm_save_state(lbm_solve* s, void* buf, size_t bytes) {
   if (!s || !buf) return fail(LBM_E_INVALID, "null argument");
-  if (s->rigid) return fail(LBM_E_STATE, "the state of a solve with a rigid body is not checkpointed");
   if (!s->initialised) return fail(LBM_E_STATE, "save_state before upload_macro / upload_f / load_state");
   const size_t payload = (size_t)s->N * s->q * s->elem;
-  if (bytes < sizeof(lbm_state_header) + payload)
-    return fail(LBM_E_INVALID, "state buffer of %zu bytes, need %zu", bytes, sizeof(lbm_state_header) + payload);
+  const size_t body = s->rigid ? sizeof(RigidBody) : 0;
+  if (bytes < sizeof(lbm_state_header) + payload + body)
+    return fail(LBM_E_INVALID, "state buffer of %zu bytes, need %zu", bytes,
+                sizeof(lbm_state_header) + payload + body);
   // The payload holds plane-major populations ([Q][cells]); the lattice's
   // blocks are reordered on the host.
   std::vector<char> lat(lattice_elems(s->q, s->N) * s->elem);
   HIPCHK(hipMemcpyAsync(lat.data(), s->f[s->parity], lat.size(), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   blocks_to_planes(lat.data(), static_cast<char*>(buf) + sizeof(lbm_state_header), s->q, s->N, s->elem, true);
+  if (body) {
+    // the body's record after the populations (its pose matches the lattice)
+    HIPCHK(hipMemcpyAsync(static_cast<char*>(buf) + sizeof(lbm_state_header) + payload, s->rigid, body,
+                          hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
   lbm_state_header h{};
   memcpy(h.magic, "LBMSTATE", 8);
   h.version = 1;
@@ -813,6 +823,7 @@ int lbm_save_state(lbm_solve* s, void* buf, size_t bytes) {
   h.salt = solve_salt(s);
   h.step = s->step;
   h.payload_bytes = payload;
+  h.body_bytes = (uint32_t)body;
   memcpy(buf, &h, sizeof h);
   return LBM_OK;
 }
@@ -822,6 +833,9 @@ int lbm_load_state(lbm_solve* s, const void* buf, size_t bytes) {
   lbm_state_header h{};
   int rc = state_header_check(&s->d, buf, bytes, &h);
   if (rc) return rc;
+  if ((h.body_bytes != 0) != (s->rigid != nullptr))
+    return fail(LBM_E_INVALID, h.body_bytes ? "the state carries a rigid body; attach one before loading it"
+                                            : "the solve has a rigid body and the state carries none");
   std::vector<char> lat(lattice_elems(s->q, s->N) * s->elem, 0);
   blocks_to_planes(lat.data(), const_cast<char*>(static_cast<const char*>(buf)) + sizeof(lbm_state_header), s->q, s->N,
                    s->elem, false);
@@ -829,6 +843,14 @@ int lbm_load_state(lbm_solve* s, const void* buf, size_t bytes) {
   HIPCHK(hipMemcpyAsync(s->f[1], s->f[0], lat.size(), hipMemcpyDeviceToDevice, s->stream));
   const uint32_t words[2] = {h.salt, h.salt};
   HIPCHK(hipMemcpyAsync(s->salt, words, 8, hipMemcpyHostToDevice, s->stream));
+  if (s->rigid) {
+    // the saved record, then the cells of its pose (everything else cleared)
+    HIPCHK(hipMemcpyAsync(s->rigid, static_cast<const char*>(buf) + sizeof(lbm_state_header) + h.payload_bytes,
+                          sizeof(RigidBody), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemsetAsync(s->cls, 0, s->N, s->stream));
+    HIPCHK(hipMemsetAsync(s->links, 0, (size_t)s->N * 4, s->stream));
+    HIPCHK(launch_rigid_classify(s->q, s->rargs, s->stream));
+  }
   HIPCHK(hipStreamSynchronize(s->stream));
   s->parity = 0;
   s->step = h.step;

@@ -628,8 +628,10 @@ int lbm_serve_submit_state(lbm_serve* sv, const lbm_desc* desc, const float* mac
     return fail(LBM_E_NOMEM, "request needs %zu bytes, more than the whole budget (%.0f)", r->footprint,
                 sv->d.budget_bytes);
   if (state) {
-    int rc = state_header_check(&d, state, state_bytes, nullptr);
+    lbm_state_header h{};
+    int rc = state_header_check(&d, state, state_bytes, &h);
     if (rc) return rc;
+    if (h.body_bytes) return fail(LBM_E_INVALID, "a serve request cannot carry a rigid body");
     const uint8_t* b = static_cast<const uint8_t*>(state);
     r->state_in.assign(b, b + state_bytes);
   } else {

@@ -114,10 +114,14 @@ typedef struct lbm_state_header {
   uint32_t salt;         /* dither counter of the next step                     */
   int64_t step;          /* steps done                                          */
   uint64_t payload_bytes;/* q * ncells * element bytes that follow              */
-  uint8_t reserved[8];
+  uint32_t body_bytes;   /* 0, or the size of the moving rigid body's record
+                            (FP64 state and parameters) after the populations  */
+  uint8_t reserved[4];
 } lbm_state_header;
 
-/* Bytes of this solve's state (header + populations). */
+/* Bytes of this solve's state (header + populations [+ rigid body]).  A
+ * state with a rigid body loads only into a solve with a body attached, whose
+ * record it replaces; one without loads only into a solve without. */
 int lbm_state_size(lbm_solve* s, size_t* bytes);
 /* Writes the state after the work enqueued so far; buf holds >= state_size. */
 int lbm_save_state(lbm_solve* s, void* buf, size_t bytes);

@@ -219,7 +219,7 @@ def test_sphere_full_config_short_run_and_symmetry():
 
 
 TILES = [(64, 1, 1), (256, 1, 2), (128, 2, 0x43), (64, 4, 0x40), (64, 1, 0x42), (32, 8, 0),
-         (256, 1, 0x4), (64, 1, 0x6), (128, 1, 0x47), (256, 1, 0x46)]
+         (256, 1, 0x4), (64, 1, 0x6), (128, 1, 0x47), (256, 1, 0x46), (256, 1, 0x62), (256, 1, 0x66)]
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])

@@ -175,6 +175,39 @@ def test_rigid_refusals():
         with pytest.raises(pylbm.LbmError):
             s.read_rigid()
         s.upload_macro(uniform_mac(16 ** 3, (0.0, 0.0, 0.0)))
+        st = s.save_state()
         s.attach_rigid({"shape": {"kind": 0, "a": 3.0}, "x": [8, 8, 8]})
         with pytest.raises(pylbm.LbmError):
-            s.save_state()
+            s.load_state(st)  # a state without a body into a solve with one
+
+
+def test_checkpoint_with_a_rigid_body_resumes_bit_for_bit():
+    """30 steps, save, 40 more; a fresh solve with the body attached, loaded
+    from the state, stepped 40: the same field, force records and body."""
+    nx, ny, nz = 40, 24, 24
+    desc = {"q": 19, "nx": nx, "ny": ny, "nz": nz, "omega": 1.0 / 0.7}
+    rigid = {"shape": {"kind": 2, "a": 5.0, "b": 3.0, "c": 2.0, "r": 0.5}, "x": [12.4, 12.1, 11.8],
+             "q": [float(np.cos(0.3)), 0.0, float(np.sin(0.3)), 0.0], "density": 1.4, "gravity": [2e-4, 0.0, 0.0],
+             "omega": [0.0, 0.0, 0.004]}
+    mac = uniform_mac(nx * ny * nz, (0.0, 0.0, 0.0))
+    print("case rigid checkpoint", flush=True)
+    with pylbm.Solve(desc) as s:
+        s.upload_macro(mac)
+        s.attach_rigid(rigid)
+        s.step(30)
+        st = s.save_state()
+        s.step(40)
+        s.sync()
+        ma, fa, ba = s.read_macro(), s.read_forces(), s.read_rigid()
+    with pylbm.Solve(desc) as s:
+        with pytest.raises(pylbm.LbmError):
+            s.load_state(st)  # no body attached
+        s.attach_rigid(dict(rigid, x=[20.0, 12.0, 12.0]))
+        s.load_state(st)
+        s.step(40)
+        s.sync()
+        mb, fb, bb = s.read_macro(), s.read_forces(), s.read_rigid()
+    assert np.array_equal(ma, mb)
+    assert np.array_equal(fa[-40:], fb)
+    for k in ("x", "q", "v", "L", "omega"):
+        assert bb[k] == ba[k], k
