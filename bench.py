@@ -128,6 +128,29 @@ def measure_towed(steps, warmup, kernel_reps, re=36.0):
             "travel_cells": -st["d"][0], "finite": bool(np.all(np.isfinite(forces)))}
 
 
+def measure_plate():
+    """The reference's blunt-plate gate (benchmarks/d3.json plate_cases, C16):
+    a 16 x 16 x 2 plate normal to the flow at Re 200 (tau 0.512), pinned in a
+    -0.05 stream and half-towed (tow 0.025, stream -0.025), Cd averaged as its
+    harness does (the last 30 % of 20 chord/U)."""
+    out = {}
+    for name, tow, stream in (("pinned", 0.0, -0.05), ("half_tow", 0.025, -0.025)):
+        desc, mac, rigid, meta = sc.plate(tow=tow, stream=stream)
+        settle = int(round(20 * meta["convective"]))
+        warm = int(round(settle * 0.7))
+        with pylbm.Solve(desc) as s:
+            s.upload_macro(mac)
+            s.attach_rigid(rigid)
+            s.step(warm)
+            s.step(settle - warm)
+            s.sync()
+            f = s.read_forces(settle - warm)
+        out[name] = float((meta["sign"] * f[:, 0] / (0.5 * meta["u_rel"] ** 2 * meta["area"])).mean())
+    return {"cd_pinned": out["pinned"], "cd_pinned_reference_measured": 1.2081, "cd_literature": 1.18,
+            "cd_half_tow": out["half_tow"], "half_tow_vs_pinned": out["half_tow"] / out["pinned"] - 1.0,
+            "half_tow_vs_pinned_reference_measured": -0.0321}
+
+
 def cpu_baseline(steps, threads):
     import oracle_runner
 
@@ -399,10 +422,12 @@ def main():
             extra[st + "_store"] = r
             print("%s: %s" % (st, json.dumps(r)), file=sys.stderr, flush=True)
 
-    towed = None
+    towed = plate = None
     if not args.no_towed:
         towed = measure_towed(args.steps, args.warmup, 50)
         print("towed sphere: %s" % json.dumps(towed), file=sys.stderr, flush=True)
+        plate = measure_plate()
+        print("plate: %s" % json.dumps(plate), file=sys.stderr, flush=True)
 
     extra_amr_interp = None
     if not args.no_amr:
@@ -482,6 +507,8 @@ def main():
             "cd_towed_last1000": towed["cd_last1000"], "cd_pinned_last1000": towed["cd_pinned_last1000"],
             "cd_towed_vs_pinned": round(towed["cd_last1000"] / towed["cd_pinned_last1000"] - 1.0, 4),
             "travel_cells": round(towed["travel_cells"], 3), "finite": towed["finite"]}
+    if plate:
+        out["plate_d3q19_c16_re200"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in plate.items()}
     if extra_turb:
         out["turbulence_d3q19_512"] = extra_turb
     if extra_serve:

@@ -82,6 +82,50 @@ def sphere(nx=256, ny=128, nz=128, radius=12.0, centre=(64.0, 64.0, 64.0), u0=0.
     return desc, mac.reshape(-1), meta
 
 
+def plate(n=16, re=200.0, u_t=0.05, aspect=0.125, tow=0.0, stream=-0.05, i_star=0.17):
+    """The blunt square plate normal to the flow with prescribed relative
+    motion (SCENARIOS.card with tow or stream, span 1, tilt 0;
+    d3-scenarios.mjs:703-903): chord n, thickness aspect * n, Re on the chord,
+    a rounded box with r = 0 turned by 120 degrees about (1, 1, 1) so that its
+    thickness lies along x, placed with its faces on cell faces (the centre on
+    a cell corner or centre as the half-extent is integer or not), in an 8n
+    cube with a sponge of width max(6, n/2) toward `stream`.  `tow` moves the
+    body (force measured, never applied; moving window on x anchored at the
+    start), `stream` moves the fluid; only stream - tow is physical.  BGK.
+    Returns (desc, mac, rigid, meta); meta["sign"] * F_x / (1/2 U^2 A) is Cd."""
+    a = n / 2.0
+    b = n / 2.0
+    c = aspect * n / 2.0
+    nu = 2.0 * u_t * a / re
+    tau = nu / CS2 + 0.5
+    rho_b = max(1.05, i_star * 2.0 * a ** 3 / (c * (a * a + c * c)))
+    d = [8 * n, 8 * n, int(round(max(8 * n, 2 * n + 6 * n)))]
+    towed, streamed = tow != 0.0, stream != 0.0
+    rel_sign = math.copysign(1.0, tow - stream)
+    x0 = d[0] - 2 * n if rel_sign >= 0 else 2 * n
+
+    def snap(centre, h):
+        return round(centre) + 0.5 if float(h).is_integer() else float(round(centre))
+
+    place = [snap(x0, c), snap(d[1] / 2.0, a), snap(d[2] / 2.0, b)]
+    rigid = {"shape": {"kind": 2, "a": a, "b": b, "c": c, "r": 0.0}, "x": place, "q": [0.5, 0.5, 0.5, 0.5],
+             "v": [tow, 0.0, 0.0], "density": rho_b, "pinned": streamed and not towed, "noFluidForce": towed,
+             "window": [1, 0, 0] if towed else [0, 0, 0], "anchor": [math.floor(v) for v in place]}
+    desc = {"q": 19, "nx": d[0], "ny": d[1], "nz": d[2], "omega": f32(1.0 / tau),
+            "sponge": {"width": float(max(6, round(n / 2.0))), "axes": [1, 1, 1], "u": [stream, 0.0, 0.0]},
+            "solidEq": 1}
+    ncell = d[0] * d[1] * d[2]
+    mac = np.empty((ncell, 4), dtype=np.float32)
+    mac[:, 0] = 1.0 - stream * stream / (2.0 * CS2)
+    mac[:, 1] = stream
+    mac[:, 2] = 0.0
+    mac[:, 3] = 0.0
+    u_rel = abs(stream - tow)
+    meta = {"tau": tau, "area": 4.0 * a * b, "u_rel": u_rel, "sign": math.copysign(1.0, stream - tow),
+            "convective": max(1, round(n / u_rel))}
+    return desc, mac.reshape(-1), rigid, meta
+
+
 def beltrami(n=32, q=19, tau=0.7, amp=(0.02, 0.015, 0.01)):
     """Periodic ABC (Beltrami) flow, curl u = k u, exact decay
     u(t) = u0 exp(-nu k^2 t) (d3-scenarios.mjs:19-27)."""
