@@ -146,9 +146,24 @@ def measure_plate():
             s.sync()
             f = s.read_forces(settle - warm)
         out[name] = float((meta["sign"] * f[:, 0] / (0.5 * meta["u_rel"] ** 2 * meta["area"])).mean())
-    return {"cd_pinned": out["pinned"], "cd_pinned_reference_measured": 1.2081, "cd_literature": 1.18,
-            "cd_half_tow": out["half_tow"], "half_tow_vs_pinned": out["half_tow"] / out["pinned"] - 1.0,
-            "half_tow_vs_pinned_reference_measured": -0.0321}
+    # the reference's dense sphere pair at tau 0.6 (D 12, U 0.025, 16 D/U;
+    # benchmarks/d3.json tow_amr_note): pinned 5.8957, towed 5.8481
+    for name, tow, stream in (("sphere_pinned", 0.0, -0.025), ("sphere_towed", 0.025, 0.0)):
+        desc, mac, rigid, meta = sc.fall(tow=tow, stream=stream)
+        settle = int(round(16 * meta["convective"]))
+        warm = int(round(settle * 0.7))
+        with pylbm.Solve(desc) as s:
+            s.upload_macro(mac)
+            s.attach_rigid(rigid)
+            s.step(settle)
+            s.sync()
+            f = s.read_forces(settle - warm)
+        out[name] = float((meta["sign"] * f[:, 0] / (0.5 * meta["u_rel"] ** 2 * meta["area"])).mean())
+    return {"plate_cd_pinned": out["pinned"], "plate_cd_pinned_reference_measured": 1.2081, "plate_cd_literature": 1.18,
+            "plate_cd_half_tow": out["half_tow"], "plate_half_tow_vs_pinned": out["half_tow"] / out["pinned"] - 1.0,
+            "plate_half_tow_vs_pinned_reference_measured": -0.0321,
+            "sphere_tau06_cd_pinned": out["sphere_pinned"], "sphere_tau06_cd_pinned_reference_measured": 5.8957,
+            "sphere_tau06_cd_towed": out["sphere_towed"], "sphere_tau06_cd_towed_reference_measured": 5.8481}
 
 
 def cpu_baseline(steps, threads):
@@ -508,7 +523,7 @@ def main():
             "cd_towed_vs_pinned": round(towed["cd_last1000"] / towed["cd_pinned_last1000"] - 1.0, 4),
             "travel_cells": round(towed["travel_cells"], 3), "finite": towed["finite"]}
     if plate:
-        out["plate_d3q19_c16_re200"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in plate.items()}
+        out["reference_body_cases"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in plate.items()}
     if extra_turb:
         out["turbulence_d3q19_512"] = extra_turb
     if extra_serve:

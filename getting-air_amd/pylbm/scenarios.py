@@ -126,6 +126,38 @@ def plate(n=16, re=200.0, u_t=0.05, aspect=0.125, tow=0.0, stream=-0.05, i_star=
     return desc, mac.reshape(-1), rigid, meta
 
 
+def fall(n=12, re=14.4, u_t=0.04, tow=0.0, stream=-0.025, rho_b=4.0):
+    """A sphere of diameter n with prescribed relative motion (SCENARIOS.fall
+    with tow or stream, d3-scenarios.mjs:422-600): nu = u_t n / Re, a 16n x 6n
+    x 6n box, the sphere 2n from the downstream face, sponge of width
+    max(6, n/2) toward `stream`, moving window on x for a towed body anchored
+    at its start.  BGK.  Returns (desc, mac, rigid, meta) like plate()."""
+    nu = u_t * n / re
+    tau = nu / CS2 + 0.5
+    d = [16 * n, 6 * n, 6 * n]
+    towed, streamed = tow != 0.0, stream != 0.0
+    rel_sign = math.copysign(1.0, tow - stream)
+    x0 = d[0] - 2 * n if rel_sign >= 0 else 2 * n
+    place = [float(x0), d[1] / 2.0, d[2] / 2.0]
+    rigid = {"shape": {"kind": 0, "a": n / 2.0}, "x": place, "v": [tow, 0.0, 0.0], "density": rho_b,
+             "pinned": streamed and not towed, "noFluidForce": towed,
+             "window": [0, 0, 0] if (streamed and not towed) else [1, 0, 0],
+             "anchor": [math.floor(v) for v in place]}
+    desc = {"q": 19, "nx": d[0], "ny": d[1], "nz": d[2], "omega": f32(1.0 / tau),
+            "sponge": {"width": float(max(6, round(n / 2.0))), "axes": [1, 1, 1], "u": [stream, 0.0, 0.0]},
+            "solidEq": 1}
+    ncell = d[0] * d[1] * d[2]
+    mac = np.empty((ncell, 4), dtype=np.float32)
+    mac[:, 0] = 1.0 - stream * stream / (2.0 * CS2)
+    mac[:, 1] = stream
+    mac[:, 2] = 0.0
+    mac[:, 3] = 0.0
+    u_rel = abs(stream - tow)
+    meta = {"tau": tau, "area": math.pi * (n / 2.0) ** 2, "u_rel": u_rel, "sign": math.copysign(1.0, stream - tow),
+            "convective": max(1, round(n / u_rel))}
+    return desc, mac.reshape(-1), rigid, meta
+
+
 def beltrami(n=32, q=19, tau=0.7, amp=(0.02, 0.015, 0.01)):
     """Periodic ABC (Beltrami) flow, curl u = k u, exact decay
     u(t) = u0 exp(-nu k^2 t) (d3-scenarios.mjs:19-27)."""

@@ -213,12 +213,12 @@ def test_checkpoint_with_a_rigid_body_resumes_bit_for_bit():
         assert bb[k] == ba[k], k
 
 
-def plate_cd(tow, stream, settle_convective=20):
-    """The reference's prescribed-plate protocol (tools/lib/d3-body-metrics.js
-    runPrescribedCase): settle 70 % of settle_convective chord/U, then the
-    mean of sign * F_x / (1/2 U^2 A) over the remaining 30 %."""
+def plate_cd(tow, stream, settle_convective=20, scenario="plate"):
+    """The reference's prescribed-body protocol (tools/lib/d3-body-metrics.js
+    runPrescribedCase): settle 70 % of settle_convective D/U, then the mean of
+    sign * F_x / (1/2 U^2 A) over the remaining 30 %."""
     from pylbm import scenarios as sc
-    desc, mac, rigid, meta = sc.plate(tow=tow, stream=stream)
+    desc, mac, rigid, meta = getattr(sc, scenario)(tow=tow, stream=stream)
     settle = int(round(settle_convective * meta["convective"]))
     warm = int(round(settle * 0.7))
     with pylbm.Solve(desc) as s:
@@ -250,3 +250,18 @@ def test_plate_matches_the_reference_measurements():
     assert abs(cd_pin / 1.2081 - 1.0) < 0.02
     assert abs(cd_pin / 1.18 - 1.0) < 0.10
     assert abs(cd_half / cd_pin - 1.0) < 0.06
+
+
+def test_towed_sphere_matches_the_reference_pair_at_tau_06():
+    """benchmarks/d3.json tow_amr_note: the dense pair at tau 0.6 (D 12,
+    U 0.025, 16 D/U) measured pinned 5.8957, towed 5.8481 (-0.8 %); the
+    reference gates its pair at 3 %."""
+    print("case fall pinned D12 tau 0.6", flush=True)
+    cd_pin, _, _ = plate_cd(0.0, -0.025, 16, "fall")
+    print("case fall towed D12 tau 0.6", flush=True)
+    cd_tow, _, _ = plate_cd(0.025, 0.0, 16, "fall")
+    print("sphere Cd pinned %.4f (reference 5.8957), towed %.4f (reference 5.8481), %.2f %%"
+          % (cd_pin, cd_tow, 100 * (cd_tow / cd_pin - 1)))
+    assert abs(cd_pin / 5.8957 - 1.0) < 0.005
+    assert abs(cd_tow / 5.8481 - 1.0) < 0.01
+    assert abs(cd_tow / cd_pin - 1.0) < 0.03
