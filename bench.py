@@ -159,7 +159,17 @@ def measure_plate():
             s.sync()
             f = s.read_forces(settle - warm)
         out[name] = float((meta["sign"] * f[:, 0] / (0.5 * meta["u_rel"] ** 2 * meta["area"])).mean())
-    return {"plate_cd_pinned": out["pinned"], "plate_cd_pinned_reference_measured": 1.2081, "plate_cd_literature": 1.18,
+    # the reference's recorded dense pinned sphere (sphere-Re20-D16): 2.9436
+    desc, mac, _ = sc.sphere(nx=192, ny=128, nz=128, radius=8.0, centre=(48.0, 64.0, 64.0), u0=0.05, re=20.0,
+                             trt=False, sponge_width=8)
+    with pylbm.Solve(desc) as s:
+        s.upload_macro(mac)
+        s.step(30 * 320)
+        s.sync()
+        f = s.read_forces(1)
+    out["sphere_re20"] = float(f[-1, 0] / (0.5 * 0.05 ** 2 * np.pi * 64.0))
+    return {"sphere_re20_d16_cd": out["sphere_re20"], "sphere_re20_d16_cd_reference_measured": 2.9436,
+            "plate_cd_pinned": out["pinned"], "plate_cd_pinned_reference_measured": 1.2081, "plate_cd_literature": 1.18,
             "plate_cd_half_tow": out["half_tow"], "plate_half_tow_vs_pinned": out["half_tow"] / out["pinned"] - 1.0,
             "plate_half_tow_vs_pinned_reference_measured": -0.0321,
             "sphere_tau06_cd_pinned": out["sphere_pinned"], "sphere_tau06_cd_pinned_reference_measured": 5.8957,

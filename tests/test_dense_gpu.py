@@ -255,3 +255,23 @@ def test_tile_shapes_and_flags_give_the_same_field(storage, case):
             # 7 waves per SIMD of 1-wave blocks: 28 blocks per CU, and an LDS
             # share of L / 28 (rounded to 1 KiB) would still admit a 29th
             s.set_tile(64, 1, 0x70)
+
+
+def test_sphere_re20_matches_the_reference_measurement():
+    """benchmarks/d3.json sphere-Re20-D16 (SCENARIOS.sphere, d3-scenarios.mjs:
+    350-391): D3Q19 BGK, 192 x 128 x 128, D 16 at (48, 64, 64), u0 0.05,
+    tau 0.62, sponge 8; the reference records the dense pinned Cd 2.9436 after
+    30 D/U (its AMR gate's cd_dense_reference)."""
+    import math
+    desc, mac, meta = sc.sphere(nx=192, ny=128, nz=128, radius=8.0, centre=(48.0, 64.0, 64.0), u0=0.05, re=20.0,
+                                trt=False, sponge_width=8)
+    steps = 30 * 320
+    print("case sphere Re20 D16", flush=True)
+    with pylbm.Solve(desc) as s:
+        s.upload_macro(mac)
+        s.step(steps)
+        s.sync()
+        f = s.read_forces(1)
+    cd = float(f[-1, 0] / (0.5 * 0.05 ** 2 * math.pi * 64.0))
+    print("sphere Re20 D16 Cd %.4f (reference 2.9436)" % cd)
+    assert abs(cd / 2.9436 - 1.0) < 0.005
