@@ -275,3 +275,26 @@ def test_sphere_re20_matches_the_reference_measurement():
     cd = float(f[-1, 0] / (0.5 * 0.05 ** 2 * math.pi * 64.0))
     print("sphere Re20 D16 Cd %.4f (reference 2.9436)" % cd)
     assert abs(cd / 2.9436 - 1.0) < 0.005
+
+
+def test_shedding_sphere_re300_matches_the_reference_measurement():
+    """benchmarks/d3.json sphere-shed-Re300-D16: the same domain at Re 300
+    (tau 0.508) sheds; after 60 D/U the mean Cd over 12 shedding periods
+    (St 0.136) measured 0.7487-0.7496 on the reference in three runs."""
+    import math
+    desc, mac, meta = sc.sphere(nx=192, ny=128, nz=128, radius=8.0, centre=(48.0, 64.0, 64.0), u0=0.05, re=300.0,
+                                trt=False, sponge_width=8)
+    transient = 60 * 320
+    measure = int(round(12 * 320 / 0.136))
+    print("case shedding sphere Re300 D16", flush=True)
+    with pylbm.Solve(desc) as s:
+        s.upload_macro(mac)
+        s.step(transient)
+        s.step(measure)
+        s.sync()
+        f = s.read_forces(measure)
+    cd = f[:, 0] / (0.5 * 0.05 ** 2 * math.pi * 64.0)
+    lateral = np.hypot(f[:, 1], f[:, 2]) / (0.5 * 0.05 ** 2 * math.pi * 64.0)
+    print("shedding sphere Cd mean %.4f (reference 0.7487-0.7496), lateral pk %.3f" % (cd.mean(), lateral.max()))
+    assert 0.7487 * 0.995 < cd.mean() < 0.7496 * 1.005
+    assert lateral.max() > 0.02 * cd.mean()  # it sheds
