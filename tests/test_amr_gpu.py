@@ -203,3 +203,95 @@ def test_pool_exhaustion_is_a_hard_failure():
         assert info["exhausted"] == 64 - 2 and info["active_tiles"] == 2
         bs, sb = amr.read_topology()
         assert sb.tolist() == [0, 1] and (bs[2:] == -1).all()
+
+
+def beltrami_mac(n, u0):
+    """The reference's Beltrami field at t = 0 (d3-scenarios.mjs:160-175,
+    seedMacro3): rho = 1 - |u|^2 / (2 cs^2)."""
+    k = 2 * np.pi / n
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    u = np.stack([u0 * (np.sin(k * z) + np.cos(k * y)), u0 * (np.sin(k * x) + np.cos(k * z)),
+                  u0 * (np.sin(k * y) + np.cos(k * x))], -1).reshape(-1, 3)
+    mac = np.empty((n ** 3, 4), np.float32)
+    mac[:, 0] = 1.0 - (u ** 2).sum(1) * 1.5
+    mac[:, 1:] = u
+    return mac.reshape(-1)
+
+
+def beltrami_run(interface, blocks, checkpoints):
+    """Beltrami flow (N 48, tau 0.8, u0 0.04, D3Q19, RB 4) with the given
+    blocks refined, static; the L0 field sampled every 2nd cell against the
+    analytic decay at each checkpoint (in decay times), as
+    tools/lib/d3-metrics.js:109-160 runBeltramiCase scores it.  Returns the
+    worst field L2rel and the decay-rate relative error fitted from the RMS
+    speed of the last two samples."""
+    n, tau, u0 = 48, 0.8, 0.04
+    nu = (tau - 0.5) / 3.0
+    kk = 2 * np.pi / n
+    td = 1.0 / (nu * kk * kk)
+    desc = {"q": 19, "nx": n, "ny": n, "nz": n, "rb": 4, "maxSlots": len(blocks), "omega": 1.0 / tau,
+            "interface": interface}
+    bs = np.full(12 ** 3, -1, np.int32)
+    sb = np.full(len(blocks), -1, np.int32)
+    for s, b in enumerate(sorted(blocks)):
+        bs[b], sb[s] = s, b
+    z, y, x = np.meshgrid(np.arange(0, n, 2), np.arange(0, n, 2), np.arange(0, n, 2), indexing="ij")
+    shape = np.stack([np.sin(kk * z) + np.cos(kk * y), np.sin(kk * x) + np.cos(kk * z),
+                      np.sin(kk * y) + np.cos(kk * x)], -1).reshape(-1, 3) * u0
+    samples = []
+    with pylbm.Amr(desc) as amr:
+        amr.upload_topology(bs, sb)
+        amr.upload_macro(beltrami_mac(n, u0))
+        prev = 0
+        for m in checkpoints:
+            t = int(round(m * td))
+            amr.step(t - prev)
+            prev = t
+            u = amr.read_macro().reshape(n, n, n, 4)[::2, ::2, ::2].reshape(-1, 4)[:, 1:].astype(np.float64)
+            e = shape * np.exp(-nu * kk * kk * t)
+            samples.append((t, np.sqrt(((u - e) ** 2).sum() / (e ** 2).sum()), np.sqrt((u ** 2).sum(1).mean())))
+    (ta, _, ra), (tb, _, rb_) = samples[-2:]
+    rate = -np.log(rb_ / ra) / (tb - ta)
+    return max(s[1] for s in samples), (rate - 1.0 / td) * td, [s[1] for s in samples]
+
+
+BOX = [(bz * 12 + by) * 12 + bx for bz in range(3, 9) for by in range(3, 9) for bx in range(3, 9)]
+
+
+@pytest.mark.parametrize("interface,l2_tol,rate_tol,l2_ref,rate_ref,margin", [
+    ("explode", 0.0035, 0.004, 2.484e-3, 1.826e-3, 0.10),
+    ("interp", 0.012, 0.014, 9.012e-3, 1.024e-2, 0.02)])
+def test_box_refined_beltrami_matches_the_reference_measurements(interface, l2_tol, rate_tol, l2_ref, rate_ref,
+                                                                   margin):
+    """benchmarks/d3.json amr_box_explode_cases[0] / amr_box_cases[0]: the
+    centred 6x6x6 cube of 4-cell blocks refined (blocks whose centres lie in
+    [N/4, 3N/4)), checkpoints 0.25 and 0.5 decay times.  Held to the
+    reference's own gates (field_l2_tol, decay_rate_tol) and to its recorded
+    measurements: interp reproduces 9.012e-3 / 1.024e-2 to 0.1 %.  Explode
+    reads 2.475e-3 at 0.5 decay times, where the reference records 2.48e-3
+    (benchmarks/d3.json amr_interface_note), and its decay rate to 1.3 %;
+    its worst checkpoint is the 0.25 one at 2.654e-3, 6.8 % above the
+    recorded worst, so the early transient off the seam is where the two
+    differ and the margin is 10 %.  The no-interface control is
+    test_all_refined_beltrami_matches_the_reference_control."""
+    say("box-refined Beltrami", interface)
+    l2, rate_err, per = beltrami_run(interface, BOX, (0.25, 0.5))
+    say("%s: field L2rel %s worst %.4e (reference %.4e), decay-rate rel err %.4e (reference %.4e)"
+        % (interface, ["%.4e" % v for v in per], l2, l2_ref, rate_err, rate_ref))
+    assert l2 < l2_tol and abs(rate_err) < rate_tol
+    assert abs(l2 / l2_ref - 1.0) < margin
+    assert abs(rate_err / rate_ref - 1.0) < 0.1
+
+
+def test_all_refined_beltrami_matches_the_reference_control():
+    """benchmarks/d3.json amr_cases[0] (amr-all-RB4): every block refined, so
+    there is no coarse/fine interface; checkpoints 0.5, 1.5 and 3 decay
+    times, gates field_l2_tol 0.01 and decay_rate_tol 0.005.  The reference
+    records 2.293e-3 worst field L2rel (the no-interface control of the box
+    case, field_l2_control_all) and a decay-rate error of -1.222e-4."""
+    say("all-refined Beltrami")
+    l2, rate_err, per = beltrami_run("explode", list(range(12 ** 3)), (0.5, 1.5, 3.0))
+    say("all: field L2rel %s worst %.4e (reference 2.293e-3), decay-rate rel err %.4e (reference -1.222e-4)"
+        % (["%.4e" % v for v in per], l2, rate_err))
+    assert l2 < 0.01 and abs(rate_err) < 0.005
+    assert abs(l2 / 2.293e-3 - 1.0) < 0.02
