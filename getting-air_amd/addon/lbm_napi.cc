@@ -484,7 +484,8 @@ lbm_amr* amr_handle(napi_env env, napi_value v) {
 
 // createAmr({q, nx, ny, nz, rb, maxSlots, omega, omegaMinus, force,
 //            manageEvery, refineAt, keepAt, lead, explodeLinear, orphans,
-//            interface: 'explode' | 'interp', reflux})
+//            interface: 'explode' | 'interp', reflux, sponge: {width, axes, u},
+//            body: {a, centre}})
 napi_value CreateAmr(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
@@ -530,6 +531,23 @@ napi_value CreateAmr(napi_env env, napi_callback_info info) {
     }
   }
   d.reflux = (int)num(env, o, "reflux", 0);
+  if (get_prop(env, o, "sponge", &v)) {
+    d.sponge_width = (float)num(env, v, "width", 0);
+    double ax[3] = {1, 1, 1}, su[3] = {0, 0, 0};
+    num_array(env, v, "axes", ax, 3);
+    num_array(env, v, "u", su, 3);
+    for (int k = 0; k < 3; k++) {
+      d.sponge_axes[k] = ax[k] != 0;
+      d.sponge_u[k] = (float)su[k];
+    }
+  }
+  if (get_prop(env, o, "body", &v)) {
+    d.body.shape = LBM_BODY_SPHERE;
+    d.body.radius = (float)num(env, v, "a", 0);
+    double c[3] = {0, 0, 0};
+    num_array(env, v, "centre", c, 3);
+    for (int k = 0; k < 3; k++) d.body.centre[k] = (float)c[k];
+  }
   lbm_amr* a = nullptr;
   int rc = lbm_amr_create(&d, &a);
   if (rc) return throw_lbm(env, rc);
@@ -613,6 +631,24 @@ napi_value AmrReadActive(napi_env env, napi_callback_info info) {
   int rc = lbm_amr_read_active(amr_handle(env, argv[0]), static_cast<int32_t*>(data));
   if (rc) return throw_lbm(env, rc);
   return argv[1];
+}
+
+// amrReadForces(amr, maxRecords) -> Float64Array [fx, fy, fz, tx, ty, tz] * n
+napi_value AmrReadForces(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  int32_t maxr = 1;
+  if (argc > 1) napi_get_value_int32(env, argv[1], &maxr);
+  if (maxr < 0) maxr = 0;
+  napi_value ab, arr;
+  void* data;
+  NAPI_OK(napi_create_arraybuffer(env, (size_t)maxr * 48 + 8, &data, &ab));
+  int32_t count = 0;
+  int rc = lbm_amr_read_forces(amr_handle(env, argv[0]), static_cast<double*>(data), maxr, &count);
+  if (rc) return throw_lbm(env, rc);
+  NAPI_OK(napi_create_typedarray(env, napi_float64_array, (size_t)count * 6, ab, 0, &arr));
+  return arr;
 }
 
 napi_value AmrReadTotals(napi_env env, napi_callback_info info) {
@@ -1002,6 +1038,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"amrReadCriterion", AmrReadCriterion},
       {"amrReadActive", AmrReadActive},
       {"amrReadTotals", AmrReadTotals},
+      {"amrReadForces", AmrReadForces},
       {"amrManageNow", AmrManageNow},
       {"amrInfo", AmrInfo},
       {"amrStep", AmrStep},

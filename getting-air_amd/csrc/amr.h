@@ -18,6 +18,13 @@ namespace lbm {
 
 constexpr int kGhost = 2;
 
+// A pinned sphere solved on the fine level alone, in L0 cell units; the
+// refined region covers every L0 cell within 1 of it.
+struct PoolBody {
+  int32_t on;
+  float cx, cy, cz, r;
+};
+
 struct PoolGeom {
   int32_t nx, ny, nz;        // L0 cells
   int32_t rb, fb;            // block edge (L0 cells), tile edge (fine cells)
@@ -28,6 +35,7 @@ struct PoolGeom {
   const int32_t* block_slot;     // per block: slot or -1
   const int32_t* slot_to_block;  // per slot: block or -1
   const uint8_t* ccls;           // per L0 cell: kCovered | kNear | kSeam
+  PoolBody body;
 };
 
 // L0 cell classes, rebuilt by launch_cell_class after every topology change.
@@ -77,6 +85,12 @@ hipError_t launch_cell_class(int q, uint8_t* ccls, const PoolGeom& g, const Surf
 hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
                             const CollideParams& fine, const int32_t* active, int nactive, int collide_ring,
                             hipStream_t s);
+// Momentum exchange on the fine-level body from the pool state at the start
+// of the macro-step: one FP64 [fx, fy, fz, tx, ty, tz] partial per thread
+// block (pool_force_blocks of them), weighted dx^2 = 1/4.
+uint32_t pool_force_blocks(int rb, int nactive);
+hipError_t launch_pool_force(int q, const float* gpool, const PoolGeom& g, const int32_t* active, int nactive,
+                             double* partials, hipStream_t s);
 // Seam flux of the fine solver over one substep, from the pool buffer that
 // substep streams: [dm, dpx, dpy, dpz] / 8 per flux cell, written
 // (accumulate 0) or added (1).

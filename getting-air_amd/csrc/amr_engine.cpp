@@ -50,15 +50,21 @@ struct lbm_amr {
   double* partials = nullptr;
   unsigned* counter = nullptr;
   double* totals = nullptr;
+  double* force_partials = nullptr;       // one FP64 [6] per pool force block
+  double* history = nullptr;              // force records, a ring of kForceCap
+  unsigned long long* written = nullptr;  // records written since the seed
   CollideParams fine{};
   float c2f = 0.f, f2c = 0.f;
   int nactive = 0;
   int exhausted = 0;
   long long step = 0;
   bool seeded = false;
+  bool body_covered = false;  // the uploaded topology holds the body (check_topology)
 };
 
 namespace {
+
+const uint32_t kForceCap = 1u << 16;
 
 template <int Q>
 void fine_params(CollideParams& p, float omega, float omega_m, const float* F) {
@@ -101,6 +107,45 @@ int validate(const lbm_amr_desc* d) {
   // create (main-3d.js:1333-1336)
   if (d->reflux && !d->interface_interp)
     return fail(LBM_E_INVALID, "reflux needs the interp interface: explode/coalesce is conservative by construction");
+  if (d->body.shape != LBM_BODY_NONE) {
+    if (d->body.shape != LBM_BODY_SPHERE) return fail(LBM_E_INVALID, "AMR body shape %d: only a sphere", d->body.shape);
+    if (!(d->body.radius > 0.f)) return fail(LBM_E_INVALID, "AMR body radius %g", d->body.radius);
+    if (d->interface_interp) return fail(LBM_E_INVALID, "an AMR body needs the explode interface");
+    // the body must stay inside the refined region, which only a static
+    // topology guarantees here
+    if (d->manage_every != 0) return fail(LBM_E_INVALID, "an AMR body needs a static topology (manage_every = 0)");
+  }
+  return LBM_OK;
+}
+
+// Refuses a topology that leaves an L0 cell within kBodyReach (1 cell) of the
+// body unrefined (the body lives on the fine level alone, main-3d.js:3127-3136)
+// or refines a block that reaches into the sponge band (the fine step does
+// not relax toward the far field).
+int check_topology(const lbm_amr_desc& d, const int32_t* block_slot) {
+  const int rb = d.rb, nbx = d.nx / rb, nby = d.ny / rb, nbz = d.nz / rb;
+  const int n[3] = {d.nx, d.ny, d.nz};
+  for (int b = 0; b < nbx * nby * nbz; b++) {
+    const int bc[3] = {b % nbx, (b / nbx) % nby, b / (nbx * nby)};
+    if (block_slot[b] < 0 && d.body.shape == LBM_BODY_SPHERE) {
+      // distance from the centre to the block's box of cell centres
+      double s2 = 0.0;
+      for (int k = 0; k < 3; k++) {
+        const double lo = bc[k] * rb, hi = lo + rb - 1, c = d.body.centre[k];
+        const double e = c < lo ? lo - c : (c > hi ? c - hi : 0.0);
+        s2 += e * e;
+      }
+      if (std::sqrt(s2) - d.body.radius < 1.0)
+        return fail(LBM_E_INVALID, "block %d lies within one cell of the body and is not refined", b);
+    }
+    if (block_slot[b] >= 0 && d.sponge_width > 0.f) {
+      for (int k = 0; k < 3; k++) {
+        const int lo = bc[k] * rb, hi = lo + rb - 1;
+        if (d.sponge_axes[k] && (lo < d.sponge_width || (n[k] - 1) - hi < d.sponge_width))
+          return fail(LBM_E_INVALID, "refined block %d reaches into the sponge band", b);
+      }
+    }
+  }
   return LBM_OK;
 }
 
@@ -108,7 +153,8 @@ void release(lbm_amr* a) {
   if (!a) return;
   void* ptrs[] = {a->pool[0], a->pool[1], a->mac_pool, a->block_slot, a->slot_to_block, a->want, a->crit,
                   a->dying, a->fresh, a->active, a->free_list, a->status, a->pairs, a->ccls, a->partials, a->counter, a->totals,
-                  a->plan.seam, a->plan.near, a->plan.ring, a->plan.flux, a->plan.count, a->flux_acc};
+                  a->plan.seam, a->plan.near, a->plan.ring, a->plan.flux, a->plan.count, a->flux_acc,
+                  a->force_partials, a->history, a->written};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (a->l0) lbm_solve_destroy(a->l0);
@@ -254,6 +300,13 @@ int macro_step(lbm_amr* a) {
     return LBM_OK;
   }
   float* l0 = solve_lattice(a->l0, 0);
+  if (a->G.body.on) {
+    // the one force pass of the macro-step, on the fine state at t
+    // (main-3d.js:3127-3141, 1596-1597)
+    HIPCHK(launch_pool_force(d.q, a->pool[0], a->G, a->active, a->nactive, a->force_partials, a->stream));
+    ForceFinalizeArgs fa{a->force_partials, pool_force_blocks(d.rb, a->nactive), a->history, a->written, kForceCap};
+    HIPCHK(launch_force_finalize(fa, a->stream));
+  }
   HIPCHK(launch_explode(d.q, l0, a->pool[0], a->G, a->plan, a->nring, d.explode_linear, a->stream));
   // only the second substep's field is read (coalesce, lbm_amr_read_pool_macro)
   HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->active, a->nactive, 0, a->stream));
@@ -287,6 +340,11 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
   l0.omega_minus = desc->omega_minus;
   for (int k = 0; k < 3; k++) l0.force[k] = desc->force[k];
   l0.storage = LBM_STORE_F32;
+  l0.sponge_width = desc->sponge_width;
+  for (int k = 0; k < 3; k++) {
+    l0.sponge_axes[k] = desc->sponge_axes[k];
+    l0.sponge_u[k] = desc->sponge_u[k];
+  }
   if ((rc = lbm_solve_create(&l0, &a->l0))) {
     release(a);
     return rc;
@@ -326,6 +384,14 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
     release(a);
     return rc;
   }
+  if (desc->body.shape == LBM_BODY_SPHERE) {
+    G.body = PoolBody{1, desc->body.centre[0], desc->body.centre[1], desc->body.centre[2], desc->body.radius};
+    if ((rc = dalloc(&a->force_partials, (size_t)pool_force_blocks(rb, desc->max_slots) * 48)) ||
+        (rc = dalloc(&a->history, (size_t)kForceCap * 48)) || (rc = dalloc(&a->written, 8))) {
+      release(a);
+      return rc;
+    }
+  }
   if (desc->reflux) {
     a->plan.cap_flux = G.ncells;
     if ((rc = dalloc(&a->plan.flux, (size_t)G.ncells * 4)) || (rc = dalloc(&a->flux_acc, (size_t)G.ncells * 16))) {
@@ -345,6 +411,7 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
   if (e == hipSuccess) e = hipMemsetAsync(a->counter, 0, 16, a->stream);
   if (e == hipSuccess) e = hipMemsetAsync(a->mac_pool, 0, (size_t)G.pool_plane * 16, a->stream);
   if (e == hipSuccess) e = hipMemsetAsync(a->ccls, 0, (size_t)G.ncells, a->stream);
+  if (e == hipSuccess && a->written) e = hipMemsetAsync(a->written, 0, 8, a->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(a->stream);
   if (e != hipSuccess) {
     release(a);
@@ -383,6 +450,9 @@ int lbm_amr_upload_topology(lbm_amr* a, const int32_t* block_slot, const int32_t
     if (b >= a->nblocks || (b >= 0 && block_slot[b] != s))
       return fail(LBM_E_INVALID, "slotToBlock[%d] = %d is not inverted by blockSlot", s, b);
   }
+  int rc_geo = check_topology(a->d, block_slot);
+  if (rc_geo) return rc_geo;
+  a->body_covered = true;
   if (a->seeded) {
     int rc0 = restrict_active(a);
     if (rc0) return rc0;
@@ -421,6 +491,7 @@ int lbm_amr_upload_macro(lbm_amr* a, const float* mac4) {
   if (rc) return rc;
   a->seeded = true;
   a->step = 0;
+  if (a->written) HIPCHK(hipMemsetAsync(a->written, 0, 8, a->stream));
   // fill every tile of the current topology from the seeded L0 field
   std::vector<int32_t> s2b(a->d.max_slots);
   HIPCHK(hipMemcpy(s2b.data(), a->slot_to_block, s2b.size() * 4, hipMemcpyDeviceToHost));
@@ -439,6 +510,7 @@ int lbm_amr_upload_macro(lbm_amr* a, const float* mac4) {
 int lbm_amr_step(lbm_amr* a, int32_t n) {
   if (!a) return fail(LBM_E_INVALID, "null argument");
   if (!a->seeded) return fail(LBM_E_STATE, "step before upload_macro");
+  if (a->G.body.on && !a->body_covered) return fail(LBM_E_STATE, "step with a body before a topology that holds it");
   for (int s = 0; s < n; s++) {
     int rc = macro_step(a);
     if (rc) return rc;
@@ -450,6 +522,7 @@ int lbm_amr_step(lbm_amr* a, int32_t n) {
 int lbm_amr_manage_now(lbm_amr* a) {
   if (!a) return fail(LBM_E_INVALID, "null argument");
   if (!a->seeded) return fail(LBM_E_STATE, "manage before upload_macro");
+  if (a->G.body.on) return fail(LBM_E_INVALID, "an AMR body needs a static topology: the manager does not keep it");
   int rc = manage(a);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(a->stream));
@@ -499,6 +572,26 @@ int lbm_amr_read_criterion(lbm_amr* a, float* crit, uint8_t* want) {
 int lbm_amr_read_active(lbm_amr* a, int32_t* list) {
   if (!a || !list) return fail(LBM_E_INVALID, "null argument");
   HIPCHK(hipMemcpy(list, a->active, (size_t)(a->nactive + 1) * 4, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
+int lbm_amr_read_forces(lbm_amr* a, double* out, int32_t max_records, int32_t* count) {
+  if (!a || !count || (max_records > 0 && !out)) return fail(LBM_E_INVALID, "null argument");
+  *count = 0;
+  if (!a->written) return LBM_OK;  // no body: no records
+  unsigned long long written = 0;
+  HIPCHK(hipMemcpyAsync(&written, a->written, 8, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  const unsigned long long avail = std::min<unsigned long long>(written, kForceCap);
+  const unsigned long long take = std::min<unsigned long long>(avail, (unsigned long long)std::max(0, max_records));
+  if (take) {
+    std::vector<double> ring((size_t)kForceCap * 6);
+    HIPCHK(hipMemcpyAsync(ring.data(), a->history, ring.size() * 8, hipMemcpyDeviceToHost, a->stream));
+    HIPCHK(hipStreamSynchronize(a->stream));
+    for (unsigned long long k = 0; k < take; k++)
+      memcpy(out + k * 6, ring.data() + ((written - take + k) % kForceCap) * 6, 48);
+  }
+  *count = (int32_t)take;
   return LBM_OK;
 }
 

@@ -107,6 +107,24 @@ __device__ __forceinline__ void direction(int i, int& ex, int& ey, int& ez, floa
     if (i == k) ex = L::ex[k], ey = L::ey[k], ez = L::ez[k], w = L::w[k];
 }
 
+// Centre of fine cell j (tile-local, ring included) of a block whose first L0
+// cell is `origin`, in L0 units (fineToCoarseUnit, d3-amr.mjs:45-47).
+__device__ __forceinline__ float fine_to_l0(int origin, int j) {
+  return (float)origin - 0.25f + 0.5f * (float)(j - kGhost);
+}
+
+// Signed distance to the pinned sphere in binary32, evaluated as the dense
+// classification evaluates it (common_d3_geometry.wgsl:119, :174-187), so a
+// fine cell and the dense cell at the same point agree on which side they are.
+__device__ __forceinline__ float sphere_phi(const PoolBody& b, float x, float y, float z) {
+  const float dx = sub_rn(x, b.cx), dy = sub_rn(y, b.cy), dz = sub_rn(z, b.cz);
+  return sub_rn(sqrt_rn(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz))), b.r);
+}
+
+// A pull source lies at most sqrt(3)/2 L0 cells from the cell, so a cell at
+// phi >= 1 has no solid source.
+constexpr float kBodyReach = 1.f;
+
 // ---------------------------------------------------------------------------
 // L0 cell classes
 // ---------------------------------------------------------------------------
@@ -183,7 +201,13 @@ void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint3
 // ---------------------------------------------------------------------------
 // pool step
 // ---------------------------------------------------------------------------
-template <int Q, int RB>
+// BODY: the fine-level pinned sphere (its own instantiation, so that the
+// body-free kernel keeps its register budget).  A link whose geometric source
+// is solid reflects this cell's own opposite population (halfway bounce-back,
+// common_d3_amr_step1.wgsl:258-276; the body is at rest, so Ladd's term is
+// zero), and a solid cell is held at feq(1, 0) with [1, 0, 0, 0] as its
+// moments (:387-403).
+template <int Q, int RB, bool BODY>
 __global__ void __launch_bounds__(256)
 k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __restrict__ mac_pool, PoolGeom G,
             CollideParams P, const int32_t* __restrict__ active, int nactive, int collide_ring) {
@@ -239,10 +263,23 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
     sz = min(max(sz, 0), FB - 1);
     g[i] = *at(gin + (size_t)i * G.pool_plane, pool_cell<RB>(src_slot, sx, sy, sz));
   }
+  const uint32_t cell = pool_cell<RB>(slot, fx, fy, fz);
+  float phi = 1e30f;
+  if (BODY) {
+    const float px = fine_to_l0(bx * RB, fx), py = fine_to_l0(by * RB, fy), pz = fine_to_l0(bz * RB, fz);
+    phi = sphere_phi(G.body, px, py, pz);
+    if (phi < kBodyReach) {
+#pragma unroll
+      for (int i = 1; i < Q; i++) {
+        const float sx = px - 0.5f * (float)L::ex[i], sy = py - 0.5f * (float)L::ey[i],
+                    sz = pz - 0.5f * (float)L::ez[i];
+        if (sphere_phi(G.body, sx, sy, sz) < 0.f) g[i] = *at(gin + (size_t)L::opp[i] * G.pool_plane, cell);
+      }
+    }
+  }
 
   float drho, rho, ux, uy, uz;
   moments<Q>(g, P.F, drho, rho, ux, uy, uz);
-  const uint32_t cell = pool_cell<RB>(slot, fx, fy, fz);
   if (mac_pool) {
     *at(mac_pool, cell) = rho;
     *at(mac_pool + G.pool_plane, cell) = ux;
@@ -254,8 +291,77 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
   // COLLIDE_RING = 0): coalesce averages uncollided interface states.  The
   // interp interface collides them too (COLLIDE_RING = 1, main-3d.js:1395).
   if (interior || collide_ring) collide<Q>(g, P, drho, rho, ux, uy, uz);
+  if (BODY && phi < 0.f) {
+#pragma unroll
+    for (int i = 0; i < Q; i++) g[i] = 0.f;  // feq(1, 0) - w
+    if (mac_pool) {
+      *at(mac_pool, cell) = 1.f;
+      *at(mac_pool + G.pool_plane, cell) = 0.f;
+      *at(mac_pool + 2 * (size_t)G.pool_plane, cell) = 0.f;
+      *at(mac_pool + 3 * (size_t)G.pool_plane, cell) = 0.f;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < Q; i++) *at(gout + (size_t)i * G.pool_plane, cell) = g[i];
+}
+
+// Momentum exchange on the fine-level body (common_d3_force_pool.wgsl, the
+// bounce-back branch of a body at rest): every interior fluid cell with a
+// solid geometric source of direction i adds -e_i 2 f_opp(i), read from the
+// pool state the macro-step starts from; the sum is weighted by the fine
+// cell's face area dx^2 = 1/4 and its torque taken about the centre in L0
+// units.  One thread per interior cell of each active tile; each block leaves
+// its FP64 partial in `partials`, summed in block order afterwards.
+template <int Q, int RB>
+__global__ void __launch_bounds__(256)
+k_pool_force(const float* __restrict__ gp, PoolGeom G, const int32_t* __restrict__ active, int nactive,
+             double* __restrict__ partials) {
+  using L = Lattice<Q>;
+  constexpr int RB2 = 2 * RB;
+  constexpr uint32_t per = RB2 * RB2 * RB2;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = t / per;
+  double fv[6] = {0, 0, 0, 0, 0, 0};
+  if (k < (uint32_t)nactive) {
+    const uint32_t r = t - k * per;
+    const int fx = kGhost + (int)(r % RB2), fy = kGhost + (int)((r / RB2) % RB2), fz = kGhost + (int)(r / (RB2 * RB2));
+    const int slot = active[1 + k];
+    const int id = G.slot_to_block[slot];
+    const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
+    const float px = fine_to_l0(bx * RB, fx), py = fine_to_l0(by * RB, fy), pz = fine_to_l0(bz * RB, fz);
+    const float phi = sphere_phi(G.body, px, py, pz);
+    if (phi >= 0.f && phi < kBodyReach) {
+      const uint32_t cell = pool_cell<RB>(slot, fx, fy, fz);
+      double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+#pragma unroll
+      for (int i = 1; i < Q; i++) {
+        const float sx = px - 0.5f * (float)L::ex[i], sy = py - 0.5f * (float)L::ey[i],
+                    sz = pz - 0.5f * (float)L::ez[i];
+        if (sphere_phi(G.body, sx, sy, sz) < 0.f) {
+          const double m = 2.0 * ((double)*at(gp + (size_t)L::opp[i] * G.pool_plane, cell) + (double)L::w[i]);
+          f0 -= L::ex[i] * m;
+          f1 -= L::ey[i] * m;
+          f2 -= L::ez[i] * m;
+        }
+      }
+      f0 *= 0.25;
+      f1 *= 0.25;
+      f2 *= 0.25;
+      const double rx = (double)px - G.body.cx, ry = (double)py - G.body.cy, rz = (double)pz - G.body.cz;
+      fv[0] = f0;
+      fv[1] = f1;
+      fv[2] = f2;
+      fv[3] = ry * f2 - rz * f1;
+      fv[4] = rz * f0 - rx * f2;
+      fv[5] = rx * f1 - ry * f0;
+    }
+  }
+  __shared__ double lds[4 * 6];
+  block_sum<6>(fv, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; c++) partials[(size_t)blockIdx.x * 6 + c] = fv[c];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -871,8 +977,21 @@ template <int Q, int RB>
 struct OpPoolStep {
   static hipError_t run(const float* gin, float* gout, float* mac_pool, const PoolGeom& g, const CollideParams& p,
                         const int32_t* active, int nactive, int collide_ring, hipStream_t s) {
-    hipLaunchKernelGGL((k_pool_step<Q, RB>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s, gin,
-                       gout, mac_pool, g, p, active, nactive, collide_ring);
+    if (g.body.on)
+      hipLaunchKernelGGL((k_pool_step<Q, RB, true>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s,
+                         gin, gout, mac_pool, g, p, active, nactive, collide_ring);
+    else
+      hipLaunchKernelGGL((k_pool_step<Q, RB, false>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s,
+                         gin, gout, mac_pool, g, p, active, nactive, collide_ring);
+    return hipGetLastError();
+  }
+};
+template <int Q, int RB>
+struct OpPoolForce {
+  static hipError_t run(const float* gp, const PoolGeom& g, const int32_t* active, int nactive, double* partials,
+                        hipStream_t s) {
+    hipLaunchKernelGGL((k_pool_force<Q, RB>), dim3(pool_force_blocks(RB, nactive)), dim3(256), 0, s, gp, g, active,
+                       nactive, partials);
     return hipGetLastError();
   }
 };
@@ -955,6 +1074,14 @@ hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_poo
                             hipStream_t s) {
   if (nactive <= 0) return hipSuccess;
   return dispatch_q_rb<OpPoolStep>(q, g.rb, gin, gout, mac_pool, g, fine, active, nactive, collide_ring, s);
+}
+uint32_t pool_force_blocks(int rb, int nactive) {
+  const uint64_t per = (uint64_t)(2 * rb) * (2 * rb) * (2 * rb);
+  return (uint32_t)std::max<uint64_t>(1, ((uint64_t)std::max(0, nactive) * per + 255) / 256);
+}
+hipError_t launch_pool_force(int q, const float* gpool, const PoolGeom& g, const int32_t* active, int nactive,
+                             double* partials, hipStream_t s) {
+  return dispatch_q_rb<OpPoolForce>(q, g.rb, gpool, g, active, nactive, partials, s);
 }
 hipError_t launch_flux(int q, const float* gp, float* acc, const PoolGeom& g, const SurfacePlan& plan, uint32_t nflux,
                        int accumulate, hipStream_t s) {

@@ -361,6 +361,14 @@ export class AmrSolver {
     const t = this.addon.amrReadTotals(this.handle);
     return { step: this.step, mass: t[0], px: t[1], py: t[2], pz: t[3] };
   }
+
+  // The fine-level body's force: the latest record [fx, fy, fz, tx, ty, tz]
+  // of the macro-step's one force pass (readBody's force and torque).
+  async readBody() {
+    const f = this.addon.amrReadForces(this.handle, 1);
+    if (f.length < 6) return null;
+    return { fx: f[0], fy: f[1], fz: f[2], tx: f[3], ty: f[4], tz: f[5] };
+  }
 }
 
 // Batched serve loop: many independent solves resident in one device pool,

@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblbm.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 BGK, TRT = 0, 1
 STORE = {"f32": 0, "f16": 1, "bf16": 2}
 BODY_NONE, BODY_SPHERE, BODY_CIRCLE = 0, 1, 2
@@ -26,7 +26,7 @@ EXPORTS = (
     "lbm_amr_create", "lbm_amr_destroy", "lbm_amr_upload_topology", "lbm_amr_read_topology",
     "lbm_amr_upload_macro", "lbm_amr_step", "lbm_amr_manage_now", "lbm_amr_read_macro",
     "lbm_amr_read_pool_macro", "lbm_amr_read_f", "lbm_amr_read_criterion", "lbm_amr_read_active",
-    "lbm_amr_read_totals", "lbm_amr_info",
+    "lbm_amr_read_totals", "lbm_amr_info", "lbm_amr_read_forces",
     "lbm_serve_create", "lbm_serve_destroy", "lbm_serve_submit", "lbm_serve_wait", "lbm_serve_read_macro",
     "lbm_serve_read_forces", "lbm_serve_release", "lbm_serve_drain", "lbm_serve_read_stats", "lbm_serve_clock",
     "lbm_serve_submit_state", "lbm_serve_read_state", "lbm_rigid_attach", "lbm_rigid_read",
@@ -59,6 +59,8 @@ class AmrDesc(ctypes.Structure):
         ("manage_every", ctypes.c_int32), ("refine_at", ctypes.c_float), ("keep_at", ctypes.c_float),
         ("lead", ctypes.c_int32), ("explode_linear", ctypes.c_int32), ("orphans", ctypes.c_int32),
         ("interface_interp", ctypes.c_int32), ("reflux", ctypes.c_int32),
+        ("sponge_width", ctypes.c_float), ("sponge_axes", ctypes.c_int32 * 3), ("sponge_u", ctypes.c_float * 3),
+        ("body", Body),
     ]
 
 
@@ -190,6 +192,7 @@ def load():
         lib.lbm_amr_read_active.argtypes = [P, P]
         lib.lbm_amr_read_totals.argtypes = [P, P]
         lib.lbm_amr_info.argtypes = [P, i64p, i32p, i32p]
+        lib.lbm_amr_read_forces.argtypes = [P, P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
         lib.lbm_serve_create.argtypes = [ctypes.POINTER(ServeDesc), ctypes.POINTER(P)]
         lib.lbm_serve_destroy.argtypes = [P]
         lib.lbm_serve_submit.argtypes = [P, ctypes.POINTER(Desc), P, ctypes.c_int32, i64p]
@@ -393,7 +396,8 @@ class Solve:
 def make_amr_desc(d):
     """AmrDesc from a dict {q, nx, ny, nz, rb, maxSlots, omega, omegaMinus,
     force, manageEvery, refineAt, keepAt, lead, explodeLinear, orphans,
-    interface ('explode' or 'interp'), reflux}."""
+    interface ('explode' or 'interp'), reflux, sponge: {width, axes, u} on
+    L0, body: {a, centre} a pinned sphere solved on the fine level}."""
     a = AmrDesc()
     a.abi_version = ABI_VERSION
     a.q = int(d["q"])
@@ -417,6 +421,18 @@ def make_amr_desc(d):
         raise ValueError("interface must be 'explode' or 'interp'")
     a.interface_interp = int(iface == "interp")
     a.reflux = int(bool(d.get("reflux", 0)))
+    sp = d.get("sponge") or {}
+    a.sponge_width = float(sp.get("width", 0.0))
+    axes = sp.get("axes", [1, 1, 1])
+    for k in range(3):
+        a.sponge_axes[k] = int(bool(axes[k]))
+        a.sponge_u[k] = float((list(sp.get("u", [0, 0, 0])) + [0, 0, 0])[k])
+    body = d.get("body")
+    if body:
+        a.body.shape = BODY_SPHERE
+        a.body.radius = float(body["a"])
+        for k in range(3):
+            a.body.centre[k] = float(body["centre"][k])
     return a
 
 
@@ -510,6 +526,13 @@ class Amr:
         out = np.empty(4, dtype=np.float64)
         _check(self.lib.lbm_amr_read_totals(self.h, out.ctypes.data))
         return out
+
+    def read_forces(self, max_records=1 << 16):
+        """The fine-level body's force records, one per macro-step."""
+        out = np.empty(6 * max_records, dtype=np.float64)
+        cnt = ctypes.c_int32()
+        _check(self.lib.lbm_amr_read_forces(self.h, out.ctypes.data, int(max_records), ctypes.byref(cnt)))
+        return out[: 6 * cnt.value].reshape(-1, 6)
 
     def info(self):
         step, act, exh = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()

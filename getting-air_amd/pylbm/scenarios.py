@@ -82,6 +82,64 @@ def sphere(nx=256, ny=128, nz=128, radius=12.0, centre=(64.0, 64.0, 64.0), u0=0.
     return desc, mac.reshape(-1), meta
 
 
+SQRT3_UP = 1.7320509  # d3-amr.mjs:120, sqrt(3) rounded up
+BLOCK_BB_DEPTH = 3    # d3-amr.mjs:125
+
+
+def near_body_want(sdf, margin):
+    """Whether any point of a block's box [lo, hi) comes within `margin` of
+    the body (nearBodyWant, d3-amr.mjs:139-159): the SDF is 1-Lipschitz, so
+    phi(centre) - h sqrt(3) bounds the box minimum; an ambiguous box splits
+    into octants down to depth 3 and is accepted at the bottom."""
+    def near(c, h, depth):
+        phi = sdf(c)
+        if phi - h * SQRT3_UP > margin:
+            return False
+        if phi <= margin or depth == 0:
+            return True
+        q = h / 2
+        return any(near([c[0] + (q if k & 1 else -q), c[1] + (q if k & 2 else -q), c[2] + (q if k & 4 else -q)],
+                        q, depth - 1) for k in range(8))
+    return lambda lo, hi: near([(lo[k] + hi[k]) / 2 for k in range(3)], (hi[0] - lo[0]) / 2, BLOCK_BB_DEPTH)
+
+
+def refine_near_sphere(nx, ny, nz, rb, centre, radius, margin=2.0, max_slots=None):
+    """refineNearBody with a sphere's SDF (d3-amr.mjs:80-106): blocks in
+    ascending id order take slots 0, 1, ...; returns (block_slot,
+    slot_to_block) sized for max_slots (default: the count)."""
+    want = near_body_want(lambda c: math.sqrt(sum((c[k] - centre[k]) ** 2 for k in range(3))) - radius, margin)
+    nbx, nby, nbz = nx // rb, ny // rb, nz // rb
+    blocks = []
+    for b in range(nbx * nby * nbz):
+        bx, by, bz = b % nbx, (b // nbx) % nby, b // (nbx * nby)
+        lo = [bx * rb, by * rb, bz * rb]
+        if want(lo, [v + rb for v in lo]):
+            blocks.append(b)
+    slots = len(blocks) if max_slots is None else max_slots
+    if len(blocks) > slots:
+        raise ValueError("refinement needs %d tiles, max_slots %d" % (len(blocks), slots))
+    bs = np.full(nbx * nby * nbz, -1, np.int32)
+    sb = np.full(slots, -1, np.int32)
+    for k, b in enumerate(blocks):
+        bs[b], sb[k] = k, b
+    return bs, sb
+
+
+def sphere_amr(nx=192, ny=128, nz=128, radius=8.0, centre=(48.0, 64.0, 64.0), u0=0.05, re=20.0, rb=4,
+               sponge_width=8, margin=2.0):
+    """The pinned sphere of `sphere` (BGK on L0) on two levels: the body on
+    the fine level alone, the blocks within `margin` L0 cells of it refined
+    (main-3d.js:482-530, ?refine=body, margin 2), explode interface, static
+    topology.  Returns (AMR descriptor dict, L0 seed, (block_slot,
+    slot_to_block), meta)."""
+    desc, mac, meta = sphere(nx=nx, ny=ny, nz=nz, radius=radius, centre=centre, u0=u0, re=re, trt=False,
+                             sponge_width=sponge_width)
+    bs, sb = refine_near_sphere(nx, ny, nz, rb, centre, radius, margin)
+    amr = {"q": 19, "nx": nx, "ny": ny, "nz": nz, "rb": rb, "maxSlots": len(sb), "omega": desc["omega"],
+           "sponge": desc["sponge"], "body": desc["body"]}
+    return amr, mac, (bs, sb), meta
+
+
 def plate(n=16, re=200.0, u_t=0.05, aspect=0.125, tow=0.0, stream=-0.05, i_star=0.17):
     """The blunt square plate normal to the flow with prescribed relative
     motion (SCENARIOS.card with tow or stream, span 1, tilt 0;

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 2
+#define LBM_ABI_VERSION 3
 
 enum {
   LBM_OK = 0,
@@ -279,6 +279,20 @@ typedef struct lbm_amr_desc {
   int32_t reflux;       /* ?reflux=1 (main-3d.js:1330-1343): seam mass and
                          * momentum flux correction, common_d3_amr_flux.wgsl +
                          * common_d3_amr_reflux.wgsl; needs interface_interp  */
+  float sponge_width;   /* L0 sponge (SPONGE_W, as lbm_desc); <= 0 disables.
+                         * The fine step does not relax, so refined blocks
+                         * must stay out of the band (upload_topology).       */
+  int32_t sponge_axes[3];
+  float sponge_u[3];
+  lbm_body body;        /* pinned sphere (LBM_BODY_SPHERE) or NONE.  It lives
+                         * on the fine level alone (main-3d.js:3127-3136): the
+                         * pool step bounces back off it and holds its
+                         * interior at feq(1, 0) (common_d3_amr_step1.wgsl:
+                         * 234-276, :387-403), and one force pass per
+                         * macro-step integrates it with weight dx^2 = 1/4
+                         * (common_d3_force_pool.wgsl).  Needs the explode
+                         * interface and a static topology that covers every
+                         * L0 cell within 1 of it.                            */
 } lbm_amr_desc;
 
 typedef struct lbm_amr lbm_amr;
@@ -318,6 +332,12 @@ int lbm_amr_read_f(lbm_amr* a, float* f_l0, float* f_pool);
 int lbm_amr_read_criterion(lbm_amr* a, float* crit, uint8_t* want);
 /* [count, slot ...] of the slots in use, ascending. */
 int lbm_amr_read_active(lbm_amr* a, int32_t* list);
+/* Force history of the fine-level body, one FP64 record [fx, fy, fz, tx,
+ * ty, tz] per macro-step in L0 units (fine momentum exchange times dx^2 =
+ * 1/4, torque about the centre in L0 cells), the most recent max_records,
+ * oldest first; as lbm_read_forces.  Replaces the pool force pass's
+ * readback (common_d3_force_pool.wgsl, main-3d.js:3138-3141). */
+int lbm_amr_read_forces(lbm_amr* a, double* out, int32_t max_records, int32_t* count);
 /* FP64 [mass, px, py, pz] of the composite state (unrefined L0 cells plus
  * fine interiors weighted 1/8). */
 int lbm_amr_read_totals(lbm_amr* a, double* out4);

@@ -199,7 +199,8 @@ def active_list(slot_to_block):
 class AmrOracle:
     """desc keys: q, nx, ny, nz, rb, maxSlots, omega, omegaMinus (TRT on L0
     if present), force, manageEvery, refineAt, keepAt, lead, explodeLinear,
-    orphans, exactWeights."""
+    orphans, exactWeights, body ({a, centre}: a pinned sphere on the fine
+    level, in L0 units)."""
 
     def __init__(self, desc):
         self.T = T = Tables(int(desc["q"]), bool(desc.get("exactWeights", False)))
@@ -231,6 +232,11 @@ class AmrOracle:
         if self.reflux_on and not self.interp_iface:
             raise ValueError("reflux needs the interp interface")
         self.flux_acc = None
+        body = desc.get("body")
+        self.body = None
+        if body:
+            self.body = (np.array(body["centre"], dtype=np.float32), np.float32(body["a"]))
+        self.forces = []
         self.block_slot = np.full(self.nblocks, -1, dtype=np.int32)
         self.slot_to_block = np.full(self.max_slots, -1, dtype=np.int32)
         n = self.fb
@@ -267,6 +273,52 @@ class AmrOracle:
         d = np.where(d < -(nf // 2), d + nf, d)
         d = np.where(d > nf // 2, d - nf, d)
         return d + GHOST
+
+    def tile_positions(self, slot):
+        """L0-unit centres (float32) of every fine cell of tile `slot`, ring
+        included: origin - 1/4 + (j - GHOST) / 2 (fineToCoarseUnit,
+        d3-amr.mjs:45-47)."""
+        bx, by, bz = self.block_xyz(int(self.slot_to_block[slot]))
+        j = np.arange(self.fb, dtype=np.float32)
+        h = np.float32(0.5) * (j - np.float32(GHOST)) - np.float32(0.25)
+        z, y, x = np.meshgrid(np.float32(bz * self.rb) + h, np.float32(by * self.rb) + h,
+                              np.float32(bx * self.rb) + h, indexing="ij")
+        return x, y, z
+
+    def phi(self, x, y, z):
+        """Signed distance to the body in binary32, as the dense
+        classification computes it (common_d3_geometry.wgsl:119)."""
+        c, a = self.body
+        dx, dy, dz = (np.asarray(v, dtype=np.float32) - c[k] for k, v in enumerate((x, y, z)))
+        return np.sqrt(dx * dx + dy * dy + dz * dz) - a
+
+    def body_force(self):
+        """The macro-step's one force pass (common_d3_force_pool.wgsl,
+        bounce-back branch, body at rest): every interior fluid cell adds
+        -e_i 2 f_opp(i) for each direction whose geometric source is solid,
+        from the pool state at t; the total is weighted dx^2 = 1/4 and the
+        torque taken about the centre in L0 units."""
+        T, lo, hi = self.T, GHOST, GHOST + self.rb2
+        c = self.body[0].astype(np.float64)
+        out = np.zeros(6)
+        for slot in active_list(self.slot_to_block):
+            x, y, z = (v[lo:hi, lo:hi, lo:hi] for v in self.tile_positions(int(slot)))
+            fluid = self.phi(x, y, z) >= 0
+            fsum = np.zeros((3,) + x.shape)
+            for i in range(1, T.q):
+                e = (T.ex[i], T.ey[i], T.ez[i])
+                src = fluid & (self.phi(x - np.float32(0.5 * e[0]), y - np.float32(0.5 * e[1]),
+                                        z - np.float32(0.5 * e[2])) < 0)
+                m = 2.0 * self.pool[T.opp[i], slot, lo:hi, lo:hi, lo:hi]
+                for k in range(3):
+                    fsum[k] -= np.where(src, e[k] * m, 0.0)
+            fsum *= 0.25
+            r = (x.astype(np.float64) - c[0], y.astype(np.float64) - c[1], z.astype(np.float64) - c[2])
+            out[:3] += fsum.reshape(3, -1).sum(1)
+            out[3] += (r[1] * fsum[2] - r[2] * fsum[1]).sum()
+            out[4] += (r[2] * fsum[0] - r[0] * fsum[2]).sum()
+            out[5] += (r[0] * fsum[1] - r[1] * fsum[0]).sum()
+        self.forces.append(out)
 
     # --- set-up -------------------------------------------------------------
     def upload_topology(self, block_slot, slot_to_block):
@@ -341,11 +393,26 @@ class AmrOracle:
         for slot in active_list(self.slot_to_block):
             srcs = self._tile_sources(int(slot))
             g = np.stack([flat[i][srcs[i]] for i in range(self.q)])
+            if self.body is not None:
+                # halfway bounce-back off the body at rest: a direction whose
+                # geometric source is solid takes this cell's own opposite
+                # population (common_d3_amr_step1.wgsl:258-276)
+                x, y, z = self.tile_positions(int(slot))
+                solid = self.phi(x, y, z) < 0
+                for i in range(1, T.q):
+                    hit = self.phi(x - np.float32(0.5 * T.ex[i]), y - np.float32(0.5 * T.ey[i]),
+                                   z - np.float32(0.5 * T.ez[i])) < 0
+                    g[i] = np.where(hit, self.pool[T.opp[i], slot], g[i])
             rho, ux, uy, uz = moments(T, g, self.F)
             for k, v in enumerate((rho, ux, uy, uz)):
                 self.mac_pool[k, slot] = v
             col = collide(T, g, rho, ux, uy, uz, self.omega_f, self.omega_f, self.F)
             new[:, slot] = col if self.interp_iface else np.where(interior, col, g)
+            if self.body is not None:
+                # the solid interior held at feq(1, 0) (:387-403)
+                new[:, slot] = np.where(solid, T.w.reshape(-1, 1, 1, 1), new[:, slot])
+                for k, v in enumerate((1.0, 0.0, 0.0, 0.0)):
+                    self.mac_pool[k, slot] = np.where(solid, v, self.mac_pool[k, slot])
         self.pool = new
 
     def explode(self):
@@ -564,6 +631,8 @@ class AmrOracle:
         if self.interp_iface:
             self._macro_step_interp()
         else:
+            if self.body is not None:
+                self.body_force()
             self.explode()
             self.pool_step()
             self.pool_step()

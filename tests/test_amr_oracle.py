@@ -114,6 +114,44 @@ def test_all_refined_is_a_dense_fine_lattice():
     assert np.abs(fine_global(o).reshape(o.q, -1) - js["f"]).max() < 2e-9
 
 
+@pytest.mark.parametrize("q", [19, 27])
+def test_all_refined_body_is_the_dense_body_at_twice_the_resolution(q):
+    """With every block refined the fine level is a dense lattice of 2n
+    cells whose cell J sits at (J - 1/2) / 2 in L0 units, so a sphere of
+    radius a at c on the fine level is the dense sphere of radius 2a at
+    2c + 1/2 (the binary32 signed distance scales exactly by 2).  The pool
+    state must follow the dense JS oracle's bounce-back and SOLID_EQ step,
+    and each macro-step's force record must be the dense force of the same
+    fine state times dx^2 = 1/4 (torque 1/8: the arm is in L0 cells).  A
+    wrong source position, reflected direction, weight or arm fails this."""
+    n, rb = 12, 4
+    c, a = [f32(5.3), f32(6.1), f32(5.7)], f32(2.6)
+    desc = {"q": q, "nx": n, "ny": n, "nz": n, "rb": rb, "maxSlots": 27, "omega": f32(1 / 0.62),
+            "body": {"a": a, "centre": c}}
+    o = ao.AmrOracle(desc)
+    o.upload_topology(*topology(o, range(o.nblocks)))
+    mac = smooth_field(n, n, n, 9)
+    mac[:, 1] += 0.04
+    o.upload_macro(mac)
+    o.pool = o.pool.astype(np.float32).astype(np.float64)
+    f_start = fine_global(o).reshape(o.q, -1)
+    for _ in range(3):
+        o.macro_step()
+    cf = [2 * v + 0.5 for v in c]
+    js = run_js({"q": q, "nx": 2 * n, "ny": 2 * n, "nz": 2 * n, "omega": o.omega_f, "solidEq": 1,
+                 "body": {"a": 2 * a, "centre": cf}}, f=f_start, steps=6, want=("f", "force", "cls"))
+    assert (js["cls"] & 1).sum() > 100  # a real body, not an empty set
+    assert np.abs(fine_global(o).reshape(o.q, -1) - js["f"]).max() < 2e-9
+    # the JS oracle rounds omega_f to binary32, so the states agree to 2e-9
+    # after the first record and the forces to that relative size
+    amr_f = np.array(o.forces)
+    dense_f = js["force"][::2]
+    assert np.abs(amr_f[0] - np.r_[0.25 * dense_f[0, :3], 0.125 * dense_f[0, 3:]]).max() < 1e-13
+    assert np.abs(dense_f[:, 0]).min() > 1e-3
+    np.testing.assert_allclose(amr_f[:, :3], 0.25 * dense_f[:, :3], rtol=0, atol=1e-8 * np.abs(dense_f).max())
+    np.testing.assert_allclose(amr_f[:, 3:], 0.125 * dense_f[:, 3:], rtol=0, atol=1e-8 * np.abs(dense_f).max())
+
+
 SEAMS = {
     # flat faces, convex edges and corners, a concave corner, a lone block
     # and a block that wraps the periodic boundary
@@ -323,3 +361,41 @@ const cases = JSON.parse(require('fs').readFileSync(0, 'utf8'));
         assert nbs.tolist() == jbs and nsb.tolist() == jsb and exh == jexh
         act = ao.active_list(nsb)
         assert jact == [len(act)] + act.tolist()
+
+
+def test_sphere_refinement_matches_the_js_module():
+    """pylbm.scenarios.refine_near_sphere (the host side of ?refine=body)
+    against the JS module's refineNearBody with the same sphere SDF, and the
+    geometric fact the fine-level body relies on: every unrefined block lies
+    more than the margin from the surface (checked by brute force on its
+    box's cell centres)."""
+    sys.path.insert(0, os.path.join(ROOT, "getting-air_amd"))
+    from pylbm import scenarios as sc
+    cases = [(192, 128, 128, 4, (48.0, 64.0, 64.0), 8.0, 2.0), (32, 32, 32, 4, (15.3, 16.7, 14.1), 3.7, 2.0),
+             (48, 32, 40, 8, (20.5, 15.0, 19.25), 6.0, 1.0)]
+    js = """
+import { makePool, refineNearBody } from '%s';
+const cases = JSON.parse(require('fs').readFileSync(0, 'utf8'));
+""" % os.path.join(ROOT, "getting-air_amd", "js", "d3-amr.mjs")
+    script = ("import { createRequire } from 'module';\nconst require = createRequire(import.meta.url);\n" + js +
+              "const out = cases.map(([nx, ny, nz, rb, c, a, m]) => {"
+              " const pool = makePool({ dims: [nx, ny, nz], rb: rb, maxSlots: nx * ny * nz / (rb * rb * rb) });"
+              " const sdf = (p) => Math.hypot(p[0] - c[0], p[1] - c[1], p[2] - c[2]) - a;"
+              " return Array.from(refineNearBody(pool, sdf, m).blockSlot); });\n"
+              "process.stdout.write(JSON.stringify(out));\n")
+    path = os.path.join(ROOT, "tests", "js", ".near_body_probe.mjs")
+    with open(path, "w") as fh:
+        fh.write(script)
+    try:
+        r = subprocess.run(["node", path], input=json.dumps(cases), capture_output=True, text=True, timeout=60)
+    finally:
+        os.remove(path)
+    assert r.returncode == 0, r.stderr
+    for (nx, ny, nz, rb, c, a, m), jbs in zip(cases, json.loads(r.stdout)):
+        bs, sb = sc.refine_near_sphere(nx, ny, nz, rb, c, a, m)
+        assert bs.tolist() == jbs and (sb >= 0).sum() > 0
+        nbx, nby = nx // rb, ny // rb
+        for b in np.nonzero(bs < 0)[0]:
+            lo = np.array([b % nbx, (b // nbx) % nby, b // (nbx * nby)]) * rb
+            g = np.stack(np.meshgrid(*[np.arange(v, v + rb) for v in lo], indexing="ij"), -1).reshape(-1, 3)
+            assert (np.sqrt(((g - np.array(c)) ** 2).sum(1)) - a).min() > m
