@@ -3,7 +3,7 @@
 // Run: node tests/js_gpu/test-host-gpu.mjs   (needs an MI355X)
 import { createRequire } from 'module';
 import { DenseSolver, AmrSolver, ServeClient, sphereScenario, tgvScenario, deviceInfo } from '../../getting-air_amd/js/lbm-host.mjs';
-import { makePool, refineWhere, manageSlots, activeSlotList } from '../../getting-air_amd/js/d3-amr.mjs';
+import { makePool, refineWhere, refineNearBody, manageSlots, activeSlotList } from '../../getting-air_amd/js/d3-amr.mjs';
 
 const require = createRequire(import.meta.url);
 const O = require('../../oracle/lbm_oracle.js');
@@ -16,6 +16,7 @@ async function ok(name, fn) {
   }
 }
 function assert(c, m) { if (!c) throw new Error(m); }
+function pool_blocks(dims) { return dims[0] * dims[1] * dims[2] / 64; }
 function relErr(gpuMac, oracleMac, comp) {
   let num = 0, den = 0;
   const n = oracleMac.length / 4;
@@ -220,6 +221,22 @@ function relErr(gpuMac, oracleMac, comp) {
     const eq = Math.max(...[b.qw, b.qx, b.qy, b.qz].map((v, k) => Math.abs(v - o.B.q[k])));
     assert(eu < 1e-5 && ex < 1e-6 && eq < 1e-6, 'u ' + eu + ' x ' + ex + ' q ' + eq);
     return 'u ' + eu.toExponential(2) + ', x ' + ex.toExponential(2) + ', vx ' + b.vx.toExponential(3);
+  });
+
+  await ok('AMR sphere on the fine level (sphere-amr-Re20-D16) through the host', async () => {
+    const sc = sphereScenario({ re: 20, R: 8, u0: 0.05, spongeWidth: 8 });
+    const pool = makePool({ dims: sc.params.dims, rb: 4, maxSlots: pool_blocks(sc.params.dims) });
+    const c = sc.params.centre;
+    const topo = refineNearBody(pool, (p) => Math.hypot(p[0] - c[0], p[1] - c[1], p[2] - c[2]) - 8, 2);
+    const desc = { q: 19, nx: sc.desc.nx, ny: sc.desc.ny, nz: sc.desc.nz, rb: 4, maxSlots: topo.activeSlots,
+      omega: sc.desc.omega, sponge: { width: 8, axes: [1, 1, 1], u: [0.05, 0, 0] }, body: sc.desc.body };
+    const s = new AmrSolver({ desc: desc, macro: sc.macro, topology: topo });
+    await s.reset();
+    await s.debugStepSync(30 * 320);
+    const b = await s.readBody();
+    const cd = b.fx / (0.5 * 0.05 * 0.05 * sc.params.area);
+    assert(topo.activeSlots === 160 && Math.abs(cd / 2.9416 - 1) < 0.005, 'tiles ' + topo.activeSlots + ' Cd ' + cd);
+    return topo.activeSlots + ' tiles, Cd ' + cd.toFixed(4) + ' (reference 2.9416)';
   });
 
   await ok('invalid descriptor raises with the library message', () => {
