@@ -168,7 +168,21 @@ def measure_plate():
         s.sync()
         f = s.read_forces(1)
     out["sphere_re20"] = float(f[-1, 0] / (0.5 * 0.05 ** 2 * np.pi * 64.0))
+    # the same sphere on the fine level of two (sphere-amr-Re20-D16): 2.9416
+    adesc, amac, (bs, sb), _ = sc.sphere_amr()
+    with pylbm.Amr(adesc) as a:
+        a.upload_topology(bs, sb)
+        a.upload_macro(amac)
+        a.step(1)
+        a.read_forces(1)  # waits for the stream
+        t0 = time.perf_counter()
+        a.step(30 * 320 - 1)
+        f = a.read_forces(1)
+        amr_ms = (time.perf_counter() - t0) * 1e3 / (30 * 320 - 1)
+    out["sphere_amr"] = float(f[-1, 0] / (0.5 * 0.05 ** 2 * np.pi * 64.0))
     return {"sphere_re20_d16_cd": out["sphere_re20"], "sphere_re20_d16_cd_reference_measured": 2.9436,
+            "sphere_amr_re20_d16_cd": out["sphere_amr"], "sphere_amr_re20_d16_cd_reference_measured": 2.9416,
+            "sphere_amr_re20_d16_tiles": int((sb >= 0).sum()), "sphere_amr_re20_d16_ms_per_macro_step": amr_ms,
             "plate_cd_pinned": out["pinned"], "plate_cd_pinned_reference_measured": 1.2081, "plate_cd_literature": 1.18,
             "plate_cd_half_tow": out["half_tow"], "plate_half_tow_vs_pinned": out["half_tow"] / out["pinned"] - 1.0,
             "plate_half_tow_vs_pinned_reference_measured": -0.0321,
