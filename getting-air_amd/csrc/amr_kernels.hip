@@ -370,18 +370,18 @@ k_pool_force(const float* __restrict__ gp, PoolGeom G, const int32_t* __restrict
 template <int Q, int RB>
 __global__ void __launch_bounds__(256)
 k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const uint32_t* __restrict__ ring,
-          uint32_t nring, int linear) {
+          uint32_t nring, int linear, int per) {
+  using L = Lattice<Q>;
   using T = Tile<RB>;
   constexpr int FB = T::FB, RB2 = T::RB2;
   const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nring) return;
-  // One thread per (ring cell, direction i = grid row): a ring cell's Q
-  // populations are independent, and a launch of nring threads alone leaves
-  // too few waves to hide its chain of dependent lookups.
-  const int i = blockIdx.y;
-  int ex, ey, ez;
-  float wi;
-  direction<Q>(i, ex, ey, ez, wi);
+  // Grid row g takes directions [g per, (g + 1) per): a ring cell's Q
+  // populations are independent, so a small ring spreads them over threads
+  // (one thread per ring cell leaves too few waves to hide the chain of
+  // dependent lookups) and a large one keeps them in one thread (which
+  // shares the index work among them).
+  const int i0 = (int)blockIdx.y * per, i1 = min(Q, i0 + per);
   // a ring cell over an unrefined L0 cell (k_ring_plan)
   const uint32_t cell = ring[t];
   const int slot = (int)(cell / (uint32_t)T::CELLS);
@@ -408,32 +408,38 @@ k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, cons
   const float dx = ((float)(gx - 2 * vx) - 0.5f) * 0.5f;
   const float dy = ((float)(gy - 2 * vy) - 0.5f) * 0.5f;
   const float dz = ((float)(gz - 2 * vz) - 0.5f) * 0.5f;
-  auto l0 = [&](uint32_t c) { return gc[lat_elem(Q, i, c)]; };  // population i of L0 cell c
-  float out;
-  if (!covered_at(G, vx + ex, vy + ey, vz + ez)) {
-    // The coarse target is unrefined: only a population that streams from
-    // this ring cell into this tile's interior is kept (in-orphan, uniform
-    // injection); everything else is zeroed (f = 0, i.e. g = -w).
-    const int dxg = wrap1(gx + ex, NFx), dyg = wrap1(gy + ey, NFy), dzg = wrap1(gz + ez, NFz);
-    const bool into = covered_at(G, dxg >> 1, dyg >> 1, dzg >> 1) && (dxg >> (T::LOG_RB + 1)) == bx &&
-                      (dyg >> (T::LOG_RB + 1)) == by && (dzg >> (T::LOG_RB + 1)) == bz;
-    out = into ? l0(vc) : -wi;
-  } else if (!linear || i == 0) {
-    out = l0(vc);
-  } else {
-    // linear reconstruction d . G with the gradient projected
-    // perpendicular to e_i (common_d3_amr_explode.wgsl:197-212)
-    float Gx = okx ? 0.5f * (l0(cpx) - l0(cmx)) : 0.f;
-    float Gy = oky ? 0.5f * (l0(cpy) - l0(cmy)) : 0.f;
-    float Gz = okz ? 0.5f * (l0(cpz) - l0(cmz)) : 0.f;
-    const float ee = (float)(ex * ex + ey * ey + ez * ez);
-    const float proj = (Gx * ex + Gy * ey + Gz * ez) / ee;
-    Gx -= ex * proj;
-    Gy -= ey * proj;
-    Gz -= ez * proj;
-    out = l0(vc) + (dx * Gx + dy * Gy + dz * Gz);
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    if (i < i0 || i >= i1) continue;  // wave-uniform
+    const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
+    const float wi = L::w[i];
+    auto l0 = [&](uint32_t c) { return gc[lat_elem(Q, i, c)]; };  // population i of L0 cell c
+    float out;
+    if (!covered_at(G, vx + ex, vy + ey, vz + ez)) {
+      // The coarse target is unrefined: only a population that streams from
+      // this ring cell into this tile's interior is kept (in-orphan, uniform
+      // injection); everything else is zeroed (f = 0, i.e. g = -w).
+      const int dxg = wrap1(gx + ex, NFx), dyg = wrap1(gy + ey, NFy), dzg = wrap1(gz + ez, NFz);
+      const bool into = covered_at(G, dxg >> 1, dyg >> 1, dzg >> 1) && (dxg >> (T::LOG_RB + 1)) == bx &&
+                        (dyg >> (T::LOG_RB + 1)) == by && (dzg >> (T::LOG_RB + 1)) == bz;
+      out = into ? l0(vc) : -wi;
+    } else if (!linear || i == 0) {
+      out = l0(vc);
+    } else {
+      // linear reconstruction d . G with the gradient projected
+      // perpendicular to e_i (common_d3_amr_explode.wgsl:197-212)
+      float Gx = okx ? 0.5f * (l0(cpx) - l0(cmx)) : 0.f;
+      float Gy = oky ? 0.5f * (l0(cpy) - l0(cmy)) : 0.f;
+      float Gz = okz ? 0.5f * (l0(cpz) - l0(cmz)) : 0.f;
+      const float ee = (float)(ex * ex + ey * ey + ez * ez);
+      const float proj = (Gx * ex + Gy * ey + Gz * ez) / ee;
+      Gx -= ex * proj;
+      Gy -= ey * proj;
+      Gz -= ez * proj;
+      out = l0(vc) + (dx * Gx + dy * Gy + dz * Gz);
+    }
+    *at(gp + (size_t)i * G.pool_plane, cell) = out;
   }
-  *at(gp + (size_t)i * G.pool_plane, cell) = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -1019,7 +1025,12 @@ template <int Q, int RB>
 struct OpExplode {
   static hipError_t run(const float* gc, float* gp, const PoolGeom& g, const SurfacePlan& plan, uint32_t nring,
                         int linear, hipStream_t s) {
-    hipLaunchKernelGGL((k_explode<Q, RB>), grid_for(nring, Q), dim3(256), 0, s, gc, gp, g, plan.ring, nring, linear);
+    // about 2^20 threads: every direction its own thread for a small ring,
+    // all Q in one thread for a ring of a million cells
+    const int groups = (int)std::min<uint64_t>(Q, std::max<uint64_t>(1, ((1u << 20) + nring - 1) / nring));
+    const int per = (Q + groups - 1) / groups;
+    hipLaunchKernelGGL((k_explode<Q, RB>), grid_for(nring, (Q + per - 1) / per), dim3(256), 0, s, gc, gp, g, plan.ring,
+                       nring, linear, per);
     return hipGetLastError();
   }
 };
