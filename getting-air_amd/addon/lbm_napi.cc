@@ -6,6 +6,7 @@
 #define NAPI_VERSION 8
 #include <node_api.h>
 
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -57,11 +58,74 @@ void num_array(napi_env env, napi_value obj, const char* name, double* out, int 
   }
 }
 
-lbm_solve* handle(napi_env env, napi_value v) {
+// Every engine object reaches JavaScript as an external holding a Handle:
+// the object's kind, the element counts its buffers must hold (checked
+// before native code touches a typed array) and the number of asynchronous
+// calls in flight on it.  While one is in flight, destroy and the
+// synchronous calls throw instead of racing with the libuv thread.  The
+// Handle outlives destroy (marked dead) and is freed by the GC finalizer.
+enum Kind { kSolve = 1, kAmr = 2, kServe = 3 };
+struct Handle {
   void* p = nullptr;
-  napi_get_value_external(env, v, &p);
-  return static_cast<lbm_solve*>(p);
+  Kind kind = kSolve;
+  size_t cells = 0;       // solve: lattice cells; amr: L0 cells
+  int q = 0;
+  size_t nblocks = 0;     // amr: L0 blocks
+  size_t slots = 0;       // amr: pool slots
+  std::atomic<int> busy{0};
+  bool dead = false;
+};
+
+void handle_finalize(napi_env, void* data, void*) {
+  Handle* h = static_cast<Handle*>(data);
+  if (h->busy.load() == 0) delete h;  // else in-flight work still reads it (a few bytes leak)
 }
+
+napi_value wrap(napi_env env, Handle* h) {
+  napi_value ext;
+  if (napi_create_external(env, h, handle_finalize, nullptr, &ext) != napi_ok) {
+    napi_throw_error(env, nullptr, "N-API call failed: napi_create_external");
+    return nullptr;
+  }
+  return ext;
+}
+
+// The Handle of argument v, or null with a pending exception: not a handle
+// of this kind, destroyed, or (idle) an asynchronous call still in flight.
+Handle* get_handle(napi_env env, napi_value v, Kind kind, bool idle) {
+  static const char* names[] = {"", "solve", "AMR solve", "server"};
+  void* p = nullptr;
+  napi_valuetype t;
+  if (napi_typeof(env, v, &t) != napi_ok || t != napi_external || napi_get_value_external(env, v, &p) != napi_ok ||
+      !p || static_cast<Handle*>(p)->kind != kind) {
+    napi_throw_type_error(env, nullptr, (std::string("expected a ") + names[kind] + " handle").c_str());
+    return nullptr;
+  }
+  Handle* h = static_cast<Handle*>(p);
+  if (h->dead) {
+    napi_throw_error(env, nullptr, (std::string("the ") + names[kind] + " was destroyed").c_str());
+    return nullptr;
+  }
+  if (idle && h->busy.load() != 0) {
+    napi_throw_error(env, nullptr,
+                     (std::string("the ") + names[kind] + " has an asynchronous call in flight; await it first").c_str());
+    return nullptr;
+  }
+  return h;
+}
+
+// Throws a RangeError unless a typed array of `len` elements holds `need`.
+bool long_enough(napi_env env, size_t len, size_t need, const char* what) {
+  if (len >= need) return true;
+  std::string m = std::string(what) + ": the array holds " + std::to_string(len) + " elements, " +
+                  std::to_string(need) + " needed";
+  napi_throw_range_error(env, nullptr, m.c_str());
+  return false;
+}
+
+#define GET(var, arg, kind, idle)                      \
+  Handle* var##_h = get_handle(env, arg, kind, idle); \
+  if (!var##_h) return nullptr
 
 // createSolve({q, nx, ny, nz, collision: 'bgk'|'trt', omega, omegaMinus,
 //              force, walls, wallU, sponge: {width, axes, u},
@@ -142,18 +206,26 @@ napi_value CreateSolve(napi_env env, napi_callback_info info) {
   lbm_solve* s = nullptr;
   int rc = lbm_solve_create(&d, &s);
   if (rc) return throw_lbm(env, rc);
-  napi_value ext;
-  NAPI_OK(napi_create_external(env, s, nullptr, nullptr, &ext));
-  return ext;
+  Handle* h = new Handle();
+  h->p = s;
+  h->kind = kSolve;
+  h->q = d.q;
+  h->cells = (size_t)d.nx * d.ny * d.nz;
+  return wrap(env, h);
 }
 
 napi_value Destroy(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  lbm_solve_destroy(handle(env, argv[0]));
+  GET(s, argv[0], kSolve, true);
+  lbm_solve_destroy(static_cast<lbm_solve*>(s_h->p));
+  s_h->p = nullptr;
+  s_h->dead = true;
   return nullptr;
 }
+
+lbm_solve* solve_of(Handle* h) { return static_cast<lbm_solve*>(h->p); }
 
 // Typed-array argument as (data, length in elements).
 bool typed(napi_env env, napi_value v, napi_typedarray_type want, void** data, size_t* len) {
@@ -177,7 +249,9 @@ napi_value UploadMacro(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "uploadMacro(solve, Float32Array)");
     return nullptr;
   }
-  int rc = lbm_upload_macro(handle(env, argv[0]), static_cast<float*>(data));
+  GET(s, argv[0], kSolve, true);
+  if (!long_enough(env, len, 4 * s_h->cells, "uploadMacro")) return nullptr;
+  int rc = lbm_upload_macro(solve_of(s_h), static_cast<float*>(data));
   if (rc) return throw_lbm(env, rc);
   return nullptr;
 }
@@ -192,7 +266,9 @@ napi_value UploadF(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "uploadF(solve, Float32Array)");
     return nullptr;
   }
-  int rc = lbm_upload_f(handle(env, argv[0]), static_cast<float*>(data));
+  GET(s, argv[0], kSolve, true);
+  if (!long_enough(env, len, (size_t)s_h->q * s_h->cells, "uploadF")) return nullptr;
+  int rc = lbm_upload_f(solve_of(s_h), static_cast<float*>(data));
   if (rc) return throw_lbm(env, rc);
   return nullptr;
 }
@@ -210,7 +286,8 @@ napi_value SaveState(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  lbm_solve* s = handle(env, argv[0]);
+  GET(sh, argv[0], kSolve, true);
+  lbm_solve* s = solve_of(sh_h);
   size_t n = 0;
   int rc = lbm_state_size(s, &n);
   if (rc) return throw_lbm(env, rc);
@@ -233,13 +310,15 @@ napi_value LoadState(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "loadState(solve, ArrayBuffer | Uint8Array)");
     return nullptr;
   }
-  int rc = lbm_load_state(handle(env, argv[0]), data, len);
+  GET(s, argv[0], kSolve, true);
+  int rc = lbm_load_state(solve_of(s_h), data, len);
   if (rc) return throw_lbm(env, rc);
   return nullptr;
 }
 
-// readInto(solve, TypedArray): fills a caller-sized array.
-template <int (*Fn)(lbm_solve*, float*)>
+// readInto(solve, TypedArray): fills a caller-sized array of PerCell floats
+// per cell (0: q, the populations).
+template <int (*Fn)(lbm_solve*, float*), int PerCell>
 napi_value ReadFloat(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
@@ -250,7 +329,9 @@ napi_value ReadFloat(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "expected (solve, Float32Array)");
     return nullptr;
   }
-  int rc = Fn(handle(env, argv[0]), static_cast<float*>(data));
+  GET(s, argv[0], kSolve, true);
+  if (!long_enough(env, len, (size_t)(PerCell ? PerCell : s_h->q) * s_h->cells, "read")) return nullptr;
+  int rc = Fn(solve_of(s_h), static_cast<float*>(data));
   if (rc) return throw_lbm(env, rc);
   return argv[1];
 }
@@ -261,8 +342,9 @@ napi_value ReadTotals(napi_env env, napi_callback_info info) {
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   napi_value ab, arr;
   void* data;
+  GET(s, argv[0], kSolve, true);
   NAPI_OK(napi_create_arraybuffer(env, 32, &data, &ab));
-  int rc = lbm_read_totals(handle(env, argv[0]), static_cast<double*>(data));
+  int rc = lbm_read_totals(solve_of(s_h), static_cast<double*>(data));
   if (rc) return throw_lbm(env, rc);
   NAPI_OK(napi_create_typedarray(env, napi_float64_array, 4, ab, 0, &arr));
   return arr;
@@ -278,8 +360,9 @@ napi_value ReadForces(napi_env env, napi_callback_info info) {
   napi_value ab, arr;
   void* data;
   NAPI_OK(napi_create_arraybuffer(env, (size_t)maxr * 48 + 8, &data, &ab));
+  GET(s, argv[0], kSolve, true);
   int32_t count = 0;
-  int rc = lbm_read_forces(handle(env, argv[0]), static_cast<double*>(data), maxr, &count);
+  int rc = lbm_read_forces(solve_of(s_h), static_cast<double*>(data), maxr, &count);
   if (rc) return throw_lbm(env, rc);
   NAPI_OK(napi_create_typedarray(env, napi_float64_array, (size_t)count * 6, ab, 0, &arr));
   return arr;
@@ -295,7 +378,9 @@ napi_value ReadClasses(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "readClasses(solve, Uint8Array)");
     return nullptr;
   }
-  int rc = lbm_read_classes(handle(env, argv[0]), static_cast<uint8_t*>(data));
+  GET(s, argv[0], kSolve, true);
+  if (!long_enough(env, len, s_h->cells, "readClasses")) return nullptr;
+  int rc = lbm_read_classes(solve_of(s_h), static_cast<uint8_t*>(data));
   if (rc) return throw_lbm(env, rc);
   return argv[1];
 }
@@ -303,7 +388,7 @@ napi_value ReadClasses(napi_env env, napi_callback_info info) {
 struct StepWork {
   napi_async_work work;
   napi_deferred deferred;
-  lbm_solve* s;
+  Handle* h;
   int n;
   int rc;
   std::string err;
@@ -311,13 +396,15 @@ struct StepWork {
 
 void StepExecute(napi_env, void* p) {
   StepWork* w = static_cast<StepWork*>(p);
-  w->rc = lbm_step(w->s, w->n);
-  if (!w->rc) w->rc = lbm_sync(w->s);
+  lbm_solve* s = solve_of(w->h);
+  w->rc = lbm_step(s, w->n);
+  if (!w->rc) w->rc = lbm_sync(s);
   if (w->rc) w->err = lbm_last_error();
 }
 
 void StepComplete(napi_env env, napi_status, void* p) {
   StepWork* w = static_cast<StepWork*>(p);
+  w->h->busy.fetch_sub(1);
   if (w->rc) {
     napi_value msg, err;
     std::string m = "lbm error " + std::to_string(w->rc) + ": " + w->err;
@@ -333,20 +420,30 @@ void StepComplete(napi_env env, napi_status, void* p) {
   delete w;
 }
 
+// Queues async work that holds `h` busy until its complete callback.
+bool queue_busy(napi_env env, napi_async_work work, Handle* h) {
+  h->busy.fetch_add(1);
+  if (napi_queue_async_work(env, work) == napi_ok) return true;
+  h->busy.fetch_sub(1);
+  napi_throw_error(env, nullptr, "N-API call failed: napi_queue_async_work");
+  return false;
+}
+
 // step(solve, n) -> Promise resolved once the n steps have completed.
 napi_value Step(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  GET(s, argv[0], kSolve, true);
   StepWork* w = new StepWork();
-  w->s = handle(env, argv[0]);
+  w->h = s_h;
   w->n = 1;
   if (argc > 1) napi_get_value_int32(env, argv[1], &w->n);
   napi_value promise, name;
   NAPI_OK(napi_create_promise(env, &w->deferred, &promise));
   NAPI_OK(napi_create_string_utf8(env, "lbm_step", NAPI_AUTO_LENGTH, &name));
   NAPI_OK(napi_create_async_work(env, nullptr, name, StepExecute, StepComplete, w, &w->work));
-  NAPI_OK(napi_queue_async_work(env, w->work));
+  if (!queue_busy(env, w->work, s_h)) return nullptr;
   return promise;
 }
 
@@ -356,7 +453,8 @@ napi_value Info(napi_env env, napi_callback_info info) {
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   int64_t step = 0, bytes = 0;
   int32_t bx = 0, by = 0, flags = 0;
-  int rc = lbm_solve_info(handle(env, argv[0]), &step, &bx, &by, &flags, &bytes);
+  GET(s, argv[0], kSolve, true);
+  int rc = lbm_solve_info(solve_of(s_h), &step, &bx, &by, &flags, &bytes);
   if (rc) return throw_lbm(env, rc);
   napi_value o, v;
   NAPI_OK(napi_create_object(env, &o));
@@ -418,7 +516,8 @@ napi_value RigidAttach(napi_env env, napi_callback_info info) {
   num_array(env, argv[1], "window", w3, 3);
   for (int k = 0; k < 3; k++) b.window[k] = w3[k] != 0;
   num_array(env, argv[1], "anchor", b.window_anchor, 3);
-  int rc = lbm_rigid_attach(handle(env, argv[0]), &b);
+  GET(s, argv[0], kSolve, true);
+  int rc = lbm_rigid_attach(solve_of(s_h), &b);
   if (rc) return throw_lbm(env, rc);
   return nullptr;
 }
@@ -428,7 +527,8 @@ napi_value RigidRead(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   lbm_rigid_state st;
-  int rc = lbm_rigid_read(handle(env, argv[0]), &st);
+  GET(s, argv[0], kSolve, true);
+  int rc = lbm_rigid_read(solve_of(s_h), &st);
   if (rc) return throw_lbm(env, rc);
   napi_value o;
   NAPI_OK(napi_create_object(env, &o));
@@ -476,11 +576,7 @@ napi_value DeviceInfo(napi_env env, napi_callback_info) {
 // ---------------------------------------------------------------------------
 // Two-level AMR solve (lbm_amr_*)
 // ---------------------------------------------------------------------------
-lbm_amr* amr_handle(napi_env env, napi_value v) {
-  void* p = nullptr;
-  napi_get_value_external(env, v, &p);
-  return static_cast<lbm_amr*>(p);
-}
+lbm_amr* amr_of(Handle* h) { return static_cast<lbm_amr*>(h->p); }
 
 // createAmr({q, nx, ny, nz, rb, maxSlots, omega, omegaMinus, force,
 //            manageEvery, refineAt, keepAt, lead, explodeLinear, orphans,
@@ -551,16 +647,24 @@ napi_value CreateAmr(napi_env env, napi_callback_info info) {
   lbm_amr* a = nullptr;
   int rc = lbm_amr_create(&d, &a);
   if (rc) return throw_lbm(env, rc);
-  napi_value ext;
-  NAPI_OK(napi_create_external(env, a, nullptr, nullptr, &ext));
-  return ext;
+  Handle* h = new Handle();
+  h->p = a;
+  h->kind = kAmr;
+  h->q = d.q;
+  h->cells = (size_t)d.nx * d.ny * d.nz;
+  h->nblocks = d.rb > 0 ? (size_t)(d.nx / d.rb) * (d.ny / d.rb) * (d.nz / d.rb) : 0;
+  h->slots = (size_t)d.max_slots;
+  return wrap(env, h);
 }
 
 napi_value AmrDestroy(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  lbm_amr_destroy(amr_handle(env, argv[0]));
+  GET(a, argv[0], kAmr, true);
+  lbm_amr_destroy(amr_of(a_h));
+  a_h->p = nullptr;
+  a_h->dead = true;
   return nullptr;
 }
 
@@ -577,7 +681,10 @@ napi_value AmrTopology(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "expected (amr, Int32Array blockSlot, Int32Array slotToBlock)");
     return nullptr;
   }
-  lbm_amr* a = amr_handle(env, argv[0]);
+  GET(ah, argv[0], kAmr, true);
+  if (!long_enough(env, nb, ah_h->nblocks, "blockSlot") || !long_enough(env, ns, ah_h->slots, "slotToBlock"))
+    return nullptr;
+  lbm_amr* a = amr_of(ah_h);
   const int rc = Upload ? lbm_amr_upload_topology(a, static_cast<int32_t*>(bs), static_cast<int32_t*>(sb))
                         : lbm_amr_read_topology(a, static_cast<int32_t*>(bs), static_cast<int32_t*>(sb));
   if (rc) return throw_lbm(env, rc);
@@ -596,7 +703,9 @@ napi_value AmrFloat(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "expected (amr, Float32Array)");
     return nullptr;
   }
-  int rc = Fn(amr_handle(env, argv[0]), static_cast<float*>(data));
+  GET(a, argv[0], kAmr, true);
+  if (!long_enough(env, len, 4 * a_h->cells, "AMR macroscopic field")) return nullptr;
+  int rc = Fn(amr_of(a_h), static_cast<float*>(data));
   if (rc) return throw_lbm(env, rc);
   return argv[1];
 }
@@ -613,7 +722,10 @@ napi_value AmrReadCriterion(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "amrReadCriterion(amr, Float32Array, Uint8Array)");
     return nullptr;
   }
-  int rc = lbm_amr_read_criterion(amr_handle(env, argv[0]), static_cast<float*>(crit), static_cast<uint8_t*>(want));
+  GET(a, argv[0], kAmr, true);
+  if (!long_enough(env, n1, a_h->nblocks, "criterion") || !long_enough(env, n2, a_h->nblocks, "want"))
+    return nullptr;
+  int rc = lbm_amr_read_criterion(amr_of(a_h), static_cast<float*>(crit), static_cast<uint8_t*>(want));
   if (rc) return throw_lbm(env, rc);
   return nullptr;
 }
@@ -628,7 +740,9 @@ napi_value AmrReadActive(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "amrReadActive(amr, Int32Array(maxSlots + 1))");
     return nullptr;
   }
-  int rc = lbm_amr_read_active(amr_handle(env, argv[0]), static_cast<int32_t*>(data));
+  GET(a, argv[0], kAmr, true);
+  if (!long_enough(env, len, a_h->slots + 1, "amrReadActive")) return nullptr;
+  int rc = lbm_amr_read_active(amr_of(a_h), static_cast<int32_t*>(data));
   if (rc) return throw_lbm(env, rc);
   return argv[1];
 }
@@ -645,7 +759,8 @@ napi_value AmrReadForces(napi_env env, napi_callback_info info) {
   void* data;
   NAPI_OK(napi_create_arraybuffer(env, (size_t)maxr * 48 + 8, &data, &ab));
   int32_t count = 0;
-  int rc = lbm_amr_read_forces(amr_handle(env, argv[0]), static_cast<double*>(data), maxr, &count);
+  GET(a, argv[0], kAmr, true);
+  int rc = lbm_amr_read_forces(amr_of(a_h), static_cast<double*>(data), maxr, &count);
   if (rc) return throw_lbm(env, rc);
   NAPI_OK(napi_create_typedarray(env, napi_float64_array, (size_t)count * 6, ab, 0, &arr));
   return arr;
@@ -657,8 +772,9 @@ napi_value AmrReadTotals(napi_env env, napi_callback_info info) {
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   napi_value ab, arr;
   void* data;
+  GET(a, argv[0], kAmr, true);
   NAPI_OK(napi_create_arraybuffer(env, 32, &data, &ab));
-  int rc = lbm_amr_read_totals(amr_handle(env, argv[0]), static_cast<double*>(data));
+  int rc = lbm_amr_read_totals(amr_of(a_h), static_cast<double*>(data));
   if (rc) return throw_lbm(env, rc);
   NAPI_OK(napi_create_typedarray(env, napi_float64_array, 4, ab, 0, &arr));
   return arr;
@@ -668,7 +784,8 @@ napi_value AmrManageNow(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  int rc = lbm_amr_manage_now(amr_handle(env, argv[0]));
+  GET(a, argv[0], kAmr, true);
+  int rc = lbm_amr_manage_now(amr_of(a_h));
   if (rc) return throw_lbm(env, rc);
   return nullptr;
 }
@@ -679,7 +796,8 @@ napi_value AmrInfo(napi_env env, napi_callback_info info) {
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   int64_t step = 0;
   int32_t act = 0, exh = 0;
-  int rc = lbm_amr_info(amr_handle(env, argv[0]), &step, &act, &exh);
+  GET(a, argv[0], kAmr, true);
+  int rc = lbm_amr_info(amr_of(a_h), &step, &act, &exh);
   if (rc) return throw_lbm(env, rc);
   napi_value o, v;
   NAPI_OK(napi_create_object(env, &o));
@@ -695,7 +813,7 @@ napi_value AmrInfo(napi_env env, napi_callback_info info) {
 struct AmrStepWork {
   napi_async_work work;
   napi_deferred deferred;
-  lbm_amr* a;
+  Handle* h;
   int n;
   int rc;
   std::string err;
@@ -703,12 +821,13 @@ struct AmrStepWork {
 
 void AmrStepExecute(napi_env, void* p) {
   AmrStepWork* w = static_cast<AmrStepWork*>(p);
-  w->rc = lbm_amr_step(w->a, w->n);
+  w->rc = lbm_amr_step(amr_of(w->h), w->n);
   if (w->rc) w->err = lbm_last_error();
 }
 
 void AmrStepComplete(napi_env env, napi_status, void* p) {
   AmrStepWork* w = static_cast<AmrStepWork*>(p);
+  w->h->busy.fetch_sub(1);
   if (w->rc) {
     napi_value msg, err;
     std::string m = "lbm error " + std::to_string(w->rc) + ": " + w->err;
@@ -729,26 +848,23 @@ napi_value AmrStep(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  GET(a, argv[0], kAmr, true);
   AmrStepWork* w = new AmrStepWork();
-  w->a = amr_handle(env, argv[0]);
+  w->h = a_h;
   w->n = 1;
   if (argc > 1) napi_get_value_int32(env, argv[1], &w->n);
   napi_value promise, name;
   NAPI_OK(napi_create_promise(env, &w->deferred, &promise));
   NAPI_OK(napi_create_string_utf8(env, "lbm_amr_step", NAPI_AUTO_LENGTH, &name));
   NAPI_OK(napi_create_async_work(env, nullptr, name, AmrStepExecute, AmrStepComplete, w, &w->work));
-  NAPI_OK(napi_queue_async_work(env, w->work));
+  if (!queue_busy(env, w->work, a_h)) return nullptr;
   return promise;
 }
 
 // ---------------------------------------------------------------------------
 // Batched serving (lbm_serve_*)
 // ---------------------------------------------------------------------------
-lbm_serve* serve_handle(napi_env env, napi_value v) {
-  void* p = nullptr;
-  napi_get_value_external(env, v, &p);
-  return static_cast<lbm_serve*>(p);
-}
+lbm_serve* serve_of(Handle* h) { return static_cast<lbm_serve*>(h->p); }
 
 int64_t ticket_of(napi_env env, napi_value v) {
   int64_t t = 0;
@@ -776,16 +892,20 @@ napi_value ServeCreate(napi_env env, napi_callback_info info) {
   lbm_serve* sv = nullptr;
   int rc = lbm_serve_create(&d, &sv);
   if (rc) return throw_lbm(env, rc);
-  napi_value ext;
-  NAPI_OK(napi_create_external(env, sv, nullptr, nullptr, &ext));
-  return ext;
+  Handle* h = new Handle();
+  h->p = sv;
+  h->kind = kServe;
+  return wrap(env, h);
 }
 
 napi_value ServeDestroy(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  lbm_serve_destroy(serve_handle(env, argv[0]));
+  GET(sv, argv[0], kServe, true);  // no serveWait in flight
+  lbm_serve_destroy(serve_of(sv_h));
+  sv_h->p = nullptr;
+  sv_h->dead = true;
   return nullptr;
 }
 
@@ -810,7 +930,8 @@ napi_value ServeSubmit(napi_env env, napi_callback_info info) {
   int32_t steps = 0;
   napi_get_value_int32(env, argv[3], &steps);
   int64_t t = 0;
-  int rc = lbm_serve_submit(serve_handle(env, argv[0]), &d, static_cast<float*>(data), steps, &t);
+  GET(sv, argv[0], kServe, false);
+  int rc = lbm_serve_submit(serve_of(sv_h), &d, static_cast<float*>(data), steps, &t);
   if (rc) return throw_lbm(env, rc);
   napi_value v;
   NAPI_OK(napi_create_int64(env, t, &v));
@@ -853,7 +974,8 @@ napi_value ServeSubmitState(napi_env env, napi_callback_info info) {
   bool keep = false;
   napi_get_value_bool(env, argv[4], &keep);
   int64_t t = 0;
-  int rc = lbm_serve_submit_state(serve_handle(env, argv[0]), &d, mac, state, state_bytes, steps, keep ? 1 : 0, &t);
+  GET(sv, argv[0], kServe, false);
+  int rc = lbm_serve_submit_state(serve_of(sv_h), &d, mac, state, state_bytes, steps, keep ? 1 : 0, &t);
   if (rc) return throw_lbm(env, rc);
   napi_value v;
   NAPI_OK(napi_create_int64(env, t, &v));
@@ -865,7 +987,8 @@ napi_value ServeReadState(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  lbm_serve* sv = serve_handle(env, argv[0]);
+  GET(svh, argv[0], kServe, false);
+  lbm_serve* sv = serve_of(svh_h);
   const int64_t t = ticket_of(env, argv[1]);
   size_t n = 0;
   int rc = lbm_serve_read_state(sv, t, nullptr, &n);
@@ -881,7 +1004,7 @@ napi_value ServeReadState(napi_env env, napi_callback_info info) {
 struct WaitWork {
   napi_async_work work;
   napi_deferred deferred;
-  lbm_serve* sv;
+  Handle* h;
   int64_t ticket;
   double timeout;
   lbm_serve_result r;
@@ -891,12 +1014,13 @@ struct WaitWork {
 
 void WaitExecute(napi_env, void* p) {
   WaitWork* w = static_cast<WaitWork*>(p);
-  w->rc = lbm_serve_wait(w->sv, w->ticket, w->timeout, &w->r);
+  w->rc = lbm_serve_wait(serve_of(w->h), w->ticket, w->timeout, &w->r);
   if (w->rc) w->err = lbm_last_error();
 }
 
 void WaitComplete(napi_env env, napi_status, void* p) {
   WaitWork* w = static_cast<WaitWork*>(p);
+  w->h->busy.fetch_sub(1);
   if (w->rc) {
     napi_value msg, err;
     std::string m = "lbm error " + std::to_string(w->rc) + ": " + w->err;
@@ -935,8 +1059,9 @@ napi_value ServeWait(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value argv[3];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  GET(sv, argv[0], kServe, false);
   WaitWork* w = new WaitWork();
-  w->sv = serve_handle(env, argv[0]);
+  w->h = sv_h;
   w->ticket = ticket_of(env, argv[1]);
   w->timeout = -1;
   if (argc > 2) napi_get_value_double(env, argv[2], &w->timeout);
@@ -944,7 +1069,7 @@ napi_value ServeWait(napi_env env, napi_callback_info info) {
   NAPI_OK(napi_create_promise(env, &w->deferred, &promise));
   NAPI_OK(napi_create_string_utf8(env, "lbm_serve_wait", NAPI_AUTO_LENGTH, &name));
   NAPI_OK(napi_create_async_work(env, nullptr, name, WaitExecute, WaitComplete, w, &w->work));
-  NAPI_OK(napi_queue_async_work(env, w->work));
+  if (!queue_busy(env, w->work, sv_h)) return nullptr;
   return promise;
 }
 
@@ -959,7 +1084,13 @@ napi_value ServeReadMacro(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "serveReadMacro(server, ticket, Float32Array)");
     return nullptr;
   }
-  int rc = lbm_serve_read_macro(serve_handle(env, argv[0]), ticket_of(env, argv[1]), static_cast<float*>(data));
+  GET(sv, argv[0], kServe, false);
+  // the request's cell count, from its result (finished requests return at once)
+  lbm_serve_result res;
+  int rc = lbm_serve_wait(serve_of(sv_h), ticket_of(env, argv[1]), 0.0, &res);
+  if (rc) return throw_lbm(env, rc);
+  if (!long_enough(env, len, 4 * (size_t)res.cells, "serveReadMacro")) return nullptr;
+  rc = lbm_serve_read_macro(serve_of(sv_h), ticket_of(env, argv[1]), static_cast<float*>(data));
   if (rc) return throw_lbm(env, rc);
   return argv[2];
 }
@@ -976,7 +1107,8 @@ napi_value ServeReadForces(napi_env env, napi_callback_info info) {
   void* data;
   NAPI_OK(napi_create_arraybuffer(env, (size_t)maxr * 48 + 8, &data, &ab));
   int32_t count = 0;
-  int rc = lbm_serve_read_forces(serve_handle(env, argv[0]), ticket_of(env, argv[1]), static_cast<double*>(data), maxr,
+  GET(sv, argv[0], kServe, false);
+  int rc = lbm_serve_read_forces(serve_of(sv_h), ticket_of(env, argv[1]), static_cast<double*>(data), maxr,
                                  &count);
   if (rc) return throw_lbm(env, rc);
   NAPI_OK(napi_create_typedarray(env, napi_float64_array, (size_t)count * 6, ab, 0, &arr));
@@ -987,7 +1119,8 @@ napi_value ServeRelease(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
-  int rc = lbm_serve_release(serve_handle(env, argv[0]), ticket_of(env, argv[1]));
+  GET(sv, argv[0], kServe, false);
+  int rc = lbm_serve_release(serve_of(sv_h), ticket_of(env, argv[1]));
   if (rc) return throw_lbm(env, rc);
   return nullptr;
 }
@@ -997,7 +1130,8 @@ napi_value ServeStats(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   lbm_serve_stats st;
-  int rc = lbm_serve_read_stats(serve_handle(env, argv[0]), &st);
+  GET(sv, argv[0], kServe, false);
+  int rc = lbm_serve_read_stats(serve_of(sv_h), &st);
   if (rc) return throw_lbm(env, rc);
   napi_value o, v;
   NAPI_OK(napi_create_object(env, &o));
@@ -1021,8 +1155,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"destroy", Destroy},
       {"uploadMacro", UploadMacro},
       {"uploadF", UploadF},
-      {"readF", ReadFloat<lbm_read_f>},
-      {"readMacro", ReadFloat<lbm_read_macro>},
+      {"readF", ReadFloat<lbm_read_f, 0>},
+      {"readMacro", ReadFloat<lbm_read_macro, 4>},
       {"readTotals", ReadTotals},
       {"readForces", ReadForces},
       {"readClasses", ReadClasses},

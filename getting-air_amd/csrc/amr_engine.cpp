@@ -349,6 +349,7 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
     release(a);
     return rc;
   }
+  solve_no_eager_graphs(a->l0);
   a->stream = solve_stream(a->l0);
   const int rb = desc->rb;
   a->fb = 2 * rb + 2 * kGhost;

@@ -128,6 +128,10 @@ struct lbm_solve {
   int graph_bx = 0, graph_by = 0, graph_swz = 0, graph_nt = 0, graph_occ = 0, graph_lin = 0;
   Arena* arena = nullptr;           // pool the buffers come from (serve loop), or null
   bool own_stream = true;
+  // Both parity graphs and the autotuned tile are made when the state is set
+  // (upload, load, tile change), so lbm_step only replays.  Off for serve
+  // members and the AMR engine's L0 solve, which never run lbm_step chunks.
+  bool eager_graphs = true;
   // moving rigid body (lbm_rigid_attach): device record, force partials,
   // launch arguments of its kernels
   lbm::RigidBody* rigid = nullptr;
@@ -359,6 +363,7 @@ float* solve_lattice(lbm_solve* s, int which) {
   return static_cast<float*>(s->f[which ? 1 - s->parity : s->parity]);
 }
 float* solve_mac(lbm_solve* s) { return s->mac; }
+void solve_no_eager_graphs(lbm_solve* s) { s->eager_graphs = false; }
 hipStream_t solve_stream(lbm_solve* s) { return s->stream; }
 
 }  // namespace lbm
@@ -395,6 +400,31 @@ int build_graph(lbm_solve* s) {
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
   s->graph_bx = s->bx; s->graph_by = s->by; s->graph_swz = s->swz; s->graph_nt = s->nt; s->graph_occ = s->occ;
   s->graph_lin = s->lin;
+  return LBM_OK;
+}
+
+int autotune(lbm_solve* s);
+
+// Autotunes (when the descriptor asks and no tile is set yet) and captures
+// the graph chunks of both lattice parities, so that the first long
+// lbm_step neither times tiles nor instantiates graphs.
+int prepare_graphs(lbm_solve* s) {
+  if (!s->eager_graphs || !s->initialised) return LBM_OK;
+  if (s->d.autotune && !s->tuned) {
+    int rc = autotune(s);
+    if (rc) return rc;
+  }
+  if (s->graph_bx != s->bx || s->graph_by != s->by || s->graph_swz != s->swz || s->graph_nt != s->nt ||
+      s->graph_occ != s->occ || s->graph_lin != s->lin)
+    drop_graphs(s);
+  const int p0 = s->parity;
+  for (int p = 0; p < 2; p++) {
+    if (s->graph[p]) continue;
+    s->parity = p;
+    int rc = build_graph(s);
+    s->parity = p0;
+    if (rc) return rc;
+  }
   return LBM_OK;
 }
 
@@ -546,6 +576,7 @@ int create_impl(const lbm_desc* desc, Arena* arena, hipStream_t stream, int bx, 
   if (rc) return rc;
   lbm_solve* s = new lbm_solve();
   s->arena = arena;
+  s->eager_graphs = arena == nullptr;
   s->d = *desc;
   s->q = desc->q;
   s->dim = desc->q == 9 ? 2 : 3;
@@ -750,7 +781,7 @@ int lbm_upload_macro(lbm_solve* s, const float* mac4) {
   int rc = reset_history(s);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
-  return LBM_OK;
+  return prepare_graphs(s);
 }
 
 int lbm_upload_f(lbm_solve* s, const float* f) {
@@ -771,7 +802,9 @@ int lbm_upload_f(lbm_solve* s, const float* f) {
   s->salt0 = 0;
   s->step_at_salt0 = 0;
   s->initialised = true;
-  return reset_history(s);
+  int rc = reset_history(s);
+  if (rc) return rc;
+  return prepare_graphs(s);
 }
 
 // Copies between the lattice's block layout and plane-major order:
@@ -858,7 +891,9 @@ int lbm_load_state(lbm_solve* s, const void* buf, size_t bytes) {
   s->step_at_salt0 = h.step;
   s->initialised = true;
   s->force_pending = false;
-  return reset_history(s);
+  rc = reset_history(s);
+  if (rc) return rc;
+  return prepare_graphs(s);
 }
 
 int lbm_read_f(lbm_solve* s, float* f) {
@@ -1038,7 +1073,9 @@ int lbm_set_tile(lbm_solve* s, int32_t block_x, int32_t block_y, int32_t flags) 
   s->lin = (flags >> 2) & 1;
   s->occ = occ;
   s->tuned = true;
-  return build_force_slots(s);
+  int rc = build_force_slots(s);
+  if (rc) return rc;
+  return prepare_graphs(s);
 }
 
 int lbm_probe_copy_bandwidth(size_t bytes, int32_t reps, double* gbs, double* ms_per_copy) {
@@ -1196,7 +1233,7 @@ int lbm_rigid_attach(lbm_solve* s, const lbm_rigid_body* b) {
     s->tuned = false;
   }
   drop_graphs(s);
-  return LBM_OK;
+  return prepare_graphs(s);
 }
 
 int lbm_rigid_read(lbm_solve* s, lbm_rigid_state* out) {

@@ -20,6 +20,9 @@ int solve_enqueue_covered_step(lbm_solve* s, const int32_t* cover, int rb, int n
 float* solve_lattice(lbm_solve* s, int which);
 float* solve_mac(lbm_solve* s);
 hipStream_t solve_stream(lbm_solve* s);
+// Uploads stop capturing graph chunks ahead of time (a solve stepped only
+// through the hooks above).
+void solve_no_eager_graphs(lbm_solve* s);
 
 // Hooks the serve loop uses.  A solve can live in a larger device pool: its
 // buffers then come from `Arena` instead of hipMalloc, and it runs on the

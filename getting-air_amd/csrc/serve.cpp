@@ -325,12 +325,18 @@ int grow(T** p, size_t* cap, size_t n) {
 
 // Uploads every class's solve table for this chunk and, when membership
 // changed, its tile table.  Returns whether the chunk graph must be rebuilt.
+// The instantiated graph holds the tables' device addresses, so it is
+// destroyed before any table is reallocated.
 int prepare(lbm_serve* sv, bool* rebuild) {
   *rebuild = sv->graph == nullptr;
   for (auto& kv : sv->classes) {
     Class* c = kv.second.get();
     if (c->dirty) {
       *rebuild = true;
+      if (sv->graph) {
+        (void)hipGraphExecDestroy(sv->graph);
+        sv->graph = nullptr;
+      }
       std::vector<BatchTile>& tiles = c->h_tiles;
       tiles.clear();
       for (uint32_t k = 0; k < c->members.size(); k++)

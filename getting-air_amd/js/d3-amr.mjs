@@ -3,7 +3,7 @@
 // arguments and return shapes (plain objects, Int32Array maps, Sets of
 // "bx,by,bz" keys).  The HIP manager and pool kernels
 // (getting-air_amd/csrc/amr_kernels.hip) implement the same addressing, and
-// tests/js/test-d3-amr.mjs checks this module by independent routes.
+// tests/js/test-d3-amr.mjs checks it by independent routes.
 //
 // Coordinates: COARSE (L0 cell), BLOCK (RB^3 coarse cells), TILE-LOCAL (fine
 // cell in a slot, [0, FB)^3 with FB = 2 RB + 2 GHOST, interior

@@ -19,8 +19,9 @@
 // body velocity and window through stepSlab's `R`.  Moving walls (the 2D
 // Couette form, shaders/common_walls.wgsl:27-31) keep their correction term.
 //
-// Parity pins for this file live in tests/js/test-oracle.js and
-// tests/test_oracle.py (closed forms and invariants, see those files).
+// Parity pins for this file live in tests/js/test-oracle.js (closed forms,
+// invariants and the sponge profile's table) and tests/test_amr_oracle.py
+// (the AMR oracle's no-refinement and all-refined limits against this one).
 
 // --- lattice tables (restated; lattice-2d.mjs:97-145, lattice-3d.mjs:146-200)
 function d3Tables(Q) {
@@ -150,6 +151,19 @@ function normalise(d) {
   return out;
 }
 
+// Sponge weight (common_d3_sponge.wgsl:10-14; the 2D form common_sponge.wgsl:7-11
+// takes the minimum of two distances): cubic smoothstep of
+// clamp(1 - min(dx, dy, dz) / width, 0, 1), the minimum over the axes (not a
+// product), a masked axis passing 1e30.  Called through `hooks` so that the
+// pin in tests/js/test-oracle.js can plant a wrong profile and watch it fail.
+function spongeWeight3(dx, dy, dz, sw) {
+  if (!(sw > 0)) return 0;
+  let t = 1 - Math.min(dx, Math.min(dy, dz)) / sw;
+  t = t < 0 ? 0 : (t > 1 ? 1 : t);
+  return t * t * (3 - 2 * t);
+}
+const hooks = { spongeWeight3: spongeWeight3 };
+
 // One fused pull-stream + collide over the x-rows r = z * NY + y in [r0, r1).
 // `fin`/`fout` are plane-major: f[i * N + (z * NY + y) * NX + x].  `mac`
 // receives [rho, ux, uy, uz] per cell when non-null.  `R` (optional) is a
@@ -259,9 +273,7 @@ function stepSlab(S, T, cls, fin, fout, mac, r0, r1, R) {
           const dX = sax[0] ? Math.min(wp[0], NX - 1 - wp[0]) : 1e30;
           const dY = sax[1] ? Math.min(wp[1], NY - 1 - wp[1]) : 1e30;
           const dZ = sax[2] ? Math.min(wp[2], NZ - 1 - wp[2]) : 1e30;
-          let t = 1 - Math.min(dX, Math.min(dY, dZ)) / sw;
-          t = t < 0 ? 0 : (t > 1 ? 1 : t);
-          sp = t * t * (3 - 2 * t);
+          sp = hooks.spongeWeight3(dX, dY, dZ, sw);
         }
         // 4. collide: BGK line plus the TRT delta terms.
         for (let i = 0; i < Q; i++) {
@@ -385,5 +397,5 @@ class DenseOracle {
 
 module.exports = {
   tables, classify, phiF32, normalise, stepSlab, forceSlab, DenseOracle,
-  CLS_SOLID, CLS_NEAR, roundHalfEven,
+  CLS_SOLID, CLS_NEAR, roundHalfEven, spongeWeight3, hooks,
 };

@@ -14,6 +14,10 @@
 //     uniformly forced box equals the total force applied to the fluid
 //   - the f32-emulated classification of a sphere at an integer centre equals
 //     the exact integer test |p - c|^2 < a^2
+//   - the sponge profile read back cell by cell after one step from rest
+//     equals the smoothstep table of common_d3_sponge.wgsl (and a linear ramp
+//     or a per-axis product planted in the oracle fails that check)
+//   - D3Q19 / D3Q27 plane Couette between two moving y walls is exact
 // Run: node tests/js/test-oracle.js   (exit code 1 on any failure)
 
 const O = require('../../oracle/lbm_oracle.js');
@@ -205,5 +209,107 @@ ok('uniform flow at the sponge velocity stays uniform under the sponge (D3Q19)',
   for (let c = 0; c < o.N; c++) worst = Math.max(worst, Math.abs(o.mac[4 * c + 1] - Math.fround(0.05)), Math.abs(o.mac[4 * c] - 1));
   assert(worst < 1e-14, 'deviation ' + worst);
 });
+
+// The sponge profile.  From rest (f = w_i) streaming and the collision leave
+// every population at w_i (the f32 weights sum to 1 exactly, so rho = 1), so
+// after one step f_i = (1 - s) w_i + s feq_i(1, U) and s can be read back
+// per cell.  It must follow the reference's spongeWeight3 (smoothstep of
+// clamp(1 - d / W), common_d3_sponge.wgsl:10-14) with d the distance to the
+// nearest enabled face, x = 0 and x = NX - 1 both being faces
+// (common_d3_step.wgsl:317-320).  At width 5 the smoothstep takes the values
+// tabled here by hand for d = 0..5 (t = 1, .8, .6, .4, .2, 0).
+const SPONGE_TABLE_W5 = [1, 0.896, 0.648, 0.352, 0.104, 0];
+function spongeReadBack(axes) {
+  const NX = 20, NY = 16, NZ = 12, U = [Math.fround(0.05), 0, 0];
+  const o = new O.DenseOracle({ q: 19, nx: NX, ny: NY, nz: NZ, omega: 1 / 0.6, sponge: { width: 5, axes: axes, u: U } });
+  const N = NX * NY * NZ;
+  o.initEq(uniformMac(N, 1, [0, 0, 0]));
+  o.step(1);
+  const w1 = o.T.w[1];  // direction 1 is e = (1, 0, 0)
+  const target = w1 * (1 + 3 * U[0] + 4.5 * U[0] * U[0] - 1.5 * U[0] * U[0]);
+  const s = new Float64Array(N);
+  for (let c = 0; c < N; c++) s[c] = (o.f[1 * N + c] - w1) / (target - w1);
+  return { s: s, at: (x, y, z) => s[(z * NY + y) * NX + x], NX: NX, NY: NY, NZ: NZ };
+}
+function spongeProfileError(axes) {
+  const r = spongeReadBack(axes);
+  let worst = 0;
+  for (let z = 0; z < r.NZ; z++) {
+    for (let y = 0; y < r.NY; y++) {
+      for (let x = 0; x < r.NX; x++) {
+        const d = Math.min(axes[0] ? Math.min(x, r.NX - 1 - x) : 99, axes[1] ? Math.min(y, r.NY - 1 - y) : 99,
+          axes[2] ? Math.min(z, r.NZ - 1 - z) : 99);
+        worst = Math.max(worst, Math.abs(r.at(x, y, z) - SPONGE_TABLE_W5[Math.min(d, 5)]));
+      }
+    }
+  }
+  return { worst: worst, r: r };
+}
+
+ok('sponge profile: one step from rest gives the smoothstep table at every cell, faces at 0 and N-1, min at corners', () => {
+  const all = spongeProfileError([1, 1, 1]);
+  assert(all.worst < 1e-14, 'all axes: worst deviation ' + all.worst);
+  const r = all.r;
+  // single cells named explicitly: the far face x = NX-1, a corner (min, not a
+  // product: 0.896 * 0.896 = 0.803 would be a product), the interior
+  const cases = [[0, 8, 6, 1], [19, 8, 6, 1], [18, 8, 6, 0.896], [1, 1, 6, 0.896], [2, 8, 6, 0.648],
+    [17, 13, 6, 0.648], [3, 8, 6, 0.352], [4, 8, 6, 0.104], [5, 8, 6, 0], [10, 8, 5, 0], [1, 2, 11, 1]];
+  for (const [x, y, z, want] of cases) {
+    assert(Math.abs(r.at(x, y, z) - want) < 1e-14, 's(' + x + ',' + y + ',' + z + ') = ' + r.at(x, y, z) + ' want ' + want);
+  }
+  const masked = spongeProfileError([1, 0, 1]);
+  assert(masked.worst < 1e-14, 'y masked: worst deviation ' + masked.worst);
+  assert(Math.abs(masked.r.at(10, 0, 6)) < 1e-14 && Math.abs(r.at(10, 0, 6) - 1) < 1e-14, 'masked y face');
+  return 'worst ' + Math.max(all.worst, masked.worst).toExponential(1);
+});
+
+ok('sponge profile pin has teeth: a linear ramp and a per-axis product both fail it', () => {
+  const real = O.hooks.spongeWeight3;
+  const clampT = (d, sw) => { const t = 1 - d / sw; return t < 0 ? 0 : (t > 1 ? 1 : t); };
+  const smooth = (t) => t * t * (3 - 2 * t);
+  const mutants = {
+    linear: (dx, dy, dz, sw) => clampT(Math.min(dx, dy, dz), sw),
+    product: (dx, dy, dz, sw) => 1 - (1 - smooth(clampT(dx, sw))) * (1 - smooth(clampT(dy, sw))) * (1 - smooth(clampT(dz, sw))),
+    productOfWeights: (dx, dy, dz, sw) => smooth(clampT(dx, sw)) * smooth(clampT(dy, sw)) * smooth(clampT(dz, sw)),
+  };
+  const out = [];
+  try {
+    for (const name of Object.keys(mutants)) {
+      O.hooks.spongeWeight3 = mutants[name];
+      const e = spongeProfileError([1, 1, 1]).worst;
+      assert(e > 1e-2, name + ' mutant passes the pin (deviation ' + e + ')');
+      out.push(name + ' ' + e.toFixed(3));
+    }
+  } finally {
+    O.hooks.spongeWeight3 = real;
+  }
+  return out.join(', ');
+});
+
+// Moving y-walls in 3D (this build's extension of the reference's stationary
+// halfway walls, common_d3_step.wgsl:264-269, with the 2D moving-wall term of
+// common_walls.wgsl:27-31): BGK plane Couette between a bottom wall moving at
+// U0 and a top wall at U1 is linear, u_x(y) = U0 + (U1 - U0)(y + 1/2)/H, and
+// halfway bounce-back reproduces it exactly.
+for (const q of [19, 27]) {
+  ok('BGK plane Couette between two moving y walls is exact (D3Q' + q + ')', () => {
+    const H = 10, U0 = Math.fround(-0.004), U1 = Math.fround(0.01), nx = 2, nz = 2;
+    const o = new O.DenseOracle({ q: q, nx: nx, ny: H, nz: nz, omega: 1 / 0.8, walls: [0, 1, 0], wallU: [U0, U1], exactWeights: true });
+    o.initEq(uniformMac(nx * H * nz, 1, [0, 0, 0]));
+    o.step(8000);
+    let worst = 0, cross = 0;
+    for (let z = 0; z < nz; z++) {
+      for (let y = 0; y < H; y++) {
+        const c = (z * H + y) * nx;
+        const exact = U0 + (U1 - U0) * (y + 0.5) / H;
+        worst = Math.max(worst, Math.abs(o.mac[4 * c + 1] - exact) / U1);
+        cross = Math.max(cross, Math.abs(o.mac[4 * c + 2]), Math.abs(o.mac[4 * c + 3]));
+      }
+    }
+    assert(worst < 1e-7, 'max err / U ' + worst);
+    assert(cross < 1e-12, 'cross-stream velocity ' + cross);
+    return worst.toExponential(1);
+  });
+}
 
 process.exitCode = failures ? 1 : 0;

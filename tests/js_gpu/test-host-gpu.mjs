@@ -245,5 +245,33 @@ function relErr(gpuMac, oracleMac, comp) {
     assert(/omega/.test(msg), 'message: ' + msg);
   });
 
+  await ok('addon: short arrays raise RangeError, calls during a step in flight and after destroy raise', async () => {
+    const a = (await import('../../getting-air_amd/js/lbm-host.mjs')).loadAddon();
+    const h = a.createSolve({ q: 19, nx: 16, ny: 8, nz: 8, omega: 1.2 });
+    const n = 16 * 8 * 8;
+    const throwsAs = (fn, cls, re) => {
+      let e = null;
+      try { fn(); } catch (x) { e = x; }
+      assert(e instanceof cls && re.test(e.message), 'expected ' + cls.name + ' ' + re + ', got ' + e);
+    };
+    throwsAs(() => a.uploadMacro(h, new Float32Array(4 * n - 1)), RangeError, /needed/);
+    const mac = new Float32Array(4 * n);
+    for (let c = 0; c < n; c++) mac[4 * c] = 1;
+    a.uploadMacro(h, mac);
+    throwsAs(() => a.readMacro(h, new Float32Array(4 * n - 4)), RangeError, /needed/);
+    throwsAs(() => a.readF(h, new Float32Array(19 * n - 1)), RangeError, /needed/);
+    throwsAs(() => a.uploadF(h, new Float32Array(18 * n)), RangeError, /needed/);
+    throwsAs(() => a.readClasses(h, new Uint8Array(n - 1)), RangeError, /needed/);
+    const p = a.step(h, 64);
+    throwsAs(() => a.readMacro(h, new Float32Array(4 * n)), Error, /in flight/);
+    throwsAs(() => a.destroy(h), Error, /in flight/);
+    await p;
+    a.readMacro(h, mac);
+    assert(Math.abs(mac[0] - 1) < 1e-6, 'rho ' + mac[0]);
+    a.destroy(h);
+    throwsAs(() => a.readMacro(h, mac), Error, /destroyed/);
+    throwsAs(() => a.amrInfo(h), TypeError, /AMR solve handle/);
+  });
+
   process.exitCode = failures ? 1 : 0;
 })();
