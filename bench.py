@@ -275,18 +275,7 @@ def measure_amr_slab(steps, reflux):
 
 
 def enstrophy(mac, n):
-    """0.5 <|curl u|^2> with central differences on the periodic box
-    (main-3d.js:3627-3630), u from [rho, ux, uy, uz] planes."""
-    m = mac.reshape(n, n, n, 4).astype(np.float64)
-    u = [m[..., k] for k in (1, 2, 3)]
-
-    def d(c, axis):  # axis 0 = z, 1 = y, 2 = x
-        return 0.5 * (np.roll(u[c], -1, axis) - np.roll(u[c], 1, axis))
-
-    wx = d(2, 1) - d(1, 0)
-    wy = d(0, 0) - d(2, 2)
-    wz = d(1, 2) - d(0, 1)
-    return 0.5 * float(np.mean(wx * wx + wy * wy + wz * wz))
+    return sc.enstrophy(mac, n)
 
 
 def measure_turbulence(n=512, steps=2000, n_small=128, threads=16):

@@ -580,8 +580,8 @@ lbm_amr* amr_of(Handle* h) { return static_cast<lbm_amr*>(h->p); }
 
 // createAmr({q, nx, ny, nz, rb, maxSlots, omega, omegaMinus, force,
 //            manageEvery, refineAt, keepAt, lead, explodeLinear, orphans,
-//            interface: 'explode' | 'interp', reflux, sponge: {width, axes, u},
-//            body: {a, centre}})
+//            interface: 'explode' | 'interp', reflux, explodeDirs,
+//            sponge: {width, axes, u}, body: {a, centre}})
 napi_value CreateAmr(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
@@ -627,6 +627,7 @@ napi_value CreateAmr(napi_env env, napi_callback_info info) {
     }
   }
   d.reflux = (int)num(env, o, "reflux", 0);
+  d.explode_dirs = (int)num(env, o, "explodeDirs", 0);
   if (get_prop(env, o, "sponge", &v)) {
     d.sponge_width = (float)num(env, v, "width", 0);
     double ax[3] = {1, 1, 1}, su[3] = {0, 0, 0};

@@ -100,8 +100,9 @@ hipError_t launch_flux(int q, const float* gpool, float* acc, const PoolGeom& g,
 // (gc_t1, mac) from the coarse state at t and the accumulated fine flux.
 hipError_t launch_reflux(int q, const float* gc_t, float* gc_t1, float* mac, const PoolGeom& g, const SurfacePlan& plan,
                          uint32_t nflux, float* acc, const float* force, hipStream_t s);
+// dirs: directions per thread (0: about 2^20 threads for the ring).
 hipError_t launch_explode(int q, const float* gcoarse, float* gpool, const PoolGeom& g, const SurfacePlan& plan,
-                          uint32_t nring, int linear, hipStream_t s);
+                          uint32_t nring, int linear, int dirs, hipStream_t s);
 hipError_t launch_coalesce(int q, const float* gpool, float* gcoarse, float* mac, const float* mac_pool,
                            const PoolGeom& g, const int32_t* active, int nactive, const SurfacePlan& plan,
                            uint32_t nseam, uint32_t nnear, int orphans, hipStream_t s);

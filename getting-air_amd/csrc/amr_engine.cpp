@@ -93,6 +93,8 @@ int validate(const lbm_amr_desc* d) {
   if (d->nx / d->rb < 3 || d->ny / d->rb < 3 || d->nz / d->rb < 3)
     return fail(LBM_E_INVALID, "grid %dx%dx%d has fewer than 3 blocks of %d on an axis", d->nx, d->ny, d->nz, d->rb);
   if (d->max_slots < 1) return fail(LBM_E_INVALID, "max_slots %d", d->max_slots);
+  if (d->explode_dirs < 0 || d->explode_dirs > d->q)
+    return fail(LBM_E_INVALID, "explode_dirs %d outside 0..q", d->explode_dirs);
   const double tile = std::pow(2.0 * d->rb + 2 * kGhost, 3);
   if ((double)d->max_slots * tile * 4.0 >= 4294967296.0)
     return fail(LBM_E_INVALID, "pool plane of %d tiles exceeds 4 GiB", d->max_slots);
@@ -307,7 +309,7 @@ int macro_step(lbm_amr* a) {
     ForceFinalizeArgs fa{a->force_partials, pool_force_blocks(d.rb, a->nactive), a->history, a->written, kForceCap};
     HIPCHK(launch_force_finalize(fa, a->stream));
   }
-  HIPCHK(launch_explode(d.q, l0, a->pool[0], a->G, a->plan, a->nring, d.explode_linear, a->stream));
+  HIPCHK(launch_explode(d.q, l0, a->pool[0], a->G, a->plan, a->nring, d.explode_linear, d.explode_dirs, a->stream));
   // only the second substep's field is read (coalesce, lbm_amr_read_pool_macro)
   HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->active, a->nactive, 0, a->stream));
   HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->active, a->nactive, 0,

@@ -1024,11 +1024,11 @@ struct OpReflux {
 template <int Q, int RB>
 struct OpExplode {
   static hipError_t run(const float* gc, float* gp, const PoolGeom& g, const SurfacePlan& plan, uint32_t nring,
-                        int linear, hipStream_t s) {
+                        int linear, int dirs, hipStream_t s) {
     // about 2^20 threads: every direction its own thread for a small ring,
     // all Q in one thread for a ring of a million cells
     const int groups = (int)std::min<uint64_t>(Q, std::max<uint64_t>(1, ((1u << 20) + nring - 1) / nring));
-    const int per = (Q + groups - 1) / groups;
+    const int per = dirs > 0 ? std::min(dirs, Q) : (Q + groups - 1) / groups;
     hipLaunchKernelGGL((k_explode<Q, RB>), grid_for(nring, (Q + per - 1) / per), dim3(256), 0, s, gc, gp, g, plan.ring,
                        nring, linear, per);
     return hipGetLastError();
@@ -1108,9 +1108,9 @@ hipError_t launch_reflux(int q, const float* gct, float* gct1, float* mac, const
 }
 
 hipError_t launch_explode(int q, const float* gc, float* gp, const PoolGeom& g, const SurfacePlan& plan,
-                          uint32_t nring, int linear, hipStream_t s) {
+                          uint32_t nring, int linear, int dirs, hipStream_t s) {
   if (nring == 0) return hipSuccess;
-  return dispatch_q_rb<OpExplode>(q, g.rb, gc, gp, g, plan, nring, linear, s);
+  return dispatch_q_rb<OpExplode>(q, g.rb, gc, gp, g, plan, nring, linear, dirs, s);
 }
 
 hipError_t launch_coalesce(int q, const float* gp, float* gc, float* mac, const float* mac_pool, const PoolGeom& g,

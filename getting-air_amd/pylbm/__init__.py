@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblbm.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 BGK, TRT = 0, 1
 STORE = {"f32": 0, "f16": 1, "bf16": 2}
 BODY_NONE, BODY_SPHERE, BODY_CIRCLE = 0, 1, 2
@@ -60,7 +60,7 @@ class AmrDesc(ctypes.Structure):
         ("lead", ctypes.c_int32), ("explode_linear", ctypes.c_int32), ("orphans", ctypes.c_int32),
         ("interface_interp", ctypes.c_int32), ("reflux", ctypes.c_int32),
         ("sponge_width", ctypes.c_float), ("sponge_axes", ctypes.c_int32 * 3), ("sponge_u", ctypes.c_float * 3),
-        ("body", Body),
+        ("body", Body), ("explode_dirs", ctypes.c_int32),
     ]
 
 
@@ -397,7 +397,8 @@ def make_amr_desc(d):
     """AmrDesc from a dict {q, nx, ny, nz, rb, maxSlots, omega, omegaMinus,
     force, manageEvery, refineAt, keepAt, lead, explodeLinear, orphans,
     interface ('explode' or 'interp'), reflux, sponge: {width, axes, u} on
-    L0, body: {a, centre} a pinned sphere solved on the fine level}."""
+    L0, body: {a, centre} a pinned sphere solved on the fine level,
+    explodeDirs: directions per explode thread (0: automatic)}."""
     a = AmrDesc()
     a.abi_version = ABI_VERSION
     a.q = int(d["q"])
@@ -427,6 +428,7 @@ def make_amr_desc(d):
     for k in range(3):
         a.sponge_axes[k] = int(bool(axes[k]))
         a.sponge_u[k] = float((list(sp.get("u", [0, 0, 0])) + [0, 0, 0])[k])
+    a.explode_dirs = int(d.get("explodeDirs", 0))
     body = d.get("body")
     if body:
         a.body.shape = BODY_SPHERE

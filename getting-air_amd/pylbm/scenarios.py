@@ -295,6 +295,25 @@ def turbulence(n=512, seed=2024, urms=0.04, kpeak=8.0, tau=0.55, q=19, storage="
     return desc, mac.reshape(-1)
 
 
+def enstrophy(mac, n, dtype=np.float64):
+    """0.5 <|curl u|^2> with central differences on the periodic n^3 box
+    (main-3d.js:3627-3630), u from the [rho, ux, uy, uz] field.  dtype
+    float32 halves the memory of a 512^3 evaluation (the mean is still
+    accumulated in float64)."""
+    m = mac.reshape(n, n, n, 4)
+    u = [m[..., k].astype(dtype) for k in (1, 2, 3)]  # ux, uy, uz
+
+    def d(c, axis):  # central difference of u[c]; axis 0 = z, 1 = y, 2 = x
+        return (np.roll(u[c], -1, axis) - np.roll(u[c], 1, axis)) * dtype(0.5)
+
+    acc = 0.0
+    # wx = duz/dy - duy/dz, wy = dux/dz - duz/dx, wz = duy/dx - dux/dy
+    for ca, a, cb, b in ((2, 1, 1, 0), (0, 0, 2, 2), (1, 2, 0, 1)):
+        w = d(ca, a) - d(cb, b)
+        acc += float(np.mean(w * w, dtype=np.float64))
+    return 0.5 * acc
+
+
 def mulberry32(seed):
     """Seeded 32-bit PRNG, the generator the reference seeds its
     perturbations with (d3-scenarios.mjs:207-216); same sequence as the JS

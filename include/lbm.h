@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 3
+#define LBM_ABI_VERSION 4
 
 enum {
   LBM_OK = 0,
@@ -293,6 +293,9 @@ typedef struct lbm_amr_desc {
                          * (common_d3_force_pool.wgsl).  Needs the explode
                          * interface and a static topology that covers every
                          * L0 cell within 1 of it.                            */
+  int32_t explode_dirs; /* directions per explode thread, 1..q; 0: chosen from
+                         * the ring size (about 2^20 threads per launch).  A
+                         * launch-shape knob: results do not depend on it.    */
 } lbm_amr_desc;
 
 typedef struct lbm_amr lbm_amr;

@@ -70,9 +70,10 @@ def test_amr_descriptor_layout_matches_c():
 #include <stddef.h>
 #include "lbm.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_amr_desc), offsetof(lbm_amr_desc, omega),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_amr_desc), offsetof(lbm_amr_desc, omega),
          offsetof(lbm_amr_desc, force), offsetof(lbm_amr_desc, manage_every),
-         offsetof(lbm_amr_desc, lead), offsetof(lbm_amr_desc, orphans), offsetof(lbm_amr_desc, reflux));
+         offsetof(lbm_amr_desc, lead), offsetof(lbm_amr_desc, orphans), offsetof(lbm_amr_desc, reflux),
+         offsetof(lbm_amr_desc, body), offsetof(lbm_amr_desc, explode_dirs));
   return 0;
 }
 """
@@ -84,7 +85,7 @@ int main(void) {
         got = [int(v) for v in subprocess.check_output([exe]).split()]
     A = pylbm.AmrDesc
     want = [ctypes.sizeof(A), A.omega.offset, A.force.offset, A.manage_every.offset, A.lead.offset,
-            A.orphans.offset, A.reflux.offset]
+            A.orphans.offset, A.reflux.offset, A.body.offset, A.explode_dirs.offset]
     assert got == want
 
 
@@ -110,6 +111,9 @@ def test_amr_refusals_without_gpu():
     d = pylbm.make_amr_desc(dict(base, reflux=1))
     assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
     assert b"interp" in lib.lbm_last_error()
+    d = pylbm.make_amr_desc(dict(base, explodeDirs=28))
+    assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"explode_dirs" in lib.lbm_last_error()
 
 
 def test_serve_struct_layouts_match_c():

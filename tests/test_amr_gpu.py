@@ -380,3 +380,71 @@ def test_sphere_amr_re20_matches_the_reference_measurement():
     say("sphere AMR Re20 D16 Cd %.4f (reference AMR 2.9416, dense 2.9436)" % cd)
     assert abs(cd / 2.9416 - 1.0) < 0.005
     assert abs(cd / 2.9436 - 1.0) < 0.005
+
+
+@pytest.mark.parametrize("dirs", [0, 1, 2, 27])
+def test_rb8_seams_match_the_oracle_for_every_explode_split(dirs):
+    """The block size config 3 runs (RB 8: 20^3 tiles, k_pool_step<27, 8> and
+    k_explode<27, 8>) on seams with faces, convex edges and corners, a
+    concave corner, a lone block and a periodic wrap.  The explode kernel
+    gives each thread `dirs` of a ring cell's directions (0: the automatic
+    split, here all 27 directions in separate threads); every split must give
+    the oracle's populations."""
+    n, rb = 32, 8
+    desc = {"q": 27, "nx": n, "ny": n, "nz": n, "rb": rb, "maxSlots": 8, "omega": 1 / 0.6, "explodeDirs": dirs}
+    say("rb 8 seams, explode dirs per thread", dirs)
+    orc0 = ao.AmrOracle(desc)
+    amr, orc = make(desc, smooth_field(n, n, n, 17, amp=0.04), ids(orc0, SEAMS["mixed"]))
+    with amr:
+        t0 = amr.read_totals()
+        for k in range(2):
+            amr.step(2)
+            orc.macro_step()
+            orc.macro_step()
+            compare(amr, orc, label="rb 8 dirs %d macro-step %d" % (dirs, 2 * k + 2))
+        t = amr.read_totals()
+        assert abs(t[0] - t0[0]) < 1e-6 * t0[0]
+
+
+def test_rb8_manager_coarsens_and_refines_bit_exact_and_fields_follow():
+    """RB 8 under the field-criterion manager: a planted layer of refined
+    blocks at z = 0 partly coarsens (drain through k_average<27, 8>) while the
+    vortex rings' blocks refine (interp fill) on the first pass; criterion,
+    decisions, block maps and active list bit-exact, fields within 1e-5."""
+    n, rb = 32, 8
+    desc = {"q": 27, "nx": n, "ny": n, "nz": n, "rb": rb, "maxSlots": 40, "omega": 1 / 0.7,
+            "manageEvery": 2, "lead": 0, "refineAt": 8.2e-4, "keepAt": 7.6e-4}
+    _, mac, _ = scenarios.vortex_rings(n=n, radius=6.0, core=1.5, circulation=0.05, seed=3, jitter=2.0,
+                                       tau=0.7, q=27, rings=2)
+    say("rb 8 manager")
+    orc0 = ao.AmrOracle(desc)
+    amr, orc = make(desc, mac, [orc0.block_id(bx, by, 0) for bx in range(4) for by in range(4)])
+    with amr:
+        prev_bs, prev_sb = amr.read_topology()
+        dying_total = fresh_total = 0
+        for step in range(4):
+            manage = step % 2 == 0
+            if manage:
+                m = amr.read_macro().reshape(n, n, n, 4)
+                want_crit = ao.block_criterion(
+                    ao.vorticity_magnitude(np.moveaxis(m, -1, 0), np.float32), rb, desc["lead"])
+            amr.step(1)
+            if manage:
+                crit, want = amr.read_criterion()
+                assert (crit.view(np.uint32) == want_crit.view(np.uint32)).all(), step
+                dec = ao.decide(prev_bs, crit, np.float32(desc["refineAt"]), np.float32(desc["keepAt"]))
+                assert (want == dec).all(), step
+                bs, sb = amr.read_topology()
+                tbs, tsb, dying, fresh, exh = ao.manage_slots(prev_bs, prev_sb, dec)
+                assert (bs == tbs).all() and (sb == tsb).all(), step
+                assert (amr.read_active() == ao.active_list(tsb)).all(), step
+                assert exh == 0
+                dying_total += len(dying)
+                fresh_total += len(fresh)
+                prev_bs, prev_sb = bs, sb
+                orc.macro_step(crit=crit)
+            else:
+                orc.macro_step()
+            compare(amr, orc, label="rb 8 manager step %d" % step)
+        say("coarsened", dying_total, "refined", fresh_total)
+        assert dying_total > 0 and fresh_total > 0

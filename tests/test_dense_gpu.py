@@ -298,3 +298,46 @@ def test_shedding_sphere_re300_matches_the_reference_measurement():
     print("shedding sphere Cd mean %.4f (reference 0.7487-0.7496), lateral pk %.3f" % (cd.mean(), lateral.max()))
     assert 0.7487 * 0.995 < cd.mean() < 0.7496 * 1.005
     assert lateral.max() > 0.02 * cd.mean()  # it sheds
+
+
+def test_sphere_trt_sponge_5000_steps_matches_oracle():
+    """Config 2's physics (D3Q19 TRT, halfway bounce-back, sponge on every
+    face, per-step FP64 drag/lift) at config 2's step count, on a 64x32x32
+    box: after 5000 steps rho/u within 1e-5 of the FP64 oracle and every
+    step's drag/lift within 1e-6 of the largest drag."""
+    desc, mac, meta = sphere_small(trt=True)
+    steps = 5000
+    print("case sphere 64x32x32 TRT sponge, %d steps" % steps, flush=True)
+    g = run_gpu(desc, mac, steps)
+    r = oracle_runner.run(desc, mac, steps=steps, threads=8, want=("mac", "force"), timeout=1200)
+    e = field_errors(g["mac"], r["mac"])
+    dev = force_dev(g["force"], r["force"])
+    print("after %d steps:" % steps, e, "max force deviation / max|Fx|:", dev)
+    assert g["force"].shape == r["force"].shape == (steps, 6)
+    assert e["rho"] < 1e-5 and e["u"] < 1e-5, e
+    assert dev.max() < 1e-6, dev
+
+
+@pytest.mark.parametrize("re,trt,cd_ref", [(20.0, True, 2.8772), (200.0, False, 0.8622), (200.0, True, 0.8386)])
+def test_sphere_trt_records_match_the_reference_measurements(re, trt, cd_ref):
+    """The reference's recorded pinned-sphere drag under BGK and under TRT at
+    Lambda = 3/16 (plans/TRT.md:736-738): sphere-Re20-D16 moves from 2.9436
+    (BGK, reproduced above) to 2.8772 with TRT; sphere-Re200-D16 at tau
+    0.512 gives 0.8622 (BGK) and 0.8386 (TRT).  Same domain and settling as
+    benchmarks/d3.json sphere-Re20-D16: 192x128x128, D 16 at (48, 64, 64),
+    u0 0.05, sponge 8, Cd after 30 D/U."""
+    import math
+    desc, mac, meta = sc.sphere(nx=192, ny=128, nz=128, radius=8.0, centre=(48.0, 64.0, 64.0), u0=0.05, re=re,
+                                trt=trt, sponge_width=8)
+    steps = 30 * 320
+    print("case sphere Re%g D16 %s" % (re, "TRT" if trt else "BGK"), flush=True)
+    with pylbm.Solve(desc) as s:
+        s.upload_macro(mac)
+        s.step(steps)
+        s.sync()
+        f = s.read_forces(320)
+    cd = f[:, 0] / (0.5 * 0.05 ** 2 * math.pi * 64.0)
+    print("sphere Re%g D16 %s: Cd %.4f (last 1 D/U: %.4f-%.4f), reference %.4f" %
+          (re, "TRT" if trt else "BGK", cd[-1], cd.min(), cd.max(), cd_ref))
+    assert np.all(np.isfinite(cd))
+    assert abs(cd[-1] / cd_ref - 1.0) < 0.005

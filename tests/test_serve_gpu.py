@@ -174,3 +174,51 @@ def test_concurrent_submitters():
         st = sv.stats()
     assert len(results) == len(reqs) and all(r["status"] == 0 for r in results.values())
     assert st["completed"] == len(reqs)
+
+
+def test_config5_mix_at_its_sizes_served_equals_plain_and_matches_the_oracle():
+    """Config 5's request shapes (fp16 storage, fp32 collision): 2D cylinder
+    channels from nx 256 to 1024 and 3D boxed spheres from 64^3 to 192^3, Re
+    from the seeded mix, submitted in a staggered stream with uneven step
+    counts.  Every served result equals the plain single-solve path bit for
+    bit (field and force history); the two smallest members are also scored
+    against the FP64 oracle with the shifted-fp16 bounds of the dense tests
+    (u within 3e-2, mean drag of the last 20 steps within 5e-3)."""
+    import oracle_runner
+
+    mixed = sc.serve_mix(count=4, seed=21, steps=0, storage="f16")
+    reqs = [sc.cylinder2d(nx=256, re=90.0, storage="f16"), sc.cylinder2d(nx=1024, re=400.0, storage="f16"),
+            sc.boxed_sphere(n=64, re=120.0, storage="f16"), sc.boxed_sphere(n=192, re=350.0, storage="f16")]
+    reqs += [(d, m, meta) for _, d, m, meta, _ in mixed]
+    steps = [160, 96, 200, 64, 120, 80, 144, 112]
+    say("mix:", [(d["q"], d["nx"], d["ny"], d.get("nz", 1), round(meta["re"])) for d, _, meta in reqs])
+    served = {}
+    with pylbm.Server(budget_bytes=16 << 30, chunk_steps=32) as sv:
+        tickets = []
+        for k, (d, m, _) in enumerate(reqs):
+            tickets.append(sv.submit(d, m, steps[k]))
+            time.sleep(0.01)
+        sv.drain(600)
+        st = sv.stats()
+        for k, t in enumerate(tickets):
+            assert sv.wait(t, 1)["status"] == 0, k
+            served[k] = (sv.read_macro(t), sv.read_forces(t))
+    say("stats", {k: st[k] for k in ("completed", "peak_resident", "chunks", "p50_latency", "p99_latency", "mlups")})
+    assert st["completed"] == len(reqs) and st["peak_resident"] >= 2
+    for k, (d, m, meta) in enumerate(reqs):
+        with pylbm.Solve(dict(d, autotune=0)) as s:
+            s.upload_macro(m)
+            s.step(steps[k])
+            pmac, pf = s.read_macro(), s.read_forces(steps[k])
+        smac, sf = served[k]
+        say(k, "served == plain:", np.array_equal(smac.view(np.uint32), pmac.view(np.uint32)), np.array_equal(sf, pf))
+        assert np.array_equal(smac.view(np.uint32), pmac.view(np.uint32)), k
+        assert np.array_equal(sf, pf), k
+    for k in (0, 2):
+        d, m, meta = reqs[k]
+        r = oracle_runner.run(dict(d, storage="f32"), m, steps=steps[k], threads=8, want=("mac", "force"))
+        g = served[k][0].reshape(-1, 4).astype(np.float64)
+        eu = float(np.linalg.norm(g[:, 1:] - r["mac"][:, 1:]) / np.linalg.norm(r["mac"][:, 1:]))
+        drag = abs(served[k][1][-20:, 0].mean() / r["force"][-20:, 0].mean() - 1.0)
+        say(k, "vs FP64 oracle: u %.3e, drag %.3e" % (eu, drag))
+        assert eu < 3e-2 and drag < 5e-3, (k, eu, drag)
