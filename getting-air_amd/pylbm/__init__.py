@@ -30,6 +30,7 @@ EXPORTS = (
     "lbm_serve_create", "lbm_serve_destroy", "lbm_serve_submit", "lbm_serve_wait", "lbm_serve_read_macro",
     "lbm_serve_read_forces", "lbm_serve_release", "lbm_serve_drain", "lbm_serve_read_stats", "lbm_serve_clock",
     "lbm_serve_submit_state", "lbm_serve_read_state", "lbm_rigid_attach", "lbm_rigid_read",
+    "lbm_totals_noalloc",
 )
 SHAPE = {"sphere": 0, "spheroid": 1, "roundbox": 2}
 GHOST = 2
@@ -161,6 +162,7 @@ def load():
         lib.lbm_read_macro.argtypes = [P, P]
         lib.lbm_read_forces.argtypes = [P, P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
         lib.lbm_read_totals.argtypes = [P, P]
+        lib.lbm_totals_noalloc.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, P, P, P]
         lib.lbm_read_classes.argtypes = [P, P]
         lib.lbm_solve_info.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),

@@ -158,6 +158,27 @@ int lbm_read_forces(lbm_solve* s, double* out, int32_t max_records, int32_t* cou
  * (the extensive budget of readStats, main-3d.js:3610-3650). */
 int lbm_read_totals(lbm_solve* s, double* out4);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the totals of lbm_read_totals for a lattice the caller
+ * holds in device memory, written to the caller's device array out4
+ * ([mass, px, py, pz], FP64) and enqueued on `stream` (a hipStream_t; null:
+ * the null stream) without waiting.  The lattice is in the engine's device
+ * layout: shifted populations g = f - w_i in the storage format (LBM_STORE_*),
+ * cells in blocks of 256, each block holding its q populations as q runs of
+ * 256 elements (population i of cell c at element ((c / 256) q + i) 256 +
+ * c % 256).  For the lattice a solve holds, the four sums equal
+ * lbm_read_totals' bit for bit.  Allocates nothing: the per-block partial
+ * sums and the completion counter live in one fixed device array of the
+ * library, LBM_NOALLOC_SCRATCH_BYTES on each GPU it runs on, held until the
+ * process ends.  ncells from 1 to LBM_NOALLOC_MAX_CELLS (LBM_E_INVALID
+ * beyond).  Every call on a GPU shares that array: calls that overlap on one
+ * GPU (from two streams or two threads) race. */
+#define LBM_NOALLOC_SCRATCH_BYTES 33024
+#define LBM_NOALLOC_MAX_CELLS 2147483648LL
+int lbm_totals_noalloc(int32_t q, int32_t storage, int64_t ncells, const void* lattice, double* out4,
+                       void* stream);
+
 /* Solid classification, one byte per cell: bit 0 = inside the body
  * (phi < 0), bit 1 = phi < 2 (a cell that can have a solid neighbour). */
 int lbm_read_classes(lbm_solve* s, uint8_t* out);

@@ -151,6 +151,21 @@ def test_serve_refusals_without_gpu():
     assert b"budget" in lib.lbm_last_error()
 
 
+def test_totals_noalloc_refusals_without_gpu():
+    """The no-allocation totals refuse bad arguments before any device call:
+    sizes beyond LBM_NOALLOC_MAX_CELLS, empty lattices, unknown q or
+    storage, null pointers."""
+    lib = pylbm.load()
+    text = open(HEADER).read()
+    max_cells = int(re.search(r"#define LBM_NOALLOC_MAX_CELLS (\d+)LL", text).group(1))
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: every call below is refused first
+    for q, st, n, lat, out, what in ((19, 0, max_cells + 1, fake, fake, b"ncells"), (19, 0, 0, fake, fake, b"ncells"),
+                                     (13, 0, 64, fake, fake, b"q must be"), (19, 3, 64, fake, fake, b"storage"),
+                                     (19, 0, 64, None, fake, b"null"), (27, 1, 64, fake, None, b"null")):
+        assert lib.lbm_totals_noalloc(q, st, n, lat, out, None) == -1, (q, st, n)
+        assert what in lib.lbm_last_error(), lib.lbm_last_error()
+
+
 def test_state_header_layout():
     src = r"""
 #include <stdio.h>
