@@ -190,7 +190,36 @@ def measure_plate():
             "sphere_tau06_cd_towed": out["sphere_towed"], "sphere_tau06_cd_towed_reference_measured": 5.8481}
 
 
+def measure_drag_vs_fp64(threads, steps=3000, window=1000):
+    """Drag of every storage format against the FP64 oracle over a settled
+    window, independent of --steps: the headline's sphere scaled down to
+    96x48x48 (radius 4.5 at (24, 24, 24), sponge 6) at the headline's tau
+    0.518 (u0 0.05, Re 75), D3Q19 TRT, `steps` steps, Cd averaged over the
+    last `window` steps."""
+    import oracle_runner
+
+    desc, mac, meta = sc.sphere(nx=96, ny=48, nz=48, radius=4.5, centre=(24.0, 24.0, 24.0), re=75.0,
+                                sponge_width=6)
+    q = 0.5 * meta["u0"] ** 2 * meta["area"]
+    r = oracle_runner.run(desc, mac, steps=steps, threads=threads, want=("force",))
+    cd64 = float(r["force"][-window:, 0].mean() / q)
+    out = {"grid": [96, 48, 48], "tau": round(meta["tau"], 4), "steps": steps, "window": window,
+           "cd_fp64_oracle": cd64}
+    for st in ("f32", "f16", "bf16"):
+        with pylbm.Solve(dict(desc, storage=st, autotune=0)) as s:
+            s.upload_macro(mac)
+            s.step(steps)
+            f = s.read_forces(steps)
+        cd = float(f[-window:, 0].mean() / q)
+        out[st] = {"cd": cd, "drag_dev_vs_fp64_pct": 100.0 * (cd / cd64 - 1.0)}
+    return out
+
+
 def cpu_baseline(steps, threads):
+    """The FP64 JS oracle (node, worker threads over x-rows) on `steps` steps
+    of the benchmark grid.  `threads` is the CPU share one GPU's job gets on
+    the box (16 cores), not the host's whole CPU count, which is reported
+    beside it."""
     import oracle_runner
 
     desc, mac, _ = sc.sphere()
@@ -198,8 +227,10 @@ def cpu_baseline(steps, threads):
     s = r["summary"]
     return {
         "value": s["mlups"], "unit": "MLUPS", "cores": s["threads"], "kind": "port",
-        "sample": "FP64 JS oracle (node, %d worker threads, %s), %d steps of the 256x128x128 "
-                  "D3Q19 TRT sphere grid (%.1f s)" % (s["threads"], s["cpu_model"], steps, s["seconds"]),
+        "host_logical_cpus": s.get("cpu_count"), "cpu_model": s["cpu_model"],
+        "sample": "FP64 JS oracle (node, %d worker threads on %d of the host's %s logical CPUs, %s), %d steps "
+                  "of the 256x128x128 D3Q19 TRT sphere grid (%.1f s)" %
+                  (s["threads"], s["threads"], s.get("cpu_count"), s["cpu_model"], steps, s["seconds"]),
     }
 
 
@@ -374,11 +405,20 @@ def measure_serve(count, rate=50.0, seed=5, steps=2000, budget=200e9, scale=1.0,
                 s.read_forces(nsteps)
         return time.perf_counter() - t
 
+    # the chunk graphs' own device time (HIP events in the serve loop) and
+    # the algorithmic bytes they moved: the batched kernel's bandwidth with
+    # the launch gaps inside a chunk included and the host work between
+    # chunks left out
+    device = {"device_seconds": st["device_seconds"], "device_bytes": st["device_bytes"],
+              "device_gbs": st["device_bytes"] / st["device_seconds"] / 1e9 if st["device_seconds"] else None,
+              "device_busy_frac": st["device_seconds"] / wall,
+              "graph_instantiations": st["graph_instantiations"], "graph_updates": st["graph_updates"]}
     served = {"requests": count, "arrival_rate_per_s": rate, "last_arrival_s": mix[-1][0],
               "served_wall_s": wall, "served_solves_per_s": count / wall, "served_mlups": updates / wall / 1e6,
               "p50_latency_s": st["p50_latency"], "p99_latency_s": st["p99_latency"],
               "peak_resident": st["peak_resident"], "peak_pool_bytes": st["peak_resident_bytes"],
               "chunks": st["chunks"], "cell_updates": updates, "finite": bool(finite), "budget_bytes": budget}
+    served.update(device)
     if not baseline:
         return served
     seq = one_at_a_time("f16")
@@ -390,7 +430,7 @@ def measure_serve(count, rate=50.0, seed=5, steps=2000, budget=200e9, scale=1.0,
             "chunks": st["chunks"], "one_at_a_time_wall_s": seq, "one_at_a_time_solves_per_s": count / seq,
             "one_at_a_time_mlups": updates / seq / 1e6, "speedup_solves_per_s": seq / wall,
             "one_at_a_time_f32_wall_s": seq32, "speedup_vs_one_at_a_time_f32": seq32 / wall,
-            "cell_updates": updates, "finite": bool(finite), "budget_bytes": budget}
+            "cell_updates": updates, "finite": bool(finite), "budget_bytes": budget, **device}
 
 
 def main():
@@ -398,7 +438,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-16bit", action="store_true")
     ap.add_argument("--amr-steps", type=int, default=2000)
@@ -450,6 +490,11 @@ def main():
             extra[st + "_store"] = r
             print("%s: %s" % (st, json.dumps(r)), file=sys.stderr, flush=True)
 
+    drag64 = None
+    if not args.no_16bit and not args.no_cpu:
+        drag64 = measure_drag_vs_fp64(min(16, os.cpu_count() or 1))
+        print("drag vs FP64: %s" % json.dumps(drag64), file=sys.stderr, flush=True)
+
     towed = plate = None
     if not args.no_towed:
         towed = measure_towed(args.steps, args.warmup, 50)
@@ -489,6 +534,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        # the box gives one GPU's job 16 cores, whatever os.cpu_count() shows
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(args.cpu_steps, threads)
         print("cpu: %s" % json.dumps(cpu), file=sys.stderr, flush=True)
@@ -522,11 +568,14 @@ def main():
     }
     for k, v in extra.items():
         out[k] = {"mlups": round(v["mlups"], 1), "speedup_vs_f32": round(v["speedup_vs_f32"], 3),
+                  "kernel_speedup_vs_f32": round(main_run["kernel_ms"] / v["kernel_ms"], 3),
                   "kernel_ms": round(v["kernel_ms"], 5), "achieved_gbs": round(v["achieved_gbs"], 1),
                   "pct_hbm_peak": round(100.0 * v["achieved_gbs"] / HBM_PEAK_GBS, 2),
                   "cd_last1000": v["cd_last1000"],
                   "drag_dev_vs_f32_pct": (100.0 * (v["cd_last1000"] / main_run["cd_last1000"] - 1.0)
                                           if v["cd_last1000"] is not None and main_run["cd_last1000"] else None)}
+    if drag64:
+        out["drag_vs_fp64_sphere_d3q19_96x48x48"] = drag64
     if towed:
         out["towed_sphere_d3q19_256x128x128"] = {
             "mlups": round(towed["mlups"], 1), "ms_per_step": round(towed["ms_per_step"], 5),

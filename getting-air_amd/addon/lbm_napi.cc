@@ -581,7 +581,8 @@ lbm_amr* amr_of(Handle* h) { return static_cast<lbm_amr*>(h->p); }
 // createAmr({q, nx, ny, nz, rb, maxSlots, omega, omegaMinus, force,
 //            manageEvery, refineAt, keepAt, lead, explodeLinear, orphans,
 //            interface: 'explode' | 'interp', reflux, explodeDirs,
-//            sponge: {width, axes, u}, body: {a, centre}})
+//            criterion: 'vorticity' | 'q', sponge: {width, axes, u},
+//            body: {a, centre}})
 napi_value CreateAmr(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
@@ -628,6 +629,17 @@ napi_value CreateAmr(napi_env env, napi_callback_info info) {
   }
   d.reflux = (int)num(env, o, "reflux", 0);
   d.explode_dirs = (int)num(env, o, "explodeDirs", 0);
+  napi_value cv;
+  if (get_prop(env, o, "criterion", &cv)) {
+    char buf[16] = {0};
+    size_t len = 0;
+    napi_get_value_string_utf8(env, cv, buf, sizeof buf, &len);
+    if (!strcmp(buf, "q")) d.criterion = 1;
+    else if (strcmp(buf, "vorticity")) {
+      napi_throw_type_error(env, nullptr, "criterion must be 'vorticity' or 'q'");
+      return nullptr;
+    }
+  }
   if (get_prop(env, o, "sponge", &v)) {
     d.sponge_width = (float)num(env, v, "width", 0);
     double ax[3] = {1, 1, 1}, su[3] = {0, 0, 0};
@@ -1141,7 +1153,9 @@ napi_value ServeStats(napi_env env, napi_callback_info info) {
       {"resident", (double)st.resident}, {"queued", (double)st.queued}, {"cellUpdates", st.cell_updates},
       {"p50Latency", st.p50_latency}, {"p99Latency", st.p99_latency}, {"mlups", st.mlups},
       {"poolBytes", st.pool_bytes}, {"peakResidentBytes", st.peak_resident_bytes},
-      {"peakResident", (double)st.peak_resident}, {"chunks", (double)st.chunks}};
+      {"peakResident", (double)st.peak_resident}, {"chunks", (double)st.chunks},
+      {"deviceSeconds", st.device_seconds}, {"deviceBytes", st.device_bytes},
+      {"graphInstantiations", (double)st.graph_instantiations}, {"graphUpdates", (double)st.graph_updates}};
   for (const auto& e : f) {
     napi_create_double(env, e.v, &v);
     napi_set_named_property(env, o, e.k, v);

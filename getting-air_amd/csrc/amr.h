@@ -35,8 +35,16 @@ struct PoolGeom {
   const int32_t* block_slot;     // per block: slot or -1
   const int32_t* slot_to_block;  // per slot: block or -1
   const uint8_t* ccls;           // per L0 cell: kCovered | kNear | kSeam
+  const int32_t* tile_table;     // kTileWords per active tile (k_tile_table)
   PoolBody body;
 };
+
+// Per active tile (index into the active list), rebuilt with the cell
+// classes: words 0-26 the slots of the 27 blocks around it, word (ox + 1) +
+// 3 (oy + 1) + 9 (oz + 1) for the block at offset (ox, oy, oz) (-1 when
+// unrefined; word 13 is the tile's own slot), word 27 a mask of the refined
+// ones, words 28-30 the tile's block coordinates.
+constexpr int kTileWords = 32;
 
 // L0 cell classes, rebuilt by launch_cell_class after every topology change.
 constexpr uint8_t kCovered = 1;  // the cell's block holds a tile
@@ -52,6 +60,8 @@ struct ManageArgs {
   const float* crit;         // per block criterion value
   float refine_at;           // an unrefined block refines when crit >= refine_at
   float keep_at;             // a refined block stays while crit >= keep_at
+  const uint8_t* geom;       // per block, or null: bit 0 the body's geometry wants it (always
+                             // refined), bit 1 the field criterion may not refine it
   int32_t* block_slot;
   int32_t* slot_to_block;
   uint8_t* want;             // out: per block decision
@@ -68,7 +78,8 @@ struct ManageArgs {
 struct SurfacePlan {
   uint32_t* seam;   // covered L0 cells with an unrefined lattice neighbour
   uint32_t* near;   // unrefined L0 cells with a covered lattice neighbour
-  uint32_t* ring;   // pool cells (slot * FB^3 + local) of ring cells over unrefined L0 cells
+  uint4* ring;      // ring cells over unrefined L0 cells: pool cell (slot * FB^3 + local),
+                    // L0 cell, explode decisions, L0 coordinates (k_ring_plan)
   uint32_t* flux;   // unrefined L0 cells with a covered cell among their 26 neighbours
                     // (the seam flux correction's cells); null without it
   uint32_t* count;  // [seam, near, ring, flux]
@@ -83,8 +94,7 @@ hipError_t launch_cell_class(int q, uint8_t* ccls, const PoolGeom& g, const Surf
 // mac_pool null: the substep's macroscopic field is not written.
 // collide_ring: ring cells collide as well (the interp interface).
 hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
-                            const CollideParams& fine, const int32_t* active, int nactive, int collide_ring,
-                            hipStream_t s);
+                            const CollideParams& fine, int nactive, int collide_ring, hipStream_t s);
 // Momentum exchange on the fine-level body from the pool state at the start
 // of the macro-step: one FP64 [fx, fy, fz, tx, ty, tz] partial per thread
 // block (pool_force_blocks of them), weighted dx^2 = 1/4.
@@ -111,7 +121,8 @@ hipError_t launch_interp(int q, const float* gcoarse, float* gpool, const PoolGe
                          int npairs, float rescale_c2f, int ring_only, hipStream_t s);
 hipError_t launch_average(int q, const float* gpool, float* gcoarse, float* mac, const PoolGeom& g,
                           const int32_t* pairs, int npairs, float rescale_f2c, hipStream_t s);
-hipError_t launch_criterion(const float* mac, float* crit, const PoolGeom& g, float lead, hipStream_t s);
+// kind 0: |curl u|; 1: Q = 1/2 (|Omega|^2 - |S|^2)
+hipError_t launch_criterion(const float* mac, float* crit, const PoolGeom& g, float lead, int kind, hipStream_t s);
 hipError_t launch_manage(const ManageArgs& a, hipStream_t s);
 // active = [count, slots with a block ...] ascending
 hipError_t launch_active_list(const int32_t* slot_to_block, int max_slots, int32_t* active, hipStream_t s);

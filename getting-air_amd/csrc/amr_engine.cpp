@@ -44,6 +44,7 @@ struct lbm_amr {
   int32_t* status = nullptr;
   int32_t* pairs = nullptr;
   uint8_t* ccls = nullptr;
+  int32_t* tile_table = nullptr;  // kTileWords per active tile
   SurfacePlan plan{};
   uint32_t nseam = 0, nnear = 0, nring = 0, nflux = 0;
   float* flux_acc = nullptr;        // [dm, dpx, dpy, dpz] per flux cell (reflux only)
@@ -60,6 +61,7 @@ struct lbm_amr {
   long long step = 0;
   bool seeded = false;
   bool body_covered = false;  // the uploaded topology holds the body (check_topology)
+  uint8_t* geom = nullptr;    // per block (manager with a body): geometry_mask
 };
 
 namespace {
@@ -93,6 +95,10 @@ int validate(const lbm_amr_desc* d) {
   if (d->nx / d->rb < 3 || d->ny / d->rb < 3 || d->nz / d->rb < 3)
     return fail(LBM_E_INVALID, "grid %dx%dx%d has fewer than 3 blocks of %d on an axis", d->nx, d->ny, d->nz, d->rb);
   if (d->max_slots < 1) return fail(LBM_E_INVALID, "max_slots %d", d->max_slots);
+  // explode's ring plan packs L0 coordinates in 10 bits each
+  if (d->nx > 1024 || d->ny > 1024 || d->nz > 1024)
+    return fail(LBM_E_INVALID, "grid %dx%dx%d: the AMR L0 grid is at most 1024 per axis", d->nx, d->ny, d->nz);
+  if (d->criterion != 0 && d->criterion != 1) return fail(LBM_E_INVALID, "criterion %d: 0 (|curl u|) or 1 (Q)", d->criterion);
   if (d->explode_dirs < 0 || d->explode_dirs > d->q)
     return fail(LBM_E_INVALID, "explode_dirs %d outside 0..q", d->explode_dirs);
   const double tile = std::pow(2.0 * d->rb + 2 * kGhost, 3);
@@ -113,12 +119,61 @@ int validate(const lbm_amr_desc* d) {
     if (d->body.shape != LBM_BODY_SPHERE) return fail(LBM_E_INVALID, "AMR body shape %d: only a sphere", d->body.shape);
     if (!(d->body.radius > 0.f)) return fail(LBM_E_INVALID, "AMR body radius %g", d->body.radius);
     if (d->interface_interp) return fail(LBM_E_INVALID, "an AMR body needs the explode interface");
-    // the body must stay inside the refined region, which only a static
-    // topology guarantees here
-    if (d->manage_every != 0) return fail(LBM_E_INVALID, "an AMR body needs a static topology (manage_every = 0)");
   }
   return LBM_OK;
 }
+
+// The manager's geometry term for the pinned sphere (blockWanted,
+// common_d3_manage.wgsl:257-302; a pinned body has no look-ahead): a block
+// whose box of L0 cells comes within kBodyMargin of the surface is always
+// refined.  The reference's default margin (main-3d.js:485), which covers the
+// fine step's bounce-back and force reach of one L0 cell.
+constexpr double kBodyMargin = 2.0;
+// sqrt(3) rounded up, the literal of d3-amr.mjs:120 and
+// common_d3_manage.wgsl:178 (a box's circumradius bound must not round down)
+constexpr double kSqrt3Up = 1.7320509;
+
+// Whether any point of the cube centred at c with half-edge h comes within
+// `margin` of the sphere: the SDF is 1-Lipschitz, so phi(c) - h sqrt(3)
+// bounds the cube's minimum from below; an ambiguous cube splits into octants
+// down to `depth`, where it is accepted (nearBodyWant, d3-amr.mjs:139-159).
+// Evaluated in FP64 without contraction, as the host twin evaluates it.
+#pragma clang fp contract(off)
+bool near_sphere(const double c[3], double h, int depth, const double ctr[3], double r, double margin) {
+  const double dx = c[0] - ctr[0], dy = c[1] - ctr[1], dz = c[2] - ctr[2];
+  const double phi = std::sqrt(dx * dx + dy * dy + dz * dz) - r;
+  if (phi - h * kSqrt3Up > margin) return false;
+  if (phi <= margin || depth == 0) return true;
+  const double q = h / 2;
+  for (int k = 0; k < 8; k++) {
+    const double cc[3] = {c[0] + ((k & 1) ? q : -q), c[1] + ((k & 2) ? q : -q), c[2] + ((k & 4) ? q : -q)};
+    if (near_sphere(cc, q, depth - 1, ctr, r, margin)) return true;
+  }
+  return false;
+}
+
+// Per block: bit 0 the body's geometry wants it refined, bit 1 it reaches
+// into the sponge band (the fine step does not relax toward the far field),
+// so the field criterion may not refine it.
+std::vector<uint8_t> geometry_mask(const lbm_amr_desc& d) {
+  const int rb = d.rb, nbx = d.nx / rb, nby = d.ny / rb, nbz = d.nz / rb;
+  const int n[3] = {d.nx, d.ny, d.nz};
+  const double ctr[3] = {d.body.centre[0], d.body.centre[1], d.body.centre[2]};
+  std::vector<uint8_t> m((size_t)nbx * nby * nbz, 0);
+  for (int b = 0; b < nbx * nby * nbz; b++) {
+    const int bc[3] = {b % nbx, (b / nbx) % nby, b / (nbx * nby)};
+    const double c[3] = {bc[0] * rb + rb / 2.0, bc[1] * rb + rb / 2.0, bc[2] * rb + rb / 2.0};
+    if (near_sphere(c, rb / 2.0, 3, ctr, d.body.radius, kBodyMargin)) m[b] |= 1;
+    if (d.sponge_width > 0.f) {
+      for (int k = 0; k < 3; k++) {
+        const int lo = bc[k] * rb, hi = lo + rb - 1;
+        if (d.sponge_axes[k] && (lo < d.sponge_width || (n[k] - 1) - hi < d.sponge_width)) m[b] |= 2;
+      }
+    }
+  }
+  return m;
+}
+#pragma clang fp contract(on)
 
 // Refuses a topology that leaves an L0 cell within kBodyReach (1 cell) of the
 // body unrefined (the body lives on the fine level alone, main-3d.js:3127-3136)
@@ -156,7 +211,7 @@ void release(lbm_amr* a) {
   void* ptrs[] = {a->pool[0], a->pool[1], a->mac_pool, a->block_slot, a->slot_to_block, a->want, a->crit,
                   a->dying, a->fresh, a->active, a->free_list, a->status, a->pairs, a->ccls, a->partials, a->counter, a->totals,
                   a->plan.seam, a->plan.near, a->plan.ring, a->plan.flux, a->plan.count, a->flux_acc,
-                  a->force_partials, a->history, a->written};
+                  a->force_partials, a->history, a->written, a->geom, a->tile_table};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (a->l0) lbm_solve_destroy(a->l0);
@@ -223,7 +278,7 @@ int restrict_active(lbm_amr* a) {
 
 int manage(lbm_amr* a) {
   const lbm_amr_desc& d = a->d;
-  HIPCHK(launch_criterion(solve_mac(a->l0), a->crit, a->G, (float)d.lead, a->stream));
+  HIPCHK(launch_criterion(solve_mac(a->l0), a->crit, a->G, (float)d.lead, d.criterion, a->stream));
   int rc = restrict_active(a);
   if (rc) return rc;
   ManageArgs m{};
@@ -232,6 +287,7 @@ int manage(lbm_amr* a) {
   m.crit = a->crit;
   m.refine_at = d.refine_at;
   m.keep_at = d.keep_at;
+  m.geom = a->geom;
   m.block_slot = a->block_slot;
   m.slot_to_block = a->slot_to_block;
   m.want = a->want;
@@ -257,6 +313,9 @@ int manage(lbm_amr* a) {
   if (counts[3] > 0)
     return fail(LBM_E_NOMEM, "AMR pool exhausted: %d wanted tiles found no free slot (max_slots %d)", counts[3],
                 d.max_slots);
+  // every block the geometry wants is placed, and they cover each L0 cell
+  // within kBodyReach of the body (the margin exceeds that reach)
+  if (a->geom) a->body_covered = true;
   return LBM_OK;
 }
 
@@ -276,10 +335,10 @@ int macro_step_interp(lbm_amr* a) {
     HIPCHK(launch_active_pairs(a->active, a->slot_to_block, d.max_slots, a->pairs, a->stream));
     HIPCHK(launch_interp(d.q, ct, a->pool[0], a->G, a->pairs, a->nactive, a->c2f, 1, a->stream));
     if (d.reflux) HIPCHK(launch_flux(d.q, a->pool[0], a->flux_acc, a->G, a->plan, a->nflux, 0, a->stream));
-    HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->active, a->nactive, 1,
+    HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->nactive, 1,
                             a->stream));
     if (d.reflux) HIPCHK(launch_flux(d.q, a->pool[1], a->flux_acc, a->G, a->plan, a->nflux, 1, a->stream));
-    HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->active, a->nactive, 1,
+    HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->nactive, 1,
                             a->stream));
     HIPCHK(launch_average(d.q, a->pool[0], ct1, solve_mac(a->l0), a->G, a->pairs, a->nactive, a->f2c, a->stream));
     if (d.reflux)
@@ -311,9 +370,8 @@ int macro_step(lbm_amr* a) {
   }
   HIPCHK(launch_explode(d.q, l0, a->pool[0], a->G, a->plan, a->nring, d.explode_linear, d.explode_dirs, a->stream));
   // only the second substep's field is read (coalesce, lbm_amr_read_pool_macro)
-  HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->active, a->nactive, 0, a->stream));
-  HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->active, a->nactive, 0,
-                          a->stream));
+  HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->nactive, 0, a->stream));
+  HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->nactive, 0, a->stream));
   HIPCHK(launch_coalesce(d.q, a->pool[0], l0, solve_mac(a->l0), a->mac_pool, a->G, a->active, a->nactive, a->plan,
                          a->nseam, a->nnear, d.orphans, a->stream));
   int rc = solve_enqueue_covered_step(a->l0, a->block_slot, d.rb, a->G.nbx, a->G.nby);
@@ -382,7 +440,8 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
       (rc = dalloc(&a->ccls, (size_t)G.ncells)) || (rc = dalloc(&a->plan.count, 16)) ||
       (rc = dalloc(&a->plan.seam, (size_t)a->plan.cap_seam * 4)) ||
       (rc = dalloc(&a->plan.near, (size_t)a->plan.cap_near * 4)) ||
-      (rc = dalloc(&a->plan.ring, (size_t)a->plan.cap_ring * 4)) ||
+      (rc = dalloc(&a->plan.ring, (size_t)a->plan.cap_ring * 16)) ||
+      (rc = dalloc(&a->tile_table, (size_t)desc->max_slots * kTileWords * 4)) ||
       (rc = dalloc(&a->partials, 256 * 4 * 8)) || (rc = dalloc(&a->counter, 16)) || (rc = dalloc(&a->totals, 32))) {
     release(a);
     return rc;
@@ -393,6 +452,15 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
         (rc = dalloc(&a->history, (size_t)kForceCap * 48)) || (rc = dalloc(&a->written, 8))) {
       release(a);
       return rc;
+    }
+    if (desc->manage_every > 0) {
+      const std::vector<uint8_t> m = geometry_mask(*desc);
+      hipError_t e = hipSuccess;
+      if ((rc = dalloc(&a->geom, m.size())) ||
+          (e = hipMemcpy(a->geom, m.data(), m.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+        release(a);
+        return rc ? rc : hip_fail(e, "geometry mask");
+      }
     }
   }
   if (desc->reflux) {
@@ -405,6 +473,7 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
   G.block_slot = a->block_slot;
   G.slot_to_block = a->slot_to_block;
   G.ccls = a->ccls;
+  G.tile_table = a->tile_table;
   hipError_t e = hipMemsetAsync(a->block_slot, 0xff, (size_t)a->nblocks * 4, a->stream);
   if (e == hipSuccess) e = hipMemsetAsync(a->slot_to_block, 0xff, (size_t)desc->max_slots * 4, a->stream);
   if (e == hipSuccess) e = hipMemsetAsync(a->want, 0, (size_t)a->nblocks, a->stream);
@@ -513,7 +582,8 @@ int lbm_amr_upload_macro(lbm_amr* a, const float* mac4) {
 int lbm_amr_step(lbm_amr* a, int32_t n) {
   if (!a) return fail(LBM_E_INVALID, "null argument");
   if (!a->seeded) return fail(LBM_E_STATE, "step before upload_macro");
-  if (a->G.body.on && !a->body_covered) return fail(LBM_E_STATE, "step with a body before a topology that holds it");
+  if (a->G.body.on && !a->body_covered && a->d.manage_every <= 0)
+    return fail(LBM_E_STATE, "step with a body before a topology that holds it");
   for (int s = 0; s < n; s++) {
     int rc = macro_step(a);
     if (rc) return rc;
@@ -525,7 +595,6 @@ int lbm_amr_step(lbm_amr* a, int32_t n) {
 int lbm_amr_manage_now(lbm_amr* a) {
   if (!a) return fail(LBM_E_INVALID, "null argument");
   if (!a->seeded) return fail(LBM_E_STATE, "manage before upload_macro");
-  if (a->G.body.on) return fail(LBM_E_INVALID, "an AMR body needs a static topology: the manager does not keep it");
   int rc = manage(a);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(a->stream));

@@ -167,10 +167,18 @@ __global__ void __launch_bounds__(256) k_cell_class(uint8_t* __restrict__ ccls, 
   }
 }
 
-// Ring cells that explode writes: those whose L0 cell is unrefined.
-template <int RB>
+// Ring cells that explode writes (those whose L0 cell is unrefined), with
+// what explode needs to know about each, all fixed by the topology:
+//   .x  pool cell (slot * FB^3 + tile-local index)
+//   .y  the L0 cell v under it
+//   .z  bit i (i < Q): the L0 cell v + e_i is refined; bits 27-29: the two
+//       L0 neighbours of v along x / y / z are both unrefined (the linear
+//       reconstruction's gradient along that axis is taken)
+//   .w  v's coordinates, x | y << 10 | z << 20
+template <int Q, int RB>
 __global__ void __launch_bounds__(256) k_ring_plan(PoolGeom G, SurfacePlan plan, const int32_t* __restrict__ active,
                                                    int nactive) {
+  using L = Lattice<Q>;
   using T = Tile<RB>;
   constexpr int FB = T::FB, RB2 = T::RB2, LO = T::LO, HI = T::HI;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -185,9 +193,19 @@ __global__ void __launch_bounds__(256) k_ring_plan(PoolGeom G, SurfacePlan plan,
   const int gx = wrap1(bx * RB2 + fx - kGhost, 2 * G.nx);
   const int gy = wrap1(by * RB2 + fy - kGhost, 2 * G.ny);
   const int gz = wrap1(bz * RB2 + fz - kGhost, 2 * G.nz);
-  if (G.ccls[coarse_cell(G, gx >> 1, gy >> 1, gz >> 1)] & kCovered) return;
+  const int vx = gx >> 1, vy = gy >> 1, vz = gz >> 1;
+  if (G.ccls[coarse_cell(G, vx, vy, vz)] & kCovered) return;
+  uint32_t bits = 0;
+#pragma unroll
+  for (int i = 0; i < Q; i++)
+    if (covered_at(G, vx + L::ex[i], vy + L::ey[i], vz + L::ez[i])) bits |= 1u << i;
+  if (!covered_at(G, vx - 1, vy, vz) && !covered_at(G, vx + 1, vy, vz)) bits |= 1u << 27;
+  if (!covered_at(G, vx, vy - 1, vz) && !covered_at(G, vx, vy + 1, vz)) bits |= 1u << 28;
+  if (!covered_at(G, vx, vy, vz - 1) && !covered_at(G, vx, vy, vz + 1)) bits |= 1u << 29;
   const uint32_t e = atomicAdd(&plan.count[2], 1u);
-  if (e < plan.cap_ring) plan.ring[e] = (uint32_t)slot * T::CELLS + r;
+  if (e < plan.cap_ring)
+    plan.ring[e] = make_uint4((uint32_t)slot * T::CELLS + r, coarse_cell(G, vx, vy, vz), bits,
+                              (uint32_t)vx | (uint32_t)vy << 10 | (uint32_t)vz << 20);
 }
 
 void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint32_t* near, uint32_t* ring) {
@@ -198,9 +216,38 @@ void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint3
   *ring = (uint32_t)((uint64_t)max_slots * (fb * fb * fb - inner));
 }
 
+// The per-tile neighbour table (kTileWords words per active tile).  The pool
+// step reads it with wave-uniform loads in place of per-lane block-map
+// lookups (slot -> block -> neighbour block -> slot), which sat on every
+// lane's path to its population loads.
+__global__ void __launch_bounds__(256) k_tile_table(PoolGeom G, const int32_t* __restrict__ active, int nactive,
+                                                    int32_t* __restrict__ table) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nactive) return;
+  const int slot = active[1 + k];
+  const int id = G.slot_to_block[slot];
+  const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
+  int32_t* t = table + (size_t)k * kTileWords;
+  uint32_t mask = 0;
+  for (int n = 0; n < 27; n++) {
+    const int ox = n % 3 - 1, oy = (n / 3) % 3 - 1, oz = n / 9 - 1;
+    const int s = G.block_slot[block_id(G, wrap1(bx + ox, G.nbx), wrap1(by + oy, G.nby), wrap1(bz + oz, G.nbz))];
+    t[n] = s;
+    if (s >= 0) mask |= 1u << n;
+  }
+  t[27] = (int32_t)mask;
+  t[28] = bx;
+  t[29] = by;
+  t[30] = bz;
+  t[31] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // pool step
 // ---------------------------------------------------------------------------
+// One pool cell's substep: pull (the DIRECT_GHOST gather), moments, and,
+// when `collide_cell`, the fine collision; `mac_pool` (or null) receives
+// the moments.
 // BODY: the fine-level pinned sphere (its own instantiation, so that the
 // body-free kernel keeps its register budget).  A link whose geometric source
 // is solid reflects this cell's own opposite population (halfway bounce-back,
@@ -208,34 +255,31 @@ void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint3
 // zero), and a solid cell is held at feq(1, 0) with [1, 0, 0, 0] as its
 // moments (:387-403).
 template <int Q, int RB, bool BODY>
-__global__ void __launch_bounds__(256)
-k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __restrict__ mac_pool, PoolGeom G,
-            CollideParams P, const int32_t* __restrict__ active, int nactive, int collide_ring) {
+__device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, float* __restrict__ gout,
+                                               float* __restrict__ mac_pool, const PoolGeom& G, const CollideParams& P,
+                                               const int32_t* __restrict__ tt, int fx, int fy, int fz,
+                                               bool collide_cell) {
+  const int slot = tt[13], bx = tt[28], by = tt[29], bz = tt[30];
   using L = Lattice<Q>;
   using T = Tile<RB>;
   constexpr int FB = T::FB, RB2 = T::RB2, LO = T::LO, HI = T::HI;
-  const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
-  const uint32_t k = wave_tile<RB>(t);
-  if (k >= (uint32_t)nactive) return;
-  const uint32_t r = t - k * (uint32_t)T::CELLS;
-  const int slot = active[1 + k];
-  const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
-  const int id = G.slot_to_block[slot];
-  const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
+  const uint32_t cell = pool_cell<RB>(slot, fx, fy, fz);
+  float g[Q];
   const int offx = fx <= LO ? -1 : (fx + 1 >= HI ? 1 : 0);
   const int offy = fy <= LO ? -1 : (fy + 1 >= HI ? 1 : 0);
   const int offz = fz <= LO ? -1 : (fz + 1 >= HI ? 1 : 0);
-  const int nbxw = wrap1(bx + offx, G.nbx), nbyw = wrap1(by + offy, G.nby), nbzw = wrap1(bz + offz, G.nbz);
-  int nX = -1, nY = -1, nZ = -1, nXY = -1, nXZ = -1, nYZ = -1, nXYZ = -1;
-  if (offx) nX = G.block_slot[block_id(G, nbxw, by, bz)];
-  if (offy) nY = G.block_slot[block_id(G, bx, nbyw, bz)];
-  if (offz) nZ = G.block_slot[block_id(G, bx, by, nbzw)];
-  if (offx && offy) nXY = G.block_slot[block_id(G, nbxw, nbyw, bz)];
-  if (offx && offz) nXZ = G.block_slot[block_id(G, nbxw, by, nbzw)];
-  if (offy && offz) nYZ = G.block_slot[block_id(G, bx, nbyw, nbzw)];
-  if (offx && offy && offz) nXYZ = G.block_slot[block_id(G, nbxw, nbyw, nbzw)];
-
-  float g[Q];
+  // the slots of the neighbour blocks this cell's pulls can reach, selected
+  // from the tile's table (word (ox + 1) + 3 (oy + 1) + 9 (oz + 1))
+  const int nX = offx < 0 ? tt[12] : (offx > 0 ? tt[14] : -1);
+  const int nY = offy < 0 ? tt[10] : (offy > 0 ? tt[16] : -1);
+  const int nZ = offz < 0 ? tt[4] : (offz > 0 ? tt[22] : -1);
+  const int nXY = !(offx && offy) ? -1 : (offy < 0 ? (offx < 0 ? tt[9] : tt[11]) : (offx < 0 ? tt[15] : tt[17]));
+  const int nXZ = !(offx && offz) ? -1 : (offz < 0 ? (offx < 0 ? tt[3] : tt[5]) : (offx < 0 ? tt[21] : tt[23]));
+  const int nYZ = !(offy && offz) ? -1 : (offz < 0 ? (offy < 0 ? tt[1] : tt[7]) : (offy < 0 ? tt[19] : tt[25]));
+  const int nXYZ = !(offx && offy && offz)
+                       ? -1
+                       : (offz < 0 ? (offy < 0 ? (offx < 0 ? tt[0] : tt[2]) : (offx < 0 ? tt[6] : tt[8]))
+                                   : (offy < 0 ? (offx < 0 ? tt[18] : tt[20]) : (offx < 0 ? tt[24] : tt[26])));
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     int sx = fx - L::ex[i], sy = fy - L::ey[i], sz = fz - L::ez[i];
@@ -263,7 +307,6 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
     sz = min(max(sz, 0), FB - 1);
     g[i] = *at(gin + (size_t)i * G.pool_plane, pool_cell<RB>(src_slot, sx, sy, sz));
   }
-  const uint32_t cell = pool_cell<RB>(slot, fx, fy, fz);
   float phi = 1e30f;
   if (BODY) {
     const float px = fine_to_l0(bx * RB, fx), py = fine_to_l0(by * RB, fy), pz = fine_to_l0(bz * RB, fz);
@@ -278,19 +321,17 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
     }
   }
 
-  float drho, rho, ux, uy, uz;
-  moments<Q>(g, P.F, drho, rho, ux, uy, uz);
-  if (mac_pool) {
-    *at(mac_pool, cell) = rho;
-    *at(mac_pool + G.pool_plane, cell) = ux;
-    *at(mac_pool + 2 * (size_t)G.pool_plane, cell) = uy;
-    *at(mac_pool + 3 * (size_t)G.pool_plane, cell) = uz;
+  if (mac_pool || collide_cell) {
+    float drho, rho, ux, uy, uz;
+    moments<Q>(g, P.F, drho, rho, ux, uy, uz);
+    if (mac_pool) {
+      *at(mac_pool, cell) = rho;
+      *at(mac_pool + G.pool_plane, cell) = ux;
+      *at(mac_pool + 2 * (size_t)G.pool_plane, cell) = uy;
+      *at(mac_pool + 3 * (size_t)G.pool_plane, cell) = uz;
+    }
+    if (collide_cell) collide<Q>(g, P, drho, rho, ux, uy, uz);
   }
-  const bool interior = fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI;
-  // Ring cells advect and store without colliding (common_d3_amr_step1.wgsl
-  // COLLIDE_RING = 0): coalesce averages uncollided interface states.  The
-  // interp interface collides them too (COLLIDE_RING = 1, main-3d.js:1395).
-  if (interior || collide_ring) collide<Q>(g, P, drho, rho, ux, uy, uz);
   if (BODY && phi < 0.f) {
 #pragma unroll
     for (int i = 0; i < Q; i++) g[i] = 0.f;  // feq(1, 0) - w
@@ -303,6 +344,31 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
   }
 #pragma unroll
   for (int i = 0; i < Q; i++) *at(gout + (size_t)i * G.pool_plane, cell) = g[i];
+}
+
+// One thread per pool cell of every active tile, in memory order (a wave is
+// 64 consecutive cells of one tile).  Every cell is stepped, ring included:
+// leaving out the ring cells nothing reads afterwards (those over refined L0
+// cells, and the outer layer in the first substep) was measured slower on
+// the vortex-ring workload (1.17 against 1.04 ms per macro-step), the masked
+// lanes costing the waves more than the traffic they save.
+template <int Q, int RB, bool BODY>
+__global__ void __launch_bounds__(256)
+k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __restrict__ mac_pool, PoolGeom G,
+            CollideParams P, int nactive, int collide_ring) {
+  using T = Tile<RB>;
+  constexpr int FB = T::FB, LO = T::LO, HI = T::HI;
+  const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
+  const uint32_t k = wave_tile<RB>(t);
+  if (k >= (uint32_t)nactive) return;
+  const uint32_t r = t - k * (uint32_t)T::CELLS;
+  const int32_t* tt = G.tile_table + (size_t)k * kTileWords;
+  const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
+  const bool interior = fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI;
+  // Ring cells advect and store without colliding (common_d3_amr_step1.wgsl
+  // COLLIDE_RING = 0): coalesce averages uncollided interface states.  The
+  // interp interface collides them too (COLLIDE_RING = 1, main-3d.js:1395).
+  pool_cell_step<Q, RB, BODY>(gin, gout, mac_pool, G, P, tt, fx, fy, fz, interior || collide_ring);
 }
 
 // Momentum exchange on the fine-level body (common_d3_force_pool.wgsl, the
@@ -369,45 +435,36 @@ k_pool_force(const float* __restrict__ gp, PoolGeom G, const int32_t* __restrict
 // ---------------------------------------------------------------------------
 template <int Q, int RB>
 __global__ void __launch_bounds__(256)
-k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const uint32_t* __restrict__ ring,
+k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const uint4* __restrict__ ring,
           uint32_t nring, int linear, int per) {
   using L = Lattice<Q>;
   using T = Tile<RB>;
-  constexpr int FB = T::FB, RB2 = T::RB2;
+  constexpr int FB = T::FB, LO = T::LO, HI = T::HI;
   const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nring) return;
   // Grid row g takes directions [g per, (g + 1) per): a ring cell's Q
   // populations are independent, so a small ring spreads them over threads
-  // (one thread per ring cell leaves too few waves to hide the chain of
-  // dependent lookups) and a large one keeps them in one thread (which
-  // shares the index work among them).
+  // (one thread per ring cell leaves too few waves to hide the loads) and a
+  // large one keeps them in one thread (which shares the index work among
+  // them).
   const int i0 = (int)blockIdx.y * per, i1 = min(Q, i0 + per);
-  // a ring cell over an unrefined L0 cell (k_ring_plan)
-  const uint32_t cell = ring[t];
-  const int slot = (int)(cell / (uint32_t)T::CELLS);
-  const uint32_t r = cell - (uint32_t)slot * T::CELLS;
+  // a ring cell over an unrefined L0 cell, and its explode decisions
+  // (k_ring_plan)
+  const uint4 d = ring[t];
+  const uint32_t cell = d.x, vc = d.y, bits = d.z;
+  const uint32_t r = cell % (uint32_t)T::CELLS;
   const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
-  const int id = G.slot_to_block[slot];
-  const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
-  const int NFx = 2 * G.nx, NFy = 2 * G.ny, NFz = 2 * G.nz;
-  const int gx = wrap1(bx * RB2 + fx - kGhost, NFx);
-  const int gy = wrap1(by * RB2 + fy - kGhost, NFy);
-  const int gz = wrap1(bz * RB2 + fz - kGhost, NFz);
-  const int vx = gx >> 1, vy = gy >> 1, vz = gz >> 1;
-  const uint32_t vc = coarse_cell(G, vx, vy, vz);
-  const int xm = wrap1(vx - 1, G.nx), xp = wrap1(vx + 1, G.nx);
-  const int ym = wrap1(vy - 1, G.ny), yp = wrap1(vy + 1, G.ny);
-  const int zm = wrap1(vz - 1, G.nz), zp = wrap1(vz + 1, G.nz);
-  const uint32_t cmx = coarse_cell(G, xm, vy, vz), cpx = coarse_cell(G, xp, vy, vz);
-  const uint32_t cmy = coarse_cell(G, vx, ym, vz), cpy = coarse_cell(G, vx, yp, vz);
-  const uint32_t cmz = coarse_cell(G, vx, vy, zm), cpz = coarse_cell(G, vx, vy, zp);
-  const bool okx = !((G.ccls[cmx] | G.ccls[cpx]) & kCovered);
-  const bool oky = !((G.ccls[cmy] | G.ccls[cpy]) & kCovered);
-  const bool okz = !((G.ccls[cmz] | G.ccls[cpz]) & kCovered);
-  // position of the fine cell inside its coarse cell, in coarse units
-  const float dx = ((float)(gx - 2 * vx) - 0.5f) * 0.5f;
-  const float dy = ((float)(gy - 2 * vy) - 0.5f) * 0.5f;
-  const float dz = ((float)(gz - 2 * vz) - 0.5f) * 0.5f;
+  const int vx = d.w & 1023, vy = (d.w >> 10) & 1023, vz = d.w >> 20;
+  const uint32_t cmx = coarse_cell(G, wrap1(vx - 1, G.nx), vy, vz), cpx = coarse_cell(G, wrap1(vx + 1, G.nx), vy, vz);
+  const uint32_t cmy = coarse_cell(G, vx, wrap1(vy - 1, G.ny), vz), cpy = coarse_cell(G, vx, wrap1(vy + 1, G.ny), vz);
+  const uint32_t cmz = coarse_cell(G, vx, vy, wrap1(vz - 1, G.nz)), cpz = coarse_cell(G, vx, vy, wrap1(vz + 1, G.nz));
+  const bool okx = (bits >> 27) & 1, oky = (bits >> 28) & 1, okz = (bits >> 29) & 1;
+  // position of the fine cell inside its coarse cell, in coarse units: the
+  // global fine coordinate is 2 * block + local - 2, so its parity is the
+  // local one's
+  const float dx = ((float)(fx & 1) - 0.5f) * 0.5f;
+  const float dy = ((float)(fy & 1) - 0.5f) * 0.5f;
+  const float dz = ((float)(fz & 1) - 0.5f) * 0.5f;
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     if (i < i0 || i >= i1) continue;  // wave-uniform
@@ -415,13 +472,12 @@ k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, cons
     const float wi = L::w[i];
     auto l0 = [&](uint32_t c) { return gc[lat_elem(Q, i, c)]; };  // population i of L0 cell c
     float out;
-    if (!covered_at(G, vx + ex, vy + ey, vz + ez)) {
+    if (!((bits >> i) & 1)) {
       // The coarse target is unrefined: only a population that streams from
       // this ring cell into this tile's interior is kept (in-orphan, uniform
       // injection); everything else is zeroed (f = 0, i.e. g = -w).
-      const int dxg = wrap1(gx + ex, NFx), dyg = wrap1(gy + ey, NFy), dzg = wrap1(gz + ez, NFz);
-      const bool into = covered_at(G, dxg >> 1, dyg >> 1, dzg >> 1) && (dxg >> (T::LOG_RB + 1)) == bx &&
-                        (dyg >> (T::LOG_RB + 1)) == by && (dzg >> (T::LOG_RB + 1)) == bz;
+      const int tx = fx + ex, ty = fy + ey, tz = fz + ez;
+      const bool into = tx >= LO && tx < HI && ty >= LO && ty < HI && tz >= LO && tz < HI;
       out = into ? l0(vc) : -wi;
     } else if (!linear || i == 0) {
       out = l0(vc);
@@ -790,8 +846,37 @@ __device__ __forceinline__ float vort_mag(const float4* mac, const PoolGeom& G, 
   return sqrt_rn(add_rn(add_rn(mul_rn(wx, wx), mul_rn(wy, wy)), mul_rn(wz, wz)));
 }
 
+// Q = 1/2 (|Omega|^2 - |S|^2) at an L0 cell from the same centred gradient,
+// J[j][i] = du_i/dx_j, summed over (i, j) in the order of qAt
+// (common_d3_criterion.wgsl:102-120; host twin qOfGrad, d3-criterion.mjs).
+// Pure shear gives exactly 0, solid rotation at rate W gives W^2.
+__device__ __forceinline__ float q_crit(const float4* mac, const PoolGeom& G, int x, int y, int z) {
+  const float3 xp = mac_u(mac, G, x + 1, y, z), xm = mac_u(mac, G, x - 1, y, z);
+  const float3 yp = mac_u(mac, G, x, y + 1, z), ym = mac_u(mac, G, x, y - 1, z);
+  const float3 zp = mac_u(mac, G, x, y, z + 1), zm = mac_u(mac, G, x, y, z - 1);
+  const float J[3][3] = {
+      {mul_rn(sub_rn(xp.x, xm.x), 0.5f), mul_rn(sub_rn(xp.y, xm.y), 0.5f), mul_rn(sub_rn(xp.z, xm.z), 0.5f)},
+      {mul_rn(sub_rn(yp.x, ym.x), 0.5f), mul_rn(sub_rn(yp.y, ym.y), 0.5f), mul_rn(sub_rn(yp.z, ym.z), 0.5f)},
+      {mul_rn(sub_rn(zp.x, zm.x), 0.5f), mul_rn(sub_rn(zp.y, zm.y), 0.5f), mul_rn(sub_rn(zp.z, zm.z), 0.5f)}};
+  float s2 = 0.f, om2 = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const float a = J[j][i], b = J[i][j];
+      const float S = mul_rn(0.5f, add_rn(a, b)), O = mul_rn(0.5f, sub_rn(a, b));
+      s2 = add_rn(s2, mul_rn(S, S));
+      om2 = add_rn(om2, mul_rn(O, O));
+    }
+  }
+  return mul_rn(0.5f, sub_rn(om2, s2));
+}
+
+// One block per AMR block: the maximum of |curl u| (kind 0) or Q (kind 1)
+// over the block's L0 box dilated by `lead` cells (reduceQ,
+// common_d3_criterion.wgsl:122-158).
 __global__ void __launch_bounds__(64) k_criterion(const float4* __restrict__ mac, float* __restrict__ crit, PoolGeom G,
-                                                  int lead) {
+                                                  int lead, int kind) {
   const int b = blockIdx.x;
   const int bx = b % G.nbx, by = (b / G.nbx) % G.nby, bz = b / (G.nbx * G.nby);
   const int x0 = bx * G.rb - lead, y0 = by * G.rb - lead, z0 = bz * G.rb - lead;
@@ -800,7 +885,7 @@ __global__ void __launch_bounds__(64) k_criterion(const float4* __restrict__ mac
   float best = -1e30f;
   for (int k = threadIdx.x; k < total; k += 64) {
     const int kx = k % n, ky = (k / n) % n, kz = k / (n * n);
-    best = fmaxf(best, vort_mag(mac, G, x0 + kx, y0 + ky, z0 + kz));
+    best = fmaxf(best, kind ? q_crit(mac, G, x0 + kx, y0 + ky, z0 + kz) : vort_mag(mac, G, x0 + kx, y0 + ky, z0 + kz));
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) best = fmaxf(best, __shfl_xor(best, off, 64));
@@ -844,11 +929,14 @@ __global__ void __launch_bounds__(1024) k_manage(ManageArgs a) {
   __shared__ int nfree_s;
   const int tid = threadIdx.x;
   // 1. decide, with hysteresis (common_d3_manage.wgsl decide: a held block
-  //    is judged against the lower coarsen threshold)
+  //    is judged against the lower coarsen threshold), the body's geometry
+  //    OR the field (:443-462)
   for (int b = tid; b < a.nblocks; b += 1024) {
     const bool held = a.block_slot[b] >= 0;
     const float c = a.crit[b];
-    a.want[b] = held ? (c >= a.keep_at ? 1 : 0) : (c >= a.refine_at ? 1 : 0);
+    const uint8_t gm = a.geom ? a.geom[b] : 0;
+    const bool field = !(gm & 2) && (held ? c >= a.keep_at : c >= a.refine_at);
+    a.want[b] = ((gm & 1) || field) ? 1 : 0;
   }
   __syncthreads();
   // 2. dying blocks (held, no longer wanted), ascending
@@ -975,22 +1063,26 @@ struct OpCellClass {
   static hipError_t run(uint8_t* ccls, const PoolGeom& g, const SurfacePlan& plan, const int32_t* active, int nactive,
                         hipStream_t s) {
     hipLaunchKernelGGL((k_cell_class<Q, RB>), grid_for(g.ncells), dim3(256), 0, s, ccls, g, plan);
-    if (nactive > 0)
-      hipLaunchKernelGGL((k_ring_plan<RB>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s, g, plan,
+    if (nactive > 0) {
+      hipLaunchKernelGGL((k_ring_plan<Q, RB>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s, g, plan,
                          active, nactive);
+      hipLaunchKernelGGL(k_tile_table, grid_for((uint64_t)nactive), dim3(256), 0, s, g, active, nactive,
+                         const_cast<int32_t*>(g.tile_table));
+    }
     return hipGetLastError();
   }
 };
 template <int Q, int RB>
 struct OpPoolStep {
   static hipError_t run(const float* gin, float* gout, float* mac_pool, const PoolGeom& g, const CollideParams& p,
-                        const int32_t* active, int nactive, int collide_ring, hipStream_t s) {
+                        int nactive, int collide_ring, hipStream_t s) {
+    const dim3 grid = grid_for((uint64_t)nactive * Tile<RB>::CELLS);
     if (g.body.on)
-      hipLaunchKernelGGL((k_pool_step<Q, RB, true>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s,
-                         gin, gout, mac_pool, g, p, active, nactive, collide_ring);
+      hipLaunchKernelGGL((k_pool_step<Q, RB, true>), grid, dim3(256), 0, s, gin, gout, mac_pool, g, p, nactive,
+                         collide_ring);
     else
-      hipLaunchKernelGGL((k_pool_step<Q, RB, false>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s,
-                         gin, gout, mac_pool, g, p, active, nactive, collide_ring);
+      hipLaunchKernelGGL((k_pool_step<Q, RB, false>), grid, dim3(256), 0, s, gin, gout, mac_pool, g, p, nactive,
+                         collide_ring);
     return hipGetLastError();
   }
 };
@@ -1083,10 +1175,9 @@ hipError_t launch_cell_class(int q, uint8_t* ccls, const PoolGeom& g, const Surf
 }
 
 hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
-                            const CollideParams& fine, const int32_t* active, int nactive, int collide_ring,
-                            hipStream_t s) {
+                            const CollideParams& fine, int nactive, int collide_ring, hipStream_t s) {
   if (nactive <= 0) return hipSuccess;
-  return dispatch_q_rb<OpPoolStep>(q, g.rb, gin, gout, mac_pool, g, fine, active, nactive, collide_ring, s);
+  return dispatch_q_rb<OpPoolStep>(q, g.rb, gin, gout, mac_pool, g, fine, nactive, collide_ring, s);
 }
 uint32_t pool_force_blocks(int rb, int nactive) {
   const uint64_t per = (uint64_t)(2 * rb) * (2 * rb) * (2 * rb);
@@ -1132,9 +1223,9 @@ hipError_t launch_average(int q, const float* gp, float* gc, float* mac, const P
   return dispatch_q_rb<OpAverage>(q, g.rb, gp, gc, mac, g, pairs, npairs, rescale, s);
 }
 
-hipError_t launch_criterion(const float* mac, float* crit, const PoolGeom& g, float lead, hipStream_t s) {
+hipError_t launch_criterion(const float* mac, float* crit, const PoolGeom& g, float lead, int kind, hipStream_t s) {
   hipLaunchKernelGGL(k_criterion, dim3(g.nbx * g.nby * g.nbz), dim3(64), 0, s, (const float4*)mac, crit, g,
-                     (int)lead);
+                     (int)lead, kind);
   return hipGetLastError();
 }
 

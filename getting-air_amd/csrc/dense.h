@@ -35,9 +35,12 @@ __host__ __device__ inline uint32_t div_invariant(uint32_t n, InvariantDiv v) {
 // A step's 2Q streams then walk a few contiguous regions of HBM (the blocks
 // of the source rows and of the own row) instead of 2Q planes a lattice-size
 // apart, whose concurrent accesses crowd the same DRAM channels and banks.
-// Every access of a wave stays one contiguous 256 B (f32) or 128 B (16-bit)
-// span, as in a plane-major layout.
-constexpr uint32_t kLatBlock = 256;
+// A block is one wave's cells: a wave's Q stores fill one contiguous span of
+// Q * 256 B (f32), and each of its loads is one 256 B (f32) or 128 B (16-bit)
+// span.  Measured with a stream-only probe of the D3Q19 pull on 256x128x128
+// (19 x-shifted loads, 19 stores per cell): blocks of 32 / 64 / 128 / 256 /
+// 1024 cells moved 6.29 / 6.31 / 6.12 / 6.09 / 6.13 TB/s, plane-major 5.26.
+constexpr uint32_t kLatBlock = 64;
 __host__ __device__ constexpr size_t lattice_elems(int q, uint32_t ncells) {
   return (size_t)((ncells + kLatBlock - 1) / kLatBlock) * (size_t)q * kLatBlock;
 }
@@ -180,9 +183,11 @@ hipError_t launch_unpack(int q, int storage, const void* f, float* wire, uint32_
 
 // Row mode of the step kernels: every wave of a tile lies in one grid row and
 // rows are whole layout blocks, so a source row's address is wave-uniform.
+// Waves are 64 consecutive threads; a linear tile's waves start at multiples
+// of 64 cells, a 2D tile's at multiples of 64 in x when bx is one.
 inline bool step_row_mode(int nx, bool linear, int bx, int by) {
-  if (nx % (int)kLatBlock != 0) return false;
-  return linear ? (int)kLatBlock % (bx * by) == 0 : bx % 64 == 0;
+  if (nx % (int)kLatBlock != 0 || nx % 64 != 0 || (bx * by) % 64 != 0) return false;
+  return linear || bx % 64 == 0;
 }
 hipError_t launch_classify(const ClassifyArgs& a, hipStream_t s);
 // Link masks of the cells next to the body (StepArgs::links) from the class

@@ -15,8 +15,8 @@
 // which is the same collision with the large constant w_i taken out before
 // rounding.
 //
-// Layout: blocks of 256 consecutive cells, each holding its Q populations as
-// Q runs of 256 elements (dense.h kLatBlock, lat_elem).
+// Layout: blocks of kLatBlock (64) consecutive cells, each holding its Q
+// populations as Q runs of 64 elements (dense.h kLatBlock, lat_elem).
 // One thread owns one cell; a wave covers 64 consecutive x, so every plane
 // load and store of a wave is one contiguous 256 B (f32) or 128 B (16-bit)
 // span, shifted by at most one element for the x-moving directions.  All Q
@@ -63,7 +63,7 @@ __device__ __forceinline__ T* block_of(T* g, uint32_t c) {
 // (rows are whole layout blocks).
 template <int Q>
 __device__ __forceinline__ uint32_t row_off(int x) {
-  return (uint32_t)(x >> 8) * (Q * kLatBlock) + (uint32_t)(x & 255);
+  return (uint32_t)x / kLatBlock * (Q * kLatBlock) + (uint32_t)x % kLatBlock;
 }
 
 // One step of tile b of a solve: every thread of the block pulls, collides

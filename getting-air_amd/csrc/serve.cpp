@@ -179,6 +179,8 @@ struct lbm_serve {
   hipStream_t stream = nullptr;
   hipStream_t setup = nullptr;       // admissions and copy-back
   hipEvent_t setup_done = nullptr;   // recorded after each admission round
+  hipEvent_t chunk_ev[2] = {nullptr, nullptr};  // around each chunk's graph launch
+  double chunk_bytes_pending = 0;    // algorithmic bytes of the chunk in flight
   std::vector<std::unique_ptr<Member>> finishing;  // done, results not yet copied back
   std::vector<int> finishing_adv;
   double resident_cells = 0;
@@ -191,8 +193,13 @@ struct lbm_serve {
   int64_t next_ticket = 1;
   bool stop = false;
   std::thread worker;
-  // graph of the current chunk (all classes); rebuilt when any class changes
+  // graph of the current chunk (all classes); re-captured when any class
+  // changes, then updated in place or, when its shape changed, instantiated
   hipGraphExec_t graph = nullptr;
+  int64_t graph_instantiations = 0, graph_updates = 0;
+  // device time of the chunk graphs (HIP events on the compute stream) and
+  // the algorithmic bytes they moved: 2 Q s per cell update
+  double device_seconds = 0, device_bytes = 0;
   // stats
   int64_t submitted = 0, completed = 0, failed = 0;
   double cell_updates = 0, first_submit = -1, last_finish = 0;
@@ -324,19 +331,17 @@ int grow(T** p, size_t* cap, size_t n) {
 }
 
 // Uploads every class's solve table for this chunk and, when membership
-// changed, its tile table.  Returns whether the chunk graph must be rebuilt.
-// The instantiated graph holds the tables' device addresses, so it is
-// destroyed before any table is reallocated.
+// changed, its tile table.  Returns whether the chunk graph must be
+// re-captured.  The executable graph holds the tables' device addresses; it
+// is not launched again before launch_chunk has updated it with the new
+// capture, so tables may be reallocated under it (the stream is idle here:
+// complete_chunk synchronised it).
 int prepare(lbm_serve* sv, bool* rebuild) {
   *rebuild = sv->graph == nullptr;
   for (auto& kv : sv->classes) {
     Class* c = kv.second.get();
     if (c->dirty) {
       *rebuild = true;
-      if (sv->graph) {
-        (void)hipGraphExecDestroy(sv->graph);
-        sv->graph = nullptr;
-      }
       std::vector<BatchTile>& tiles = c->h_tiles;
       tiles.clear();
       for (uint32_t k = 0; k < c->members.size(); k++)
@@ -395,8 +400,10 @@ int launch_chunk(lbm_serve* sv) {
   int rc = prepare(sv, &rebuild);
   if (rc) return rc;
   if (rebuild) {
-    if (sv->graph) (void)hipGraphExecDestroy(sv->graph);
-    sv->graph = nullptr;
+    // Re-capture the chunk; when only launch shapes and table addresses
+    // changed (the same classes, so the same kernels in the same order), the
+    // executable graph is updated in place, which costs far less host time
+    // than instantiating a new one.
     hipGraph_t g = nullptr;
     HIPCHK_RET(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal));
     rc = enqueue_chunk(sv, K);
@@ -406,11 +413,43 @@ int launch_chunk(lbm_serve* sv) {
       return rc;
     }
     if (e != hipSuccess) return hip_fail(e, "chunk capture");
-    e = hipGraphInstantiate(&sv->graph, g, nullptr, nullptr, 0);
+    bool updated = false;
+    if (sv->graph) {
+      hipGraphNode_t bad = nullptr;
+      hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
+      updated = hipGraphExecUpdate(sv->graph, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+      (void)hipGetLastError();
+      if (!updated) {
+        (void)hipGraphExecDestroy(sv->graph);
+        sv->graph = nullptr;
+      }
+    }
+    if (!updated) {
+      e = hipGraphInstantiate(&sv->graph, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(e, "chunk instantiate");
+      }
+      sv->graph_instantiations++;
+    } else {
+      sv->graph_updates++;
+    }
     (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return hip_fail(e, "chunk instantiate");
   }
+  {
+    // bytes of this chunk: every member steps min(remaining, K) times
+    std::lock_guard<std::mutex> g(sv->mu);
+    double bytes = 0;
+    for (auto& m : sv->resident) {
+      const int n = std::min(m->r->steps - m->done_steps, K);
+      const double elem = m->r->d.storage == LBM_STORE_F32 ? 4.0 : 2.0;
+      bytes += (double)n * m->r->cells * 2.0 * m->r->d.q * elem;
+    }
+    sv->chunk_bytes_pending = bytes;
+  }
+  HIPCHK_RET(hipEventRecord(sv->chunk_ev[0], sv->stream));
   HIPCHK_RET(hipGraphLaunch(sv->graph, sv->stream));
+  HIPCHK_RET(hipEventRecord(sv->chunk_ev[1], sv->stream));
   return LBM_OK;
 }
 
@@ -420,8 +459,12 @@ int launch_chunk(lbm_serve* sv) {
 int complete_chunk(lbm_serve* sv) {
   const int K = std::max(1, sv->d.chunk_steps);
   HIPCHK_RET(hipStreamSynchronize(sv->stream));
+  float ms = 0.f;
+  HIPCHK_RET(hipEventElapsedTime(&ms, sv->chunk_ev[0], sv->chunk_ev[1]));
   std::lock_guard<std::mutex> g(sv->mu);
   sv->chunks++;
+  sv->device_seconds += ms * 1e-3;
+  sv->device_bytes += sv->chunk_bytes_pending;
   std::vector<std::unique_ptr<Member>> keep;
   for (auto& m : sv->resident) {
     if (!m->in_chunk) {
@@ -582,6 +625,8 @@ int lbm_serve_create(const lbm_serve_desc* desc, lbm_serve** out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->setup, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->setup_done, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventRecord(sv->setup_done, sv->setup);
+  if (e == hipSuccess) e = hipEventCreate(&sv->chunk_ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&sv->chunk_ev[1]);
   if (e != hipSuccess) return hip_fail(e, "serve streams");
   lbm_serve* p = sv.release();
   p->worker = std::thread(worker, p);
@@ -605,6 +650,8 @@ int lbm_serve_destroy(lbm_serve* sv) {
   if (sv->stream) (void)hipStreamDestroy(sv->stream);
   if (sv->setup) (void)hipStreamDestroy(sv->setup);
   if (sv->setup_done) (void)hipEventDestroy(sv->setup_done);
+  for (hipEvent_t ev : sv->chunk_ev)
+    if (ev) (void)hipEventDestroy(ev);
   delete sv;
   return LBM_OK;
 }
@@ -765,6 +812,10 @@ int lbm_serve_read_stats(lbm_serve* sv, lbm_serve_stats* out) {
   out->peak_resident_bytes = (double)sv->pool.peak();
   out->peak_resident = sv->peak_resident;
   out->chunks = sv->chunks;
+  out->device_seconds = sv->device_seconds;
+  out->device_bytes = sv->device_bytes;
+  out->graph_instantiations = sv->graph_instantiations;
+  out->graph_updates = sv->graph_updates;
   return LBM_OK;
 }
 

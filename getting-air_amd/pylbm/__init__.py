@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblbm.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 BGK, TRT = 0, 1
 STORE = {"f32": 0, "f16": 1, "bf16": 2}
 BODY_NONE, BODY_SPHERE, BODY_CIRCLE = 0, 1, 2
@@ -61,7 +61,7 @@ class AmrDesc(ctypes.Structure):
         ("lead", ctypes.c_int32), ("explode_linear", ctypes.c_int32), ("orphans", ctypes.c_int32),
         ("interface_interp", ctypes.c_int32), ("reflux", ctypes.c_int32),
         ("sponge_width", ctypes.c_float), ("sponge_axes", ctypes.c_int32 * 3), ("sponge_u", ctypes.c_float * 3),
-        ("body", Body), ("explode_dirs", ctypes.c_int32),
+        ("body", Body), ("explode_dirs", ctypes.c_int32), ("criterion", ctypes.c_int32),
     ]
 
 
@@ -130,7 +130,9 @@ class ServeStats(ctypes.Structure):
                 ("first_submit", ctypes.c_double), ("last_finish", ctypes.c_double),
                 ("p50_latency", ctypes.c_double), ("p99_latency", ctypes.c_double), ("mlups", ctypes.c_double),
                 ("pool_bytes", ctypes.c_double), ("peak_resident_bytes", ctypes.c_double),
-                ("peak_resident", ctypes.c_int32), ("chunks", ctypes.c_int32)]
+                ("peak_resident", ctypes.c_int32), ("chunks", ctypes.c_int32),
+                ("device_seconds", ctypes.c_double), ("device_bytes", ctypes.c_double),
+                ("graph_instantiations", ctypes.c_int64), ("graph_updates", ctypes.c_int64)]
 
 
 class LbmError(RuntimeError):
@@ -400,7 +402,8 @@ def make_amr_desc(d):
     force, manageEvery, refineAt, keepAt, lead, explodeLinear, orphans,
     interface ('explode' or 'interp'), reflux, sponge: {width, axes, u} on
     L0, body: {a, centre} a pinned sphere solved on the fine level,
-    explodeDirs: directions per explode thread (0: automatic)}."""
+    explodeDirs: directions per explode thread (0: automatic),
+    criterion: "vorticity" (|curl u|, default) or "q"}."""
     a = AmrDesc()
     a.abi_version = ABI_VERSION
     a.q = int(d["q"])
@@ -431,6 +434,7 @@ def make_amr_desc(d):
         a.sponge_axes[k] = int(bool(axes[k]))
         a.sponge_u[k] = float((list(sp.get("u", [0, 0, 0])) + [0, 0, 0])[k])
     a.explode_dirs = int(d.get("explodeDirs", 0))
+    a.criterion = {"vorticity": 0, "q": 1}[d.get("criterion", "vorticity")]
     body = d.get("body")
     if body:
         a.body.shape = BODY_SPHERE

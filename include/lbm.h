@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 4
+#define LBM_ABI_VERSION 5
 
 enum {
   LBM_OK = 0,
@@ -165,9 +165,9 @@ int lbm_read_totals(lbm_solve* s, double* out4);
  * ([mass, px, py, pz], FP64) and enqueued on `stream` (a hipStream_t; null:
  * the null stream) without waiting.  The lattice is in the engine's device
  * layout: shifted populations g = f - w_i in the storage format (LBM_STORE_*),
- * cells in blocks of 256, each block holding its q populations as q runs of
- * 256 elements (population i of cell c at element ((c / 256) q + i) 256 +
- * c % 256).  For the lattice a solve holds, the four sums equal
+ * cells in blocks of 64, each block holding its q populations as q runs of
+ * 64 elements (population i of cell c at element ((c / 64) q + i) 64 +
+ * c % 64).  For the lattice a solve holds, the four sums equal
  * lbm_read_totals' bit for bit.  Allocates nothing: the per-block partial
  * sums and the completion counter live in one fixed device array of the
  * library, LBM_NOALLOC_SCRATCH_BYTES on each GPU it runs on, held until the
@@ -312,11 +312,22 @@ typedef struct lbm_amr_desc {
                          * 234-276, :387-403), and one force pass per
                          * macro-step integrates it with weight dx^2 = 1/4
                          * (common_d3_force_pool.wgsl).  Needs the explode
-                         * interface and a static topology that covers every
-                         * L0 cell within 1 of it.                            */
+                         * interface.  With manage_every = 0 an uploaded
+                         * topology must cover every L0 cell within 1 of it;
+                         * with the manager, every block whose box comes
+                         * within 2 L0 cells of the body is kept refined
+                         * whatever the field says (the geometry term of
+                         * common_d3_manage.wgsl:257-302, OR'ed with the
+                         * field at :443-462), and blocks reaching into the
+                         * sponge band are never refined by the field.       */
   int32_t explode_dirs; /* directions per explode thread, 1..q; 0: chosen from
                          * the ring size (about 2^20 threads per launch).  A
                          * launch-shape knob: results do not depend on it.    */
+  int32_t criterion;    /* the field refine_at / keep_at are thresholds on:
+                         * 0 the vorticity magnitude |curl u|; 1 Q = 1/2
+                         * (|Omega|^2 - |S|^2) (qAt, common_d3_criterion.wgsl:
+                         * 102-120).  Either is the block maximum over the
+                         * block's L0 box dilated by `lead`.                 */
 } lbm_amr_desc;
 
 typedef struct lbm_amr lbm_amr;
@@ -411,6 +422,11 @@ typedef struct lbm_serve_stats {
   double pool_bytes, peak_resident_bytes;
   int32_t peak_resident;
   int32_t chunks;        /* chunks executed                                    */
+  double device_seconds; /* device time of the chunk graphs (HIP events)       */
+  double device_bytes;   /* algorithmic bytes they moved: 2 Q s per cell update
+                            (s = 4 for f32, 2 for the 16-bit formats)         */
+  int64_t graph_instantiations; /* chunk graphs instantiated                    */
+  int64_t graph_updates; /* chunk graphs updated in place for a new membership */
 } lbm_serve_stats;
 
 typedef struct lbm_serve lbm_serve;

@@ -52,6 +52,8 @@ The interp interface is not conservative; with the flux correction the
 composite mass and momentum are conserved to float64 rounding (the claim of
 common_d3_amr_flux.wgsl:8-11), and the same check fails without it.
 """
+import math
+
 import numpy as np
 
 GHOST = 2
@@ -142,6 +144,29 @@ def vorticity_magnitude(mac, dtype=np.float64):
     return np.sqrt((wx * wx + wy * wy) + wz * wz)
 
 
+def q_criterion(mac, dtype=np.float64):
+    """Q = 1/2 (|Omega|^2 - |S|^2) per L0 cell (qAt, common_d3_criterion.wgsl:
+    102-120; qOfGrad, d3-criterion.mjs), J[j][i] = du_i/dx_j by central
+    differences with periodic wrap.  dtype=float32 rounds every operation to
+    binary32 in the kernel's order: J = (u(x+1) - u(x-1)) * 0.5, then over i
+    (outer) and j (inner) S = 0.5 (J[j][i] + J[i][j]), O = 0.5 (J[j][i] -
+    J[i][j]), s2 += S S, om2 += O O, and Q = 0.5 (om2 - s2)."""
+    u = [np.asarray(mac[k], dtype=dtype) for k in (1, 2, 3)]
+    h = dtype(0.5)
+    axes = (2, 1, 0)  # d/dx, d/dy, d/dz on [z, y, x] arrays
+    J = [[(np.roll(u[i], -1, axes[j]) - np.roll(u[i], 1, axes[j])) * h for i in range(3)] for j in range(3)]
+    s2 = np.zeros_like(u[0])
+    om2 = np.zeros_like(u[0])
+    for i in range(3):
+        for j in range(3):
+            a, b = J[j][i], J[i][j]
+            S = h * (a + b)
+            O = h * (a - b)
+            s2 = s2 + S * S
+            om2 = om2 + O * O
+    return h * (om2 - s2)
+
+
 def block_criterion(vm, rb, lead):
     """Per block max of vm over the block's L0 box dilated by `lead` cells
     (common_d3_criterion.wgsl reduceQ), block id (bz nby + by) nbx + bx."""
@@ -159,12 +184,61 @@ def block_criterion(vm, rb, lead):
     return out
 
 
-def decide(block_slot, crit, refine_at, keep_at):
+def decide(block_slot, crit, refine_at, keep_at, geom=None):
     """Want flag with hysteresis: a held block stays while crit >= keep_at,
     an unrefined one refines when crit >= refine_at (common_d3_manage.wgsl
-    decide, common_refine.wgsl:54-79 ladder at two levels)."""
+    decide, common_refine.wgsl:54-79 ladder at two levels).  `geom`
+    (geometry_mask): bit 0 forces a block in, whatever the field says
+    (geom || field, common_d3_manage.wgsl:443-462); bit 1 keeps the field
+    criterion out of a block."""
     held = block_slot >= 0
-    return np.where(held, crit >= keep_at, crit >= refine_at).astype(np.uint8)
+    field = np.where(held, crit >= keep_at, crit >= refine_at)
+    if geom is None:
+        return field.astype(np.uint8)
+    return (((geom & 1) != 0) | (field & ((geom & 2) == 0))).astype(np.uint8)
+
+
+SQRT3_UP = 1.7320509  # sqrt(3) rounded up (d3-amr.mjs:120, common_d3_manage.wgsl:178)
+
+
+def near_body(sdf, lo, hi, margin, depth=3):
+    """Whether any point of the box [lo, hi) comes within `margin` of the
+    body: the Lipschitz branch and bound of nearBodyWant (d3-amr.mjs:139-159)
+    and boxNearBody (common_d3_manage.wgsl:226-255).  phi(centre) - h sqrt(3)
+    bounds the box minimum from below; an ambiguous box splits into octants,
+    down to `depth`, where it is accepted."""
+    def near(c, h, d):
+        phi = sdf(c)
+        if phi - h * SQRT3_UP > margin:
+            return False
+        if phi <= margin or d == 0:
+            return True
+        q = h / 2
+        for k in range(8):
+            cc = [c[0] + (q if k & 1 else -q), c[1] + (q if k & 2 else -q), c[2] + (q if k & 4 else -q)]
+            if near(cc, q, d - 1):
+                return True
+        return False
+    return near([(lo[k] + hi[k]) / 2 for k in range(3)], (hi[0] - lo[0]) / 2, depth)
+
+
+def geometry_mask(nbx, nby, nbz, rb, centre, radius, margin, sponge_width=0.0, sponge_axes=(0, 0, 0), dims=None):
+    """Per block: bit 0 the body's geometry wants it refined (blockWanted,
+    common_d3_manage.wgsl:257-302, a pinned body: no look-ahead term), bit 1
+    it reaches into the sponge band, so the field criterion may not refine it
+    (the fine step does not relax toward the far field)."""
+    out = np.zeros(nbx * nby * nbz, dtype=np.uint8)
+    sdf = lambda c: math.sqrt(sum((c[k] - centre[k]) ** 2 for k in range(3))) - radius
+    for b in range(out.size):
+        bc = (b % nbx, (b // nbx) % nby, b // (nbx * nby))
+        lo = [bc[k] * rb for k in range(3)]
+        if radius is not None and near_body(sdf, lo, [v + rb for v in lo], margin):
+            out[b] |= 1
+        if sponge_width > 0:
+            for k in range(3):
+                if sponge_axes[k] and (lo[k] < sponge_width or (dims[k] - 1) - (lo[k] + rb - 1) < sponge_width):
+                    out[b] |= 2
+    return out
 
 
 def manage_slots(block_slot, slot_to_block, want):
@@ -225,6 +299,7 @@ class AmrOracle:
         self.refine_at = float(desc.get("refineAt", 1e30))
         self.keep_at = float(desc.get("keepAt", 1e30))
         self.lead = int(desc.get("lead", 0))
+        self.criterion_kind = desc.get("criterion", "vorticity")
         self.linear = bool(desc.get("explodeLinear", 1))
         self.orphans = bool(desc.get("orphans", 1))
         self.interp_iface = desc.get("interface", "explode") == "interp"
@@ -236,6 +311,13 @@ class AmrOracle:
         self.body = None
         if body:
             self.body = (np.array(body["centre"], dtype=np.float32), np.float32(body["a"]))
+        self.geom = None
+        if body and self.manage_every > 0:
+            sp = desc.get("sponge") or {}
+            self.geom = geometry_mask(self.nbx, self.nby, self.nbz, self.rb, [float(v) for v in self.body[0]],
+                                      float(self.body[1]), float(desc.get("bodyMargin", 2.0)),
+                                      float(sp.get("width", 0.0)), sp.get("axes", (0, 0, 0)),
+                                      (self.nx, self.ny, self.nz))
         self.forces = []
         self.block_slot = np.full(self.nblocks, -1, dtype=np.int32)
         self.slot_to_block = np.full(self.max_slots, -1, dtype=np.int32)
@@ -608,9 +690,9 @@ class AmrOracle:
         self.average([(int(self.slot_to_block[s]), int(s)) for s in active_list(self.slot_to_block)])
 
     def criterion(self):
-        """Per block |curl u| maximum, binary32 as the engine computes it."""
-        vm = vorticity_magnitude(self.mac0.astype(np.float32), np.float32)
-        return block_criterion(vm, self.rb, self.lead)
+        """Per block |curl u| (or Q) maximum, binary32 as the engine computes it."""
+        field = q_criterion if self.criterion_kind == "q" else vorticity_magnitude
+        return block_criterion(field(self.mac0.astype(np.float32), np.float32), self.rb, self.lead)
 
     def manage(self, crit=None):
         """Criterion -> restrict every tile -> decide -> coarsen -> refine ->
@@ -618,7 +700,8 @@ class AmrOracle:
         follow its integer decisions exactly)."""
         self.crit = self.criterion() if crit is None else np.asarray(crit, dtype=np.float32)
         self.restrict_active()
-        self.want = decide(self.block_slot, self.crit, np.float32(self.refine_at), np.float32(self.keep_at))
+        self.want = decide(self.block_slot, self.crit, np.float32(self.refine_at), np.float32(self.keep_at),
+                           self.geom)
         bs, sb, dying, fresh, exh = manage_slots(self.block_slot, self.slot_to_block, self.want)
         self.block_slot, self.slot_to_block = bs, sb
         self.last_dying, self.last_fresh = dying, fresh

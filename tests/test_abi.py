@@ -70,10 +70,10 @@ def test_amr_descriptor_layout_matches_c():
 #include <stddef.h>
 #include "lbm.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_amr_desc), offsetof(lbm_amr_desc, omega),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_amr_desc), offsetof(lbm_amr_desc, omega),
          offsetof(lbm_amr_desc, force), offsetof(lbm_amr_desc, manage_every),
          offsetof(lbm_amr_desc, lead), offsetof(lbm_amr_desc, orphans), offsetof(lbm_amr_desc, reflux),
-         offsetof(lbm_amr_desc, body), offsetof(lbm_amr_desc, explode_dirs));
+         offsetof(lbm_amr_desc, body), offsetof(lbm_amr_desc, explode_dirs), offsetof(lbm_amr_desc, criterion));
   return 0;
 }
 """
@@ -85,7 +85,7 @@ int main(void) {
         got = [int(v) for v in subprocess.check_output([exe]).split()]
     A = pylbm.AmrDesc
     want = [ctypes.sizeof(A), A.omega.offset, A.force.offset, A.manage_every.offset, A.lead.offset,
-            A.orphans.offset, A.reflux.offset, A.body.offset, A.explode_dirs.offset]
+            A.orphans.offset, A.reflux.offset, A.body.offset, A.explode_dirs.offset, A.criterion.offset]
     assert got == want
 
 
@@ -114,6 +114,13 @@ def test_amr_refusals_without_gpu():
     d = pylbm.make_amr_desc(dict(base, explodeDirs=28))
     assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
     assert b"explode_dirs" in lib.lbm_last_error()
+    d = pylbm.make_amr_desc(base)
+    d.criterion = 2
+    assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"criterion" in lib.lbm_last_error()
+    d = pylbm.make_amr_desc(dict(base, nx=1032, rb=8))  # L0 coordinates are packed in 10 bits
+    assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"1024" in lib.lbm_last_error()
 
 
 def test_serve_struct_layouts_match_c():
@@ -122,10 +129,11 @@ def test_serve_struct_layouts_match_c():
 #include <stddef.h>
 #include "lbm.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_serve_desc), offsetof(lbm_serve_desc, budget_bytes),
-         offsetof(lbm_serve_desc, autotune), sizeof(lbm_serve_result), offsetof(lbm_serve_result, t_submit),
-         offsetof(lbm_serve_result, force), sizeof(lbm_serve_stats), offsetof(lbm_serve_stats, peak_resident),
-         offsetof(lbm_serve_stats, p99_latency));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_serve_desc),
+         offsetof(lbm_serve_desc, budget_bytes), offsetof(lbm_serve_desc, autotune), sizeof(lbm_serve_result),
+         offsetof(lbm_serve_result, t_submit), offsetof(lbm_serve_result, force), sizeof(lbm_serve_stats),
+         offsetof(lbm_serve_stats, peak_resident), offsetof(lbm_serve_stats, p99_latency),
+         offsetof(lbm_serve_stats, device_bytes), offsetof(lbm_serve_stats, graph_updates));
   return 0;
 }
 """
@@ -137,7 +145,8 @@ int main(void) {
         got = [int(v) for v in subprocess.check_output([exe]).split()]
     D, R, S = pylbm.ServeDesc, pylbm.ServeResult, pylbm.ServeStats
     want = [ctypes.sizeof(D), D.budget_bytes.offset, D.autotune.offset, ctypes.sizeof(R), R.t_submit.offset,
-            R.force.offset, ctypes.sizeof(S), S.peak_resident.offset, S.p99_latency.offset]
+            R.force.offset, ctypes.sizeof(S), S.peak_resident.offset, S.p99_latency.offset, S.device_bytes.offset,
+            S.graph_updates.offset]
     assert got == want
 
 

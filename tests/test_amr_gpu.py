@@ -189,6 +189,45 @@ def test_manager_is_bit_exact_and_fields_follow():
         assert amr.info()["active_tiles"] > 0 and changes > 38
 
 
+def test_q_criterion_is_bit_exact_and_drives_the_manager():
+    """criterion 'q': the block maximum of Q = 1/2 (|Omega|^2 - |S|^2)
+    (qAt, common_d3_criterion.wgsl:102-158) bit-exact against the oracle's
+    binary32 restatement, over a box dilated by one cell, and the manager's
+    decisions and maps from it as the oracle twin's."""
+    n, rb = 32, 4
+    desc = {"q": 19, "nx": n, "ny": n, "nz": n, "rb": rb, "maxSlots": 128, "omega": 1 / 0.56,
+            "manageEvery": 2, "lead": 1, "refineAt": 4.0e-6, "keepAt": 3.0e-6, "criterion": "q"}
+    _, mac, _ = scenarios.vortex_rings(n=n, radius=5.0, core=1.5, circulation=0.05, seed=3, jitter=2.0,
+                                       tau=0.56, q=19, rings=2)
+    say("Q criterion")
+    amr, orc = make(desc, mac)
+    with amr:
+        prev_bs, prev_sb = amr.read_topology()
+        refined = 0
+        for step in range(6):
+            manage = step % 2 == 0
+            if manage:
+                m = amr.read_macro().reshape(n, n, n, 4)
+                want_crit = ao.block_criterion(ao.q_criterion(np.moveaxis(m, -1, 0), np.float32), rb, desc["lead"])
+            amr.step(1)
+            if manage:
+                crit, want = amr.read_criterion()
+                assert (crit.view(np.uint32) == want_crit.view(np.uint32)).all(), step
+                dec = ao.decide(prev_bs, crit, np.float32(desc["refineAt"]), np.float32(desc["keepAt"]))
+                assert (want == dec).all(), step
+                bs, sb = amr.read_topology()
+                tbs, tsb, _, _, exh = ao.manage_slots(prev_bs, prev_sb, dec)
+                assert (bs == tbs).all() and (sb == tsb).all() and exh == 0, step
+                refined = max(refined, int((bs >= 0).sum()))
+                say("step", step, "Q max %.3e" % crit.max(), "refined", int((bs >= 0).sum()))
+                prev_bs, prev_sb = bs, sb
+                orc.macro_step(crit=crit)
+            else:
+                orc.macro_step()
+            compare(amr, orc, label="step %d" % step)
+        assert 0 < refined < (n // rb) ** 3
+
+
 def test_pool_exhaustion_is_a_hard_failure():
     n = 16
     desc = {"q": 19, "nx": n, "ny": n, "nz": n, "rb": 4, "maxSlots": 2, "omega": 1 / 0.6,
@@ -333,11 +372,61 @@ def test_fine_level_body_matches_the_oracle(q):
         assert np.abs(f - ref).max() < 1e-5 * scale
 
 
+def test_fine_level_body_with_the_manager_matches_the_oracle():
+    """The pinned sphere with the manager running (the geometry term of
+    common_d3_manage.wgsl blockWanted OR'ed with the |curl u| criterion,
+    :443-462): every block within 2 L0 cells of the body is forced in at the
+    first pass and kept whatever the field says, while the field refines and
+    coarsens blocks in the wake.  Decisions, block maps and active list
+    bit-exact against the oracle's twin; fields and force records follow."""
+    n, rb = 24, 4
+    c, a = [9.3, 11.9, 12.1], 2.6
+    desc = {"q": 27, "nx": n, "ny": n, "nz": n, "rb": rb, "maxSlots": 96, "omega": 1 / 0.6,
+            "manageEvery": 2, "lead": 0, "refineAt": 0.0016, "keepAt": 0.0014, "body": {"a": a, "centre": c}}
+    _, mac, _ = scenarios.vortex_rings(n=n, radius=4.0, core=1.5, circulation=0.05, seed=5, jitter=2.0,
+                                       tau=0.6, q=27, rings=2)
+    mac = np.asarray(mac, np.float32).reshape(-1, 4).copy()
+    mac[:, 1] += 0.03
+    mac = mac.reshape(-1)
+    geom = ao.geometry_mask(n // rb, n // rb, n // rb, rb, [float(v) for v in np.float32(c)], float(np.float32(a)),
+                            2.0)
+    say("AMR body with the manager: forced blocks", int((geom & 1).sum()))
+    amr, orc = pylbm.Amr(desc), ao.AmrOracle(desc)
+    with amr:
+        amr.upload_macro(mac)
+        orc.upload_macro(mac)
+        prev_bs, prev_sb = amr.read_topology()
+        changes, field_only = 0, 0
+        for step in range(8):
+            manage = step % 2 == 0
+            amr.step(1)
+            if manage:
+                crit, want = amr.read_criterion()
+                dec = ao.decide(prev_bs, crit, np.float32(desc["refineAt"]), np.float32(desc["keepAt"]), geom)
+                assert (want == dec).all(), step
+                assert (want[(geom & 1) != 0] == 1).all(), step
+                bs, sb = amr.read_topology()
+                tbs, tsb, dying, fresh, exh = ao.manage_slots(prev_bs, prev_sb, dec)
+                assert (bs == tbs).all() and (sb == tsb).all(), step
+                assert (amr.read_active() == ao.active_list(tsb)).all(), step
+                assert exh == 0 and amr.info()["exhausted"] == 0
+                changes += len(dying) + len(fresh)
+                field_only = max(field_only, int(((bs >= 0) & ((geom & 1) == 0)).sum()))
+                prev_bs, prev_sb = bs, sb
+                orc.macro_step(crit=crit)
+            else:
+                orc.macro_step()
+            compare(amr, orc, label="step %d" % step)
+        f = amr.read_forces()
+        ref = np.array(orc.forces)
+        say("topology changes", changes, "field-refined blocks (max)", field_only, "forces", f[-1].tolist())
+        assert changes > int((geom & 1).sum()) and field_only > 0
+        assert f.shape == ref.shape and np.abs(f - ref).max() < 1e-5 * np.abs(ref).max()
+
+
 def test_amr_body_refusals():
     desc, mac, bs, sb = body_case(19)
     say("AMR body refusals")
-    with pytest.raises(pylbm.LbmError):
-        pylbm.Amr(dict(desc, manageEvery=4))
     with pytest.raises(pylbm.LbmError):
         pylbm.Amr(dict(desc, interface="interp"))
     with pylbm.Amr(desc) as amr:
@@ -352,8 +441,6 @@ def test_amr_body_refusals():
         with pytest.raises(pylbm.LbmError):
             amr.upload_topology(hole, hole_sb)  # a block next to the body left coarse
         amr.upload_topology(bs, sb)
-        with pytest.raises(pylbm.LbmError):
-            amr.manage_now()
         amr.step(1)
         assert amr.read_forces().shape == (1, 6)
     with pylbm.Amr(dict(desc, sponge={"width": 6.0, "axes": [1, 0, 0], "u": [0.04, 0, 0]})) as amr:

@@ -308,6 +308,38 @@ def test_criterion_closed_forms():
     assert c1[0] == 1.0 and c1[1] == 1.0 and c1[2] == 0.0
 
 
+def test_q_criterion_closed_forms():
+    """Q = 1/2 (|Omega|^2 - |S|^2) (d3-criterion.mjs qOfGrad; the closed forms
+    of the reference's test-d3-criterion.js): pure shear is exactly 0 (the
+    reason the criterion is Q and not |omega|), solid rotation at rate W gives
+    W^2, pure strain -W^2; and |omega| differs from Q where it should (shear
+    has |omega| = gamma > 0).  Q does not change when the gradient is
+    transposed (Omega only changes sign), so the last case pins the
+    off-diagonal terms instead: u = (a y, b x, 0) has Q = -a b."""
+    n = 16
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    gamma, W = 2.0 ** -6, 2.0 ** -7
+    mac = np.zeros((4, n, n, n))
+    mac[0] = 1.0
+    mac[1] = gamma * y
+    for dt in (np.float32, np.float64):
+        q = ao.q_criterion(mac, dt)
+        assert q.dtype == dt and (q[:, 1:-1, :] == 0).all()
+        assert (ao.vorticity_magnitude(mac, dt)[:, 1:-1, :] == dt(gamma)).all()
+    mac[1] = -W * (y - 7.5)
+    mac[2] = W * (x - 7.5)
+    assert (ao.q_criterion(mac, np.float32)[:, 1:-1, 1:-1] == np.float32(W * W)).all()
+    mac[1] = W * (x - 7.5)
+    mac[2] = -W * (y - 7.5)
+    assert (ao.q_criterion(mac, np.float32)[:, 1:-1, 1:-1] == np.float32(-W * W)).all()
+    # u = (a y, b x, 0): Omega_xy = (b - a)/2, S_xy = (a + b)/2, so
+    # Q = ((b - a)^2 - (a + b)^2) / 4 = -a b; with a = 2 gamma, b = -gamma:
+    # Q = 2 gamma^2 (the diagonal terms alone would give 0).
+    mac[1] = 2 * gamma * (y - 7.5)
+    mac[2] = -gamma * (x - 7.5)
+    assert np.allclose(ao.q_criterion(mac)[:, 1:-1, 1:-1], 2 * gamma * gamma, rtol=0, atol=1e-18)
+
+
 def test_manager_ascending_ids_and_hysteresis():
     bs = np.array([-1, 0, -1, 2, -1, 1], dtype=np.int32)
     sb = np.array([1, 5, 3, -1], dtype=np.int32)

@@ -203,8 +203,14 @@ def test_config5_mix_at_its_sizes_served_equals_plain_and_matches_the_oracle():
         for k, t in enumerate(tickets):
             assert sv.wait(t, 1)["status"] == 0, k
             served[k] = (sv.read_macro(t), sv.read_forces(t))
-    say("stats", {k: st[k] for k in ("completed", "peak_resident", "chunks", "p50_latency", "p99_latency", "mlups")})
+    say("stats", {k: st[k] for k in ("completed", "peak_resident", "chunks", "p50_latency", "p99_latency", "mlups",
+                                     "device_seconds", "device_bytes", "graph_instantiations", "graph_updates")})
     assert st["completed"] == len(reqs) and st["peak_resident"] >= 2
+    # the chunk graphs' algorithmic bytes are every member's steps x 2 Q s
+    cells = [d["nx"] * d["ny"] * (1 if d["q"] == 9 else d["nz"]) for d, _, _ in reqs]
+    want_bytes = sum(c * n * 2 * d["q"] * 2 for c, n, (d, _, _) in zip(cells, steps, reqs))
+    assert st["device_bytes"] == want_bytes and st["device_seconds"] > 0
+    assert st["graph_instantiations"] >= 1 and st["graph_instantiations"] + st["graph_updates"] >= 2
     for k, (d, m, meta) in enumerate(reqs):
         with pylbm.Solve(dict(d, autotune=0)) as s:
             s.upload_macro(m)

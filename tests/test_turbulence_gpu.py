@@ -8,8 +8,14 @@ shifted fp16 and shifted bf16 storage, collision in fp32.
   error, measured: f16 ~1e-4, bf16 ~1e-3 relative in u).
 - At the full 512^3, 100 steps of each format with the properties the size
   allows: finite fields, FP64 mass drift of the fp32 run under 1e-6 relative
-  (the periodic box conserves mass exactly up to fp32 rounding), enstrophy
-  decaying, and the 16-bit runs within stated bounds of the fp32 run.
+  (the periodic box conserves mass exactly up to fp32 rounding), the energy
+  budget of decaying turbulence, and the 16-bit runs within stated bounds of
+  the fp32 run.  Enstrophy is not monotone here: from random phases it first
+  grows (vortex stretching builds the cascade), while the kinetic energy
+  E = <|u|^2>/2 falls at the rate dE/dt = -2 nu Omega.  With Omega from
+  central differences (which underestimate |curl u|^2) and the trapezoidal
+  rule over the 100 steps, the FP64 oracle at 64^3 gives a ratio of measured
+  to predicted energy loss of 1.04-1.11; the test asks for 0.9-1.25.
 """
 import numpy as np
 import pytest
@@ -59,6 +65,11 @@ def test_turbulence_64_every_format_against_the_fp64_oracle():
         assert rows[st]["u"] < BOUNDS[st]["u"] and abs(rows[st]["ens"]) < BOUNDS[st]["ens"], (st, rows[st])
 
 
+def kinetic_energy(m):
+    u = m.reshape(-1, 4)[:, 1:].astype(np.float64)
+    return 0.5 * float((u * u).sum(axis=1).mean())
+
+
 def test_turbulence_512_full_size_properties():
     n, steps = 512, 100
     desc, mac = sc.turbulence(n=n)
@@ -68,9 +79,13 @@ def test_turbulence_512_full_size_properties():
     assert np.isfinite(ref).all()
     drift = abs(t1[0] - t0[0]) / t0[0]
     ens_ref = sc.enstrophy(ref, n, np.float32)
-    print("f32: mass drift %.3e, enstrophy %.6e -> %.6e" % (drift, ens0, ens_ref), flush=True)
+    e0, e1 = kinetic_energy(mac), kinetic_energy(ref)
+    nu = (1.0 / desc["omega"] - 0.5) / 3.0
+    ratio = (e1 - e0) / (-2.0 * nu * 0.5 * (ens0 + ens_ref) * steps)
+    print("f32: mass drift %.3e, enstrophy %.6e -> %.6e, energy %.6e -> %.6e, loss / (2 nu Omega t) %.4f" %
+          (drift, ens0, ens_ref, e0, e1, ratio), flush=True)
     assert drift < 1e-6
-    assert ens_ref < ens0
+    assert e1 < e0 and 0.9 < ratio < 1.25, ratio
     for st in ("f16", "bf16"):
         m, _, _ = run(desc, mac, st, steps)
         assert np.isfinite(m).all(), st

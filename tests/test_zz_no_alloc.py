@@ -7,8 +7,8 @@ bit for bit: the test takes the lattice out of an exact checkpoint
 include/lbm.h documents, copies it to device memory it allocates itself, and
 calls the entry point through the C ABI on calls in a row of decreasing
 size (the fixed array keeps the previous call's contents), in every
-lattice and storage format, with a grid that is not a multiple of the
-256-cell block."""
+lattice and storage format, with grids that are not a multiple of the
+64-cell block."""
 import ctypes
 import os
 import struct
@@ -48,12 +48,15 @@ def device_copy(h, arr):
     return p
 
 
+BLOCK = 64  # cells per layout block (include/lbm.h, lbm_totals_noalloc)
+
+
 def block_layout(plane, q, n):
-    """[q][n] plane-major -> population i of cell c at ((c // 256) q + i) 256 + c % 256."""
-    nb = (n + 255) // 256
-    padded = np.zeros((q, nb * 256), dtype=plane.dtype)
+    """[q][n] plane-major -> population i of cell c at ((c // B) q + i) B + c % B."""
+    nb = (n + BLOCK - 1) // BLOCK
+    padded = np.zeros((q, nb * BLOCK), dtype=plane.dtype)
     padded[:, :n] = plane
-    return np.ascontiguousarray(padded.reshape(q, nb, 256).transpose(1, 0, 2)).reshape(-1)
+    return np.ascontiguousarray(padded.reshape(q, nb, BLOCK).transpose(1, 0, 2)).reshape(-1)
 
 
 def case(kind, storage):
