@@ -129,7 +129,7 @@ bool long_enough(napi_env env, size_t len, size_t need, const char* what) {
 
 // createSolve({q, nx, ny, nz, collision: 'bgk'|'trt', omega, omegaMinus,
 //              force, walls, wallU, sponge: {width, axes, u},
-//              body: {a, centre} | null, solidEq, storage: 'f32'|'f16'|'bf16',
+//              body: {a, centre} | null, solidEq, diffuse, chiEps, storage: 'f32'|'f16'|'bf16',
 //              autotune})
 // Fills an lbm_desc from a JS descriptor object; false (with a pending JS
 // exception) on a bad field.
@@ -178,6 +178,8 @@ bool parse_desc(napi_env env, napi_value o, lbm_desc& d) {
   }
   d.solid_eq = (int)num(env, o, "solidEq", 1);
   d.autotune = (int)num(env, o, "autotune", 0);
+  d.diffuse = (int)num(env, o, "diffuse", 0);
+  d.chi_eps = (float)num(env, o, "chiEps", 0);
   d.storage = LBM_STORE_F32;
   if (get_prop(env, o, "storage", &v)) {
     char buf[16] = {0};

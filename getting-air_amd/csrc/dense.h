@@ -101,6 +101,12 @@ struct StepArgs {
   uint32_t hist_cap;
   double body_c[3];           // torque arm origin
   int32_t periodic_arm;       // 2D: nearest periodic image for the arm
+  // The diffuse penalty coupling of the pinned body (the DIFF kernels, in
+  // place of bounce-back): its signed distance as k_classify evaluates it and
+  // chi = 1/2 (1 - tanh(clamp(phi / chi_eps, -20, 20))).
+  int32_t body_shape;         // 1 sphere, 2 circle (periodic nearest image)
+  float body_r, body_cf[3];
+  float chi_eps;
   // tiling of the 1-D grid onto (x-tiles, y-tiles, z), or, with `linear`,
   // block b covering cells [b T, (b + 1) T) of the flattened index (T threads
   // per block), so that no lane idles when nx is not a multiple of the block
@@ -159,15 +165,20 @@ struct ClassifyArgs {
   int32_t shape;           // 1 sphere, 2 circle (periodic nearest image)
   float radius;
   float cx, cy, cz;
+  float band;              // class bit 2 where phi < band (diffuse coupling's force cells)
   uint8_t* cls;
   uint32_t* links;         // filled by launch_links from cls
 };
+// Class byte bits: solid (phi < 0), near (phi < 2: a link can reach the
+// solid), band (phi < ClassifyArgs::band).
+constexpr uint8_t kClsSolid = 1, kClsNear = 2, kClsBand = 4;
 
 // `occ` caps residency at that many waves per SIMD (0: no cap) by giving
 // each block its share of the CU's LDS; the step kernels stream HBM, and the
 // best number of loads in flight depends on the element width and the grid.
+// `diffuse`: the pinned body's diffuse penalty coupling (with `general`).
 hipError_t launch_step(int q, int storage, bool general, bool nontemporal, int occ, const void* fin, void* fout,
-                       const StepArgs& a, int bx, int by, hipStream_t s);
+                       const StepArgs& a, int bx, int by, hipStream_t s, bool diffuse = false);
 // `rows`: step_row_mode holds for every solve of the batch.
 hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, bool rows,
                              const BatchArgs& b, uint32_t ntiles, int bx, int by, hipStream_t s);

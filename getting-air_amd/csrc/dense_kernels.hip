@@ -66,6 +66,28 @@ __device__ __forceinline__ uint32_t row_off(int x) {
   return (uint32_t)x / kLatBlock * (Q * kLatBlock) + (uint32_t)x % kLatBlock;
 }
 
+// Signed distance of the pinned body at cell (x, y, z), binary32 with one
+// rounding per operation (no contraction), the same value k_classify and the
+// oracle compute:
+//   sphere  phi = sqrt((x-cx)^2 + (y-cy)^2 + (z-cz)^2) - a
+//           (common_d3_geometry.wgsl:119, :174-187)
+//   circle  nearest periodic image of (x-cx, y-cy), then length - a
+//           (common_geometry.wgsl:94-108)
+__device__ __forceinline__ float body_phi(int shape, float radius, float cx, float cy, float cz, int nx, int ny,
+                                          int x, int y, int z) {
+  float dx = sub_rn((float)x, cx);
+  float dy = sub_rn((float)y, cy);
+  if (shape == 2) {
+    const float W = (float)nx, H = (float)ny;
+    dx = sub_rn(dx, mul_rn(W, rintf(div_rn(dx, W))));
+    dy = sub_rn(dy, mul_rn(H, rintf(div_rn(dy, H))));
+    return sub_rn(sqrt_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy))), radius);
+  }
+  const float dz = sub_rn((float)z, cz);
+  const float s = add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz));
+  return sub_rn(sqrt_rn(s), radius);
+}
+
 // One step of tile b of a solve: every thread of the block pulls, collides
 // and stores its cell; a tile with a force slot leaves its FP64 force there.
 // `salt` seeds the dither of 16-bit stores; `mac` (or null) receives the
@@ -75,7 +97,12 @@ __device__ __forceinline__ uint32_t row_off(int x) {
 // per lane; otherwise each lane locates its sources' blocks itself.
 // RIGID (with GEN): a moving rigid body, a.rigid (its own instantiations, so
 // that the kernels without one keep their register budget).
-template <int Q, class ST, bool GEN, bool NT, bool ROW, bool RIGID = false>
+// DIFF (with GEN): the pinned body's diffuse penalty coupling in place of
+// bounce-back (common_d3_step.wgsl:288-300; the reference's default without
+// ?bounceback): every cell takes the Guo force F + rho chi (0 - u*), u* = m /
+// rho, chi = 1/2 (1 - tanh(clamp(phi / eps, -20, 20))), and a cell with chi >=
+// 1e-6 adds rho chi u* to the body's force (common_d3_force.wgsl:242-268).
+template <int Q, class ST, bool GEN, bool NT, bool ROW, bool RIGID = false, bool DIFF = false>
 __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout,
                                           const StepArgs& a, uint32_t b, uint32_t salt, float* mac) {
   using L = Lattice<Q>;
@@ -158,7 +185,8 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
 
     uint8_t c = 0;
     if constexpr (GEN) {
-      if (a.cls) c = a.cls[cell];
+      // under the diffuse coupling the body has no links and no solid cells
+      if (a.cls && !DIFF) c = a.cls[cell];
       const bool onFace = (a.wall_mask & 1 && (x == 0 || x == nx - 1)) ||
                           (a.wall_mask & 2 && (y == 0 || y == ny - 1)) ||
                           (a.wall_mask & 4 && (z == 0 || z == nz - 1));
@@ -247,7 +275,30 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
     }
     const float rho = 1.f + drho;
     const float rinv = 1.0f / fmaxf(rho, 1e-6f);
-    const float Fx = a.F[0], Fy = a.F[1], Fz = a.F[2];
+    float Fx = a.F[0], Fy = a.F[1], Fz = a.F[2];
+    if constexpr (DIFF) {
+      const float phi = body_phi(a.body_shape, a.body_r, a.body_cf[0], a.body_cf[1], a.body_cf[2], nx, ny, x, y, z);
+      const float chi = 0.5f * (1.f - tanhf(fminf(fmaxf(phi / a.chi_eps, -20.f), 20.f)));
+      const float usx = mx * rinv, usy = my * rinv, usz = mz * rinv;
+      const float rc = rho * chi;
+      Fx += rc * (0.f - usx);
+      Fy += rc * (0.f - usy);
+      Fz += rc * (0.f - usz);
+      if (a.force_slot && chi >= 1e-6f) {
+        // the body feels the penalty's negative, rho chi (u* - U_s)
+        const double fx = (double)rc * usx, fy = (double)rc * usy, fz = (double)rc * usz;
+        double rx = x - a.body_c[0], ry = y - a.body_c[1], rz = z - a.body_c[2];
+        if (a.periodic_arm) {
+          rx -= nx * rint(rx / nx);
+          ry -= ny * rint(ry / ny);
+          rz = 0.0;
+        }
+        fv[0] = fx; fv[1] = fy; fv[2] = fz;
+        fv[3] = ry * fz - rz * fy;
+        fv[4] = rz * fx - rx * fz;
+        fv[5] = rx * fy - ry * fx;
+      }
+    }
     // Guo: u = u* + F / (2 rho)
     const float ux = (mx + 0.5f * Fx) * rinv;
     const float uy = (my + 0.5f * Fy) * rinv;
@@ -343,10 +394,18 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
         const float t = rho * eu;
         const float A = w * (c0 + 4.5f * (t * eu));
         const float B = w * (3.f * t);
-        const float Se = a.c1[k] * eu - w * g3uF;
+        // Guo source, even and odd parts: (1 - omega/2) w (9 (e.u)(e.F) -
+        // 3 u.F) and (1 - omega_m/2) 3 w (e.F); a uniform F has them per pair
+        float c1k = a.c1[k], sok = a.so[k];
+        if constexpr (DIFF) {
+          const float eF = (float)L::ex[i] * Fx + (float)L::ey[i] * Fy + (float)L::ez[i] * Fz;
+          c1k = 9.f * a.guo * w * eF;
+          sok = (1.f - 0.5f * omm) * 3.f * w * eF;
+        }
+        const float Se = c1k * eu - w * g3uF;
         const float hS = 0.5f * S[k], hD = 0.5f * D[k];
         float P = hS - om * (hS - A) + Se;
-        float M = hD - omm * (hD - B) + a.so[k];
+        float M = hD - omm * (hD - B) + sok;
         if constexpr (GEN) {
           P += sp * (a.ae[k] - P);
           M += sp * (a.bo[k] - M);
@@ -374,7 +433,7 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
   }
 }
 
-template <int Q, class ST, bool GEN, bool NT, bool ROW, bool RIGID = false>
+template <int Q, class ST, bool GEN, bool NT, bool ROW, bool RIGID = false, bool DIFF = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q == 27 ? 6 : 8)))
 k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout, StepArgs a) {
   if constexpr (GEN) {
@@ -394,7 +453,7 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
     salt = *a.salt_in;
     if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *a.salt_out = salt + 1;
   }
-  step_tile<Q, ST, GEN, NT, ROW, RIGID>(gin, gout, a, b, salt, a.mac);
+  step_tile<Q, ST, GEN, NT, ROW, RIGID, DIFF>(gin, gout, a, b, salt, a.mac);
 }
 
 // Many solves in one launch (the serve loop's packed pool): block k steps
@@ -493,13 +552,8 @@ k_totals(const typename ST::T* __restrict__ g, TotalsArgs a) {
   partials_then_fixed_order_sum<4>(v, a.partials, a.counter, [&](int j, double s) { a.out[j] = s; });
 }
 
-// Per-cell class byte from the signed distance of the pinned body, evaluated
-// in f32 with one rounding per operation (no contraction) so the host oracle
-// reproduces it exactly:
-//   sphere  phi = sqrt((x-cx)^2 + (y-cy)^2 + (z-cz)^2) - a
-//           (common_d3_geometry.wgsl:119, :174-187)
-//   circle  nearest periodic image of (x-cx, y-cy), then length - a
-//           (common_geometry.wgsl:94-108)
+// Per-cell class byte from the signed distance of the pinned body (body_phi),
+// which the host oracle reproduces exactly.
 __global__ void __launch_bounds__(256) k_classify(ClassifyArgs a) {
   const uint32_t n = (uint32_t)a.nx * a.ny * a.nz;
   const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
@@ -507,22 +561,11 @@ __global__ void __launch_bounds__(256) k_classify(ClassifyArgs a) {
   const int x = cell % a.nx;
   const int y = (cell / a.nx) % a.ny;
   const int z = cell / (a.nx * a.ny);
-  float dx = sub_rn((float)x, a.cx);
-  float dy = sub_rn((float)y, a.cy);
-  float phi;
-  if (a.shape == 2) {
-    const float W = (float)a.nx, H = (float)a.ny;
-    dx = sub_rn(dx, mul_rn(W, rintf(div_rn(dx, W))));
-    dy = sub_rn(dy, mul_rn(H, rintf(div_rn(dy, H))));
-    phi = sub_rn(sqrt_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy))), a.radius);
-  } else {
-    const float dz = sub_rn((float)z, a.cz);
-    const float s = add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz));
-    phi = sub_rn(sqrt_rn(s), a.radius);
-  }
+  const float phi = body_phi(a.shape, a.radius, a.cx, a.cy, a.cz, a.nx, a.ny, x, y, z);
   uint8_t c = 0;
-  if (phi < 0.f) c |= 1;
-  if (phi < 2.f) c |= 2;
+  if (phi < 0.f) c |= kClsSolid;
+  if (phi < 2.f) c |= kClsNear;
+  if (phi < a.band) c |= kClsBand;
   a.cls[cell] = c;
 }
 
@@ -638,9 +681,18 @@ static void with_flags(bool general, bool nt, bool rows, F f) {
 
 template <int Q, class ST> struct OpStep {
   static hipError_t run(bool general, bool nt, int occ, const void* fin, void* fout, StepArgs a, int bx, int by,
-                        hipStream_t s) {
+                        hipStream_t s, bool diffuse) {
     using T = typename ST::T;
     const bool rows = step_row_mode(a.nx, a.linear != 0, bx, by);
+    if (diffuse) {
+      // the pinned body's diffuse coupling runs its own general instantiation
+      with_flags(true, nt, rows, [&](auto, auto N, auto R) {
+        constexpr auto k = k_step<Q, ST, true, decltype(N)::value, decltype(R)::value, false, true>;
+        hipLaunchKernelGGL(k, dim3(a.nblocks), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, (const T*)fin,
+                           (T*)fout, a);
+      });
+      return hipGetLastError();
+    }
     if constexpr (Q != 9) {
       if (a.rigid) {
         // a moving body runs the general kernel's rigid instantiation
@@ -696,8 +748,8 @@ template <int Q, class ST> struct OpTotals {
 };
 
 hipError_t launch_step(int q, int storage, bool general, bool nontemporal, int occ, const void* fin, void* fout,
-                       const StepArgs& a, int bx, int by, hipStream_t s) {
-  return dispatch<OpStep>(q, storage, general, nontemporal, occ, fin, fout, a, bx, by, s);
+                       const StepArgs& a, int bx, int by, hipStream_t s, bool diffuse) {
+  return dispatch<OpStep>(q, storage, general, nontemporal, occ, fin, fout, a, bx, by, s, diffuse);
 }
 hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, bool rows,
                              const BatchArgs& b, uint32_t ntiles, int bx, int by, hipStream_t s) {

@@ -67,9 +67,10 @@ const float kD2Q9WeightSumMinus1 =
 const int kShapes[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {256, 1}, {32, 4}, {32, 8}};
 
 struct TuneKey {
-  int q, storage, general, nx, ny, nz;
+  int q, storage, general, nx, ny, nz, diffuse;
   bool operator<(const TuneKey& o) const {
-    return std::tie(q, storage, general, nx, ny, nz) < std::tie(o.q, o.storage, o.general, o.nx, o.ny, o.nz);
+    return std::tie(q, storage, general, nx, ny, nz, diffuse) <
+           std::tie(o.q, o.storage, o.general, o.nx, o.ny, o.nz, o.diffuse);
   }
 };
 struct TuneVal { int bx, by, swz, nt, occ, lin; };
@@ -186,6 +187,9 @@ int validate(const lbm_desc* d) {
     if (d->q != 9 && sh != LBM_BODY_SPHERE) return fail(LBM_E_INVALID, "3D bodies must be spheres");
     if (!(d->body.radius > 0.f)) return fail(LBM_E_INVALID, "body radius %g", d->body.radius);
   }
+  if (d->diffuse && sh == LBM_BODY_NONE) return fail(LBM_E_INVALID, "diffuse coupling without a body");
+  if (d->diffuse != 0 && d->diffuse != 1) return fail(LBM_E_INVALID, "diffuse %d: 0 or 1", d->diffuse);
+  if (!std::isfinite(d->chi_eps)) return fail(LBM_E_INVALID, "chi_eps %g", d->chi_eps);
   for (int a = 0; a < 3; a++) {
     if (!std::isfinite(d->force[a]) || !std::isfinite(d->sponge_u[a]))
       return fail(LBM_E_INVALID, "non-finite force or sponge velocity");
@@ -212,6 +216,9 @@ void pair_constants(StepArgs& a, const lbm_desc& d) {
   }
   a.ae[13] = L::w[0] * (-1.5f * uu);
 }
+
+// CHI_EPS of the diffuse coupling (main-3d.js:1026 default 1.5).
+float chi_eps(const lbm_desc& d) { return d.chi_eps > 0.f ? d.chi_eps : 1.5f; }
 
 StepArgs step_args(lbm_solve* s, bool write_mac, bool with_force) {
   StepArgs a{};
@@ -252,6 +259,12 @@ StepArgs step_args(lbm_solve* s, bool write_mac, bool with_force) {
   a.body_c[1] = d.body.centre[1];
   a.body_c[2] = s->dim == 3 ? d.body.centre[2] : 0.0;
   a.periodic_arm = s->dim == 2;
+  a.body_shape = d.body.shape == LBM_BODY_CIRCLE ? 2 : 1;
+  a.body_r = d.body.radius;
+  a.body_cf[0] = d.body.centre[0];
+  a.body_cf[1] = d.body.centre[1];
+  a.body_cf[2] = s->dim == 3 ? d.body.centre[2] : 0.f;
+  a.chi_eps = chi_eps(d);
   if (s->lin) {
     const uint32_t t = (uint32_t)(s->bx * s->by);
     a.linear = 1;
@@ -321,7 +334,8 @@ int enqueue_step(lbm_solve* s, bool write_mac, bool finalize_prev) {
   }
   StepArgs a = step_args(s, write_mac, true);
   a.finalize_prev = finalize_prev && a.force_slot ? 1 : 0;
-  hipError_t e = launch_step(s->q, s->storage, s->general, s->nt, s->occ, fin, fout, a, s->bx, s->by, s->stream);
+  hipError_t e = launch_step(s->q, s->storage, s->general, s->nt, s->occ, fin, fout, a, s->bx, s->by, s->stream,
+                             s->d.diffuse != 0);
   if (e != hipSuccess) return hip_fail(e, "step kernel launch");
   s->force_pending = a.force_slot != nullptr;
   s->parity ^= 1;
@@ -434,7 +448,7 @@ int prepare_graphs(lbm_solve* s) {
 // keeps the fastest of those: one short timing per candidate is too noisy to
 // separate shapes within a few percent.
 int autotune(lbm_solve* s) {
-  TuneKey key{s->q, s->storage, s->general ? 1 : 0, s->d.nx, s->d.ny, s->d.nz};
+  TuneKey key{s->q, s->storage, s->general ? 1 : 0, s->d.nx, s->d.ny, s->d.nz, s->d.diffuse};
   TuneVal bestv{s->bx, s->by, s->swz, s->nt, s->occ, s->lin};
   bool cached = false;
   {
@@ -452,10 +466,11 @@ int autotune(lbm_solve* s) {
     auto time_of = [&](const TuneVal& v, int n, float* ms) -> int {
       s->bx = v.bx; s->by = v.by; s->swz = v.swz; s->nt = v.nt; s->occ = v.occ; s->lin = v.lin;
       StepArgs a = step_args(s, false, false);
-      HIPCHK(launch_step(s->q, s->storage, s->general, v.nt, v.occ, fin, fout, a, v.bx, v.by, s->stream));
+      const bool diff = s->d.diffuse != 0;
+      HIPCHK(launch_step(s->q, s->storage, s->general, v.nt, v.occ, fin, fout, a, v.bx, v.by, s->stream, diff));
       HIPCHK(hipEventRecord(s->ev0, s->stream));
       for (int r = 0; r < n; r++)
-        HIPCHK(launch_step(s->q, s->storage, s->general, v.nt, v.occ, fin, fout, a, v.bx, v.by, s->stream));
+        HIPCHK(launch_step(s->q, s->storage, s->general, v.nt, v.occ, fin, fout, a, v.bx, v.by, s->stream, diff));
       HIPCHK(hipEventRecord(s->ev1, s->stream));
       HIPCHK(hipEventSynchronize(s->ev1));
       HIPCHK(hipEventElapsedTime(ms, s->ev0, s->ev1));
@@ -509,6 +524,9 @@ int build_boundary(lbm_solve* s) {
   c.shape = d.body.shape == LBM_BODY_CIRCLE ? 2 : 1;
   c.radius = d.body.radius;
   c.cx = d.body.centre[0]; c.cy = d.body.centre[1]; c.cz = s->dim == 3 ? d.body.centre[2] : 0.f;
+  // the diffuse coupling's force cells: chi >= 1e-6 needs phi / eps <
+  // atanh(1 - 2e-6) = 6.9; the band is wider, the kernel tests chi itself
+  c.band = d.diffuse ? 7.5f * chi_eps(d) : -1e30f;
   c.cls = s->cls;
   c.links = s->links;
   HIPCHK(launch_classify(c, s->stream));
@@ -523,6 +541,10 @@ int build_boundary(lbm_solve* s) {
       for (int x = 0; x < nx; x++) {
         const uint32_t cell = (uint32_t)((z * ny + y) * nx + x);
         const uint8_t cc = h[cell];
+        if (d.diffuse) {
+          if (cc & kClsBand) s->boundary.push_back(cell);
+          continue;
+        }
         if (!(cc & 2) || (cc & 1)) continue;
         for (int i = 1; i < s->q; i++) {
           const int ex = s->q == 9 ? kEx9[i] : kEx3[i];
@@ -1046,10 +1068,12 @@ int lbm_time_step_kernel(lbm_solve* s, int32_t n, double* ms_per_launch) {
   const void* fin = s->f[s->parity];
   void* fout = s->f[1 - s->parity];
   StepArgs a = step_args(s, false, true);
-  HIPCHK(launch_step(s->q, s->storage, s->general, s->nt, s->occ, fin, fout, a, s->bx, s->by, s->stream));
+  HIPCHK(launch_step(s->q, s->storage, s->general, s->nt, s->occ, fin, fout, a, s->bx, s->by, s->stream,
+                     s->d.diffuse != 0));
   HIPCHK(hipEventRecord(s->ev0, s->stream));
   for (int r = 0; r < n; r++)
-    HIPCHK(launch_step(s->q, s->storage, s->general, s->nt, s->occ, fin, fout, a, s->bx, s->by, s->stream));
+    HIPCHK(launch_step(s->q, s->storage, s->general, s->nt, s->occ, fin, fout, a, s->bx, s->by, s->stream,
+                     s->d.diffuse != 0));
   HIPCHK(hipEventRecord(s->ev1, s->stream));
   HIPCHK(hipEventSynchronize(s->ev1));
   float t = 0.f;

@@ -672,6 +672,7 @@ int lbm_serve_submit_state(lbm_serve* sv, const lbm_desc* desc, const float* mac
   if (d.abi_version != LBM_ABI_VERSION) return fail(LBM_E_INVALID, "descriptor abi_version %d", d.abi_version);
   if (d.q != 9 && d.q != 19 && d.q != 27) return fail(LBM_E_INVALID, "q must be 9, 19 or 27 (got %d)", d.q);
   if (d.nx < 1 || d.ny < 1 || d.nz < 1) return fail(LBM_E_INVALID, "grid %dx%dx%d", d.nx, d.ny, d.nz);
+  if (d.diffuse) return fail(LBM_E_INVALID, "the serve loop runs the bounce-back coupling (diffuse = 0)");
   auto r = std::make_shared<Request>();
   r->d = d;
   r->steps = steps;

@@ -48,6 +48,7 @@ class Desc(ctypes.Structure):
         ("force", ctypes.c_float * 3), ("walls", ctypes.c_int32 * 3), ("wall_u", ctypes.c_float * 2),
         ("sponge_width", ctypes.c_float), ("sponge_axes", ctypes.c_int32 * 3), ("sponge_u", ctypes.c_float * 3),
         ("body", Body), ("solid_eq", ctypes.c_int32), ("storage", ctypes.c_int32), ("autotune", ctypes.c_int32),
+        ("diffuse", ctypes.c_int32), ("chi_eps", ctypes.c_float),
     ]
 
 
@@ -223,7 +224,8 @@ def _check(rc):
 def make_desc(d):
     """Builds a Desc from the oracle-style dict used throughout the tests:
     {q, nx, ny, nz, omega, omegaMinus, force, walls, wallU,
-     sponge: {width, axes, u}, body: {a, centre}, solidEq, storage, autotune}."""
+     sponge: {width, axes, u}, body: {a, centre}, solidEq, storage, autotune,
+     diffuse, chiEps}."""
     q = int(d["q"])
     desc = Desc()
     desc.abi_version = ABI_VERSION
@@ -256,6 +258,8 @@ def make_desc(d):
     desc.solid_eq = int(d.get("solidEq", 1))
     desc.storage = STORE[d.get("storage", "f32")]
     desc.autotune = int(d.get("autotune", 0))
+    desc.diffuse = int(bool(d.get("diffuse", 0)))
+    desc.chi_eps = float(d.get("chiEps", 0.0))
     return desc
 
 

@@ -69,6 +69,17 @@ typedef struct lbm_desc {
   int32_t solid_eq;     /* SOLID_EQ: hold the body interior at feq(1, 0)      */
   int32_t storage;      /* LBM_STORE_*                                        */
   int32_t autotune;     /* 1: time candidate tile shapes on first step        */
+  int32_t diffuse;      /* the body's coupling.  0: halfway bounce-back with
+                         * momentum exchange (?bounceback, USE_BOUNCEBACK).
+                         * 1: the diffuse penalty coupling, the reference's
+                         * default (common_d3_step.wgsl:288-300): no bounce-
+                         * back, every cell takes the Guo force rho chi
+                         * (0 - u*), u* = m / rho, chi = 1/2 (1 - tanh(phi /
+                         * chi_eps)), and the body feels rho chi u* summed
+                         * over the cells with chi >= 1e-6
+                         * (common_d3_force.wgsl:242-268).  SOLID_EQ does not
+                         * apply.  Standalone solves (not the serve loop).   */
+  float chi_eps;        /* CHI_EPS (main-3d.js:1026); <= 0: 1.5               */
 } lbm_desc;
 
 typedef struct lbm_solve lbm_solve;

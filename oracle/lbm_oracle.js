@@ -142,6 +142,10 @@ function normalise(d) {
     },
     body: d.body ? { a: F32(d.body.a), centre: d.body.centre.map(F32) } : null,
     solidEq: d.solidEq === undefined ? 1 : d.solidEq,
+    // the diffuse penalty coupling (the reference's default: no ?bounceback,
+    // common_d3_step.wgsl:288-300; CHI_EPS main-3d.js:1026)
+    diffuse: !!d.diffuse,
+    chiEps: F32(d.chiEps === undefined || !(d.chiEps > 0) ? 1.5 : d.chiEps),
     storage: d.storage || 'f32',
     exactWeights: !!d.exactWeights,
   };
@@ -171,6 +175,14 @@ const hooks = { spongeWeight3: spongeWeight3 };
 // solid), us(x, y, z) (body velocity at the cell), rhoWLocal, winCoord(x, y,
 // z) (sponge coordinates) } (common_d3_step.wgsl:224-400 with HAS_BODY and
 // USE_BOUNCEBACK).
+// The diffuse coupling's solid fraction (chiFromPhiEps3,
+// common_d3_geometry.wgsl:251; chiFromPhiEps, common_geometry.wgsl):
+// 1/2 (1 - tanh(clamp(phi / eps, -20, 20))).
+function chiOf(phi, eps) {
+  const t = Math.max(-20, Math.min(20, phi / eps));
+  return 0.5 * (1 - Math.tanh(t));
+}
+
 function stepSlab(S, T, cls, fin, fout, mac, r0, r1, R) {
   const Q = T.Q, NX = S.nx, NY = S.ny, NZ = S.nz, N = NX * NY * NZ;
   const ex = T.ex, ey = T.ey, ez = T.ez, w = T.w, opp = T.opp;
@@ -181,7 +193,10 @@ function stepSlab(S, T, cls, fin, fout, mac, r0, r1, R) {
   const sw = S.sponge.width, sax = S.sponge.axes, su = S.sponge.u;
   const uFarSq = su[0] * su[0] + su[1] * su[1] + su[2] * su[2];
   const hasBody = !!S.body || !!R;
-  const solidEq = hasBody && S.solidEq;
+  const diffuse = S.diffuse && !!S.body && !R;
+  // under the diffuse coupling there is no bounce-back and no SOLID_EQ
+  // (common_d3_step.wgsl:244-255, :388)
+  const solidEq = hasBody && S.solidEq && !diffuse;
   const f = new Float64Array(Q);
   const fo = new Float64Array(Q);
   const fTarget = new Float64Array(Q);
@@ -214,7 +229,7 @@ function stepSlab(S, T, cls, fin, fout, mac, r0, r1, R) {
         for (let i = 0; i < Q; i++) {
           const sx = x - ex[i];
           const src = rowBase[i] + (sx < 0 ? sx + NX : (sx >= NX ? sx - NX : sx));
-          if (R ? ((lm >>> i) & 1) : (c !== 0 && (c & CLS_NEAR) && (cls[src] & CLS_SOLID))) {
+          if (R ? ((lm >>> i) & 1) : (!diffuse && c !== 0 && (c & CLS_NEAR) && (cls[src] & CLS_SOLID))) {
             // Ladd's moving-wall term 2 w rho_w (e . u_body) / cs2
             const corr = R ? 2 * w[i] * rhoW * (ex[i] * us[0] + ey[i] * us[1] + ez[i] * us[2]) * 3 : 0;
             f[i] = fin[opp[i] * N + cell] + corr;
@@ -236,8 +251,17 @@ function stepSlab(S, T, cls, fin, fout, mac, r0, r1, R) {
           rho += f[i]; mx += f[i] * ex[i]; my += f[i] * ey[i]; mz += f[i] * ez[i];
         }
         const rhoDen = Math.max(rho, 1e-6);
-        // 3. Guo forcing with the uniform body force (bounce-back: chi = 0).
-        const Fx = Fx0, Fy = Fy0, Fz = Fz0;
+        // 3. Guo forcing: the uniform body force, plus under the diffuse
+        // coupling the penalty rho chi (U_s - u*) toward the body's velocity
+        // (zero for a pinned body), u* = m / rho (common_d3_step.wgsl:288-300);
+        // under bounce-back chi = 0.
+        let Fx = Fx0, Fy = Fy0, Fz = Fz0;
+        if (diffuse) {
+          const chi = chiOf(phiF32(S, x, y, z), S.chiEps);
+          Fx += rho * chi * (0 - mx / rhoDen);
+          Fy += rho * chi * (0 - my / rhoDen);
+          Fz += rho * chi * (0 - mz / rhoDen);
+        }
         const ux = mx / rhoDen + Fx / (2 * rhoDen);
         const uy = my / rhoDen + Fy / (2 * rhoDen);
         const uz = mz / rhoDen + Fz / (2 * rhoDen);
@@ -305,6 +329,7 @@ function stepSlab(S, T, cls, fin, fout, mac, r0, r1, R) {
 function forceSlab(S, T, cls, fin, r0, r1) {
   const out = [0, 0, 0, 0, 0, 0];
   if (!S.body) return out;
+  if (S.diffuse) return diffuseForceSlab(S, T, fin, r0, r1);
   const Q = T.Q, NX = S.nx, NY = S.ny, NZ = S.nz, N = NX * NY * NZ;
   const cx = S.body.centre[0], cy = S.body.centre[1], cz = S.dim === 3 ? S.body.centre[2] : 0;
   for (let row = r0; row < r1; row++) {
@@ -335,6 +360,46 @@ function forceSlab(S, T, cls, fin, r0, r1) {
         out[4] += rz * fx - rx * fz;
         out[5] += rx * fy - ry * fx;
       }
+    }
+  }
+  return out;
+}
+
+// Force and torque of the diffuse coupling on the pinned body, from the
+// pre-stream populations: every cell with chi >= 1e-6 gathers rho and
+// u* = m / rho from its periodic pull sources (as the step does) and the body
+// feels the negative of the penalty, rho chi (u* - U_s), U_s = 0
+// (common_d3_force.wgsl:242-268).
+function diffuseForceSlab(S, T, fin, r0, r1) {
+  const out = [0, 0, 0, 0, 0, 0];
+  const Q = T.Q, NX = S.nx, NY = S.ny, NZ = S.nz, N = NX * NY * NZ;
+  const cx = S.body.centre[0], cy = S.body.centre[1], cz = S.dim === 3 ? S.body.centre[2] : 0;
+  for (let row = r0; row < r1; row++) {
+    const z = Math.floor(row / NY), y = row - z * NY;
+    for (let x = 0; x < NX; x++) {
+      const chi = chiOf(phiF32(S, x, y, z), S.chiEps);
+      if (!(chi >= 1e-6)) continue;
+      let rho = 0, mx = 0, my = 0, mz = 0;
+      for (let i = 0; i < Q; i++) {
+        let sx = x - T.ex[i], sy = y - T.ey[i], sz = z - T.ez[i];
+        sx = sx < 0 ? sx + NX : (sx >= NX ? sx - NX : sx);
+        sy = sy < 0 ? sy + NY : (sy >= NY ? sy - NY : sy);
+        sz = sz < 0 ? sz + NZ : (sz >= NZ ? sz - NZ : sz);
+        const fi = fin[i * N + (sz * NY + sy) * NX + sx];
+        rho += fi; mx += fi * T.ex[i]; my += fi * T.ey[i]; mz += fi * T.ez[i];
+      }
+      const den = Math.max(rho, 1e-6);
+      const fx = rho * chi * (mx / den), fy = rho * chi * (my / den), fz = rho * chi * (mz / den);
+      let rx = x - cx, ry = y - cy;
+      const rz = z - cz;
+      if (S.dim === 2) {
+        rx -= NX * roundHalfEven(rx / NX);
+        ry -= NY * roundHalfEven(ry / NY);
+      }
+      out[0] += fx; out[1] += fy; out[2] += fz;
+      out[3] += ry * fz - rz * fy;
+      out[4] += rz * fx - rx * fz;
+      out[5] += rx * fy - ry * fx;
     }
   }
   return out;
@@ -396,6 +461,6 @@ class DenseOracle {
 }
 
 module.exports = {
-  tables, classify, phiF32, normalise, stepSlab, forceSlab, DenseOracle,
+  tables, classify, phiF32, normalise, stepSlab, forceSlab, DenseOracle, chiOf,
   CLS_SOLID, CLS_NEAR, roundHalfEven, spongeWeight3, hooks,
 };

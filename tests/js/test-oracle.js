@@ -312,4 +312,46 @@ for (const q of [19, 27]) {
   });
 }
 
+// The diffuse coupling's momentum budget.  In a periodic box without walls,
+// streaming conserves momentum and Guo's forcing adds exactly the cell's
+// force F = F0 + rho chi (U_s - u*) to it (BGK and TRT alike), so the fluid's
+// momentum change per step is N F0 minus the force the body feels.  On a
+// 12^3 box around a radius-2 sphere every cell has chi >= 1e-6, so the force
+// record covers every penalised cell and the budget closes to round-off.
+// A force taken from the post-forcing velocity u instead of u*, a dropped
+// rho, or the wrong sign each break it.
+for (const trt of [false, true]) {
+  ok('diffuse coupling: the force record closes the momentum budget (D3Q19, ' + (trt ? 'TRT' : 'BGK') + ')', () => {
+    const n = 12, N = n * n * n, F0 = [Math.fround(2e-5), Math.fround(-1e-5), 0];
+    const om = Math.fround(1 / 0.6);
+    // exact weights: with the f32 weights the equilibrium's momentum is
+    // rho u (1 + ~1e-8), a drift of its own that would hide in the budget
+    const desc = { q: 19, nx: n, ny: n, nz: n, omega: om, force: F0, diffuse: 1, chiEps: 1.5, exactWeights: true,
+                   body: { a: 2, centre: [6.3, 5.8, 6.1] } };
+    if (trt) desc.omegaMinus = Math.fround(1 / (0.5 + (3 / 16) / (1 / om - 0.5)));
+    const o = new O.DenseOracle(desc);
+    let minChi = 1;
+    for (let z = 0; z < n; z++) for (let y = 0; y < n; y++) for (let x = 0; x < n; x++) {
+      minChi = Math.min(minChi, O.chiOf(O.phiF32(o.S, x, y, z), o.S.chiEps));
+    }
+    assert(minChi >= 1e-6, 'a cell has chi < 1e-6: ' + minChi);
+    o.initEq(uniformMac(N, 1, [0.03, 0.01, -0.005]));
+    let worst = 0, scale = 0;
+    for (let s = 0; s < 20; s++) {
+      const p0 = o.totals();
+      o.step(1);
+      const p1 = o.totals();
+      const rec = o.forceHistory[o.forceHistory.length - 1];
+      for (let k = 0; k < 3; k++) {
+        const budget = N * F0[k] - rec[k];
+        worst = Math.max(worst, Math.abs((p1[k + 1] - p0[k + 1]) - budget));
+        scale = Math.max(scale, Math.abs(rec[k]));
+      }
+    }
+    assert(scale > 0.05, 'force too small to test: ' + scale);
+    assert(worst < 1e-10 * scale, 'budget off by ' + worst + ' (force ' + scale + ')');
+    return 'off by ' + worst.toExponential(1) + ' of a force ' + scale.toFixed(3);
+  });
+}
+
 process.exitCode = failures ? 1 : 0;

@@ -151,6 +151,52 @@ def test_circle_cylinder_d2q9_force():
     assert dev.max() < 1e-6, dev
 
 
+@pytest.mark.parametrize("q,trt", [(19, True), (27, False), (9, True)])
+def test_diffuse_coupling_matches_the_oracle(q, trt):
+    """The diffuse penalty coupling (the reference's default, no ?bounceback:
+    common_d3_step.wgsl:288-300, force common_d3_force.wgsl:242-268) on a
+    pinned sphere (circle for D2Q9) in a freestream with the sponge: rho/u
+    within 1e-5 of the oracle after the run and every step's force record
+    within 1e-6 of the largest drag."""
+    if q == 9:
+        desc = {"q": 9, "nx": 128, "ny": 48, "nz": 1, "omega": sc.f32(1 / 0.56),
+                "sponge": {"width": 8, "u": [0.04, 0, 0]}, "body": {"a": 5.0, "centre": [32.0, 24.3]}}
+        n = 128 * 48
+        mac = np.tile(np.array([1 - 0.04 ** 2 * 1.5, 0.04, 0, 0], np.float32), n)
+        if trt:
+            desc["omegaMinus"] = sc.trt_omega_minus(desc["omega"])
+    else:
+        desc, mac, _ = sphere_small(trt=trt)
+        desc["q"] = q
+    desc = dict(desc, diffuse=1, chiEps=1.5)
+    steps = 300
+    print("case diffuse q=%d trt=%s" % (q, trt), flush=True)
+    g = run_gpu(desc, mac, steps)
+    r = oracle_runner.run(desc, mac, steps=steps, threads=4, want=("mac", "force"))
+    e = field_errors(g["mac"], r["mac"])
+    dev = force_dev(g["force"], r["force"])
+    print(e, "force dev", dev, "last drag", g["force"][-1, 0], r["force"][-1, 0])
+    assert g["force"].shape == r["force"].shape == (steps, 6)
+    assert e["rho"] < 1e-5 and e["u"] < 1e-5, e
+    assert dev.max() < 1e-6, dev
+    # the penalty damps the flow inside the body: the centre is nearly at rest
+    m = g["mac"].reshape(-1, 4)
+    c = desc["body"]["centre"]
+    nx, ny = desc["nx"], desc["ny"]
+    ic = (int(round(c[2])) * ny + int(round(c[1]))) * nx + int(round(c[0])) if q != 9 else \
+        int(round(c[1])) * nx + int(round(c[0]))
+    assert abs(m[ic, 1]) < 0.2 * 0.04
+
+
+def test_diffuse_coupling_refusals():
+    desc, mac, _ = sphere_small()
+    with pytest.raises(pylbm.LbmError):
+        pylbm.Solve(dict({k: v for k, v in desc.items() if k != "body"}, diffuse=1))
+    with pylbm.Server(budget_bytes=1 << 28) as sv:
+        with pytest.raises(pylbm.LbmError):
+            sv.submit(dict(desc, diffuse=1), mac, 4)
+
+
 @pytest.mark.parametrize("storage,tol_u,tol_f", [("f16", 3e-2, 5e-3), ("bf16", 2e-1, 5e-2)])
 def test_shifted_16bit_storage_error_vs_fp64(storage, tol_u, tol_f):
     """Shifted 16-bit storage (f - w_i) with FP32 collision, scored against the
