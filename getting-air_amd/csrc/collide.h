@@ -20,6 +20,28 @@ struct CollideParams {
   float so[13];               // (1 - omega_m/2) 3 w (e.F) per pair
 };
 
+// e_i . (x, y, z) for a direction known at compile time: the zero components
+// are left out and the sum starts from -0, the identity of floating-point
+// addition, so an axis direction costs no instruction and a diagonal one add.
+template <int Q>
+__device__ __forceinline__ float edot(int i, float x, float y, float z) {
+  using L = Lattice<Q>;
+  float s = -0.f;
+  if (L::ex[i] == 1) s += x; else if (L::ex[i] == -1) s -= x;
+  if (L::ey[i] == 1) s += y; else if (L::ey[i] == -1) s -= y;
+  if (L::ez[i] == 1) s += z; else if (L::ez[i] == -1) s -= z;
+  return s;
+}
+
+// 1 / x for a normal x: the hardware reciprocal (1 ulp) refined by one
+// Newton step, r + r (1 - x r), whose error is a small fraction of an ulp
+// and of either sign, so it leaves no systematic bias in rho u / rho.  A
+// correctly rounded division costs about three times the instructions.
+__device__ __forceinline__ float recip_nr(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(__builtin_fmaf(-x, r, 1.f), r, r);
+}
+
 // Moments of g: rho - 1, and rho u with the Guo half-force added.
 template <int Q>
 __device__ __forceinline__ void moments(const float (&g)[Q], const float* F, float& drho, float& rho, float& ux,

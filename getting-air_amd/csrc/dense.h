@@ -58,12 +58,13 @@ struct StepArgs {
   float wsum_m1;     // (sum of the f32 weights) - 1: 0 for D3Q19/D3Q27, ~1e-8 for D2Q9
   float F[3];        // uniform Guo body force
   // Collision constants, uniform over the grid (computed on the host).
-  // Pair k couples direction i = lo(k) with opp(i); see k_step.
+  // Pair k couples direction i = lo(k) with opp(i); see k_step.  The pair
+  // constants are doubled, as the kernel's doubled pair form uses them.
   float omega_m;     // odd rate (TRT omega_minus; = omega for BGK)
-  float c1[13];      // 9 (1 - omega/2) w_i (e_i . F)
-  float so[13];      // (1 - omega_m/2) 3 w_i (e_i . F): odd Guo source
-  float ae[14];      // sponge target, even part w_i (4.5 (e_i.U)^2 - 1.5 U.U); [13] = rest
-  float bo[13];      // sponge target, odd part 3 w_i (e_i . U)
+  float c1[13];      // 2 x 9 (1 - omega/2) w_i (e_i . F)
+  float so[13];      // 2 x (1 - omega_m/2) 3 w_i (e_i . F): odd Guo source
+  float ae[14];      // sponge target, even part 2 x w_i (4.5 (e_i.U)^2 - 1.5 U.U); [13] = rest, not doubled
+  float bo[13];      // sponge target, odd part 2 x 3 w_i (e_i . U)
   int32_t wall_mask; // bit a: halfway walls on the two faces normal to axis a
   float wall_u0, wall_u1;
   float sponge_w;    // <= 0 disables
@@ -179,8 +180,9 @@ constexpr uint8_t kClsSolid = 1, kClsNear = 2, kClsBand = 4;
 // `diffuse`: the pinned body's diffuse penalty coupling (with `general`).
 hipError_t launch_step(int q, int storage, bool general, bool nontemporal, int occ, const void* fin, void* fout,
                        const StepArgs& a, int bx, int by, hipStream_t s, bool diffuse = false);
-// `rows`: step_row_mode holds for every solve of the batch.
-hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, bool rows,
+// `rows`: step_row_mode holds for every solve of the batch; `force`: some
+// solve of the batch has a nonzero body force.
+hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, bool rows, bool force,
                              const BatchArgs& b, uint32_t ntiles, int bx, int by, hipStream_t s);
 // Whether `occ` waves per SIMD can be set exactly for blocks of `threads`.
 bool occupancy_cap_ok(int occ, int threads);

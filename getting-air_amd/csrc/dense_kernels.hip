@@ -24,6 +24,7 @@
 // to a wall or the body then patch their reflected directions.
 #include <hip/hip_runtime.h>
 #include <type_traits>
+#include "collide.h"
 #include "dense.h"
 #include "device_common.h"
 #include "lattice_pairs.h"
@@ -63,7 +64,8 @@ __device__ __forceinline__ T* block_of(T* g, uint32_t c) {
 // (rows are whole layout blocks).
 template <int Q>
 __device__ __forceinline__ uint32_t row_off(int x) {
-  return (uint32_t)x / kLatBlock * (Q * kLatBlock) + (uint32_t)x % kLatBlock;
+  // a 24-bit multiply is full rate (a 32-bit one a quarter); x < 2^24
+  return (uint32_t)__umul24((uint32_t)x / kLatBlock, Q * kLatBlock) + (uint32_t)x % kLatBlock;
 }
 
 // Signed distance of the pinned body at cell (x, y, z), binary32 with one
@@ -102,9 +104,10 @@ __device__ __forceinline__ float body_phi(int shape, float radius, float cx, flo
 // ?bounceback): every cell takes the Guo force F + rho chi (0 - u*), u* = m /
 // rho, chi = 1/2 (1 - tanh(clamp(phi / eps, -20, 20))), and a cell with chi >=
 // 1e-6 adds rho chi u* to the body's force (common_d3_force.wgsl:242-268).
-template <int Q, class ST, bool GEN, bool NT, bool ROW, bool RIGID = false, bool DIFF = false>
+template <int Q, class ST, bool GEN, bool NT, bool ROW, bool RIGID = false, bool DIFF = false, bool FRC = true>
 __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout,
                                           const StepArgs& a, uint32_t b, uint32_t salt, float* mac) {
+  static_assert(!DIFF || FRC, "the diffuse coupling forces every cell");
   using L = Lattice<Q>;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   int x, y, z;
@@ -255,7 +258,9 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
       }
     }
 
-    // 2. pair sums and differences; moments from them
+    // 2. pair sums and differences; moments from them.  The momentum sums
+    // start from -0, the identity of floating-point addition, so that their
+    // first term costs no instruction (0 + x is not x when x is -0).
     constexpr int NP = Pairs<Q>::NP;
     float S[NP], D[NP];
 #pragma unroll
@@ -264,7 +269,8 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
       S[k] = g[i] + g[L::opp[i]];
       D[k] = g[i] - g[L::opp[i]];
     }
-    float drho = g[0] + a.wsum_m1, mx = 0.f, my = 0.f, mz = 0.f;
+    // D2Q9's f32 weights do not sum to 1 (wsum_m1); the 3D sets' do exactly
+    float drho = Q == 9 ? g[0] + a.wsum_m1 : g[0], mx = -0.f, my = -0.f, mz = -0.f;
 #pragma unroll
     for (int k = 0; k < NP; k++) {
       const int i = Pairs<Q>::lo(k);
@@ -274,8 +280,9 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
       if (L::ez[i] == 1) mz += D[k]; else if (L::ez[i] == -1) mz -= D[k];
     }
     const float rho = 1.f + drho;
-    const float rinv = 1.0f / fmaxf(rho, 1e-6f);
-    float Fx = a.F[0], Fy = a.F[1], Fz = a.F[2];
+    const float rinv = recip_nr(fmaxf(rho, 1e-6f));
+    // FRC false: no body force (F = 0), so every Guo term vanishes
+    float Fx = FRC ? a.F[0] : 0.f, Fy = FRC ? a.F[1] : 0.f, Fz = FRC ? a.F[2] : 0.f;
     if constexpr (DIFF) {
       const float phi = body_phi(a.body_shape, a.body_r, a.body_cf[0], a.body_cf[1], a.body_cf[2], nx, ny, x, y, z);
       const float chi = 0.5f * (1.f - tanhf(fminf(fmaxf(phi / a.chi_eps, -20.f), 20.f)));
@@ -300,11 +307,11 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
       }
     }
     // Guo: u = u* + F / (2 rho)
-    const float ux = (mx + 0.5f * Fx) * rinv;
-    const float uy = (my + 0.5f * Fy) * rinv;
-    const float uz = (mz + 0.5f * Fz) * rinv;
+    const float ux = FRC ? (mx + 0.5f * Fx) * rinv : mx * rinv;
+    const float uy = FRC ? (my + 0.5f * Fy) * rinv : my * rinv;
+    const float uz = FRC ? (mz + 0.5f * Fz) * rinv : mz * rinv;
     const float usq = ux * ux + uy * uy + uz * uz;
-    const float uF = ux * Fx + uy * Fy + uz * Fz;
+    const float uF = FRC ? ux * Fx + uy * Fy + uz * Fz : 0.f;
 
     bool solid = false;
     if constexpr (GEN) solid = a.solid_eq && (c & 1);
@@ -368,6 +375,13 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
       // regrouped (omega_m = omega gives BGK).  The sponge blends P and M
       // toward the even and odd parts of feq(1, U_far) - w.
       //
+      // The kernel evaluates it doubled: P2 = 2P and M2 = 2M from S = 2 g+
+      // and D = 2 g- with 2w in place of w (and the host's pair constants
+      // doubled), then g_i' = fma(1/2, P2, M2/2), g_j' = fma(1/2, P2, -M2/2).
+      // Scaling by two commutes with rounding, so every intermediate is
+      // exactly twice its undoubled value and the outputs are bit for bit
+      // P + M and P - M, without the two halvings per pair.
+      //
       // Rounding: every relaxation keeps the form h - omega (h - eq), and the
       // exact f32 weight w is the LAST factor of A and B.  A pre-rounded
       // constant such as f32(3 w) or f32((1 - omega_m)/2) would carry the
@@ -378,41 +392,51 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
       const float c0 = drho - 1.5f * rho * usq;
       // random bits of this cell and step for the 16-bit stores
       const uint32_t hsh = ST::kBytes == 2 ? cell_hash(cell, salt) : 0u;
-      const float g3uF = 3.f * a.guo * uF;
-      {
-        const float w = L::w[0];
-        float p = g[0] - om * (g[0] - w * c0) - w * g3uF;
-        if constexpr (GEN) p += sp * (a.ae[13] - p);
-        st_sr<ST, NT>(out_self, self_lo, p, ST::dither(hsh, 0));
-      }
+      const float g3uF2 = FRC ? 6.f * a.guo * uF : 0.f;  // 2 (3 (1 - omega/2) u.F)
+      // The sponge blend only where a lane of the wave lies in the band.
+      auto collide_pairs = [&](auto SPONGE) {
+        constexpr bool SP = decltype(SPONGE)::value;
+        {
+          const float w = L::w[0];
+          float p = g[0] - om * (g[0] - w * c0);
+          if constexpr (FRC) p -= w * (0.5f * g3uF2);
+          if constexpr (SP) p += sp * (a.ae[13] - p);
+          st_sr<ST, NT>(out_self, self_lo, p, ST::dither(hsh, 0));
+        }
 #pragma unroll
-      for (int k = 0; k < NP; k++) {
-        const int i = Pairs<Q>::lo(k);
-        const int j = L::opp[i];
-        const float w = L::w[i];
-        const float eu = (float)L::ex[i] * ux + (float)L::ey[i] * uy + (float)L::ez[i] * uz;
-        const float t = rho * eu;
-        const float A = w * (c0 + 4.5f * (t * eu));
-        const float B = w * (3.f * t);
-        // Guo source, even and odd parts: (1 - omega/2) w (9 (e.u)(e.F) -
-        // 3 u.F) and (1 - omega_m/2) 3 w (e.F); a uniform F has them per pair
-        float c1k = a.c1[k], sok = a.so[k];
-        if constexpr (DIFF) {
-          const float eF = (float)L::ex[i] * Fx + (float)L::ey[i] * Fy + (float)L::ez[i] * Fz;
-          c1k = 9.f * a.guo * w * eF;
-          sok = (1.f - 0.5f * omm) * 3.f * w * eF;
+        for (int k = 0; k < NP; k++) {
+          const int i = Pairs<Q>::lo(k);
+          const int j = L::opp[i];
+          const float w = L::w[i], w2 = 2.f * L::w[i];
+          const float eu = edot<Q>(i, ux, uy, uz);
+          const float t = rho * eu;
+          const float X = c0 + 4.5f * (t * eu);
+          float P2 = S[k] - om * (S[k] - w2 * X);
+          float M2 = D[k] - omm * (D[k] - w2 * (3.f * t));
+          if constexpr (FRC) {
+            // Guo source, even and odd parts (doubled): (1 - omega/2) w (9
+            // (e.u)(e.F) - 3 u.F) and (1 - omega_m/2) 3 w (e.F); a uniform F
+            // has them per pair
+            float c1k = a.c1[k], sok = a.so[k];
+            if constexpr (DIFF) {
+              const float eF = edot<Q>(i, Fx, Fy, Fz);
+              c1k = 18.f * a.guo * w * eF;
+              sok = (1.f - 0.5f * omm) * 6.f * w * eF;
+            }
+            P2 += c1k * eu - w * g3uF2;
+            M2 += sok;
+          }
+          if constexpr (SP) {
+            P2 += sp * (a.ae[k] - P2);
+            M2 += sp * (a.bo[k] - M2);
+          }
+          const float hM = 0.5f * M2;
+          st_sr_pair<ST, NT>(out_self + i * kLatBlock, out_self + j * kLatBlock, self_lo,
+                             __builtin_fmaf(0.5f, P2, hM), __builtin_fmaf(0.5f, P2, -hM),
+                             ST::dither(hsh, (11 * (2 * k + 1)) % 17), ST::dither(hsh, (11 * (2 * k + 2)) % 17));
         }
-        const float Se = c1k * eu - w * g3uF;
-        const float hS = 0.5f * S[k], hD = 0.5f * D[k];
-        float P = hS - om * (hS - A) + Se;
-        float M = hD - omm * (hD - B) + sok;
-        if constexpr (GEN) {
-          P += sp * (a.ae[k] - P);
-          M += sp * (a.bo[k] - M);
-        }
-        st_sr<ST, NT>(out_self + i * kLatBlock, self_lo, P + M, ST::dither(hsh, (11 * (2 * k + 1)) % 17));
-        st_sr<ST, NT>(out_self + j * kLatBlock, self_lo, P - M, ST::dither(hsh, (11 * (2 * k + 2)) % 17));
-      }
+      };
+      collide_pairs(std::integral_constant<bool, GEN>{});
     }
   }
 
@@ -433,7 +457,7 @@ __device__ __forceinline__ void step_tile(const typename ST::T* __restrict__ gin
   }
 }
 
-template <int Q, class ST, bool GEN, bool NT, bool ROW, bool RIGID = false, bool DIFF = false>
+template <int Q, class ST, bool GEN, bool NT, bool ROW, bool RIGID = false, bool DIFF = false, bool FRC = true>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q == 27 ? 6 : 8)))
 k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout, StepArgs a) {
   if constexpr (GEN) {
@@ -453,7 +477,7 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
     salt = *a.salt_in;
     if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *a.salt_out = salt + 1;
   }
-  step_tile<Q, ST, GEN, NT, ROW, RIGID, DIFF>(gin, gout, a, b, salt, a.mac);
+  step_tile<Q, ST, GEN, NT, ROW, RIGID, DIFF, FRC>(gin, gout, a, b, salt, a.mac);
 }
 
 // Many solves in one launch (the serve loop's packed pool): block k steps
@@ -461,7 +485,7 @@ k_step(const typename ST::T* __restrict__ gin, typename ST::T* __restrict__ gout
 // launch < remaining; its lattice parity is its base parity flipped by the
 // launch index, so every launch of a captured chunk is one step of every
 // active solve.
-template <int Q, class ST, bool GEN, bool NT, bool ROW>
+template <int Q, class ST, bool GEN, bool NT, bool ROW, bool FRC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q == 27 ? 6 : 8))) k_step_batch(BatchArgs B) {
   // The tables are written before the launch and only read during it: read
   // them through the constant address space, so their fields are scalar loads
@@ -483,7 +507,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q == 2
   // the dither counter of the solve is its step count
   const uint32_t salt = S.step0 + B.launch;
   float* mac = B.launch + 1 == S.remaining ? S.mac : nullptr;
-  step_tile<Q, ST, GEN, NT, ROW>(static_cast<const typename ST::T*>(S.f[p]),
+  step_tile<Q, ST, GEN, NT, ROW, false, false, FRC>(static_cast<const typename ST::T*>(S.f[p]),
                                  static_cast<typename ST::T*>(S.f[p ^ 1]), a, tile, salt, mac);
 }
 
@@ -704,20 +728,32 @@ template <int Q, class ST> struct OpStep {
         return hipGetLastError();
       }
     }
+    // a solve without a body force runs the instantiation without Guo terms
+    const bool frc = a.F[0] != 0.f || a.F[1] != 0.f || a.F[2] != 0.f;
     with_flags(general, nt, rows, [&](auto G, auto N, auto R) {
-      constexpr auto k = k_step<Q, ST, decltype(G)::value, decltype(N)::value, decltype(R)::value>;
-      hipLaunchKernelGGL(k, dim3(a.nblocks), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, (const T*)fin,
-                         (T*)fout, a);
+      auto go = [&](auto F) {
+        constexpr auto k = k_step<Q, ST, decltype(G)::value, decltype(N)::value, decltype(R)::value, false, false,
+                                  decltype(F)::value>;
+        hipLaunchKernelGGL(k, dim3(a.nblocks), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, (const T*)fin,
+                           (T*)fout, a);
+      };
+      if (frc) go(std::true_type{});
+      else go(std::false_type{});
     });
     return hipGetLastError();
   }
 };
 template <int Q, class ST> struct OpStepBatch {
-  static hipError_t run(bool general, bool nt, int occ, bool rows, BatchArgs b, uint32_t ntiles, int bx, int by,
-                        hipStream_t s) {
+  static hipError_t run(bool general, bool nt, int occ, bool rows, bool force, BatchArgs b, uint32_t ntiles, int bx,
+                        int by, hipStream_t s) {
     with_flags(general, nt, rows, [&](auto G, auto N, auto R) {
-      constexpr auto k = k_step_batch<Q, ST, decltype(G)::value, decltype(N)::value, decltype(R)::value>;
-      hipLaunchKernelGGL(k, dim3(ntiles), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, b);
+      auto go = [&](auto F) {
+        constexpr auto k =
+            k_step_batch<Q, ST, decltype(G)::value, decltype(N)::value, decltype(R)::value, decltype(F)::value>;
+        hipLaunchKernelGGL(k, dim3(ntiles), dim3(bx, by), occ_dynamic_lds<k>(occ, bx * by), s, b);
+      };
+      if (force) go(std::true_type{});
+      else go(std::false_type{});
     });
     return hipGetLastError();
   }
@@ -751,10 +787,10 @@ hipError_t launch_step(int q, int storage, bool general, bool nontemporal, int o
                        const StepArgs& a, int bx, int by, hipStream_t s, bool diffuse) {
   return dispatch<OpStep>(q, storage, general, nontemporal, occ, fin, fout, a, bx, by, s, diffuse);
 }
-hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, bool rows,
+hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, bool rows, bool force,
                              const BatchArgs& b, uint32_t ntiles, int bx, int by, hipStream_t s) {
   if (ntiles == 0) return hipSuccess;
-  return dispatch<OpStepBatch>(q, storage, general, nontemporal, occ, rows, b, ntiles, bx, by, s);
+  return dispatch<OpStepBatch>(q, storage, general, nontemporal, occ, rows, force, b, ntiles, bx, by, s);
 }
 hipError_t launch_init_eq(int q, int storage, const float* mac, void* f, uint32_t N, hipStream_t s) {
   return dispatch<OpInit>(q, storage, mac, f, N, s);

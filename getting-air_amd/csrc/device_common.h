@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace lbm {
 
@@ -115,6 +116,31 @@ __device__ __forceinline__ void st_sr(typename ST::T* p, uint32_t i, float g, ui
   const typename ST::T v = ST::round_sr(g, r);
   if constexpr (NT) __builtin_nontemporal_store(v, at(p, i));
   else *at(p, i) = v;
+}
+
+// Two stochastically rounded stores of one cell (planes pi and pj).  binary16
+// converts both values with one packed round-toward-zero conversion, the
+// same bits as two single conversions.  The high half is taken from the
+// packed word by a shift: a store of the vector's element 1 itself was
+// compiled into a store of the low half.
+template <class ST, bool NT>
+__device__ __forceinline__ void st_sr_pair(typename ST::T* pi, typename ST::T* pj, uint32_t i, float gi, float gj,
+                                           uint32_t ri, uint32_t rj) {
+  if constexpr (std::is_same_v<ST, StoreF16>) {
+    const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(__uint_as_float(__float_as_uint(gi) + ri),
+                                                                               __uint_as_float(__float_as_uint(gj) + rj)));
+    const uint16_t hi = (uint16_t)h, hj = (uint16_t)(h >> 16);
+    if constexpr (NT) {
+      __builtin_nontemporal_store(hi, at(pi, i));
+      __builtin_nontemporal_store(hj, at(pj, i));
+    } else {
+      *at(pi, i) = hi;
+      *at(pj, i) = hj;
+    }
+  } else {
+    st_sr<ST, NT>(pi, i, gi, ri);
+    st_sr<ST, NT>(pj, i, gj, rj);
+  }
 }
 
 // Per-cell, per-step 32-bit hash for the dither (a murmur3-style finalizer).

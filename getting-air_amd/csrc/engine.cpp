@@ -197,7 +197,9 @@ int validate(const lbm_desc* d) {
   return LBM_OK;
 }
 
-// Per-pair collision constants of k_step (see the kernel's pair form).
+// Per-pair collision constants of k_step, doubled: the kernel evaluates the
+// pair form in doubled variables (P2 = 2P, M2 = 2M), and twice a binary32
+// value is exact.
 template <int Q>
 void pair_constants(StepArgs& a, const lbm_desc& d) {
   using L = Lattice<Q>;
@@ -209,12 +211,12 @@ void pair_constants(StepArgs& a, const lbm_desc& d) {
     const float w = L::w[i];
     const float eF = L::ex[i] * d.force[0] + L::ey[i] * d.force[1] + L::ez[i] * d.force[2];
     const float eU = L::ex[i] * U[0] + L::ey[i] * U[1] + L::ez[i] * U[2];
-    a.c1[k] = 9.f * (1.f - 0.5f * om) * w * eF;
-    a.so[k] = (1.f - 0.5f * omm) * 3.f * w * eF;
-    a.ae[k] = w * (4.5f * eU * eU - 1.5f * uu);
-    a.bo[k] = 3.f * w * eU;
+    a.c1[k] = 2.f * (9.f * (1.f - 0.5f * om) * w * eF);
+    a.so[k] = 2.f * ((1.f - 0.5f * omm) * 3.f * w * eF);
+    a.ae[k] = 2.f * (w * (4.5f * eU * eU - 1.5f * uu));
+    a.bo[k] = 2.f * (3.f * w * eU);
   }
-  a.ae[13] = L::w[0] * (-1.5f * uu);
+  a.ae[13] = L::w[0] * (-1.5f * uu);  // the rest population is not doubled
 }
 
 // CHI_EPS of the diffuse coupling (main-3d.js:1026 default 1.5).

@@ -144,9 +144,10 @@ struct Member {
 
 struct ClassKey {
   int q, storage, general;
-  int rows;  // nx is a multiple of the layout block (row-mode kernels)
+  int rows;   // nx is a multiple of the layout block (row-mode kernels)
+  int force;  // a nonzero body force (the kernels with Guo terms)
   bool operator<(const ClassKey& o) const {
-    return std::tie(q, storage, general, rows) < std::tie(o.q, o.storage, o.general, o.rows);
+    return std::tie(q, storage, general, rows, force) < std::tie(o.q, o.storage, o.general, o.rows, o.force);
   }
 };
 
@@ -271,7 +272,8 @@ void admit(lbm_serve* sv) {
                         (r->d.walls[2] && r->d.q != 9) || r->d.sponge_width > 0.f)
                            ? 1
                            : 0,
-                       r->d.nx % (int)kLatBlock == 0 ? 1 : 0};
+                       r->d.nx % (int)kLatBlock == 0 ? 1 : 0,
+                       (r->d.force[0] != 0.f || r->d.force[1] != 0.f || r->d.force[2] != 0.f) ? 1 : 0};
     auto& cp = sv->classes[key];
     if (!cp) {
       cp = std::make_unique<Class>();
@@ -380,8 +382,8 @@ int enqueue_chunk(lbm_serve* sv, int K) {
       if (c->members.empty()) continue;
       BatchArgs b{c->d_solves, c->d_tiles, j};
       const bool rows = c->key.rows && step_row_mode((int)kLatBlock, c->lin != 0, c->bx, c->by);
-      HIPCHK_RET(launch_step_batch(c->key.q, c->key.storage, c->key.general != 0, c->nt != 0, c->occ, rows, b,
-                                   c->ntiles, c->bx, c->by, sv->stream));
+      HIPCHK_RET(launch_step_batch(c->key.q, c->key.storage, c->key.general != 0, c->nt != 0, c->occ, rows,
+                                   c->key.force != 0, b, c->ntiles, c->bx, c->by, sv->stream));
     }
   }
   return LBM_OK;
