@@ -78,8 +78,9 @@ void fine_params(CollideParams& p, float omega, float omega_m, const float* F) {
   for (int k = 0; k < Pairs<Q>::NP; k++) {
     const int i = Pairs<Q>::lo(k);
     const float eF = L::ex[i] * F[0] + L::ey[i] * F[1] + L::ez[i] * F[2];
-    p.c1[k] = 9.f * (1.f - 0.5f * omega) * L::w[i] * eF;
-    p.so[k] = (1.f - 0.5f * omega_m) * 3.f * L::w[i] * eF;
+    // doubled, as collide_pairs uses them
+    p.c1[k] = 2.f * (9.f * (1.f - 0.5f * omega) * L::w[i] * eF);
+    p.so[k] = 2.f * ((1.f - 0.5f * omega_m) * 3.f * L::w[i] * eF);
   }
 }
 

@@ -27,6 +27,7 @@
 // being re-derived from the block map in every pass.  Neighbour coordinates
 // move by at most one period, so they wrap with a compare, never a modulo.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include <algorithm>
 
@@ -254,7 +255,7 @@ __global__ void __launch_bounds__(256) k_tile_table(PoolGeom G, const int32_t* _
 // common_d3_amr_step1.wgsl:258-276; the body is at rest, so Ladd's term is
 // zero), and a solid cell is held at feq(1, 0) with [1, 0, 0, 0] as its
 // moments (:387-403).
-template <int Q, int RB, bool BODY>
+template <int Q, int RB, bool BODY, bool FRC>
 __device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, float* __restrict__ gout,
                                                float* __restrict__ mac_pool, const PoolGeom& G, const CollideParams& P,
                                                const int32_t* __restrict__ tt, int fx, int fy, int fz,
@@ -323,14 +324,16 @@ __device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, fl
 
   if (mac_pool || collide_cell) {
     float drho, rho, ux, uy, uz;
-    moments<Q>(g, P.F, drho, rho, ux, uy, uz);
+    float S[Pairs<Q>::NP], D[Pairs<Q>::NP];
+    pair_sums<Q>(g, S, D);
+    pair_moments<Q, FRC>(g[0], S, D, P.F, drho, rho, ux, uy, uz);
     if (mac_pool) {
       *at(mac_pool, cell) = rho;
       *at(mac_pool + G.pool_plane, cell) = ux;
       *at(mac_pool + 2 * (size_t)G.pool_plane, cell) = uy;
       *at(mac_pool + 3 * (size_t)G.pool_plane, cell) = uz;
     }
-    if (collide_cell) collide<Q>(g, P, drho, rho, ux, uy, uz);
+    if (collide_cell) collide_pairs<Q, FRC>(g, S, D, P, drho, rho, ux, uy, uz);
   }
   if (BODY && phi < 0.f) {
 #pragma unroll
@@ -352,7 +355,7 @@ __device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, fl
 // cells, and the outer layer in the first substep) was measured slower on
 // the vortex-ring workload (1.17 against 1.04 ms per macro-step), the masked
 // lanes costing the waves more than the traffic they save.
-template <int Q, int RB, bool BODY>
+template <int Q, int RB, bool BODY, bool FRC>
 __global__ void __launch_bounds__(256)
 k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __restrict__ mac_pool, PoolGeom G,
             CollideParams P, int nactive, int collide_ring) {
@@ -368,7 +371,7 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
   // Ring cells advect and store without colliding (common_d3_amr_step1.wgsl
   // COLLIDE_RING = 0): coalesce averages uncollided interface states.  The
   // interp interface collides them too (COLLIDE_RING = 1, main-3d.js:1395).
-  pool_cell_step<Q, RB, BODY>(gin, gout, mac_pool, G, P, tt, fx, fy, fz, interior || collide_ring);
+  pool_cell_step<Q, RB, BODY, FRC>(gin, gout, mac_pool, G, P, tt, fx, fy, fz, interior || collide_ring);
 }
 
 // Momentum exchange on the fine-level body (common_d3_force_pool.wgsl, the
@@ -1077,12 +1080,19 @@ struct OpPoolStep {
   static hipError_t run(const float* gin, float* gout, float* mac_pool, const PoolGeom& g, const CollideParams& p,
                         int nactive, int collide_ring, hipStream_t s) {
     const dim3 grid = grid_for((uint64_t)nactive * Tile<RB>::CELLS);
-    if (g.body.on)
-      hipLaunchKernelGGL((k_pool_step<Q, RB, true>), grid, dim3(256), 0, s, gin, gout, mac_pool, g, p, nactive,
-                         collide_ring);
-    else
-      hipLaunchKernelGGL((k_pool_step<Q, RB, false>), grid, dim3(256), 0, s, gin, gout, mac_pool, g, p, nactive,
-                         collide_ring);
+    // a level without a body force runs the instantiation without Guo terms
+    const bool frc = p.F[0] != 0.f || p.F[1] != 0.f || p.F[2] != 0.f;
+    auto go = [&](auto B, auto F) {
+      hipLaunchKernelGGL((k_pool_step<Q, RB, decltype(B)::value, decltype(F)::value>), grid, dim3(256), 0, s, gin,
+                         gout, mac_pool, g, p, nactive, collide_ring);
+    };
+    if (g.body.on) {
+      if (frc) go(std::true_type{}, std::true_type{});
+      else go(std::true_type{}, std::false_type{});
+    } else {
+      if (frc) go(std::false_type{}, std::true_type{});
+      else go(std::false_type{}, std::false_type{});
+    }
     return hipGetLastError();
   }
 };
