@@ -375,26 +375,33 @@ def measure_serve(count, rate=50.0, seed=5, steps=2000, budget=200e9, scale=1.0,
     mix = sc.serve_mix(count=count, rate=rate, seed=seed, steps=steps, storage="f16", scale=scale)
     cells = [d["nx"] * d["ny"] * (1 if d["q"] == 9 else d["nz"]) for _, d, _, _, _ in mix]
     updates = float(sum(c * st for c, (_, _, _, _, st) in zip(cells, mix)))
-    # 48M resident cells fill the device; 64-step chunks hide admissions
-    with pylbm.Server(budget_bytes=budget, chunk_steps=64, autotune=1, target_cells=48e6) as sv:
-        tickets = []
+    def serve_run(persistent):
+        # 48M resident cells fill the device; 64-step chunks hide admissions
+        with pylbm.Server(budget_bytes=budget, chunk_steps=64, autotune=1, target_cells=48e6,
+                          persistent=persistent) as sv:
+            tickets = []
 
-        def submitter():
-            t0 = sv.clock()
-            for arrival, d, m, _, st in mix:
-                dt = arrival - (sv.clock() - t0)
-                if dt > 0:
-                    time.sleep(dt)
-                tickets.append(sv.submit(d, m, st))
+            def submitter():
+                t0 = sv.clock()
+                for arrival, d, m, _, st in mix:
+                    dt = arrival - (sv.clock() - t0)
+                    if dt > 0:
+                        time.sleep(dt)
+                    tickets.append(sv.submit(d, m, st))
 
-        th = threading.Thread(target=submitter)
-        t_start = time.perf_counter()
-        th.start()
-        th.join()
-        sv.drain(-1)
-        wall = time.perf_counter() - t_start
-        st = sv.stats()
-        finite = all(np.isfinite(sv.read_macro(t)).all() for t in tickets[:: max(1, len(tickets) // 16)])
+            th = threading.Thread(target=submitter)
+            t_start = time.perf_counter()
+            th.start()
+            th.join()
+            sv.drain(-1)
+            wall = time.perf_counter() - t_start
+            st = sv.stats()
+            finite = all(np.isfinite(sv.read_macro(t)).all() for t in tickets[:: max(1, len(tickets) // 16)])
+        return wall, st, finite
+
+    wall, st, finite = serve_run(0)
+    print("serve: batched stream done in %.2f s" % wall, file=sys.stderr, flush=True)
+
     def one_at_a_time(storage):
         t = time.perf_counter()
         for _, d, m, _, nsteps in mix:
@@ -422,6 +429,7 @@ def measure_serve(count, rate=50.0, seed=5, steps=2000, budget=200e9, scale=1.0,
     if not baseline:
         return served
     seq = one_at_a_time("f16")
+    print("serve: one at a time (f16) done in %.2f s" % seq, file=sys.stderr, flush=True)
     seq32 = one_at_a_time("f32")
     return {"requests": count, "arrival_rate_per_s": rate, "last_arrival_s": mix[-1][0],
             "served_wall_s": wall, "served_solves_per_s": count / wall, "served_mlups": updates / wall / 1e6,
@@ -431,6 +439,38 @@ def measure_serve(count, rate=50.0, seed=5, steps=2000, budget=200e9, scale=1.0,
             "one_at_a_time_mlups": updates / seq / 1e6, "speedup_solves_per_s": seq / wall,
             "one_at_a_time_f32_wall_s": seq32, "speedup_vs_one_at_a_time_f32": seq32 / wall,
             "cell_updates": updates, "finite": bool(finite), "budget_bytes": budget, **device}
+
+
+def measure_persistent_chunks(n=192, steps=128, chunk=64):
+    """The serve loop's two chunk forms on one resident request (a shifted-fp16
+    boxed sphere of n^3 cells, nothing else queued): a captured graph of
+    `chunk` batched launches, and one cooperative persistent launch that
+    steps every tile `chunk` times with a grid barrier between steps.  Device
+    time per step from the serve loop's HIP events around each chunk."""
+    desc, mac, _ = sc.boxed_sphere(n=n, re=200.0, storage="f16")
+    cells = n ** 3
+    out = {"workload": "D3Q19 f16 boxed sphere %d^3, %d steps, %d-step chunks" % (n, steps, chunk)}
+    for persistent in (0, 1):
+        print("persistent chunks: mode %d start" % persistent, file=sys.stderr, flush=True)
+        with pylbm.Server(budget_bytes=8 << 30, chunk_steps=chunk, persistent=persistent) as sv:
+            t = sv.submit(desc, mac, steps)
+            r = sv.wait(t, 120)
+            st = sv.stats()
+            mac_out = sv.read_macro(t)
+        print("persistent chunks: mode %d done, %.3f s device" % (persistent, st["device_seconds"]),
+              file=sys.stderr, flush=True)
+        key = "persistent" if persistent else "graph"
+        out[key] = {"status": r["status"], "device_ms_per_step": st["device_seconds"] / steps * 1e3,
+                    "device_gbs": st["device_bytes"] / st["device_seconds"] / 1e9 if st["device_seconds"] else None,
+                    "mlups": cells * steps / st["device_seconds"] / 1e6 if st["device_seconds"] else None,
+                    "chunks": st["chunks"]}
+        out[key + "_mac"] = mac_out
+    a, b = out.pop("graph_mac"), out.pop("persistent_mac")
+    out["bit_identical"] = bool(np.array_equal(a.view(np.uint32), b.view(np.uint32)))
+    g, p = out["graph"]["device_ms_per_step"], out["persistent"]["device_ms_per_step"]
+    # the persistent launch's extra time per step is its grid barrier
+    out["barrier_us_per_step"] = (p - g) * 1e3 if p and g else None
+    return out
 
 
 def main():
@@ -468,6 +508,7 @@ def main():
     if args.only_serve:
         r = measure_serve(args.serve_requests, baseline=not args.no_serve_baseline)
         print(json.dumps({"serve_config5": r}), flush=True)
+        print(json.dumps({"serve_chunk_forms": measure_persistent_chunks()}), flush=True)
         return
 
     dev = pylbm.device_info()
@@ -523,6 +564,9 @@ def main():
         print("serve: %s" % json.dumps(extra_serve), file=sys.stderr, flush=True)
 
     extra_turb = None
+    extra_persistent = None
+    if not args.no_serve:
+        extra_persistent = measure_persistent_chunks()
     if args.turbulence:
         extra_turb = measure_turbulence()
         print("turbulence: %s" % json.dumps(extra_turb), file=sys.stderr, flush=True)
@@ -590,6 +634,11 @@ def main():
         out["turbulence_d3q19_512"] = extra_turb
     if extra_serve:
         out["serve_config5"] = {k: (float("%.5g" % v) if isinstance(v, float) else v) for k, v in extra_serve.items()}
+    if extra_persistent:
+        out["serve_chunk_forms"] = {k: ({kk: (float("%.5g" % vv) if isinstance(vv, float) else vv)
+                                         for kk, vv in v.items()} if isinstance(v, dict) else
+                                        (float("%.5g" % v) if isinstance(v, float) else v))
+                                    for k, v in extra_persistent.items()}
     def short(d):
         return {k: (float("%.5g" % v) if isinstance(v, float) else v) for k, v in d.items()}
 

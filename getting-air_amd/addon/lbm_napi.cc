@@ -887,7 +887,7 @@ int64_t ticket_of(napi_env env, napi_value v) {
   return t;
 }
 
-// serveCreate({budgetBytes, maxResident, chunkSteps, autotune})
+// serveCreate({budgetBytes, maxResident, chunkSteps, autotune, targetCells, persistent})
 napi_value ServeCreate(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
@@ -904,6 +904,7 @@ napi_value ServeCreate(napi_env env, napi_callback_info info) {
   d.chunk_steps = (int)num(env, argv[0], "chunkSteps", 32);
   d.autotune = (int)num(env, argv[0], "autotune", 0);
   d.target_cells = num(env, argv[0], "targetCells", 0);
+  d.persistent = (int)num(env, argv[0], "persistent", 0);
   lbm_serve* sv = nullptr;
   int rc = lbm_serve_create(&d, &sv);
   if (rc) return throw_lbm(env, rc);

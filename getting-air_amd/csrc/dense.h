@@ -184,6 +184,19 @@ hipError_t launch_step(int q, int storage, bool general, bool nontemporal, int o
 // solve of the batch has a nonzero body force.
 hipError_t launch_step_batch(int q, int storage, bool general, bool nontemporal, int occ, bool rows, bool force,
                              const BatchArgs& b, uint32_t ntiles, int bx, int by, hipStream_t s);
+// Device word of the persistent chunk's grid barrier (zeroed on the stream
+// before each launch): the arrivals, with bit 31 set after the launch when a
+// barrier wait ran out (the launch then stopped early; its results are void).
+struct PersistBarrier {
+  uint32_t arrived;
+  uint32_t pad;
+};
+// The same `steps` launches as one cooperative persistent launch (b.launch is
+// ignored): the grid holds as many blocks as fit on the device at once and
+// meets at a grid barrier between steps.  Not capturable in a graph.
+hipError_t launch_step_persistent(int q, int storage, bool general, bool nontemporal, int occ, bool rows, bool force,
+                                  const BatchArgs& b, uint32_t ntiles, int bx, int by, int steps, PersistBarrier* bar,
+                                  hipStream_t s);
 // Whether `occ` waves per SIMD can be set exactly for blocks of `threads`.
 bool occupancy_cap_ok(int occ, int threads);
 hipError_t launch_force_finalize(const ForceFinalizeArgs& a, hipStream_t s);

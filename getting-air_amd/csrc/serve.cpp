@@ -160,6 +160,7 @@ struct Class {
   BatchTile* d_tiles = nullptr;
   size_t cap_solves = 0, cap_tiles = 0;
   uint32_t ntiles = 0;
+  PersistBarrier* d_bar = nullptr;  // the persistent chunk's grid barrier words
   // host sources of the tables' async uploads: they must outlive the copies,
   // so they live here until the next chunk (each chunk ends with a sync)
   std::vector<BatchSolve> h_solves;
@@ -389,6 +390,19 @@ int enqueue_chunk(lbm_serve* sv, int K) {
   return LBM_OK;
 }
 
+// Algorithmic bytes of the chunk about to run: every member steps
+// min(remaining, K) times.
+void set_chunk_bytes(lbm_serve* sv, int K) {
+  std::lock_guard<std::mutex> g(sv->mu);
+  double bytes = 0;
+  for (auto& m : sv->resident) {
+    const int n = std::min(m->r->steps - m->done_steps, K);
+    const double elem = m->r->d.storage == LBM_STORE_F32 ? 4.0 : 2.0;
+    bytes += (double)n * m->r->cells * 2.0 * m->r->d.q * elem;
+  }
+  sv->chunk_bytes_pending = bytes;
+}
+
 // Launches the chunk of the current resident set on the compute stream
 // (after the set-up stream's last admissions) and returns without waiting.
 int launch_chunk(lbm_serve* sv) {
@@ -401,6 +415,22 @@ int launch_chunk(lbm_serve* sv) {
   bool rebuild = false;
   int rc = prepare(sv, &rebuild);
   if (rc) return rc;
+  if (sv->d.persistent) {
+    // one cooperative launch per lattice class runs the whole chunk
+    set_chunk_bytes(sv, K);
+    HIPCHK_RET(hipEventRecord(sv->chunk_ev[0], sv->stream));
+    for (auto& kv : sv->classes) {
+      Class* c = kv.second.get();
+      if (c->members.empty()) continue;
+      BatchArgs b{c->d_solves, c->d_tiles, 0};
+      const bool rows = c->key.rows && step_row_mode((int)kLatBlock, c->lin != 0, c->bx, c->by);
+      if (!c->d_bar) HIPCHK_RET(hipMalloc((void**)&c->d_bar, sizeof(PersistBarrier)));
+      HIPCHK_RET(launch_step_persistent(c->key.q, c->key.storage, c->key.general != 0, c->nt != 0, c->occ, rows,
+                                        c->key.force != 0, b, c->ntiles, c->bx, c->by, K, c->d_bar, sv->stream));
+    }
+    HIPCHK_RET(hipEventRecord(sv->chunk_ev[1], sv->stream));
+    return LBM_OK;
+  }
   if (rebuild) {
     // Re-capture the chunk; when only launch shapes and table addresses
     // changed (the same classes, so the same kernels in the same order), the
@@ -438,17 +468,7 @@ int launch_chunk(lbm_serve* sv) {
     }
     (void)hipGraphDestroy(g);
   }
-  {
-    // bytes of this chunk: every member steps min(remaining, K) times
-    std::lock_guard<std::mutex> g(sv->mu);
-    double bytes = 0;
-    for (auto& m : sv->resident) {
-      const int n = std::min(m->r->steps - m->done_steps, K);
-      const double elem = m->r->d.storage == LBM_STORE_F32 ? 4.0 : 2.0;
-      bytes += (double)n * m->r->cells * 2.0 * m->r->d.q * elem;
-    }
-    sv->chunk_bytes_pending = bytes;
-  }
+  set_chunk_bytes(sv, K);
   HIPCHK_RET(hipEventRecord(sv->chunk_ev[0], sv->stream));
   HIPCHK_RET(hipGraphLaunch(sv->graph, sv->stream));
   HIPCHK_RET(hipEventRecord(sv->chunk_ev[1], sv->stream));
@@ -461,6 +481,17 @@ int launch_chunk(lbm_serve* sv) {
 int complete_chunk(lbm_serve* sv) {
   const int K = std::max(1, sv->d.chunk_steps);
   HIPCHK_RET(hipStreamSynchronize(sv->stream));
+  if (sv->d.persistent) {
+    // a persistent launch whose grid barrier ran out stopped early
+    for (auto& kv : sv->classes) {
+      Class* c = kv.second.get();
+      if (c->members.empty() || !c->d_bar) continue;
+      uint32_t arrived = 0;
+      HIPCHK_RET(hipMemcpy(&arrived, &c->d_bar->arrived, sizeof arrived, hipMemcpyDeviceToHost));
+      if (arrived >> 31)
+        return fail(LBM_E_DEVICE, "persistent chunk: a grid barrier wait ran out (blocks not co-resident)");
+    }
+  }
   float ms = 0.f;
   HIPCHK_RET(hipEventElapsedTime(&ms, sv->chunk_ev[0], sv->chunk_ev[1]));
   std::lock_guard<std::mutex> g(sv->mu);
@@ -615,6 +646,14 @@ int lbm_serve_create(const lbm_serve_desc* desc, lbm_serve** out) {
   if (!(desc->budget_bytes >= 1 << 20)) return fail(LBM_E_INVALID, "budget_bytes %g below 1 MiB", desc->budget_bytes);
   if (desc->max_resident < 0 || desc->chunk_steps < 0 || desc->target_cells < 0)
     return fail(LBM_E_INVALID, "negative max_resident / chunk_steps / target_cells");
+  if (desc->persistent != 0 && desc->persistent != 1) return fail(LBM_E_INVALID, "persistent %d: 0 or 1", desc->persistent);
+  if (desc->persistent) {
+    int dev = 0, coop = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
+    if (e != hipSuccess) return hip_fail(e, "cooperative launch attribute");
+    if (!coop) return fail(LBM_E_DEVICE, "persistent: the device has no cooperative launch");
+  }
   auto sv = std::make_unique<lbm_serve>();
   sv->d = *desc;
   if (sv->d.chunk_steps == 0) sv->d.chunk_steps = 32;
@@ -648,6 +687,7 @@ int lbm_serve_destroy(lbm_serve* sv) {
   for (auto& kv : sv->classes) {
     if (kv.second->d_solves) (void)hipFree(kv.second->d_solves);
     if (kv.second->d_tiles) (void)hipFree(kv.second->d_tiles);
+    if (kv.second->d_bar) (void)hipFree(kv.second->d_bar);
   }
   if (sv->stream) (void)hipStreamDestroy(sv->stream);
   if (sv->setup) (void)hipStreamDestroy(sv->setup);

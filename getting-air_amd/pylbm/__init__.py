@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblbm.so")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 BGK, TRT = 0, 1
 STORE = {"f32": 0, "f16": 1, "bf16": 2}
 BODY_NONE, BODY_SPHERE, BODY_CIRCLE = 0, 1, 2
@@ -116,7 +116,8 @@ def make_rigid(p):
 
 class ServeDesc(ctypes.Structure):
     _fields_ = [("abi_version", ctypes.c_int32), ("max_resident", ctypes.c_int32), ("budget_bytes", ctypes.c_double),
-                ("chunk_steps", ctypes.c_int32), ("autotune", ctypes.c_int32), ("target_cells", ctypes.c_double)]
+                ("chunk_steps", ctypes.c_int32), ("autotune", ctypes.c_int32), ("target_cells", ctypes.c_double),
+                ("persistent", ctypes.c_int32)]
 
 
 class ServeResult(ctypes.Structure):
@@ -555,7 +556,7 @@ class Amr:
 class Server:
     """Batched serve loop on the GPU (owns an lbm_serve handle)."""
 
-    def __init__(self, budget_bytes, max_resident=0, chunk_steps=32, autotune=0, target_cells=0.0):
+    def __init__(self, budget_bytes, max_resident=0, chunk_steps=32, autotune=0, target_cells=0.0, persistent=0):
         self.lib = load()
         d = ServeDesc()
         d.abi_version = ABI_VERSION
@@ -564,6 +565,7 @@ class Server:
         d.chunk_steps = int(chunk_steps)
         d.autotune = int(autotune)
         d.target_cells = float(target_cells)
+        d.persistent = int(persistent)
         h = ctypes.c_void_p()
         _check(self.lib.lbm_serve_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h

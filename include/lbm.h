@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 5
+#define LBM_ABI_VERSION 6
 
 enum {
   LBM_OK = 0,
@@ -411,6 +411,11 @@ typedef struct lbm_serve_desc {
                             request, when that holds 2^20 cells or more       */
   double target_cells;   /* stop admitting once this many cells are resident
                             (enough to fill the device; 0: no limit)          */
+  int32_t persistent;    /* 0: a chunk is a captured graph of chunk_steps
+                            batched launches per lattice class; 1: one
+                            cooperative persistent launch per class, whose
+                            resident blocks step every tile chunk_steps times
+                            with a grid barrier between steps              */
 } lbm_serve_desc;
 
 typedef struct lbm_serve_result {

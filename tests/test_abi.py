@@ -129,8 +129,9 @@ def test_serve_struct_layouts_match_c():
 #include <stddef.h>
 #include "lbm.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_serve_desc),
-         offsetof(lbm_serve_desc, budget_bytes), offsetof(lbm_serve_desc, autotune), sizeof(lbm_serve_result),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_serve_desc),
+         offsetof(lbm_serve_desc, budget_bytes), offsetof(lbm_serve_desc, autotune),
+         offsetof(lbm_serve_desc, persistent), sizeof(lbm_serve_result),
          offsetof(lbm_serve_result, t_submit), offsetof(lbm_serve_result, force), sizeof(lbm_serve_stats),
          offsetof(lbm_serve_stats, peak_resident), offsetof(lbm_serve_stats, p99_latency),
          offsetof(lbm_serve_stats, device_bytes), offsetof(lbm_serve_stats, graph_updates));
@@ -144,7 +145,8 @@ int main(void) {
         subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe])
         got = [int(v) for v in subprocess.check_output([exe]).split()]
     D, R, S = pylbm.ServeDesc, pylbm.ServeResult, pylbm.ServeStats
-    want = [ctypes.sizeof(D), D.budget_bytes.offset, D.autotune.offset, ctypes.sizeof(R), R.t_submit.offset,
+    want = [ctypes.sizeof(D), D.budget_bytes.offset, D.autotune.offset, D.persistent.offset, ctypes.sizeof(R),
+            R.t_submit.offset,
             R.force.offset, ctypes.sizeof(S), S.peak_resident.offset, S.p99_latency.offset, S.device_bytes.offset,
             S.graph_updates.offset]
     assert got == want
@@ -158,6 +160,10 @@ def test_serve_refusals_without_gpu():
     d.budget_bytes = 1000.0
     assert lib.lbm_serve_create(ctypes.byref(d), ctypes.byref(h)) == -1
     assert b"budget" in lib.lbm_last_error()
+    d.budget_bytes = float(1 << 30)
+    d.persistent = 2
+    assert lib.lbm_serve_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"persistent" in lib.lbm_last_error()
 
 
 def test_totals_noalloc_refusals_without_gpu():

@@ -88,6 +88,40 @@ def test_served_equals_the_plain_solve(k):
     assert np.array_equal(sf, pf)
 
 
+def test_persistent_chunks_equal_graph_chunks():
+    """The persistent form of a chunk (one cooperative launch per lattice
+    class, a grid barrier between steps) gives every request of the uneven
+    mix the same field and force history, bit for bit, as the captured graph
+    of batched launches; requests join while chunks run in both."""
+    reqs = mix()
+    steps = [100, 150, 120, 130, 90, 150]
+    out = {}
+    for persistent in (0, 1):
+        say("persistent", persistent)
+        with pylbm.Server(budget_bytes=2 << 30, chunk_steps=24, persistent=persistent) as sv:
+            tickets = {}
+            for k in (0, 2, 3):
+                tickets[k] = sv.submit(reqs[k][0], reqs[k][1], steps[k])
+            time.sleep(0.05)
+            for k in (1, 4, 5):
+                tickets[k] = sv.submit(reqs[k][0], reqs[k][1], steps[k])
+            sv.drain(300)
+            st = sv.stats()
+            res = {}
+            for k, t in tickets.items():
+                assert sv.wait(t, 1)["status"] == 0, (persistent, k)
+                res[k] = (sv.read_macro(t), sv.read_forces(t))
+        say("stats", {k: st[k] for k in ("completed", "chunks", "device_seconds", "graph_instantiations")})
+        assert st["completed"] == 6
+        if persistent:
+            assert st["graph_instantiations"] == 0 and st["graph_updates"] == 0
+        out[persistent] = res
+    for k in range(6):
+        (a, fa), (b, fb) = out[0][k], out[1][k]
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k
+        assert np.array_equal(fa, fb), k
+
+
 def test_budget_admission_and_refusal():
     reqs = [sc.boxed_sphere(n=48, re=100.0, storage="f32") for _ in range(5)]
     # the footprint as the library computes it, from the refusal of a budget
