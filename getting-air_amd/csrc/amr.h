@@ -1,12 +1,17 @@
 // Launch interface of the block-pool (AMR) kernels, amr_kernels.hip.
 //
-// Two levels: the dense L0 lattice and one pool of fine tiles.  L0 is cut
-// into blocks of RB^3 coarse cells; a refined block owns a slot holding
-// FB^3 fine cells, FB = 2 RB + 2 GHOST (interior [GHOST, GHOST + 2 RB) plus a
-// two-cell ring).  The pool is plane-major:
+// Level 0 is a dense solve's lattice (its block layout, dense.h lat_elem);
+// each level m >= 1 is a pool of tiles.  Level m is cut into blocks of RB^3
+// cells of its PARENT level m - 1; a refined block owns a slot holding FB^3
+// level-m cells, FB = 2 RB + 2 GHOST (interior [GHOST, GHOST + 2 RB) plus a
+// two-cell ring).  A pool is plane-major:
 // g[i * pool_plane + (slot * FB + fz) * FB * FB + fy * FB + fx], with
 // pool_plane = max_slots * FB^3, holding the shifted populations g = f - w.
-// L0 is a dense solve's lattice in its block layout (dense.h lat_elem).
+// A level's PoolGeom describes its parent grid: nx, ny, nz are the parent
+// level's global cell counts (L0's times 2^(m-1)), so its blocks, cell
+// classes and surface lists are those of level m over level m - 1, and only
+// how a parent cell's data is addressed differs between the dense parent
+// (m = 1) and a pool parent (m >= 2, ParentRef).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -26,17 +31,41 @@ struct PoolBody {
 };
 
 struct PoolGeom {
-  int32_t nx, ny, nz;        // L0 cells
-  int32_t rb, fb;            // block edge (L0 cells), tile edge (fine cells)
+  int32_t nx, ny, nz;        // parent-level cells (L0's at level 1)
+  int32_t rb, fb;            // block edge (parent cells), tile edge (this level's cells)
   int32_t nbx, nby, nbz;     // blocks per axis
-  uint32_t ncells;           // L0 cells
+  uint32_t ncells;           // parent-level cells
   uint32_t pool_plane;       // max_slots * fb^3
   int32_t max_slots;
   const int32_t* block_slot;     // per block: slot or -1
   const int32_t* slot_to_block;  // per slot: block or -1
-  const uint8_t* ccls;           // per L0 cell: kCovered | kNear | kSeam
+  const uint8_t* ccls;           // per parent cell: kCovered | kNear | kSeam
   const int32_t* tile_table;     // kTileWords per active tile (k_tile_table)
   PoolBody body;
+  // Position of this level's cells in L0 units: p = u * l0_scale + l0_offset
+  // with u the parent-unit position origin - 1/4 + (j - GHOST)/2;
+  // l0_scale = 2^-(m-1), l0_offset = -(1 - 2^-(m-1))/2 (cell-centred
+  // refinement, common_d3_amr_step1.wgsl:64-84).  1 and 0 at level 1.
+  float l0_scale, l0_offset;
+  float dx2;                     // the body force's weight dx^2 = 4^-m
+  // The next finer level's block map (null: none): an interior cell under
+  // one of its tiles is solved there, and this level's step leaves it alone
+  // (coveredByChild, common_d3_amr_step1.wgsl:107-143).
+  const int32_t* child_block_slot;
+  int32_t cnbx, cnby;            // the child level's blocks per axis (2 nx / rb ...)
+};
+
+// Where a parent cell's populations and moments live.  The dense parent (L0)
+// holds them in the lattice's block layout and float4 moments; a pool parent
+// (level m - 1 >= 1) in its tiles, planar, found through its block map: the
+// tile that owns the cell, at interior index cell - 2 RB block + GHOST
+// (parentIndex, common_d3_parent_pool.wgsl:37-59).
+struct ParentRef {
+  float* g;                   // the parent's populations at the coupling's time level
+  float* mac;                 // its moments
+  const int32_t* block_slot;  // pool parent: its block map; null: dense parent
+  int32_t nbx, nby, nbz;      // pool parent: its blocks per axis
+  uint32_t plane;             // pool parent: its pool plane
 };
 
 // Per active tile (index into the active list), rebuilt with the cell
@@ -89,8 +118,10 @@ struct SurfacePlan {
 void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint32_t* near, uint32_t* ring);
 
 // Rebuilds ccls and the plan lists (count must be zeroed first).
+// parent: which parent cells exist (a pool parent's cells outside its tiles
+// are never listed); the classes cover the whole parent grid.
 hipError_t launch_cell_class(int q, uint8_t* ccls, const PoolGeom& g, const SurfacePlan& plan, const int32_t* active,
-                             int nactive, hipStream_t s);
+                             int nactive, const ParentRef& parent, hipStream_t s);
 // mac_pool null: the substep's macroscopic field is not written.
 // collide_ring: ring cells collide as well (the interp interface).
 hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
@@ -111,15 +142,17 @@ hipError_t launch_flux(int q, const float* gpool, float* acc, const PoolGeom& g,
 hipError_t launch_reflux(int q, const float* gc_t, float* gc_t1, float* mac, const PoolGeom& g, const SurfacePlan& plan,
                          uint32_t nflux, float* acc, const float* force, hipStream_t s);
 // dirs: directions per thread (0: about 2^20 threads for the ring).
-hipError_t launch_explode(int q, const float* gcoarse, float* gpool, const PoolGeom& g, const SurfacePlan& plan,
+hipError_t launch_explode(int q, const ParentRef& parent, float* gpool, const PoolGeom& g, const SurfacePlan& plan,
                           uint32_t nring, int linear, int dirs, hipStream_t s);
-hipError_t launch_coalesce(int q, const float* gpool, float* gcoarse, float* mac, const float* mac_pool,
+hipError_t launch_coalesce(int q, const float* gpool, const ParentRef& parent, const float* mac_pool,
                            const PoolGeom& g, const int32_t* active, int nactive, const SurfacePlan& plan,
                            uint32_t nseam, uint32_t nnear, int orphans, hipStream_t s);
 // ring_only: the ring cells alone (the interp interface's per-step refresh).
-hipError_t launch_interp(int q, const float* gcoarse, float* gpool, const PoolGeom& g, const int32_t* pairs,
+// A pool parent is sampled in the parent tile that holds the child block
+// (its ring included), at the parent's own time level.
+hipError_t launch_interp(int q, const ParentRef& parent, float* gpool, const PoolGeom& g, const int32_t* pairs,
                          int npairs, float rescale_c2f, int ring_only, hipStream_t s);
-hipError_t launch_average(int q, const float* gpool, float* gcoarse, float* mac, const PoolGeom& g,
+hipError_t launch_average(int q, const float* gpool, const ParentRef& parent, const PoolGeom& g,
                           const int32_t* pairs, int npairs, float rescale_f2c, hipStream_t s);
 // kind 0: |curl u|; 1: Q = 1/2 (|Omega|^2 - |S|^2)
 hipError_t launch_criterion(const float* mac, float* crit, const PoolGeom& g, float lead, int kind, hipStream_t s);
@@ -131,5 +164,9 @@ hipError_t launch_active_pairs(const int32_t* active, const int32_t* slot_to_blo
                                hipStream_t s);
 hipError_t launch_pool_totals(int q, const float* gcoarse, const float* gpool, const PoolGeom& g, const int32_t* active,
                               double* partials, unsigned* counter, double* out, hipStream_t s);
+// FP64 [mass, px, py, pz] of the interior cells of one pool level's active
+// tiles that no finer tile covers, each weighted `weight` (8^-m).
+hipError_t launch_level_totals(int q, const float* gpool, const PoolGeom& g, const int32_t* active, double weight,
+                               double* partials, unsigned* counter, double* out, hipStream_t s);
 
 }  // namespace lbm

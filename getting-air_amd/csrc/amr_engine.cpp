@@ -1,8 +1,9 @@
-// Host side of the two-level AMR engine: the lbm_amr object behind the
+// Host side of the block-structured AMR engine: the lbm_amr object behind the
 // lbm_amr_* entry points of include/lbm.h.  It owns a dense L0 solve (its
-// lattices, macroscopic field and stream) and the fine pool: two population
-// buffers (A holds the state between macro-steps), the planar pool
-// macroscopic field, the block maps and the manager's lists.
+// lattices, macroscopic field and stream) and a pool per finer level: two
+// population buffers (A holds the state between macro-steps), the planar
+// pool macroscopic field, the block maps and the surface work lists; level 1
+// also carries the manager's lists.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,8 +24,34 @@ using namespace lbm;
     if (e_ != hipSuccess) return hip_fail(e_, #call); \
   } while (0)
 
+// A pool level m >= 2, whose parent is the pool of level m - 1.  Level 1
+// (the L0 parent) lives in lbm_amr's own fields.
+struct DeepLevel {
+  int m = 0;
+  PoolGeom G{};
+  int nblocks = 0;
+  size_t pool_cells = 0;
+  float* pool[2] = {nullptr, nullptr};
+  float* mac_pool = nullptr;
+  int32_t* block_slot = nullptr;
+  int32_t* slot_to_block = nullptr;
+  int32_t* active = nullptr;
+  int32_t* pairs = nullptr;
+  uint8_t* ccls = nullptr;
+  int32_t* tile_table = nullptr;
+  SurfacePlan plan{};
+  uint32_t nseam = 0, nnear = 0, nring = 0;
+  CollideParams fine{};
+  float c2f = 0.f;  // non-equilibrium rescale from level m - 1
+  int nactive = 0;
+  std::vector<int32_t> h_block_slot, h_slot_to_block;  // host copies (topology checks)
+};
+
 struct lbm_amr {
   lbm_amr_desc d{};
+  int levels = 2;                // including L0
+  std::vector<DeepLevel> deep;   // levels 2 .. levels - 1
+  std::vector<int32_t> h_block_slot1;  // level 1's uploaded block map (host copy)
   lbm_solve* l0 = nullptr;
   hipStream_t stream = nullptr;
   PoolGeom G{};
@@ -61,6 +88,7 @@ struct lbm_amr {
   long long step = 0;
   bool seeded = false;
   bool body_covered = false;  // the uploaded topology holds the body (check_topology)
+  bool has_body = false;      // a pinned sphere, solved on the finest level
   uint8_t* geom = nullptr;    // per block (manager with a body): geometry_mask
 };
 
@@ -116,6 +144,25 @@ int validate(const lbm_amr_desc* d) {
   // create (main-3d.js:1333-1336)
   if (d->reflux && !d->interface_interp)
     return fail(LBM_E_INVALID, "reflux needs the interp interface: explode/coalesce is conservative by construction");
+  const int levels = d->levels == 0 ? 2 : d->levels;
+  if (levels < 2 || levels > 4) return fail(LBM_E_INVALID, "levels %d: 2, 3 or 4", d->levels);
+  if (levels >= 3) {
+    // the pool-parent coupling is explode/coalesce (main-3d.js:1283-1288)
+    if (d->interface_interp) return fail(LBM_E_REFUSED, "levels %d needs the explode interface", levels);
+    if (d->manage_every > 0)
+      return fail(LBM_E_INVALID, "levels %d: the hierarchy is static here (manage_every = 0)", levels);
+    // explode's ring plan packs parent-grid coordinates in 10 bits each
+    const int s = 1 << (levels - 2);
+    if ((long)d->nx * s > 1024 || (long)d->ny * s > 1024 || (long)d->nz * s > 1024)
+      return fail(LBM_E_INVALID, "levels %d: level %d's parent grid exceeds 1024 cells per axis", levels, levels - 1);
+    // every level but the finest is a parent whose rescale divides by tau - 1
+    double tau = 1.0 / d->omega;
+    for (int m = 1; m < levels - 1; m++) {
+      tau = 2.0 * tau - 0.5;
+      if (std::fabs(tau - 1.0) < 1e-3)
+        return fail(LBM_E_REFUSED, "tau at level %d = %g: the non-equilibrium rescale is singular at tau = 1", m, tau);
+    }
+  }
   if (d->body.shape != LBM_BODY_NONE) {
     if (d->body.shape != LBM_BODY_SPHERE) return fail(LBM_E_INVALID, "AMR body shape %d: only a sphere", d->body.shape);
     if (!(d->body.radius > 0.f)) return fail(LBM_E_INVALID, "AMR body radius %g", d->body.radius);
@@ -215,6 +262,12 @@ void release(lbm_amr* a) {
                   a->force_partials, a->history, a->written, a->geom, a->tile_table};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (DeepLevel& L : a->deep) {
+    void* dp[] = {L.pool[0], L.pool[1], L.mac_pool, L.block_slot, L.slot_to_block, L.active, L.pairs, L.ccls,
+                  L.tile_table, L.plan.seam, L.plan.near, L.plan.ring, L.plan.count};
+    for (void* p : dp)
+      if (p) (void)hipFree(p);
+  }
   if (a->l0) lbm_solve_destroy(a->l0);
   delete a;
 }
@@ -229,10 +282,33 @@ int dalloc(T** p, size_t bytes) {
   return LBM_OK;
 }
 
+// L0 as the parent of level 1: population buffer g, its moments.
+ParentRef dense_parent(lbm_amr* a, float* g) {
+  ParentRef P{};
+  P.g = g;
+  P.mac = solve_mac(a->l0);
+  return P;
+}
+
+// Pool level m >= 1 as the parent of level m + 1, its population buffer p.
+ParentRef pool_parent(lbm_amr* a, int m, int p) {
+  ParentRef P{};
+  const PoolGeom& G = m == 1 ? a->G : a->deep[m - 2].G;
+  P.g = m == 1 ? a->pool[p] : a->deep[m - 2].pool[p];
+  P.mac = m == 1 ? a->mac_pool : a->deep[m - 2].mac_pool;
+  P.block_slot = G.block_slot;
+  P.nbx = G.nbx;
+  P.nby = G.nby;
+  P.nbz = G.nbz;
+  P.plane = G.pool_plane;
+  return P;
+}
+
 // Fills the tiles listed in `pairs` ([count, (block, slot)...], device) from
 // the current L0 state.
 int fill_tiles(lbm_amr* a, const int32_t* pairs, int n) {
-  HIPCHK(launch_interp(a->d.q, solve_lattice(a->l0, 0), a->pool[0], a->G, pairs, n, a->c2f, 0, a->stream));
+  HIPCHK(launch_interp(a->d.q, dense_parent(a, solve_lattice(a->l0, 0)), a->pool[0], a->G, pairs, n, a->c2f, 0,
+                       a->stream));
   return LBM_OK;
 }
 
@@ -240,7 +316,8 @@ int fill_tiles(lbm_amr* a, const int32_t* pairs, int n) {
 // current block maps and active list (a->nactive must be current).
 int rebuild_plan(lbm_amr* a) {
   HIPCHK(hipMemsetAsync(a->plan.count, 0, 16, a->stream));
-  HIPCHK(launch_cell_class(a->d.q, a->ccls, a->G, a->plan, a->active, a->nactive, a->stream));
+  HIPCHK(launch_cell_class(a->d.q, a->ccls, a->G, a->plan, a->active, a->nactive, dense_parent(a, nullptr),
+                           a->stream));
   uint32_t c[4] = {0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(c, a->plan.count, 16, hipMemcpyDeviceToHost, a->stream));
   HIPCHK(hipStreamSynchronize(a->stream));
@@ -265,6 +342,96 @@ int refresh_active(lbm_amr* a) {
   return LBM_OK;
 }
 
+// Rebuilds pool level L's cell classes over its parent grid and its surface
+// work lists, for its current active list.
+int rebuild_level_plan(lbm_amr* a, DeepLevel& L) {
+  HIPCHK(hipMemsetAsync(L.plan.count, 0, 16, a->stream));
+  HIPCHK(launch_cell_class(a->d.q, L.ccls, L.G, L.plan, L.active, L.nactive, pool_parent(a, L.m - 1, 0),
+                           a->stream));
+  uint32_t c[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(c, L.plan.count, 16, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  if (c[0] > L.plan.cap_seam || c[1] > L.plan.cap_near || c[2] > L.plan.cap_ring)
+    return fail(LBM_E_DEVICE, "level %d surface work lists overflow (%u/%u seam, %u/%u near, %u/%u ring)", L.m, c[0],
+                L.plan.cap_seam, c[1], L.plan.cap_near, c[2], L.plan.cap_ring);
+  L.nseam = c[0];
+  L.nnear = c[1];
+  L.nring = c[2];
+  return LBM_OK;
+}
+
+int refresh_level(lbm_amr* a, DeepLevel& L) {
+  HIPCHK(launch_active_list(L.slot_to_block, a->d.max_slots, L.active, a->stream));
+  int32_t n = 0;
+  HIPCHK(hipMemcpyAsync(&n, L.active, 4, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  L.nactive = n;
+  return rebuild_level_plan(a, L);
+}
+
+// Fills every tile of pool level L from its parent level's state (buffer 0,
+// which holds it between macro-steps), ring included.
+int fill_level(lbm_amr* a, DeepLevel& L) {
+  if (L.nactive <= 0) return LBM_OK;
+  HIPCHK(launch_active_pairs(L.active, L.slot_to_block, a->d.max_slots, L.pairs, a->stream));
+  HIPCHK(launch_interp(a->d.q, pool_parent(a, L.m - 1, 0), L.pool[0], L.G, L.pairs, L.nactive, L.c2f, 0, a->stream));
+  return LBM_OK;
+}
+
+// Host checks of a level-m (m >= 2) block map against its parent's: every
+// tile nests in a parent tile, and every ring cell's parent cell that no
+// level-m tile covers lies in a parent tile (checkRingParentCoverage,
+// d3-amr.mjs:348-390), so explode and the seam passes always find parent
+// data.  On the finest level with a body, every block within one L0 cell of
+// it must be refined (the coarser levels never see the body).
+int check_level_topology(lbm_amr* a, int m, const int32_t* bs) {
+  const lbm_amr_desc& d = a->d;
+  const int rb = d.rb, rb2 = 2 * rb, fb = rb2 + 2 * kGhost;
+  const DeepLevel& L = a->deep[m - 2];
+  const std::vector<int32_t>& pbs = m == 2 ? a->h_block_slot1 : a->deep[m - 3].h_block_slot;
+  if (pbs.empty()) return fail(LBM_E_STATE, "level %d uploaded before level %d", m, m - 1);
+  const int nbx = L.G.nbx, nby = L.G.nby, nbz = L.G.nbz;
+  const int px = L.G.nx, py = L.G.ny, pz = L.G.nz;         // parent grid
+  const int pnbx = px / rb2, pnby = py / rb2;              // parent blocks per axis
+  auto parent_has = [&](int vx, int vy, int vz) { return pbs[((vz / rb2) * pnby + vy / rb2) * pnbx + vx / rb2] >= 0; };
+  auto covered = [&](int vx, int vy, int vz) { return bs[((vz / rb) * nby + vy / rb) * nbx + vx / rb] >= 0; };
+  for (int b = 0; b < nbx * nby * nbz; b++) {
+    if (bs[b] < 0) continue;
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    if (!parent_has(bx * rb, by * rb, bz * rb))
+      return fail(LBM_E_INVALID, "level %d block %d does not nest in a level %d tile", m, b, m - 1);
+    for (int f = 0; f < fb * fb * fb; f++) {
+      const int fx = f % fb, fy = (f / fb) % fb, fz = f / (fb * fb);
+      if (fx >= kGhost && fx < kGhost + rb2 && fy >= kGhost && fy < kGhost + rb2 && fz >= kGhost && fz < kGhost + rb2)
+        continue;
+      const int gx = ((bx * rb2 + fx - kGhost) % (2 * px) + 2 * px) % (2 * px);
+      const int gy = ((by * rb2 + fy - kGhost) % (2 * py) + 2 * py) % (2 * py);
+      const int gz = ((bz * rb2 + fz - kGhost) % (2 * pz) + 2 * pz) % (2 * pz);
+      const int vx = gx / 2, vy = gy / 2, vz = gz / 2;
+      if (!covered(vx, vy, vz) && !parent_has(vx, vy, vz))
+        return fail(LBM_E_INVALID, "level %d block %d: a ring cell's parent cell lies in no level %d tile", m, b, m - 1);
+    }
+  }
+  if (a->has_body && m == a->levels - 1) {
+    // the block's box of parent-cell centres in L0 units, +- half a cell
+    const double sc = std::ldexp(1.0, -(m - 1)), off = -0.5 * (1.0 - sc);
+    for (int b = 0; b < nbx * nby * nbz; b++) {
+      if (bs[b] >= 0) continue;
+      const int bc[3] = {b % nbx, (b / nbx) % nby, b / (nbx * nby)};
+      double s2 = 0.0;
+      for (int k = 0; k < 3; k++) {
+        const double lo = (bc[k] * rb) * sc + off - 0.5 * sc, hi = (bc[k] * rb + rb - 1) * sc + off + 0.5 * sc;
+        const double c = d.body.centre[k];
+        const double e = c < lo ? lo - c : (c > hi ? c - hi : 0.0);
+        s2 += e * e;
+      }
+      if (std::sqrt(s2) - d.body.radius < 1.0)
+        return fail(LBM_E_INVALID, "level %d block %d lies within one L0 cell of the body and is not refined", m, b);
+    }
+  }
+  return LBM_OK;
+}
+
 // Restricts every tile in use onto the L0 cells under it (the average/drain
 // pass).  Between manager passes the covered L0 step leaves those cells
 // untouched, so this is what makes them a valid interpolation source for a
@@ -272,7 +439,7 @@ int refresh_active(lbm_amr* a) {
 int restrict_active(lbm_amr* a) {
   if (a->nactive <= 0) return LBM_OK;
   HIPCHK(launch_active_pairs(a->active, a->slot_to_block, a->d.max_slots, a->pairs, a->stream));
-  HIPCHK(launch_average(a->d.q, a->pool[0], solve_lattice(a->l0, 0), solve_mac(a->l0), a->G, a->pairs, a->nactive,
+  HIPCHK(launch_average(a->d.q, a->pool[0], dense_parent(a, solve_lattice(a->l0, 0)), a->G, a->pairs, a->nactive,
                         a->f2c, a->stream));
   return LBM_OK;
 }
@@ -334,17 +501,49 @@ int macro_step_interp(lbm_amr* a) {
   float* ct1 = solve_lattice(a->l0, 0);       // the one it wrote (t + 1)
   if (a->nactive > 0) {
     HIPCHK(launch_active_pairs(a->active, a->slot_to_block, d.max_slots, a->pairs, a->stream));
-    HIPCHK(launch_interp(d.q, ct, a->pool[0], a->G, a->pairs, a->nactive, a->c2f, 1, a->stream));
+    HIPCHK(launch_interp(d.q, dense_parent(a, const_cast<float*>(ct)), a->pool[0], a->G, a->pairs, a->nactive, a->c2f,
+                         1, a->stream));
     if (d.reflux) HIPCHK(launch_flux(d.q, a->pool[0], a->flux_acc, a->G, a->plan, a->nflux, 0, a->stream));
     HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->nactive, 1,
                             a->stream));
     if (d.reflux) HIPCHK(launch_flux(d.q, a->pool[1], a->flux_acc, a->G, a->plan, a->nflux, 1, a->stream));
     HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->nactive, 1,
                             a->stream));
-    HIPCHK(launch_average(d.q, a->pool[0], ct1, solve_mac(a->l0), a->G, a->pairs, a->nactive, a->f2c, a->stream));
+    HIPCHK(launch_average(d.q, a->pool[0], dense_parent(a, ct1), a->G, a->pairs, a->nactive, a->f2c, a->stream));
     if (d.reflux)
       HIPCHK(launch_reflux(d.q, ct, ct1, solve_mac(a->l0), a->G, a->plan, a->nflux, a->flux_acc, d.force,
                            a->stream));
+  }
+  return LBM_OK;
+}
+
+// One substep of pool level m, reading its buffer p (advanceLevel,
+// main-3d.js:3169-3180): with a finer level, that level's ring is exploded
+// from this state, the finer level takes its two substeps and is coalesced
+// back into this state; then this level steps.  Only the second substep's
+// moments are written (the parent's coalesce and the readbacks take them).
+int advance(lbm_amr* a, int m, int p) {
+  const lbm_amr_desc& d = a->d;
+  if (m + 1 < a->levels) {
+    DeepLevel& c = a->deep[m - 1];
+    if (c.nactive > 0) {
+      const ParentRef P = pool_parent(a, m, p);
+      HIPCHK(launch_explode(d.q, P, c.pool[0], c.G, c.plan, c.nring, d.explode_linear, d.explode_dirs, a->stream));
+      int rc = advance(a, m + 1, 0);
+      if (rc) return rc;
+      rc = advance(a, m + 1, 1);
+      if (rc) return rc;
+      HIPCHK(launch_coalesce(d.q, c.pool[0], P, c.mac_pool, c.G, c.active, c.nactive, c.plan, c.nseam, c.nnear,
+                             d.orphans, a->stream));
+    }
+  }
+  if (m == 1) {
+    HIPCHK(launch_pool_step(d.q, a->pool[p], a->pool[p ^ 1], p ? a->mac_pool : nullptr, a->G, a->fine, a->nactive, 0,
+                            a->stream));
+  } else {
+    DeepLevel& L = a->deep[m - 2];
+    HIPCHK(launch_pool_step(d.q, L.pool[p], L.pool[p ^ 1], p ? L.mac_pool : nullptr, L.G, L.fine, L.nactive, 0,
+                            a->stream));
   }
   return LBM_OK;
 }
@@ -362,22 +561,96 @@ int macro_step(lbm_amr* a) {
     return LBM_OK;
   }
   float* l0 = solve_lattice(a->l0, 0);
-  if (a->G.body.on) {
-    // the one force pass of the macro-step, on the fine state at t
+  if (a->has_body) {
+    // the one force pass of the macro-step, on the finest level's state at t
     // (main-3d.js:3127-3141, 1596-1597)
-    HIPCHK(launch_pool_force(d.q, a->pool[0], a->G, a->active, a->nactive, a->force_partials, a->stream));
-    ForceFinalizeArgs fa{a->force_partials, pool_force_blocks(d.rb, a->nactive), a->history, a->written, kForceCap};
+    const bool deep = a->levels > 2;
+    const PoolGeom& G = deep ? a->deep.back().G : a->G;
+    const int n = deep ? a->deep.back().nactive : a->nactive;
+    HIPCHK(launch_pool_force(d.q, deep ? a->deep.back().pool[0] : a->pool[0], G,
+                             deep ? a->deep.back().active : a->active, n, a->force_partials, a->stream));
+    ForceFinalizeArgs fa{a->force_partials, pool_force_blocks(d.rb, n), a->history, a->written, kForceCap};
     HIPCHK(launch_force_finalize(fa, a->stream));
   }
-  HIPCHK(launch_explode(d.q, l0, a->pool[0], a->G, a->plan, a->nring, d.explode_linear, d.explode_dirs, a->stream));
-  // only the second substep's field is read (coalesce, lbm_amr_read_pool_macro)
-  HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->nactive, 0, a->stream));
-  HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->nactive, 0, a->stream));
-  HIPCHK(launch_coalesce(d.q, a->pool[0], l0, solve_mac(a->l0), a->mac_pool, a->G, a->active, a->nactive, a->plan,
-                         a->nseam, a->nnear, d.orphans, a->stream));
+  const ParentRef P0 = dense_parent(a, l0);
+  HIPCHK(launch_explode(d.q, P0, a->pool[0], a->G, a->plan, a->nring, d.explode_linear, d.explode_dirs, a->stream));
+  int rc0 = advance(a, 1, 0);
+  if (rc0) return rc0;
+  rc0 = advance(a, 1, 1);
+  if (rc0) return rc0;
+  HIPCHK(launch_coalesce(d.q, a->pool[0], P0, a->mac_pool, a->G, a->active, a->nactive, a->plan, a->nseam, a->nnear,
+                         d.orphans, a->stream));
   int rc = solve_enqueue_covered_step(a->l0, a->block_slot, d.rb, a->G.nbx, a->G.nby);
   if (rc) return rc;
   a->step++;
+  return LBM_OK;
+}
+
+// Pool levels 2 .. levels - 1: level m refines level m - 1's global grid
+// (L0's times 2^(m-1)) in blocks of rb, with tau_m = 2 tau_(m-1) - 1/2 and
+// the same tile shape and slot count as level 1.  Each level's step skips
+// the interior cells its child covers.
+int create_deep_levels(lbm_amr* a, double tau1) {
+  const lbm_amr_desc& d = a->d;
+  double tau_p = tau1;
+  for (int m = 2; m < a->levels; m++) {
+    a->deep.emplace_back();
+    DeepLevel& L = a->deep.back();
+    L.m = m;
+    const int s = 1 << (m - 1);
+    PoolGeom& G = L.G;
+    G.nx = d.nx * s; G.ny = d.ny * s; G.nz = d.nz * s;
+    G.rb = d.rb; G.fb = a->fb;
+    G.nbx = G.nx / d.rb; G.nby = G.ny / d.rb; G.nbz = G.nz / d.rb;
+    G.ncells = (uint32_t)((size_t)G.nx * G.ny * G.nz);
+    G.max_slots = d.max_slots;
+    G.pool_plane = a->G.pool_plane;
+    G.l0_scale = (float)std::ldexp(1.0, -(m - 1));
+    G.l0_offset = (float)(-0.5 * (1.0 - std::ldexp(1.0, -(m - 1))));
+    G.dx2 = (float)std::ldexp(1.0, -2 * m);
+    L.nblocks = G.nbx * G.nby * G.nbz;
+    L.pool_cells = a->pool_cells;
+    plan_capacity(d.rb, d.max_slots, G.ncells, &L.plan.cap_seam, &L.plan.cap_near, &L.plan.cap_ring);
+    const size_t pool_bytes = (size_t)G.pool_plane * d.q * 4;
+    int rc;
+    if ((rc = dalloc(&L.pool[0], pool_bytes)) || (rc = dalloc(&L.pool[1], pool_bytes)) ||
+        (rc = dalloc(&L.mac_pool, (size_t)G.pool_plane * 16)) || (rc = dalloc(&L.block_slot, (size_t)L.nblocks * 4)) ||
+        (rc = dalloc(&L.slot_to_block, (size_t)d.max_slots * 4)) ||
+        (rc = dalloc(&L.active, (size_t)(d.max_slots + 1) * 4)) ||
+        (rc = dalloc(&L.pairs, (size_t)(2 * d.max_slots + 1) * 4)) || (rc = dalloc(&L.ccls, (size_t)G.ncells)) ||
+        (rc = dalloc(&L.tile_table, (size_t)d.max_slots * kTileWords * 4)) || (rc = dalloc(&L.plan.count, 16)) ||
+        (rc = dalloc(&L.plan.seam, (size_t)L.plan.cap_seam * 4)) ||
+        (rc = dalloc(&L.plan.near, (size_t)L.plan.cap_near * 4)) ||
+        (rc = dalloc(&L.plan.ring, (size_t)L.plan.cap_ring * 16)))
+      return rc;
+    G.block_slot = L.block_slot;
+    G.slot_to_block = L.slot_to_block;
+    G.ccls = L.ccls;
+    G.tile_table = L.tile_table;
+    HIPCHK(hipMemsetAsync(L.block_slot, 0xff, (size_t)L.nblocks * 4, a->stream));
+    HIPCHK(hipMemsetAsync(L.slot_to_block, 0xff, (size_t)d.max_slots * 4, a->stream));
+    HIPCHK(hipMemsetAsync(L.active, 0, 4, a->stream));
+    HIPCHK(hipMemsetAsync(L.mac_pool, 0, (size_t)G.pool_plane * 16, a->stream));
+    HIPCHK(hipMemsetAsync(L.ccls, 0, (size_t)G.ncells, a->stream));
+    const double tau = 2.0 * tau_p - 0.5;
+    if (d.q == 19) fine_params<19>(L.fine, (float)(1.0 / tau), (float)(1.0 / tau), d.force);
+    else fine_params<27>(L.fine, (float)(1.0 / tau), (float)(1.0 / tau), d.force);
+    L.c2f = (float)(0.5 * (tau - 1.0) / (tau_p - 1.0));
+    tau_p = tau;
+  }
+  // each level knows its child's block map (coveredByChild)
+  for (int m = 1; m + 1 < a->levels; m++) {
+    PoolGeom& G = m == 1 ? a->G : a->deep[m - 2].G;
+    const DeepLevel& c = a->deep[m - 1];
+    G.child_block_slot = c.block_slot;
+    G.cnbx = c.G.nbx;
+    G.cnby = c.G.nby;
+  }
+  if (a->has_body && a->levels > 2) {
+    a->deep.back().G.body =
+        PoolBody{1, d.body.centre[0], d.body.centre[1], d.body.centre[2], d.body.radius};
+  }
+  HIPCHK(hipStreamSynchronize(a->stream));
   return LBM_OK;
 }
 
@@ -447,8 +720,15 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
     release(a);
     return rc;
   }
+  a->levels = desc->levels == 0 ? 2 : desc->levels;
+  G.l0_scale = 1.f;
+  G.l0_offset = 0.f;
+  G.dx2 = 0.25f;
   if (desc->body.shape == LBM_BODY_SPHERE) {
-    G.body = PoolBody{1, desc->body.centre[0], desc->body.centre[1], desc->body.centre[2], desc->body.radius};
+    a->has_body = true;
+    // the body lives on the finest level alone (main-3d.js:1577-1597)
+    if (a->levels == 2)
+      G.body = PoolBody{1, desc->body.centre[0], desc->body.centre[1], desc->body.centre[2], desc->body.radius};
     if ((rc = dalloc(&a->force_partials, (size_t)pool_force_blocks(rb, desc->max_slots) * 48)) ||
         (rc = dalloc(&a->history, (size_t)kForceCap * 48)) || (rc = dalloc(&a->written, 8))) {
       release(a);
@@ -500,6 +780,10 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
   // post-collision non-equilibrium rescale (common_d3_pool.wgsl:172-187)
   a->c2f = (float)(0.5 * (tf - 1.0) / (tc - 1.0));
   a->f2c = (float)(2.0 * (tc - 1.0) / (tf - 1.0));
+  if ((rc = create_deep_levels(a, tf))) {
+    release(a);
+    return rc;
+  }
   *out = a;
   return LBM_OK;
 }
@@ -525,7 +809,9 @@ int lbm_amr_upload_topology(lbm_amr* a, const int32_t* block_slot, const int32_t
   }
   int rc_geo = check_topology(a->d, block_slot);
   if (rc_geo) return rc_geo;
-  a->body_covered = true;
+  // with finer levels, the finest one's upload is what places the body
+  a->body_covered = a->levels == 2;
+  a->h_block_slot1.assign(block_slot, block_slot + a->nblocks);
   if (a->seeded) {
     int rc0 = restrict_active(a);
     if (rc0) return rc0;
@@ -534,6 +820,9 @@ int lbm_amr_upload_topology(lbm_amr* a, const int32_t* block_slot, const int32_t
   HIPCHK(hipMemcpyAsync(a->slot_to_block, slot_to_block, (size_t)a->d.max_slots * 4, hipMemcpyHostToDevice, a->stream));
   int rc = refresh_active(a);
   if (rc) return rc;
+  // finer levels' parent cells live in these tiles: rebuild their lists
+  for (DeepLevel& D : a->deep)
+    if (!D.h_block_slot.empty() && (rc = refresh_level(a, D))) return rc;
   if (a->seeded) {
     // fill every tile of the new topology from the current L0 state
     std::vector<int32_t> pairs(1 + 2 * a->nactive);
@@ -546,6 +835,41 @@ int lbm_amr_upload_topology(lbm_amr* a, const int32_t* block_slot, const int32_t
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(a->stream));
   }
+  return LBM_OK;
+}
+
+int lbm_amr_upload_level_topology(lbm_amr* a, int32_t level, const int32_t* block_slot,
+                                  const int32_t* slot_to_block) {
+  if (!a || !block_slot || !slot_to_block) return fail(LBM_E_INVALID, "null argument");
+  if (level == 1) return lbm_amr_upload_topology(a, block_slot, slot_to_block);
+  if (level < 1 || level >= a->levels) return fail(LBM_E_INVALID, "level %d outside 1 .. %d", level, a->levels - 1);
+  DeepLevel& L = a->deep[level - 2];
+  for (int b = 0; b < L.nblocks; b++) {
+    const int s = block_slot[b];
+    if (s >= a->d.max_slots || (s >= 0 && slot_to_block[s] != b))
+      return fail(LBM_E_INVALID, "level %d blockSlot[%d] = %d is not inverted by slotToBlock", level, b, s);
+  }
+  for (int s = 0; s < a->d.max_slots; s++) {
+    const int b = slot_to_block[s];
+    if (b >= L.nblocks || (b >= 0 && block_slot[b] != s))
+      return fail(LBM_E_INVALID, "level %d slotToBlock[%d] = %d is not inverted by blockSlot", level, s, b);
+  }
+  int rc = check_level_topology(a, level, block_slot);
+  if (rc) return rc;
+  L.h_block_slot.assign(block_slot, block_slot + L.nblocks);
+  L.h_slot_to_block.assign(slot_to_block, slot_to_block + a->d.max_slots);
+  HIPCHK(hipMemcpyAsync(L.block_slot, block_slot, (size_t)L.nblocks * 4, hipMemcpyHostToDevice, a->stream));
+  HIPCHK(hipMemcpyAsync(L.slot_to_block, slot_to_block, (size_t)a->d.max_slots * 4, hipMemcpyHostToDevice,
+                        a->stream));
+  // this level's lists, and its children's (whose parent cells changed)
+  for (int m = level; m < a->levels; m++) {
+    DeepLevel& D = a->deep[m - 2];
+    if (m > level && D.h_block_slot.empty()) continue;
+    if ((rc = refresh_level(a, D))) return rc;
+    if (a->seeded && (rc = fill_level(a, D))) return rc;
+  }
+  if (level == a->levels - 1) a->body_covered = true;
+  HIPCHK(hipStreamSynchronize(a->stream));
   return LBM_OK;
 }
 
@@ -577,13 +901,19 @@ int lbm_amr_upload_macro(lbm_amr* a, const float* mac4) {
   rc = fill_tiles(a, a->fresh, n);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(a->stream));
-  return refresh_active(a);
+  rc = refresh_active(a);
+  if (rc) return rc;
+  // each finer level from its just-filled parent
+  for (DeepLevel& L : a->deep)
+    if ((rc = fill_level(a, L))) return rc;
+  HIPCHK(hipStreamSynchronize(a->stream));
+  return LBM_OK;
 }
 
 int lbm_amr_step(lbm_amr* a, int32_t n) {
   if (!a) return fail(LBM_E_INVALID, "null argument");
   if (!a->seeded) return fail(LBM_E_STATE, "step before upload_macro");
-  if (a->G.body.on && !a->body_covered && a->d.manage_every <= 0)
+  if (a->has_body && !a->body_covered && a->d.manage_every <= 0)
     return fail(LBM_E_STATE, "step with a body before a topology that holds it");
   for (int s = 0; s < n; s++) {
     int rc = macro_step(a);
@@ -635,6 +965,34 @@ int lbm_amr_read_f(lbm_amr* a, float* f_l0, float* f_pool) {
   return LBM_OK;
 }
 
+int lbm_amr_read_level_f(lbm_amr* a, int32_t level, float* f_pool) {
+  if (!a || !f_pool) return fail(LBM_E_INVALID, "null argument");
+  if (level < 1 || level >= a->levels) return fail(LBM_E_INVALID, "level %d outside 1 .. %d", level, a->levels - 1);
+  if (level == 1) return lbm_amr_read_f(a, nullptr, f_pool);
+  const DeepLevel& L = a->deep[level - 2];
+  const size_t n = a->pool_cells;
+  HIPCHK(hipStreamSynchronize(a->stream));
+  for (int i = 0; i < a->d.q; i++) {
+    HIPCHK(hipMemcpy(f_pool + (size_t)i * n, L.pool[0] + (size_t)i * L.G.pool_plane, n * 4, hipMemcpyDeviceToHost));
+    const float w = a->d.q == 19 ? Lattice<19>::w[i] : Lattice<27>::w[i];
+    float* p = f_pool + (size_t)i * n;
+    for (size_t c = 0; c < n; c++) p[c] += w;
+  }
+  return LBM_OK;
+}
+
+int lbm_amr_read_level_macro(lbm_amr* a, int32_t level, float* mac_pool) {
+  if (!a || !mac_pool) return fail(LBM_E_INVALID, "null argument");
+  if (level < 1 || level >= a->levels) return fail(LBM_E_INVALID, "level %d outside 1 .. %d", level, a->levels - 1);
+  if (level == 1) return lbm_amr_read_pool_macro(a, mac_pool);
+  const DeepLevel& L = a->deep[level - 2];
+  HIPCHK(hipStreamSynchronize(a->stream));
+  for (int k = 0; k < 4; k++)
+    HIPCHK(hipMemcpy(mac_pool + (size_t)k * a->pool_cells, L.mac_pool + (size_t)k * L.G.pool_plane,
+                     a->pool_cells * 4, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
 int lbm_amr_read_criterion(lbm_amr* a, float* crit, uint8_t* want) {
   if (!a) return fail(LBM_E_INVALID, "null argument");
   if (crit) HIPCHK(hipMemcpy(crit, a->crit, (size_t)a->nblocks * 4, hipMemcpyDeviceToHost));
@@ -674,6 +1032,16 @@ int lbm_amr_read_totals(lbm_amr* a, double* out4) {
                             a->totals, a->stream));
   HIPCHK(hipMemcpyAsync(out4, a->totals, 32, hipMemcpyDeviceToHost, a->stream));
   HIPCHK(hipStreamSynchronize(a->stream));
+  // finer levels in order, each weighted 8^-m, summed here in FP64
+  for (const DeepLevel& L : a->deep) {
+    if (L.nactive <= 0) continue;
+    double v[4];
+    HIPCHK(launch_level_totals(a->d.q, L.pool[0], L.G, L.active, std::ldexp(1.0, -3 * L.m), a->partials, a->counter,
+                               a->totals, a->stream));
+    HIPCHK(hipMemcpyAsync(v, a->totals, 32, hipMemcpyDeviceToHost, a->stream));
+    HIPCHK(hipStreamSynchronize(a->stream));
+    for (int k = 0; k < 4; k++) out4[k] += v[k];
+  }
   return LBM_OK;
 }
 

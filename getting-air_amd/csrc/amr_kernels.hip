@@ -96,6 +96,60 @@ __device__ __forceinline__ int local_in_tile(int g, int b, int nfine) {
   return d + kGhost;
 }
 
+// Parent cells: index in the parent's storage (the dense parent's cell, or
+// the pool index in the parent tile that owns the cell, kAbsent when that
+// block holds no tile), and the population / moment accessors.  (x, y, z) is
+// wrapped into the parent grid.
+constexpr uint32_t kAbsent = 0xffffffffu;
+template <bool PP, int RB>
+__device__ __forceinline__ uint32_t parent_index(const PoolGeom& G, const ParentRef& P, int x, int y, int z) {
+  if constexpr (!PP) {
+    return coarse_cell(G, x, y, z);
+  } else {
+    constexpr int SH = Tile<RB>::LOG_RB + 1;
+    const int bx = x >> SH, by = y >> SH, bz = z >> SH;
+    const int slot = P.block_slot[(bz * P.nby + by) * P.nbx + bx];
+    if (slot < 0) return kAbsent;
+    return pool_cell<RB>(slot, x - (bx << SH) + kGhost, y - (by << SH) + kGhost, z - (bz << SH) + kGhost);
+  }
+}
+template <bool PP, int Q>
+__device__ __forceinline__ float* ppop(const ParentRef& P, uint32_t idx, int i) {
+  if constexpr (PP) return P.g + (size_t)i * P.plane + idx;
+  else return P.g + lat_elem(Q, i, idx);
+}
+template <bool PP>
+__device__ __forceinline__ void pmac_put(const ParentRef& P, uint32_t idx, float4 v) {
+  if constexpr (PP) {
+    P.mac[idx] = v.x;
+    P.mac[(size_t)P.plane + idx] = v.y;
+    P.mac[2 * (size_t)P.plane + idx] = v.z;
+    P.mac[3 * (size_t)P.plane + idx] = v.w;
+  } else {
+    reinterpret_cast<float4*>(P.mac)[idx] = v;
+  }
+}
+
+// L0-unit position of this level's cell j (tile-local, ring included) of a
+// block whose first parent cell is `origin`: the parent-unit position
+// origin - 1/4 + (j - GHOST)/2 (fineToCoarseUnit3), mapped to L0 by the
+// level's affine scale (1 and 0 at level 1).
+__device__ __forceinline__ float level_to_l0(const PoolGeom& G, int origin, int j) {
+  return ((float)origin - 0.25f + 0.5f * (float)(j - kGhost)) * G.l0_scale + G.l0_offset;
+}
+
+// Is this level's interior cell, at its global coordinate (gx, gy, gz) of
+// this level's grid (2 nx wide), under a tile of the next finer level.
+template <int RB>
+__device__ __forceinline__ bool covered_by_child(const PoolGeom& G, int gx, int gy, int gz) {
+  if (!G.child_block_slot) return false;
+  gx = wrap1(gx, 2 * G.nx);
+  gy = wrap1(gy, 2 * G.ny);
+  gz = wrap1(gz, 2 * G.nz);
+  constexpr int SH = Tile<RB>::LOG_RB;
+  return G.child_block_slot[((gz >> SH) * G.cnby + (gy >> SH)) * G.cnbx + (gx >> SH)] >= 0;
+}
+
 // Velocity and weight of direction i when i is uniform over the block (a
 // surface pass launched with one grid row per direction): compares and
 // selects on the scalar unit instead of a table indexed at run time.
@@ -106,12 +160,6 @@ __device__ __forceinline__ void direction(int i, int& ex, int& ey, int& ez, floa
 #pragma unroll
   for (int k = 1; k < Q; k++)
     if (i == k) ex = L::ex[k], ey = L::ey[k], ez = L::ez[k], w = L::w[k];
-}
-
-// Centre of fine cell j (tile-local, ring included) of a block whose first L0
-// cell is `origin`, in L0 units (fineToCoarseUnit, d3-amr.mjs:45-47).
-__device__ __forceinline__ float fine_to_l0(int origin, int j) {
-  return (float)origin - 0.25f + 0.5f * (float)(j - kGhost);
 }
 
 // Signed distance to the pinned sphere in binary32, evaluated as the dense
@@ -129,8 +177,9 @@ constexpr float kBodyReach = 1.f;
 // ---------------------------------------------------------------------------
 // L0 cell classes
 // ---------------------------------------------------------------------------
-template <int Q, int RB>
-__global__ void __launch_bounds__(256) k_cell_class(uint8_t* __restrict__ ccls, PoolGeom G, SurfacePlan plan) {
+template <int Q, int RB, bool PP>
+__global__ void __launch_bounds__(256) k_cell_class(uint8_t* __restrict__ ccls, PoolGeom G, SurfacePlan plan,
+                                                    ParentRef P) {
   using L = Lattice<Q>;
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= G.ncells) return;
@@ -146,6 +195,8 @@ __global__ void __launch_bounds__(256) k_cell_class(uint8_t* __restrict__ ccls, 
 #pragma unroll
   for (int i = 1; i < Q; i++) other = other || (cov(x + L::ex[i], y + L::ey[i], z + L::ez[i]) != self);
   ccls[c] = self ? (uint8_t)(kCovered | (other ? kSeam : 0)) : (uint8_t)(other ? kNear : 0);
+  // a pool parent's cells outside its tiles hold no data: never listed
+  if (PP && parent_index<PP, RB>(G, P, x, y, z) == kAbsent) return;
   if (plan.flux && !self) {
     // a refined block among the blocks the 26 neighbours span
     // (common_d3_amr_flux.wgsl:101-122): any covered cell in the 3x3x3 box
@@ -176,9 +227,9 @@ __global__ void __launch_bounds__(256) k_cell_class(uint8_t* __restrict__ ccls, 
 //       L0 neighbours of v along x / y / z are both unrefined (the linear
 //       reconstruction's gradient along that axis is taken)
 //   .w  v's coordinates, x | y << 10 | z << 20
-template <int Q, int RB>
+template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256) k_ring_plan(PoolGeom G, SurfacePlan plan, const int32_t* __restrict__ active,
-                                                   int nactive) {
+                                                   int nactive, ParentRef P) {
   using L = Lattice<Q>;
   using T = Tile<RB>;
   constexpr int FB = T::FB, RB2 = T::RB2, LO = T::LO, HI = T::HI;
@@ -196,16 +247,27 @@ __global__ void __launch_bounds__(256) k_ring_plan(PoolGeom G, SurfacePlan plan,
   const int gz = wrap1(bz * RB2 + fz - kGhost, 2 * G.nz);
   const int vx = gx >> 1, vy = gy >> 1, vz = gz >> 1;
   if (G.ccls[coarse_cell(G, vx, vy, vz)] & kCovered) return;
+  // the parent cell always lies in a parent tile (the ring-parent coverage
+  // the topology checks guarantee; checkRingParentCoverage, d3-amr.mjs:348)
+  const uint32_t vc = parent_index<PP, RB>(G, P, vx, vy, vz);
+  if (vc == kAbsent) return;
+  // a gradient axis also needs both parent neighbours present; where one is
+  // missing the explosion is uniform along it (parentPresent,
+  // common_d3_parent_pool.wgsl:31-35)
+  auto open = [&](int x, int y, int z) {
+    return !covered_at(G, x, y, z) &&
+           (!PP || parent_index<PP, RB>(G, P, wrap1(x, G.nx), wrap1(y, G.ny), wrap1(z, G.nz)) != kAbsent);
+  };
   uint32_t bits = 0;
 #pragma unroll
   for (int i = 0; i < Q; i++)
     if (covered_at(G, vx + L::ex[i], vy + L::ey[i], vz + L::ez[i])) bits |= 1u << i;
-  if (!covered_at(G, vx - 1, vy, vz) && !covered_at(G, vx + 1, vy, vz)) bits |= 1u << 27;
-  if (!covered_at(G, vx, vy - 1, vz) && !covered_at(G, vx, vy + 1, vz)) bits |= 1u << 28;
-  if (!covered_at(G, vx, vy, vz - 1) && !covered_at(G, vx, vy, vz + 1)) bits |= 1u << 29;
+  if (open(vx - 1, vy, vz) && open(vx + 1, vy, vz)) bits |= 1u << 27;
+  if (open(vx, vy - 1, vz) && open(vx, vy + 1, vz)) bits |= 1u << 28;
+  if (open(vx, vy, vz - 1) && open(vx, vy, vz + 1)) bits |= 1u << 29;
   const uint32_t e = atomicAdd(&plan.count[2], 1u);
   if (e < plan.cap_ring)
-    plan.ring[e] = make_uint4((uint32_t)slot * T::CELLS + r, coarse_cell(G, vx, vy, vz), bits,
+    plan.ring[e] = make_uint4((uint32_t)slot * T::CELLS + r, vc, bits,
                               (uint32_t)vx | (uint32_t)vy << 10 | (uint32_t)vz << 20);
 }
 
@@ -310,13 +372,15 @@ __device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, fl
   }
   float phi = 1e30f;
   if (BODY) {
-    const float px = fine_to_l0(bx * RB, fx), py = fine_to_l0(by * RB, fy), pz = fine_to_l0(bz * RB, fz);
+    // positions in L0 units (exact: every term is a small dyadic rational)
+    const float px = level_to_l0(G, bx * RB, fx), py = level_to_l0(G, by * RB, fy), pz = level_to_l0(G, bz * RB, fz);
     phi = sphere_phi(G.body, px, py, pz);
     if (phi < kBodyReach) {
 #pragma unroll
       for (int i = 1; i < Q; i++) {
-        const float sx = px - 0.5f * (float)L::ex[i], sy = py - 0.5f * (float)L::ey[i],
-                    sz = pz - 0.5f * (float)L::ez[i];
+        // the geometric source, one cell of this level back
+        const float sx = level_to_l0(G, bx * RB, fx - L::ex[i]), sy = level_to_l0(G, by * RB, fy - L::ey[i]),
+                    sz = level_to_l0(G, bz * RB, fz - L::ez[i]);
         if (sphere_phi(G.body, sx, sy, sz) < 0.f) g[i] = *at(gin + (size_t)L::opp[i] * G.pool_plane, cell);
       }
     }
@@ -368,6 +432,15 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
   const int32_t* tt = G.tile_table + (size_t)k * kTileWords;
   const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
   const bool interior = fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI;
+  // An interior cell a finer level solves is left alone, populations and
+  // moments both: the child's coalesce wrote its outbox and moments into
+  // this time level (coveredByChild, common_d3_amr_step1.wgsl:107-143).  Ring
+  // cells keep advecting whatever covers them: they are this level's ghost
+  // for its own parent.
+  if (interior && G.child_block_slot &&
+      covered_by_child<RB>(G, tt[28] * T::RB2 + fx - kGhost, tt[29] * T::RB2 + fy - kGhost,
+                           tt[30] * T::RB2 + fz - kGhost))
+    return;
   // Ring cells advect and store without colliding (common_d3_amr_step1.wgsl
   // COLLIDE_RING = 0): coalesce averages uncollided interface states.  The
   // interp interface collides them too (COLLIDE_RING = 1, main-3d.js:1395).
@@ -397,15 +470,15 @@ k_pool_force(const float* __restrict__ gp, PoolGeom G, const int32_t* __restrict
     const int slot = active[1 + k];
     const int id = G.slot_to_block[slot];
     const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
-    const float px = fine_to_l0(bx * RB, fx), py = fine_to_l0(by * RB, fy), pz = fine_to_l0(bz * RB, fz);
+    const float px = level_to_l0(G, bx * RB, fx), py = level_to_l0(G, by * RB, fy), pz = level_to_l0(G, bz * RB, fz);
     const float phi = sphere_phi(G.body, px, py, pz);
     if (phi >= 0.f && phi < kBodyReach) {
       const uint32_t cell = pool_cell<RB>(slot, fx, fy, fz);
       double f0 = 0.0, f1 = 0.0, f2 = 0.0;
 #pragma unroll
       for (int i = 1; i < Q; i++) {
-        const float sx = px - 0.5f * (float)L::ex[i], sy = py - 0.5f * (float)L::ey[i],
-                    sz = pz - 0.5f * (float)L::ez[i];
+        const float sx = level_to_l0(G, bx * RB, fx - L::ex[i]), sy = level_to_l0(G, by * RB, fy - L::ey[i]),
+                    sz = level_to_l0(G, bz * RB, fz - L::ez[i]);
         if (sphere_phi(G.body, sx, sy, sz) < 0.f) {
           const double m = 2.0 * ((double)*at(gp + (size_t)L::opp[i] * G.pool_plane, cell) + (double)L::w[i]);
           f0 -= L::ex[i] * m;
@@ -413,9 +486,10 @@ k_pool_force(const float* __restrict__ gp, PoolGeom G, const int32_t* __restrict
           f2 -= L::ez[i] * m;
         }
       }
-      f0 *= 0.25;
-      f1 *= 0.25;
-      f2 *= 0.25;
+      // weighted dx^2 = 4^-m (common_d3_force_pool.wgsl DX_WEIGHT)
+      f0 *= (double)G.dx2;
+      f1 *= (double)G.dx2;
+      f2 *= (double)G.dx2;
       const double rx = (double)px - G.body.cx, ry = (double)py - G.body.cy, rz = (double)pz - G.body.cz;
       fv[0] = f0;
       fv[1] = f1;
@@ -436,10 +510,10 @@ k_pool_force(const float* __restrict__ gp, PoolGeom G, const int32_t* __restrict
 // ---------------------------------------------------------------------------
 // explode: coarse state at t -> ring of the pool buffer the first substep reads
 // ---------------------------------------------------------------------------
-template <int Q, int RB>
+template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256)
-k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const uint4* __restrict__ ring,
-          uint32_t nring, int linear, int per) {
+k_explode(ParentRef P, float* __restrict__ gp, PoolGeom G, const uint4* __restrict__ ring, uint32_t nring, int linear,
+          int per) {
   using L = Lattice<Q>;
   using T = Tile<RB>;
   constexpr int FB = T::FB, LO = T::LO, HI = T::HI;
@@ -458,10 +532,12 @@ k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, cons
   const uint32_t r = cell % (uint32_t)T::CELLS;
   const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
   const int vx = d.w & 1023, vy = (d.w >> 10) & 1023, vz = d.w >> 20;
-  const uint32_t cmx = coarse_cell(G, wrap1(vx - 1, G.nx), vy, vz), cpx = coarse_cell(G, wrap1(vx + 1, G.nx), vy, vz);
-  const uint32_t cmy = coarse_cell(G, vx, wrap1(vy - 1, G.ny), vz), cpy = coarse_cell(G, vx, wrap1(vy + 1, G.ny), vz);
-  const uint32_t cmz = coarse_cell(G, vx, vy, wrap1(vz - 1, G.nz)), cpz = coarse_cell(G, vx, vy, wrap1(vz + 1, G.nz));
   const bool okx = (bits >> 27) & 1, oky = (bits >> 28) & 1, okz = (bits >> 29) & 1;
+  // the gradient's neighbours (present wherever their axis is ok)
+  auto pidx = [&](int x, int y, int z) { return parent_index<PP, RB>(G, P, x, y, z); };
+  const uint32_t cmx = okx ? pidx(wrap1(vx - 1, G.nx), vy, vz) : vc, cpx = okx ? pidx(wrap1(vx + 1, G.nx), vy, vz) : vc;
+  const uint32_t cmy = oky ? pidx(vx, wrap1(vy - 1, G.ny), vz) : vc, cpy = oky ? pidx(vx, wrap1(vy + 1, G.ny), vz) : vc;
+  const uint32_t cmz = okz ? pidx(vx, vy, wrap1(vz - 1, G.nz)) : vc, cpz = okz ? pidx(vx, vy, wrap1(vz + 1, G.nz)) : vc;
   // position of the fine cell inside its coarse cell, in coarse units: the
   // global fine coordinate is 2 * block + local - 2, so its parity is the
   // local one's
@@ -473,7 +549,7 @@ k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, cons
     if (i < i0 || i >= i1) continue;  // wave-uniform
     const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
     const float wi = L::w[i];
-    auto l0 = [&](uint32_t c) { return gc[lat_elem(Q, i, c)]; };  // population i of L0 cell c
+    auto l0 = [&](uint32_t c) { return *ppop<PP, Q>(P, c, i); };  // population i of parent cell c
     float out;
     if (!((bits >> i) & 1)) {
       // The coarse target is unrefined: only a population that streams from
@@ -507,10 +583,10 @@ k_explode(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, cons
 // ---------------------------------------------------------------------------
 // mac projection of every covered L0 cell from its 8 children (rho mean,
 // mass-weighted u; common_d3_amr_coalesce.wgsl:245-261)
-template <int RB>
+template <int RB, bool PP>
 __global__ void __launch_bounds__(256)
-k_coalesce_mac(float* __restrict__ mac, const float* __restrict__ mac_pool, PoolGeom G,
-               const int32_t* __restrict__ active, int nactive) {
+k_coalesce_mac(ParentRef P, const float* __restrict__ mac_pool, PoolGeom G, const int32_t* __restrict__ active,
+               int nactive) {
   constexpr uint32_t per = RB * RB * RB;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k = t / per;
@@ -532,16 +608,17 @@ k_coalesce_mac(float* __restrict__ mac, const float* __restrict__ mac_pool, Pool
     mz += rho * *at(mac_pool + 3 * (size_t)G.pool_plane, pc);
   }
   const float inv = 1.f / fmaxf(rs, 1e-6f);
-  const uint32_t cc = coarse_cell(G, bx * RB + lx, by * RB + ly, bz * RB + lz);
-  reinterpret_cast<float4*>(mac)[cc] = make_float4(rs * 0.125f, mx * inv, my * inv, mz * inv);
+  const uint32_t cc = parent_index<PP, RB>(G, P, bx * RB + lx, by * RB + ly, bz * RB + lz);
+  if (PP && cc == kAbsent) return;  // a tile's parent cells lie in parent tiles
+  pmac_put<PP>(P, cc, make_float4(rs * 0.125f, mx * inv, my * inv, mz * inv));
 }
 
 // Covered seam cells: for every direction whose target is unrefined, the
 // mean of the 8 ring cells covering the target (common_d3_amr_coalesce.wgsl:
 // 264-284).  One thread per (seam cell, direction i = 1 + grid row).
-template <int Q, int RB>
+template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256)
-k_coalesce_seam(const float* __restrict__ gp, float* __restrict__ gc, PoolGeom G, const uint32_t* __restrict__ seam,
+k_coalesce_seam(const float* __restrict__ gp, ParentRef P, PoolGeom G, const uint32_t* __restrict__ seam,
                 uint32_t nseam) {
   using T = Tile<RB>;
   constexpr int FB = T::FB;
@@ -567,17 +644,17 @@ k_coalesce_seam(const float* __restrict__ gp, float* __restrict__ gc, PoolGeom G
   float s = 0.f;
 #pragma unroll
   for (int k = 0; k < 8; k++) s += *at(p, b0 + (k & 1) + ((k >> 1) & 1) * FB + ((k >> 2) & 1) * FB * FB);
-  gc[lat_elem(Q, i, c)] = s * 0.125f;  // an average of g is the average of f minus w
+  // an average of g is the average of f minus w
+  *ppop<PP, Q>(P, PP ? parent_index<PP, RB>(G, P, cx, cy, cz) : c, i) = s * 0.125f;
 }
 
 // Orphans on unrefined L0 cells next to the refined region
 // (coalesceOrphans, common_d3_amr_coalesce.wgsl:126-211).  One thread per
 // (near cell, direction i = 1 + grid row); the 8 child lookups of a
 // direction are independent loads.
-template <int Q, int RB>
+template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256)
-k_orphans(const float* __restrict__ gp, float* __restrict__ gc, PoolGeom G, const uint32_t* __restrict__ near,
-          uint32_t nnear) {
+k_orphans(const float* __restrict__ gp, ParentRef P, PoolGeom G, const uint32_t* __restrict__ near, uint32_t nnear) {
   using T = Tile<RB>;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nnear) return;
@@ -607,7 +684,7 @@ k_orphans(const float* __restrict__ gp, float* __restrict__ gc, PoolGeom G, cons
     const int hx = wrap1(2 * cx + ox + ex, NFx), hy = wrap1(2 * cy + oy + ey, NFy), hz = wrap1(2 * cz + oz + ez, NFz);
     if (G.ccls[coarse_cell(G, hx >> 1, hy >> 1, hz >> 1)] & kCovered) nIn += 1.f;
   }
-  float& gci = gc[lat_elem(Q, i, c)];
+  float& gci = *ppop<PP, Q>(P, PP ? parent_index<PP, RB>(G, P, cx, cy, cz) : c, i);
   const float fc = gci + wi;
   const float net = s - nIn * fc;
   if (net != 0.f) gci = (fc + net * 0.125f) - wi;
@@ -616,12 +693,12 @@ k_orphans(const float* __restrict__ gp, float* __restrict__ gc, PoolGeom G, cons
 // ---------------------------------------------------------------------------
 // interp fill of newly refined tiles, and average/drain of dying ones
 // ---------------------------------------------------------------------------
-template <int Q>
-__device__ __forceinline__ void coarse_sample(const float* __restrict__ gc, uint32_t cell, float& rho,
-                                              float& ux, float& uy, float& uz, float (&fneq)[Q]) {
+template <int Q, bool PP>
+__device__ __forceinline__ void coarse_sample(const ParentRef& P, uint32_t cell, float& rho, float& ux, float& uy,
+                                              float& uz, float (&fneq)[Q]) {
   float g[Q];
 #pragma unroll
-  for (int i = 0; i < Q; i++) g[i] = gc[lat_elem(Q, i, cell)];
+  for (int i = 0; i < Q; i++) g[i] = *ppop<PP, Q>(P, cell, i);
   const float F0[3] = {0.f, 0.f, 0.f};
   float drho;
   moments<Q>(g, F0, drho, rho, ux, uy, uz);
@@ -629,10 +706,10 @@ __device__ __forceinline__ void coarse_sample(const float* __restrict__ gc, uint
   for (int i = 0; i < Q; i++) fneq[i] = g[i] - geq<Q>(i, rho, ux, uy, uz);
 }
 
-template <int Q, int RB>
+template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256)
-k_interp(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const int32_t* __restrict__ pairs,
-         int npairs, float rescale, int ring_only) {
+k_interp(ParentRef P, float* __restrict__ gp, PoolGeom G, const int32_t* __restrict__ pairs, int npairs,
+         float rescale, int ring_only) {
   using T = Tile<RB>;
   constexpr int FB = T::FB, LO = T::LO, HI = T::HI;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -651,15 +728,30 @@ k_interp(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const
   const float pz = (float)(bz * RB) - 0.25f + 0.5f * (float)(fz - kGhost);
   const int c0x = (int)floorf(px), c0y = (int)floorf(py), c0z = (int)floorf(pz);
   const float tx = px - c0x, ty = py - c0y, tz = pz - c0z;
+  // A pool parent is sampled in the one parent tile that holds this block,
+  // ring included: the stencil of a block's whole tile reaches parent-local
+  // [0, FB) (common_interp_parent_pool.wgsl:17-28), so it needs no wrap and
+  // no second tile.
+  int ps = 0, pox = 0, poy = 0, poz = 0;
+  if constexpr (PP) {
+    constexpr int SH = Tile<RB>::LOG_RB + 1;
+    const int pbx = (bx * RB) >> SH, pby = (by * RB) >> SH, pbz = (bz * RB) >> SH;
+    ps = P.block_slot[(pbz * P.nby + pby) * P.nbx + pbx];
+    if (ps < 0) return;  // a tile's parent block holds a tile
+    pox = kGhost - (pbx << SH);
+    poy = kGhost - (pby << SH);
+    poz = kGhost - (pbz << SH);
+  }
   float rho = 0.f, ux = 0.f, uy = 0.f, uz = 0.f, fneq[Q];
 #pragma unroll
   for (int i = 0; i < Q; i++) fneq[i] = 0.f;
   for (int s = 0; s < 8; s++) {
     const int i = s & 1, j = (s >> 1) & 1, kk = (s >> 2) & 1;
     const float w = (i ? tx : 1.f - tx) * (j ? ty : 1.f - ty) * (kk ? tz : 1.f - tz);
-    const uint32_t cell = coarse_cell(G, wrap1(c0x + i, G.nx), wrap1(c0y + j, G.ny), wrap1(c0z + kk, G.nz));
+    const uint32_t cell = PP ? pool_cell<RB>(ps, c0x + i + pox, c0y + j + poy, c0z + kk + poz)
+                             : coarse_cell(G, wrap1(c0x + i, G.nx), wrap1(c0y + j, G.ny), wrap1(c0z + kk, G.nz));
     float sr, sx, sy, sz, sf[Q];
-    coarse_sample<Q>(gc, cell, sr, sx, sy, sz, sf);
+    coarse_sample<Q, PP>(P, cell, sr, sx, sy, sz, sf);
     rho += w * sr;
     ux += w * sx;
     uy += w * sy;
@@ -672,10 +764,10 @@ k_interp(const float* __restrict__ gc, float* __restrict__ gp, PoolGeom G, const
   for (int q = 0; q < Q; q++) gp[(size_t)q * G.pool_plane + cell] = geq<Q>(q, rho, ux, uy, uz) + rescale * fneq[q];
 }
 
-template <int Q, int RB>
+template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256)
-k_average(const float* __restrict__ gp, float* __restrict__ gc, float* __restrict__ mac, PoolGeom G,
-          const int32_t* __restrict__ pairs, int npairs, float rescale) {
+k_average(const float* __restrict__ gp, ParentRef P, PoolGeom G, const int32_t* __restrict__ pairs, int npairs,
+          float rescale) {
   constexpr uint32_t per = RB * RB * RB;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k = t / per;
@@ -706,10 +798,11 @@ k_average(const float* __restrict__ gp, float* __restrict__ gc, float* __restric
   const float rho = rs * 0.125f;
   const float inv = 1.f / fmaxf(rs, 1e-6f);
   const float ux = mx * inv, uy = my * inv, uz = mz * inv;
-  const uint32_t cc = coarse_cell(G, bx * RB + lx, by * RB + ly, bz * RB + lz);
+  const uint32_t cc = parent_index<PP, RB>(G, P, bx * RB + lx, by * RB + ly, bz * RB + lz);
+  if (PP && cc == kAbsent) return;  // a tile's parent cells lie in parent tiles
 #pragma unroll
-  for (int q = 0; q < Q; q++) gc[lat_elem(Q, q, cc)] = geq<Q>(q, rho, ux, uy, uz) + rescale * (sneq[q] * 0.125f);
-  reinterpret_cast<float4*>(mac)[cc] = make_float4(rho, ux, uy, uz);
+  for (int q = 0; q < Q; q++) *ppop<PP, Q>(P, cc, q) = geq<Q>(q, rho, ux, uy, uz) + rescale * (sneq[q] * 0.125f);
+  pmac_put<PP>(P, cc, make_float4(rho, ux, uy, uz));
 }
 
 // ---------------------------------------------------------------------------
@@ -1013,12 +1106,49 @@ __global__ void __launch_bounds__(256) k_active_pairs(const int32_t* active, con
 // FP64 totals of the composite state: unrefined L0 cells + interior fine
 // cells weighted 1/8 (a fine cell is 1/8 of a coarse cell's volume)
 // ---------------------------------------------------------------------------
+// Adds weight x [f, f ex, f ey, f ez] over the interior cells of a pool
+// level's active tiles that no finer tile covers (grid-stride over the
+// launch).
+template <int Q, int RB>
+__device__ __forceinline__ void level_sums(const float* __restrict__ gp, const PoolGeom& G,
+                                           const int32_t* __restrict__ active, double weight, double (&v)[4]) {
+  using L = Lattice<Q>;
+  constexpr uint32_t rb2 = 2 * RB, per = rb2 * rb2 * rb2;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const int nact = active[0];
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < (uint32_t)nact * per; t += stride) {
+    const uint32_t k = t / per, r = t - k * per;
+    const int slot = active[1 + k];
+    const int lx = r % rb2, ly = (r / rb2) % rb2, lz = r / (rb2 * rb2);
+    if (G.child_block_slot) {
+      const int id = G.slot_to_block[slot];
+      const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
+      if (covered_by_child<RB>(G, bx * (int)rb2 + lx, by * (int)rb2 + ly, bz * (int)rb2 + lz)) continue;
+    }
+    const uint32_t pc = pool_cell<RB>(slot, kGhost + lx, kGhost + ly, kGhost + lz);
+    for (int i = 0; i < Q; i++) {
+      const double f = weight * ((double)gp[(size_t)i * G.pool_plane + pc] + (double)L::w[i]);
+      v[0] += f; v[1] += L::ex[i] * f; v[2] += L::ey[i] * f; v[3] += L::ez[i] * f;
+    }
+  }
+}
+
+template <int Q, int RB>
+__global__ void __launch_bounds__(256)
+k_level_totals(const float* __restrict__ gp, PoolGeom G, const int32_t* __restrict__ active, double weight,
+               double* partials, unsigned* counter, double* out) {
+  __shared__ double lds[16 * 4];
+  double v[4] = {0, 0, 0, 0};
+  level_sums<Q, RB>(gp, G, active, weight, v);
+  block_sum<4>(v, lds);
+  partials_then_fixed_order_sum<4>(v, partials, counter, [&](int j, double s) { out[j] = s; });
+}
+
 template <int Q, int RB>
 __global__ void __launch_bounds__(256)
 k_pool_totals(const float* __restrict__ gc, const float* __restrict__ gp, PoolGeom G, const int32_t* __restrict__ active,
               double* partials, unsigned* counter, double* out) {
   using L = Lattice<Q>;
-  constexpr uint32_t rb2 = 2 * RB, per = rb2 * rb2 * rb2;
   __shared__ double lds[16 * 4];
   double v[4] = {0, 0, 0, 0};
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -1029,16 +1159,7 @@ k_pool_totals(const float* __restrict__ gc, const float* __restrict__ gp, PoolGe
       v[0] += f; v[1] += L::ex[i] * f; v[2] += L::ey[i] * f; v[3] += L::ez[i] * f;
     }
   }
-  const int nact = active[0];
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < (uint32_t)nact * per; t += stride) {
-    const uint32_t k = t / per, r = t - k * per;
-    const int slot = active[1 + k];
-    const uint32_t pc = pool_cell<RB>(slot, kGhost + r % rb2, kGhost + (r / rb2) % rb2, kGhost + r / (rb2 * rb2));
-    for (int i = 0; i < Q; i++) {
-      const double f = 0.125 * ((double)gp[(size_t)i * G.pool_plane + pc] + (double)L::w[i]);
-      v[0] += f; v[1] += L::ex[i] * f; v[2] += L::ey[i] * f; v[3] += L::ez[i] * f;
-    }
-  }
+  level_sums<Q, RB>(gp, G, active, 0.125, v);
   block_sum<4>(v, lds);
   partials_then_fixed_order_sum<4>(v, partials, counter, [&](int j, double s) { out[j] = s; });
 }
@@ -1061,17 +1182,28 @@ static hipError_t dispatch_q_rb(int q, int rb, Args&&... args) {
   }
 }
 
+// Runs f(std::true_type{}) for a pool parent, f(std::false_type{}) for the
+// dense one.
+template <class F>
+static void with_parent(const ParentRef& p, F f) {
+  if (p.block_slot) f(std::true_type{});
+  else f(std::false_type{});
+}
+
 template <int Q, int RB>
 struct OpCellClass {
   static hipError_t run(uint8_t* ccls, const PoolGeom& g, const SurfacePlan& plan, const int32_t* active, int nactive,
-                        hipStream_t s) {
-    hipLaunchKernelGGL((k_cell_class<Q, RB>), grid_for(g.ncells), dim3(256), 0, s, ccls, g, plan);
-    if (nactive > 0) {
-      hipLaunchKernelGGL((k_ring_plan<Q, RB>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s, g, plan,
-                         active, nactive);
+                        const ParentRef& P, hipStream_t s) {
+    with_parent(P, [&](auto PP) {
+      constexpr bool pp = decltype(PP)::value;
+      hipLaunchKernelGGL((k_cell_class<Q, RB, pp>), grid_for(g.ncells), dim3(256), 0, s, ccls, g, plan, P);
+      if (nactive > 0)
+        hipLaunchKernelGGL((k_ring_plan<Q, RB, pp>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s, g,
+                           plan, active, nactive, P);
+    });
+    if (nactive > 0)
       hipLaunchKernelGGL(k_tile_table, grid_for((uint64_t)nactive), dim3(256), 0, s, g, active, nactive,
                          const_cast<int32_t*>(g.tile_table));
-    }
     return hipGetLastError();
   }
 };
@@ -1125,48 +1257,58 @@ struct OpReflux {
 };
 template <int Q, int RB>
 struct OpExplode {
-  static hipError_t run(const float* gc, float* gp, const PoolGeom& g, const SurfacePlan& plan, uint32_t nring,
+  static hipError_t run(const ParentRef& P, float* gp, const PoolGeom& g, const SurfacePlan& plan, uint32_t nring,
                         int linear, int dirs, hipStream_t s) {
     // about 2^20 threads: every direction its own thread for a small ring,
     // all Q in one thread for a ring of a million cells
     const int groups = (int)std::min<uint64_t>(Q, std::max<uint64_t>(1, ((1u << 20) + nring - 1) / nring));
     const int per = dirs > 0 ? std::min(dirs, Q) : (Q + groups - 1) / groups;
-    hipLaunchKernelGGL((k_explode<Q, RB>), grid_for(nring, (Q + per - 1) / per), dim3(256), 0, s, gc, gp, g, plan.ring,
-                       nring, linear, per);
+    with_parent(P, [&](auto PP) {
+      hipLaunchKernelGGL((k_explode<Q, RB, decltype(PP)::value>), grid_for(nring, (Q + per - 1) / per), dim3(256), 0,
+                         s, P, gp, g, plan.ring, nring, linear, per);
+    });
     return hipGetLastError();
   }
 };
 template <int Q, int RB>
 struct OpCoalesce {
-  static hipError_t run(const float* gp, float* gc, float* mac, const float* mac_pool, const PoolGeom& g,
+  static hipError_t run(const float* gp, const ParentRef& P, const float* mac_pool, const PoolGeom& g,
                         const int32_t* active, int nactive, const SurfacePlan& plan, uint32_t nseam, uint32_t nnear,
                         int orphans, hipStream_t s) {
-    if (nactive > 0)
-      hipLaunchKernelGGL((k_coalesce_mac<RB>), grid_for((uint64_t)nactive * RB * RB * RB), dim3(256), 0, s, mac,
-                         mac_pool, g, active, nactive);
-    if (nseam > 0)
-      hipLaunchKernelGGL((k_coalesce_seam<Q, RB>), grid_for(nseam, Q - 1), dim3(256), 0, s, gp, gc, g, plan.seam,
-                         nseam);
-    if (orphans && nnear > 0)
-      hipLaunchKernelGGL((k_orphans<Q, RB>), grid_for(nnear, Q - 1), dim3(256), 0, s, gp, gc, g, plan.near, nnear);
+    with_parent(P, [&](auto PP) {
+      constexpr bool pp = decltype(PP)::value;
+      if (nactive > 0)
+        hipLaunchKernelGGL((k_coalesce_mac<RB, pp>), grid_for((uint64_t)nactive * RB * RB * RB), dim3(256), 0, s, P,
+                           mac_pool, g, active, nactive);
+      if (nseam > 0)
+        hipLaunchKernelGGL((k_coalesce_seam<Q, RB, pp>), grid_for(nseam, Q - 1), dim3(256), 0, s, gp, P, g, plan.seam,
+                           nseam);
+      if (orphans && nnear > 0)
+        hipLaunchKernelGGL((k_orphans<Q, RB, pp>), grid_for(nnear, Q - 1), dim3(256), 0, s, gp, P, g, plan.near,
+                           nnear);
+    });
     return hipGetLastError();
   }
 };
 template <int Q, int RB>
 struct OpInterp {
-  static hipError_t run(const float* gc, float* gp, const PoolGeom& g, const int32_t* pairs, int npairs, float rescale,
-                        int ring_only, hipStream_t s) {
-    hipLaunchKernelGGL((k_interp<Q, RB>), grid_for((uint64_t)npairs * Tile<RB>::CELLS), dim3(256), 0, s, gc, gp, g,
-                       pairs, npairs, rescale, ring_only);
+  static hipError_t run(const ParentRef& P, float* gp, const PoolGeom& g, const int32_t* pairs, int npairs,
+                        float rescale, int ring_only, hipStream_t s) {
+    with_parent(P, [&](auto PP) {
+      hipLaunchKernelGGL((k_interp<Q, RB, decltype(PP)::value>), grid_for((uint64_t)npairs * Tile<RB>::CELLS),
+                         dim3(256), 0, s, P, gp, g, pairs, npairs, rescale, ring_only);
+    });
     return hipGetLastError();
   }
 };
 template <int Q, int RB>
 struct OpAverage {
-  static hipError_t run(const float* gp, float* gc, float* mac, const PoolGeom& g, const int32_t* pairs, int npairs,
+  static hipError_t run(const float* gp, const ParentRef& P, const PoolGeom& g, const int32_t* pairs, int npairs,
                         float rescale, hipStream_t s) {
-    hipLaunchKernelGGL((k_average<Q, RB>), grid_for((uint64_t)npairs * RB * RB * RB), dim3(256), 0, s, gp, gc, mac, g,
-                       pairs, npairs, rescale);
+    with_parent(P, [&](auto PP) {
+      hipLaunchKernelGGL((k_average<Q, RB, decltype(PP)::value>), grid_for((uint64_t)npairs * RB * RB * RB),
+                         dim3(256), 0, s, gp, P, g, pairs, npairs, rescale);
+    });
     return hipGetLastError();
   }
 };
@@ -1178,10 +1320,19 @@ struct OpTotals {
     return hipGetLastError();
   }
 };
+template <int Q, int RB>
+struct OpLevelTotals {
+  static hipError_t run(const float* gp, const PoolGeom& g, const int32_t* active, double weight, double* partials,
+                        unsigned* counter, double* out, hipStream_t s) {
+    hipLaunchKernelGGL((k_level_totals<Q, RB>), dim3(256), dim3(256), 0, s, gp, g, active, weight, partials, counter,
+                       out);
+    return hipGetLastError();
+  }
+};
 
 hipError_t launch_cell_class(int q, uint8_t* ccls, const PoolGeom& g, const SurfacePlan& plan, const int32_t* active,
-                             int nactive, hipStream_t s) {
-  return dispatch_q_rb<OpCellClass>(q, g.rb, ccls, g, plan, active, nactive, s);
+                             int nactive, const ParentRef& parent, hipStream_t s) {
+  return dispatch_q_rb<OpCellClass>(q, g.rb, ccls, g, plan, active, nactive, parent, s);
 }
 
 hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
@@ -1208,29 +1359,29 @@ hipError_t launch_reflux(int q, const float* gct, float* gct1, float* mac, const
   return dispatch_q_rb<OpReflux>(q, g.rb, gct, gct1, mac, g, plan, nflux, acc, force, s);
 }
 
-hipError_t launch_explode(int q, const float* gc, float* gp, const PoolGeom& g, const SurfacePlan& plan,
+hipError_t launch_explode(int q, const ParentRef& parent, float* gp, const PoolGeom& g, const SurfacePlan& plan,
                           uint32_t nring, int linear, int dirs, hipStream_t s) {
   if (nring == 0) return hipSuccess;
-  return dispatch_q_rb<OpExplode>(q, g.rb, gc, gp, g, plan, nring, linear, dirs, s);
+  return dispatch_q_rb<OpExplode>(q, g.rb, parent, gp, g, plan, nring, linear, dirs, s);
 }
 
-hipError_t launch_coalesce(int q, const float* gp, float* gc, float* mac, const float* mac_pool, const PoolGeom& g,
+hipError_t launch_coalesce(int q, const float* gp, const ParentRef& parent, const float* mac_pool, const PoolGeom& g,
                            const int32_t* active, int nactive, const SurfacePlan& plan, uint32_t nseam,
                            uint32_t nnear, int orphans, hipStream_t s) {
-  return dispatch_q_rb<OpCoalesce>(q, g.rb, gp, gc, mac, mac_pool, g, active, nactive, plan, nseam, nnear, orphans,
+  return dispatch_q_rb<OpCoalesce>(q, g.rb, gp, parent, mac_pool, g, active, nactive, plan, nseam, nnear, orphans,
                                    s);
 }
 
-hipError_t launch_interp(int q, const float* gc, float* gp, const PoolGeom& g, const int32_t* pairs, int npairs,
-                         float rescale, int ring_only, hipStream_t s) {
+hipError_t launch_interp(int q, const ParentRef& parent, float* gp, const PoolGeom& g, const int32_t* pairs,
+                         int npairs, float rescale, int ring_only, hipStream_t s) {
   if (npairs <= 0) return hipSuccess;
-  return dispatch_q_rb<OpInterp>(q, g.rb, gc, gp, g, pairs, npairs, rescale, ring_only, s);
+  return dispatch_q_rb<OpInterp>(q, g.rb, parent, gp, g, pairs, npairs, rescale, ring_only, s);
 }
 
-hipError_t launch_average(int q, const float* gp, float* gc, float* mac, const PoolGeom& g, const int32_t* pairs,
+hipError_t launch_average(int q, const float* gp, const ParentRef& parent, const PoolGeom& g, const int32_t* pairs,
                           int npairs, float rescale, hipStream_t s) {
   if (npairs <= 0) return hipSuccess;
-  return dispatch_q_rb<OpAverage>(q, g.rb, gp, gc, mac, g, pairs, npairs, rescale, s);
+  return dispatch_q_rb<OpAverage>(q, g.rb, gp, parent, g, pairs, npairs, rescale, s);
 }
 
 hipError_t launch_criterion(const float* mac, float* crit, const PoolGeom& g, float lead, int kind, hipStream_t s) {
@@ -1258,6 +1409,10 @@ hipError_t launch_active_pairs(const int32_t* active, const int32_t* slot_to_blo
 hipError_t launch_pool_totals(int q, const float* gc, const float* gp, const PoolGeom& g, const int32_t* active,
                               double* partials, unsigned* counter, double* out, hipStream_t s) {
   return dispatch_q_rb<OpTotals>(q, g.rb, gc, gp, g, active, partials, counter, out, s);
+}
+hipError_t launch_level_totals(int q, const float* gp, const PoolGeom& g, const int32_t* active, double weight,
+                               double* partials, unsigned* counter, double* out, hipStream_t s) {
+  return dispatch_q_rb<OpLevelTotals>(q, g.rb, gp, g, active, weight, partials, counter, out, s);
 }
 
 }  // namespace lbm

@@ -26,7 +26,8 @@ EXPORTS = (
     "lbm_amr_create", "lbm_amr_destroy", "lbm_amr_upload_topology", "lbm_amr_read_topology",
     "lbm_amr_upload_macro", "lbm_amr_step", "lbm_amr_manage_now", "lbm_amr_read_macro",
     "lbm_amr_read_pool_macro", "lbm_amr_read_f", "lbm_amr_read_criterion", "lbm_amr_read_active",
-    "lbm_amr_read_totals", "lbm_amr_info", "lbm_amr_read_forces",
+    "lbm_amr_read_totals", "lbm_amr_info", "lbm_amr_read_forces", "lbm_amr_upload_level_topology",
+    "lbm_amr_read_level_f", "lbm_amr_read_level_macro",
     "lbm_serve_create", "lbm_serve_destroy", "lbm_serve_submit", "lbm_serve_wait", "lbm_serve_read_macro",
     "lbm_serve_read_forces", "lbm_serve_release", "lbm_serve_drain", "lbm_serve_read_stats", "lbm_serve_clock",
     "lbm_serve_submit_state", "lbm_serve_read_state", "lbm_rigid_attach", "lbm_rigid_read",
@@ -63,6 +64,7 @@ class AmrDesc(ctypes.Structure):
         ("interface_interp", ctypes.c_int32), ("reflux", ctypes.c_int32),
         ("sponge_width", ctypes.c_float), ("sponge_axes", ctypes.c_int32 * 3), ("sponge_u", ctypes.c_float * 3),
         ("body", Body), ("explode_dirs", ctypes.c_int32), ("criterion", ctypes.c_int32),
+        ("levels", ctypes.c_int32),
     ]
 
 
@@ -199,6 +201,9 @@ def load():
         lib.lbm_amr_read_totals.argtypes = [P, P]
         lib.lbm_amr_info.argtypes = [P, i64p, i32p, i32p]
         lib.lbm_amr_read_forces.argtypes = [P, P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        lib.lbm_amr_upload_level_topology.argtypes = [P, ctypes.c_int32, P, P]
+        lib.lbm_amr_read_level_f.argtypes = [P, ctypes.c_int32, P]
+        lib.lbm_amr_read_level_macro.argtypes = [P, ctypes.c_int32, P]
         lib.lbm_serve_create.argtypes = [ctypes.POINTER(ServeDesc), ctypes.POINTER(P)]
         lib.lbm_serve_destroy.argtypes = [P]
         lib.lbm_serve_submit.argtypes = [P, ctypes.POINTER(Desc), P, ctypes.c_int32, i64p]
@@ -408,7 +413,8 @@ def make_amr_desc(d):
     interface ('explode' or 'interp'), reflux, sponge: {width, axes, u} on
     L0, body: {a, centre} a pinned sphere solved on the fine level,
     explodeDirs: directions per explode thread (0: automatic),
-    criterion: "vorticity" (|curl u|, default) or "q"}."""
+    criterion: "vorticity" (|curl u|, default) or "q", levels: 2 (default),
+    3 or 4 including L0}."""
     a = AmrDesc()
     a.abi_version = ABI_VERSION
     a.q = int(d["q"])
@@ -440,6 +446,7 @@ def make_amr_desc(d):
         a.sponge_u[k] = float((list(sp.get("u", [0, 0, 0])) + [0, 0, 0])[k])
     a.explode_dirs = int(d.get("explodeDirs", 0))
     a.criterion = {"vorticity": 0, "q": 1}[d.get("criterion", "vorticity")]
+    a.levels = int(d.get("levels", 2))
     body = d.get("body")
     if body:
         a.body.shape = BODY_SPHERE
@@ -450,7 +457,8 @@ def make_amr_desc(d):
 
 
 class Amr:
-    """Two-level block-pool solve on the GPU (owns an lbm_amr handle)."""
+    """Block-pool AMR solve on the GPU (owns an lbm_amr handle): L0 plus
+    `levels - 1` pool levels."""
 
     def __init__(self, desc_dict):
         self.lib = load()
@@ -462,6 +470,7 @@ class Amr:
         self.nblocks = (d.nx // d.rb) * (d.ny // d.rb) * (d.nz // d.rb)
         self.fb = 2 * d.rb + 2 * GHOST
         self.pool_plane = d.max_slots * self.fb ** 3
+        self.levels = d.levels or 2
         h = ctypes.c_void_p()
         _check(self.lib.lbm_amr_create(ctypes.byref(self.desc), ctypes.byref(h)))
         self.h = h
@@ -489,6 +498,33 @@ class Amr:
         if bs.size != self.nblocks or sb.size != self.max_slots:
             raise ValueError("topology sizes %d/%d, want %d/%d" % (bs.size, sb.size, self.nblocks, self.max_slots))
         _check(self.lib.lbm_amr_upload_topology(self.h, bs.ctypes.data, sb.ctypes.data))
+
+    def level_blocks(self, level):
+        """Blocks of pool level `level` (over the parent level's global grid)."""
+        s = 1 << (level - 1)
+        d = self.desc
+        return (d.nx * s // d.rb) * (d.ny * s // d.rb) * (d.nz * s // d.rb)
+
+    def upload_level_topology(self, level, block_slot, slot_to_block):
+        bs = np.ascontiguousarray(block_slot, dtype=np.int32)
+        sb = np.ascontiguousarray(slot_to_block, dtype=np.int32)
+        nb = self.level_blocks(level)
+        # a level outside 1 .. levels - 1 goes to the library, which refuses it
+        # before reading the maps
+        if 1 <= level < self.levels and (bs.size != nb or sb.size != self.max_slots):
+            raise ValueError("level %d topology sizes %d/%d, want %d/%d" % (level, bs.size, sb.size, nb,
+                                                                             self.max_slots))
+        _check(self.lib.lbm_amr_upload_level_topology(self.h, int(level), bs.ctypes.data, sb.ctypes.data))
+
+    def read_level_f(self, level):
+        out = np.empty(self.q * self.pool_plane, dtype=np.float32)
+        _check(self.lib.lbm_amr_read_level_f(self.h, int(level), out.ctypes.data))
+        return out
+
+    def read_level_macro(self, level):
+        out = np.empty(4 * self.pool_plane, dtype=np.float32)
+        _check(self.lib.lbm_amr_read_level_macro(self.h, int(level), out.ctypes.data))
+        return out
 
     def read_topology(self):
         bs = np.empty(self.nblocks, dtype=np.int32)

@@ -125,19 +125,48 @@ def refine_near_sphere(nx, ny, nz, rb, centre, radius, margin=2.0, max_slots=Non
     return bs, sb
 
 
+def sphere_hierarchy(nx, ny, nz, rb, levels, centre, radius, margin=2.0):
+    """refineHierarchy for a sphere (d3-amr.mjs:603-651): the finest level
+    wants every block within `margin` L0 cells (nearBodyWant) and cascade21
+    closes the coarser ones.  Runs the package's JS module under Node
+    (js/hierarchy-cli.mjs).  Returns [None, (block_slot, slot_to_block), ...]
+    per level, every slot array padded with -1 to the largest level's count."""
+    import json
+    import os
+    import subprocess
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "js", "hierarchy-cli.mjs")
+    arg = json.dumps({"dims": [nx, ny, nz], "rb": rb, "levels": levels, "centre": list(map(float, centre)),
+                      "radius": float(radius), "margin": float(margin)})
+    out = json.loads(subprocess.run(["node", cli, arg], check=True, capture_output=True, text=True).stdout)
+    slots = max(lv["activeSlots"] for lv in out["byLevel"][1:])
+    maps = [None]
+    for lv in out["byLevel"][1:]:
+        sb = np.full(slots, -1, np.int32)
+        sb[:lv["activeSlots"]] = lv["slotToBlock"]
+        maps.append((np.array(lv["blockSlot"], np.int32), sb))
+    return maps
+
+
 def sphere_amr(nx=192, ny=128, nz=128, radius=8.0, centre=(48.0, 64.0, 64.0), u0=0.05, re=20.0, rb=4,
-               sponge_width=8, margin=2.0):
-    """The pinned sphere of `sphere` (BGK on L0) on two levels: the body on
-    the fine level alone, the blocks within `margin` L0 cells of it refined
-    (main-3d.js:482-530, ?refine=body, margin 2), explode interface, static
-    topology.  Returns (AMR descriptor dict, L0 seed, (block_slot,
-    slot_to_block), meta)."""
+               sponge_width=8, margin=2.0, levels=2):
+    """The pinned sphere of `sphere` (BGK on L0) on `levels` levels: the body
+    on the finest level alone, the blocks within `margin` L0 cells of it
+    refined at the finest level and the coarser levels closed around them
+    (main-3d.js:482-530, ?refine=body, margin 2; refineHierarchy at depth),
+    explode interface, static topology.  Returns (AMR descriptor dict, L0
+    seed, topology, meta) with topology (block_slot, slot_to_block) for two
+    levels and [None, (block_slot, slot_to_block) per level ...] for more."""
     desc, mac, meta = sphere(nx=nx, ny=ny, nz=nz, radius=radius, centre=centre, u0=u0, re=re, trt=False,
                              sponge_width=sponge_width)
-    bs, sb = refine_near_sphere(nx, ny, nz, rb, centre, radius, margin)
-    amr = {"q": 19, "nx": nx, "ny": ny, "nz": nz, "rb": rb, "maxSlots": len(sb), "omega": desc["omega"],
-           "sponge": desc["sponge"], "body": desc["body"]}
-    return amr, mac, (bs, sb), meta
+    if levels == 2:
+        topo = refine_near_sphere(nx, ny, nz, rb, centre, radius, margin)
+        slots = len(topo[1])
+    else:
+        topo = sphere_hierarchy(nx, ny, nz, rb, levels, centre, radius, margin)
+        slots = len(topo[1][1])
+    amr = {"q": 19, "nx": nx, "ny": ny, "nz": nz, "rb": rb, "maxSlots": slots, "omega": desc["omega"],
+           "sponge": desc["sponge"], "body": desc["body"], "levels": levels}
+    return amr, mac, topo, meta
 
 
 def plate(n=16, re=200.0, u_t=0.05, aspect=0.125, tow=0.0, stream=-0.05, i_star=0.17):

@@ -339,6 +339,14 @@ typedef struct lbm_amr_desc {
                          * (|Omega|^2 - |S|^2) (qAt, common_d3_criterion.wgsl:
                          * 102-120).  Either is the block maximum over the
                          * block's L0 box dilated by `lead`.                 */
+  int32_t levels;       /* levels including L0: 2 (0 reads as 2), 3 or 4.
+                         * Level m >= 2 refines level m - 1's tiles the way
+                         * level 1 refines L0 (pool parents, common_d3_parent_
+                         * pool.wgsl; the per-level schedule, main-3d.js:
+                         * 3169-3180): tau_m = 2 tau_(m-1) - 1/2, the body on
+                         * the finest level alone.  Levels >= 3 take the
+                         * explode interface and a static hierarchy
+                         * (manage_every = 0), uploaded level by level.      */
 } lbm_amr_desc;
 
 typedef struct lbm_amr lbm_amr;
@@ -354,8 +362,21 @@ int lbm_amr_destroy(lbm_amr* a);
 int lbm_amr_upload_topology(lbm_amr* a, const int32_t* block_slot, const int32_t* slot_to_block);
 int lbm_amr_read_topology(lbm_amr* a, int32_t* block_slot, int32_t* slot_to_block);
 
+/* Block maps of pool level `level` (1 .. levels - 1): blockSlot over the
+ * level's blocks (the parent level's global grid, L0's times 2^(level-1),
+ * in blocks of rb), slotToBlock[max_slots].  Level 1 is
+ * lbm_amr_upload_topology.  A deeper level's tiles must nest in the parent
+ * level's tiles and every ring cell's parent cell must lie in one
+ * (checkRingParentCoverage, d3-amr.mjs:348-390); with a body, the finest
+ * level must refine every block within one L0 cell of it.  Upload every
+ * level before lbm_amr_upload_macro.  Replaces uploadLevelTopology for
+ * m >= 2, main-3d.js:822, with refineHierarchy's levels (d3-amr.mjs:603). */
+int lbm_amr_upload_level_topology(lbm_amr* a, int32_t level, const int32_t* block_slot,
+                                  const int32_t* slot_to_block);
+
 /* L0 [rho, ux, uy, uz] seed -> equilibrium, then every refined tile is
- * filled from L0 by interpolation (reset, main-3d.js:2973-3035). */
+ * filled from L0 by interpolation (reset, main-3d.js:2973-3035), level by
+ * level from its parent. */
 int lbm_amr_upload_macro(lbm_amr* a, const float* mac4);
 
 /* Advances n macro-steps (encodeSteps, main-3d.js:3122-3335).  Returns
@@ -374,6 +395,12 @@ int lbm_amr_read_macro(lbm_amr* a, float* mac4);
 int lbm_amr_read_pool_macro(lbm_amr* a, float* mac_pool);
 /* Populations, f32 wire format: L0 [q][ncells]; pool [q][max_slots * fb^3]. */
 int lbm_amr_read_f(lbm_amr* a, float* f_l0, float* f_pool);
+/* Pool level `level` (1 .. levels - 1): populations, f32 wire format
+ * [q][max_slots * fb^3], and the planar macroscopic field [4][max_slots *
+ * fb^3] of its last substep.  Interior cells a finer level covers hold that
+ * level's outbox and coalesced moments, not a solution of this level. */
+int lbm_amr_read_level_f(lbm_amr* a, int32_t level, float* f_pool);
+int lbm_amr_read_level_macro(lbm_amr* a, int32_t level, float* mac_pool);
 /* Per block |curl u| maximum from the last manager pass, and its decision. */
 int lbm_amr_read_criterion(lbm_amr* a, float* crit, uint8_t* want);
 /* [count, slot ...] of the slots in use, ascending. */
@@ -384,8 +411,8 @@ int lbm_amr_read_active(lbm_amr* a, int32_t* list);
  * oldest first; as lbm_read_forces.  Replaces the pool force pass's
  * readback (common_d3_force_pool.wgsl, main-3d.js:3138-3141). */
 int lbm_amr_read_forces(lbm_amr* a, double* out, int32_t max_records, int32_t* count);
-/* FP64 [mass, px, py, pz] of the composite state (unrefined L0 cells plus
- * fine interiors weighted 1/8). */
+/* FP64 [mass, px, py, pz] of the composite state: unrefined L0 cells plus,
+ * at each level m, the interior cells no finer tile covers, weighted 8^-m. */
 int lbm_amr_read_totals(lbm_amr* a, double* out4);
 /* step count, tiles in use, wanted tiles left unplaced (sticky). */
 int lbm_amr_info(lbm_amr* a, int64_t* step, int32_t* active_tiles, int32_t* exhausted);
