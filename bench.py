@@ -180,9 +180,26 @@ def measure_plate():
         f = a.read_forces(1)
         amr_ms = (time.perf_counter() - t0) * 1e3 / (30 * 320 - 1)
     out["sphere_amr"] = float(f[-1, 0] / (0.5 * 0.05 ** 2 * np.pi * 64.0))
+    # the same sphere on the finest of three levels (sphere-amr-L3-Re20-D16,
+    # refineHierarchy with margin 2): 2.9015
+    adesc, amac, topo, _ = sc.sphere_amr(levels=3)
+    with pylbm.Amr(adesc) as a:
+        for m in (1, 2):
+            a.upload_level_topology(m, *topo[m])
+        a.upload_macro(amac)
+        a.step(1)
+        a.read_forces(1)
+        t0 = time.perf_counter()
+        a.step(30 * 320 - 1)
+        f = a.read_forces(1)
+        l3_ms = (time.perf_counter() - t0) * 1e3 / (30 * 320 - 1)
+    out["sphere_amr_l3"] = float(f[-1, 0] / (0.5 * 0.05 ** 2 * np.pi * 64.0))
     return {"sphere_re20_d16_cd": out["sphere_re20"], "sphere_re20_d16_cd_reference_measured": 2.9436,
             "sphere_amr_re20_d16_cd": out["sphere_amr"], "sphere_amr_re20_d16_cd_reference_measured": 2.9416,
             "sphere_amr_re20_d16_tiles": int((sb >= 0).sum()), "sphere_amr_re20_d16_ms_per_macro_step": amr_ms,
+            "sphere_amr_l3_re20_d16_cd": out["sphere_amr_l3"], "sphere_amr_l3_re20_d16_cd_reference_measured": 2.9015,
+            "sphere_amr_l3_re20_d16_tiles": [int((t[1] >= 0).sum()) for t in topo[1:]],
+            "sphere_amr_l3_re20_d16_ms_per_macro_step": l3_ms,
             "plate_cd_pinned": out["pinned"], "plate_cd_pinned_reference_measured": 1.2081, "plate_cd_literature": 1.18,
             "plate_cd_half_tow": out["half_tow"], "plate_half_tow_vs_pinned": out["half_tow"] / out["pinned"] - 1.0,
             "plate_half_tow_vs_pinned_reference_measured": -0.0321,

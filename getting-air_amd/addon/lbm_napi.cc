@@ -72,6 +72,7 @@ struct Handle {
   int q = 0;
   size_t nblocks = 0;     // amr: L0 blocks
   size_t slots = 0;       // amr: pool slots
+  int levels = 0;         // amr: levels of the hierarchy (L0 included)
   std::atomic<int> busy{0};
   bool dead = false;
 };
@@ -584,7 +585,7 @@ lbm_amr* amr_of(Handle* h) { return static_cast<lbm_amr*>(h->p); }
 //            manageEvery, refineAt, keepAt, lead, explodeLinear, orphans,
 //            interface: 'explode' | 'interp', reflux, explodeDirs,
 //            criterion: 'vorticity' | 'q', sponge: {width, axes, u},
-//            body: {a, centre}})
+//            body: {a, centre}, levels})
 napi_value CreateAmr(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
@@ -631,6 +632,7 @@ napi_value CreateAmr(napi_env env, napi_callback_info info) {
   }
   d.reflux = (int)num(env, o, "reflux", 0);
   d.explode_dirs = (int)num(env, o, "explodeDirs", 0);
+  d.levels = (int)num(env, o, "levels", 2);
   napi_value cv;
   if (get_prop(env, o, "criterion", &cv)) {
     char buf[16] = {0};
@@ -669,6 +671,7 @@ napi_value CreateAmr(napi_env env, napi_callback_info info) {
   h->cells = (size_t)d.nx * d.ny * d.nz;
   h->nblocks = d.rb > 0 ? (size_t)(d.nx / d.rb) * (d.ny / d.rb) * (d.nz / d.rb) : 0;
   h->slots = (size_t)d.max_slots;
+  h->levels = d.levels;
   return wrap(env, h);
 }
 
@@ -702,6 +705,37 @@ napi_value AmrTopology(napi_env env, napi_callback_info info) {
   lbm_amr* a = amr_of(ah_h);
   const int rc = Upload ? lbm_amr_upload_topology(a, static_cast<int32_t*>(bs), static_cast<int32_t*>(sb))
                         : lbm_amr_read_topology(a, static_cast<int32_t*>(bs), static_cast<int32_t*>(sb));
+  if (rc) return throw_lbm(env, rc);
+  return nullptr;
+}
+
+// amrUploadLevelTopology(amr, level, Int32Array blockSlot, Int32Array
+// slotToBlock): level m's maps over level m - 1's global grid (L0 dims times
+// 2^(m-1), blocks of rb of its cells), the reference's per-level
+// uploadLevelTopology (main-3d.js:822) fed from refineHierarchy's byLevel.
+napi_value AmrUploadLevelTopology(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void *bs, *sb;
+  size_t nb, ns;
+  int32_t level = 0;
+  if (argc < 4 || napi_get_value_int32(env, argv[1], &level) != napi_ok ||
+      !typed(env, argv[2], napi_int32_array, &bs, &nb) || !typed(env, argv[3], napi_int32_array, &sb, &ns)) {
+    napi_throw_type_error(env, nullptr, "expected (amr, level, Int32Array blockSlot, Int32Array slotToBlock)");
+    return nullptr;
+  }
+  GET(ah, argv[0], kAmr, true);
+  if (level < 1 || level >= ah_h->levels) {
+    napi_throw_range_error(env, nullptr, "level outside 1 .. levels - 1");
+    return nullptr;
+  }
+  // blocks at level m: the L0 block count times 8^(m-1)
+  const size_t want_blocks = ah_h->nblocks << (3 * (level - 1));
+  if (!long_enough(env, nb, want_blocks, "blockSlot") || !long_enough(env, ns, ah_h->slots, "slotToBlock"))
+    return nullptr;
+  const int rc = lbm_amr_upload_level_topology(amr_of(ah_h), level, static_cast<int32_t*>(bs),
+                                               static_cast<int32_t*>(sb));
   if (rc) return throw_lbm(env, rc);
   return nullptr;
 }
@@ -1185,6 +1219,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"amrDestroy", AmrDestroy},
       {"amrUploadTopology", AmrTopology<true>},
       {"amrReadTopology", AmrTopology<false>},
+      {"amrUploadLevelTopology", AmrUploadLevelTopology},
       {"amrUploadMacro", AmrFloat<amr_upload_macro>},
       {"amrReadMacro", AmrFloat<lbm_amr_read_macro>},
       {"amrReadCriterion", AmrReadCriterion},

@@ -303,6 +303,9 @@ export class AmrSolver {
     this.desc = scenario.desc;
     this.macroFn = scenario.macro;
     this.topology = scenario.topology || null;
+    // refineHierarchy's result (d3-amr.mjs:603-651) for desc.levels >= 3:
+    // byLevel[m] holds level m's blockSlot / slotToBlock
+    this.hierarchy = scenario.hierarchy || null;
     this.NX = this.desc.nx; this.NY = this.desc.ny; this.NZ = this.desc.nz;
     this.NCELLS = this.NX * this.NY * this.NZ;
     this.Q = this.desc.q;
@@ -321,6 +324,15 @@ export class AmrSolver {
   // (main-3d.js:822, :2973-3035).
   reset() {
     if (this.topology) this.addon.amrUploadTopology(this.handle, this.topology.blockSlot, this.topology.slotToBlock);
+    if (this.hierarchy) {
+      for (let m = 1; m < this.hierarchy.levels; m++) {
+        const lv = this.hierarchy.byLevel[m];
+        // the library wants maxSlots entries of slotToBlock, -1 past the used ones
+        const sb = new Int32Array(this.maxSlots).fill(-1);
+        sb.set(lv.slotToBlock.subarray(0, Math.min(lv.slotToBlock.length, this.maxSlots)));
+        this.addon.amrUploadLevelTopology(this.handle, m, Int32Array.from(lv.blockSlot), sb);
+      }
+    }
     this.addon.amrUploadMacro(this.handle, this.macroFn());
     this.step = 0;
     return Promise.resolve();

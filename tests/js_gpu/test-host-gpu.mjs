@@ -3,7 +3,7 @@
 // Run: node tests/js_gpu/test-host-gpu.mjs   (needs an MI355X)
 import { createRequire } from 'module';
 import { DenseSolver, AmrSolver, ServeClient, sphereScenario, tgvScenario, deviceInfo } from '../../getting-air_amd/js/lbm-host.mjs';
-import { makePool, refineWhere, refineNearBody, manageSlots, activeSlotList } from '../../getting-air_amd/js/d3-amr.mjs';
+import { makePool, refineWhere, refineNearBody, nearBodyWant, refineHierarchy, manageSlots, activeSlotList } from '../../getting-air_amd/js/d3-amr.mjs';
 
 const require = createRequire(import.meta.url);
 const O = require('../../oracle/lbm_oracle.js');
@@ -237,6 +237,26 @@ function relErr(gpuMac, oracleMac, comp) {
     const cd = b.fx / (0.5 * 0.05 * 0.05 * sc.params.area);
     assert(topo.activeSlots === 160 && Math.abs(cd / 2.9416 - 1) < 0.005, 'tiles ' + topo.activeSlots + ' Cd ' + cd);
     return topo.activeSlots + ' tiles, Cd ' + cd.toFixed(4) + ' (reference 2.9416)';
+  });
+
+  await ok('AMR sphere on the finest of three levels (sphere-amr-L3-Re20-D16) through the host', async () => {
+    const sc = sphereScenario({ re: 20, R: 8, u0: 0.05, spongeWidth: 8 });
+    const pool = makePool({ dims: sc.params.dims, rb: 4, maxSlots: pool_blocks(sc.params.dims) });
+    const c = sc.params.centre;
+    const sdf = (p) => Math.hypot(p[0] - c[0], p[1] - c[1], p[2] - c[2]) - 8;
+    const h = refineHierarchy(pool, { levels: 3, want: nearBodyWant(sdf, 2) });
+    const tiles = [h.byLevel[1].activeSlots, h.byLevel[2].activeSlots];
+    const desc = { q: 19, nx: sc.desc.nx, ny: sc.desc.ny, nz: sc.desc.nz, rb: 4, maxSlots: Math.max(...tiles),
+      omega: sc.desc.omega, sponge: { width: 8, axes: [1, 1, 1], u: [0.05, 0, 0] }, body: sc.desc.body, levels: 3 };
+    const s = new AmrSolver({ desc: desc, macro: sc.macro, hierarchy: h });
+    await s.reset();
+    await s.debugStepSync(30 * 320);
+    const b = await s.readBody();
+    s.destroy();
+    const cd = b.fx / (0.5 * 0.05 * 0.05 * sc.params.area);
+    assert(tiles[0] === 432 && tiles[1] === 1280 && Math.abs(cd / 2.9015 - 1) < 0.005 && Math.abs(cd / 2.9436 - 1) < 0.03,
+      'tiles ' + tiles + ' Cd ' + cd);
+    return tiles.join(' + ') + ' tiles, Cd ' + cd.toFixed(4) + ' (reference 2.9015)';
   });
 
   await ok('invalid descriptor raises with the library message', () => {
