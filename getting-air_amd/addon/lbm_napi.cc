@@ -585,7 +585,7 @@ lbm_amr* amr_of(Handle* h) { return static_cast<lbm_amr*>(h->p); }
 //            manageEvery, refineAt, keepAt, lead, explodeLinear, orphans,
 //            interface: 'explode' | 'interp', reflux, explodeDirs,
 //            criterion: 'vorticity' | 'q', sponge: {width, axes, u},
-//            body: {a, centre}, levels})
+//            body: {a, centre}, levels, ladderOctaves})
 napi_value CreateAmr(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
@@ -633,6 +633,7 @@ napi_value CreateAmr(napi_env env, napi_callback_info info) {
   d.reflux = (int)num(env, o, "reflux", 0);
   d.explode_dirs = (int)num(env, o, "explodeDirs", 0);
   d.levels = (int)num(env, o, "levels", 2);
+  d.ladder_octaves = (float)num(env, o, "ladderOctaves", 0);
   napi_value cv;
   if (get_prop(env, o, "criterion", &cv)) {
     char buf[16] = {0};

@@ -98,7 +98,23 @@ struct ManageArgs {
   int32_t* fresh;            // out: [count, (block, slot) pairs ...] ascending block id
   int32_t* active;           // out: [count, slot ...] ascending
   int32_t* free_list;        // scratch, max_slots
-  int32_t* status;           // [0] wanted blocks left without a slot (sticky sum)
+  int32_t* status;           // [0] wanted blocks left without a slot (sticky sum); [1] with
+                             // `decided`, wanted blocks whose geom bit 1 is set (sticky sum)
+  int32_t decided;           // 1: want[] already holds the decision (the manager at depth);
+                             // crit, refine_at, keep_at unused, geom only counted into status[1]
+};
+
+// The manager at depth decides every level's want before any level
+// allocates (common_d3_manage.wgsl decide, main-3d.js:3086-3107).
+struct DecideArgs {
+  int32_t nbx, nby, nbz;       // this level's blocks per axis
+  int32_t shift;               // m - 1: a level-m block's level-1 block is (b >> shift)
+  int32_t cnbx, cnby;          // the criterion's grid (level-1 blocks per axis)
+  const float* crit;           // per level-1 block, or null: no field term
+  float refine_at, keep_at;    // this level's rung of the ladder
+  const uint8_t* geom;         // per block, or null: bit 0 the body wants it, bit 1 the field may not
+  const int32_t* block_slot;   // held blocks are judged against keep_at
+  uint8_t* want;               // out
 };
 
 // Work lists of the surface passes, rebuilt with the cell classes after every
@@ -157,6 +173,15 @@ hipError_t launch_average(int q, const float* gpool, const ParentRef& parent, co
 // kind 0: |curl u|; 1: Q = 1/2 (|Omega|^2 - |S|^2)
 hipError_t launch_criterion(const float* mac, float* crit, const PoolGeom& g, float lead, int kind, hipStream_t s);
 hipError_t launch_manage(const ManageArgs& a, hipStream_t s);
+hipError_t launch_decide(const DecideArgs& a, hipStream_t s);
+// Octet completion of a level >= 2 (completeOctets): a block is wanted when
+// any block of its octet is (nbx, nby, nbz even).
+hipError_t launch_complete_octets(uint8_t* want, int nbx, int nby, int nbz, hipStream_t s);
+// The parent closure (balance): parent block p of the level with pnb blocks
+// per axis becomes wanted when any child block in [2p - 1, 2p + 2]^3
+// (periodic, on the child grid of 2 pnb per axis) is; ORs, never clears.
+hipError_t launch_balance(uint8_t* want_parent, int pnbx, int pnby, int pnbz, const uint8_t* want_child,
+                          hipStream_t s);
 // active = [count, slots with a block ...] ascending
 hipError_t launch_active_list(const int32_t* slot_to_block, int max_slots, int32_t* active, hipStream_t s);
 // pairs = [count, (block, slot) ...] of an active list (2 max_slots + 1 ints)

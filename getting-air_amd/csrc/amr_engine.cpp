@@ -43,8 +43,16 @@ struct DeepLevel {
   uint32_t nseam = 0, nnear = 0, nring = 0;
   CollideParams fine{};
   float c2f = 0.f;  // non-equilibrium rescale from level m - 1
+  float f2c = 0.f;  // and back to it (the restriction)
   int nactive = 0;
   std::vector<int32_t> h_block_slot, h_slot_to_block;  // host copies (topology checks)
+  // the manager (manage_every > 0)
+  uint8_t* want = nullptr;
+  uint8_t* geom = nullptr;     // level_mask
+  int32_t* dying = nullptr;
+  int32_t* fresh = nullptr;
+  int32_t* free_list = nullptr;
+  int32_t* status = nullptr;
 };
 
 struct lbm_amr {
@@ -89,7 +97,8 @@ struct lbm_amr {
   bool seeded = false;
   bool body_covered = false;  // the uploaded topology holds the body (check_topology)
   bool has_body = false;      // a pinned sphere, solved on the finest level
-  uint8_t* geom = nullptr;    // per block (manager with a body): geometry_mask
+  bool managed = false;       // the manager has changed the block maps since the host copies
+  uint8_t* geom = nullptr;    // per level-1 block (the manager): level_mask
 };
 
 namespace {
@@ -140,6 +149,7 @@ int validate(const lbm_amr_desc* d) {
   if (d->collision == LBM_TRT && !(d->omega_minus > 0.f && d->omega_minus < 2.f))
     return fail(LBM_E_INVALID, "omega_minus %g outside (0, 2)", d->omega_minus);
   if (d->manage_every < 0 || d->lead < 0) return fail(LBM_E_INVALID, "manage_every / lead must be >= 0");
+  if (!(d->ladder_octaves <= 16.f)) return fail(LBM_E_INVALID, "ladder_octaves %g outside [0, 16]", d->ladder_octaves);
   // the flux correction measures a mismatch that explode/coalesce does not
   // create (main-3d.js:1333-1336)
   if (d->reflux && !d->interface_interp)
@@ -149,8 +159,6 @@ int validate(const lbm_amr_desc* d) {
   if (levels >= 3) {
     // the pool-parent coupling is explode/coalesce (main-3d.js:1283-1288)
     if (d->interface_interp) return fail(LBM_E_REFUSED, "levels %d needs the explode interface", levels);
-    if (d->manage_every > 0)
-      return fail(LBM_E_INVALID, "levels %d: the hierarchy is static here (manage_every = 0)", levels);
     // explode's ring plan packs parent-grid coordinates in 10 bits each
     const int s = 1 << (levels - 2);
     if ((long)d->nx * s > 1024 || (long)d->ny * s > 1024 || (long)d->nz * s > 1024)
@@ -200,26 +208,38 @@ bool near_sphere(const double c[3], double h, int depth, const double ctr[3], do
   return false;
 }
 
-// Per block: bit 0 the body's geometry wants it refined, bit 1 it reaches
-// into the sponge band (the fine step does not relax toward the far field),
-// so the field criterion may not refine it.
-std::vector<uint8_t> geometry_mask(const lbm_amr_desc& d) {
-  const int rb = d.rb, nbx = d.nx / rb, nby = d.ny / rb, nbz = d.nz / rb;
+// Per block of level m (rb cells of level m - 1, an L0 box of edge
+// rb 2^-(m-1)): bit 0 (the finest level, with a body) the body's geometry
+// wants it refined, whatever the field says; bit 1 the field criterion may
+// not refine it, because its L0 box comes within the sponge band (the fine
+// step does not relax toward the far field) dilated by how far the 2:1
+// closure can carry a level-m want toward level 1: octet completion and the
+// 27-neighbour parent gather spread a want by at most three of its own
+// blocks per level, 3 rb (1 - 2^-(m-1)) L0 cells down to level 1 in all.
+// The box is scaled as refineHierarchy scales it (a pure 2^-(m-1), the
+// BOX_SCALE of common_d3_manage.wgsl:150-163), so the geometry set equals
+// the host hierarchy's.
+std::vector<uint8_t> level_mask(const lbm_amr_desc& d, int m, bool body) {
+  const int s = 1 << (m - 1), rb = d.rb;
+  const int nbx = d.nx * s / rb, nby = d.ny * s / rb, nbz = d.nz * s / rb;
   const int n[3] = {d.nx, d.ny, d.nz};
+  const double sc = std::ldexp(1.0, -(m - 1)), edge = rb * sc, dil = 3.0 * rb * (1.0 - sc);
   const double ctr[3] = {d.body.centre[0], d.body.centre[1], d.body.centre[2]};
-  std::vector<uint8_t> m((size_t)nbx * nby * nbz, 0);
-  for (int b = 0; b < nbx * nby * nbz; b++) {
-    const int bc[3] = {b % nbx, (b / nbx) % nby, b / (nbx * nby)};
-    const double c[3] = {bc[0] * rb + rb / 2.0, bc[1] * rb + rb / 2.0, bc[2] * rb + rb / 2.0};
-    if (near_sphere(c, rb / 2.0, 3, ctr, d.body.radius, kBodyMargin)) m[b] |= 1;
+  std::vector<uint8_t> mask((size_t)nbx * nby * nbz, 0);
+  for (size_t b = 0; b < mask.size(); b++) {
+    const int bc[3] = {(int)(b % nbx), (int)((b / nbx) % nby), (int)(b / ((size_t)nbx * nby))};
+    if (body) {
+      const double c[3] = {(bc[0] * rb + rb / 2.0) * sc, (bc[1] * rb + rb / 2.0) * sc, (bc[2] * rb + rb / 2.0) * sc};
+      if (near_sphere(c, edge / 2.0, 3, ctr, d.body.radius, kBodyMargin)) mask[b] |= 1;
+    }
     if (d.sponge_width > 0.f) {
       for (int k = 0; k < 3; k++) {
-        const int lo = bc[k] * rb, hi = lo + rb - 1;
-        if (d.sponge_axes[k] && (lo < d.sponge_width || (n[k] - 1) - hi < d.sponge_width)) m[b] |= 2;
+        const double lo = bc[k] * edge;
+        if (d.sponge_axes[k] && (lo < d.sponge_width + dil || lo + edge > n[k] - d.sponge_width - dil)) mask[b] |= 2;
       }
     }
   }
-  return m;
+  return mask;
 }
 #pragma clang fp contract(on)
 
@@ -264,7 +284,8 @@ void release(lbm_amr* a) {
     if (p) (void)hipFree(p);
   for (DeepLevel& L : a->deep) {
     void* dp[] = {L.pool[0], L.pool[1], L.mac_pool, L.block_slot, L.slot_to_block, L.active, L.pairs, L.ccls,
-                  L.tile_table, L.plan.seam, L.plan.near, L.plan.ring, L.plan.count};
+                  L.tile_table, L.plan.seam, L.plan.near, L.plan.ring, L.plan.count, L.want, L.geom,
+                  L.dying, L.fresh, L.free_list, L.status};
     for (void* p : dp)
       if (p) (void)hipFree(p);
   }
@@ -432,6 +453,25 @@ int check_level_topology(lbm_amr* a, int m, const int32_t* bs) {
   return LBM_OK;
 }
 
+// Host copies of every level's block map, from the device (after the
+// manager has moved tiles; the level uploads check against them).
+int sync_host_maps(lbm_amr* a) {
+  a->h_block_slot1.resize(a->nblocks);
+  HIPCHK(hipMemcpyAsync(a->h_block_slot1.data(), a->block_slot, (size_t)a->nblocks * 4, hipMemcpyDeviceToHost,
+                        a->stream));
+  for (DeepLevel& L : a->deep) {
+    L.h_block_slot.resize(L.nblocks);
+    L.h_slot_to_block.resize(a->d.max_slots);
+    HIPCHK(hipMemcpyAsync(L.h_block_slot.data(), L.block_slot, (size_t)L.nblocks * 4, hipMemcpyDeviceToHost,
+                          a->stream));
+    HIPCHK(hipMemcpyAsync(L.h_slot_to_block.data(), L.slot_to_block, (size_t)a->d.max_slots * 4,
+                          hipMemcpyDeviceToHost, a->stream));
+  }
+  HIPCHK(hipStreamSynchronize(a->stream));
+  a->managed = false;
+  return LBM_OK;
+}
+
 // Restricts every tile in use onto the L0 cells under it (the average/drain
 // pass).  Between manager passes the covered L0 step leaves those cells
 // untouched, so this is what makes them a valid interpolation source for a
@@ -444,8 +484,112 @@ int restrict_active(lbm_amr* a) {
   return LBM_OK;
 }
 
+// The field ladder's rung at level m: refine and keep thresholds scaled by
+// r^(m-1), r = 2^octaves for a rotation rate, 4^octaves for Q.
+void ladder_rung(const lbm_amr_desc& d, int m, float* refine_at, float* keep_at) {
+  const double oct = d.ladder_octaves > 0.f ? d.ladder_octaves : 1.0;
+  const double f = std::exp2(oct * (d.criterion == 1 ? 2.0 : 1.0) * (m - 1));
+  *refine_at = (float)(d.refine_at * f);
+  *keep_at = (float)(d.keep_at * f);
+}
+
+// The manager over a hierarchy of three or four levels (main-3d.js:3251-3321):
+// the criterion on L0's field, every tile restricted into its parent
+// finest-first (the drain; it also makes covered parent cells a valid fill
+// source), every level's want (the body at the finest level, the field on
+// its rung), the 2:1 closure finest-first, each level's slots (the ordered
+// allocator of the two-level manager), the plans rebuilt, and the new tiles
+// filled from their parents coarsest-first, so that a tile whose parent is
+// also new reads the parent's fill.
+int manage_deep(lbm_amr* a) {
+  const lbm_amr_desc& d = a->d;
+  const int D = a->levels - 1;
+  HIPCHK(launch_criterion(solve_mac(a->l0), a->crit, a->G, (float)d.lead, d.criterion, a->stream));
+  for (int m = D; m >= 2; m--) {
+    DeepLevel& L = a->deep[m - 2];
+    if (L.nactive <= 0) continue;
+    HIPCHK(launch_active_pairs(L.active, L.slot_to_block, d.max_slots, L.pairs, a->stream));
+    HIPCHK(launch_average(d.q, L.pool[0], pool_parent(a, m - 1, 0), L.G, L.pairs, L.nactive, L.f2c, a->stream));
+  }
+  int rc = restrict_active(a);
+  if (rc) return rc;
+  for (int m = 1; m <= D; m++) {
+    DecideArgs x{};
+    const PoolGeom& G = m == 1 ? a->G : a->deep[m - 2].G;
+    x.nbx = G.nbx;
+    x.nby = G.nby;
+    x.nbz = G.nbz;
+    x.shift = m - 1;
+    x.cnbx = a->G.nbx;
+    x.cnby = a->G.nby;
+    x.crit = a->crit;
+    ladder_rung(d, m, &x.refine_at, &x.keep_at);
+    x.geom = m == 1 ? a->geom : a->deep[m - 2].geom;
+    x.block_slot = G.block_slot;
+    x.want = m == 1 ? a->want : a->deep[m - 2].want;
+    HIPCHK(launch_decide(x, a->stream));
+  }
+  for (int m = D; m >= 2; m--) {
+    DeepLevel& L = a->deep[m - 2];
+    HIPCHK(launch_complete_octets(L.want, L.G.nbx, L.G.nby, L.G.nbz, a->stream));
+    const PoolGeom& P = m == 2 ? a->G : a->deep[m - 3].G;
+    HIPCHK(launch_balance(m == 2 ? a->want : a->deep[m - 3].want, P.nbx, P.nby, P.nbz, L.want, a->stream));
+  }
+  for (int m = 1; m <= D; m++) {
+    ManageArgs x{};
+    const bool top = m == 1;
+    DeepLevel* L = top ? nullptr : &a->deep[m - 2];
+    x.nblocks = top ? a->nblocks : L->nblocks;
+    x.max_slots = d.max_slots;
+    x.geom = top ? a->geom : nullptr;  // level 1: count wanted blocks in the sponge band
+    x.block_slot = top ? a->block_slot : L->block_slot;
+    x.slot_to_block = top ? a->slot_to_block : L->slot_to_block;
+    x.want = top ? a->want : L->want;
+    x.dying = top ? a->dying : L->dying;
+    x.fresh = top ? a->fresh : L->fresh;
+    x.active = top ? a->active : L->active;
+    x.free_list = top ? a->free_list : L->free_list;
+    x.status = top ? a->status : L->status;
+    x.decided = 1;
+    HIPCHK(launch_manage(x, a->stream));
+  }
+  std::vector<int32_t> counts(4 * D, 0);  // per level: fresh, active, status[0], status[1]
+  for (int m = 1; m <= D; m++) {
+    int32_t* c = &counts[4 * (m - 1)];
+    const bool top = m == 1;
+    HIPCHK(hipMemcpyAsync(&c[0], top ? a->fresh : a->deep[m - 2].fresh, 4, hipMemcpyDeviceToHost, a->stream));
+    HIPCHK(hipMemcpyAsync(&c[1], top ? a->active : a->deep[m - 2].active, 4, hipMemcpyDeviceToHost, a->stream));
+    HIPCHK(hipMemcpyAsync(&c[2], top ? a->status : a->deep[m - 2].status, 8, hipMemcpyDeviceToHost, a->stream));
+  }
+  HIPCHK(hipStreamSynchronize(a->stream));
+  a->nactive = counts[1];
+  for (int m = 2; m <= D; m++) a->deep[m - 2].nactive = counts[4 * (m - 1) + 1];
+  if ((rc = rebuild_plan(a))) return rc;
+  for (int m = 2; m <= D; m++)
+    if ((rc = rebuild_level_plan(a, a->deep[m - 2]))) return rc;
+  if ((rc = fill_tiles(a, a->fresh, counts[0]))) return rc;
+  for (int m = 2; m <= D; m++) {
+    DeepLevel& L = a->deep[m - 2];
+    const int n = counts[4 * (m - 1)];
+    if (n > 0)
+      HIPCHK(launch_interp(d.q, pool_parent(a, m - 1, 0), L.pool[0], L.G, L.fresh, n, L.c2f, 0, a->stream));
+  }
+  int exhausted = 0;
+  for (int m = 1; m <= D; m++) exhausted += counts[4 * (m - 1) + 2];
+  a->exhausted = exhausted;
+  if (counts[3] > 0)
+    return fail(LBM_E_INVALID, "the 2:1 closure carried %d level-1 blocks into the sponge band", counts[3]);
+  if (exhausted > 0)
+    return fail(LBM_E_NOMEM, "AMR pool exhausted: %d wanted tiles found no free slot (max_slots %d per level)",
+                exhausted, d.max_slots);
+  if (a->has_body) a->body_covered = true;
+  return LBM_OK;
+}
+
 int manage(lbm_amr* a) {
   const lbm_amr_desc& d = a->d;
+  a->managed = true;
+  if (a->levels > 2) return manage_deep(a);
   HIPCHK(launch_criterion(solve_mac(a->l0), a->crit, a->G, (float)d.lead, d.criterion, a->stream));
   int rc = restrict_active(a);
   if (rc) return rc;
@@ -627,6 +771,17 @@ int create_deep_levels(lbm_amr* a, double tau1) {
     G.slot_to_block = L.slot_to_block;
     G.ccls = L.ccls;
     G.tile_table = L.tile_table;
+    if (d.manage_every > 0) {
+      const std::vector<uint8_t> mask = level_mask(d, m, a->has_body && m == a->levels - 1);
+      if ((rc = dalloc(&L.want, (size_t)L.nblocks)) || (rc = dalloc(&L.geom, (size_t)L.nblocks)) ||
+          (rc = dalloc(&L.dying, (size_t)(2 * d.max_slots + 1) * 4)) ||
+          (rc = dalloc(&L.fresh, (size_t)(2 * d.max_slots + 1) * 4)) ||
+          (rc = dalloc(&L.free_list, (size_t)d.max_slots * 4)) || (rc = dalloc(&L.status, 16)))
+        return rc;
+      HIPCHK(hipMemcpy(L.geom, mask.data(), mask.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMemsetAsync(L.want, 0, (size_t)L.nblocks, a->stream));
+      HIPCHK(hipMemsetAsync(L.status, 0, 16, a->stream));
+    }
     HIPCHK(hipMemsetAsync(L.block_slot, 0xff, (size_t)L.nblocks * 4, a->stream));
     HIPCHK(hipMemsetAsync(L.slot_to_block, 0xff, (size_t)d.max_slots * 4, a->stream));
     HIPCHK(hipMemsetAsync(L.active, 0, 4, a->stream));
@@ -636,6 +791,7 @@ int create_deep_levels(lbm_amr* a, double tau1) {
     if (d.q == 19) fine_params<19>(L.fine, (float)(1.0 / tau), (float)(1.0 / tau), d.force);
     else fine_params<27>(L.fine, (float)(1.0 / tau), (float)(1.0 / tau), d.force);
     L.c2f = (float)(0.5 * (tau - 1.0) / (tau_p - 1.0));
+    L.f2c = (float)(2.0 * (tau_p - 1.0) / (tau - 1.0));
     tau_p = tau;
   }
   // each level knows its child's block map (coveredByChild)
@@ -734,14 +890,15 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
       release(a);
       return rc;
     }
-    if (desc->manage_every > 0) {
-      const std::vector<uint8_t> m = geometry_mask(*desc);
-      hipError_t e = hipSuccess;
-      if ((rc = dalloc(&a->geom, m.size())) ||
-          (e = hipMemcpy(a->geom, m.data(), m.size(), hipMemcpyHostToDevice)) != hipSuccess) {
-        release(a);
-        return rc ? rc : hip_fail(e, "geometry mask");
-      }
+  }
+  if (desc->manage_every > 0) {
+    // the body is wanted on the finest level alone
+    const std::vector<uint8_t> m = level_mask(*desc, 1, a->has_body && a->levels == 2);
+    hipError_t e = hipSuccess;
+    if ((rc = dalloc(&a->geom, m.size())) ||
+        (e = hipMemcpy(a->geom, m.data(), m.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+      release(a);
+      return rc ? rc : hip_fail(e, "geometry mask");
     }
   }
   if (desc->reflux) {
@@ -796,6 +953,10 @@ int lbm_amr_destroy(lbm_amr* a) {
 
 int lbm_amr_upload_topology(lbm_amr* a, const int32_t* block_slot, const int32_t* slot_to_block) {
   if (!a || !block_slot || !slot_to_block) return fail(LBM_E_INVALID, "null argument");
+  if (a->managed) {
+    int rc0 = sync_host_maps(a);
+    if (rc0) return rc0;
+  }
   // the two maps must be inverse to each other
   for (int b = 0; b < a->nblocks; b++) {
     const int s = block_slot[b];
@@ -841,6 +1002,11 @@ int lbm_amr_upload_topology(lbm_amr* a, const int32_t* block_slot, const int32_t
 int lbm_amr_upload_level_topology(lbm_amr* a, int32_t level, const int32_t* block_slot,
                                   const int32_t* slot_to_block) {
   if (!a || !block_slot || !slot_to_block) return fail(LBM_E_INVALID, "null argument");
+  if (a->managed) {
+    // the manager moved tiles since the host copies were taken
+    int rc0 = sync_host_maps(a);
+    if (rc0) return rc0;
+  }
   if (level == 1) return lbm_amr_upload_topology(a, block_slot, slot_to_block);
   if (level < 1 || level >= a->levels) return fail(LBM_E_INVALID, "level %d outside 1 .. %d", level, a->levels - 1);
   DeepLevel& L = a->deep[level - 2];
@@ -878,6 +1044,18 @@ int lbm_amr_read_topology(lbm_amr* a, int32_t* block_slot, int32_t* slot_to_bloc
   if (block_slot) HIPCHK(hipMemcpyAsync(block_slot, a->block_slot, (size_t)a->nblocks * 4, hipMemcpyDeviceToHost, a->stream));
   if (slot_to_block)
     HIPCHK(hipMemcpyAsync(slot_to_block, a->slot_to_block, (size_t)a->d.max_slots * 4, hipMemcpyDeviceToHost, a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  return LBM_OK;
+}
+
+int lbm_amr_read_level_topology(lbm_amr* a, int32_t level, int32_t* block_slot, int32_t* slot_to_block) {
+  if (!a) return fail(LBM_E_INVALID, "null argument");
+  if (level == 1) return lbm_amr_read_topology(a, block_slot, slot_to_block);
+  if (level < 1 || level >= a->levels) return fail(LBM_E_INVALID, "level %d outside 1 .. %d", level, a->levels - 1);
+  const DeepLevel& L = a->deep[level - 2];
+  if (block_slot) HIPCHK(hipMemcpyAsync(block_slot, L.block_slot, (size_t)L.nblocks * 4, hipMemcpyDeviceToHost, a->stream));
+  if (slot_to_block)
+    HIPCHK(hipMemcpyAsync(slot_to_block, L.slot_to_block, (size_t)a->d.max_slots * 4, hipMemcpyDeviceToHost, a->stream));
   HIPCHK(hipStreamSynchronize(a->stream));
   return LBM_OK;
 }

@@ -728,10 +728,13 @@ k_interp(ParentRef P, float* __restrict__ gp, PoolGeom G, const int32_t* __restr
   const float pz = (float)(bz * RB) - 0.25f + 0.5f * (float)(fz - kGhost);
   const int c0x = (int)floorf(px), c0y = (int)floorf(py), c0z = (int)floorf(pz);
   const float tx = px - c0x, ty = py - c0y, tz = pz - c0z;
-  // A pool parent is sampled in the one parent tile that holds this block,
-  // ring included: the stencil of a block's whole tile reaches parent-local
-  // [0, FB) (common_interp_parent_pool.wgsl:17-28), so it needs no wrap and
-  // no second tile.
+  // A pool parent cell is read in the parent tile that OWNS it, at its
+  // interior index (parentIndex, common_d3_parent_pool.wgsl:37-59), so a
+  // tile filled mid-run sees solved parent cells rather than a parent ring
+  // that only advects.  Where no parent tile owns a stencil cell (the outer
+  // ring's stencil reaches one parent cell beyond the ring-parent cover),
+  // the cell is read in the ring of the parent tile that holds this block,
+  // which the stencil of a whole tile never leaves (parent-local [0, FB)).
   int ps = 0, pox = 0, poy = 0, poz = 0;
   if constexpr (PP) {
     constexpr int SH = Tile<RB>::LOG_RB + 1;
@@ -748,8 +751,9 @@ k_interp(ParentRef P, float* __restrict__ gp, PoolGeom G, const int32_t* __restr
   for (int s = 0; s < 8; s++) {
     const int i = s & 1, j = (s >> 1) & 1, kk = (s >> 2) & 1;
     const float w = (i ? tx : 1.f - tx) * (j ? ty : 1.f - ty) * (kk ? tz : 1.f - tz);
-    const uint32_t cell = PP ? pool_cell<RB>(ps, c0x + i + pox, c0y + j + poy, c0z + kk + poz)
-                             : coarse_cell(G, wrap1(c0x + i, G.nx), wrap1(c0y + j, G.ny), wrap1(c0z + kk, G.nz));
+    const int vx = wrap1(c0x + i, G.nx), vy = wrap1(c0y + j, G.ny), vz = wrap1(c0z + kk, G.nz);
+    uint32_t cell = parent_index<PP, RB>(G, P, vx, vy, vz);
+    if (PP && cell == kAbsent) cell = pool_cell<RB>(ps, c0x + i + pox, c0y + j + poy, c0z + kk + poz);
     float sr, sx, sy, sz, sf[Q];
     coarse_sample<Q, PP>(P, cell, sr, sx, sy, sz, sf);
     rho += w * sr;
@@ -1026,13 +1030,20 @@ __global__ void __launch_bounds__(1024) k_manage(ManageArgs a) {
   const int tid = threadIdx.x;
   // 1. decide, with hysteresis (common_d3_manage.wgsl decide: a held block
   //    is judged against the lower coarsen threshold), the body's geometry
-  //    OR the field (:443-462)
-  for (int b = tid; b < a.nblocks; b += 1024) {
-    const bool held = a.block_slot[b] >= 0;
-    const float c = a.crit[b];
-    const uint8_t gm = a.geom ? a.geom[b] : 0;
-    const bool field = !(gm & 2) && (held ? c >= a.keep_at : c >= a.refine_at);
-    a.want[b] = ((gm & 1) || field) ? 1 : 0;
+  //    OR the field (:443-462).  Decided beforehand at depth; there the
+  //    geometry mask only counts wanted blocks in the sponge band.
+  if (!a.decided) {
+    for (int b = tid; b < a.nblocks; b += 1024) {
+      const bool held = a.block_slot[b] >= 0;
+      const float c = a.crit[b];
+      const uint8_t gm = a.geom ? a.geom[b] : 0;
+      const bool field = !(gm & 2) && (held ? c >= a.keep_at : c >= a.refine_at);
+      a.want[b] = ((gm & 1) || field) ? 1 : 0;
+    }
+  } else if (a.geom) {
+    int n = 0;
+    for (int b = tid; b < a.nblocks; b += 1024) n += (a.want[b] && (a.geom[b] & 2)) ? 1 : 0;
+    if (n) atomicAdd(&a.status[1], n);
   }
   __syncthreads();
   // 2. dying blocks (held, no longer wanted), ascending
@@ -1079,6 +1090,50 @@ __global__ void __launch_bounds__(1024) k_manage(ManageArgs a) {
     a.active[0] = nactive;
     a.status[0] += max(0, nwant - nfree);
   }
+}
+
+// ---------------------------------------------------------------------------
+// the manager at depth: every level's want, then the 2:1 closure finest-first
+// (common_d3_manage.wgsl decide / completeOctets / balance), each pass one
+// thread per output block reading what it does not write
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_decide(DecideArgs a) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.nbx * a.nby * a.nbz) return;
+  const uint8_t gm = a.geom ? a.geom[b] : 0;
+  // the geometry first: a block the body needs is never put to the field
+  bool w = (gm & 1) != 0;
+  if (!w && a.crit && !(gm & 2)) {
+    const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
+    const float c = a.crit[((bz >> a.shift) * a.cnby + (by >> a.shift)) * a.cnbx + (bx >> a.shift)];
+    w = a.block_slot[b] >= 0 ? c >= a.keep_at : c >= a.refine_at;
+  }
+  a.want[b] = w ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(256) k_complete_octets(uint8_t* want, int nbx, int nby, int nbz) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ox = nbx >> 1, oy = nby >> 1, oz = nbz >> 1;
+  if (o >= ox * oy * oz) return;
+  const int x = 2 * (o % ox), y = 2 * ((o / ox) % oy), z = 2 * (o / (ox * oy));
+  uint8_t any = 0;
+  for (int k = 0; k < 8; k++) any |= want[((z + (k >> 2)) * nby + y + ((k >> 1) & 1)) * nbx + x + (k & 1)];
+  for (int k = 0; k < 8; k++) want[((z + (k >> 2)) * nby + y + ((k >> 1) & 1)) * nbx + x + (k & 1)] = any;
+}
+
+__global__ void __launch_bounds__(256) k_balance(uint8_t* wp, int pnbx, int pnby, int pnbz, const uint8_t* wc) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pnbx * pnby * pnbz) return;
+  const int px = p % pnbx, py = (p / pnbx) % pnby, pz = p / (pnbx * pnby);
+  const int cnx = 2 * pnbx, cny = 2 * pnby, cnz = 2 * pnbz;
+  uint8_t any = 0;
+  for (int k = 0; k < 64 && !any; k++) {
+    const int cx = wrap1(2 * px + (k & 3) - 1, cnx);
+    const int cy = wrap1(2 * py + ((k >> 2) & 3) - 1, cny);
+    const int cz = wrap1(2 * pz + (k >> 4) - 1, cnz);
+    any = wc[(cz * cny + cy) * cnx + cx];
+  }
+  if (any) wp[p] = 1;
 }
 
 // Ordered active-slot list only (after a topology upload).
@@ -1392,6 +1447,24 @@ hipError_t launch_criterion(const float* mac, float* crit, const PoolGeom& g, fl
 
 hipError_t launch_manage(const ManageArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_manage, dim3(1), dim3(1024), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_decide(const DecideArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_decide, grid_for((uint64_t)a.nbx * a.nby * a.nbz), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_complete_octets(uint8_t* want, int nbx, int nby, int nbz, hipStream_t s) {
+  hipLaunchKernelGGL(k_complete_octets, grid_for((uint64_t)(nbx / 2) * (nby / 2) * (nbz / 2)), dim3(256), 0, s,
+                     want, nbx, nby, nbz);
+  return hipGetLastError();
+}
+
+hipError_t launch_balance(uint8_t* want_parent, int pnbx, int pnby, int pnbz, const uint8_t* want_child,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_balance, grid_for((uint64_t)pnbx * pnby * pnbz), dim3(256), 0, s, want_parent, pnbx, pnby,
+                     pnbz, want_child);
   return hipGetLastError();
 }
 

@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblbm.so")
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 BGK, TRT = 0, 1
 STORE = {"f32": 0, "f16": 1, "bf16": 2}
 BODY_NONE, BODY_SPHERE, BODY_CIRCLE = 0, 1, 2
@@ -27,7 +27,7 @@ EXPORTS = (
     "lbm_amr_upload_macro", "lbm_amr_step", "lbm_amr_manage_now", "lbm_amr_read_macro",
     "lbm_amr_read_pool_macro", "lbm_amr_read_f", "lbm_amr_read_criterion", "lbm_amr_read_active",
     "lbm_amr_read_totals", "lbm_amr_info", "lbm_amr_read_forces", "lbm_amr_upload_level_topology",
-    "lbm_amr_read_level_f", "lbm_amr_read_level_macro",
+    "lbm_amr_read_level_f", "lbm_amr_read_level_macro", "lbm_amr_read_level_topology",
     "lbm_serve_create", "lbm_serve_destroy", "lbm_serve_submit", "lbm_serve_wait", "lbm_serve_read_macro",
     "lbm_serve_read_forces", "lbm_serve_release", "lbm_serve_drain", "lbm_serve_read_stats", "lbm_serve_clock",
     "lbm_serve_submit_state", "lbm_serve_read_state", "lbm_rigid_attach", "lbm_rigid_read",
@@ -64,7 +64,7 @@ class AmrDesc(ctypes.Structure):
         ("interface_interp", ctypes.c_int32), ("reflux", ctypes.c_int32),
         ("sponge_width", ctypes.c_float), ("sponge_axes", ctypes.c_int32 * 3), ("sponge_u", ctypes.c_float * 3),
         ("body", Body), ("explode_dirs", ctypes.c_int32), ("criterion", ctypes.c_int32),
-        ("levels", ctypes.c_int32),
+        ("levels", ctypes.c_int32), ("ladder_octaves", ctypes.c_float),
     ]
 
 
@@ -190,6 +190,7 @@ def load():
         lib.lbm_amr_destroy.argtypes = [P]
         lib.lbm_amr_upload_topology.argtypes = [P, P, P]
         lib.lbm_amr_read_topology.argtypes = [P, P, P]
+        lib.lbm_amr_read_level_topology.argtypes = [P, ctypes.c_int32, P, P]
         lib.lbm_amr_upload_macro.argtypes = [P, P]
         lib.lbm_amr_step.argtypes = [P, ctypes.c_int32]
         lib.lbm_amr_manage_now.argtypes = [P]
@@ -414,7 +415,8 @@ def make_amr_desc(d):
     L0, body: {a, centre} a pinned sphere solved on the fine level,
     explodeDirs: directions per explode thread (0: automatic),
     criterion: "vorticity" (|curl u|, default) or "q", levels: 2 (default),
-    3 or 4 including L0}."""
+    3 or 4 including L0, ladderOctaves: octaves of rotation rate each
+    further level's field rung costs (default 1)}."""
     a = AmrDesc()
     a.abi_version = ABI_VERSION
     a.q = int(d["q"])
@@ -447,6 +449,7 @@ def make_amr_desc(d):
     a.explode_dirs = int(d.get("explodeDirs", 0))
     a.criterion = {"vorticity": 0, "q": 1}[d.get("criterion", "vorticity")]
     a.levels = int(d.get("levels", 2))
+    a.ladder_octaves = float(d.get("ladderOctaves", 0.0))
     body = d.get("body")
     if body:
         a.body.shape = BODY_SPHERE
@@ -530,6 +533,13 @@ class Amr:
         bs = np.empty(self.nblocks, dtype=np.int32)
         sb = np.empty(self.max_slots, dtype=np.int32)
         _check(self.lib.lbm_amr_read_topology(self.h, bs.ctypes.data, sb.ctypes.data))
+        return bs, sb
+
+    def read_level_topology(self, level):
+        """Level `level`'s (block_slot, slot_to_block) as the manager left them."""
+        bs = np.empty(self.level_blocks(level), dtype=np.int32)
+        sb = np.empty(self.max_slots, dtype=np.int32)
+        _check(self.lib.lbm_amr_read_level_topology(self.h, int(level), bs.ctypes.data, sb.ctypes.data))
         return bs, sb
 
     def upload_macro(self, mac4):

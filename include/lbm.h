@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 6
+#define LBM_ABI_VERSION 7
 
 enum {
   LBM_OK = 0,
@@ -345,8 +345,23 @@ typedef struct lbm_amr_desc {
                          * pool.wgsl; the per-level schedule, main-3d.js:
                          * 3169-3180): tau_m = 2 tau_(m-1) - 1/2, the body on
                          * the finest level alone.  Levels >= 3 take the
-                         * explode interface and a static hierarchy
-                         * (manage_every = 0), uploaded level by level.      */
+                         * explode interface; a static hierarchy is uploaded
+                         * level by level, and with manage_every > 0 the
+                         * manager decides every level: the body's geometry
+                         * at the finest level, the field per level on the
+                         * ladder below, then the 2:1 closure (octets and
+                         * the 27-neighbour parent gather, common_d3_manage.
+                         * wgsl completeOctets/balance), drain finest-first
+                         * and fill coarsest-first (main-3d.js:3251-3321). */
+  float ladder_octaves; /* the field ladder (desiredLevelAt, common_refine.
+                         * wgsl:54-79): each further level costs this many
+                         * octaves of rotation rate (N_REFINE_INC, ?qinc=),
+                         * so level m wants a block when the criterion
+                         * reaches refine_at x r^(m-1) (keep_at x r^(m-1)
+                         * for a held one), r = 2^octaves for |curl u| and
+                         * 4^octaves for Q (a squared rate).  <= 0 reads as
+                         * 1.  The criterion is the L0 field's, per level-1
+                         * block, as the reference reads it.                 */
 } lbm_amr_desc;
 
 typedef struct lbm_amr lbm_amr;
@@ -361,6 +376,10 @@ int lbm_amr_destroy(lbm_amr* a);
  * Replaces uploadLevelTopology, main-3d.js:822. */
 int lbm_amr_upload_topology(lbm_amr* a, const int32_t* block_slot, const int32_t* slot_to_block);
 int lbm_amr_read_topology(lbm_amr* a, int32_t* block_slot, int32_t* slot_to_block);
+/* Level m's block maps (blockSlot over level m - 1's global grid, L0 dims x
+ * 2^(m-1) / rb blocks; slotToBlock[max_slots]), as the manager left them
+ * (poolStateAt, main-3d.js:4248-4260). */
+int lbm_amr_read_level_topology(lbm_amr* a, int32_t level, int32_t* block_slot, int32_t* slot_to_block);
 
 /* Block maps of pool level `level` (1 .. levels - 1): blockSlot over the
  * level's blocks (the parent level's global grid, L0's times 2^(level-1),

@@ -10,9 +10,9 @@ same FB^3 shape, and tau_m = 2 tau_(m-1) - 1/2.  What differs at m >= 2 is
 only where a parent cell's data lives (the parent level's tile that owns it,
 common_d3_parent_pool.wgsl:37-59), so explode and coalesce run here against
 a dense view of the parent level's tile interiors (absent cells are NaN) and
-write back into those tiles; the interpolation fill samples the one parent
-tile that holds the child block, ring included (common_interp_parent_pool.
-wgsl:17-28).
+write back into those tiles; the interpolation fill reads each parent cell
+in the parent tile that owns it, falling back to the ring of the parent tile
+that holds the child block where no parent tile owns it.
 
   schedule            advanceLevel, main-3d.js:3169-3180: explode level m +
                       1 from level m, two substeps of level m + 1, coalesce
@@ -38,13 +38,16 @@ hierarchy whose third level has no tile reproducing the two-level run.
 """
 import numpy as np
 
-from amr_oracle import GHOST, Tables, active_list, collide, feq, moments
+import math
+
+from amr_oracle import (GHOST, Tables, active_list, block_criterion, collide, feq, manage_slots, moments, near_body,
+                        q_criterion, vorticity_magnitude)
 
 
 class Level:
     """Pool level m >= 1 over its parent grid (nx, ny, nz cells)."""
 
-    def __init__(self, m, q, nx, ny, nz, rb, max_slots, omega_f, c2f):
+    def __init__(self, m, q, nx, ny, nz, rb, max_slots, omega_f, c2f, f2c):
         self.m = m
         self.q = q
         self.nx, self.ny, self.nz = nx, ny, nz
@@ -55,6 +58,7 @@ class Level:
         self.max_slots = max_slots
         self.omega_f = omega_f
         self.c2f = c2f
+        self.f2c = f2c
         self.scale = 2.0 ** -(m - 1)
         self.offset = -0.5 * (1.0 - self.scale)
         self.dx2 = 4.0 ** -m
@@ -126,11 +130,29 @@ class MultiLevelOracle:
             tf = 2.0 * tau - 0.5
             s = 1 << (m - 1)
             self.lv.append(Level(m, self.q, self.nx * s, self.ny * s, self.nz * s, self.rb, self.max_slots,
-                                 1.0 / tf, 0.5 * (tf - 1.0) / (tau - 1.0)))
+                                 1.0 / tf, 0.5 * (tf - 1.0) / (tau - 1.0), 2.0 * (tau - 1.0) / (tf - 1.0)))
             tau = tf
         self.forces = []
         self.f0 = None
         self.mac0 = None
+        # the manager
+        self.manage_every = int(desc.get("manageEvery", 0))
+        self.refine_at = float(np.float32(desc.get("refineAt", 1e30)))
+        self.keep_at = float(np.float32(desc.get("keepAt", 1e30)))
+        self.lead = int(desc.get("lead", 0))
+        self.criterion_kind = desc.get("criterion", "vorticity")
+        oct_ = float(np.float32(desc.get("ladderOctaves", 0.0)))
+        self.ladder_octaves = oct_ if oct_ > 0 else 1.0
+        sp = desc.get("sponge") or {}
+        self.sponge_width = float(np.float32(sp.get("width", 0.0)))
+        self.sponge_axes = [int(bool(v)) for v in sp.get("axes", [1, 1, 1])]
+        self.geom = [None] + [self.level_mask(m) for m in range(1, self.levels)] if self.manage_every > 0 else None
+        self.step_count = 0
+        self.exhausted = 0
+        self.band = 0
+        self.crit = None
+        self.wants = None
+        self.last_fresh = None
 
     # --- set-up -------------------------------------------------------------
     def upload_topology(self, level, block_slot, slot_to_block):
@@ -187,13 +209,17 @@ class MultiLevelOracle:
             P.mac_pool[:, s, lo:hi, lo:hi, lo:hi] = mac[(slice(None),) + sl]
 
     # --- passes -------------------------------------------------------------
-    def interp(self, m):
+    def interp(self, m, owner=True, pairs=None):
         """Fills every tile of level m from its parent: trilinear rho, u and
-        fneq x c2f (common_d3_pool.wgsl:172-220); a pool parent is sampled in
-        the parent tile that holds the block, ring included."""
+        fneq x c2f (common_d3_pool.wgsl:172-220); a pool parent cell is read
+        in the parent tile that owns it (parentIndex, common_d3_parent_pool.
+        wgsl:37-59), or, where none does, in the ring of the parent tile that
+        holds the block.  owner=False reads the holding tile's ring alone
+        (the variant that the tests show to be the same at a seed).  pairs:
+        the (block, slot) tiles to fill (default every tile)."""
         C = self.lv[m]
         n = C.fb
-        for b, slot in C.tiles():
+        for b, slot in (C.tiles() if pairs is None else pairs):
             bx, by, bz = C.block_xyz(b)
             fz, fy, fx = (a.ravel() for a in np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"))
             Pp = [o * C.rb - 0.25 + 0.5 * (j - GHOST) for o, j in ((bx, fx), (by, fy), (bz, fz))]
@@ -214,9 +240,17 @@ class MultiLevelOracle:
                 if m == 1:
                     f = self.f0[:, cz % self.nz, cy % self.ny, cx % self.nx]
                 else:
+                    # the owning parent tile's interior; the holding tile's
+                    # ring where no parent tile owns the cell
+                    vx, vy, vz = cx % C.nx, cy % C.ny, cz % C.nz
+                    os_ = P.block_slot[P.block_id(vx // P.rb2, vy // P.rb2, vz // P.rb2)]
+                    own = os_ >= 0
                     lx, ly, lz = (c - p * P.rb2 + GHOST for c, p in zip((cx, cy, cz), pb))
                     assert all(((v >= 0) & (v < P.fb)).all() for v in (lx, ly, lz))
                     f = P.pool[:, ps, lz, ly, lx]
+                    if owner and own.any():
+                        ox, oy, oz = (v % P.rb2 + GHOST for v in (vx, vy, vz))
+                        f = np.where(own[None], P.pool[:, np.where(own, os_, 0), oz, oy, ox], f)
                 sr, sx, sy, sz = moments(self.T, f)
                 rho += w * sr
                 u[0] += w * sx
@@ -469,7 +503,10 @@ class MultiLevelOracle:
             self.coalesce(m + 1)
         self.pool_step(m)
 
-    def macro_step(self):
+    def macro_step(self, crit=None):
+        if self.manage_every > 0 and self.step_count % self.manage_every == 0:
+            self.manage(crit)
+        self.step_count += 1
         if self.body is not None:
             self.body_force()
         self.explode(1)
@@ -477,6 +514,159 @@ class MultiLevelOracle:
         self.advance(1)
         self.coalesce(1)
         self.l0_step()
+
+    # --- the manager ----------------------------------------------------------
+    def level_mask(self, m):
+        """Per block of level m: bit 0 (the finest level, with a body) the
+        body's geometry wants it (nearBodyWant at margin 2 on the block's L0
+        box, scaled by 2^-(m-1) as refineHierarchy scales it, d3-amr.mjs:
+        603-651); bit 1 the field may not refine it: its L0 box comes within
+        the sponge band dilated by 3 rb (1 - 2^-(m-1)) L0 cells, the most the
+        2:1 closure can carry a level-m want toward level 1."""
+        L = self.lv[m]
+        sc = 2.0 ** -(m - 1)
+        edge = self.rb * sc
+        dil = 3.0 * self.rb * (1.0 - sc)
+        dims = (self.nx, self.ny, self.nz)
+        out = np.zeros(L.nblocks, dtype=np.uint8)
+        body = self.body is not None and m == self.levels - 1
+        if body:
+            c = [float(v) for v in self.body[0]]
+            r = float(self.body[1])
+            sdf = lambda p: math.sqrt(sum((p[k] - c[k]) ** 2 for k in range(3))) - r
+        for b in range(L.nblocks):
+            bc = L.block_xyz(b)
+            lo = [bc[k] * edge for k in range(3)]
+            if body and near_body(sdf, lo, [v + edge for v in lo], 2.0):
+                out[b] |= 1
+            if self.sponge_width > 0:
+                for k in range(3):
+                    if self.sponge_axes[k] and (lo[k] < self.sponge_width + dil
+                                                or lo[k] + edge > dims[k] - self.sponge_width - dil):
+                        out[b] |= 2
+        return out
+
+    def criterion(self):
+        """Per level-1 block maximum of |curl u| (or Q) over the L0 field,
+        binary32 as the engine computes it."""
+        field = q_criterion if self.criterion_kind == "q" else vorticity_magnitude
+        return block_criterion(field(self.mac0.astype(np.float32), np.float32), self.rb, self.lead)
+
+    def rung(self, m):
+        """Level m's (refine, keep) thresholds: the ladder desiredLevelAt
+        (common_refine.wgsl:54-79) asks whether the criterion's level reaches
+        m, i.e. rate >= base x 2^(octaves (m - 1)); Q is a squared rate."""
+        f = 2.0 ** (self.ladder_octaves * (2.0 if self.criterion_kind == "q" else 1.0) * (m - 1))
+        return np.float32(self.refine_at * f), np.float32(self.keep_at * f)
+
+    def decide(self, crit):
+        """Every level's want (level_wants), closed (close)."""
+        return self.close(self.level_wants(crit))
+
+    def level_wants(self, crit):
+        """Each level's own want: the body's geometry first, then the field
+        on the level's rung, a held block judged against the keep threshold
+        (common_d3_manage.wgsl decide)."""
+        wants = [None]
+        crit = np.asarray(crit, dtype=np.float32)
+        L1 = self.lv[1]
+        for m in range(1, self.levels):
+            L = self.lv[m]
+            refine, keep = self.rung(m)
+            g = self.geom[m]
+            b = np.arange(L.nblocks)
+            bx, by, bz = L.block_xyz(b)
+            sh = m - 1
+            c = crit[((bz >> sh) * L1.nby + (by >> sh)) * L1.nbx + (bx >> sh)]
+            field = np.where(L.block_slot >= 0, c >= keep, c >= refine) & ((g & 2) == 0)
+            wants.append((((g & 1) != 0) | field).astype(np.uint8))
+        return wants
+
+    def close(self, wants):
+        """The 2:1 closure finest-first: octet completion at m >= 2
+        (completeOctets), then the parent gather over the child blocks
+        [2p - 1, 2p + 2]^3 (balance, common_d3_manage.wgsl:318-372), which
+        ORs into the parent level's own want."""
+        wants = [None] + [np.array(w, dtype=np.uint8) for w in wants[1:]]
+        for m in range(self.levels - 1, 1, -1):
+            L, P = self.lv[m], self.lv[m - 1]
+            w = wants[m].reshape(L.nbz, L.nby, L.nbx)
+            o = w.reshape(L.nbz // 2, 2, L.nby // 2, 2, L.nbx // 2, 2).max(axis=(1, 3, 5))
+            w = np.repeat(np.repeat(np.repeat(o, 2, 0), 2, 1), 2, 2)
+            wants[m] = w.reshape(-1).copy()
+            gathered = np.zeros((P.nbz, P.nby, P.nbx), dtype=np.uint8)
+            for dz in (-1, 0, 1, 2):
+                for dy in (-1, 0, 1, 2):
+                    for dx in (-1, 0, 1, 2):
+                        # parent p takes child 2p + d (periodic on the child grid)
+                        sub = np.roll(w, (-dz, -dy, -dx), (0, 1, 2))[::2, ::2, ::2]
+                        gathered |= sub
+            wants[m - 1] = wants[m - 1] | gathered.reshape(-1)
+        return wants
+
+    def _average(self, m, pairs):
+        """Restriction of level m's listed tiles onto their parent cells
+        (AmrOracle.average against the parent level: rho mean, mass-weighted
+        u, fneq mean x 2 (tau_(m-1) - 1)/(tau_m - 1)), a pool parent's cells
+        written in the tile that owns them."""
+        T, C = self.T, self.lv[m]
+        P = self.lv[m - 1] if m >= 2 else None
+        for b, slot in pairs:
+            bx, by, bz = C.block_xyz(b)
+            r = np.arange(C.rb)
+            lz, ly, lx = (a.ravel() for a in np.meshgrid(r, r, r, indexing="ij"))
+            rs = np.zeros(lx.size)
+            mm = [np.zeros(lx.size) for _ in range(3)]
+            sneq = np.zeros((self.q, lx.size))
+            for k in range(8):
+                f = C.pool[:, slot, GHOST + 2 * lz + ((k >> 2) & 1), GHOST + 2 * ly + ((k >> 1) & 1),
+                           GHOST + 2 * lx + (k & 1)]
+                rho, ux, uy, uz = moments(T, f)
+                rs += rho
+                mm[0] += rho * ux
+                mm[1] += rho * uy
+                mm[2] += rho * uz
+                sneq += f - feq(T, rho, ux, uy, uz)
+            rho = rs * 0.125
+            inv = 1.0 / np.maximum(rs, 1e-6)
+            u = [v * inv for v in mm]
+            out = feq(T, rho, u[0], u[1], u[2]) + C.f2c * (sneq * 0.125)
+            cx, cy, cz = bx * C.rb + lx, by * C.rb + ly, bz * C.rb + lz
+            if P is None:
+                self.f0[:, cz, cy, cx] = out
+                for k, v in enumerate((rho, u[0], u[1], u[2])):
+                    self.mac0[k, cz, cy, cx] = v
+            else:
+                ps = P.block_slot[P.block_id(cx // P.rb2, cy // P.rb2, cz // P.rb2)]
+                assert (ps >= 0).all(), "a tile's parent cells lie in parent tiles"
+                ox, oy, oz = cx % P.rb2 + GHOST, cy % P.rb2 + GHOST, cz % P.rb2 + GHOST
+                P.pool[:, ps, oz, oy, ox] = out
+                for k, v in enumerate((rho, u[0], u[1], u[2])):
+                    P.mac_pool[k, ps, oz, oy, ox] = v
+
+    def manage(self, crit=None):
+        """The manager over the hierarchy (main-3d.js:3251-3321): criterion,
+        every tile restricted into its parent finest-first, every level's
+        want and the closure, each level's slots (coarsen, then the k-th new
+        block in ascending id takes the k-th free slot), the new tiles filled
+        from their parents coarsest-first.  `crit` overrides the criterion
+        (the engine's own values, to follow its decisions exactly)."""
+        self.crit = self.criterion() if crit is None else np.asarray(crit, dtype=np.float32)
+        for m in range(self.levels - 1, 0, -1):
+            self._average(m, self.lv[m].tiles())
+        self.wants = self.decide(self.crit)
+        self.band += int((self.wants[1] & ((self.geom[1] & 2) != 0)).sum())
+        fresh = [None]
+        for m in range(1, self.levels):
+            L = self.lv[m]
+            bs, sb, _, fr, exh = manage_slots(L.block_slot, L.slot_to_block, self.wants[m])
+            L.block_slot, L.slot_to_block = bs, sb
+            fresh.append(fr)
+            self.exhausted += exh
+        for m in range(1, self.levels):
+            if fresh[m]:
+                self.interp(m, pairs=fresh[m])
+        self.last_fresh = fresh
 
     # --- read-back ------------------------------------------------------------
     def totals(self):

@@ -70,11 +70,11 @@ def test_amr_descriptor_layout_matches_c():
 #include <stddef.h>
 #include "lbm.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_amr_desc), offsetof(lbm_amr_desc, omega),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lbm_amr_desc), offsetof(lbm_amr_desc, omega),
          offsetof(lbm_amr_desc, force), offsetof(lbm_amr_desc, manage_every),
          offsetof(lbm_amr_desc, lead), offsetof(lbm_amr_desc, orphans), offsetof(lbm_amr_desc, reflux),
          offsetof(lbm_amr_desc, body), offsetof(lbm_amr_desc, explode_dirs), offsetof(lbm_amr_desc, criterion),
-         offsetof(lbm_amr_desc, levels));
+         offsetof(lbm_amr_desc, levels), offsetof(lbm_amr_desc, ladder_octaves));
   return 0;
 }
 """
@@ -87,7 +87,7 @@ int main(void) {
     A = pylbm.AmrDesc
     want = [ctypes.sizeof(A), A.omega.offset, A.force.offset, A.manage_every.offset, A.lead.offset,
             A.orphans.offset, A.reflux.offset, A.body.offset, A.explode_dirs.offset, A.criterion.offset,
-            A.levels.offset]
+            A.levels.offset, A.ladder_octaves.offset]
     assert got == want
 
 
@@ -123,17 +123,20 @@ def test_amr_refusals_without_gpu():
     d = pylbm.make_amr_desc(dict(base, nx=1032, rb=8))  # L0 coordinates are packed in 10 bits
     assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
     assert b"1024" in lib.lbm_last_error()
-    # deeper hierarchies: 2 to 4 levels, explode only, static, parent grids
-    # within the 10-bit packing, no parent level at tau = 1
+    # deeper hierarchies: 2 to 4 levels, explode only, parent grids within
+    # the 10-bit packing, no parent level at tau = 1, a ladder of at most 16
+    # octaves per level
     for lv in (1, 5):
         d = pylbm.make_amr_desc(dict(base, levels=lv))
         assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
         assert b"levels" in lib.lbm_last_error()
     d = pylbm.make_amr_desc(dict(base, levels=3, interface="interp"))
     assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -4
-    d = pylbm.make_amr_desc(dict(base, levels=3, manageEvery=10))
+    d = pylbm.make_amr_desc(dict(base, levels=3, manageEvery=10, ladderOctaves=17))
     assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
-    assert b"static" in lib.lbm_last_error()
+    assert b"ladder_octaves" in lib.lbm_last_error()
+    d = pylbm.make_amr_desc(dict(base, levels=3, manageEvery=10, ladderOctaves=float("nan")))
+    assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
     d = pylbm.make_amr_desc(dict(base, levels=4, nx=264, ny=264, nz=264))
     assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -1
     assert b"1024" in lib.lbm_last_error()

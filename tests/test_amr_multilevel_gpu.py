@@ -188,3 +188,94 @@ def test_sphere_amr_depth_three_matches_the_reference_measurement():
     say("sphere AMR L3 Re20 D16 Cd %.4f (reference 2.9015, dense 2.9436)" % cd)
     assert abs(cd / 2.9015 - 1.0) < 0.005
     assert abs(cd / 2.9436 - 1.0) < 0.03
+
+
+# --- the manager at depth ---------------------------------------------------
+
+def test_manager_at_depth_is_bit_exact_and_fields_follow():
+    """Three levels under the manager with a quarter-octave |curl u| ladder
+    (the oracle's dynamic invariant case): at every pass the engine's
+    criterion is the binary32 restatement of its own L0 field, every level's
+    block maps equal the oracle's (fed the same criterion), and the fields
+    follow the oracle while level 2 refines and drains."""
+    from test_amr_oracle import smooth_field as sf
+    import amr_oracle as ao
+    desc = dict(BASE, levels=3, manageEvery=2, criterion="vorticity", refineAt=0.024, keepAt=0.022,
+                ladderOctaves=0.25, maxSlots=64)
+    say("case manager at depth")
+    amr = pylbm.Amr(desc)
+    orc = mo.MultiLevelOracle(desc)
+    mac = sf(16, 16, 16, 21)
+    amr.upload_macro(mac)
+    orc.upload_macro(mac)
+    counts = []
+    with amr:
+        for step in range(8):
+            manage = step % 2 == 0
+            if manage:
+                m = amr.read_macro().reshape(16, 16, 16, 4)
+                want_crit = ao.block_criterion(ao.vorticity_magnitude(np.moveaxis(m, -1, 0), np.float32), 4, 0)
+            amr.step(1)
+            if manage:
+                crit, _ = amr.read_criterion()
+                assert (crit.view(np.uint32) == want_crit.view(np.uint32)).all(), step
+                orc.macro_step(crit=crit)
+                for lv in (1, 2):
+                    bs, sb = amr.read_level_topology(lv)
+                    assert np.array_equal(bs, orc.lv[lv].block_slot), (step, lv)
+                    assert np.array_equal(sb, orc.lv[lv].slot_to_block), (step, lv)
+                counts.append(tuple(int((orc.lv[lv].block_slot >= 0).sum()) for lv in (1, 2)))
+            else:
+                orc.macro_step()
+            compare(amr, orc, label="manager at depth step %d" % step)
+        assert amr.info()["exhausted"] == 0
+    say("tiles per pass", counts)
+    assert max(c[1] for c in counts) > 0 and min(c[1] for c in counts) == 0
+
+
+def test_pinned_sphere_under_the_manager_equals_the_static_hierarchy():
+    """benchmarks/d3.json sphere-amr-dynamic-L3-Re20-D16: the depth-3 sphere
+    with the manager running every 16 macro-steps (the reference's default)
+    from an empty pool.  Its first pass builds refineHierarchy's maps and
+    every later pass changes nothing, so the force history equals the static
+    run's bit for bit (the reference's gate: Cd 2.9015, bit-identical)."""
+    desc, mac, topo, meta = scenarios.sphere_amr(levels=3)
+    steps = 30 * 320
+    say("case sphere AMR L3 static vs manager, %d steps" % steps)
+    with pylbm.Amr(desc) as amr:
+        for m in (1, 2):
+            amr.upload_level_topology(m, *topo[m])
+        amr.upload_macro(mac)
+        amr.step(steps)
+        f_static = amr.read_forces(steps)
+    with pylbm.Amr(dict(desc, manageEvery=16)) as amr:
+        amr.upload_macro(mac)
+        amr.step(steps)
+        f_dyn = amr.read_forces(steps)
+        for m in (1, 2):
+            bs, sb = amr.read_level_topology(m)
+            assert np.array_equal(bs, topo[m][0]) and np.array_equal(sb, topo[m][1]), m
+    cd = float(f_dyn[-1, 0] / (0.5 * 0.05 ** 2 * math.pi * 64.0))
+    say("static vs manager: %d differing records, Cd %.4f" % (int((f_static != f_dyn).any(axis=1).sum()), cd))
+    assert np.array_equal(f_static.view(np.uint64), f_dyn.view(np.uint64))
+    assert abs(cd / 2.9015 - 1.0) < 0.005
+
+
+def test_closure_into_the_sponge_band_is_a_hard_failure():
+    """A body whose depth-3 geometry set, closed 2:1, would carry level-1
+    tiles into the sponge band (where the pool step does not relax): the
+    manager refuses the pass with an error rather than refine there."""
+    desc = dict(BASE, nx=32, ny=32, nz=32, levels=3, manageEvery=4, maxSlots=512,
+                sponge={"width": 4.0, "axes": [1, 0, 0], "u": [0.0, 0.0, 0.0]},
+                body={"a": 1.5, "centre": [9.0, 16.0, 16.0]})
+    orc = mo.MultiLevelOracle(desc)
+    mac = np.tile(np.array([1.0, 0.0, 0.0, 0.0], np.float32), (32 ** 3, 1))
+    orc.upload_macro(mac)
+    orc.manage()
+    assert orc.band > 0
+    say("case closure into the sponge band (oracle: %d level-1 blocks)" % orc.band)
+    with pylbm.Amr(desc) as amr:
+        amr.upload_macro(mac)
+        with pytest.raises(pylbm.LbmError) as e:
+            amr.step(1)
+        assert "sponge band" in str(e.value)
