@@ -191,15 +191,31 @@ def measure_plate():
         a.read_forces(1)
         t0 = time.perf_counter()
         a.step(30 * 320 - 1)
-        f = a.read_forces(1)
+        f_static = a.read_forces(30 * 320)
         l3_ms = (time.perf_counter() - t0) * 1e3 / (30 * 320 - 1)
-    out["sphere_amr_l3"] = float(f[-1, 0] / (0.5 * 0.05 ** 2 * np.pi * 64.0))
+    out["sphere_amr_l3"] = float(f_static[-1, 0] / (0.5 * 0.05 ** 2 * np.pi * 64.0))
+    # the same with the manager every 16 macro-steps from an empty pool
+    # (sphere-amr-dynamic-L3-Re20-D16): the reference's gate is a force
+    # history bit-identical to the static run's
+    with pylbm.Amr(dict(adesc, manageEvery=16)) as a:
+        a.upload_macro(amac)
+        a.step(1)
+        a.read_forces(1)
+        t0 = time.perf_counter()
+        a.step(30 * 320 - 1)
+        f_dyn = a.read_forces(30 * 320)
+        l3d_ms = (time.perf_counter() - t0) * 1e3 / (30 * 320 - 1)
+    out["sphere_amr_l3_dynamic"] = float(f_dyn[-1, 0] / (0.5 * 0.05 ** 2 * np.pi * 64.0))
+    l3d_same = bool(np.array_equal(f_static.view(np.uint64), f_dyn.view(np.uint64)))
     return {"sphere_re20_d16_cd": out["sphere_re20"], "sphere_re20_d16_cd_reference_measured": 2.9436,
             "sphere_amr_re20_d16_cd": out["sphere_amr"], "sphere_amr_re20_d16_cd_reference_measured": 2.9416,
             "sphere_amr_re20_d16_tiles": int((sb >= 0).sum()), "sphere_amr_re20_d16_ms_per_macro_step": amr_ms,
             "sphere_amr_l3_re20_d16_cd": out["sphere_amr_l3"], "sphere_amr_l3_re20_d16_cd_reference_measured": 2.9015,
             "sphere_amr_l3_re20_d16_tiles": [int((t[1] >= 0).sum()) for t in topo[1:]],
             "sphere_amr_l3_re20_d16_ms_per_macro_step": l3_ms,
+            "sphere_amr_dynamic_l3_re20_d16_cd": out["sphere_amr_l3_dynamic"],
+            "sphere_amr_dynamic_l3_bit_identical_to_static": l3d_same,
+            "sphere_amr_dynamic_l3_ms_per_macro_step": l3d_ms,
             "plate_cd_pinned": out["pinned"], "plate_cd_pinned_reference_measured": 1.2081, "plate_cd_literature": 1.18,
             "plate_cd_half_tow": out["half_tow"], "plate_half_tow_vs_pinned": out["half_tow"] / out["pinned"] - 1.0,
             "plate_half_tow_vs_pinned_reference_measured": -0.0321,
