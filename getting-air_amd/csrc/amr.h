@@ -140,8 +140,10 @@ hipError_t launch_cell_class(int q, uint8_t* ccls, const PoolGeom& g, const Surf
                              int nactive, const ParentRef& parent, hipStream_t s);
 // mac_pool null: the substep's macroscopic field is not written.
 // collide_ring: ring cells collide as well (the interp interface).
+// sub: 0 or 1, the first or second substep of the explode interface, whose
+// cells nothing reads afterwards load nothing; -1 steps every cell.
 hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
-                            const CollideParams& fine, int nactive, int collide_ring, hipStream_t s);
+                            const CollideParams& fine, int nactive, int collide_ring, int sub, hipStream_t s);
 // Momentum exchange on the fine-level body from the pool state at the start
 // of the macro-step: one FP64 [fx, fy, fz, tx, ty, tz] partial per thread
 // block (pool_force_blocks of them), weighted dx^2 = 1/4.

@@ -648,10 +648,10 @@ int macro_step_interp(lbm_amr* a) {
     HIPCHK(launch_interp(d.q, dense_parent(a, const_cast<float*>(ct)), a->pool[0], a->G, a->pairs, a->nactive, a->c2f,
                          1, a->stream));
     if (d.reflux) HIPCHK(launch_flux(d.q, a->pool[0], a->flux_acc, a->G, a->plan, a->nflux, 0, a->stream));
-    HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->nactive, 1,
+    HIPCHK(launch_pool_step(d.q, a->pool[0], a->pool[1], nullptr, a->G, a->fine, a->nactive, 1, -1,
                             a->stream));
     if (d.reflux) HIPCHK(launch_flux(d.q, a->pool[1], a->flux_acc, a->G, a->plan, a->nflux, 1, a->stream));
-    HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->nactive, 1,
+    HIPCHK(launch_pool_step(d.q, a->pool[1], a->pool[0], a->mac_pool, a->G, a->fine, a->nactive, 1, -1,
                             a->stream));
     HIPCHK(launch_average(d.q, a->pool[0], dense_parent(a, ct1), a->G, a->pairs, a->nactive, a->f2c, a->stream));
     if (d.reflux)
@@ -683,10 +683,10 @@ int advance(lbm_amr* a, int m, int p) {
   }
   if (m == 1) {
     HIPCHK(launch_pool_step(d.q, a->pool[p], a->pool[p ^ 1], p ? a->mac_pool : nullptr, a->G, a->fine, a->nactive, 0,
-                            a->stream));
+                            p, a->stream));
   } else {
     DeepLevel& L = a->deep[m - 2];
-    HIPCHK(launch_pool_step(d.q, L.pool[p], L.pool[p ^ 1], p ? L.mac_pool : nullptr, L.G, L.fine, L.nactive, 0,
+    HIPCHK(launch_pool_step(d.q, L.pool[p], L.pool[p ^ 1], p ? L.mac_pool : nullptr, L.G, L.fine, L.nactive, 0, p,
                             a->stream));
   }
   return LBM_OK;

@@ -310,7 +310,9 @@ __global__ void __launch_bounds__(256) k_tile_table(PoolGeom G, const int32_t* _
 // ---------------------------------------------------------------------------
 // One pool cell's substep: pull (the DIRECT_GHOST gather), moments, and,
 // when `collide_cell`, the fine collision; `mac_pool` (or null) receives
-// the moments.
+// the moments.  A lane whose result nothing reads (`load` false) loads
+// nothing and stores zeros, so that every wave still writes whole cache
+// lines; `put_mac` false keeps the cell's moments as they are.
 // BODY: the fine-level pinned sphere (its own instantiation, so that the
 // body-free kernel keeps its register budget).  A link whose geometric source
 // is solid reflects this cell's own opposite population (halfway bounce-back,
@@ -321,7 +323,7 @@ template <int Q, int RB, bool BODY, bool FRC>
 __device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, float* __restrict__ gout,
                                                float* __restrict__ mac_pool, const PoolGeom& G, const CollideParams& P,
                                                const int32_t* __restrict__ tt, int fx, int fy, int fz,
-                                               bool collide_cell) {
+                                               bool collide_cell, bool load = true, bool put_mac = true) {
   const int slot = tt[13], bx = tt[28], by = tt[29], bz = tt[30];
   using L = Lattice<Q>;
   using T = Tile<RB>;
@@ -368,14 +370,14 @@ __device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, fl
     sx = min(max(sx, 0), FB - 1);
     sy = min(max(sy, 0), FB - 1);
     sz = min(max(sz, 0), FB - 1);
-    g[i] = *at(gin + (size_t)i * G.pool_plane, pool_cell<RB>(src_slot, sx, sy, sz));
+    g[i] = load ? *at(gin + (size_t)i * G.pool_plane, pool_cell<RB>(src_slot, sx, sy, sz)) : 0.f;
   }
   float phi = 1e30f;
   if (BODY) {
     // positions in L0 units (exact: every term is a small dyadic rational)
     const float px = level_to_l0(G, bx * RB, fx), py = level_to_l0(G, by * RB, fy), pz = level_to_l0(G, bz * RB, fz);
     phi = sphere_phi(G.body, px, py, pz);
-    if (phi < kBodyReach) {
+    if (load && phi < kBodyReach) {
 #pragma unroll
       for (int i = 1; i < Q; i++) {
         // the geometric source, one cell of this level back
@@ -391,7 +393,7 @@ __device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, fl
     float S[Pairs<Q>::NP], D[Pairs<Q>::NP];
     pair_sums<Q>(g, S, D);
     pair_moments<Q, FRC>(g[0], S, D, P.F, drho, rho, ux, uy, uz);
-    if (mac_pool) {
+    if (mac_pool && put_mac) {
       *at(mac_pool, cell) = rho;
       *at(mac_pool + G.pool_plane, cell) = ux;
       *at(mac_pool + 2 * (size_t)G.pool_plane, cell) = uy;
@@ -402,7 +404,7 @@ __device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, fl
   if (BODY && phi < 0.f) {
 #pragma unroll
     for (int i = 0; i < Q; i++) g[i] = 0.f;  // feq(1, 0) - w
-    if (mac_pool) {
+    if (mac_pool && put_mac) {
       *at(mac_pool, cell) = 1.f;
       *at(mac_pool + G.pool_plane, cell) = 0.f;
       *at(mac_pool + 2 * (size_t)G.pool_plane, cell) = 0.f;
@@ -414,15 +416,26 @@ __device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, fl
 }
 
 // One thread per pool cell of every active tile, in memory order (a wave is
-// 64 consecutive cells of one tile).  Every cell is stepped, ring included:
-// leaving out the ring cells nothing reads afterwards (those over refined L0
-// cells, and the outer layer in the first substep) was measured slower on
-// the vortex-ring workload (1.17 against 1.04 ms per macro-step), the masked
-// lanes costing the waves more than the traffic they save.
+// 64 consecutive cells of one tile), and every cell stores: a 20-cell row is
+// not a whole number of cache lines, so a lane that stored nothing would
+// leave every line of its wave partly written.  On the explode interface
+// (sub 0 or 1, the first or second substep) the cells whose result nothing
+// reads load nothing and store zeros:
+// - ring cells over a refined neighbour block, in both substeps: pulls into
+//   that region take the neighbour tile's interior (the gather above), and
+//   explode, coalesce and the orphans touch only ring cells over unrefined
+//   blocks;
+// - the outer ring layer in the first substep: what it holds can only reach,
+//   in the second, the inward and tangential populations of the ring, and
+//   coalesce and the orphans read populations leaving the interior alone;
+// - interior cells a finer tile covers (their moments are the child's
+//   coalesced ones, kept; their populations are read by nothing before the
+//   next coalesce or restriction rewrites the ones that are).
+// sub -1 (the interp interface) steps every cell.
 template <int Q, int RB, bool BODY, bool FRC>
 __global__ void __launch_bounds__(256)
 k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __restrict__ mac_pool, PoolGeom G,
-            CollideParams P, int nactive, int collide_ring) {
+            CollideParams P, int nactive, int collide_ring, int sub) {
   using T = Tile<RB>;
   constexpr int FB = T::FB, LO = T::LO, HI = T::HI;
   const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
@@ -432,19 +445,26 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
   const int32_t* tt = G.tile_table + (size_t)k * kTileWords;
   const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
   const bool interior = fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI;
-  // An interior cell a finer level solves is left alone, populations and
-  // moments both: the child's coalesce wrote its outbox and moments into
-  // this time level (coveredByChild, common_d3_amr_step1.wgsl:107-143).  Ring
-  // cells keep advecting whatever covers them: they are this level's ghost
-  // for its own parent.
-  if (interior && G.child_block_slot &&
-      covered_by_child<RB>(G, tt[28] * T::RB2 + fx - kGhost, tt[29] * T::RB2 + fy - kGhost,
-                           tt[30] * T::RB2 + fz - kGhost))
-    return;
+  bool load = true, put_mac = true;
+  if (interior) {
+    // An interior cell a finer level solves keeps the moments the child's
+    // coalesce wrote (coveredByChild, common_d3_amr_step1.wgsl:107-143).
+    if (G.child_block_slot && covered_by_child<RB>(G, tt[28] * T::RB2 + fx - kGhost, tt[29] * T::RB2 + fy - kGhost,
+                                                   tt[30] * T::RB2 + fz - kGhost))
+      load = put_mac = false;
+  } else if (sub >= 0) {
+    const int ox = fx < LO ? -1 : (fx >= HI ? 1 : 0);
+    const int oy = fy < LO ? -1 : (fy >= HI ? 1 : 0);
+    const int oz = fz < LO ? -1 : (fz >= HI ? 1 : 0);
+    const bool over_refined = ((uint32_t)tt[27] >> ((ox + 1) + 3 * (oy + 1) + 9 * (oz + 1))) & 1u;
+    const bool outer = fx == 0 || fx == FB - 1 || fy == 0 || fy == FB - 1 || fz == 0 || fz == FB - 1;
+    load = !over_refined && !(sub == 0 && outer);
+  }
   // Ring cells advect and store without colliding (common_d3_amr_step1.wgsl
   // COLLIDE_RING = 0): coalesce averages uncollided interface states.  The
   // interp interface collides them too (COLLIDE_RING = 1, main-3d.js:1395).
-  pool_cell_step<Q, RB, BODY, FRC>(gin, gout, mac_pool, G, P, tt, fx, fy, fz, interior || collide_ring);
+  pool_cell_step<Q, RB, BODY, FRC>(gin, gout, mac_pool, G, P, tt, fx, fy, fz, load && (interior || collide_ring),
+                                   load, put_mac);
 }
 
 // Momentum exchange on the fine-level body (common_d3_force_pool.wgsl, the
@@ -1265,13 +1285,13 @@ struct OpCellClass {
 template <int Q, int RB>
 struct OpPoolStep {
   static hipError_t run(const float* gin, float* gout, float* mac_pool, const PoolGeom& g, const CollideParams& p,
-                        int nactive, int collide_ring, hipStream_t s) {
+                        int nactive, int collide_ring, int sub, hipStream_t s) {
     const dim3 grid = grid_for((uint64_t)nactive * Tile<RB>::CELLS);
     // a level without a body force runs the instantiation without Guo terms
     const bool frc = p.F[0] != 0.f || p.F[1] != 0.f || p.F[2] != 0.f;
     auto go = [&](auto B, auto F) {
       hipLaunchKernelGGL((k_pool_step<Q, RB, decltype(B)::value, decltype(F)::value>), grid, dim3(256), 0, s, gin,
-                         gout, mac_pool, g, p, nactive, collide_ring);
+                         gout, mac_pool, g, p, nactive, collide_ring, sub);
     };
     if (g.body.on) {
       if (frc) go(std::true_type{}, std::true_type{});
@@ -1391,9 +1411,9 @@ hipError_t launch_cell_class(int q, uint8_t* ccls, const PoolGeom& g, const Surf
 }
 
 hipError_t launch_pool_step(int q, const float* gin, float* gout, float* mac_pool, const PoolGeom& g,
-                            const CollideParams& fine, int nactive, int collide_ring, hipStream_t s) {
+                            const CollideParams& fine, int nactive, int collide_ring, int sub, hipStream_t s) {
   if (nactive <= 0) return hipSuccess;
-  return dispatch_q_rb<OpPoolStep>(q, g.rb, gin, gout, mac_pool, g, fine, nactive, collide_ring, s);
+  return dispatch_q_rb<OpPoolStep>(q, g.rb, gin, gout, mac_pool, g, fine, nactive, collide_ring, sub, s);
 }
 uint32_t pool_force_blocks(int rb, int nactive) {
   const uint64_t per = (uint64_t)(2 * rb) * (2 * rb) * (2 * rb);
