@@ -564,19 +564,28 @@ k_explode(ParentRef P, float* __restrict__ gp, PoolGeom G, const uint4* __restri
   const float dx = ((float)(fx & 1) - 0.5f) * 0.5f;
   const float dy = ((float)(fy & 1) - 0.5f) * 0.5f;
   const float dz = ((float)(fz & 1) - 0.5f) * 0.5f;
+  // Ring populations only advect (the ring does not collide), so the one in
+  // direction e at ring cell p reaches the interior within the macro-step's
+  // two substeps iff p + e or p + 2e is an interior cell; nothing else reads
+  // the others (coalesce and the orphans take populations that left the
+  // interior at t).  Without a body on this level only those are written;
+  // with one, the ring's bounce-back reflects a cell's own opposite
+  // population, so every direction is.
+  const bool all_dirs = G.body.on != 0;
+  auto inside = [&](int x, int y, int z) { return x >= LO && x < HI && y >= LO && y < HI && z >= LO && z < HI; };
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     if (i < i0 || i >= i1) continue;  // wave-uniform
     const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
     const float wi = L::w[i];
+    const bool into = inside(fx + ex, fy + ey, fz + ez);
+    if (!all_dirs && !into && !inside(fx + 2 * ex, fy + 2 * ey, fz + 2 * ez)) continue;
     auto l0 = [&](uint32_t c) { return *ppop<PP, Q>(P, c, i); };  // population i of parent cell c
     float out;
     if (!((bits >> i) & 1)) {
       // The coarse target is unrefined: only a population that streams from
       // this ring cell into this tile's interior is kept (in-orphan, uniform
       // injection); everything else is zeroed (f = 0, i.e. g = -w).
-      const int tx = fx + ex, ty = fy + ey, tz = fz + ez;
-      const bool into = tx >= LO && tx < HI && ty >= LO && ty < HI && tz >= LO && tz < HI;
       out = into ? l0(vc) : -wi;
     } else if (!linear || i == 0) {
       out = l0(vc);
