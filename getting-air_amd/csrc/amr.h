@@ -131,7 +131,7 @@ struct SurfacePlan {
   uint32_t cap_seam, cap_near, cap_ring, cap_flux;
 };
 // Capacity of the lists for a pool of max_slots tiles.
-void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint32_t* near, uint32_t* ring);
+void plan_capacity(int rb, int max_slots, uint32_t ncells, int dim, uint32_t* seam, uint32_t* near, uint32_t* ring);
 
 // Rebuilds ccls and the plan lists (count must be zeroed first).
 // parent: which parent cells exist (a pool parent's cells outside its tiles

@@ -105,6 +105,11 @@ namespace {
 
 const uint32_t kForceCap = 1u << 16;
 
+// The f32 weight of direction i of the pool's lattice.
+float lattice_w(int q, int i) {
+  return q == 9 ? Lattice<9>::w[i] : (q == 19 ? Lattice<19>::w[i] : Lattice<27>::w[i]);
+}
+
 template <int Q>
 void fine_params(CollideParams& p, float omega, float omega_m, const float* F) {
   using L = Lattice<Q>;
@@ -119,18 +124,26 @@ void fine_params(CollideParams& p, float omega, float omega_m, const float* F) {
     p.c1[k] = 2.f * (9.f * (1.f - 0.5f * omega) * L::w[i] * eF);
     p.so[k] = 2.f * ((1.f - 0.5f * omega_m) * 3.f * L::w[i] * eF);
   }
+  // D2Q9's f32 weights sum to 1 + this (the dense step's constant)
+  double ws = 0.0;
+  for (int i = 0; i < Q; i++) ws += (double)L::w[i];
+  p.wsum_m1 = Q == 9 ? (float)(ws - 1.0) : 0.f;
 }
 
 int validate(const lbm_amr_desc* d) {
   if (!d) return fail(LBM_E_INVALID, "null descriptor");
   if (d->abi_version != LBM_ABI_VERSION) return fail(LBM_E_INVALID, "descriptor abi_version %d", d->abi_version);
-  if (d->q != 19 && d->q != 27) return fail(LBM_E_INVALID, "AMR needs q = 19 or 27 (got %d)", d->q);
+  if (d->q != 9 && d->q != 19 && d->q != 27)
+    return fail(LBM_E_INVALID, "AMR needs q = 9 (the 2D quadtree), 19 or 27 (got %d)", d->q);
+  const bool two_d = d->q == 9;
+  if (two_d && d->nz != 1) return fail(LBM_E_INVALID, "the D2Q9 quadtree needs nz = 1 (got %d)", d->nz);
   if (!supported_rb(d->rb)) return fail(LBM_E_INVALID, "rb %d: the pool kernels are built for rb = 4, 8, 16", d->rb);
-  if (d->nx < d->rb || d->ny < d->rb || d->nz < d->rb || d->nx % d->rb || d->ny % d->rb || d->nz % d->rb)
+  if (d->nx < d->rb || d->ny < d->rb || d->nx % d->rb || d->ny % d->rb ||
+      (!two_d && (d->nz < d->rb || d->nz % d->rb)))
     return fail(LBM_E_INVALID, "grid %dx%dx%d is not a multiple of rb=%d", d->nx, d->ny, d->nz, d->rb);
   // the nearest-periodic-image tile lookup needs three blocks per axis
   // (localInTile, common_d3_pool.wgsl:81-93)
-  if (d->nx / d->rb < 3 || d->ny / d->rb < 3 || d->nz / d->rb < 3)
+  if (d->nx / d->rb < 3 || d->ny / d->rb < 3 || (!two_d && d->nz / d->rb < 3))
     return fail(LBM_E_INVALID, "grid %dx%dx%d has fewer than 3 blocks of %d on an axis", d->nx, d->ny, d->nz, d->rb);
   if (d->max_slots < 1) return fail(LBM_E_INVALID, "max_slots %d", d->max_slots);
   // explode's ring plan packs L0 coordinates in 10 bits each
@@ -139,7 +152,7 @@ int validate(const lbm_amr_desc* d) {
   if (d->criterion != 0 && d->criterion != 1) return fail(LBM_E_INVALID, "criterion %d: 0 (|curl u|) or 1 (Q)", d->criterion);
   if (d->explode_dirs < 0 || d->explode_dirs > d->q)
     return fail(LBM_E_INVALID, "explode_dirs %d outside 0..q", d->explode_dirs);
-  const double tile = std::pow(2.0 * d->rb + 2 * kGhost, 3);
+  const double tile = std::pow(2.0 * d->rb + 2 * kGhost, two_d ? 2 : 3);
   if ((double)d->max_slots * tile * 4.0 >= 4294967296.0)
     return fail(LBM_E_INVALID, "pool plane of %d tiles exceeds 4 GiB", d->max_slots);
   if (!(d->omega > 0.f && d->omega < 2.f)) return fail(LBM_E_INVALID, "omega %g outside (0, 2)", d->omega);
@@ -156,6 +169,12 @@ int validate(const lbm_amr_desc* d) {
     return fail(LBM_E_INVALID, "reflux needs the interp interface: explode/coalesce is conservative by construction");
   const int levels = d->levels == 0 ? 2 : d->levels;
   if (levels < 2 || levels > 4) return fail(LBM_E_INVALID, "levels %d: 2, 3 or 4", d->levels);
+  if (two_d) {
+    // the D2Q9 quadtree: two levels on the explode/coalesce interface, no body
+    if (levels != 2) return fail(LBM_E_REFUSED, "levels %d: the D2Q9 quadtree runs two levels", levels);
+    if (d->interface_interp) return fail(LBM_E_REFUSED, "the D2Q9 quadtree runs the explode interface");
+    if (d->body.shape != LBM_BODY_NONE) return fail(LBM_E_REFUSED, "the D2Q9 quadtree takes no body");
+  }
   if (levels >= 3) {
     // the pool-parent coupling is explode/coalesce (main-3d.js:1283-1288)
     if (d->interface_interp) return fail(LBM_E_REFUSED, "levels %d needs the explode interface", levels);
@@ -221,7 +240,7 @@ bool near_sphere(const double c[3], double h, int depth, const double ctr[3], do
 // the host hierarchy's.
 std::vector<uint8_t> level_mask(const lbm_amr_desc& d, int m, bool body) {
   const int s = 1 << (m - 1), rb = d.rb;
-  const int nbx = d.nx * s / rb, nby = d.ny * s / rb, nbz = d.nz * s / rb;
+  const int nbx = d.nx * s / rb, nby = d.ny * s / rb, nbz = d.q == 9 ? 1 : d.nz * s / rb;
   const int n[3] = {d.nx, d.ny, d.nz};
   const double sc = std::ldexp(1.0, -(m - 1)), edge = rb * sc, dil = 3.0 * rb * (1.0 - sc);
   const double ctr[3] = {d.body.centre[0], d.body.centre[1], d.body.centre[2]};
@@ -233,7 +252,7 @@ std::vector<uint8_t> level_mask(const lbm_amr_desc& d, int m, bool body) {
       if (near_sphere(c, edge / 2.0, 3, ctr, d.body.radius, kBodyMargin)) mask[b] |= 1;
     }
     if (d.sponge_width > 0.f) {
-      for (int k = 0; k < 3; k++) {
+      for (int k = 0; k < (d.q == 9 ? 2 : 3); k++) {
         const double lo = bc[k] * edge;
         if (d.sponge_axes[k] && (lo < d.sponge_width + dil || lo + edge > n[k] - d.sponge_width - dil)) mask[b] |= 2;
       }
@@ -248,7 +267,7 @@ std::vector<uint8_t> level_mask(const lbm_amr_desc& d, int m, bool body) {
 // or refines a block that reaches into the sponge band (the fine step does
 // not relax toward the far field).
 int check_topology(const lbm_amr_desc& d, const int32_t* block_slot) {
-  const int rb = d.rb, nbx = d.nx / rb, nby = d.ny / rb, nbz = d.nz / rb;
+  const int rb = d.rb, nbx = d.nx / rb, nby = d.ny / rb, nbz = d.q == 9 ? 1 : d.nz / rb;
   const int n[3] = {d.nx, d.ny, d.nz};
   for (int b = 0; b < nbx * nby * nbz; b++) {
     const int bc[3] = {b % nbx, (b / nbx) % nby, b / (nbx * nby)};
@@ -264,7 +283,7 @@ int check_topology(const lbm_amr_desc& d, const int32_t* block_slot) {
         return fail(LBM_E_INVALID, "block %d lies within one cell of the body and is not refined", b);
     }
     if (block_slot[b] >= 0 && d.sponge_width > 0.f) {
-      for (int k = 0; k < 3; k++) {
+      for (int k = 0; k < (d.q == 9 ? 2 : 3); k++) {
         const int lo = bc[k] * rb, hi = lo + rb - 1;
         if (d.sponge_axes[k] && (lo < d.sponge_width || (n[k] - 1) - hi < d.sponge_width))
           return fail(LBM_E_INVALID, "refined block %d reaches into the sponge band", b);
@@ -754,7 +773,7 @@ int create_deep_levels(lbm_amr* a, double tau1) {
     G.dx2 = (float)std::ldexp(1.0, -2 * m);
     L.nblocks = G.nbx * G.nby * G.nbz;
     L.pool_cells = a->pool_cells;
-    plan_capacity(d.rb, d.max_slots, G.ncells, &L.plan.cap_seam, &L.plan.cap_near, &L.plan.cap_ring);
+    plan_capacity(d.rb, d.max_slots, G.ncells, 3, &L.plan.cap_seam, &L.plan.cap_near, &L.plan.cap_ring);
     const size_t pool_bytes = (size_t)G.pool_plane * d.q * 4;
     int rc;
     if ((rc = dalloc(&L.pool[0], pool_bytes)) || (rc = dalloc(&L.pool[1], pool_bytes)) ||
@@ -846,17 +865,18 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
   PoolGeom& G = a->G;
   G.nx = desc->nx; G.ny = desc->ny; G.nz = desc->nz;
   G.rb = rb; G.fb = a->fb;
-  G.nbx = desc->nx / rb; G.nby = desc->ny / rb; G.nbz = desc->nz / rb;
+  const int dim = desc->q == 9 ? 2 : 3;
+  G.nbx = desc->nx / rb; G.nby = desc->ny / rb; G.nbz = dim == 2 ? 1 : desc->nz / rb;
   G.ncells = (uint32_t)((size_t)desc->nx * desc->ny * desc->nz);
   G.max_slots = desc->max_slots;
   // Planes are padded by 4 KiB: the plane size max_slots * fb^3 is a
   // multiple of 2^14 elements for every power-of-two slot count, and the
   // pool step's Q streams at one plane stride apart would share their low
   // address bits (the same DRAM channels and banks).
-  a->pool_cells = (size_t)desc->max_slots * a->fb * a->fb * a->fb;
+  a->pool_cells = (size_t)desc->max_slots * a->fb * a->fb * (dim == 3 ? a->fb : 1);
   G.pool_plane = (uint32_t)(a->pool_cells + 1024);
   a->nblocks = G.nbx * G.nby * G.nbz;
-  plan_capacity(rb, desc->max_slots, G.ncells, &a->plan.cap_seam, &a->plan.cap_near, &a->plan.cap_ring);
+  plan_capacity(rb, desc->max_slots, G.ncells, dim, &a->plan.cap_seam, &a->plan.cap_near, &a->plan.cap_ring);
   const size_t pool_bytes = (size_t)G.pool_plane * desc->q * 4;
   if ((rc = dalloc(&a->pool[0], pool_bytes)) || (rc = dalloc(&a->pool[1], pool_bytes)) ||
       (rc = dalloc(&a->mac_pool, (size_t)G.pool_plane * 16)) || (rc = dalloc(&a->block_slot, (size_t)a->nblocks * 4)) ||
@@ -932,7 +952,8 @@ int lbm_amr_create(const lbm_amr_desc* desc, lbm_amr** out) {
   // pool step's overrides are set (main-3d.js:1387-1399,
   // common_d3_amr_step1.wgsl:347-386), whatever the L0 collision is.
   const double tc = 1.0 / desc->omega, tf = 2.0 * tc - 0.5;
-  if (desc->q == 19) fine_params<19>(a->fine, (float)(1.0 / tf), (float)(1.0 / tf), desc->force);
+  if (desc->q == 9) fine_params<9>(a->fine, (float)(1.0 / tf), (float)(1.0 / tf), desc->force);
+  else if (desc->q == 19) fine_params<19>(a->fine, (float)(1.0 / tf), (float)(1.0 / tf), desc->force);
   else fine_params<27>(a->fine, (float)(1.0 / tf), (float)(1.0 / tf), desc->force);
   // post-collision non-equilibrium rescale (common_d3_pool.wgsl:172-187)
   a->c2f = (float)(0.5 * (tf - 1.0) / (tc - 1.0));
@@ -1135,7 +1156,7 @@ int lbm_amr_read_f(lbm_amr* a, float* f_l0, float* f_pool) {
       HIPCHK(hipMemcpy(f_pool + (size_t)i * n, a->pool[0] + (size_t)i * a->G.pool_plane, n * 4,
                        hipMemcpyDeviceToHost));
     for (int i = 0; i < a->d.q; i++) {
-      const float w = a->d.q == 19 ? Lattice<19>::w[i] : Lattice<27>::w[i];
+      const float w = lattice_w(a->d.q, i);
       float* p = f_pool + (size_t)i * n;
       for (size_t c = 0; c < n; c++) p[c] += w;
     }
@@ -1152,7 +1173,7 @@ int lbm_amr_read_level_f(lbm_amr* a, int32_t level, float* f_pool) {
   HIPCHK(hipStreamSynchronize(a->stream));
   for (int i = 0; i < a->d.q; i++) {
     HIPCHK(hipMemcpy(f_pool + (size_t)i * n, L.pool[0] + (size_t)i * L.G.pool_plane, n * 4, hipMemcpyDeviceToHost));
-    const float w = a->d.q == 19 ? Lattice<19>::w[i] : Lattice<27>::w[i];
+    const float w = lattice_w(a->d.q, i);
     float* p = f_pool + (size_t)i * n;
     for (size_t c = 0; c < n; c++) p[c] += w;
   }

@@ -39,17 +39,31 @@
 
 namespace lbm {
 
-template <int RB>
+// A tile of a D-dimensional pool (D = 3: D3Q19/D3Q27; D = 2: the D2Q9
+// quadtree).  A 2D tile is one layer of FB x FB cells: its z coordinate is
+// pinned at LO (an interior layer, so every z test passes) and addresses
+// ignore it, and its global z is 0 on a grid one cell deep.
+template <int RB, int D = 3>
 struct Tile {
   static constexpr int RB2 = 2 * RB;
   static constexpr int FB = RB2 + 2 * kGhost;
-  static constexpr int CELLS = FB * FB * FB;
+  static constexpr int CELLS = D == 3 ? FB * FB * FB : FB * FB;
   static constexpr int LO = kGhost, HI = kGhost + RB2;
   static constexpr int LOG_RB = RB == 4 ? 2 : (RB == 8 ? 3 : 4);
-  // FB = 4 (RB/2 + 1), so a tile is a whole number of 64-lane waves and
+  static constexpr int NCH = 1 << D;               // children of a parent cell
+  static constexpr float INV_NCH = D == 3 ? 0.125f : 0.25f;
+  // 3D: FB = 4 (RB/2 + 1), so a tile is a whole number of 64-lane waves and
   // every wave of a tile-major launch lies in one tile
-  static_assert(CELLS % 64 == 0, "tile cells must fill whole waves");
+  static_assert(D == 2 || CELLS % 64 == 0, "tile cells must fill whole waves");
+  // tile-local coordinates of cell r
+  __device__ static void xyz(uint32_t r, int& fx, int& fy, int& fz) {
+    fx = (int)(r % FB);
+    fy = (int)((r / FB) % FB);
+    fz = D == 3 ? (int)(r / (FB * FB)) : LO;
+  }
 };
+template <int Q, int RB>
+using TileQ = Tile<RB, Lattice<Q>::D>;
 
 // Block b of a launch runs on XCD b % 8, and each XCD has its own L2.
 // Renumbering so that each XCD walks one contiguous eighth of the blocks
@@ -61,10 +75,13 @@ __device__ __forceinline__ uint32_t xcd_block() {
   return b < per * 8u ? (b & 7u) * per + (b >> 3) : b;
 }
 
-// Tile index of a tile-major launch, made wave-uniform (scalar registers).
-template <int RB>
+// Tile index of a tile-major launch, made wave-uniform (scalar registers)
+// where a wave lies in one tile (3D); per lane for a 2D tile, which is not a
+// whole number of waves.
+template <int RB, int D = 3>
 __device__ __forceinline__ uint32_t wave_tile(uint32_t t) {
-  return __builtin_amdgcn_readfirstlane(t / (uint32_t)Tile<RB>::CELLS);
+  if constexpr (D == 3) return __builtin_amdgcn_readfirstlane(t / (uint32_t)Tile<RB>::CELLS);
+  else return t / (uint32_t)Tile<RB, 2>::CELLS;
 }
 
 // v in [-n, 2n) -> [0, n)
@@ -73,10 +90,11 @@ __device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >
 __device__ __forceinline__ uint32_t coarse_cell(const PoolGeom& G, int x, int y, int z) {
   return (uint32_t)((z * G.ny + y) * G.nx + x);
 }
-template <int RB>
+template <int RB, int D = 3>
 __device__ __forceinline__ uint32_t pool_cell(int slot, int fx, int fy, int fz) {
-  constexpr int FB = Tile<RB>::FB;
-  return (uint32_t)(slot * Tile<RB>::CELLS + (fz * FB + fy) * FB + fx);
+  constexpr int FB = Tile<RB, D>::FB;
+  if constexpr (D == 3) return (uint32_t)(slot * Tile<RB>::CELLS + (fz * FB + fy) * FB + fx);
+  else return (uint32_t)(slot * Tile<RB, 2>::CELLS + fy * FB + fx);
 }
 __device__ __forceinline__ int block_id(const PoolGeom& G, int bx, int by, int bz) {
   return (bz * G.nby + by) * G.nbx + bx;
@@ -231,13 +249,14 @@ template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256) k_ring_plan(PoolGeom G, SurfacePlan plan, const int32_t* __restrict__ active,
                                                    int nactive, ParentRef P) {
   using L = Lattice<Q>;
-  using T = Tile<RB>;
-  constexpr int FB = T::FB, RB2 = T::RB2, LO = T::LO, HI = T::HI;
+  using T = TileQ<Q, RB>;
+  constexpr int RB2 = T::RB2, LO = T::LO, HI = T::HI;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k = t / (uint32_t)T::CELLS;
   if (k >= (uint32_t)nactive) return;
   const uint32_t r = t - k * (uint32_t)T::CELLS;
-  const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
+  int fx, fy, fz;
+  T::xyz(r, fx, fy, fz);
   if (fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI) return;
   const int slot = active[1 + k];
   const int id = G.slot_to_block[slot];
@@ -271,12 +290,13 @@ __global__ void __launch_bounds__(256) k_ring_plan(PoolGeom G, SurfacePlan plan,
                               (uint32_t)vx | (uint32_t)vy << 10 | (uint32_t)vz << 20);
 }
 
-void plan_capacity(int rb, int max_slots, uint32_t ncells, uint32_t* seam, uint32_t* near, uint32_t* ring) {
-  const uint64_t fb = 2 * rb + 2 * kGhost, inner = (uint64_t)(2 * rb) * (2 * rb) * (2 * rb);
-  const uint64_t shell = (uint64_t)rb * rb * rb - (uint64_t)(rb - 2) * (rb - 2) * (rb - 2);
+void plan_capacity(int rb, int max_slots, uint32_t ncells, int dim, uint32_t* seam, uint32_t* near, uint32_t* ring) {
+  auto pw = [&](uint64_t v) { return dim == 3 ? v * v * v : v * v; };
+  const uint64_t fb = 2 * rb + 2 * kGhost;
+  const uint64_t shell = pw((uint64_t)rb) - pw((uint64_t)(rb - 2));
   *seam = (uint32_t)std::min<uint64_t>((uint64_t)max_slots * shell, ncells);
   *near = ncells;
-  *ring = (uint32_t)((uint64_t)max_slots * (fb * fb * fb - inner));
+  *ring = (uint32_t)((uint64_t)max_slots * (pw(fb) - pw((uint64_t)(2 * rb))));
 }
 
 // The per-tile neighbour table (kTileWords words per active tile).  The pool
@@ -326,9 +346,9 @@ __device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, fl
                                                bool collide_cell, bool load = true, bool put_mac = true) {
   const int slot = tt[13], bx = tt[28], by = tt[29], bz = tt[30];
   using L = Lattice<Q>;
-  using T = Tile<RB>;
-  constexpr int FB = T::FB, RB2 = T::RB2, LO = T::LO, HI = T::HI;
-  const uint32_t cell = pool_cell<RB>(slot, fx, fy, fz);
+  using T = TileQ<Q, RB>;
+  constexpr int D = L::D, FB = T::FB, RB2 = T::RB2, LO = T::LO, HI = T::HI;
+  const uint32_t cell = pool_cell<RB, D>(slot, fx, fy, fz);
   float g[Q];
   const int offx = fx <= LO ? -1 : (fx + 1 >= HI ? 1 : 0);
   const int offy = fy <= LO ? -1 : (fy + 1 >= HI ? 1 : 0);
@@ -370,7 +390,7 @@ __device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, fl
     sx = min(max(sx, 0), FB - 1);
     sy = min(max(sy, 0), FB - 1);
     sz = min(max(sz, 0), FB - 1);
-    g[i] = load ? *at(gin + (size_t)i * G.pool_plane, pool_cell<RB>(src_slot, sx, sy, sz)) : 0.f;
+    g[i] = load ? *at(gin + (size_t)i * G.pool_plane, pool_cell<RB, D>(src_slot, sx, sy, sz)) : 0.f;
   }
   float phi = 1e30f;
   if (BODY) {
@@ -392,7 +412,9 @@ __device__ __forceinline__ void pool_cell_step(const float* __restrict__ gin, fl
     float drho, rho, ux, uy, uz;
     float S[Pairs<Q>::NP], D[Pairs<Q>::NP];
     pair_sums<Q>(g, S, D);
-    pair_moments<Q, FRC>(g[0], S, D, P.F, drho, rho, ux, uy, uz);
+    // D2Q9's f32 weights do not sum to 1: the density adds their excess, as
+    // the dense step does
+    pair_moments<Q, FRC>(Q == 9 ? g[0] + P.wsum_m1 : g[0], S, D, P.F, drho, rho, ux, uy, uz);
     if (mac_pool && put_mac) {
       *at(mac_pool, cell) = rho;
       *at(mac_pool + G.pool_plane, cell) = ux;
@@ -436,14 +458,15 @@ template <int Q, int RB, bool BODY, bool FRC>
 __global__ void __launch_bounds__(256)
 k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __restrict__ mac_pool, PoolGeom G,
             CollideParams P, int nactive, int collide_ring, int sub) {
-  using T = Tile<RB>;
+  using T = TileQ<Q, RB>;
   constexpr int FB = T::FB, LO = T::LO, HI = T::HI;
   const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
-  const uint32_t k = wave_tile<RB>(t);
+  const uint32_t k = wave_tile<RB, Lattice<Q>::D>(t);
   if (k >= (uint32_t)nactive) return;
   const uint32_t r = t - k * (uint32_t)T::CELLS;
   const int32_t* tt = G.tile_table + (size_t)k * kTileWords;
-  const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
+  int fx, fy, fz;
+  T::xyz(r, fx, fy, fz);
   const bool interior = fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI;
   bool load = true, put_mac = true;
   if (interior) {
@@ -535,8 +558,8 @@ __global__ void __launch_bounds__(256)
 k_explode(ParentRef P, float* __restrict__ gp, PoolGeom G, const uint4* __restrict__ ring, uint32_t nring, int linear,
           int per) {
   using L = Lattice<Q>;
-  using T = Tile<RB>;
-  constexpr int FB = T::FB, LO = T::LO, HI = T::HI;
+  using T = TileQ<Q, RB>;
+  constexpr int LO = T::LO, HI = T::HI;
   const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nring) return;
   // Grid row g takes directions [g per, (g + 1) per): a ring cell's Q
@@ -549,8 +572,8 @@ k_explode(ParentRef P, float* __restrict__ gp, PoolGeom G, const uint4* __restri
   // (k_ring_plan)
   const uint4 d = ring[t];
   const uint32_t cell = d.x, vc = d.y, bits = d.z;
-  const uint32_t r = cell % (uint32_t)T::CELLS;
-  const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
+  int fx, fy, fz;
+  T::xyz(cell % (uint32_t)T::CELLS, fx, fy, fz);
   const int vx = d.w & 1023, vy = (d.w >> 10) & 1023, vz = d.w >> 20;
   const bool okx = (bits >> 27) & 1, oky = (bits >> 28) & 1, okz = (bits >> 29) & 1;
   // the gradient's neighbours (present wherever their axis is ok)
@@ -610,13 +633,15 @@ k_explode(ParentRef P, float* __restrict__ gp, PoolGeom G, const uint4* __restri
 // coalesce: fine ring outflux -> time-t coarse cells under refined blocks,
 // plus the orphan correction on unrefined cells next to them
 // ---------------------------------------------------------------------------
-// mac projection of every covered L0 cell from its 8 children (rho mean,
-// mass-weighted u; common_d3_amr_coalesce.wgsl:245-261)
-template <int RB, bool PP>
+// mac projection of every covered L0 cell from its 8 children (4 in 2D;
+// rho mean, mass-weighted u; common_d3_amr_coalesce.wgsl:245-261)
+template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256)
 k_coalesce_mac(ParentRef P, const float* __restrict__ mac_pool, PoolGeom G, const int32_t* __restrict__ active,
                int nactive) {
-  constexpr uint32_t per = RB * RB * RB;
+  using T = TileQ<Q, RB>;
+  constexpr int D = Lattice<Q>::D;
+  constexpr uint32_t per = D == 3 ? RB * RB * RB : RB * RB;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k = t / per;
   if (k >= (uint32_t)nactive) return;
@@ -628,8 +653,8 @@ k_coalesce_mac(ParentRef P, const float* __restrict__ mac_pool, PoolGeom G, cons
   const int f0x = kGhost + 2 * lx, f0y = kGhost + 2 * ly, f0z = kGhost + 2 * lz;
   float rs = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
 #pragma unroll
-  for (int c = 0; c < 8; c++) {
-    const uint32_t pc = pool_cell<RB>(slot, f0x + (c & 1), f0y + ((c >> 1) & 1), f0z + ((c >> 2) & 1));
+  for (int c = 0; c < T::NCH; c++) {
+    const uint32_t pc = pool_cell<RB, D>(slot, f0x + (c & 1), f0y + ((c >> 1) & 1), f0z + ((c >> 2) & 1));
     const float rho = *at(mac_pool, pc);
     rs += rho;
     mx += rho * *at(mac_pool + G.pool_plane, pc);
@@ -639,7 +664,7 @@ k_coalesce_mac(ParentRef P, const float* __restrict__ mac_pool, PoolGeom G, cons
   const float inv = 1.f / fmaxf(rs, 1e-6f);
   const uint32_t cc = parent_index<PP, RB>(G, P, bx * RB + lx, by * RB + ly, bz * RB + lz);
   if (PP && cc == kAbsent) return;  // a tile's parent cells lie in parent tiles
-  pmac_put<PP>(P, cc, make_float4(rs * 0.125f, mx * inv, my * inv, mz * inv));
+  pmac_put<PP>(P, cc, make_float4(rs * T::INV_NCH, mx * inv, my * inv, mz * inv));
 }
 
 // Covered seam cells: for every direction whose target is unrefined, the
@@ -649,8 +674,8 @@ template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256)
 k_coalesce_seam(const float* __restrict__ gp, ParentRef P, PoolGeom G, const uint32_t* __restrict__ seam,
                 uint32_t nseam) {
-  using T = Tile<RB>;
-  constexpr int FB = T::FB;
+  using T = TileQ<Q, RB>;
+  constexpr int FB = T::FB, D = Lattice<Q>::D;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nseam) return;
   const int i = 1 + blockIdx.y;
@@ -669,12 +694,12 @@ k_coalesce_seam(const float* __restrict__ gp, ParentRef P, PoolGeom G, const uin
   const int ly = local_in_tile<RB>(2 * ty, by, 2 * G.ny);
   const int lz = local_in_tile<RB>(2 * tz, bz, 2 * G.nz);
   const float* p = gp + (size_t)i * G.pool_plane;
-  const uint32_t b0 = pool_cell<RB>(slot, lx, ly, lz);
+  const uint32_t b0 = pool_cell<RB, D>(slot, lx, ly, lz);
   float s = 0.f;
 #pragma unroll
-  for (int k = 0; k < 8; k++) s += *at(p, b0 + (k & 1) + ((k >> 1) & 1) * FB + ((k >> 2) & 1) * FB * FB);
+  for (int k = 0; k < T::NCH; k++) s += *at(p, b0 + (k & 1) + ((k >> 1) & 1) * FB + ((k >> 2) & 1) * FB * FB);
   // an average of g is the average of f minus w
-  *ppop<PP, Q>(P, PP ? parent_index<PP, RB>(G, P, cx, cy, cz) : c, i) = s * 0.125f;
+  *ppop<PP, Q>(P, PP ? parent_index<PP, RB>(G, P, cx, cy, cz) : c, i) = s * T::INV_NCH;
 }
 
 // Orphans on unrefined L0 cells next to the refined region
@@ -684,7 +709,8 @@ k_coalesce_seam(const float* __restrict__ gp, ParentRef P, PoolGeom G, const uin
 template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256)
 k_orphans(const float* __restrict__ gp, ParentRef P, PoolGeom G, const uint32_t* __restrict__ near, uint32_t nnear) {
-  using T = Tile<RB>;
+  using T = TileQ<Q, RB>;
+  constexpr int D = Lattice<Q>::D;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nnear) return;
   const int i = 1 + blockIdx.y;
@@ -699,7 +725,7 @@ k_orphans(const float* __restrict__ gp, ParentRef P, PoolGeom G, const uint32_t*
   float s = 0.f;   // sum of f over the children found in refined tiles
   float nIn = 0.f;
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
+  for (int k = 0; k < T::NCH; k++) {
     const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
     const int px = 2 * tx + ox, py = 2 * ty + oy, pz = 2 * tz + oz;
     const int qx = wrap1(px - ex, NFx), qy = wrap1(py - ey, NFy), qz = wrap1(pz - ez, NFz);
@@ -707,8 +733,8 @@ k_orphans(const float* __restrict__ gp, ParentRef P, PoolGeom G, const uint32_t*
     const int qs = G.block_slot[block_id(G, qbx, qby, qbz)];
     if (qs >= 0) {
       s += *at(gp + (size_t)i * G.pool_plane,
-               pool_cell<RB>(qs, local_in_tile<RB>(px, qbx, NFx), local_in_tile<RB>(py, qby, NFy),
-                             local_in_tile<RB>(pz, qbz, NFz))) + wi;
+               pool_cell<RB, D>(qs, local_in_tile<RB>(px, qbx, NFx), local_in_tile<RB>(py, qby, NFy),
+                                local_in_tile<RB>(pz, qbz, NFz))) + wi;
     }
     const int hx = wrap1(2 * cx + ox + ex, NFx), hy = wrap1(2 * cy + oy + ey, NFy), hz = wrap1(2 * cz + oz + ez, NFz);
     if (G.ccls[coarse_cell(G, hx >> 1, hy >> 1, hz >> 1)] & kCovered) nIn += 1.f;
@@ -716,7 +742,7 @@ k_orphans(const float* __restrict__ gp, ParentRef P, PoolGeom G, const uint32_t*
   float& gci = *ppop<PP, Q>(P, PP ? parent_index<PP, RB>(G, P, cx, cy, cz) : c, i);
   const float fc = gci + wi;
   const float net = s - nIn * fc;
-  if (net != 0.f) gci = (fc + net * 0.125f) - wi;
+  if (net != 0.f) gci = (fc + net * T::INV_NCH) - wi;
 }
 
 // ---------------------------------------------------------------------------
@@ -739,14 +765,15 @@ template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256)
 k_interp(ParentRef P, float* __restrict__ gp, PoolGeom G, const int32_t* __restrict__ pairs, int npairs,
          float rescale, int ring_only) {
-  using T = Tile<RB>;
-  constexpr int FB = T::FB, LO = T::LO, HI = T::HI;
+  using T = TileQ<Q, RB>;
+  constexpr int LO = T::LO, HI = T::HI, D = Lattice<Q>::D;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k = t / (uint32_t)T::CELLS;
   if (k >= (uint32_t)npairs) return;
   const uint32_t r = t - k * (uint32_t)T::CELLS;
   const int id = pairs[1 + 2 * k], slot = pairs[2 + 2 * k];
-  const int fx = r % FB, fy = (r / FB) % FB, fz = r / (FB * FB);
+  int fx, fy, fz;
+  T::xyz(r, fx, fy, fz);
   // GHOST_ONLY (common_d3_amr_interp.wgsl:4-9): interior cells keep their
   // evolved state
   if (ring_only && fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI) return;
@@ -755,8 +782,9 @@ k_interp(ParentRef P, float* __restrict__ gp, PoolGeom G, const int32_t* __restr
   const float px = (float)(bx * RB) - 0.25f + 0.5f * (float)(fx - kGhost);
   const float py = (float)(by * RB) - 0.25f + 0.5f * (float)(fy - kGhost);
   const float pz = (float)(bz * RB) - 0.25f + 0.5f * (float)(fz - kGhost);
-  const int c0x = (int)floorf(px), c0y = (int)floorf(py), c0z = (int)floorf(pz);
-  const float tx = px - c0x, ty = py - c0y, tz = pz - c0z;
+  const int c0x = (int)floorf(px), c0y = (int)floorf(py), c0z = D == 3 ? (int)floorf(pz) : 0;
+  // bilinear in 2D: one parent layer, weight 1
+  const float tx = px - c0x, ty = py - c0y, tz = D == 3 ? pz - c0z : 0.f;
   // A pool parent cell is read in the parent tile that OWNS it, at its
   // interior index (parentIndex, common_d3_parent_pool.wgsl:37-59), so a
   // tile filled mid-run sees solved parent cells rather than a parent ring
@@ -777,12 +805,12 @@ k_interp(ParentRef P, float* __restrict__ gp, PoolGeom G, const int32_t* __restr
   float rho = 0.f, ux = 0.f, uy = 0.f, uz = 0.f, fneq[Q];
 #pragma unroll
   for (int i = 0; i < Q; i++) fneq[i] = 0.f;
-  for (int s = 0; s < 8; s++) {
+  for (int s = 0; s < T::NCH; s++) {
     const int i = s & 1, j = (s >> 1) & 1, kk = (s >> 2) & 1;
     const float w = (i ? tx : 1.f - tx) * (j ? ty : 1.f - ty) * (kk ? tz : 1.f - tz);
     const int vx = wrap1(c0x + i, G.nx), vy = wrap1(c0y + j, G.ny), vz = wrap1(c0z + kk, G.nz);
     uint32_t cell = parent_index<PP, RB>(G, P, vx, vy, vz);
-    if (PP && cell == kAbsent) cell = pool_cell<RB>(ps, c0x + i + pox, c0y + j + poy, c0z + kk + poz);
+    if (PP && cell == kAbsent) cell = pool_cell<RB, D>(ps, c0x + i + pox, c0y + j + poy, c0z + kk + poz);
     float sr, sx, sy, sz, sf[Q];
     coarse_sample<Q, PP>(P, cell, sr, sx, sy, sz, sf);
     rho += w * sr;
@@ -792,7 +820,7 @@ k_interp(ParentRef P, float* __restrict__ gp, PoolGeom G, const int32_t* __restr
 #pragma unroll
     for (int q = 0; q < Q; q++) fneq[q] += w * sf[q];
   }
-  const uint32_t cell = pool_cell<RB>(slot, fx, fy, fz);
+  const uint32_t cell = pool_cell<RB, D>(slot, fx, fy, fz);
 #pragma unroll
   for (int q = 0; q < Q; q++) gp[(size_t)q * G.pool_plane + cell] = geq<Q>(q, rho, ux, uy, uz) + rescale * fneq[q];
 }
@@ -801,7 +829,9 @@ template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256)
 k_average(const float* __restrict__ gp, ParentRef P, PoolGeom G, const int32_t* __restrict__ pairs, int npairs,
           float rescale) {
-  constexpr uint32_t per = RB * RB * RB;
+  using T = TileQ<Q, RB>;
+  constexpr int D = Lattice<Q>::D;
+  constexpr uint32_t per = D == 3 ? RB * RB * RB : RB * RB;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t k = t / per;
   if (k >= (uint32_t)npairs) return;
@@ -812,9 +842,9 @@ k_average(const float* __restrict__ gp, ParentRef P, PoolGeom G, const int32_t* 
   float rs = 0.f, mx = 0.f, my = 0.f, mz = 0.f, sneq[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) sneq[q] = 0.f;
-  for (int c = 0; c < 8; c++) {
-    const uint32_t pc = pool_cell<RB>(slot, kGhost + 2 * lx + (c & 1), kGhost + 2 * ly + ((c >> 1) & 1),
-                                      kGhost + 2 * lz + ((c >> 2) & 1));
+  for (int c = 0; c < T::NCH; c++) {
+    const uint32_t pc = pool_cell<RB, D>(slot, kGhost + 2 * lx + (c & 1), kGhost + 2 * ly + ((c >> 1) & 1),
+                                         kGhost + 2 * lz + ((c >> 2) & 1));
     float g[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) g[q] = gp[(size_t)q * G.pool_plane + pc];
@@ -828,13 +858,13 @@ k_average(const float* __restrict__ gp, ParentRef P, PoolGeom G, const int32_t* 
 #pragma unroll
     for (int q = 0; q < Q; q++) sneq[q] += g[q] - geq<Q>(q, rho, ux, uy, uz);
   }
-  const float rho = rs * 0.125f;
+  const float rho = rs * T::INV_NCH;
   const float inv = 1.f / fmaxf(rs, 1e-6f);
   const float ux = mx * inv, uy = my * inv, uz = mz * inv;
   const uint32_t cc = parent_index<PP, RB>(G, P, bx * RB + lx, by * RB + ly, bz * RB + lz);
   if (PP && cc == kAbsent) return;  // a tile's parent cells lie in parent tiles
 #pragma unroll
-  for (int q = 0; q < Q; q++) *ppop<PP, Q>(P, cc, q) = geq<Q>(q, rho, ux, uy, uz) + rescale * (sneq[q] * 0.125f);
+  for (int q = 0; q < Q; q++) *ppop<PP, Q>(P, cc, q) = geq<Q>(q, rho, ux, uy, uz) + rescale * (sneq[q] * T::INV_NCH);
   pmac_put<PP>(P, cc, make_float4(rho, ux, uy, uz));
 }
 
@@ -1008,9 +1038,10 @@ __global__ void __launch_bounds__(64) k_criterion(const float4* __restrict__ mac
                                                   int lead, int kind) {
   const int b = blockIdx.x;
   const int bx = b % G.nbx, by = (b / G.nbx) % G.nby, bz = b / (G.nbx * G.nby);
-  const int x0 = bx * G.rb - lead, y0 = by * G.rb - lead, z0 = bz * G.rb - lead;
-  const int n = G.rb + 2 * lead;
-  const int total = n * n * n;
+  // a grid one cell deep (the 2D pool) takes its single layer
+  const int n = G.rb + 2 * lead, nzb = G.nz == 1 ? 1 : n;
+  const int x0 = bx * G.rb - lead, y0 = by * G.rb - lead, z0 = G.nz == 1 ? 0 : bz * G.rb - lead;
+  const int total = n * n * nzb;
   float best = -1e30f;
   for (int k = threadIdx.x; k < total; k += 64) {
     const int kx = k % n, ky = (k / n) % n, kz = k / (n * n);
@@ -1197,7 +1228,8 @@ template <int Q, int RB>
 __device__ __forceinline__ void level_sums(const float* __restrict__ gp, const PoolGeom& G,
                                            const int32_t* __restrict__ active, double weight, double (&v)[4]) {
   using L = Lattice<Q>;
-  constexpr uint32_t rb2 = 2 * RB, per = rb2 * rb2 * rb2;
+  constexpr int D = L::D;
+  constexpr uint32_t rb2 = 2 * RB, per = D == 3 ? rb2 * rb2 * rb2 : rb2 * rb2;
   const uint32_t stride = gridDim.x * blockDim.x;
   const int nact = active[0];
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < (uint32_t)nact * per; t += stride) {
@@ -1209,7 +1241,7 @@ __device__ __forceinline__ void level_sums(const float* __restrict__ gp, const P
       const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
       if (covered_by_child<RB>(G, bx * (int)rb2 + lx, by * (int)rb2 + ly, bz * (int)rb2 + lz)) continue;
     }
-    const uint32_t pc = pool_cell<RB>(slot, kGhost + lx, kGhost + ly, kGhost + lz);
+    const uint32_t pc = pool_cell<RB, D>(slot, kGhost + lx, kGhost + ly, kGhost + lz);
     for (int i = 0; i < Q; i++) {
       const double f = weight * ((double)gp[(size_t)i * G.pool_plane + pc] + (double)L::w[i]);
       v[0] += f; v[1] += L::ex[i] * f; v[2] += L::ey[i] * f; v[3] += L::ez[i] * f;
@@ -1243,7 +1275,7 @@ k_pool_totals(const float* __restrict__ gc, const float* __restrict__ gp, PoolGe
       v[0] += f; v[1] += L::ex[i] * f; v[2] += L::ey[i] * f; v[3] += L::ez[i] * f;
     }
   }
-  level_sums<Q, RB>(gp, G, active, 0.125, v);
+  level_sums<Q, RB>(gp, G, active, Lattice<Q>::D == 3 ? 0.125 : 0.25, v);
   block_sum<4>(v, lds);
   partials_then_fixed_order_sum<4>(v, partials, counter, [&](int j, double s) { out[j] = s; });
 }
@@ -1252,9 +1284,29 @@ k_pool_totals(const float* __restrict__ gc, const float* __restrict__ gp, PoolGe
 // launches: dispatch on (q, rb)
 // ---------------------------------------------------------------------------
 static inline dim3 grid_for(uint64_t n, unsigned rows = 1) { return dim3((unsigned)((n + 255) / 256), rows); }
+// parent cells under one block (rb^D)
+template <int Q, int RB>
+static constexpr uint64_t parent_cells() { return Lattice<Q>::D == 3 ? (uint64_t)RB * RB * RB : (uint64_t)RB * RB; }
 
+// The passes of the two-level explode interface also run the D2Q9 quadtree.
 template <template <int, int> class Op, class... Args>
 static hipError_t dispatch_q_rb(int q, int rb, Args&&... args) {
+  switch (q * 100 + rb) {
+    case 904: return Op<9, 4>::run(args...);
+    case 908: return Op<9, 8>::run(args...);
+    case 916: return Op<9, 16>::run(args...);
+    case 1904: return Op<19, 4>::run(args...);
+    case 1908: return Op<19, 8>::run(args...);
+    case 1916: return Op<19, 16>::run(args...);
+    case 2704: return Op<27, 4>::run(args...);
+    case 2708: return Op<27, 8>::run(args...);
+    case 2716: return Op<27, 16>::run(args...);
+    default: return hipErrorInvalidValue;
+  }
+}
+// The 3D-only passes (the body's force, the seam flux, levels >= 3).
+template <template <int, int> class Op, class... Args>
+static hipError_t dispatch_q3_rb(int q, int rb, Args&&... args) {
   switch (q * 100 + rb) {
     case 1904: return Op<19, 4>::run(args...);
     case 1908: return Op<19, 8>::run(args...);
@@ -1282,7 +1334,7 @@ struct OpCellClass {
       constexpr bool pp = decltype(PP)::value;
       hipLaunchKernelGGL((k_cell_class<Q, RB, pp>), grid_for(g.ncells), dim3(256), 0, s, ccls, g, plan, P);
       if (nactive > 0)
-        hipLaunchKernelGGL((k_ring_plan<Q, RB, pp>), grid_for((uint64_t)nactive * Tile<RB>::CELLS), dim3(256), 0, s, g,
+        hipLaunchKernelGGL((k_ring_plan<Q, RB, pp>), grid_for((uint64_t)nactive * TileQ<Q, RB>::CELLS), dim3(256), 0, s, g,
                            plan, active, nactive, P);
     });
     if (nactive > 0)
@@ -1295,7 +1347,7 @@ template <int Q, int RB>
 struct OpPoolStep {
   static hipError_t run(const float* gin, float* gout, float* mac_pool, const PoolGeom& g, const CollideParams& p,
                         int nactive, int collide_ring, int sub, hipStream_t s) {
-    const dim3 grid = grid_for((uint64_t)nactive * Tile<RB>::CELLS);
+    const dim3 grid = grid_for((uint64_t)nactive * TileQ<Q, RB>::CELLS);
     // a level without a body force runs the instantiation without Guo terms
     const bool frc = p.F[0] != 0.f || p.F[1] != 0.f || p.F[2] != 0.f;
     auto go = [&](auto B, auto F) {
@@ -1362,7 +1414,7 @@ struct OpCoalesce {
     with_parent(P, [&](auto PP) {
       constexpr bool pp = decltype(PP)::value;
       if (nactive > 0)
-        hipLaunchKernelGGL((k_coalesce_mac<RB, pp>), grid_for((uint64_t)nactive * RB * RB * RB), dim3(256), 0, s, P,
+        hipLaunchKernelGGL((k_coalesce_mac<Q, RB, pp>), grid_for((uint64_t)nactive * parent_cells<Q, RB>()), dim3(256), 0, s, P,
                            mac_pool, g, active, nactive);
       if (nseam > 0)
         hipLaunchKernelGGL((k_coalesce_seam<Q, RB, pp>), grid_for(nseam, Q - 1), dim3(256), 0, s, gp, P, g, plan.seam,
@@ -1379,7 +1431,7 @@ struct OpInterp {
   static hipError_t run(const ParentRef& P, float* gp, const PoolGeom& g, const int32_t* pairs, int npairs,
                         float rescale, int ring_only, hipStream_t s) {
     with_parent(P, [&](auto PP) {
-      hipLaunchKernelGGL((k_interp<Q, RB, decltype(PP)::value>), grid_for((uint64_t)npairs * Tile<RB>::CELLS),
+      hipLaunchKernelGGL((k_interp<Q, RB, decltype(PP)::value>), grid_for((uint64_t)npairs * TileQ<Q, RB>::CELLS),
                          dim3(256), 0, s, P, gp, g, pairs, npairs, rescale, ring_only);
     });
     return hipGetLastError();
@@ -1390,7 +1442,7 @@ struct OpAverage {
   static hipError_t run(const float* gp, const ParentRef& P, const PoolGeom& g, const int32_t* pairs, int npairs,
                         float rescale, hipStream_t s) {
     with_parent(P, [&](auto PP) {
-      hipLaunchKernelGGL((k_average<Q, RB, decltype(PP)::value>), grid_for((uint64_t)npairs * RB * RB * RB),
+      hipLaunchKernelGGL((k_average<Q, RB, decltype(PP)::value>), grid_for((uint64_t)npairs * parent_cells<Q, RB>()),
                          dim3(256), 0, s, gp, P, g, pairs, npairs, rescale);
     });
     return hipGetLastError();
@@ -1430,17 +1482,17 @@ uint32_t pool_force_blocks(int rb, int nactive) {
 }
 hipError_t launch_pool_force(int q, const float* gpool, const PoolGeom& g, const int32_t* active, int nactive,
                              double* partials, hipStream_t s) {
-  return dispatch_q_rb<OpPoolForce>(q, g.rb, gpool, g, active, nactive, partials, s);
+  return dispatch_q3_rb<OpPoolForce>(q, g.rb, gpool, g, active, nactive, partials, s);
 }
 hipError_t launch_flux(int q, const float* gp, float* acc, const PoolGeom& g, const SurfacePlan& plan, uint32_t nflux,
                        int accumulate, hipStream_t s) {
   if (nflux == 0) return hipSuccess;
-  return dispatch_q_rb<OpFlux>(q, g.rb, gp, acc, g, plan, nflux, accumulate, s);
+  return dispatch_q3_rb<OpFlux>(q, g.rb, gp, acc, g, plan, nflux, accumulate, s);
 }
 hipError_t launch_reflux(int q, const float* gct, float* gct1, float* mac, const PoolGeom& g, const SurfacePlan& plan,
                          uint32_t nflux, float* acc, const float* force, hipStream_t s) {
   if (nflux == 0) return hipSuccess;
-  return dispatch_q_rb<OpReflux>(q, g.rb, gct, gct1, mac, g, plan, nflux, acc, force, s);
+  return dispatch_q3_rb<OpReflux>(q, g.rb, gct, gct1, mac, g, plan, nflux, acc, force, s);
 }
 
 hipError_t launch_explode(int q, const ParentRef& parent, float* gp, const PoolGeom& g, const SurfacePlan& plan,
@@ -1514,7 +1566,7 @@ hipError_t launch_pool_totals(int q, const float* gc, const float* gp, const Poo
 }
 hipError_t launch_level_totals(int q, const float* gp, const PoolGeom& g, const int32_t* active, double weight,
                                double* partials, unsigned* counter, double* out, hipStream_t s) {
-  return dispatch_q_rb<OpLevelTotals>(q, g.rb, gp, g, active, weight, partials, counter, out, s);
+  return dispatch_q3_rb<OpLevelTotals>(q, g.rb, gp, g, active, weight, partials, counter, out, s);
 }
 
 }  // namespace lbm

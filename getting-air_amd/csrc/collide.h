@@ -19,6 +19,7 @@ struct CollideParams {
   float F[3];
   float c1[13];               // 2 x 9 (1 - omega/2) w (e.F) per pair (doubled)
   float so[13];               // 2 x (1 - omega_m/2) 3 w (e.F) per pair (doubled)
+  float wsum_m1;              // (sum of the f32 weights) - 1: nonzero for D2Q9 alone
 };
 
 // e_i . (x, y, z) for a direction known at compile time: the zero components

@@ -470,9 +470,11 @@ class Amr:
         d = self.desc
         self.q, self.rb, self.max_slots = d.q, d.rb, d.max_slots
         self.ncells = d.nx * d.ny * d.nz
-        self.nblocks = (d.nx // d.rb) * (d.ny // d.rb) * (d.nz // d.rb)
+        # q = 9: the D2Q9 quadtree, one cell deep, tiles of fb x fb cells
+        self.dim = 2 if d.q == 9 else 3
+        self.nblocks = (d.nx // d.rb) * (d.ny // d.rb) * (d.nz // d.rb if self.dim == 3 else 1)
         self.fb = 2 * d.rb + 2 * GHOST
-        self.pool_plane = d.max_slots * self.fb ** 3
+        self.pool_plane = d.max_slots * self.fb ** self.dim
         self.levels = d.levels or 2
         h = ctypes.c_void_p()
         _check(self.lib.lbm_amr_create(ctypes.byref(self.desc), ctypes.byref(h)))

@@ -143,6 +143,14 @@ def test_amr_refusals_without_gpu():
     d = pylbm.make_amr_desc(dict(base, levels=3, omega=1.0 / 0.75))  # tau_1 = 1
     assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == -4
     assert b"level 1" in lib.lbm_last_error()
+    # the D2Q9 quadtree: one cell deep, two levels, explode interface, no body
+    b2 = dict(base, q=9, nz=1)
+    for kw, code, text in ((dict(nz=8), -1, b"nz = 1"), (dict(levels=3), -4, b"two levels"),
+                           (dict(interface="interp"), -4, b"explode"),
+                           (dict(body={"a": 2.0, "centre": [12.0, 12.0, 0.0]}), -4, b"no body")):
+        d = pylbm.make_amr_desc(dict(b2, **kw))
+        assert lib.lbm_amr_create(ctypes.byref(d), ctypes.byref(h)) == code, kw
+        assert text in lib.lbm_last_error(), (kw, lib.lbm_last_error())
 
 
 def test_serve_struct_layouts_match_c():
