@@ -670,7 +670,8 @@ napi_value CreateAmr(napi_env env, napi_callback_info info) {
   h->kind = kAmr;
   h->q = d.q;
   h->cells = (size_t)d.nx * d.ny * d.nz;
-  h->nblocks = d.rb > 0 ? (size_t)(d.nx / d.rb) * (d.ny / d.rb) * (d.nz / d.rb) : 0;
+  // the D2Q9 quadtree (q 9, nz 1) tiles the xy plane only
+  h->nblocks = d.rb > 0 ? (size_t)(d.nx / d.rb) * (d.ny / d.rb) * (d.q == 9 ? 1 : d.nz / d.rb) : 0;
   h->slots = (size_t)d.max_slots;
   h->levels = d.levels;
   return wrap(env, h);
@@ -731,8 +732,8 @@ napi_value AmrUploadLevelTopology(napi_env env, napi_callback_info info) {
     napi_throw_range_error(env, nullptr, "level outside 1 .. levels - 1");
     return nullptr;
   }
-  // blocks at level m: the L0 block count times 8^(m-1)
-  const size_t want_blocks = ah_h->nblocks << (3 * (level - 1));
+  // blocks at level m: the L0 block count times 8^(m-1) (4^(m-1) in 2D)
+  const size_t want_blocks = ah_h->nblocks << ((ah_h->q == 9 ? 2 : 3) * (level - 1));
   if (!long_enough(env, nb, want_blocks, "blockSlot") || !long_enough(env, ns, ah_h->slots, "slotToBlock"))
     return nullptr;
   const int rc = lbm_amr_upload_level_topology(amr_of(ah_h), level, static_cast<int32_t*>(bs),

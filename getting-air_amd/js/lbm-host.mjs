@@ -311,7 +311,7 @@ export class AmrSolver {
     this.Q = this.desc.q;
     this.RB = this.desc.rb;
     this.maxSlots = this.desc.maxSlots;
-    this.nBlocks = (this.NX / this.RB) * (this.NY / this.RB) * (this.NZ / this.RB);
+    this.nBlocks = (this.NX / this.RB) * (this.NY / this.RB) * (this.Q === 9 ? 1 : this.NZ / this.RB);
     this.handle = a.createAmr(this.desc);
     this.step = 0;
   }
@@ -368,7 +368,8 @@ export class AmrSolver {
       active: act.slice(0, 1 + act[0]), activeTiles: info.activeTiles, exhausted: info.exhausted };
   }
 
-  // FP64 composite mass and momentum (unrefined L0 cells + fine interiors / 8).
+  // FP64 composite mass and momentum (unrefined L0 cells + fine interiors / 8,
+  // / 4 on the 2D quadtree).
   async readStats() {
     const t = this.addon.amrReadTotals(this.handle);
     return { step: this.step, mass: t[0], px: t[1], py: t[2], pz: t[3] };

@@ -4,6 +4,7 @@
 import { createRequire } from 'module';
 import { DenseSolver, AmrSolver, ServeClient, sphereScenario, tgvScenario, deviceInfo } from '../../getting-air_amd/js/lbm-host.mjs';
 import { makePool, refineWhere, refineNearBody, nearBodyWant, refineHierarchy, manageSlots, activeSlotList } from '../../getting-air_amd/js/d3-amr.mjs';
+import * as Q2 from '../../getting-air_amd/js/amr2d.mjs';
 
 const require = createRequire(import.meta.url);
 const O = require('../../oracle/lbm_oracle.js');
@@ -257,6 +258,35 @@ function relErr(gpuMac, oracleMac, comp) {
     assert(tiles[0] === 432 && tiles[1] === 1280 && Math.abs(cd / 2.9015 - 1) < 0.005 && Math.abs(cd / 2.9436 - 1) < 0.03,
       'tiles ' + tiles + ' Cd ' + cd);
     return tiles.join(' + ') + ' tiles, Cd ' + cd.toFixed(4) + ' (reference 2.9015)';
+  });
+
+  await ok('AMR 2D quadtree (D2Q9, tgv amr-N64-L2 with a refined centre) through the host', async () => {
+    const sc = tgvScenario({ n: 64, tau: 0.8, u0: 0.04 });
+    const pool = Q2.makePool({ dims: [64, 64], rb: 8, maxSlots: 16 });
+    const topo = Q2.refineWhere(pool, (b) => b.bx >= 2 && b.bx < 6 && b.by >= 2 && b.by < 6);
+    const desc = { q: 9, nx: 64, ny: 64, nz: 1, rb: 8, maxSlots: 16, omega: sc.desc.omega };
+    const s = new AmrSolver({ desc: desc, macro: sc.macro, topology: topo });
+    assert(s.nBlocks === pool.nBlocks, 'nBlocks ' + s.nBlocks);
+    await s.reset();
+    const st = await s.debugPoolState();
+    assert(st.activeTiles === 16, 'tiles ' + st.activeTiles);
+    for (let b = 0; b < pool.nBlocks; b++) assert(st.blockSlot[b] === topo.blockSlot[b], 'blockSlot ' + b);
+    const t = Math.round(0.5 * sc.params.td);
+    await s.debugStepSync(t);
+    const m = await s.readMacro();
+    s.destroy();
+    const k = sc.params.kx, dec = Math.exp(-t / sc.params.td);
+    let num = 0, den = 0;
+    for (let y = 0; y < 64; y++) for (let x = 0; x < 64; x++) {
+      const c = y * 64 + x;
+      const ax = 0.04 * Math.sin(k * x) * Math.cos(k * y) * dec, ay = -0.04 * Math.cos(k * x) * Math.sin(k * y) * dec;
+      num += (m[4 * c + 1] - ax) ** 2 + (m[4 * c + 2] - ay) ** 2;
+      den += ax * ax + ay * ay;
+    }
+    const e = Math.sqrt(num / den);
+    assert(e < 0.01, 'L2rel ' + e);
+    const mlups = Q2.cellUpdatesPerMacroStep({ rootCells: 64 * 64, rb: 8, activeByLevel: { 1: 16 } });
+    return '16 tiles, L2rel ' + e.toExponential(2) + ' at ' + t + ' steps, ' + mlups + ' cell updates per macro-step';
   });
 
   await ok('invalid descriptor raises with the library message', () => {
