@@ -123,13 +123,17 @@ struct DecideArgs {
 struct SurfacePlan {
   uint32_t* seam;   // covered L0 cells with an unrefined lattice neighbour
   uint32_t* near;   // unrefined L0 cells with a covered lattice neighbour
-  uint4* ring;      // ring cells over unrefined L0 cells: pool cell (slot * FB^3 + local),
-                    // L0 cell, explode decisions, L0 coordinates (k_ring_plan)
+  uint4* ring;      // every ring cell of each active tile (ring_per_tile entries per tile, in
+                    // class-major order): pool cell (slot * FB^3 + local) or kAbsent over a
+                    // refined L0 cell, L0 cell, explode decisions, L0 coordinates (k_ring_plan)
   uint32_t* flux;   // unrefined L0 cells with a covered cell among their 26 neighbours
                     // (the seam flux correction's cells); null without it
-  uint32_t* count;  // [seam, near, ring, flux]
+  uint32_t* count;  // [seam, near, (unused), flux]
   uint32_t cap_seam, cap_near, cap_ring, cap_flux;
 };
+// Ring cells per tile, (2 rb + 4)^dim - (2 rb)^dim: the ring list holds this
+// many entries for every active tile.
+uint32_t ring_per_tile(int rb, int dim);
 // Capacity of the lists for a pool of max_slots tiles.
 void plan_capacity(int rb, int max_slots, uint32_t ncells, int dim, uint32_t* seam, uint32_t* near, uint32_t* ring);
 

@@ -361,12 +361,12 @@ int rebuild_plan(lbm_amr* a) {
   uint32_t c[4] = {0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(c, a->plan.count, 16, hipMemcpyDeviceToHost, a->stream));
   HIPCHK(hipStreamSynchronize(a->stream));
-  if (c[0] > a->plan.cap_seam || c[1] > a->plan.cap_near || c[2] > a->plan.cap_ring || c[3] > a->plan.cap_flux)
-    return fail(LBM_E_DEVICE, "surface work lists overflow (%u/%u seam, %u/%u near, %u/%u ring, %u/%u flux)", c[0],
-                a->plan.cap_seam, c[1], a->plan.cap_near, c[2], a->plan.cap_ring, c[3], a->plan.cap_flux);
+  if (c[0] > a->plan.cap_seam || c[1] > a->plan.cap_near || c[3] > a->plan.cap_flux)
+    return fail(LBM_E_DEVICE, "surface work lists overflow (%u/%u seam, %u/%u near, %u/%u flux)", c[0],
+                a->plan.cap_seam, c[1], a->plan.cap_near, c[3], a->plan.cap_flux);
   a->nseam = c[0];
   a->nnear = c[1];
-  a->nring = c[2];
+  a->nring = (uint32_t)a->nactive * ring_per_tile(a->d.rb, a->d.q == 9 ? 2 : 3);
   a->nflux = c[3];
   return LBM_OK;
 }
@@ -391,12 +391,12 @@ int rebuild_level_plan(lbm_amr* a, DeepLevel& L) {
   uint32_t c[4] = {0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(c, L.plan.count, 16, hipMemcpyDeviceToHost, a->stream));
   HIPCHK(hipStreamSynchronize(a->stream));
-  if (c[0] > L.plan.cap_seam || c[1] > L.plan.cap_near || c[2] > L.plan.cap_ring)
-    return fail(LBM_E_DEVICE, "level %d surface work lists overflow (%u/%u seam, %u/%u near, %u/%u ring)", L.m, c[0],
-                L.plan.cap_seam, c[1], L.plan.cap_near, c[2], L.plan.cap_ring);
+  if (c[0] > L.plan.cap_seam || c[1] > L.plan.cap_near)
+    return fail(LBM_E_DEVICE, "level %d surface work lists overflow (%u/%u seam, %u/%u near)", L.m, c[0],
+                L.plan.cap_seam, c[1], L.plan.cap_near);
   L.nseam = c[0];
   L.nnear = c[1];
-  L.nring = c[2];
+  L.nring = (uint32_t)L.nactive * ring_per_tile(a->d.rb, 3);
   return LBM_OK;
 }
 

@@ -55,11 +55,40 @@ struct Tile {
   // 3D: FB = 4 (RB/2 + 1), so a tile is a whole number of 64-lane waves and
   // every wave of a tile-major launch lies in one tile
   static_assert(D == 2 || CELLS % 64 == 0, "tile cells must fill whole waves");
+  static constexpr int RING = CELLS - (D == 3 ? RB2 * RB2 * RB2 : RB2 * RB2);  // ring cells per tile
   // tile-local coordinates of cell r
   __device__ static void xyz(uint32_t r, int& fx, int& fy, int& fz) {
     fx = (int)(r % FB);
     fy = (int)((r / FB) % FB);
     fz = D == 3 ? (int)(r / (FB * FB)) : LO;
+  }
+  // Ring cell j (0 <= j < RING) in class-major order.  A ring cell's class is
+  // the side of the interior each axis lies on (low f < LO, mid, high
+  // f >= HI); faces (two mid axes; one in 2D) come first, then edges, then
+  // corners, each class x-fastest.  In 3D with RB >= 8 a face class
+  // (2 x 2RB x 2RB cells) and an edge class (2 x 2 x 2RB) are whole 64-lane
+  // waves and RING is a multiple of 64, so in a launch over tiles x RING the
+  // waves of faces and edges hold one class each; the eight corner classes
+  // share the last wave of a tile.
+  __device__ static void ring_xyz(uint32_t j, int& fx, int& fy, int& fz) {
+    int rem = (int)j;
+    fx = fy = fz = LO;
+    for (int nmid = D - 1; nmid >= 0; nmid--) {
+      for (int c = 0; c < (D == 3 ? 27 : 9); c++) {
+        const int kx = c % 3, ky = (c / 3) % 3, kz = D == 3 ? c / 9 : 1;
+        if ((kx == 1) + (ky == 1) + (D == 3 && kz == 1) != nmid) continue;
+        const int sx = kx == 1 ? RB2 : kGhost, sy = ky == 1 ? RB2 : kGhost;
+        const int sz = D == 2 ? 1 : (kz == 1 ? RB2 : kGhost);
+        if (rem < sx * sy * sz) {
+          const int px = rem % sx, py = (rem / sx) % sy, pz = rem / (sx * sy);
+          fx = kx == 0 ? px : (kx == 1 ? LO + px : HI + px);
+          fy = ky == 0 ? py : (ky == 1 ? LO + py : HI + py);
+          if (D == 3) fz = kz == 0 ? pz : (kz == 1 ? LO + pz : HI + pz);
+          return;
+        }
+        rem -= sx * sy * sz;
+      }
+    }
   }
 };
 template <int Q, int RB>
@@ -237,9 +266,11 @@ __global__ void __launch_bounds__(256) k_cell_class(uint8_t* __restrict__ ccls, 
   }
 }
 
-// Ring cells that explode writes (those whose L0 cell is unrefined), with
-// what explode needs to know about each, all fixed by the topology:
-//   .x  pool cell (slot * FB^3 + tile-local index)
+// What explode needs to know about each ring cell, all fixed by the topology:
+// one entry per ring cell of each active tile, at k * RING + j for active
+// tile k and ring cell j in class-major order (Tile::ring_xyz):
+//   .x  pool cell (slot * FB^3 + tile-local index), or kAbsent where the L0
+//       cell under the ring cell is refined (explode writes nothing there)
 //   .y  the L0 cell v under it
 //   .z  bit i (i < Q): the L0 cell v + e_i is refined; bits 27-29: the two
 //       L0 neighbours of v along x / y / z are both unrefined (the linear
@@ -250,14 +281,14 @@ __global__ void __launch_bounds__(256) k_ring_plan(PoolGeom G, SurfacePlan plan,
                                                    int nactive, ParentRef P) {
   using L = Lattice<Q>;
   using T = TileQ<Q, RB>;
-  constexpr int RB2 = T::RB2, LO = T::LO, HI = T::HI;
+  constexpr int RB2 = T::RB2, FB = T::FB;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t k = t / (uint32_t)T::CELLS;
+  const uint32_t k = t / (uint32_t)T::RING;
   if (k >= (uint32_t)nactive) return;
-  const uint32_t r = t - k * (uint32_t)T::CELLS;
   int fx, fy, fz;
-  T::xyz(r, fx, fy, fz);
-  if (fx >= LO && fx < HI && fy >= LO && fy < HI && fz >= LO && fz < HI) return;
+  T::ring_xyz(t - k * (uint32_t)T::RING, fx, fy, fz);
+  const uint32_t r = L::D == 3 ? (uint32_t)((fz * FB + fy) * FB + fx) : (uint32_t)(fy * FB + fx);
+  plan.ring[t] = make_uint4(kAbsent, 0u, 0u, 0u);
   const int slot = active[1 + k];
   const int id = G.slot_to_block[slot];
   const int bx = id % G.nbx, by = (id / G.nbx) % G.nby, bz = id / (G.nbx * G.nby);
@@ -284,19 +315,21 @@ __global__ void __launch_bounds__(256) k_ring_plan(PoolGeom G, SurfacePlan plan,
   if (open(vx - 1, vy, vz) && open(vx + 1, vy, vz)) bits |= 1u << 27;
   if (open(vx, vy - 1, vz) && open(vx, vy + 1, vz)) bits |= 1u << 28;
   if (open(vx, vy, vz - 1) && open(vx, vy, vz + 1)) bits |= 1u << 29;
-  const uint32_t e = atomicAdd(&plan.count[2], 1u);
-  if (e < plan.cap_ring)
-    plan.ring[e] = make_uint4((uint32_t)slot * T::CELLS + r, vc, bits,
-                              (uint32_t)vx | (uint32_t)vy << 10 | (uint32_t)vz << 20);
+  plan.ring[t] = make_uint4((uint32_t)slot * T::CELLS + r, vc, bits,
+                            (uint32_t)vx | (uint32_t)vy << 10 | (uint32_t)vz << 20);
+}
+
+uint32_t ring_per_tile(int rb, int dim) {
+  auto pw = [&](uint32_t v) { return dim == 3 ? v * v * v : v * v; };
+  return pw(2 * rb + 2 * kGhost) - pw(2 * rb);
 }
 
 void plan_capacity(int rb, int max_slots, uint32_t ncells, int dim, uint32_t* seam, uint32_t* near, uint32_t* ring) {
   auto pw = [&](uint64_t v) { return dim == 3 ? v * v * v : v * v; };
-  const uint64_t fb = 2 * rb + 2 * kGhost;
   const uint64_t shell = pw((uint64_t)rb) - pw((uint64_t)(rb - 2));
   *seam = (uint32_t)std::min<uint64_t>((uint64_t)max_slots * shell, ncells);
   *near = ncells;
-  *ring = (uint32_t)((uint64_t)max_slots * (pw(fb) - pw((uint64_t)(2 * rb))));
+  *ring = (uint32_t)((uint64_t)max_slots * ring_per_tile(rb, dim));
 }
 
 // The per-tile neighbour table (kTileWords words per active tile).  The pool
@@ -553,6 +586,16 @@ k_pool_force(const float* __restrict__ gp, PoolGeom G, const int32_t* __restrict
 // ---------------------------------------------------------------------------
 // explode: coarse state at t -> ring of the pool buffer the first substep reads
 // ---------------------------------------------------------------------------
+// Bit i set for the directions whose component on `axis` is v.
+template <int Q>
+__device__ constexpr uint32_t dirs_with(int axis, int v) {
+  uint32_t m = 0;
+  for (int i = 0; i < Q; i++) {
+    const int e = axis == 0 ? Lattice<Q>::ex[i] : (axis == 1 ? Lattice<Q>::ey[i] : Lattice<Q>::ez[i]);
+    if (e == v) m |= 1u << i;
+  }
+  return m;
+}
 template <int Q, int RB, bool PP>
 __global__ void __launch_bounds__(256)
 k_explode(ParentRef P, float* __restrict__ gp, PoolGeom G, const uint4* __restrict__ ring, uint32_t nring, int linear,
@@ -562,6 +605,8 @@ k_explode(ParentRef P, float* __restrict__ gp, PoolGeom G, const uint4* __restri
   constexpr int LO = T::LO, HI = T::HI;
   const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nring) return;
+  const uint4 d = ring[t];
+  if (d.x == kAbsent) return;  // the L0 cell under this ring cell is refined
   // Grid row g takes directions [g per, (g + 1) per): a ring cell's Q
   // populations are independent, so a small ring spreads them over threads
   // (one thread per ring cell leaves too few waves to hide the loads) and a
@@ -570,7 +615,6 @@ k_explode(ParentRef P, float* __restrict__ gp, PoolGeom G, const uint4* __restri
   const int i0 = (int)blockIdx.y * per, i1 = min(Q, i0 + per);
   // a ring cell over an unrefined L0 cell, and its explode decisions
   // (k_ring_plan)
-  const uint4 d = ring[t];
   const uint32_t cell = d.x, vc = d.y, bits = d.z;
   int fx, fy, fz;
   T::xyz(cell % (uint32_t)T::CELLS, fx, fy, fz);
@@ -589,20 +633,29 @@ k_explode(ParentRef P, float* __restrict__ gp, PoolGeom G, const uint4* __restri
   const float dz = ((float)(fz & 1) - 0.5f) * 0.5f;
   // Ring populations only advect (the ring does not collide), so the one in
   // direction e at ring cell p reaches the interior within the macro-step's
-  // two substeps iff p + e or p + 2e is an interior cell; nothing else reads
-  // the others (coalesce and the orphans take populations that left the
-  // interior at t).  Without a body on this level only those are written;
-  // with one, the ring's bounce-back reflects a cell's own opposite
-  // population, so every direction is.
-  const bool all_dirs = G.body.on != 0;
+  // two substeps only if p + e or p + 2e is an interior cell; nothing else
+  // reads the others (coalesce and the orphans take populations that left
+  // the interior at t).  Without a body on this level explode writes the
+  // directions that point from the cell's side of the ring towards the
+  // interior on every axis (e = +1 on an axis below the interior, -1 above
+  // it, any value on an axis within it): a superset of those, fixed by the
+  // cell's ring class and so the same across a wave of one class.  With a
+  // body, the ring's bounce-back reflects a cell's own opposite population,
+  // so every direction is written.
+  constexpr uint32_t ALL = (1u << Q) - 1u;
+  auto side = [&](int f, uint32_t plus, uint32_t minus) { return f < LO ? plus : (f >= HI ? minus : ALL); };
+  const uint32_t dirs = G.body.on ? ALL
+                                  : side(fx, dirs_with<Q>(0, 1), dirs_with<Q>(0, -1)) &
+                                        side(fy, dirs_with<Q>(1, 1), dirs_with<Q>(1, -1)) &
+                                        side(fz, dirs_with<Q>(2, 1), dirs_with<Q>(2, -1));
   auto inside = [&](int x, int y, int z) { return x >= LO && x < HI && y >= LO && y < HI && z >= LO && z < HI; };
 #pragma unroll
   for (int i = 0; i < Q; i++) {
     if (i < i0 || i >= i1) continue;  // wave-uniform
+    if (!((dirs >> i) & 1u)) continue;
     const int ex = L::ex[i], ey = L::ey[i], ez = L::ez[i];
     const float wi = L::w[i];
     const bool into = inside(fx + ex, fy + ey, fz + ez);
-    if (!all_dirs && !into && !inside(fx + 2 * ex, fy + 2 * ey, fz + 2 * ez)) continue;
     auto l0 = [&](uint32_t c) { return *ppop<PP, Q>(P, c, i); };  // population i of parent cell c
     float out;
     if (!((bits >> i) & 1)) {
@@ -1334,7 +1387,7 @@ struct OpCellClass {
       constexpr bool pp = decltype(PP)::value;
       hipLaunchKernelGGL((k_cell_class<Q, RB, pp>), grid_for(g.ncells), dim3(256), 0, s, ccls, g, plan, P);
       if (nactive > 0)
-        hipLaunchKernelGGL((k_ring_plan<Q, RB, pp>), grid_for((uint64_t)nactive * TileQ<Q, RB>::CELLS), dim3(256), 0, s, g,
+        hipLaunchKernelGGL((k_ring_plan<Q, RB, pp>), grid_for((uint64_t)nactive * TileQ<Q, RB>::RING), dim3(256), 0, s, g,
                            plan, active, nactive, P);
     });
     if (nactive > 0)
