@@ -516,6 +516,10 @@ k_pool_step(const float* __restrict__ gin, float* __restrict__ gout, float* __re
     const bool outer = fx == 0 || fx == FB - 1 || fy == 0 || fy == FB - 1 || fz == 0 || fz == FB - 1;
     load = !over_refined && !(sub == 0 && outer);
   }
+  // A wave none of whose cells' results is read stores nothing either: in 3D
+  // a wave is 64 consecutive cells of one tile from a line-aligned start, so
+  // its stores would cover whole cache lines that no other wave writes.
+  if (!__any(load)) return;
   // Ring cells advect and store without colliding (common_d3_amr_step1.wgsl
   // COLLIDE_RING = 0): coalesce averages uncollided interface states.  The
   // interp interface collides them too (COLLIDE_RING = 1, main-3d.js:1395).
@@ -775,6 +779,11 @@ k_orphans(const float* __restrict__ gp, ParentRef P, PoolGeom G, const uint32_t*
   const int NFx = 2 * G.nx, NFy = 2 * G.ny, NFz = 2 * G.nz;
   const int tx = wrap1(cx + ex, G.nx), ty = wrap1(cy + ey, G.ny), tz = wrap1(cz + ez, G.nz);
   if (G.ccls[coarse_cell(G, tx, ty, tz)] & kCovered) return;
+  // Child o of the target sends population i from the fine cell
+  // q = 2 (c + e) + o - e = 2 c + o + e, which is also the child whose coarse
+  // cell the in-count tests (coarse cell (2 c + o + e) / 2 covered); a coarse
+  // cell is covered exactly when its block holds a tile, so the in-count is
+  // the number of children found.
   float s = 0.f;   // sum of f over the children found in refined tiles
   float nIn = 0.f;
 #pragma unroll
@@ -788,9 +797,8 @@ k_orphans(const float* __restrict__ gp, ParentRef P, PoolGeom G, const uint32_t*
       s += *at(gp + (size_t)i * G.pool_plane,
                pool_cell<RB, D>(qs, local_in_tile<RB>(px, qbx, NFx), local_in_tile<RB>(py, qby, NFy),
                                 local_in_tile<RB>(pz, qbz, NFz))) + wi;
+      nIn += 1.f;
     }
-    const int hx = wrap1(2 * cx + ox + ex, NFx), hy = wrap1(2 * cy + oy + ey, NFy), hz = wrap1(2 * cz + oz + ez, NFz);
-    if (G.ccls[coarse_cell(G, hx >> 1, hy >> 1, hz >> 1)] & kCovered) nIn += 1.f;
   }
   float& gci = *ppop<PP, Q>(P, PP ? parent_index<PP, RB>(G, P, cx, cy, cz) : c, i);
   const float fc = gci + wi;
