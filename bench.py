@@ -308,6 +308,50 @@ def measure_amr(steps, manage_every=50, interface="explode", reflux=0):
             "interface": interface, "reflux": reflux}
 
 
+def measure_amr2d(steps, n=1024, manage_every=50):
+    """The D2Q9 quadtree workload: a periodic double shear layer (two tanh
+    layers of thickness n/80 at u0 0.05, a sin(2 pi x / n) transverse kick of
+    5 %), tau 0.52, 8^2-cell blocks (16^2-cell tiles), |curl u| criterion with
+    hysteresis at 0.25 / 0.15 of the initial peak, manager every 50
+    macro-steps.  Cell updates per macro-step: the root grid plus two fine
+    substeps of every tile interior (cellUpdatesPerMacroStep, amr2d.mjs)."""
+    u0, delta, eps, tau, rb = 0.05, n / 80.0, 0.05, 0.52, 8
+    y, x = np.meshgrid(np.arange(n, dtype=np.float64) + 0.5, np.arange(n, dtype=np.float64), indexing="ij")
+    ux = np.where(y <= n / 2, u0 * np.tanh((y - n / 4) / delta), u0 * np.tanh((3 * n / 4 - y) / delta))
+    uy = eps * u0 * np.sin(2 * np.pi * x / n)
+    mac = np.zeros((n * n, 4), dtype=np.float32)
+    mac[:, 0] = 1.0
+    mac[:, 1] = ux.reshape(-1)
+    mac[:, 2] = uy.reshape(-1)
+    w0 = u0 / delta  # peak |du/dy| of a tanh layer
+    desc = {"q": 9, "nx": n, "ny": n, "nz": 1, "rb": rb, "maxSlots": 8192, "omega": 1.0 / tau,
+            "manageEvery": manage_every, "lead": 1, "refineAt": float(np.float32(0.25 * w0)),
+            "keepAt": float(np.float32(0.15 * w0))}
+    n0, interior = n * n, (2 * rb) ** 2
+    with pylbm.Amr(desc) as a:
+        a.upload_macro(mac)
+        a.manage_now()
+        t_start = a.read_totals()
+        updates, seconds, tiles = 0, 0.0, []
+        done = 0
+        while done < steps:
+            k = min(manage_every, steps - done)
+            t0 = time.perf_counter()
+            a.step(k)
+            seconds += time.perf_counter() - t0
+            nt = a.info()["active_tiles"]
+            tiles.append(nt)
+            updates += k * (n0 + 2 * nt * interior)
+            done += k
+        t_end = a.read_totals()
+        info = a.info()
+    return {"mlups": updates / seconds / 1e6, "seconds": seconds, "macro_steps": steps, "cell_updates": updates,
+            "tiles_min": min(tiles), "tiles_max": max(tiles), "tiles_last": tiles[-1], "exhausted": info["exhausted"],
+            "mass_drift_rel": float((t_end[0] - t_start[0]) / t_start[0]),
+            "momentum_drift_rel": float(np.abs(t_end[1:] - t_start[1:]).max() / t_start[0]),
+            "refine_at": desc["refineAt"], "keep_at": desc["keepAt"], "tau": tau}
+
+
 def measure_amr_slab(steps, reflux):
     """The seam flux correction where it is valid: a static slab of refined
     blocks (z in the middle half, every x and y), so every seam is a flat face
@@ -581,6 +625,8 @@ def main():
         r = measure_amr(args.amr_steps)
         extra_amr = r
         print("amr: %s" % json.dumps(r), file=sys.stderr, flush=True)
+        extra_amr2d = measure_amr2d(args.amr_steps)
+        print("amr 2d: %s" % json.dumps(extra_amr2d), file=sys.stderr, flush=True)
         if not args.no_amr_interp:
             extra_amr_interp = {"dynamic": measure_amr(args.amr_steps, interface="interp", reflux=0)}
             print("amr interp: %s" % json.dumps(extra_amr_interp["dynamic"]), file=sys.stderr, flush=True)
@@ -589,7 +635,7 @@ def main():
                 extra_amr_interp["slab_reflux" if rf else "slab_no_reflux"] = r
                 print("amr interp slab reflux=%d: %s" % (rf, json.dumps(r)), file=sys.stderr, flush=True)
     else:
-        extra_amr = None
+        extra_amr = extra_amr2d = None
 
     extra_serve = None
     if not args.no_serve:
@@ -677,6 +723,8 @@ def main():
 
     if extra_amr:
         out["amr_d3q27_vortex_rings_128"] = short(extra_amr)
+    if extra_amr2d:
+        out["amr_d2q9_shear_layer_1024"] = short(extra_amr2d)
     if extra_amr_interp:
         out["amr_d3q27_vortex_rings_128_interp_interface"] = {k: short(v) for k, v in extra_amr_interp.items()}
     print(json.dumps(out), flush=True)
