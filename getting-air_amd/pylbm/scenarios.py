@@ -447,6 +447,34 @@ def cylinder2d(nx=256, u0=0.1, re=100.0, trt=True, storage="f32"):
     return desc, mac.reshape(-1), meta
 
 
+def cylinder_open(re, d=16, u0=0.05, trt=True, storage="f32"):
+    """D2Q9 circular cylinder of diameter d in an unconfined uniform crossflow
+    (benchmarks/cylinder.json: open domain, low blockage): a 60 d x 30 d box
+    (blockage 1/30), the cylinder at (15 d, 15 d + 1/2) (half a cell off the
+    mid-line so the wake can shed), halfway bounce-back, freestream u0 seeded
+    everywhere and a sponge of width 2 d on all four faces relaxing toward it:
+    nu = u0 d / Re, tau = 3 nu + 1/2; Cd = 2 Fx / (u0^2 d), St = f d / u0."""
+    nx, ny = 60 * d, 30 * d
+    nu = u0 * d / re
+    tau = 3.0 * nu + 0.5
+    omega = f32(1.0 / tau)
+    desc = {
+        "q": 9, "nx": nx, "ny": ny, "nz": 1, "omega": omega,
+        "sponge": {"width": float(2 * d), "axes": [1, 1, 0], "u": [u0, 0.0, 0.0]},
+        "body": {"a": d / 2.0, "centre": [15.0 * d, 15.0 * d + 0.5]},
+        "solidEq": 1, "storage": storage,
+    }
+    if trt:
+        desc["omegaMinus"] = trt_omega_minus(omega)
+    mac = np.empty((nx * ny, 4), dtype=np.float32)
+    mac[:, 0] = 1.0 - (u0 * u0) / (2.0 * CS2)
+    mac[:, 1] = u0
+    mac[:, 2] = 0.0
+    mac[:, 3] = 0.0
+    meta = {"area": float(d), "u0": u0, "nu": nu, "tau": tau, "re": re, "d": d}
+    return desc, mac.reshape(-1), meta
+
+
 def boxed_sphere(n=64, u0=0.08, re=100.0, trt=True, storage="f32"):
     """D3Q19 sphere of diameter n/4 in an n^3 box at (n/3, n/2, n/2), sponge
     on every face toward the freestream (the boxed-sphere serving domain)."""

@@ -346,6 +346,50 @@ def test_shedding_sphere_re300_matches_the_reference_measurement():
     assert lateral.max() > 0.02 * cd.mean()  # it sheds
 
 
+# benchmarks/cylinder.json: consensus ranges for the unconfined 2D cylinder
+# (Re, Cd, Cd tolerance, St, St tolerance)
+CYLINDER_CASES = [(20, 2.05, 0.15, None, None), (40, 1.52, 0.12, None, None),
+                  (100, 1.35, 0.15, 0.165, 0.015), (200, 1.34, 0.15, 0.197, 0.015)]
+
+
+@pytest.mark.parametrize("case", CYLINDER_CASES, ids=lambda c: "Re%d" % c[0])
+def test_open_cylinder_matches_the_reference_consensus_ranges(case):
+    """benchmarks/cylinder.json: D2Q9 TRT cylinder (D 16, u0 0.05) in an open
+    60 D x 30 D box with a sponge on every face.  Steady cases: Cd averaged
+    over the last 20 D/U after 40 D/U.  Shedding cases: after 100 D/U (the
+    street needs that long to saturate in this box), Cd averaged over whole
+    periods of Cl in a 60 D/U window and St from the period of Cl."""
+    re_, cd_ref, cd_tol, st_ref, st_tol = case
+    desc, mac, meta = sc.cylinder_open(re_)
+    d, u0 = meta["d"], meta["u0"]
+    tu = d / u0
+    transient, window = (int(100 * tu), int(60 * tu)) if st_ref else (int(40 * tu), int(20 * tu))
+    print("case open cylinder Re %d D %d" % (re_, d), flush=True)
+    with pylbm.Solve(desc) as s:
+        s.upload_macro(mac)
+        s.step(transient)
+        s.step(window)
+        s.sync()
+        f = s.read_forces(window)
+    assert f.shape[0] == window
+    cd = 2 * f[:, 0] / (u0 ** 2 * d)
+    cl = 2 * f[:, 1] / (u0 ** 2 * d)
+    if st_ref is None:
+        print("Re %d: Cd %.4f (reference %.2f +- %.2f), |Cl| %.1e" % (re_, cd.mean(), cd_ref, cd_tol, np.abs(cl).max()))
+        assert np.abs(cl).max() < 1e-3  # steady: no shedding
+        assert abs(cd.mean() - cd_ref) < cd_tol
+        return
+    up = np.where((cl[:-1] < cl.mean()) & (cl[1:] >= cl.mean()))[0]
+    periods = np.diff(up)
+    st = d / (periods.mean() * u0)
+    cdm = cd[up[0]:up[-1]].mean()
+    print("Re %d: Cd %.4f (reference %.2f +- %.2f), St %.4f (reference %.3f +- %.3f) over %d periods" %
+          (re_, cdm, cd_ref, cd_tol, st, st_ref, st_tol, len(periods)))
+    assert len(periods) >= 6 and periods.max() - periods.min() <= 0.01 * periods.mean()  # a settled street
+    assert abs(cdm - cd_ref) < cd_tol
+    assert abs(st - st_ref) < st_tol
+
+
 def test_sphere_trt_sponge_5000_steps_matches_oracle():
     """Config 2's physics (D3Q19 TRT, halfway bounce-back, sponge on every
     face, per-step FP64 drag/lift) at config 2's step count, on a 64x32x32
