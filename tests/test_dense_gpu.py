@@ -8,6 +8,8 @@ Tolerances (fp32 storage, FP32 collision):
 16-bit storage is reported against the oracle with the wider tolerances
 stated in each test.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -344,6 +346,82 @@ def test_shedding_sphere_re300_matches_the_reference_measurement():
     print("shedding sphere Cd mean %.4f (reference 0.7487-0.7496), lateral pk %.3f" % (cd.mean(), lateral.max()))
     assert 0.7487 * 0.995 < cd.mean() < 0.7496 * 1.005
     assert lateral.max() > 0.02 * cd.mean()  # it sheds
+
+
+# benchmarks/channel.json: (mode, res = H, Re = U_max H / nu, L2 tolerance), tau 0.8
+CHANNEL_CASES = [("poiseuille", 16, 5, 0.01), ("poiseuille", 16, 10, 0.01), ("poiseuille", 32, 10, 0.005),
+                 ("poiseuille", 32, 20, 0.005), ("couette", 16, 5, 0.01), ("couette", 16, 10, 0.01),
+                 ("couette", 32, 10, 0.005), ("couette", 32, 20, 0.005)]
+
+
+@pytest.mark.parametrize("case", CHANNEL_CASES, ids=lambda c: "%s-H%d-Re%d" % c[:3])
+def test_channel_matches_the_closed_form_profiles(case):
+    """benchmarks/channel.json: D2Q9 BGK at tau 0.8 between halfway
+    bounce-back walls H apart (y walls at -1/2 and H - 1/2), periodic in x.
+    Poiseuille driven by F = 8 nu U_max / H^2 (u = F (y + 1/2)(H - 1/2 - y) / (2 nu)),
+    Couette by the top wall at U_max (u = U_max (y + 1/2) / H), U_max = Re nu / H
+    (main-channel.js:65-76).  After 4 H^2 / nu steps (the slowest mode has
+    decayed by e^-39) the relative L2 error of u(y) is within the case's
+    tolerance."""
+    mode, h, re_, tol = case
+    tau = 0.8
+    nu = (tau - 0.5) / 3.0
+    umax = re_ * nu / h
+    desc = {"q": 9, "nx": 16, "ny": h, "nz": 1, "omega": sc.f32(1 / tau), "walls": [0, 1, 0]}
+    y = np.arange(h) + 0.5
+    if mode == "poiseuille":
+        force = 8 * nu * umax / h ** 2
+        desc["force"] = [force, 0, 0]
+        exact = force * y * (h - y) / (2 * nu)
+    else:
+        desc["wallU"] = [0.0, umax]
+        exact = umax * y / h
+    n = 16 * h
+    mac = np.tile(np.array([1, 0, 0, 0], np.float32), n)
+    steps = int(4 * h * h / nu)
+    print("case channel %s H %d Re %d, %d steps" % (mode, h, re_, steps), flush=True)
+    with pylbm.Solve(desc) as s:
+        s.upload_macro(mac)
+        s.step(steps)
+        m = s.read_macro().reshape(h, 16, 4).astype(np.float64)
+    u = m[:, :, 1].mean(axis=1)
+    assert np.abs(m[:, :, 1] - u[:, None]).max() < 1e-6 * umax and np.abs(m[:, :, 2]).max() < 1e-6 * umax
+    err = math.sqrt(((u - exact) ** 2).sum() / (exact ** 2).sum())
+    print("%s H %d Re %d: L2rel %.3e (tolerance %.3g)" % (mode, h, re_, err, tol))
+    assert err < tol
+
+
+# benchmarks/tgv.json dense cases: (N, u0, tau); checkpoints 0.5, 1.5, 3 t_d
+TGV_CASES = [(32, 0.04, 0.8), (64, 0.04, 0.8), (128, 0.04, 0.8), (64, 0.04, 0.6), (64, 0.04, 1.0), (64, 0.08, 0.8)]
+
+
+@pytest.mark.parametrize("case", TGV_CASES, ids=lambda c: "N%d-u%g-tau%g" % c)
+def test_taylor_green_matches_the_exact_decay(case):
+    """benchmarks/tgv.json dense-*: the D2Q9 BGK Taylor-Green vortex against
+    the exact decaying solution: the velocity field's relative L2 error within
+    0.01 at 0.5, 1.5 and 3 t_d (t_d = 1 / (2 nu k^2)), and the decay rate fitted
+    between 1.5 and 3 t_d within 1 %."""
+    n, u0, tau = case
+    d, mac = sc.tgv2d(n=n, tau=tau, u0=u0)
+    nu = (tau - 0.5) / 3.0
+    k = 2 * math.pi / n
+    td = 1.0 / (nu * 2 * k * k)
+    marks = [int(round(c * td)) for c in (0.5, 1.5, 3.0)]
+    print("case TGV N %d u0 %g tau %g, checkpoints %s" % (n, u0, tau, marks), flush=True)
+    errs, rms = [], []
+    with pylbm.Solve(d) as s:
+        s.upload_macro(mac)
+        done = 0
+        for t in marks:
+            s.step(t - done)
+            done = t
+            m = s.read_macro().reshape(-1, 4).astype(np.float64)
+            ax, ay = sc.tgv2d_analytic(n, tau, u0, t)
+            errs.append(math.sqrt(((m[:, 1] - ax) ** 2 + (m[:, 2] - ay) ** 2).sum() / (ax ** 2 + ay ** 2).sum()))
+            rms.append(math.sqrt((m[:, 1] ** 2 + m[:, 2] ** 2).mean()))
+    rate_err = abs(-math.log(rms[2] / rms[1]) / (marks[2] - marks[1]) * td - 1.0)
+    print("TGV N %d u0 %g tau %g: L2rel %s, decay rate error %.2e" % (n, u0, tau, ["%.2e" % e for e in errs], rate_err))
+    assert max(errs) < 0.01 and rate_err < 0.01
 
 
 # benchmarks/cylinder.json: consensus ranges for the unconfined 2D cylinder
