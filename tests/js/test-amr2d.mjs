@@ -108,6 +108,11 @@ ok('cellUpdatesPerMacroStep: the worked example, and each term is live', () => {
   [ring, noSub, got - 1024].forEach((v) => assert(got !== v, 'a mutant gives the same count'));
 });
 
+ok('cellUpdatesPerMacroStep is the quadtree bench line\'s numerator', () => {
+  // bench.py amr_d2q9_shear_layer_1024: 1024^2 root, RB 8, 1536 tiles
+  assert(cellUpdatesPerMacroStep({ rootCells: 1024 * 1024, rb: 8, activeByLevel: { 1: 1536 } }) === 1835008);
+});
+
 ok('activeSlotsFromFreeCount reads the free count in quads and clamps', () => {
   assert(activeSlotsFromFreeCount(1164, 1164 / 4) === 0);
   assert(activeSlotsFromFreeCount(1164, 279) === 48);
