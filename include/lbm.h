@@ -288,8 +288,11 @@ int lbm_rigid_read(lbm_solve* s, lbm_rigid_state* out);
  * ------------------------------------------------------------------------ */
 typedef struct lbm_amr_desc {
   int32_t abi_version;  /* LBM_ABI_VERSION                                    */
-  int32_t q;            /* 19 or 27                                           */
-  int32_t nx, ny, nz;   /* L0 grid, periodic, multiples of rb                 */
+  int32_t q;            /* 19 or 27; 9 for the 2D quadtree (nz = 1, two       */
+                        /* levels, explode interface, no body)                */
+  int32_t nx, ny, nz;   /* L0 grid, periodic, multiples of rb, at most 1024   */
+                        /* per axis (and so is every pool level's parent      */
+                        /* grid): L0 coordinates are packed in 10 bits        */
   int32_t rb;           /* L0 cells per block edge (tile interior 2 rb)       */
   int32_t max_slots;    /* pool capacity in tiles                             */
   int32_t collision;    /* LBM_BGK or LBM_TRT                                 */
