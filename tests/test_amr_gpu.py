@@ -493,6 +493,27 @@ def test_rb8_seams_match_the_oracle_for_every_explode_split(dirs):
         assert abs(t[0] - t0[0]) < 1e-6 * t0[0]
 
 
+def test_rb16_seams_match_the_oracle():
+    """RB 16 (36^3-cell tiles): a face pair, an edge neighbour and a corner
+    across the periodic wrap on a 48^3 root (three blocks per axis), linear
+    explosion, four macro-steps against the oracle; composite mass
+    conserved."""
+    n, rb = 48, 16
+    desc = {"q": 27, "nx": n, "ny": n, "nz": n, "rb": rb, "maxSlots": 8, "omega": 1 / 0.6}
+    say("rb 16 seams")
+    orc0 = ao.AmrOracle(desc)
+    amr, orc = make(desc, smooth_field(n, n, n, 23, amp=0.04), ids(orc0, [(0, 0, 0), (1, 0, 0), (0, 1, 1), (2, 2, 2)]))
+    with amr:
+        t0 = amr.read_totals()
+        for k in range(2):
+            amr.step(2)
+            orc.macro_step()
+            orc.macro_step()
+            compare(amr, orc, label="rb 16 macro-step %d" % (2 * k + 2))
+        t = amr.read_totals()
+        assert abs(t[0] - t0[0]) < 1e-6 * t0[0]
+
+
 def test_rb8_manager_coarsens_and_refines_bit_exact_and_fields_follow():
     """RB 8 under the field-criterion manager: a planted layer of refined
     blocks at z = 0 partly coarsens (drain through k_average<27, 8>) while the
