@@ -4,7 +4,7 @@ AMR gate.
 
 - No tile: the L0 step is the oracle's dense D2Q9 step.
 - Static seams (faces, corners, a wrapped block, a bar), uniform and linear
-  explosion, RB 4 and 8: L0 moments within 1e-5, L0 and tile-interior
+  explosion, RB 4, 8 and 16: L0 moments within 1e-5, L0 and tile-interior
   populations within 1e-6 after several macro-steps.
 - The |curl u| manager: the criterion is the oracle's binary32 restatement of
   the engine's own L0 field bit for bit, the block maps follow the oracle's
@@ -89,11 +89,11 @@ def test_no_tiles_matches_the_oracle():
         compare(amr, orc, label="no tiles")
 
 
-@pytest.mark.parametrize("rb", [4, 8])
+@pytest.mark.parametrize("rb", [4, 8, 16])
 @pytest.mark.parametrize("linear", [0, 1])
 @pytest.mark.parametrize("shape", sorted(SEAMS2))
 def test_static_seams_match_the_oracle(shape, linear, rb):
-    n = 24 if rb == 4 else 48
+    n = 6 * rb  # six blocks per axis: the seam shapes use block coordinates 0-5
     desc = base(n=n, rb=rb, explodeLinear=linear)
     say("case 2D seams %s linear %d rb %d" % (shape, linear, rb))
     orc0 = a2.Amr2DOracle(desc)
