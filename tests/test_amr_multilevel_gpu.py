@@ -90,6 +90,20 @@ def test_three_levels_match_the_oracle(q, linear):
             compare(amr, orc, label="step %d" % (k + 1))
 
 
+def test_three_levels_rb8_match_the_oracle():
+    """The pool-parent passes at the AMR bench's tile size: D3Q27, RB 8
+    (20^3-cell tiles), a 32^3 root, level 1 over L0 cells [8, 24) and level 2
+    over level-1 cells [24, 40), linear explosion."""
+    desc = dict(BASE, q=27, nx=32, ny=32, nz=32, rb=8, levels=3, explodeLinear=1)
+    say("case three levels rb 8")
+    amr, orc = make(desc, smooth_field(32, 32, 32, 29), [LV1, LV2])
+    with amr:
+        for k in range(2):
+            amr.step(1)
+            orc.macro_step()
+            compare(amr, orc, label="rb 8 step %d" % (k + 1))
+
+
 def test_four_levels_match_the_oracle():
     desc = dict(BASE, levels=4)
     say("case four levels")
