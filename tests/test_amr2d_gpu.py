@@ -105,14 +105,15 @@ def test_static_seams_match_the_oracle(shape, linear, rb):
             compare(amr, orc, label="%s step %d" % (shape, k + 1))
 
 
-def test_manager_is_bit_exact_and_fields_follow():
-    n, rb = 48, 4
+@pytest.mark.parametrize("rb", [4, 8])
+def test_manager_is_bit_exact_and_fields_follow(rb):
+    n = 12 * rb  # twelve blocks per axis
     desc = base(n=n, rb=rb, maxSlots=64, manageEvery=2, lead=0)
     mac = smooth_field_2d(n, n, 21, amp=0.05)
     vm = a2.vorticity_2d(np.moveaxis(mac.reshape(n, n, 4), -1, 0)[:3].astype(np.float64))
     c = a2.block_criterion(vm, rb, 0)
     desc.update(refineAt=float(np.quantile(c, 0.8)), keepAt=float(np.quantile(c, 0.7)))
-    say("case 2D manager, thresholds", desc["refineAt"], desc["keepAt"])
+    say("case 2D manager rb %d, thresholds" % rb, desc["refineAt"], desc["keepAt"])
     amr, orc = make(desc, mac)
     changes = 0
     with amr:
