@@ -107,13 +107,12 @@ def test_static_seams_match_the_oracle(shape, linear):
         assert (amr.read_active() == ao.active_list(orc.slot_to_block)).all()
 
 
-@pytest.mark.parametrize("q", [19, 27])
-@pytest.mark.parametrize("reflux", [0, 1])
-def test_interp_interface_and_reflux_match_the_oracle(q, reflux):
-    n = 16
-    desc = {"q": q, "nx": n, "ny": n, "nz": n, "rb": 4, "maxSlots": 8, "omega": 1 / 0.6,
+@pytest.mark.parametrize("q,reflux,rb", [(19, 0, 4), (19, 1, 4), (27, 0, 4), (27, 1, 4), (27, 0, 8), (27, 1, 8)])
+def test_interp_interface_and_reflux_match_the_oracle(q, reflux, rb):
+    n = 4 * rb  # four blocks per axis: the seam shape uses block coordinates 0-3
+    desc = {"q": q, "nx": n, "ny": n, "nz": n, "rb": rb, "maxSlots": 8, "omega": 1 / 0.6,
             "interface": "interp", "reflux": reflux}
-    say("interp interface q", q, "reflux", reflux)
+    say("interp interface q", q, "reflux", reflux, "rb", rb)
     orc0 = ao.AmrOracle(desc)
     amr, orc = make(desc, smooth_field(n, n, n, 11, amp=0.04), ids(orc0, SEAMS["mixed"]))
     with amr:
