@@ -289,6 +289,33 @@ function relErr(gpuMac, oracleMac, comp) {
     return '16 tiles, L2rel ' + e.toExponential(2) + ' at ' + t + ' steps, ' + mlups + ' cell updates per macro-step';
   });
 
+  await ok('AMR 2D quadtree manager through the host: block maps follow manageSlots bit for bit', async () => {
+    const sc = tgvScenario({ n: 96, tau: 0.6, u0: 0.05 });
+    // |curl u| of the TGV peaks at 2 k u0; thresholds near it select a changing subset as it decays
+    const wmax = 2 * (2 * Math.PI / 96) * 0.05;
+    const desc = { q: 9, nx: 96, ny: 96, nz: 1, rb: 4, maxSlots: 256, omega: sc.desc.omega, manageEvery: 2, lead: 0,
+      refineAt: 0.8 * wmax, keepAt: 0.75 * wmax };
+    const s = new AmrSolver({ desc: desc, macro: sc.macro });
+    await s.reset();
+    let prev = await s.debugPoolState();
+    let changes = 0;
+    for (let step = 0; step < 40; step++) {
+      await s.debugStepSync(1);
+      if (step % 2 !== 0) continue;
+      const cur = await s.debugPoolState();
+      const twin = manageSlots(prev.blockSlot, prev.slotToBlock, cur.want);
+      for (let b = 0; b < s.nBlocks; b++) assert(twin.blockSlot[b] === cur.blockSlot[b], 'blockSlot ' + b + ' step ' + step);
+      for (let q = 0; q < 256; q++) assert(twin.slotToBlock[q] === cur.slotToBlock[q], 'slotToBlock ' + q);
+      const act = activeSlotList(twin.slotToBlock);
+      assert(act.length === cur.active.length && act.every((v, i) => v === cur.active[i]), 'active list');
+      changes += twin.coarsened.length + twin.refined.length;
+      prev = cur;
+    }
+    s.destroy();
+    assert(changes > 0, 'no topology change exercised');
+    return changes + ' topology changes over ' + s.nBlocks + ' blocks';
+  });
+
   await ok('invalid descriptor raises with the library message', () => {
     let msg = '';
     try { new DenseSolver({ desc: { q: 19, nx: 4, ny: 4, nz: 4, omega: 3 }, params: {}, macro: null }); } catch (e) { msg = e.message; }
